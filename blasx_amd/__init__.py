@@ -1,0 +1,287 @@
+"""blasx_amd -- Python view of libblasx.so, the MI355X-native tiled GEMM library.
+
+The product is the C-ABI shared library ``blasx_amd/lib/libblasx.so`` (header
+``include/blasx.h``).  This module only binds it with ctypes so tests and
+benchmarks can call the same entry points a C, Fortran or MATLAB program
+would:
+
+* ``cblas_gemm(...)`` -> ``cblas_{s,d,c,z}gemm`` (reference blas/dgemm.c:18-128)
+* ``f77_gemm(...)``   -> ``{s,d,c,z}gemm_``     (reference blas/dgemm.c:131-154)
+* ``device_gemm(...)`` -> ``blasx_{s,d,z}gemm_device`` (operands in HBM)
+
+There is no CPU fallback here: if the library is missing, ``lib()`` raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libblasx.so")
+
+S, D, Z, CPLX64 = 0, 1, 2, 3
+ROW_MAJOR, COL_MAJOR = 101, 102
+TRANS = {"N": 111, "T": 112, "C": 113}
+
+_DTYPES = {np.dtype(np.float32): S, np.dtype(np.float64): D,
+           np.dtype(np.complex128): Z, np.dtype(np.complex64): CPLX64}
+_PREFIX = {S: "s", D: "d", Z: "z", CPLX64: "c"}
+
+_vp = C.c_void_p
+_lib = None
+
+
+class Stats(C.Structure):
+    _fields_ = [("calls_gpu", C.c_uint64), ("calls_cpu", C.c_uint64),
+                ("cpu_fallbacks", C.c_uint64), ("kernel_launches", C.c_uint64),
+                ("bytes_h2d", C.c_uint64), ("bytes_d2h", C.c_uint64),
+                ("bytes_lower_bound", C.c_uint64), ("tile_hits", C.c_uint64),
+                ("tile_misses", C.c_uint64), ("tile_evictions", C.c_uint64),
+                ("tiles_generated", C.c_uint64), ("flops", C.c_double)]
+
+
+def lib():
+    """Load libblasx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with "
+                           "`python -c 'import __graft_entry__ as g; g.build()'`")
+    L = C.CDLL(LIB_PATH)
+    i, i64, d, f = C.c_int, C.c_int64, C.c_double, C.c_float
+    L.cblas_sgemm.argtypes = [i, i, i, i, i, i, f, _vp, i, _vp, i, f, _vp, i]
+    L.cblas_dgemm.argtypes = [i, i, i, i, i, i, d, _vp, i, _vp, i, d, _vp, i]
+    L.cblas_zgemm.argtypes = [i, i, i, i, i, i, _vp, _vp, i, _vp, i, _vp, _vp, i]
+    L.cblas_cgemm.argtypes = [i, i, i, i, i, i, _vp, _vp, i, _vp, i, _vp, _vp, i]
+    for p in "sdcz":
+        getattr(L, p + "gemm_").argtypes = [C.c_char_p, C.c_char_p] + [_vp] * 11
+    L.xerbla_.argtypes = [C.c_char_p, C.POINTER(C.c_int)]
+    L.blasx_init.restype = i
+    L.blasx_gpu_available.restype = i
+    L.blasx_version.restype = C.c_char_p
+    L.blasx_cpublas_path.restype = C.c_char_p
+    L.blasx_get_stats.argtypes = [C.POINTER(Stats)]
+    L.blasx_set_hbm_budget.argtypes = [C.c_size_t]
+    L.blasx_get_hbm_budget.restype = C.c_size_t
+    L.blasx_set_tile_size.argtypes = [i, i]
+    L.blasx_get_tile_size.argtypes = [i, i, i, i]
+    L.blasx_host_alloc.restype = _vp
+    L.blasx_host_alloc.argtypes = [C.c_size_t]
+    L.blasx_host_free.argtypes = [_vp]
+    L.blasx_dev_alloc.restype = _vp
+    L.blasx_dev_alloc.argtypes = [C.c_size_t]
+    L.blasx_dev_free.argtypes = [_vp]
+    L.blasx_memcpy_h2d.argtypes = [_vp, _vp, C.c_size_t]
+    L.blasx_memcpy_d2h.argtypes = [_vp, _vp, C.c_size_t]
+    L.blasx_memset_device.argtypes = [_vp, i, C.c_size_t]
+    L.blasx_compute_stream.restype = _vp
+    L.blasx_dgemm_device.argtypes = [i, i, i, i, i, d, _vp, i64, _vp, i64, d, _vp, i64, _vp]
+    L.blasx_sgemm_device.argtypes = [i, i, i, i, i, f, _vp, i64, _vp, i64, f, _vp, i64, _vp]
+    L.blasx_zgemm_device.argtypes = [i, i, i, i, i, _vp, _vp, i64, _vp, i64, _vp, _vp, i64,
+                                     _vp]
+    L.blasx_resident_register.argtypes = [_vp, C.c_size_t]
+    L.blasx_resident_unregister.argtypes = [_vp]
+    L.blasx_fill_uniform_device.argtypes = [i, _vp, i64, i64, i64, C.c_uint64, C.c_uint32, _vp]
+    L.blasx_fill_uniform_host.argtypes = [i, _vp, i64, i64, i64, C.c_uint64, C.c_uint32]
+    L.blasx_tune_table_add.argtypes = [i, i, i, i, i]
+    # CPU-side test hooks
+    L.bxtest_cache_create.restype = _vp
+    L.bxtest_cache_create.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t]
+    L.bxtest_cache_destroy.argtypes = [_vp]
+    L.bxtest_cache_acquire.argtypes = [_vp, C.c_uint64, i64, i, i, i, i64, C.POINTER(i),
+                                       C.POINTER(i64)]
+    L.bxtest_cache_lookup.argtypes = [_vp, C.c_uint64, i64, i, i, i]
+    L.bxtest_cache_capacity.argtypes = [_vp]
+    L.bxtest_cache_live.argtypes = [_vp]
+    L.bxtest_cache_take_scratch.argtypes = [_vp, i64]
+    L.bxtest_cache_return_scratch.argtypes = [_vp, i]
+    L.bxtest_cache_invalidate_all.argtypes = [_vp]
+    L.bxtest_cache_counters.argtypes = [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_uint64)]
+    L.bxtest_cache_slot_ptr.argtypes = [_vp, i]
+    L.bxtest_cache_slot_ptr.restype = _vp
+    L.bxtest_queue_create.argtypes = [i64]
+    L.bxtest_queue_create.restype = _vp
+    L.bxtest_queue_next.argtypes = [_vp]
+    L.bxtest_queue_next.restype = i64
+    L.bxtest_queue_destroy.argtypes = [_vp]
+    L.bxtest_autotune_tile.argtypes = [i, i, i, i]
+    _lib = L
+    return L
+
+
+def gpu_available():
+    return bool(lib().blasx_gpu_available())
+
+
+def dtype_code(dt):
+    return _DTYPES[np.dtype(dt)]
+
+
+def _scalar(code, v):
+    if code == S:
+        return C.c_float(v)
+    if code == D:
+        return C.c_double(v)
+    if code == Z:
+        return (C.c_double * 2)(complex(v).real, complex(v).imag)
+    return (C.c_float * 2)(complex(v).real, complex(v).imag)
+
+
+def _addr(a):
+    return a.ctypes.data if isinstance(a, np.ndarray) else a
+
+
+def cblas_gemm(order, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C_, ldc):
+    """Call cblas_?gemm on numpy buffers (dtype taken from C_)."""
+    code = dtype_code(C_.dtype)
+    L = lib()
+    fn = getattr(L, "cblas_" + _PREFIX[code] + "gemm")
+    al, be = _scalar(code, alpha), _scalar(code, beta)
+    if code in (Z, CPLX64):
+        al, be = C.cast(al, _vp), C.cast(be, _vp)
+    o = order if isinstance(order, int) else (COL_MAJOR if order == "F" else ROW_MAJOR)
+    t_a = ta if isinstance(ta, int) else TRANS[ta]
+    t_b = tb if isinstance(tb, int) else TRANS[tb]
+    fn(o, t_a, t_b, M, N, K, al, _addr(A), lda, _addr(B), ldb, be, _addr(C_), ldc)
+    return C_
+
+
+def f77_gemm(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C_, ldc):
+    """Call {s,d,c,z}gemm_ with Fortran by-reference arguments."""
+    code = dtype_code(C_.dtype)
+    fn = getattr(lib(), _PREFIX[code] + "gemm_")
+    ints = [C.c_int(v) for v in (M, N, K, lda, ldb, ldc)]
+    al, be = _scalar(code, alpha), _scalar(code, beta)
+    fn(ta.encode(), tb.encode(), C.byref(ints[0]), C.byref(ints[1]), C.byref(ints[2]),
+       C.byref(al), _vp(_addr(A)), C.byref(ints[3]), _vp(_addr(B)), C.byref(ints[4]),
+       C.byref(be), _vp(_addr(C_)), C.byref(ints[5]))
+    return C_
+
+
+def gemm(ta, tb, alpha, A, B, beta, C_):
+    """Column-major convenience wrapper: C_ <- alpha op(A) op(B) + beta C_ (in place)."""
+    M, N = C_.shape
+    K = A.shape[1] if ta == "N" else A.shape[0]
+    for a in (A, B, C_):
+        assert a.flags["F_CONTIGUOUS"], "column-major (Fortran-ordered) arrays expected"
+    return cblas_gemm(COL_MAJOR, ta, tb, M, N, K, alpha, A, max(A.shape[0], 1), B,
+                      max(B.shape[0], 1), beta, C_, max(M, 1))
+
+
+def stats():
+    s = Stats()
+    lib().blasx_get_stats(C.byref(s))
+    return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+
+def reset_stats():
+    lib().blasx_reset_stats()
+
+
+# ----------------------------------------------------------------- device memory
+class DeviceArray:
+    """A column-major matrix in HBM owned by libblasx (blasx_dev_alloc)."""
+
+    def __init__(self, rows, cols, dtype, ld=None):
+        self.rows, self.cols = rows, cols
+        self.ld = rows if ld is None else ld
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.ld * cols * self.dtype.itemsize
+        self.ptr = lib().blasx_dev_alloc(max(self.nbytes, 1))
+        if not self.ptr:
+            raise MemoryError(f"blasx_dev_alloc({self.nbytes}) failed")
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.asfortranarray(a)
+        d = cls(a.shape[0], a.shape[1], a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.asfortranarray(a, dtype=self.dtype)
+        assert a.shape == (self.ld, self.cols) or (self.ld == self.rows and a.shape == (self.rows, self.cols))
+        rc = lib().blasx_memcpy_h2d(self.ptr, a.ctypes.data, self.nbytes)
+        if rc:
+            raise RuntimeError(f"blasx_memcpy_h2d failed ({rc})")
+
+    def download(self):
+        out = np.empty((self.ld, self.cols), dtype=self.dtype, order="F")
+        rc = lib().blasx_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes)
+        if rc:
+            raise RuntimeError(f"blasx_memcpy_d2h failed ({rc})")
+        return out[: self.rows]
+
+    def fill_uniform(self, seed, mid, stream=None):
+        rc = lib().blasx_fill_uniform_device(dtype_code(self.dtype), self.ptr, self.rows,
+                                             self.cols, self.ld, seed, mid, stream)
+        if rc:
+            raise RuntimeError(f"blasx_fill_uniform_device failed ({rc})")
+
+    def free(self):
+        if self.ptr:
+            lib().blasx_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_gemm(ta, tb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, dtype, stream=None):
+    """blasx_?gemm_device on HBM pointers (asynchronous on `stream`)."""
+    code = dtype_code(dtype)
+    L = lib()
+    t_a, t_b = TRANS[ta], TRANS[tb]
+    if code == D:
+        rc = L.blasx_dgemm_device(t_a, t_b, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, stream)
+    elif code == S:
+        rc = L.blasx_sgemm_device(t_a, t_b, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, stream)
+    elif code == Z:
+        al, be = _scalar(Z, alpha), _scalar(Z, beta)
+        rc = L.blasx_zgemm_device(t_a, t_b, M, N, K, C.cast(al, _vp), dA, lda, dB, ldb,
+                                  C.cast(be, _vp), dC, ldc, stream)
+    else:
+        raise TypeError(dtype)
+    if rc:
+        raise RuntimeError(f"device gemm failed ({rc})")
+
+
+def synchronize():
+    rc = lib().blasx_device_synchronize()
+    if rc:
+        raise RuntimeError(f"blasx_device_synchronize failed ({rc})")
+
+
+class PinnedArray:
+    """Column-major host matrix in pinned memory (blasx_host_alloc)."""
+
+    def __init__(self, rows, cols, dtype):
+        self.dtype = np.dtype(dtype)
+        nbytes = rows * cols * self.dtype.itemsize
+        self.ptr = lib().blasx_host_alloc(max(nbytes, 1))
+        if not self.ptr:
+            raise MemoryError(f"blasx_host_alloc({nbytes}) failed")
+        buf = (C.c_char * max(nbytes, 1)).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=self.dtype, count=rows * cols).reshape(
+            (rows, cols), order="F")
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            lib().blasx_host_free(self.ptr)
+            self.ptr = None
+
+
+def fill_uniform_host(a, seed, mid):
+    """Seeded generator on the host, same values as the device fill."""
+    rows, cols = a.shape
+    rc = lib().blasx_fill_uniform_host(dtype_code(a.dtype), a.ctypes.data, rows, cols, rows,
+                                       seed, mid)
+    if rc:
+        raise RuntimeError(f"blasx_fill_uniform_host failed ({rc})")
+    return a

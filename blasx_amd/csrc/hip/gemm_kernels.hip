@@ -1,0 +1,689 @@
+// gemm_kernels.hip -- hand-written CDNA4 (gfx950) GEMM kernels for the
+// per-tile product the reference hands to cuBLAS (cublasDgemm at
+// blas/blasx_dgemm.c:87-94, cublasSgemm at blas/blasx_sgemm.c:87,
+// cublasZgemm at blas/blasx_zgemm.c:88).
+//
+// C[m x n] = alpha * sum_s op(A_s) op(B_s) + beta * C, column-major.
+//
+// One engine serves every precision:
+//   * FP64: V_MFMA_F64_16X16X4_F64, block tile 128x128x16
+//   * FP32: V_MFMA_F32_16X16X4_F32, block tile 128x128x32
+//   * complex128: four FP64 MFMAs per complex MAC ("4M"); conjugation is a
+//     sign flip applied when the tile is staged, block tile 128x64x8
+// 256 threads = 4 wave64s in a 2x2 grid.  Each wave owns a 64x64 (real) or
+// 64x32 (complex) sub-tile held as 16 (8x2) MFMA accumulators.
+// Global -> register prefetch of tile k+1 overlaps the MFMAs on tile k,
+// which reads from one of two LDS buffers (one barrier per K tile).
+// LDS holds both operands K-major-outer ([k][row]) so each lane reads one
+// 16-byte vector per MFMA group; rows inside a wave sub-tile are permuted so
+// that vector holds the operands of VEC adjacent MFMA blocks.
+// A problem's K range may be split into segments with separate base
+// pointers: the tile runtime passes one segment per cached K tile, so one
+// launch covers a whole row of tile products without re-reading C.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../hiplayer.h"
+#include "blasx_ext.h"
+
+namespace {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+struct cplx {
+    double re, im;
+};
+
+template <typename E> struct Cfg;
+template <> struct Cfg<double> {
+    static constexpr int BM = 128, BN = 128, BK = 16, VEC = 2, PADA = 2, PADB = 2;
+    typedef d4_t acc_t;
+};
+template <> struct Cfg<float> {
+    static constexpr int BM = 128, BN = 128, BK = 32, VEC = 4, PADA = 4, PADB = 4;
+    typedef f4_t acc_t;
+};
+template <> struct Cfg<cplx> {
+    static constexpr int BM = 128, BN = 64, BK = 8, VEC = 1, PADA = 1, PADB = 1;
+    typedef d4_t acc_t;
+};
+
+constexpr int NT = 256;
+enum { MAJ_MN = 0, MAJ_K = 1 };
+
+template <typename E, int VEC> struct alignas(16) Vec {
+    E e[VEC];
+};
+
+template <typename E> __device__ __forceinline__ E zero_of() { return E(0); }
+template <> __device__ __forceinline__ cplx zero_of<cplx>() { return cplx{0.0, 0.0}; }
+
+template <typename E> __device__ __forceinline__ E conj_of(E x) { return x; }
+template <> __device__ __forceinline__ cplx conj_of<cplx>(cplx x) { return cplx{x.re, -x.im}; }
+
+__device__ __forceinline__ d4_t mfma(double a, double b, d4_t c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4_t mfma(float a, float b, f4_t c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Tile loader: ROWS x BK tile of op(X) (rows = the M or N dimension).
+//   MAJ_MN: element (r, k) at base[r + k*ld]   (op(A)=A, op(B)=B^T)
+//   MAJ_K : element (r, k) at base[k + r*ld]   (op(A)=A^T/A^H, op(B)=B)
+// Out-of-range elements are staged as zero.
+// ---------------------------------------------------------------------------
+template <typename E, int VEC, int ROWS, int BK, int PAD, int MAJ, bool CONJ> struct Loader {
+    static constexpr int NL = ROWS * BK / (NT * VEC);
+    static constexpr int STR = ROWS + PAD;
+    static_assert(NL >= 1, "tile too small for the block");
+    Vec<E, VEC> r[NL];
+
+    __device__ __forceinline__ void load(const E *__restrict__ base, int64_t ld, int r0, int rlim,
+                                         int64_t k0, int64_t klim, bool vec_ok)
+    {
+        const int tid = threadIdx.x;
+        if constexpr (MAJ == MAJ_MN) {
+            constexpr int TPR = ROWS / VEC, RPP = NT / TPR;
+            const int rr = (tid % TPR) * VEC, kq = tid / TPR;
+#pragma unroll
+            for (int p = 0; p < NL; ++p) {
+                const int64_t gk = k0 + kq + p * RPP;
+                const int gr = r0 + rr;
+                if (vec_ok && gk < klim && gr + VEC <= rlim) {
+                    r[p] = *(const Vec<E, VEC> *)(base + gr + gk * ld);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e)
+                        r[p].e[e] = (gk < klim && gr + e < rlim) ? base[gr + e + gk * ld]
+                                                                 : zero_of<E>();
+                }
+            }
+        } else {
+            constexpr int TPR = BK / VEC, RPP = NT / TPR;
+            const int kv = (tid % TPR) * VEC, rq = tid / TPR;
+#pragma unroll
+            for (int p = 0; p < NL; ++p) {
+                const int gr = r0 + rq + p * RPP;
+                const int64_t gk = k0 + kv;
+                if (vec_ok && gr < rlim && gk + VEC <= klim) {
+                    r[p] = *(const Vec<E, VEC> *)(base + gk + (int64_t)gr * ld);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e)
+                        r[p].e[e] = (gr < rlim && gk + e < klim) ? base[gk + e + (int64_t)gr * ld]
+                                                                 : zero_of<E>();
+                }
+            }
+        }
+        if constexpr (CONJ) {
+#pragma unroll
+            for (int p = 0; p < NL; ++p)
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) r[p].e[e] = conj_of(r[p].e[e]);
+        }
+    }
+
+    // registers -> LDS laid out S[k * STR + row]
+    __device__ __forceinline__ void store(E *__restrict__ S) const
+    {
+        const int tid = threadIdx.x;
+        if constexpr (MAJ == MAJ_MN) {
+            constexpr int TPR = ROWS / VEC, RPP = NT / TPR;
+            const int rr = (tid % TPR) * VEC, kq = tid / TPR;
+#pragma unroll
+            for (int p = 0; p < NL; ++p)
+                *(Vec<E, VEC> *)(S + (kq + p * RPP) * STR + rr) = r[p];
+        } else {
+            constexpr int TPR = BK / VEC, RPP = NT / TPR;
+            const int kv = (tid % TPR) * VEC, rq = tid / TPR;
+#pragma unroll
+            for (int p = 0; p < NL; ++p)
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) S[(kv + e) * STR + rq + p * RPP] = r[p].e[e];
+        }
+    }
+};
+
+__device__ __forceinline__ bool aligned16(const void *p, int64_t ld, int esize)
+{
+    return ((reinterpret_cast<uintptr_t>(p) & 15) == 0) && (((ld * esize) & 15) == 0);
+}
+
+// block tile index -> (tm, tn), walking groups of 8 tile rows so that the
+// blocks resident at one time share A row panels and B column panels
+__device__ __forceinline__ void tile_coords(int t, int tiles_m, int tiles_n, int &tm, int &tn)
+{
+    constexpr int G = 8;
+    const int per_group = G * tiles_n;
+    const int g = t / per_group;
+    const int first = g * G;
+    const int gsz = min(tiles_m - first, G);
+    const int r = t - g * per_group;
+    tm = first + r % gsz;
+    tn = r / gsz;
+}
+
+// ---------------------------------------------------------------------------
+// LDS -> MFMA for one staged K tile (real types)
+// ---------------------------------------------------------------------------
+template <typename E>
+__device__ __forceinline__ void mma_real(const E *__restrict__ As, const E *__restrict__ Bs,
+                                         typename Cfg<E>::acc_t (&acc)[4][4], int wm, int wn,
+                                         int lane)
+{
+    constexpr int BK = Cfg<E>::BK, VEC = Cfg<E>::VEC;
+    constexpr int STRA = Cfg<E>::BM + Cfg<E>::PADA, STRB = Cfg<E>::BN + Cfg<E>::PADB;
+    const int l16 = lane & 15, lg = lane >> 4;
+#pragma unroll
+    for (int s = 0; s < BK / 4; ++s) {
+        const int k = s * 4 + lg;
+        E a[4], b[4];
+#pragma unroll
+        for (int h = 0; h < 4 / VEC; ++h) {
+            Vec<E, VEC> va = *(const Vec<E, VEC> *)(As + k * STRA + wm * 64 + h * 16 * VEC + VEC * l16);
+            Vec<E, VEC> vb = *(const Vec<E, VEC> *)(Bs + k * STRB + wn * 64 + h * 16 * VEC + VEC * l16);
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                a[h * VEC + e] = va.e[e];
+                b[h * VEC + e] = vb.e[e];
+            }
+        }
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) acc[mb][nb] = mfma(a[mb], b[nb], acc[mb][nb]);
+    }
+}
+
+// complex: acc[mb][nb][0] = real part, [1] = imaginary part
+__device__ __forceinline__ void mma_cplx(const cplx *__restrict__ As, const cplx *__restrict__ Bs,
+                                         d4_t (&acc)[4][2][2], int wm, int wn, int lane)
+{
+    constexpr int BK = Cfg<cplx>::BK;
+    constexpr int STRA = Cfg<cplx>::BM + Cfg<cplx>::PADA, STRB = Cfg<cplx>::BN + Cfg<cplx>::PADB;
+    const int l16 = lane & 15, lg = lane >> 4;
+#pragma unroll
+    for (int s = 0; s < BK / 4; ++s) {
+        const int k = s * 4 + lg;
+        cplx a[4], b[2];
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) a[mb] = As[k * STRA + wm * 64 + mb * 16 + l16];
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) b[nb] = Bs[k * STRB + wn * 32 + nb * 16 + l16];
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+            const double nai = -a[mb].im;
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb) {
+                acc[mb][nb][0] = mfma(a[mb].re, b[nb].re, acc[mb][nb][0]);
+                acc[mb][nb][1] = mfma(a[mb].re, b[nb].im, acc[mb][nb][1]);
+                acc[mb][nb][0] = mfma(nai, b[nb].im, acc[mb][nb][0]);
+                acc[mb][nb][1] = mfma(a[mb].im, b[nb].re, acc[mb][nb][1]);
+            }
+        }
+    }
+}
+
+template <typename E> struct Smem {
+    static constexpr int A_ELEMS = Cfg<E>::BK * (Cfg<E>::BM + Cfg<E>::PADA);
+    static constexpr int B_ELEMS = Cfg<E>::BK * (Cfg<E>::BN + Cfg<E>::PADB);
+    static constexpr int ELEMS = 2 * (A_ELEMS + B_ELEMS);
+};
+
+__device__ __forceinline__ const bxh_seg_t &seg_at(const bxh_seg_t *segs, const bxh_seg_t &seg0,
+                                                    int i)
+{
+    return segs ? segs[i] : seg0;
+}
+
+// ---------------------------------------------------------------------------
+// One block tile of one problem.
+// ---------------------------------------------------------------------------
+template <typename E, int MAJA, int MAJB, bool CJA, bool CJB>
+__device__ __forceinline__ void gemm_block(const bxh_problem_t &P, const bxh_seg_t *segs,
+                                           const bxh_seg_t &seg0, int tm, int tn, E *smem)
+{
+    typedef Cfg<E> C_;
+    constexpr int BM = C_::BM, BN = C_::BN, BK = C_::BK, VEC = C_::VEC;
+    constexpr bool CPLX = sizeof(E) == 16;
+    E *As0 = smem;
+    E *Bs0 = smem + 2 * Smem<E>::A_ELEMS;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int nseg = P.nseg;
+
+    Loader<E, VEC, BM, BK, C_::PADA, MAJA, CJA> la;
+    Loader<E, VEC, BN, BK, C_::PADB, MAJB, CJB> lb;
+
+    typedef typename C_::acc_t acc_t;
+    acc_t accr[4][4];
+    d4_t accz[4][2][2];
+    if constexpr (CPLX) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) accz[i][j][0] = accz[i][j][1] = d4_t{0, 0, 0, 0};
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) accr[i][j] = acc_t{0, 0, 0, 0};
+    }
+
+    auto load_both = [&](int si, int64_t k0) {
+        const bxh_seg_t &S = seg_at(segs, seg0, si);
+        const E *A = reinterpret_cast<const E *>(S.A);
+        const E *B = reinterpret_cast<const E *>(S.B);
+        la.load(A, S.lda, m0, P.m, k0, S.k, aligned16(A, S.lda, sizeof(E)));
+        lb.load(B, S.ldb, n0, P.n, k0, S.k, aligned16(B, S.ldb, sizeof(E)));
+    };
+
+    // first non-empty segment
+    int si = 0;
+    int64_t k0 = 0;
+    while (si < nseg && seg_at(segs, seg0, si).k <= 0) ++si;
+    bool have = si < nseg;
+    if (have) {
+        load_both(si, k0);
+        la.store(As0);
+        lb.store(Bs0);
+    }
+    __syncthreads();
+    int buf = 0;
+    while (have) {
+        int nsi = si;
+        int64_t nk0 = k0 + BK;
+        while (nsi < nseg && nk0 >= seg_at(segs, seg0, nsi).k) {
+            ++nsi;
+            nk0 = 0;
+        }
+        const bool nhave = nsi < nseg;
+        if (nhave) load_both(nsi, nk0);
+        const E *As = As0 + buf * Smem<E>::A_ELEMS;
+        const E *Bs = Bs0 + buf * Smem<E>::B_ELEMS;
+        if constexpr (CPLX)
+            mma_cplx(reinterpret_cast<const cplx *>(As), reinterpret_cast<const cplx *>(Bs), accz,
+                     wm, wn, lane);
+        else
+            mma_real<E>(As, Bs, accr, wm, wn, lane);
+        if (nhave) {
+            la.store(As0 + (buf ^ 1) * Smem<E>::A_ELEMS);
+            lb.store(Bs0 + (buf ^ 1) * Smem<E>::B_ELEMS);
+        }
+        __syncthreads();
+        si = nsi;
+        k0 = nk0;
+        have = nhave;
+        buf ^= 1;
+    }
+
+    // ---- epilogue: C = alpha * acc + beta * C ----
+    E *Cp = reinterpret_cast<E *>(P.C);
+    const int64_t ldc = P.ldc;
+    const int l16 = lane & 15, lg = lane >> 4;
+    if constexpr (CPLX) {
+        const double ar = P.alpha_re, ai = P.alpha_im, br = P.beta_re, bi = P.beta_im;
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb) {
+                const int col = n0 + wn * 32 + nb * 16 + l16;
+                if (col >= P.n) continue;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int row = m0 + wm * 64 + mb * 16 + 4 * lg + v;
+                    if (row >= P.m) continue;
+                    const double sr = accz[mb][nb][0][v], sim = accz[mb][nb][1][v];
+                    double orr = ar * sr - ai * sim, oi = ar * sim + ai * sr;
+                    cplx *c = Cp + row + (int64_t)col * ldc;
+                    if (!P.beta_zero) {
+                        const cplx c0 = *c;
+                        orr += br * c0.re - bi * c0.im;
+                        oi += br * c0.im + bi * c0.re;
+                    }
+                    *c = cplx{orr, oi};
+                }
+            }
+    } else {
+        const E alpha = (E)P.alpha_re, beta = (E)P.beta_re;
+        const bool cvec = aligned16(Cp, ldc, sizeof(E));
+#pragma unroll
+        for (int h = 0; h < 4 / VEC; ++h)
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                const int col = n0 + wn * 64 + (nb / VEC) * 16 * VEC + VEC * l16 + nb % VEC;
+                if (col >= P.n) continue;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int rbase = m0 + wm * 64 + h * 16 * VEC + VEC * (4 * lg + v);
+                    E *c = Cp + rbase + (int64_t)col * ldc;
+                    if (cvec && rbase + VEC <= P.m) {
+                        Vec<E, VEC> out;
+                        if (P.beta_zero) {
+#pragma unroll
+                            for (int e = 0; e < VEC; ++e) out.e[e] = alpha * accr[h * VEC + e][nb][v];
+                        } else {
+                            const Vec<E, VEC> old = *(const Vec<E, VEC> *)c;
+#pragma unroll
+                            for (int e = 0; e < VEC; ++e)
+                                out.e[e] = alpha * accr[h * VEC + e][nb][v] + beta * old.e[e];
+                        }
+                        *(Vec<E, VEC> *)c = out;
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < VEC; ++e) {
+                            if (rbase + e >= P.m) continue;
+                            const E s = alpha * accr[h * VEC + e][nb][v];
+                            c[e] = P.beta_zero ? s : s + beta * c[e];
+                        }
+                    }
+                }
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+template <typename E, int MAJA, int MAJB, bool CJA, bool CJB>
+__global__ __launch_bounds__(NT, 2) void k_gemm_plain(bxh_problem_t P, bxh_seg_t S)
+{
+    __shared__ __attribute__((aligned(16))) E smem[Smem<E>::ELEMS];
+    int b = blockIdx.x;
+    const int nb = gridDim.x;
+    // The dispatcher places block b on XCD b % 8.  Renumber so each XCD
+    // works on a contiguous range of tiles (shared panels stay in its L2).
+    if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+    int tm, tn;
+    tile_coords(b, P.tiles_m, P.tiles_n, tm, tn);
+    gemm_block<E, MAJA, MAJB, CJA, CJB>(P, nullptr, S, tm, tn, smem);
+}
+
+template <typename E, int MAJA, int MAJB, bool CJA, bool CJB>
+__device__ __forceinline__ void run_tile(const bxh_problem_t &P, const bxh_seg_t *segs, int local,
+                                         E *smem)
+{
+    int tm, tn;
+    tile_coords(local, P.tiles_m, P.tiles_n, tm, tn);
+    gemm_block<E, MAJA, MAJB, CJA, CJB>(P, segs + P.seg_off, segs[P.seg_off], tm, tn, smem);
+}
+
+// Grid-stride over the tiles of many problems (grouped GEMM / tile runtime).
+template <typename E>
+__global__ __launch_bounds__(NT, 2) void k_gemm_grouped(const bxh_problem_t *__restrict__ probs,
+                                                        int nprob,
+                                                        const bxh_seg_t *__restrict__ segs,
+                                                        int total)
+{
+    __shared__ __attribute__((aligned(16))) E smem[Smem<E>::ELEMS];
+    constexpr bool CPLX = sizeof(E) == 16;
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        int lo = 0, hi = nprob - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (probs[mid].tile_start <= t)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const bxh_problem_t P = probs[lo];
+        const int local = t - P.tile_start;
+        if constexpr (CPLX) {
+            switch (P.opa * 3 + P.opb) {
+            case 0: run_tile<E, MAJ_MN, MAJ_K, false, false>(P, segs, local, smem); break;
+            case 1: run_tile<E, MAJ_MN, MAJ_MN, false, false>(P, segs, local, smem); break;
+            case 2: run_tile<E, MAJ_MN, MAJ_MN, false, true>(P, segs, local, smem); break;
+            case 3: run_tile<E, MAJ_K, MAJ_K, false, false>(P, segs, local, smem); break;
+            case 4: run_tile<E, MAJ_K, MAJ_MN, false, false>(P, segs, local, smem); break;
+            case 5: run_tile<E, MAJ_K, MAJ_MN, false, true>(P, segs, local, smem); break;
+            case 6: run_tile<E, MAJ_K, MAJ_K, true, false>(P, segs, local, smem); break;
+            case 7: run_tile<E, MAJ_K, MAJ_MN, true, false>(P, segs, local, smem); break;
+            default: run_tile<E, MAJ_K, MAJ_MN, true, true>(P, segs, local, smem); break;
+            }
+        } else {
+            const int ta = P.opa != 0, tb = P.opb != 0;
+            switch (ta * 2 + tb) {
+            case 0: run_tile<E, MAJ_MN, MAJ_K, false, false>(P, segs, local, smem); break;
+            case 1: run_tile<E, MAJ_MN, MAJ_MN, false, false>(P, segs, local, smem); break;
+            case 2: run_tile<E, MAJ_K, MAJ_K, false, false>(P, segs, local, smem); break;
+            default: run_tile<E, MAJ_K, MAJ_MN, false, false>(P, segs, local, smem); break;
+            }
+        }
+    }
+}
+
+template <typename E>
+__global__ void k_scale(int m, int n, E beta, int beta_zero, E *C, int64_t ldc)
+{
+    const int64_t total = (int64_t)m * n;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = i % m, c = i / m;
+        E *p = C + r + c * ldc;
+        *p = beta_zero ? E(0) : beta * *p;
+    }
+}
+
+__global__ void k_scale_z(int m, int n, double br, double bi, int beta_zero, cplx *C, int64_t ldc)
+{
+    const int64_t total = (int64_t)m * n;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = i % m, c = i / m;
+        cplx *p = C + r + c * ldc;
+        if (beta_zero) {
+            *p = cplx{0.0, 0.0};
+        } else {
+            const cplx v = *p;
+            *p = cplx{br * v.re - bi * v.im, br * v.im + bi * v.re};
+        }
+    }
+}
+
+// SplitMix64 (Steele, Lea, Flood 2014); the host generator in fill.c is the same function
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x)
+{
+    uint64_t z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ double u11(uint64_t key)
+{
+    return ((double)(splitmix64(key) >> 11) * 0x1.0p-53) * 2.0 - 1.0;
+}
+
+__global__ void k_fill(int dtype, void *p, int64_t rows, int64_t cols, int64_t ld, uint64_t seed,
+                       uint32_t id)
+{
+    const int64_t total = rows * cols;
+    const uint64_t base = seed ^ ((uint64_t)id << 56);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = i % rows, c = i / rows;
+        const int64_t o = r + c * ld;
+        const uint64_t e = (uint64_t)i;
+        if (dtype == BLASX_D) {
+            ((double *)p)[o] = u11(base ^ e);
+        } else if (dtype == BLASX_S) {
+            ((float *)p)[o] = (float)u11(base ^ e);
+        } else {
+            ((double *)p)[2 * o] = u11(base ^ (2 * e));
+            ((double *)p)[2 * o + 1] = u11(base ^ (2 * e + 1));
+        }
+    }
+}
+
+thread_local char g_err[256];
+
+int fail(const char *what, hipError_t e)
+{
+    snprintf(g_err, sizeof g_err, "%s: %s", what, hipGetErrorString(e));
+    return BLASX_EHIP;
+}
+
+template <typename E, int MAJA, int MAJB, bool CJA, bool CJB>
+int launch_plain(const bxh_problem_t &P, const bxh_seg_t &S, hipStream_t st)
+{
+    const int grid = P.tiles_m * P.tiles_n;
+    hipLaunchKernelGGL((k_gemm_plain<E, MAJA, MAJB, CJA, CJB>), dim3(grid), dim3(NT), 0, st, P, S);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : fail("gemm launch", e);
+}
+
+template <typename E> int dispatch_real(int opa, int opb, const bxh_problem_t &P, const bxh_seg_t &S,
+                                        hipStream_t st)
+{
+    const int ta = opa != 0, tb = opb != 0;
+    switch (ta * 2 + tb) {
+    case 0: return launch_plain<E, MAJ_MN, MAJ_K, false, false>(P, S, st);
+    case 1: return launch_plain<E, MAJ_MN, MAJ_MN, false, false>(P, S, st);
+    case 2: return launch_plain<E, MAJ_K, MAJ_K, false, false>(P, S, st);
+    default: return launch_plain<E, MAJ_K, MAJ_MN, false, false>(P, S, st);
+    }
+}
+
+int dispatch_cplx(int opa, int opb, const bxh_problem_t &P, const bxh_seg_t &S, hipStream_t st)
+{
+    switch (opa * 3 + opb) {
+    case 0: return launch_plain<cplx, MAJ_MN, MAJ_K, false, false>(P, S, st);
+    case 1: return launch_plain<cplx, MAJ_MN, MAJ_MN, false, false>(P, S, st);
+    case 2: return launch_plain<cplx, MAJ_MN, MAJ_MN, false, true>(P, S, st);
+    case 3: return launch_plain<cplx, MAJ_K, MAJ_K, false, false>(P, S, st);
+    case 4: return launch_plain<cplx, MAJ_K, MAJ_MN, false, false>(P, S, st);
+    case 5: return launch_plain<cplx, MAJ_K, MAJ_MN, false, true>(P, S, st);
+    case 6: return launch_plain<cplx, MAJ_K, MAJ_K, true, false>(P, S, st);
+    case 7: return launch_plain<cplx, MAJ_K, MAJ_MN, true, false>(P, S, st);
+    default: return launch_plain<cplx, MAJ_K, MAJ_MN, true, true>(P, S, st);
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+const char *bxh_kernel_error(void) { return g_err; }
+
+int bxh_tile_m(int dtype)
+{
+    return dtype == BLASX_S ? Cfg<float>::BM : dtype == BLASX_D ? Cfg<double>::BM : Cfg<cplx>::BM;
+}
+
+int bxh_tile_n(int dtype)
+{
+    return dtype == BLASX_S ? Cfg<float>::BN : dtype == BLASX_D ? Cfg<double>::BN : Cfg<cplx>::BN;
+}
+
+int bxh_gemm(int dtype, int opa, int opb, int m, int n, int k, const double *alpha,
+             const void *A, int64_t lda, const void *B, int64_t ldb, const double *beta, void *C,
+             int64_t ldc, void *stream)
+{
+    if (m <= 0 || n <= 0) return 0;
+    if (opa < 0 || opa > 2 || opb < 0 || opb > 2 || k < 0) return BLASX_EINVAL;
+    bxh_problem_t P;
+    P.C = C;
+    P.ldc = ldc;
+    P.m = m;
+    P.n = n;
+    P.nseg = 1;
+    P.seg_off = 0;
+    P.opa = opa;
+    P.opb = opb;
+    P.tiles_m = (m + bxh_tile_m(dtype) - 1) / bxh_tile_m(dtype);
+    P.tiles_n = (n + bxh_tile_n(dtype) - 1) / bxh_tile_n(dtype);
+    P.tile_start = 0;
+    P.alpha_re = alpha[0];
+    P.alpha_im = dtype == BLASX_Z ? alpha[1] : 0.0;
+    P.beta_re = beta[0];
+    P.beta_im = dtype == BLASX_Z ? beta[1] : 0.0;
+    P.beta_zero = (P.beta_re == 0.0 && P.beta_im == 0.0);
+    bxh_seg_t S;
+    S.A = A;
+    S.B = B;
+    S.lda = lda;
+    S.ldb = ldb;
+    S.k = k;
+    hipStream_t st = (hipStream_t)stream;
+    switch (dtype) {
+    case BLASX_D: return dispatch_real<double>(opa, opb, P, S, st);
+    case BLASX_S: return dispatch_real<float>(opa, opb, P, S, st);
+    case BLASX_Z: return dispatch_cplx(opa, opb, P, S, st);
+    default: return BLASX_EINVAL;
+    }
+}
+
+int bxh_gemm_grouped(int dtype, const bxh_problem_t *probs_dev, int nprob,
+                     const bxh_seg_t *segs_dev, int total_tiles, int grid, void *stream)
+{
+    if (nprob <= 0 || total_tiles <= 0) return 0;
+    if (grid <= 0 || grid > total_tiles) grid = total_tiles;
+    hipStream_t st = (hipStream_t)stream;
+    switch (dtype) {
+    case BLASX_D:
+        hipLaunchKernelGGL(k_gemm_grouped<double>, dim3(grid), dim3(NT), 0, st, probs_dev, nprob,
+                           segs_dev, total_tiles);
+        break;
+    case BLASX_S:
+        hipLaunchKernelGGL(k_gemm_grouped<float>, dim3(grid), dim3(NT), 0, st, probs_dev, nprob,
+                           segs_dev, total_tiles);
+        break;
+    case BLASX_Z:
+        hipLaunchKernelGGL(k_gemm_grouped<cplx>, dim3(grid), dim3(NT), 0, st, probs_dev, nprob,
+                           segs_dev, total_tiles);
+        break;
+    default: return BLASX_EINVAL;
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : fail("grouped gemm launch", e);
+}
+
+int bxh_scale(int dtype, int m, int n, const double *beta, void *C, int64_t ldc, void *stream)
+{
+    if (m <= 0 || n <= 0) return 0;
+    const int64_t total = (int64_t)m * n;
+    int grid = (int)((total + 255) / 256);
+    if (grid > 65536) grid = 65536;
+    hipStream_t st = (hipStream_t)stream;
+    const int bz = (beta[0] == 0.0 && (dtype != BLASX_Z || beta[1] == 0.0));
+    switch (dtype) {
+    case BLASX_D:
+        hipLaunchKernelGGL(k_scale<double>, dim3(grid), dim3(256), 0, st, m, n, beta[0], bz,
+                           (double *)C, ldc);
+        break;
+    case BLASX_S:
+        hipLaunchKernelGGL(k_scale<float>, dim3(grid), dim3(256), 0, st, m, n, (float)beta[0], bz,
+                           (float *)C, ldc);
+        break;
+    case BLASX_Z:
+        hipLaunchKernelGGL(k_scale_z, dim3(grid), dim3(256), 0, st, m, n, beta[0], beta[1], bz,
+                           (cplx *)C, ldc);
+        break;
+    default: return BLASX_EINVAL;
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : fail("scale launch", e);
+}
+
+int bxh_fill_uniform(int dtype, void *p, int64_t rows, int64_t cols, int64_t ld, uint64_t seed,
+                     uint32_t id, void *stream)
+{
+    if (rows <= 0 || cols <= 0) return 0;
+    if (dtype != BLASX_S && dtype != BLASX_D && dtype != BLASX_Z) return BLASX_EINVAL;
+    const int64_t total = rows * cols;
+    int grid = (int)((total + 255) / 256);
+    if (grid > 262144) grid = 262144;
+    hipLaunchKernelGGL(k_fill, dim3(grid), dim3(256), 0, (hipStream_t)stream, dtype, p, rows, cols,
+                       ld, seed, id);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : fail("fill launch", e);
+}
+
+} // extern "C"

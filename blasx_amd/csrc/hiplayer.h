@@ -1,0 +1,101 @@
+/*
+ * hiplayer.h -- the thin C ABI between the C host runtime (abi.c, tile_rt.c,
+ * ...) and the HIP side (hip/ kernels).  Plain C types only: the C files never
+ * include HIP headers.  Streams and events travel as void*.
+ *
+ * Every bxh_* function returns 0 on success or a negative BLASX_E* code;
+ * bxh_last_error() names the failing HIP call.
+ */
+#ifndef BLASX_HIPLAYER_H
+#define BLASX_HIPLAYER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* one K-segment of a GEMM problem: op(A) columns / op(B) rows [0, k) of
+ * the segment live at A (leading dim lda) and B (ldb), stored as the
+ * problem's ops say.  A plain GEMM has one segment; the tile runtime gives
+ * one segment per cached K-tile. */
+typedef struct bxh_seg {
+    const void *A;
+    const void *B;
+    int64_t lda;
+    int64_t ldb;
+    int64_t k;
+} bxh_seg_t;
+
+/* one GEMM problem of a grouped launch: C[m x n] = alpha * sum_s op(A_s) op(B_s) + beta * C */
+typedef struct bxh_problem {
+    void *C;
+    int64_t ldc;
+    int32_t m, n;
+    int32_t nseg, seg_off;       /* segments [seg_off, seg_off + nseg) of the segment table */
+    int32_t opa, opb;            /* 0 = N, 1 = T, 2 = C (conjugate transpose) */
+    int32_t tiles_m, tiles_n;    /* block tiles of this problem */
+    int32_t tile_start;          /* prefix sum of tiles_m*tiles_n over earlier problems */
+    int32_t beta_zero;           /* C is not read when set */
+    double alpha_re, alpha_im;
+    double beta_re, beta_im;
+} bxh_problem_t;
+
+/* block-tile edges per element type (for computing tiles_m/tiles_n) */
+int bxh_tile_m(int dtype);
+int bxh_tile_n(int dtype);
+
+/* ---- runtime ---- */
+const char *bxh_last_error(void);
+int bxh_device_count(int *n);
+int bxh_set_device(int dev);
+int bxh_device_info(int dev, char *name, size_t name_len, int *cus, size_t *total_mem);
+int bxh_mem_get_info(size_t *free_b, size_t *total_b);
+int bxh_malloc(void **p, size_t bytes);
+int bxh_free(void *p);
+int bxh_host_malloc(void **p, size_t bytes);
+int bxh_host_free(void *p);
+int bxh_pointer_is_pinned(const void *p);      /* 1 pinned host, 2 device, 0 pageable/unknown */
+int bxh_stream_create(void **s);
+int bxh_stream_destroy(void *s);
+int bxh_stream_sync(void *s);
+int bxh_device_sync(void);
+int bxh_event_create(void **e, int timing);
+int bxh_event_destroy(void *e);
+int bxh_event_record(void *e, void *s);
+int bxh_event_query(void *e);                  /* 1 done, 0 pending, <0 error */
+int bxh_event_sync(void *e);
+int bxh_event_elapsed_ms(float *ms, void *start, void *stop);
+int bxh_stream_wait_event(void *s, void *e);
+int bxh_memcpy_h2d_async(void *dst, const void *src, size_t bytes, void *s);
+int bxh_memcpy_d2h_async(void *dst, const void *src, size_t bytes, void *s);
+int bxh_memcpy_d2d_async(void *dst, const void *src, size_t bytes, void *s);
+int bxh_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int bxh_memcpy_d2h(void *dst, const void *src, size_t bytes);
+/* strided 2-D copies: `height` columns of `width` bytes */
+int bxh_memcpy2d_h2d_async(void *dst, size_t dpitch, const void *src, size_t spitch,
+                           size_t width, size_t height, void *s);
+int bxh_memcpy2d_d2h_async(void *dst, size_t dpitch, const void *src, size_t spitch,
+                           size_t width, size_t height, void *s);
+int bxh_memset_async(void *dst, int v, size_t bytes, void *s);
+
+/* ---- kernels ---- */
+/* C = alpha op(A) op(B) + beta C on device pointers; alpha/beta are (re, im) */
+int bxh_gemm(int dtype, int opa, int opb, int m, int n, int k, const double *alpha,
+             const void *A, int64_t lda, const void *B, int64_t ldb, const double *beta,
+             void *C, int64_t ldc, void *stream);
+/* grouped launch over a device-resident problem and segment table; all
+ * problems share dtype.  grid = 0 picks one block per tile. */
+int bxh_gemm_grouped(int dtype, const bxh_problem_t *probs_dev, int nprob,
+                     const bxh_seg_t *segs_dev, int total_tiles, int grid, void *stream);
+/* C = beta * C (beta == 0 writes zeros without reading C) */
+int bxh_scale(int dtype, int m, int n, const double *beta, void *C, int64_t ldc, void *stream);
+int bxh_fill_uniform(int dtype, void *p, int64_t rows, int64_t cols, int64_t ld,
+                     uint64_t seed, uint32_t id, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

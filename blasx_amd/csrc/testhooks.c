@@ -1,0 +1,78 @@
+/*
+ * testhooks.c -- exported hooks that let the CPU test-suite drive the
+ * host-side policies of the library without a GPU: the tile cache over
+ * host memory and the task queue.  Not part of the BLAS ABI.
+ */
+#include <stdlib.h>
+
+#include "internal.h"
+
+static int host_chunk_alloc(void *u, void **p, size_t bytes)
+{
+    (void)u;
+    *p = malloc(bytes ? bytes : 1);
+    return *p ? 0 : BLASX_ENOMEM;
+}
+
+static void host_chunk_free(void *u, void *p)
+{
+    (void)u;
+    free(p);
+}
+
+BX_EXPORT void *bxtest_cache_create(size_t slot_bytes, size_t chunk_bytes, size_t budget)
+{
+    bx_cache_ops_t ops = {host_chunk_alloc, host_chunk_free, NULL};
+    return bx_cache_create(slot_bytes, chunk_bytes, budget, &ops);
+}
+
+BX_EXPORT void bxtest_cache_destroy(void *c) { bx_cache_destroy((bx_cache_t *)c); }
+
+BX_EXPORT int bxtest_cache_acquire(void *c, uintptr_t addr, int64_t ld, int rows, int cols,
+                                   int esize, int64_t round, int *hit, int64_t *evicted_round)
+{
+    bx_tile_key_t k = {addr, ld, rows, cols, esize, 0};
+    return bx_cache_acquire((bx_cache_t *)c, &k, round, hit, evicted_round);
+}
+
+BX_EXPORT int bxtest_cache_lookup(void *c, uintptr_t addr, int64_t ld, int rows, int cols,
+                                  int esize)
+{
+    bx_tile_key_t k = {addr, ld, rows, cols, esize, 0};
+    return bx_cache_lookup((bx_cache_t *)c, &k);
+}
+
+BX_EXPORT int bxtest_cache_capacity(void *c) { return bx_cache_capacity_slots((bx_cache_t *)c); }
+BX_EXPORT int bxtest_cache_live(void *c) { return bx_cache_live_slots((bx_cache_t *)c); }
+BX_EXPORT int bxtest_cache_take_scratch(void *c, int64_t round)
+{
+    return bx_cache_take_scratch((bx_cache_t *)c, round);
+}
+BX_EXPORT void bxtest_cache_return_scratch(void *c, int slot)
+{
+    bx_cache_return_scratch((bx_cache_t *)c, slot);
+}
+BX_EXPORT void bxtest_cache_invalidate_all(void *c) { bx_cache_invalidate_all((bx_cache_t *)c); }
+BX_EXPORT void bxtest_cache_counters(void *c, uint64_t *h, uint64_t *m, uint64_t *e)
+{
+    bx_cache_counters((bx_cache_t *)c, h, m, e);
+}
+BX_EXPORT void *bxtest_cache_slot_ptr(void *c, int slot)
+{
+    return bx_cache_slot_ptr((bx_cache_t *)c, slot);
+}
+
+BX_EXPORT void *bxtest_queue_create(int64_t n)
+{
+    bx_queue_t *q = (bx_queue_t *)malloc(sizeof *q);
+    if (q) bx_queue_init(q, n);
+    return q;
+}
+BX_EXPORT int64_t bxtest_queue_next(void *q) { return bx_queue_next((bx_queue_t *)q); }
+BX_EXPORT void bxtest_queue_destroy(void *q) { free(q); }
+
+/* tile-size heuristic, without a GPU */
+BX_EXPORT int bxtest_autotune_tile(int dt, int M, int N, int K)
+{
+    return bx_autotune_tile(dt, M, N, K);
+}

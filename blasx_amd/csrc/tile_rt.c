@@ -1,0 +1,450 @@
+/*
+ * tile_rt.c -- tiled GEMM over host-resident operands, one MI355X.
+ *
+ * Replaces the reference's tiled runtime: blasx_dgemm / blasx_gpu_dgemm /
+ * blasx_gpu_dgemm_kernel (blas/blasx_dgemm.c:5-356 and its s/z twins),
+ * the fetch path of mem_control_kernel_* (blas/blasx_mem_control.c:222-340),
+ * margin_adjustment (:24-42) and collect_final_result_* (:65-156).
+ *
+ * Same decomposition as the reference: C is cut into T x T tiles; the
+ * product for C tile (i,k) sums op(A) tile (i,j) * op(B) tile (j,k) over
+ * the K tiles j in order; for a transposed operand the stored tile (j,i)
+ * is fetched (blas/blasx_dgemm.c:42-53); edge tiles carry the remainder
+ * extents; tiles are fetched through the device cache keyed on the host
+ * address.  Redesigned for one GPU:
+ *   * the work unit is a ROUND: an RM x RN block of C tiles, traversed
+ *     block-row by block-row with alternating direction so the B panels of
+ *     one block row are still cached when the next starts;
+ *   * one grouped kernel launch per round computes every C tile of the
+ *     round over all of K (one segment per cached K tile), so C is read
+ *     and written once per call, not once per K step with
+ *     beta_inner = 1 (blas/blasx_dgemm.c:82);
+ *   * three streams: copy-in (operand and C tiles), compute, copy-out;
+ *     they are ordered with events, the host never waits per tile
+ *     (blas/blasx_dgemm.c:79 waits per tile, :186 spins per round);
+ *   * up to DEPTH rounds are in flight; C staging and descriptor buffers
+ *     rotate over DEPTH slots;
+ *   * a tile evicted while a recent round may still read it is
+ *     overwritten only after that round's compute event;
+ *   * all offsets are 64-bit (the reference's int offsets overflow at
+ *     N = 131072, blas/blasx_mem_control.c:137-138).
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "internal.h"
+
+#define BX_SLOT_BYTES ((size_t)2048 * 2048 * 8) /* one FP64 2048^2 tile: 32 MiB */
+#define BX_CHUNK_BYTES ((size_t)1 << 30)
+#define DEPTH 3
+
+static bx_cache_t *g_cache;
+
+static int dev_chunk_alloc(void *u, void **p, size_t bytes)
+{
+    (void)u;
+    return bxh_malloc(p, bytes);
+}
+
+static void dev_chunk_free(void *u, void *p)
+{
+    (void)u;
+    bxh_free(p);
+}
+
+bx_cache_t *bx_global_cache(void) { return g_cache; }
+
+static bx_cache_t *get_cache(void)
+{
+    if (!g_cache) {
+        bx_cache_ops_t ops = {dev_chunk_alloc, dev_chunk_free, NULL};
+        g_cache = bx_cache_create(BX_SLOT_BYTES, BX_CHUNK_BYTES, g_bx.hbm_budget, &ops);
+    }
+    return g_cache;
+}
+
+/* events and descriptor buffers, created once, guarded by g_bx.gpu_lock */
+static struct {
+    int ready;
+    void *ev_h2d[DEPTH], *ev_comp[DEPTH], *ev_d2h[DEPTH];
+    void *desc_host[DEPTH], *desc_dev[DEPTH];
+    size_t desc_cap;
+} g_rt;
+
+static int ensure_events(void)
+{
+    if (g_rt.ready) return 0;
+    for (int d = 0; d < DEPTH; ++d) {
+        int rc = bxh_event_create(&g_rt.ev_h2d[d], 0);
+        if (!rc) rc = bxh_event_create(&g_rt.ev_comp[d], 0);
+        if (!rc) rc = bxh_event_create(&g_rt.ev_d2h[d], 0);
+        if (rc) return rc;
+    }
+    g_rt.ready = 1;
+    return 0;
+}
+
+static int ensure_desc(size_t bytes)
+{
+    if (bytes <= g_rt.desc_cap) return 0;
+    size_t cap = g_rt.desc_cap ? g_rt.desc_cap : 65536;
+    while (cap < bytes) cap *= 2;
+    for (int d = 0; d < DEPTH; ++d) {
+        bxh_host_free(g_rt.desc_host[d]);
+        bxh_free(g_rt.desc_dev[d]);
+        g_rt.desc_host[d] = g_rt.desc_dev[d] = NULL;
+    }
+    g_rt.desc_cap = 0;
+    for (int d = 0; d < DEPTH; ++d) {
+        int rc = bxh_host_malloc(&g_rt.desc_host[d], cap);
+        if (!rc) rc = bxh_malloc(&g_rt.desc_dev[d], cap);
+        if (rc) return rc;
+    }
+    g_rt.desc_cap = cap;
+    return 0;
+}
+
+int bx_tune_lookup(int dt, int M, int N, int K); /* autotune.c */
+
+int bx_autotune_tile(int dt, int M, int N, int K)
+{
+    int t = bx_tune_lookup(dt, M, N, K);
+    if (t > 0) return t;
+    const int es = bx_esize(dt);
+    int T = es >= 16 ? 1024 : 2048;
+    /* enough C tiles to form pipelined rounds */
+    while (T > 512 && (int64_t)((M + T - 1) / T) * ((N + T - 1) / T) < 16) T /= 2;
+    return T;
+}
+
+/* entries that survive the call: tiles inside a registered resident range */
+static int keep_resident(const bx_tile_key_t *k, void *u)
+{
+    (void)u;
+    uintptr_t end = k->addr + (uintptr_t)((k->cols - 1) * k->ld + k->rows) * k->esize;
+    return bx_resident_contains(k->addr, end);
+}
+
+typedef struct plan {
+    int dt, es, opa, opb, M, N, K, T;
+    int mt, nt, kt, RM, RN;
+    const char *A, *B;
+    char *C;
+    int64_t lda, ldb, ldc;
+    double alpha[2], beta[2];
+    int beta_zero;
+} plan_t;
+
+static void choose_round_shape(plan_t *p, int cap_slots)
+{
+    const int bm = bxh_tile_m(p->dt), bn = bxh_tile_n(p->dt);
+    const int64_t blocks = (int64_t)((p->T + bm - 1) / bm) * ((p->T + bn - 1) / bn);
+    int R = (int)((2048 + blocks - 1) / blocks);
+    if (R < 1) R = 1;
+    if (R > 16) R = 16;
+    int RM = 1;
+    while ((RM + 1) * (RM + 1) <= R) ++RM;
+    int RN = (R + RM - 1) / RM;
+    if (RM > p->mt) RM = p->mt;
+    if (RN > p->nt) RN = p->nt;
+    /* two rounds of operand tiles plus DEPTH rounds of C staging must fit */
+    while ((RM > 1 || RN > 1) &&
+           (int64_t)2 * (RM + RN) * p->kt + (int64_t)DEPTH * RM * RN + 2 > cap_slots) {
+        if (RN >= RM && RN > 1)
+            --RN;
+        else
+            --RM;
+    }
+    p->RM = RM;
+    p->RN = RN;
+}
+
+static int run_plan(plan_t *p, bx_cache_t *c)
+{
+    const int es = p->es, T = p->T;
+    const int R = p->RM * p->RN;
+    const int bm = bxh_tile_m(p->dt), bn = bxh_tile_n(p->dt);
+    const size_t prob_bytes = (size_t)R * sizeof(bxh_problem_t);
+    const size_t seg_off_bytes = (prob_bytes + 255) & ~(size_t)255;
+    const size_t desc_bytes = seg_off_bytes + (size_t)R * p->kt * sizeof(bxh_seg_t);
+    int rc = ensure_desc(desc_bytes);
+    if (rc) return rc;
+
+    if ((int64_t)2 * (p->RM + p->RN) * p->kt + (int64_t)DEPTH * R + 2 >
+        bx_cache_capacity_slots(c)) {
+        bx_log(0, "tile cache budget too small: %d slots for K/T = %d", bx_cache_capacity_slots(c),
+               p->kt);
+        return BLASX_ERANGE;
+    }
+
+    /* C staging slots: DEPTH x R, taken out of the cache for this call */
+    int *cst = (int *)malloc(sizeof(int) * DEPTH * R);
+    int *sa = (int *)malloc(sizeof(int) * p->RM);
+    int *sb = (int *)malloc(sizeof(int) * p->RN);
+    if (!cst || !sa || !sb) {
+        free(cst);
+        free(sa);
+        free(sb);
+        return BLASX_ENOMEM;
+    }
+    int ntaken = 0;
+    for (; ntaken < DEPTH * R; ++ntaken) {
+        cst[ntaken] = bx_cache_take_scratch(c, 0);
+        if (cst[ntaken] < 0) {
+            rc = BLASX_EBUSY;
+            break;
+        }
+    }
+
+    uint64_t h2d = 0, d2h = 0, hits = 0, misses = 0, evictions = 0;
+    const int nbm = (p->mt + p->RM - 1) / p->RM, nbn = (p->nt + p->RN - 1) / p->RN;
+    int64_t round = 0;
+    for (int bi = 0; bi < nbm && !rc; ++bi) {
+        for (int q = 0; q < nbn && !rc; ++q, ++round) {
+            const int bj = (bi & 1) ? nbn - 1 - q : q; /* alternate direction */
+            const int d = (int)(round % DEPTH);
+            if (round >= DEPTH) {
+                /* at most DEPTH rounds in flight: round-DEPTH has computed */
+                if ((rc = bxh_event_sync(g_rt.ev_comp[d]))) break;
+                /* its C staging must be copied out before reuse */
+                if ((rc = bxh_stream_wait_event(g_bx.s_h2d, g_rt.ev_d2h[d]))) break;
+                if ((rc = bxh_stream_wait_event(g_bx.s_comp, g_rt.ev_d2h[d]))) break;
+            }
+            const int i0 = bi * p->RM, i1 = (i0 + p->RM < p->mt) ? i0 + p->RM : p->mt;
+            const int k0 = bj * p->RN, k1 = (k0 + p->RN < p->nt) ? k0 + p->RN : p->nt;
+            const int ni = i1 - i0, nk = k1 - k0;
+            char *desc = (char *)g_rt.desc_host[d];
+            bxh_problem_t *prob = (bxh_problem_t *)desc;
+            bxh_seg_t *seg = (bxh_seg_t *)(desc + seg_off_bytes);
+
+            /* C tiles in, problem descriptors */
+            int tiles_total = 0;
+            for (int ii = 0; ii < ni; ++ii)
+                for (int kk = 0; kk < nk; ++kk) {
+                    const int t = ii * nk + kk;
+                    const int i = i0 + ii, k = k0 + kk;
+                    const int mc = (p->M - i * T < T) ? p->M - i * T : T;
+                    const int nc = (p->N - k * T < T) ? p->N - k * T : T;
+                    void *cs = bx_cache_slot_ptr(c, cst[d * R + t]);
+                    if (!p->beta_zero) {
+                        const char *hc = p->C + ((int64_t)i * T + (int64_t)k * T * p->ldc) * es;
+                        rc = bxh_memcpy2d_h2d_async(cs, (size_t)T * es, hc, (size_t)p->ldc * es,
+                                                    (size_t)mc * es, (size_t)nc, g_bx.s_h2d);
+                        if (rc) break;
+                        h2d += (uint64_t)mc * nc * es;
+                    }
+                    bxh_problem_t *P = &prob[t];
+                    memset(P, 0, sizeof *P);
+                    P->C = cs;
+                    P->ldc = T;
+                    P->m = mc;
+                    P->n = nc;
+                    P->nseg = p->kt;
+                    P->seg_off = t * p->kt;
+                    P->opa = p->opa;
+                    P->opb = p->opb;
+                    P->tiles_m = (mc + bm - 1) / bm;
+                    P->tiles_n = (nc + bn - 1) / bn;
+                    P->tile_start = tiles_total;
+                    tiles_total += P->tiles_m * P->tiles_n;
+                    P->beta_zero = p->beta_zero;
+                    P->alpha_re = p->alpha[0];
+                    P->alpha_im = p->alpha[1];
+                    P->beta_re = p->beta[0];
+                    P->beta_im = p->beta[1];
+                }
+            if (rc) break;
+
+            /* operand tiles of every K step, through the cache */
+            for (int j = 0; j < p->kt && !rc; ++j) {
+                const int kext = (p->K - j * T < T) ? p->K - j * T : T;
+                for (int ii = 0; ii < ni && !rc; ++ii) {
+                    const int i = i0 + ii;
+                    const int mc = (p->M - i * T < T) ? p->M - i * T : T;
+                    int64_t r0, c0;
+                    int rows, cols;
+                    if (p->opa == BX_OP_N) {
+                        r0 = (int64_t)i * T, c0 = (int64_t)j * T, rows = mc, cols = kext;
+                    } else {
+                        r0 = (int64_t)j * T, c0 = (int64_t)i * T, rows = kext, cols = mc;
+                    }
+                    bx_tile_key_t key = {(uintptr_t)(p->A + (r0 + c0 * p->lda) * es), p->lda,
+                                         rows, cols, es, 0};
+                    int hit;
+                    int64_t ev;
+                    sa[ii] = bx_cache_acquire(c, &key, round, &hit, &ev);
+                    if (sa[ii] < 0) {
+                        rc = BLASX_EBUSY;
+                        break;
+                    }
+                    if (hit) {
+                        ++hits;
+                        continue;
+                    }
+                    ++misses;
+                    if (ev >= 0) {
+                        ++evictions;
+                        if (round - ev < DEPTH)
+                            rc = bxh_stream_wait_event(g_bx.s_h2d, g_rt.ev_comp[ev % DEPTH]);
+                    }
+                    if (!rc)
+                        rc = bxh_memcpy2d_h2d_async(bx_cache_slot_ptr(c, sa[ii]), (size_t)T * es,
+                                                    (const void *)key.addr, (size_t)p->lda * es,
+                                                    (size_t)rows * es, (size_t)cols, g_bx.s_h2d);
+                    h2d += (uint64_t)rows * cols * es;
+                }
+                for (int kk = 0; kk < nk && !rc; ++kk) {
+                    const int k = k0 + kk;
+                    const int nc = (p->N - k * T < T) ? p->N - k * T : T;
+                    int64_t r0, c0;
+                    int rows, cols;
+                    if (p->opb == BX_OP_N) {
+                        r0 = (int64_t)j * T, c0 = (int64_t)k * T, rows = kext, cols = nc;
+                    } else {
+                        r0 = (int64_t)k * T, c0 = (int64_t)j * T, rows = nc, cols = kext;
+                    }
+                    bx_tile_key_t key = {(uintptr_t)(p->B + (r0 + c0 * p->ldb) * es), p->ldb,
+                                         rows, cols, es, 0};
+                    int hit;
+                    int64_t ev;
+                    sb[kk] = bx_cache_acquire(c, &key, round, &hit, &ev);
+                    if (sb[kk] < 0) {
+                        rc = BLASX_EBUSY;
+                        break;
+                    }
+                    if (hit) {
+                        ++hits;
+                        continue;
+                    }
+                    ++misses;
+                    if (ev >= 0) {
+                        ++evictions;
+                        if (round - ev < DEPTH)
+                            rc = bxh_stream_wait_event(g_bx.s_h2d, g_rt.ev_comp[ev % DEPTH]);
+                    }
+                    if (!rc)
+                        rc = bxh_memcpy2d_h2d_async(bx_cache_slot_ptr(c, sb[kk]), (size_t)T * es,
+                                                    (const void *)key.addr, (size_t)p->ldb * es,
+                                                    (size_t)rows * es, (size_t)cols, g_bx.s_h2d);
+                    h2d += (uint64_t)rows * cols * es;
+                }
+                if (rc) break;
+                for (int ii = 0; ii < ni; ++ii)
+                    for (int kk = 0; kk < nk; ++kk) {
+                        bxh_seg_t *S = &seg[(ii * nk + kk) * p->kt + j];
+                        S->A = bx_cache_slot_ptr(c, sa[ii]);
+                        S->B = bx_cache_slot_ptr(c, sb[kk]);
+                        S->lda = T;
+                        S->ldb = T;
+                        S->k = kext;
+                    }
+            }
+            if (rc) break;
+
+            /* compute after this round's copies; descriptors ride the compute stream */
+            const int nprob = ni * nk;
+            if ((rc = bxh_event_record(g_rt.ev_h2d[d], g_bx.s_h2d))) break;
+            if ((rc = bxh_stream_wait_event(g_bx.s_comp, g_rt.ev_h2d[d]))) break;
+            if ((rc = bxh_memcpy_h2d_async(g_rt.desc_dev[d], g_rt.desc_host[d],
+                                           seg_off_bytes + (size_t)nprob * p->kt * sizeof(bxh_seg_t),
+                                           g_bx.s_comp)))
+                break;
+            char *ddev = (char *)g_rt.desc_dev[d];
+            if ((rc = bxh_gemm_grouped(p->dt, (const bxh_problem_t *)ddev, nprob,
+                                       (const bxh_seg_t *)(ddev + seg_off_bytes), tiles_total, 0,
+                                       g_bx.s_comp)))
+                break;
+            BX_STAT_ADD(kernel_launches, 1);
+            if ((rc = bxh_event_record(g_rt.ev_comp[d], g_bx.s_comp))) break;
+
+            /* C tiles out */
+            if ((rc = bxh_stream_wait_event(g_bx.s_d2h, g_rt.ev_comp[d]))) break;
+            for (int ii = 0; ii < ni && !rc; ++ii)
+                for (int kk = 0; kk < nk && !rc; ++kk) {
+                    const int t = ii * nk + kk;
+                    const int i = i0 + ii, k = k0 + kk;
+                    const int mc = (p->M - i * T < T) ? p->M - i * T : T;
+                    const int nc = (p->N - k * T < T) ? p->N - k * T : T;
+                    char *hc = p->C + ((int64_t)i * T + (int64_t)k * T * p->ldc) * es;
+                    rc = bxh_memcpy2d_d2h_async(hc, (size_t)p->ldc * es,
+                                                bx_cache_slot_ptr(c, cst[d * R + t]),
+                                                (size_t)T * es, (size_t)mc * es, (size_t)nc,
+                                                g_bx.s_d2h);
+                    d2h += (uint64_t)mc * nc * es;
+                }
+            if (!rc) rc = bxh_event_record(g_rt.ev_d2h[d], g_bx.s_d2h);
+        }
+    }
+
+    /* drain every stream before the buffers are reused or the caller reads C */
+    int rc2 = bxh_stream_sync(g_bx.s_h2d);
+    int rc3 = bxh_stream_sync(g_bx.s_comp);
+    int rc4 = bxh_stream_sync(g_bx.s_d2h);
+    if (!rc) rc = rc2 ? rc2 : rc3 ? rc3 : rc4;
+
+    for (int t = 0; t < ntaken; ++t) bx_cache_return_scratch(c, cst[t]);
+    free(cst);
+    free(sa);
+    free(sb);
+    bx_cache_invalidate(c, keep_resident, NULL);
+
+    BX_STAT_ADD(bytes_h2d, h2d);
+    BX_STAT_ADD(bytes_d2h, d2h);
+    BX_STAT_ADD(tile_hits, hits);
+    BX_STAT_ADD(tile_misses, misses);
+    BX_STAT_ADD(tile_evictions, evictions);
+    return rc;
+}
+
+int bx_tile_gemm(int dt, int opa, int opb, int M, int N, int K, const double *alpha,
+                 const void *A, int64_t lda, const void *B, int64_t ldb, const double *beta,
+                 void *C, int64_t ldc)
+{
+    if (bx_ensure_init() != 0) return BLASX_ENOGPU;
+    pthread_mutex_lock(&g_bx.gpu_lock);
+    int rc = 0;
+    bx_cache_t *c = get_cache();
+    if (!c) rc = BLASX_ENOMEM;
+    if (!rc) rc = ensure_events();
+    if (!rc) {
+        plan_t p;
+        memset(&p, 0, sizeof p);
+        p.dt = dt;
+        p.es = bx_esize(dt);
+        p.opa = opa;
+        p.opb = opb;
+        p.M = M;
+        p.N = N;
+        p.K = K;
+        int T = g_bx.tile_override[dt] ? g_bx.tile_override[dt] : bx_autotune_tile(dt, M, N, K);
+        while ((size_t)T * T * p.es > bx_cache_slot_bytes(c)) T /= 2;
+        p.T = T;
+        p.mt = (M + T - 1) / T;
+        p.nt = (N + T - 1) / T;
+        p.kt = (K + T - 1) / T;
+        p.A = (const char *)A;
+        p.B = (const char *)B;
+        p.C = (char *)C;
+        p.lda = lda;
+        p.ldb = ldb;
+        p.ldc = ldc;
+        p.alpha[0] = alpha[0];
+        p.alpha[1] = alpha[1];
+        p.beta[0] = beta[0];
+        p.beta[1] = beta[1];
+        p.beta_zero = beta[0] == 0.0 && beta[1] == 0.0;
+        choose_round_shape(&p, bx_cache_capacity_slots(c));
+        bx_cache_reset_rounds(c);
+        bx_log(2, "tile gemm dt=%d %dx%dx%d T=%d rounds of %dx%d tiles", dt, M, N, K, T, p.RM,
+               p.RN);
+        rc = run_plan(&p, c);
+        if (!rc) {
+            BX_STAT_ADD(bytes_lower_bound,
+                        (uint64_t)p.es * ((uint64_t)M * K + (uint64_t)K * N +
+                                          (uint64_t)M * N * (p.beta_zero ? 1 : 2)));
+            bx_stat_flops((dt == BLASX_Z ? 8.0 : 2.0) * M * (double)N * K);
+        }
+    }
+    pthread_mutex_unlock(&g_bx.gpu_lock);
+    return rc;
+}

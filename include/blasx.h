@@ -1,0 +1,75 @@
+/*
+ * blasx.h -- public C ABI of libblasx.so, the MI355X-native tiled GEMM
+ * library.  Drop-in for the GEMM path of a system BLAS (LD_PRELOAD,
+ * LD_LIBRARY_PATH substitution, MATLAB BLAS_VERSION).
+ *
+ * Every entry point below replaces the reference interface cited beside it.
+ * Integers are LP64 `int` (the reference's blasint, include/f77blas_auxiliary.h:70-80).
+ * Matrices are column-major; complex values are interleaved (re, im) doubles.
+ */
+#ifndef BLASX_H
+#define BLASX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* CBLAS enums, same values as the reference's include/cblas.h:11-12 */
+#ifndef CBLAS_H
+enum CBLAS_ORDER { CblasRowMajor = 101, CblasColMajor = 102 };
+enum CBLAS_TRANSPOSE { CblasNoTrans = 111, CblasTrans = 112, CblasConjTrans = 113 };
+#endif
+
+/* ---- CBLAS entry points -------------------------------------------------
+ * replaces cblas_sgemm, blas/sgemm.c:16-127 (prototype include/cblas.h:424-428) */
+void cblas_sgemm(const enum CBLAS_ORDER Order, const enum CBLAS_TRANSPOSE TransA,
+                 const enum CBLAS_TRANSPOSE TransB, const int M, const int N, const int K,
+                 const float alpha, const float *A, const int lda, const float *B,
+                 const int ldb, const float beta, float *C, const int ldc);
+
+/* replaces cblas_dgemm, blas/dgemm.c:18-128 (prototype include/cblas.h:429-433) */
+void cblas_dgemm(const enum CBLAS_ORDER Order, const enum CBLAS_TRANSPOSE TransA,
+                 const enum CBLAS_TRANSPOSE TransB, const int M, const int N, const int K,
+                 const double alpha, const double *A, const int lda, const double *B,
+                 const int ldb, const double beta, double *C, const int ldc);
+
+/* replaces cblas_cgemm, blas/cgemm.c:16-142 (CPU forward, as in the reference) */
+void cblas_cgemm(const enum CBLAS_ORDER Order, const enum CBLAS_TRANSPOSE TransA,
+                 const enum CBLAS_TRANSPOSE TransB, const int M, const int N, const int K,
+                 const void *alpha, const void *A, const int lda, const void *B,
+                 const int ldb, const void *beta, void *C, const int ldc);
+
+/* replaces cblas_zgemm, blas/zgemm.c:16-130 (prototype include/cblas.h:514-518) */
+void cblas_zgemm(const enum CBLAS_ORDER Order, const enum CBLAS_TRANSPOSE TransA,
+                 const enum CBLAS_TRANSPOSE TransB, const int M, const int N, const int K,
+                 const void *alpha, const void *A, const int lda, const void *B,
+                 const int ldb, const void *beta, void *C, const int ldc);
+
+/* ---- Fortran-77 entry points (hidden string lengths ignored) --------------
+ * replaces sgemm_, blas/sgemm.c:130-153 */
+void sgemm_(const char *transa, const char *transb, const int *m, const int *n, const int *k,
+            const float *alpha, const float *A, const int *lda, const float *B, const int *ldb,
+            const float *beta, float *C, const int *ldc);
+/* replaces dgemm_, blas/dgemm.c:131-154 */
+void dgemm_(const char *transa, const char *transb, const int *m, const int *n, const int *k,
+            const double *alpha, const double *A, const int *lda, const double *B,
+            const int *ldb, const double *beta, double *C, const int *ldc);
+/* replaces cgemm_, blas/cgemm.c (CPU forward) */
+void cgemm_(const char *transa, const char *transb, const int *m, const int *n, const int *k,
+            const void *alpha, const void *A, const int *lda, const void *B, const int *ldb,
+            const void *beta, void *C, const int *ldc);
+/* replaces zgemm_, blas/zgemm.c:133-156 */
+void zgemm_(const char *transa, const char *transb, const int *m, const int *n, const int *k,
+            const void *alpha, const void *A, const int *lda, const void *B, const int *ldb,
+            const void *beta, void *C, const int *ldc);
+
+/* replaces xerbla_, blas/error.c:5-8.  Weak: a caller (LAPACK) may override it. */
+void xerbla_(const char *srname, const int *info);
+
+#ifdef __cplusplus
+}
+#endif
+
+#include "blasx_ext.h"
+
+#endif /* BLASX_H */

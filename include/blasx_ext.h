@@ -1,0 +1,126 @@
+/*
+ * blasx_ext.h -- extension API of libblasx.so beyond the BLAS ABI.
+ *
+ * The reference has no counterpart for most of these: it exposes only the
+ * BLAS symbols and keeps device memory, streams and its tile cache private
+ * and per call (blas/dgemm.c:104-114).  The functions here give callers
+ * device-resident GEMMs, cross-call operand residency, pinned host memory,
+ * statistics, split-precision GEMM, grouped GEMM and the GEMM server.
+ *
+ * All functions return 0 on success and a negative BLASX_E* code on error
+ * unless stated otherwise.  Device pointers are HBM addresses on the current
+ * GPU; `stream` arguments are hipStream_t values passed as void* (NULL = the
+ * library's own compute stream).
+ */
+#ifndef BLASX_EXT_H
+#define BLASX_EXT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* element types */
+enum { BLASX_S = 0, BLASX_D = 1, BLASX_Z = 2, BLASX_C = 3 };
+
+/* error codes */
+enum {
+    BLASX_OK = 0,
+    BLASX_ENOGPU = -1,     /* no usable GPU */
+    BLASX_EINVAL = -2,     /* invalid argument */
+    BLASX_ENOMEM = -3,     /* device or pinned allocation failed */
+    BLASX_EHIP = -4,       /* HIP runtime error */
+    BLASX_ECPU = -5,       /* CPU BLAS could not be loaded */
+    BLASX_EBUSY = -6,      /* all cache slots pinned / server full */
+    BLASX_ERANGE = -7      /* problem outside a documented limit */
+};
+
+/* ---- library lifetime ----------------------------------------------------
+ * Lazy by default (first GEMM call), as the reference's blasx_init
+ * (blas/blasx_internal.c:79-100).  Explicit init is optional. */
+int blasx_init(void);
+void blasx_finalize(void);
+int blasx_gpu_available(void);          /* 1 if a GPU path is usable */
+const char *blasx_version(void);
+const char *blasx_cpublas_path(void);   /* the dlopen'ed CPU BLAS, "" if none */
+
+/* ---- statistics ---------------------------------------------------------- */
+typedef struct blasx_stats {
+    uint64_t calls_gpu;          /* ABI calls that ran on the GPU */
+    uint64_t calls_cpu;          /* ABI calls forwarded to the CPU BLAS by size rule */
+    uint64_t cpu_fallbacks;      /* GPU-sized calls run on the CPU (only with BLASX_CPU_FALLBACK=1) */
+    uint64_t kernel_launches;    /* GEMM kernel launches */
+    uint64_t bytes_h2d;          /* host -> device bytes moved by the tile runtime */
+    uint64_t bytes_d2h;          /* device -> host bytes */
+    uint64_t bytes_lower_bound;  /* sum over calls of e*(|A|+|B|+|C| in (+ out)) */
+    uint64_t tile_hits;
+    uint64_t tile_misses;
+    uint64_t tile_evictions;
+    uint64_t tiles_generated;    /* unused by the ABI path */
+    double flops;                /* useful FLOPs executed on the GPU (2MNK / 8MNK) */
+} blasx_stats_t;
+
+void blasx_get_stats(blasx_stats_t *out);
+void blasx_reset_stats(void);
+
+/* ---- configuration (also read from the environment at init) --------------
+ * BLASX_HBM_BUDGET bytes of tile cache (default: 90% of free HBM)
+ * BLASX_TILE       tile edge for host-resident GEMMs (default: autotuned)
+ * BLASX_GPU_MIN_DIM size threshold (default 1000 for S/D, 500 for Z,
+ *                  the reference's blas/dgemm.c:62 and blas/zgemm.c:61)
+ * BLASX_CPUBLAS    path of the CPU BLAS to dlopen
+ * BLASX_CPU_FALLBACK=1  run GPU-sized calls on the CPU when no GPU works
+ *                  (default: print an error and abort, never silently) */
+int blasx_set_hbm_budget(size_t bytes);
+size_t blasx_get_hbm_budget(void);
+int blasx_set_tile_size(int dtype, int tile);   /* 0 = autotune */
+int blasx_get_tile_size(int dtype, int M, int N, int K);
+
+/* ---- memory ---------------------------------------------------------------- */
+void *blasx_host_alloc(size_t bytes);    /* pinned host memory (hipHostMalloc) */
+void blasx_host_free(void *p);
+void *blasx_dev_alloc(size_t bytes);     /* HBM */
+void blasx_dev_free(void *p);
+int blasx_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int blasx_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int blasx_memset_device(void *dst, int value, size_t bytes);
+int blasx_device_synchronize(void);
+void *blasx_compute_stream(void);        /* hipStream_t of the library's compute stream */
+
+/* ---- device-resident GEMM (operands already in HBM) ----------------------
+ * Same argument meaning as cblas_?gemm with Order = ColMajor; trans values
+ * are the CBLAS enums.  Asynchronous on `stream`.  Returns BLASX_EINVAL with
+ * the BLAS info number written to *info (if non-NULL) on bad arguments. */
+int blasx_sgemm_device(int transa, int transb, int M, int N, int K, float alpha,
+                       const float *dA, int64_t lda, const float *dB, int64_t ldb, float beta,
+                       float *dC, int64_t ldc, void *stream);
+int blasx_dgemm_device(int transa, int transb, int M, int N, int K, double alpha,
+                       const double *dA, int64_t lda, const double *dB, int64_t ldb,
+                       double beta, double *dC, int64_t ldc, void *stream);
+int blasx_zgemm_device(int transa, int transb, int M, int N, int K, const double *alpha,
+                       const double *dA, int64_t lda, const double *dB, int64_t ldb,
+                       const double *beta, double *dC, int64_t ldc, void *stream);
+
+/* ---- cross-call residency ---------------------------------------------------
+ * The reference frees its tile cache after every call (blas/dgemm.c:114).
+ * A caller that promises not to modify [ptr, ptr+bytes) until it
+ * unregisters the range lets cached tiles of it survive between calls. */
+int blasx_resident_register(const void *host_ptr, size_t bytes);
+int blasx_resident_unregister(const void *host_ptr);
+void blasx_cache_flush(void);
+
+/* ---- seeded inputs --------------------------------------------------------
+ * Element (i,j) of a rows x cols matrix = SplitMix64(seed ^ id<<56 ^ (i+j*rows))
+ * mapped to uniform[-1,1); complex uses counters 2e (re) and 2e+1 (im). */
+int blasx_fill_uniform_device(int dtype, void *dptr, int64_t rows, int64_t cols, int64_t ld,
+                              uint64_t seed, uint32_t id, void *stream);
+int blasx_fill_uniform_host(int dtype, void *hptr, int64_t rows, int64_t cols, int64_t ld,
+                            uint64_t seed, uint32_t id);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BLASX_EXT_H */

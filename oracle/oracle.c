@@ -1,0 +1,433 @@
+/*
+ * oracle.c -- CPU restatement of the reference GEMM path, used ONLY as the
+ * checker by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+ * Nothing in the product (blasx_amd/) links, loads or calls this file.
+ *
+ * What it restates (reference file:line):
+ *   - the BLAS GEMM definition the reference delegates to cuBLAS/OpenBLAS:
+ *     C <- alpha*op(A)*op(B) + beta*C, column-major, op in {N,T,C}
+ *     (blas/dgemm.c:18-128, blas/sgemm.c:16-127, blas/zgemm.c:16-130; per
+ *     tile blas/blasx_dgemm.c:82-94, blas/blasx_zgemm.c:82)
+ *   - argument validation order and Fortran info numbers
+ *     (blas/dgemm.c:29-57, blas/zgemm.c:26-56, f77 shim blas/dgemm.c:131-154)
+ *   - the tile decomposition: tile grid (blas/blasx_dgemm.c:284-296), C tile
+ *     (i,k) = task/(y+1), task%(y+1) (blas/blasx_dgemm.c:28-29), transposed
+ *     stored-tile addressing (blas/blasx_dgemm.c:42-53), edge extents
+ *     (blas/blasx_mem_control.c:24-42), beta_inner = beta at j==0 else 1
+ *     (blas/blasx_dgemm.c:82)
+ *   - the FLOP formulas (include/flops.h:218-221)
+ *   - the seeded counter-based generator that defines every benchmark input
+ *     (no counterpart in the reference, whose driver uses unseeded rand(),
+ *     testing/gemm.c:10-24)
+ *   - a double-double (Ogita-Rump-Oishi TwoSum/TwoProd) GEMM used as the
+ *     high-accuracy oracle for the split-precision GEMM.
+ *
+ * Compiled with -ffp-contract=off so that every a*b+c below rounds twice
+ * unless fma() is written explicitly.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <complex.h>
+
+/* ------------------------------------------------------------------ */
+/* Generator: SplitMix64 (Steele, Lea, Flood 2014) of                  */
+/*   seed ^ (id << 56) ^ index                                         */
+/* uniform[-1,1)  = ((x >> 11) * 2^-53) * 2 - 1                        */
+/* ------------------------------------------------------------------ */
+uint64_t orc_splitmix64(uint64_t x)
+{
+    uint64_t z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static inline uint64_t orc_key(uint64_t seed, uint32_t id, uint64_t idx)
+{
+    return seed ^ ((uint64_t)id << 56) ^ idx;
+}
+
+double orc_uniform(uint64_t seed, uint32_t id, uint64_t idx)
+{
+    uint64_t x = orc_splitmix64(orc_key(seed, id, idx));
+    return ((double)(x >> 11) * 0x1.0p-53) * 2.0 - 1.0;
+}
+
+/* open unit interval (0,1] draw used by Box-Muller */
+static double orc_unit_open(uint64_t seed, uint32_t id, uint64_t idx)
+{
+    uint64_t x = orc_splitmix64(orc_key(seed, id, idx));
+    return ((double)(x >> 11) + 1.0) * 0x1.0p-53;
+}
+
+/* Element (i,j) of a rows x cols matrix uses linear index i + j*rows
+ * (independent of the storage leading dimension). */
+void orc_fill_uniform_d(double *out, int64_t rows, int64_t cols, int64_t ld,
+                        uint64_t seed, uint32_t id)
+{
+    for (int64_t j = 0; j < cols; ++j)
+        for (int64_t i = 0; i < rows; ++i)
+            out[i + j * ld] = orc_uniform(seed, id, (uint64_t)(i + j * rows));
+}
+
+void orc_fill_uniform_s(float *out, int64_t rows, int64_t cols, int64_t ld,
+                        uint64_t seed, uint32_t id)
+{
+    for (int64_t j = 0; j < cols; ++j)
+        for (int64_t i = 0; i < rows; ++i)
+            out[i + j * ld] = (float)orc_uniform(seed, id, (uint64_t)(i + j * rows));
+}
+
+/* complex: real part from counter 2*idx, imaginary from 2*idx+1 */
+void orc_fill_uniform_z(double *out, int64_t rows, int64_t cols, int64_t ld,
+                        uint64_t seed, uint32_t id)
+{
+    for (int64_t j = 0; j < cols; ++j)
+        for (int64_t i = 0; i < rows; ++i) {
+            uint64_t e = (uint64_t)(i + j * rows);
+            out[2 * (i + j * ld)] = orc_uniform(seed, id, 2 * e);
+            out[2 * (i + j * ld) + 1] = orc_uniform(seed, id, 2 * e + 1);
+        }
+}
+
+/* Dynamic-range matrix a = (u - 0.5) * exp(phi * g), u ~ U(0,1), g ~ N(0,1).
+ * u uses matrix id `id` (counter 3*idx), the Box-Muller pair uses counters
+ * 3*idx+1 and 3*idx+2. */
+double orc_dynrange(uint64_t seed, uint32_t id, uint64_t idx, double phi)
+{
+    double u = (orc_uniform(seed, id, 3 * idx) + 1.0) * 0.5;
+    double u1 = orc_unit_open(seed, id, 3 * idx + 1);
+    double u2 = orc_unit_open(seed, id, 3 * idx + 2);
+    double g = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    return (u - 0.5) * exp(phi * g);
+}
+
+void orc_fill_dynrange_d(double *out, int64_t rows, int64_t cols, int64_t ld,
+                         uint64_t seed, uint32_t id, double phi)
+{
+    for (int64_t j = 0; j < cols; ++j)
+        for (int64_t i = 0; i < rows; ++i)
+            out[i + j * ld] = orc_dynrange(seed, id, (uint64_t)(i + j * rows), phi);
+}
+
+/* ------------------------------------------------------------------ */
+/* Argument validation, restating blas/dgemm.c:29-57 (ColMajor).       */
+/* trans codes: 111 NoTrans, 112 Trans, 113 ConjTrans (include/cblas.h:12) */
+/* Returns the Fortran info number or 0.                               */
+/* ------------------------------------------------------------------ */
+int orc_gemm_info(int transa, int transb, int M, int N, int K, int lda, int ldb, int ldc)
+{
+    int ok_a = (transa == 111 || transa == 112 || transa == 113);
+    int ok_b = (transb == 111 || transb == 112 || transb == 113);
+    int nrowa = (transa == 111) ? M : K;
+    int nrowb = (transb == 111) ? K : N;
+    if (!ok_a) return 1;
+    if (!ok_b) return 2;
+    if (M < 0) return 3;
+    if (N < 0) return 4;
+    if (K < 0) return 5;
+    if (lda < (nrowa > 1 ? nrowa : 1)) return 8;
+    if (ldb < (nrowb > 1 ? nrowb : 1)) return 10;
+    if (ldc < (M > 1 ? M : 1)) return 13;
+    return 0;
+}
+
+/* f77 character -> CBLAS enum, restating blas/blasx_internal.c:161-174 */
+int orc_f77_trans(char c)
+{
+    if (c == 'N' || c == 'n') return 111;
+    if (c == 'T' || c == 't') return 112;
+    if (c == 'C' || c == 'c') return 113;
+    return -1;
+}
+
+/* ------------------------------------------------------------------ */
+/* Naive triple-loop GEMM in the working precision.                   */
+/* ta, tb: 'N','T','C'.  p runs 0..K-1 in order; acc starts at 0.      */
+/* ------------------------------------------------------------------ */
+#define OPIDX(t, r, c, ld) (((t) == 'N') ? ((r) + (int64_t)(c) * (ld)) : ((c) + (int64_t)(r) * (ld)))
+
+void orc_gemm_d(char ta, char tb, int M, int N, int K, double alpha,
+                const double *A, int lda, const double *B, int ldb,
+                double beta, double *C, int ldc)
+{
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < M; ++i) {
+            double acc = 0.0;
+            if (alpha != 0.0)
+                for (int p = 0; p < K; ++p)
+                    acc += A[OPIDX(ta, i, p, lda)] * B[OPIDX(tb, p, j, ldb)];
+            double *c = &C[i + (int64_t)j * ldc];
+            if (beta == 0.0)
+                *c = alpha * acc;
+            else
+                *c = alpha * acc + beta * *c;
+        }
+}
+
+void orc_gemm_s(char ta, char tb, int M, int N, int K, float alpha,
+                const float *A, int lda, const float *B, int ldb,
+                float beta, float *C, int ldc)
+{
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < M; ++i) {
+            float acc = 0.0f;
+            if (alpha != 0.0f)
+                for (int p = 0; p < K; ++p)
+                    acc += A[OPIDX(ta, i, p, lda)] * B[OPIDX(tb, p, j, ldb)];
+            float *c = &C[i + (int64_t)j * ldc];
+            if (beta == 0.0f)
+                *c = alpha * acc;
+            else
+                *c = alpha * acc + beta * *c;
+        }
+}
+
+/* Same loop accumulated in double (a more accurate FP32 reference). */
+void orc_gemm_s_acc64(char ta, char tb, int M, int N, int K, float alpha,
+                      const float *A, int lda, const float *B, int ldb,
+                      float beta, float *C, int ldc)
+{
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < M; ++i) {
+            double acc = 0.0;
+            if (alpha != 0.0f)
+                for (int p = 0; p < K; ++p)
+                    acc += (double)A[OPIDX(ta, i, p, lda)] * (double)B[OPIDX(tb, p, j, ldb)];
+            float *c = &C[i + (int64_t)j * ldc];
+            if (beta == 0.0f)
+                *c = (float)((double)alpha * acc);
+            else
+                *c = (float)((double)alpha * acc + (double)beta * (double)*c);
+        }
+}
+
+/* complex128, interleaved (re,im); alpha/beta given as (re,im) pairs */
+void orc_gemm_z(char ta, char tb, int M, int N, int K, const double *alpha,
+                const double *A, int lda, const double *B, int ldb,
+                const double *beta, double *C, int ldc)
+{
+    double ar = alpha[0], ai = alpha[1], br = beta[0], bi = beta[1];
+    int alpha_zero = (ar == 0.0 && ai == 0.0), beta_zero = (br == 0.0 && bi == 0.0);
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < M; ++i) {
+            double sr = 0.0, si = 0.0;
+            if (!alpha_zero)
+                for (int p = 0; p < K; ++p) {
+                    int64_t ia = OPIDX(ta, i, p, lda), ib = OPIDX(tb, p, j, ldb);
+                    double xr = A[2 * ia], xi = A[2 * ia + 1];
+                    double yr = B[2 * ib], yi = B[2 * ib + 1];
+                    if (ta == 'C') xi = -xi;
+                    if (tb == 'C') yi = -yi;
+                    sr += xr * yr - xi * yi;
+                    si += xr * yi + xi * yr;
+                }
+            double *c = &C[2 * (i + (int64_t)j * ldc)];
+            double outr = ar * sr - ai * si, outi = ar * si + ai * sr;
+            if (!beta_zero) {
+                outr += br * c[0] - bi * c[1];
+                outi += br * c[1] + bi * c[0];
+            }
+            c[0] = outr;
+            c[1] = outi;
+        }
+}
+
+/* ------------------------------------------------------------------ */
+/* Double-double arithmetic (Dekker 1971; Ogita, Rump, Oishi 2005).    */
+/* ------------------------------------------------------------------ */
+typedef struct { double hi, lo; } dd_t;
+
+static inline dd_t two_sum(double a, double b)
+{
+    double s = a + b;
+    double bb = s - a;
+    double err = (a - (s - bb)) + (b - bb);
+    dd_t r = {s, err};
+    return r;
+}
+
+static inline dd_t fast_two_sum(double a, double b)
+{
+    double s = a + b;
+    dd_t r = {s, b - (s - a)};
+    return r;
+}
+
+static inline dd_t two_prod(double a, double b)
+{
+    double p = a * b;
+    dd_t r = {p, fma(a, b, -p)};
+    return r;
+}
+
+/* accurate dd + dd (two TwoSums, then renormalise) */
+static inline dd_t dd_add(dd_t x, dd_t y)
+{
+    dd_t s = two_sum(x.hi, y.hi);
+    dd_t t = two_sum(x.lo, y.lo);
+    s.lo += t.hi;
+    s = fast_two_sum(s.hi, s.lo);
+    s.lo += t.lo;
+    return fast_two_sum(s.hi, s.lo);
+}
+
+static inline dd_t dd_mul_d(dd_t x, double b)
+{
+    dd_t p = two_prod(x.hi, b);
+    p.lo += x.lo * b;
+    return fast_two_sum(p.hi, p.lo);
+}
+
+static inline dd_t dd_mul(dd_t x, dd_t y)
+{
+    dd_t p = two_prod(x.hi, y.hi);
+    p.lo += x.hi * y.lo + x.lo * y.hi;
+    return fast_two_sum(p.hi, p.lo);
+}
+
+static inline dd_t dd_neg(dd_t x) { dd_t r = {-x.hi, -x.lo}; return r; }
+
+/* dd dot-product GEMM: out_hi = round(C), out_lo = residual (may be NULL).
+ * C is read for beta (not when beta == 0) and NOT written; results go to out_hi. */
+void orc_gemm_dd_d(char ta, char tb, int M, int N, int K, double alpha,
+                   const double *A, int lda, const double *B, int ldb,
+                   double beta, const double *C, int ldc,
+                   double *out_hi, double *out_lo, int ldo)
+{
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < M; ++i) {
+            dd_t s = {0.0, 0.0};
+            for (int p = 0; p < K; ++p)
+                s = dd_add(s, two_prod(A[OPIDX(ta, i, p, lda)], B[OPIDX(tb, p, j, ldb)]));
+            dd_t r = dd_mul_d(s, alpha);
+            if (beta != 0.0)
+                r = dd_add(r, two_prod(beta, C[i + (int64_t)j * ldc]));
+            out_hi[i + (int64_t)j * ldo] = r.hi;
+            if (out_lo) out_lo[i + (int64_t)j * ldo] = r.lo;
+        }
+}
+
+/* complex dd GEMM on interleaved storage; outputs interleaved hi and lo */
+void orc_gemm_dd_z(char ta, char tb, int M, int N, int K, const double *alpha,
+                   const double *A, int lda, const double *B, int ldb,
+                   const double *beta, const double *C, int ldc,
+                   double *out_hi, double *out_lo, int ldo)
+{
+    int beta_zero = (beta[0] == 0.0 && beta[1] == 0.0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < M; ++i) {
+            dd_t sr = {0, 0}, si = {0, 0};
+            for (int p = 0; p < K; ++p) {
+                int64_t ia = OPIDX(ta, i, p, lda), ib = OPIDX(tb, p, j, ldb);
+                double xr = A[2 * ia], xi = A[2 * ia + 1];
+                double yr = B[2 * ib], yi = B[2 * ib + 1];
+                if (ta == 'C') xi = -xi;
+                if (tb == 'C') yi = -yi;
+                sr = dd_add(sr, two_prod(xr, yr));
+                sr = dd_add(sr, two_prod(-xi, yi));
+                si = dd_add(si, two_prod(xr, yi));
+                si = dd_add(si, two_prod(xi, yr));
+            }
+            dd_t rr = dd_add(dd_mul_d(sr, alpha[0]), dd_neg(dd_mul_d(si, alpha[1])));
+            dd_t ri = dd_add(dd_mul_d(si, alpha[0]), dd_mul_d(sr, alpha[1]));
+            if (!beta_zero) {
+                const double *c = &C[2 * (i + (int64_t)j * ldc)];
+                rr = dd_add(rr, dd_add(two_prod(beta[0], c[0]), two_prod(-beta[1], c[1])));
+                ri = dd_add(ri, dd_add(two_prod(beta[0], c[1]), two_prod(beta[1], c[0])));
+            }
+            int64_t o = 2 * (i + (int64_t)j * ldo);
+            out_hi[o] = rr.hi;
+            out_hi[o + 1] = ri.hi;
+            if (out_lo) {
+                out_lo[o] = rr.lo;
+                out_lo[o + 1] = ri.lo;
+            }
+        }
+}
+
+/* exposed dd primitives so the tests can pin them against exact rationals */
+void orc_dd_two_sum(double a, double b, double *s, double *e)
+{
+    dd_t r = two_sum(a, b);
+    *s = r.hi;
+    *e = r.lo;
+}
+
+void orc_dd_two_prod(double a, double b, double *p, double *e)
+{
+    dd_t r = two_prod(a, b);
+    *p = r.hi;
+    *e = r.lo;
+}
+
+void orc_dd_dot(const double *x, const double *y, int n, double *hi, double *lo)
+{
+    dd_t s = {0, 0};
+    for (int p = 0; p < n; ++p) s = dd_add(s, two_prod(x[p], y[p]));
+    *hi = s.hi;
+    *lo = s.lo;
+}
+
+/* ------------------------------------------------------------------ */
+/* Tiled GEMM restating the reference's tile decomposition             */
+/* (blas/blasx_dgemm.c:28-94, blas/blasx_mem_control.c:24-42):         */
+/* C tile (i,k), K-steps j = 0..z in order, each step one tile product */
+/* with beta_inner = beta at j==0 else 1.  Used to pin tiling index    */
+/* math: on integer inputs it must equal orc_gemm_d bit for bit.       */
+/* ------------------------------------------------------------------ */
+static void orc_margin(int rows, int cols, int T, int br, int bc, int *nr, int *nc)
+{
+    /* restates blas/blasx_mem_control.c:24-42 */
+    *nr = (br == rows / T) ? rows % T : T;
+    *nc = (bc == cols / T) ? cols % T : T;
+}
+
+void orc_gemm_tiled_d(char ta, char tb, int M, int N, int K, int T, double alpha,
+                      const double *A, int lda, const double *B, int ldb,
+                      double beta, double *C, int ldc)
+{
+    int nrowa = (ta == 'N') ? M : K, ncola = (ta == 'N') ? K : M;
+    int nrowb = (tb == 'N') ? K : N, ncolb = (tb == 'N') ? N : K;
+    /* tile grid: x,y,z = ceil(dim/T)-1 (blas/blasx_dgemm.c:284-296) */
+    int x = (M + T - 1) / T - 1, y = (N + T - 1) / T - 1, z = (K + T - 1) / T - 1;
+    int task_num = (x + 1) * (y + 1);
+    for (int task = task_num - 1; task >= 0; --task) { /* queue order: high to low */
+        int i = task / (y + 1), k = task % (y + 1);
+        int mc, nc;
+        orc_margin(M, N, T, i, k, &mc, &nc);
+        double *Ct = &C[(int64_t)i * T + (int64_t)k * T * ldc];
+        for (int j = 0; j <= z; ++j) {
+            int nra, nca, nrb, ncb;
+            const double *At, *Bt;
+            if (ta == 'N') {
+                orc_margin(nrowa, ncola, T, i, j, &nra, &nca);
+                At = &A[(int64_t)i * T + (int64_t)j * T * lda];
+            } else {
+                orc_margin(nrowa, ncola, T, j, i, &nra, &nca);
+                At = &A[(int64_t)j * T + (int64_t)i * T * lda];
+            }
+            if (tb == 'N') {
+                orc_margin(nrowb, ncolb, T, j, k, &nrb, &ncb);
+                Bt = &B[(int64_t)j * T + (int64_t)k * T * ldb];
+            } else {
+                orc_margin(nrowb, ncolb, T, k, j, &nrb, &ncb);
+                Bt = &B[(int64_t)k * T + (int64_t)j * T * ldb];
+            }
+            int ka = (ta == 'N') ? nca : nra;
+            double beta_inner = (j == 0) ? beta : 1.0;
+            orc_gemm_d(ta, tb, mc, nc, ka, alpha, At, lda, Bt, ldb, beta_inner, Ct, ldc);
+        }
+    }
+}
+
+/* FLOP formulas, include/flops.h:218-221 (with the complex count the
+ * reference's zgemm.c:95 print should have used) */
+double orc_flops_gemm(int complex_, double m, double n, double k)
+{
+    return (complex_ ? 8.0 : 2.0) * m * n * k;
+}

@@ -1,0 +1,314 @@
+"""GPU parity of the GEMM hot path against the CPU oracle.
+
+Tolerance (stated once, used everywhere): relative Frobenius error
+  ||C_gpu - C_ref||_F / (|alpha| ||A||_F ||B||_F + |beta| ||C0||_F) <= c * K * eps
+with c = 1 (real) or 2 (complex) -- the gamma_K bound of Higham 3.5 -- and
+eps of the working precision.  Integer-valued inputs make every partial sum
+exact, so there the GPU must equal the naive loop bit for bit.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EPS = {np.float64: np.finfo(np.float64).eps, np.float32: np.finfo(np.float32).eps,
+       np.complex128: np.finfo(np.float64).eps}
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _op(X, t):
+    return X if t == "N" else (X.T if t == "T" else X.conj().T)
+
+
+def _mat(orc, rows, cols, dt, seed, mid):
+    return orc.fill_uniform(rows, cols, seed, mid, dtype=dt)
+
+
+def _tol(dt, K):
+    return (2 if np.dtype(dt).kind == "c" else 1) * max(K, 1) * EPS[dt]
+
+
+def _check(orc, out, ref, alpha, A, B, beta, C0, K, dt, what):
+    err = orc.relerr_fro(out, ref, alpha, A, B, beta, C0)
+    tol = _tol(dt, K)
+    assert err <= tol, f"{what}: rel err {err:.3e} > {tol:.3e}"
+    return err
+
+
+TRANS_REAL = [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")]
+TRANS_CPLX = [(a, b) for a in "NTC" for b in "NTC"]
+
+
+# ------------------------------------------------------------------ device API
+@pytest.mark.parametrize("dt", [np.float64, np.float32, np.complex128])
+def test_device_gemm_all_transposes_ragged(gpu, orc, dt):
+    bx = gpu
+    M, N, K = 261, 197, 133
+    alpha, beta = (1.5 - 0.5j, 0.5 + 0.25j) if np.dtype(dt).kind == "c" else (1.5, 0.5)
+    for (ta, tb) in (TRANS_CPLX if np.dtype(dt).kind == "c" else TRANS_REAL):
+        say("device", np.dtype(dt).name, ta, tb, M, N, K)
+        A = _mat(orc, M if ta == "N" else K, K if ta == "N" else M, dt, 1, 0)
+        B = _mat(orc, K if tb == "N" else N, N if tb == "N" else K, dt, 1, 1)
+        C0 = _mat(orc, M, N, dt, 1, 2)
+        ref = C0.copy(order="F")
+        orc.gemm(ta, tb, M, N, K, alpha, A, A.shape[0], B, B.shape[0], beta, ref, M,
+                 acc64=(dt == np.float32))
+        dA, dB, dC = (bx.DeviceArray.from_host(x) for x in (A, B, C0))
+        bx.device_gemm(ta, tb, M, N, K, alpha, dA.ptr, A.shape[0], dB.ptr, B.shape[0], beta,
+                       dC.ptr, M, dt)
+        bx.synchronize()
+        out = dC.download()
+        _check(orc, out, ref, alpha, A, B, beta, C0, K, dt, f"{ta}{tb}")
+
+
+@pytest.mark.parametrize("shape", [(128, 128, 16), (256, 384, 512), (1, 1, 1), (7, 300, 5),
+                                   (300, 1, 257), (129, 65, 17)])
+def test_device_dgemm_shapes_integer_exact(gpu, orc, shape):
+    bx = gpu
+    M, N, K = shape
+    rng = np.random.default_rng(M * 7 + N * 3 + K)
+    for ta, tb in TRANS_REAL:
+        say("device int-exact", ta, tb, shape)
+        A = np.asfortranarray(rng.integers(-256, 257, (M, K) if ta == "N" else (K, M)), dtype=np.float64)
+        B = np.asfortranarray(rng.integers(-256, 257, (K, N) if tb == "N" else (N, K)), dtype=np.float64)
+        C0 = np.asfortranarray(rng.integers(-1000, 1001, (M, N)), dtype=np.float64)
+        ref = C0.copy(order="F")
+        orc.gemm(ta, tb, M, N, K, 3.0, A, A.shape[0], B, B.shape[0], -2.0, ref, M)
+        dA, dB, dC = (bx.DeviceArray.from_host(x) for x in (A, B, C0))
+        bx.device_gemm(ta, tb, M, N, K, 3.0, dA.ptr, A.shape[0], dB.ptr, B.shape[0], -2.0,
+                       dC.ptr, M, np.float64)
+        bx.synchronize()
+        assert np.array_equal(dC.download(), ref), (ta, tb, shape)
+
+
+def test_device_sgemm_integer_exact(gpu, orc):
+    bx = gpu
+    rng = np.random.default_rng(11)
+    M, N, K = 200, 150, 1000
+    for ta, tb in TRANS_REAL:
+        say("device sgemm int-exact", ta, tb)
+        A = np.asfortranarray(rng.integers(-16, 17, (M, K) if ta == "N" else (K, M)), dtype=np.float32)
+        B = np.asfortranarray(rng.integers(-16, 17, (K, N) if tb == "N" else (N, K)), dtype=np.float32)
+        C0 = np.asfortranarray(rng.integers(-100, 101, (M, N)), dtype=np.float32)
+        ref = C0.copy(order="F")
+        orc.gemm(ta, tb, M, N, K, 1.0, A, A.shape[0], B, B.shape[0], 1.0, ref, M)
+        dA, dB, dC = (bx.DeviceArray.from_host(x) for x in (A, B, C0))
+        bx.device_gemm(ta, tb, M, N, K, 1.0, dA.ptr, A.shape[0], dB.ptr, B.shape[0], 1.0,
+                       dC.ptr, M, np.float32)
+        bx.synchronize()
+        assert np.array_equal(dC.download(), ref)
+
+
+def test_device_zgemm_integer_exact_conj(gpu, orc):
+    bx = gpu
+    rng = np.random.default_rng(12)
+    M, N, K = 70, 90, 110
+    for ta, tb in TRANS_CPLX:
+        say("device zgemm int-exact", ta, tb)
+        def ci(shape):
+            return np.asfortranarray(rng.integers(-64, 65, shape) + 1j * rng.integers(-64, 65, shape))
+        A = ci((M, K) if ta == "N" else (K, M))
+        B = ci((K, N) if tb == "N" else (N, K))
+        C0 = ci((M, N))
+        al, be = 2 - 1j, 1 + 3j
+        exact = al * (_op(A, ta) @ _op(B, tb)) + be * C0
+        dA, dB, dC = (bx.DeviceArray.from_host(x) for x in (A, B, C0))
+        bx.device_gemm(ta, tb, M, N, K, al, dA.ptr, A.shape[0], dB.ptr, B.shape[0], be,
+                       dC.ptr, M, np.complex128)
+        bx.synchronize()
+        assert np.array_equal(dC.download(), exact), (ta, tb)
+
+
+def test_device_beta_zero_ignores_nan(gpu, orc):
+    bx = gpu
+    M, N, K = 300, 200, 100
+    A = _mat(orc, M, K, np.float64, 3, 0)
+    B = _mat(orc, K, N, np.float64, 3, 1)
+    Cn = np.full((M, N), np.nan, order="F")
+    dA, dB, dC = (bx.DeviceArray.from_host(x) for x in (A, B, Cn))
+    say("device beta=0 NaN C")
+    bx.device_gemm("N", "N", M, N, K, 1.0, dA.ptr, M, dB.ptr, K, 0.0, dC.ptr, M, np.float64)
+    bx.synchronize()
+    out = dC.download()
+    assert np.all(np.isfinite(out))
+    ref = np.zeros((M, N), order="F")
+    orc.gemm("N", "N", M, N, K, 1.0, A, M, B, K, 0.0, ref, M)
+    _check(orc, out, ref, 1.0, A, B, 0.0, ref * 0, K, np.float64, "beta0")
+
+
+def test_device_fill_matches_host_generator(gpu, orc):
+    bx = gpu
+    for dt in (np.float64, np.float32, np.complex128):
+        say("device fill", np.dtype(dt).name)
+        d = bx.DeviceArray(37, 23, dt)
+        d.fill_uniform(99, 5)
+        bx.synchronize()
+        assert np.array_equal(d.download(), orc.fill_uniform(37, 23, 99, 5, dtype=dt))
+
+
+# ------------------------------------------------------------------ ABI, GPU-sized
+def test_cfg0_dgemm_abi_gpu_path(gpu, orc):
+    """The CPU-path config: M=N=K=1024, NN, alpha 1.5, beta 0.5, uniform
+    seed 0 (ids 0,1,2), ld 1024.  1024 > 1000 routes to the GPU, as in the
+    reference (blas/dgemm.c:62).  Checked against the naive triple loop and
+    against OpenBLAS on the same inputs."""
+    bx = gpu
+    n = 1024
+    A = orc.fill_uniform(n, n, 0, 0)
+    B = orc.fill_uniform(n, n, 0, 1)
+    C0 = orc.fill_uniform(n, n, 0, 2)
+    ref = C0.copy(order="F")
+    orc.gemm("N", "N", n, n, n, 1.5, A, n, B, n, 0.5, ref, n)
+    ob = C0.copy(order="F")
+    orc.openblas_gemm("N", "N", n, n, n, 1.5, A, n, B, n, 0.5, ob, n)
+    out = C0.copy(order="F")
+    bx.reset_stats()
+    say("cfg0 cblas_dgemm 1024^3")
+    bx.cblas_gemm(bx.COL_MAJOR, "N", "N", n, n, n, 1.5, A, n, B, n, 0.5, out, n)
+    st = bx.stats()
+    assert st["calls_gpu"] == 1 and st["calls_cpu"] == 0 and st["cpu_fallbacks"] == 0
+    e1 = _check(orc, out, ref, 1.5, A, B, 0.5, C0, n, np.float64, "vs naive")
+    e2 = _check(orc, out, ob, 1.5, A, B, 0.5, C0, n, np.float64, "vs OpenBLAS")
+    print(f"cfg0: rel err vs naive {e1:.3e}, vs OpenBLAS {e2:.3e}")
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32, np.complex128])
+def test_tile_runtime_ragged_all_transposes(gpu, orc, dt):
+    """Host-resident operands through the tiled runtime with a small tile
+    edge, so there are ragged edge tiles in every dimension."""
+    bx = gpu
+    code = bx.dtype_code(dt)
+    assert bx.lib().blasx_set_tile_size(code, 256) == 0
+    try:
+        cplx = np.dtype(dt).kind == "c"
+        M, N, K = (700, 530, 610) if cplx else (1100, 780, 1030)
+        alpha, beta = (0.5 + 1j, -1 + 0.5j) if cplx else (1.25, -0.75)
+        for ta, tb in (TRANS_CPLX if cplx else TRANS_REAL):
+            say("tile-rt", np.dtype(dt).name, ta, tb, M, N, K)
+            A = _mat(orc, M if ta == "N" else K, K if ta == "N" else M, dt, 4, 0)
+            B = _mat(orc, K if tb == "N" else N, N if tb == "N" else K, dt, 4, 1)
+            C0 = _mat(orc, M, N, dt, 4, 2)
+            ref = C0.copy(order="F")
+            orc.gemm(ta, tb, M, N, K, alpha, A, A.shape[0], B, B.shape[0], beta, ref, M,
+                     acc64=(dt == np.float32))
+            out = C0.copy(order="F")
+            bx.reset_stats()
+            bx.cblas_gemm(bx.COL_MAJOR, ta, tb, M, N, K, alpha, A, A.shape[0], B, B.shape[0],
+                          beta, out, M)
+            st = bx.stats()
+            assert st["calls_gpu"] == 1 and st["kernel_launches"] >= 1
+            _check(orc, out, ref, alpha, A, B, beta, C0, K, dt, f"{ta}{tb}")
+    finally:
+        bx.lib().blasx_set_tile_size(code, 0)
+
+
+def test_tile_runtime_integer_exact_and_ld(gpu, orc):
+    bx = gpu
+    assert bx.lib().blasx_set_tile_size(bx.D, 256) == 0
+    try:
+        rng = np.random.default_rng(13)
+        M, N, K, ld = 1030, 777, 1290, 1111
+        for ta, tb in TRANS_REAL:
+            say("tile-rt int-exact ld", ta, tb)
+            ra, ca = (M, K) if ta == "N" else (K, M)
+            rb, cb = (K, N) if tb == "N" else (N, K)
+            A = np.asfortranarray(rng.integers(-256, 257, (ld + 3, ca)), dtype=np.float64)
+            B = np.asfortranarray(rng.integers(-256, 257, (ld + 5, cb)), dtype=np.float64)
+            C0 = np.asfortranarray(rng.integers(-999, 1000, (ld, N)), dtype=np.float64)
+            ref = C0.copy(order="F")
+            orc.gemm(ta, tb, M, N, K, 2.0, A, ld + 3, B, ld + 5, 3.0, ref, ld)
+            out = C0.copy(order="F")
+            bx.cblas_gemm(bx.COL_MAJOR, ta, tb, M, N, K, 2.0, A, ld + 3, B, ld + 5, 3.0, out, ld)
+            assert np.array_equal(out, ref), (ta, tb)
+    finally:
+        bx.lib().blasx_set_tile_size(bx.D, 0)
+
+
+def test_abi_row_major_and_f77(gpu, orc):
+    bx = gpu
+    M, N, K = 1200, 900, 1001
+    A = orc.fill_uniform(M, K, 5, 0)
+    B = orc.fill_uniform(K, N, 5, 1)
+    C0 = orc.fill_uniform(M, N, 5, 2)
+    ref = 0.75 * (A @ B) + 1.5 * C0
+    # RowMajor: pass the row-major views (C-ordered arrays)
+    Ar, Br, Cr = np.ascontiguousarray(A), np.ascontiguousarray(B), np.ascontiguousarray(C0)
+    say("RowMajor cblas_dgemm")
+    bx.cblas_gemm(bx.ROW_MAJOR, "N", "N", M, N, K, 0.75, Ar, K, Br, N, 1.5, Cr, N)
+    _check(orc, Cr, ref, 0.75, A, B, 1.5, C0, K, np.float64, "rowmajor")
+    say("f77 dgemm_")
+    out = C0.copy(order="F")
+    bx.f77_gemm("N", "N", M, N, K, 0.75, A, M, B, K, 1.5, out, M)
+    _check(orc, out, ref, 0.75, A, B, 1.5, C0, K, np.float64, "dgemm_")
+    Az = orc.fill_uniform(600, 520, 6, 0, dtype=np.complex128)
+    Bz = orc.fill_uniform(600, 510, 6, 1, dtype=np.complex128)
+    Cz = orc.fill_uniform(520, 510, 6, 2, dtype=np.complex128)
+    refz = Cz.copy(order="F")
+    orc.gemm("C", "N", 520, 510, 600, 1 + 1j, Az, 600, Bz, 600, 0.5, refz, 520)
+    outz = Cz.copy(order="F")
+    say("f77 zgemm_ C N")
+    bx.f77_gemm("C", "N", 520, 510, 600, 1 + 1j, Az, 600, Bz, 600, 0.5, outz, 520)
+    _check(orc, outz, refz, 1 + 1j, Az, Bz, 0.5, Cz, 600, np.complex128, "zgemm_")
+
+
+def test_out_of_core_small_budget(gpu, orc):
+    """Working set (A+B tiles) larger than the cache budget: tiles are
+    evicted and re-fetched, results stay exact on integer inputs."""
+    bx = gpu
+    L = bx.lib()
+    old = L.blasx_get_hbm_budget()
+    assert L.blasx_set_tile_size(bx.D, 512) == 0
+    try:
+        L.blasx_set_hbm_budget(48 * (32 << 20))  # 48 slots of 32 MiB
+        rng = np.random.default_rng(14)
+        M = N = K = 4096  # 8x8 A tiles + 8x8 B tiles = 128 > 48 slots
+        A = np.asfortranarray(rng.integers(-64, 65, (M, K)), dtype=np.float64)
+        B = np.asfortranarray(rng.integers(-64, 65, (K, N)), dtype=np.float64)
+        C0 = np.asfortranarray(rng.integers(-64, 65, (M, N)), dtype=np.float64)
+        out = C0.copy(order="F")
+        bx.reset_stats()
+        say("out-of-core 4096^3 budget 48 slots")
+        bx.cblas_gemm(bx.COL_MAJOR, "N", "N", M, N, K, 1.0, A, M, B, K, 1.0, out, M)
+        st = bx.stats()
+        assert st["tile_evictions"] > 0, st
+        assert st["bytes_h2d"] > st["bytes_lower_bound"] // 2
+        # exact check on sampled rows/cols (full naive 4096^3 is too slow for the CPU here)
+        rows = rng.choice(M, 64, replace=False)
+        cols = rng.choice(N, 64, replace=False)
+        exact = A[rows].astype(np.int64) @ B[:, cols].astype(np.int64) + C0[np.ix_(rows, cols)]
+        assert np.array_equal(out[np.ix_(rows, cols)], exact.astype(np.float64))
+    finally:
+        L.blasx_set_hbm_budget(old)
+        L.blasx_set_tile_size(bx.D, 0)
+
+
+def test_resident_operand_reused_across_calls(gpu, orc):
+    bx = gpu
+    L = bx.lib()
+    M = N = K = 2048
+    A = orc.fill_uniform(M, K, 8, 0)
+    B = orc.fill_uniform(K, N, 8, 1)
+    assert L.blasx_resident_register(A.ctypes.data, A.nbytes) == 0
+    try:
+        C1 = np.zeros((M, N), order="F")
+        C2 = np.zeros((M, N), order="F")
+        bx.reset_stats()
+        say("resident call 1")
+        bx.gemm("N", "N", 1.0, A, B, 0.0, C1)
+        s1 = bx.stats()
+        say("resident call 2")
+        bx.gemm("N", "N", 1.0, A, B, 0.0, C2)
+        s2 = bx.stats()
+        assert np.array_equal(C1, C2)
+        h2d_second = s2["bytes_h2d"] - s1["bytes_h2d"]
+        assert h2d_second <= s1["bytes_h2d"] - A.nbytes + 1, (s1, s2)
+        assert s2["tile_hits"] > s1["tile_hits"]
+    finally:
+        assert L.blasx_resident_unregister(A.ctypes.data) == 0
+        L.blasx_cache_flush()
