@@ -110,6 +110,8 @@ typedef int (*bx_keep_fn)(const bx_tile_key_t *key, void *user);
 void bx_cache_invalidate(bx_cache_t *c, bx_keep_fn keep, void *user);
 void bx_cache_invalidate_all(bx_cache_t *c);
 void bx_cache_reset_rounds(bx_cache_t *c);   /* forget round numbers of kept tiles */
+int bx_cache_validate(const bx_cache_t *c);  /* 0 if the structural invariants hold */
+uintptr_t bx_cache_slot_addr_key(const bx_cache_t *c, int slot);
 void bx_cache_free_memory(bx_cache_t *c);   /* release every chunk (cache must be idle) */
 /* statistics of the cache itself */
 void bx_cache_counters(const bx_cache_t *c, uint64_t *hits, uint64_t *misses, uint64_t *evictions);

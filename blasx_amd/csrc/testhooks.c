@@ -76,3 +76,13 @@ BX_EXPORT int bxtest_autotune_tile(int dt, int M, int N, int K)
 {
     return bx_autotune_tile(dt, M, N, K);
 }
+
+BX_EXPORT int bxtest_cache_validate(void *c) { return bx_cache_validate((bx_cache_t *)c); }
+BX_EXPORT int bxtest_cache_set_budget(void *c, size_t bytes)
+{
+    return bx_cache_set_budget((bx_cache_t *)c, bytes);
+}
+BX_EXPORT uintptr_t bxtest_cache_slot_key(void *c, int slot)
+{
+    return bx_cache_slot_addr_key((bx_cache_t *)c, slot);
+}

@@ -31,6 +31,7 @@
 
 typedef struct slot {
     bx_tile_key_t key;
+    void *ptr;        /* device address, fixed when the slot's chunk is allocated */
     int64_t last_round;
     int valid;        /* holds a keyed tile */
     int scratch;      /* lent out by take_scratch */
@@ -44,6 +45,7 @@ struct bx_cache {
     int nslots;       /* slots backed by memory */
     int max_slots;    /* budget / slot_bytes */
     void **chunks;
+    size_t *chunk_len;  /* bytes of each chunk (the last before a budget change may be partial) */
     int nchunks, cap_chunks;
     slot_t *s;
     int s_cap;
@@ -161,6 +163,7 @@ void bx_cache_destroy(bx_cache_t *c)
     if (!c) return;
     bx_cache_free_memory(c);
     free(c->chunks);
+    free(c->chunk_len);
     free(c->s);
     free(c->bucket);
     free(c);
@@ -195,6 +198,9 @@ static int grow(bx_cache_t *c)
         void **p = (void **)realloc(c->chunks, sizeof(void *) * nc);
         if (!p) return BLASX_ENOMEM;
         c->chunks = p;
+        size_t *l = (size_t *)realloc(c->chunk_len, sizeof(size_t) * nc);
+        if (!l) return BLASX_ENOMEM;
+        c->chunk_len = l;
         c->cap_chunks = nc;
     }
     if (c->nslots + add > c->s_cap) {
@@ -211,10 +217,12 @@ static int grow(bx_cache_t *c)
         c->max_slots = c->nslots;
         return BLASX_ENOMEM;
     }
+    c->chunk_len[c->nchunks] = (size_t)add * c->slot_bytes;
     c->chunks[c->nchunks++] = mem;
     for (int k = 0; k < add; ++k) {
         int i = c->nslots + k;
         memset(&c->s[i], 0, sizeof(slot_t));
+        c->s[i].ptr = (char *)mem + (size_t)k * c->slot_bytes;
         c->s[i].prev = -1;
         c->s[i].hnext = -1;
         c->s[i].last_round = -1;
@@ -228,7 +236,9 @@ static int grow(bx_cache_t *c)
 void *bx_cache_slot_ptr(const bx_cache_t *c, int i)
 {
     if (i < 0 || i >= c->nslots) return NULL;
-    return (char *)c->chunks[i / c->slots_per_chunk] + (size_t)(i % c->slots_per_chunk) * c->slot_bytes;
+    /* chunks may be partial (the last one before a budget change), so the
+     * address is stored per slot rather than derived from the index */
+    return c->s[i].ptr;
 }
 
 int bx_cache_lookup(const bx_cache_t *c, const bx_tile_key_t *key)
@@ -343,4 +353,39 @@ void bx_cache_counters(const bx_cache_t *c, uint64_t *hits, uint64_t *misses, ui
     if (hits) *hits = c->hits;
     if (misses) *misses = c->misses;
     if (evictions) *evictions = c->evictions;
+}
+
+/* Structural invariants, for the tests: every slot's memory lies inside one
+ * allocated chunk and no two slots overlap; LRU and hash agree with the
+ * valid flags; the live count matches.  Returns 0 or the first failed check. */
+int bx_cache_validate(const bx_cache_t *c)
+{
+    int live = 0;
+    for (int i = c->lru_head, prev = -1; i >= 0; prev = i, i = c->s[i].next) {
+        if (c->s[i].prev != prev) return 1;
+        if (!c->s[i].valid) return 2;
+        if (bx_cache_lookup(c, &c->s[i].key) != i) return 3;
+        ++live;
+    }
+    if (live != c->live) return 4;
+    for (int i = 0; i < c->nslots; ++i) {
+        const char *p = (const char *)c->s[i].ptr;
+        if (!p) return 5;
+        int inside = 0;
+        for (int k = 0; k < c->nchunks && !inside; ++k) {
+            const char *b = (const char *)c->chunks[k];
+            if (p >= b && p + c->slot_bytes <= b + c->chunk_len[k]) inside = 1;
+        }
+        if (!inside) return 6;
+        for (int j = i + 1; j < c->nslots; ++j) {
+            const char *q = (const char *)c->s[j].ptr;
+            if (p < q + c->slot_bytes && q < p + c->slot_bytes) return 7;
+        }
+    }
+    return 0;
+}
+
+uintptr_t bx_cache_slot_addr_key(const bx_cache_t *c, int i)
+{
+    return (i >= 0 && i < c->nslots && c->s[i].valid) ? c->s[i].key.addr : 0;
 }

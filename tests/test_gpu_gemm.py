@@ -213,7 +213,7 @@ def test_tile_runtime_integer_exact_and_ld(gpu, orc):
     assert bx.lib().blasx_set_tile_size(bx.D, 256) == 0
     try:
         rng = np.random.default_rng(13)
-        M, N, K, ld = 1030, 777, 1290, 1111
+        M, N, K, ld = 1030, 777, 1290, 1311
         for ta, tb in TRANS_REAL:
             say("tile-rt int-exact ld", ta, tb)
             ra, ca = (M, K) if ta == "N" else (K, M)
