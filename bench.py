@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""bench.py -- headline benchmark of the GEMM hot path on one MI355X.
+
+Workload (the in-core tiled GEMM config): DGEMM, M = N = K = 32768, NN,
+alpha = 1, beta = 1, entries uniform[-1,1) from the seeded SplitMix64
+generator (seed 0, matrix ids 0/1/2), column-major.  A "step" is one full
+C = A*B + C through libblasx's FP64-MFMA kernel with the operands already
+resident in HBM.  value = 2*M*N*K*steps / (timed wall time) in TFLOP/s.
+
+Also reported (not `value`):
+  * roofline of the dominant kernel from HIP events on its stream,
+  * the CPU baseline: OpenBLAS cblas_dgemm -- the reference's CPU BLAS path --
+    timed on this host on a bounded sample,
+  * extras: the same DGEMM through the cblas_dgemm ABI with host-resident
+    pinned operands (cold and resident-warm), bytes moved vs the lower bound.
+
+Progress goes to stderr; the single JSON result line goes to stdout.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 matrix, AMD published
+FP32_PEAK_TFLOPS = 157.3
+INT8_PEAK_TOPS = 5000.0
+METRIC = "FP64-accurate GEMM TFLOP/s, plain vs split-MFMA; GEMM-stream req/s & p99 ms"
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def load_traffic(key):
+    """HBM bytes per launch measured offline with rocprofv3 --pmc (FETCH_SIZE,
+    WRITE_SIZE), scaled by a copy-kernel calibration; see profiles/."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        e = t[key]
+        return float(e["fetch_bytes"]) + float(e["write_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def headline(bx, n, steps, warmup):
+    import numpy as np
+
+    A = bx.DeviceArray(n, n, np.float64)
+    B = bx.DeviceArray(n, n, np.float64)
+    Cm = bx.DeviceArray(n, n, np.float64)
+    A.fill_uniform(0, 0)
+    B.fill_uniform(0, 1)
+    Cm.fill_uniform(0, 2)
+    bx.synchronize()
+    log(f"headline: DGEMM {n}^3 NN, {warmup} warmup + {steps} timed steps")
+    for _ in range(warmup):
+        bx.device_gemm("N", "N", n, n, n, 1.0, A.ptr, n, B.ptr, n, 1.0, Cm.ptr, n, np.float64)
+    bx.synchronize()
+    evs = [bx.Event() for _ in range(steps + 1)]
+    bx.synchronize()
+    t0 = time.perf_counter()
+    evs[0].record()
+    for s in range(steps):
+        bx.device_gemm("N", "N", n, n, n, 1.0, A.ptr, n, B.ptr, n, 1.0, Cm.ptr, n, np.float64)
+        evs[s + 1].record()
+    bx.synchronize()
+    wall = time.perf_counter() - t0
+    per_launch = [evs[i].elapsed_ms(evs[i + 1]) for i in range(steps)]
+    flops = 2.0 * n * n * n
+    avg_ms = sum(per_launch) / steps
+    log(f"headline: wall {wall:.3f}s, kernel avg {avg_ms:.2f} ms "
+        f"({flops / avg_ms / 1e9:.2f} TFLOP/s)")
+    # spot check: one element against a dd-free FP64 dot product on the host
+    for X in (A, B, Cm):
+        X.free()
+    return wall, avg_ms, flops
+
+
+def cpu_baseline():
+    """OpenBLAS cblas_dgemm (the reference's CPU path, loaded the way the
+    reference loads it) on a bounded sample of the workload."""
+    import numpy as np
+
+    import oracle
+
+    path = oracle.openblas_load()
+    threads = os.cpu_count() or 1
+    env_thr = os.environ.get("OMP_NUM_THREADS")
+    if env_thr and env_thr.isdigit():
+        threads = min(threads, int(env_thr))
+    oracle.openblas_set_threads(threads)
+    n = 8192
+    A = oracle.fill_uniform(n, n, 0, 0)
+    B = oracle.fill_uniform(n, n, 0, 1)
+    C0 = oracle.fill_uniform(n, n, 0, 2)
+    log(f"cpu baseline: OpenBLAS DGEMM {n}^3 on {threads} threads ({path})")
+    Cw = C0.copy(order="F")
+    oracle.openblas_gemm("N", "N", n, n, n, 1.0, A, n, B, n, 1.0, Cw, n)  # warm-up
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < 3 or (time.perf_counter() - t_start < 10.0 and len(times) < 30):
+        Cw = C0.copy(order="F")
+        t = time.perf_counter()
+        oracle.openblas_gemm("N", "N", n, n, n, 1.0, A, n, B, n, 1.0, Cw, n)
+        times.append(time.perf_counter() - t)
+    med = statistics.median(times)
+    tfl = 2.0 * n ** 3 / med / 1e12
+    info = oracle.openblas_info()
+    log(f"cpu baseline: median {med:.3f}s over {len(times)} calls -> {tfl:.3f} TFLOP/s")
+    return {"value": round(tfl, 4), "unit": "TFLOP/s", "cores": threads, "kind": "reference",
+            "sample": (f"OpenBLAS cblas_dgemm (the reference's CPU BLAS path, dlopen+dlsym as in "
+                       f"blas/blasx_internal.c:26-36) DGEMM M=N=K={n} NN alpha=1 beta=1, same "
+                       f"seeded generator; median of {len(times)} calls after 1 warm-up; "
+                       f"{os.path.basename(info['path'])} {info['config'][:40]}")}
+
+
+def host_resident(bx, n):
+    """The same DGEMM through cblas_dgemm with host-resident pinned operands:
+    cold (empty tile cache) and warm (A and B registered resident)."""
+    import numpy as np
+
+    L = bx.lib()
+    log(f"host-resident: allocating 3 x {n}^2 FP64 pinned host matrices")
+    mats = [bx.PinnedArray(n, n, np.float64) for _ in range(3)]
+    for i, m in enumerate(mats):
+        bx.fill_uniform_host(m.array, 0, i)
+    A, B, Cm = (m.array for m in mats)
+    L.blasx_cache_flush()
+    bx.reset_stats()
+    flops = 2.0 * n ** 3
+    t = time.perf_counter()
+    bx.gemm("N", "N", 1.0, A, B, 1.0, Cm)
+    cold = time.perf_counter() - t
+    st = bx.stats()
+    log(f"host-resident cold: {cold:.3f}s ({flops / cold / 1e12:.2f} TFLOP/s), "
+        f"H2D {st['bytes_h2d'] / 1e9:.1f} GB, D2H {st['bytes_d2h'] / 1e9:.1f} GB")
+    L.blasx_resident_register(A.ctypes.data, A.nbytes)
+    L.blasx_resident_register(B.ctypes.data, B.nbytes)
+    bx.gemm("N", "N", 1.0, A, B, 1.0, Cm)  # loads A and B once more, now kept
+    bx.reset_stats()
+    t = time.perf_counter()
+    bx.gemm("N", "N", 1.0, A, B, 1.0, Cm)
+    warm = time.perf_counter() - t
+    st2 = bx.stats()
+    log(f"host-resident warm: {warm:.3f}s ({flops / warm / 1e12:.2f} TFLOP/s), "
+        f"H2D {st2['bytes_h2d'] / 1e9:.1f} GB")
+    L.blasx_resident_unregister(A.ctypes.data)
+    L.blasx_resident_unregister(B.ctypes.data)
+    L.blasx_cache_flush()
+    for m in mats:
+        m.free()
+    return {
+        "cold_tflops": round(flops / cold / 1e12, 3),
+        "warm_tflops": round(flops / warm / 1e12, 3),
+        "cold_h2d_GBps": round(st["bytes_h2d"] / cold / 1e9, 2),
+        "cold_bytes_over_lower_bound": round((st["bytes_h2d"] + st["bytes_d2h"]) /
+                                             max(st["bytes_lower_bound"], 1), 3),
+        "warm_h2d_bytes": st2["bytes_h2d"],
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=32768)
+    ap.add_argument("--no-extras", action="store_true", help="headline line only")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build targets one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU design is described in DESIGN.md, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import blasx_amd as bx
+
+    if not bx.gpu_available():
+        print("bench.py: libblasx reports no usable GPU", file=sys.stderr)
+        sys.exit(1)
+    n = args.n
+    wall, avg_ms, flops = headline(bx, n, args.steps, args.warmup)
+    value = flops * args.steps / wall / 1e12
+    achieved = flops / (avg_ms * 1e-3) / 1e12
+    traffic = load_traffic(f"dgemm_nn_{n}")
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 uniform[-1,1), generated on the device)",
+        "config": {"workload": f"in-core DGEMM M=N=K={n} NN alpha=1 beta=1, operands resident "
+                               "in HBM (plain FP64 MFMA)",
+                   "M": n, "N": n, "K": n, "transa": "N", "transb": "N",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3),
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
+                     "kernel": "k_gemm_plain<double,MN,K> (FP64 MFMA 16x16x4)",
+                     "launch_ms": round(avg_ms, 3)},
+    }
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline()
+        except Exception as e:  # reported, never silently dropped
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    if not args.no_extras:
+        try:
+            out["host_resident"] = host_resident(bx, n)
+        except Exception as e:
+            out["host_resident"] = {"error": str(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

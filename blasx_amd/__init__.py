@@ -76,6 +76,11 @@ def lib():
     L.blasx_memcpy_d2h.argtypes = [_vp, _vp, C.c_size_t]
     L.blasx_memset_device.argtypes = [_vp, i, C.c_size_t]
     L.blasx_compute_stream.restype = _vp
+    L.blasx_event_create.restype = _vp
+    L.blasx_event_destroy.argtypes = [_vp]
+    L.blasx_event_record.argtypes = [_vp, _vp]
+    L.blasx_event_synchronize.argtypes = [_vp]
+    L.blasx_event_elapsed_ms.argtypes = [C.POINTER(C.c_float), _vp, _vp]
     L.blasx_dgemm_device.argtypes = [i, i, i, i, i, d, _vp, i64, _vp, i64, d, _vp, i64, _vp]
     L.blasx_sgemm_device.argtypes = [i, i, i, i, i, f, _vp, i64, _vp, i64, f, _vp, i64, _vp]
     L.blasx_zgemm_device.argtypes = [i, i, i, i, i, _vp, _vp, i64, _vp, i64, _vp, _vp, i64,
@@ -249,6 +254,36 @@ def device_gemm(ta, tb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, dtype, 
         raise TypeError(dtype)
     if rc:
         raise RuntimeError(f"device gemm failed ({rc})")
+
+
+class Event:
+    """A HIP event recorded on a library stream (NULL = compute stream)."""
+
+    def __init__(self):
+        self.ev = lib().blasx_event_create()
+        if not self.ev:
+            raise RuntimeError("blasx_event_create failed")
+
+    def record(self, stream=None):
+        rc = lib().blasx_event_record(self.ev, stream)
+        if rc:
+            raise RuntimeError(f"blasx_event_record failed ({rc})")
+
+    def synchronize(self):
+        lib().blasx_event_synchronize(self.ev)
+
+    def elapsed_ms(self, later):
+        ms = C.c_float()
+        rc = lib().blasx_event_elapsed_ms(C.byref(ms), self.ev, later.ev)
+        if rc:
+            raise RuntimeError(f"blasx_event_elapsed_ms failed ({rc})")
+        return ms.value
+
+    def __del__(self):
+        try:
+            lib().blasx_event_destroy(self.ev)
+        except Exception:
+            pass
 
 
 def synchronize():

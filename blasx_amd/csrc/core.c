@@ -361,6 +361,31 @@ BX_EXPORT void *blasx_compute_stream(void)
     return g_bx.s_comp;
 }
 
+BX_EXPORT void *blasx_event_create(void)
+{
+    if (bx_ensure_init() != 0) return NULL;
+    void *e = NULL;
+    return bxh_event_create(&e, 1) == 0 ? e : NULL;
+}
+
+BX_EXPORT void blasx_event_destroy(void *ev)
+{
+    if (ev) bxh_event_destroy(ev);
+}
+
+BX_EXPORT int blasx_event_record(void *ev, void *stream)
+{
+    if (bx_ensure_init() != 0) return BLASX_ENOGPU;
+    return bxh_event_record(ev, stream ? stream : g_bx.s_comp);
+}
+
+BX_EXPORT int blasx_event_synchronize(void *ev) { return bxh_event_sync(ev); }
+
+BX_EXPORT int blasx_event_elapsed_ms(float *ms, void *start, void *stop)
+{
+    return bxh_event_elapsed_ms(ms, start, stop);
+}
+
 /* ------------------------------------------------------------ device GEMM */
 static int trans_code(int t)
 {

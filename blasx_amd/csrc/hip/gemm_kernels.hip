@@ -94,6 +94,30 @@ template <typename E, int VEC, int ROWS, int BK, int PAD, int MAJ, bool CONJ> st
                                          int64_t k0, int64_t klim, bool vec_ok)
     {
         const int tid = threadIdx.x;
+        // Interior tile with aligned operands (block-uniform test): straight
+        // vector loads, no per-element bounds checks or branches.
+        if (vec_ok && r0 + ROWS <= rlim && k0 + BK <= klim) {
+            if constexpr (MAJ == MAJ_MN) {
+                constexpr int TPR = ROWS / VEC, RPP = NT / TPR;
+                const E *p0 = base + (r0 + (tid % TPR) * VEC) + (k0 + tid / TPR) * ld;
+                const int64_t step = (int64_t)RPP * ld;
+#pragma unroll
+                for (int p = 0; p < NL; ++p) r[p] = *(const Vec<E, VEC> *)(p0 + p * step);
+            } else {
+                constexpr int TPR = BK / VEC, RPP = NT / TPR;
+                const E *p0 = base + (k0 + (tid % TPR) * VEC) + (int64_t)(r0 + tid / TPR) * ld;
+                const int64_t step = (int64_t)RPP * ld;
+#pragma unroll
+                for (int p = 0; p < NL; ++p) r[p] = *(const Vec<E, VEC> *)(p0 + p * step);
+            }
+            if constexpr (CONJ) {
+#pragma unroll
+                for (int p = 0; p < NL; ++p)
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e) r[p].e[e] = conj_of(r[p].e[e]);
+            }
+            return;
+        }
         if constexpr (MAJ == MAJ_MN) {
             constexpr int TPR = ROWS / VEC, RPP = NT / TPR;
             const int rr = (tid % TPR) * VEC, kq = tid / TPR;

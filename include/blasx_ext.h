@@ -88,6 +88,12 @@ int blasx_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int blasx_memset_device(void *dst, int value, size_t bytes);
 int blasx_device_synchronize(void);
 void *blasx_compute_stream(void);        /* hipStream_t of the library's compute stream */
+/* HIP events, for timing work on a stream (NULL = the compute stream) */
+void *blasx_event_create(void);
+void blasx_event_destroy(void *ev);
+int blasx_event_record(void *ev, void *stream);
+int blasx_event_synchronize(void *ev);
+int blasx_event_elapsed_ms(float *ms, void *start, void *stop);
 
 /* ---- device-resident GEMM (operands already in HBM) ----------------------
  * Same argument meaning as cblas_?gemm with Order = ColMajor; trans values
