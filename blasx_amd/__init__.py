@@ -85,6 +85,8 @@ def lib():
     L.blasx_sgemm_device.argtypes = [i, i, i, i, i, f, _vp, i64, _vp, i64, f, _vp, i64, _vp]
     L.blasx_zgemm_device.argtypes = [i, i, i, i, i, _vp, _vp, i64, _vp, i64, _vp, _vp, i64,
                                      _vp]
+    L.blasx_dgemm_split_device.argtypes = [i, i, i, i, i, d, _vp, i64, _vp, i64, d, _vp, i64,
+                                           i, _vp]
     L.blasx_resident_register.argtypes = [_vp, C.c_size_t]
     L.blasx_resident_unregister.argtypes = [_vp]
     L.blasx_fill_uniform_device.argtypes = [i, _vp, i64, i64, i64, C.c_uint64, C.c_uint32, _vp]
@@ -284,6 +286,15 @@ class Event:
             lib().blasx_event_destroy(self.ev)
         except Exception:
             pass
+
+
+def dgemm_split_device(ta, tb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, slices=0,
+                       stream=None):
+    """Split-precision (Ozaki, INT8 MFMA) DGEMM on HBM pointers."""
+    rc = lib().blasx_dgemm_split_device(TRANS[ta], TRANS[tb], M, N, K, alpha, dA, lda, dB, ldb,
+                                        beta, dC, ldc, slices, stream)
+    if rc:
+        raise RuntimeError(f"split dgemm failed ({rc})")
 
 
 def synchronize():

@@ -109,6 +109,19 @@ int blasx_zgemm_device(int transa, int transb, int M, int N, int K, const double
                        const double *dA, int64_t lda, const double *dB, int64_t ldb,
                        const double *beta, double *dC, int64_t ldc, void *stream);
 
+/* ---- split-precision DGEMM (Ozaki scheme on INT8 MFMA) --------------------
+ * No counterpart in the reference.  C = alpha op(A) op(B) + beta C with
+ * op(A) rows and op(B) columns scaled by powers of two and cut into
+ * `slices` exact 7-bit int8 digits (2..8; 0 = default 7); the products of
+ * digit pairs with p + q <= slices + 1 run on the INT8 matrix cores with
+ * exact int32 accumulation and are summed in FP64.  K <= 133143.
+ * Accuracy: each entry of op(A) keeps 7*slices bits relative to the
+ * largest magnitude in its row (op(B): column), so inputs with a wide
+ * dynamic range within a row need more slices for FP64-level error. */
+int blasx_dgemm_split_device(int transa, int transb, int M, int N, int K, double alpha,
+                             const double *dA, int64_t lda, const double *dB, int64_t ldb,
+                             double beta, double *dC, int64_t ldc, int slices, void *stream);
+
 /* ---- cross-call residency ---------------------------------------------------
  * The reference frees its tile cache after every call (blas/dgemm.c:114).
  * A caller that promises not to modify [ptr, ptr+bytes) until it

@@ -78,6 +78,10 @@ def lib():
         L.orc_dd_two_sum.argtypes = [C.c_double, C.c_double, _dp, _dp]
         L.orc_dd_two_prod.argtypes = [C.c_double, C.c_double, _dp, _dp]
         L.orc_dd_dot.argtypes = [_dp, _dp, C.c_int, _dp, _dp]
+        L.orc_ozaki_digits.argtypes = [C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int8)]
+        L.orc_dgemm_ozaki.argtypes = [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, C.c_double,
+                                      _dp, C.c_int, _dp, C.c_int, C.c_double, _dp, C.c_int,
+                                      C.c_int, C.c_int]
         L.orc_flops_gemm.restype = C.c_double
         L.orc_flops_gemm.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
         L.orc_openblas_load.argtypes = [C.c_char_p]
@@ -199,6 +203,26 @@ def gemm_dd(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C_, ldc):
                         lo.ctypes.data_as(_dp), max(M, 1))
         return hi, lo
     raise TypeError(C_.dtype)
+
+
+def ozaki_digits(x, e, s):
+    d = (C.c_int8 * 8)()
+    lib().orc_ozaki_digits(x, e, s, d)
+    return [int(d[t]) for t in range(s)]
+
+
+def ozaki_segmax(K, block_k=64):
+    """Slice pairs per exact int32 launch: nseg * Kp * 127^2 < 2^31."""
+    kp = (K + block_k - 1) // block_k * block_k
+    return (2**31 - 1) // (kp * 127 * 127)
+
+
+def dgemm_ozaki(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C_, ldc, s, segmax=None):
+    """CPU restatement of the split-precision DGEMM; C_ updated in place."""
+    segmax = ozaki_segmax(K) if segmax is None else segmax
+    lib().orc_dgemm_ozaki(ta.encode(), tb.encode(), M, N, K, alpha, _ptr(A, _dp), lda,
+                          _ptr(B, _dp), ldb, beta, _ptr(C_, _dp), ldc, s, segmax)
+    return C_
 
 
 def two_sum(a, b):

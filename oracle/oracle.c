@@ -431,3 +431,106 @@ double orc_flops_gemm(int complex_, double m, double n, double k)
 {
     return (complex_ ? 8.0 : 2.0) * m * n * k;
 }
+
+/* ------------------------------------------------------------------ */
+/* Split-precision DGEMM (Ozaki scheme, INT8 digits) restated on the   */
+/* CPU.  No reference counterpart; follows the published method and    */
+/* the exact evaluation order of the library:                           */
+/*   e_i = frexp exponent of max_k |op(A)_ik| (0 for a zero row),       */
+/*   digits d_t = trunc(r*128), r <- r*128 - d_t, t = 1..s,             */
+/*   for g = s+1 down to 2, pairs p = 1..s with q = g-p in 1..s, taken  */
+/*   in chunks of `segmax` pairs: W (+)= 2^-7g * (integer chunk sum),   */
+/*   C = alpha * 2^(e_i+f_j) * W  (+ fma(beta, C, .)).                  */
+/* Digits and integer sums are exact, so the result is reproducible   */
+/* bit for bit.                                                         */
+/* ------------------------------------------------------------------ */
+static int orc_exp_of(double amax)
+{
+    if (amax == 0.0 || !isfinite(amax)) return 0;
+    int e;
+    frexp(amax, &e);
+    return e;
+}
+
+void orc_ozaki_digits(double x, int e, int s, int8_t *d)
+{
+    double r = ldexp(x, -e);
+    for (int t = 0; t < s; ++t) {
+        r *= 128.0;
+        double q = trunc(r);
+        d[t] = (int8_t)(int)q;
+        r -= q;
+    }
+}
+
+void orc_dgemm_ozaki(char ta, char tb, int M, int N, int K, double alpha, const double *A,
+                     int lda, const double *B, int ldb, double beta, double *C, int ldc, int s,
+                     int segmax)
+{
+    int *ea = (int *)malloc(sizeof(int) * (M > 0 ? M : 1));
+    int *eb = (int *)malloc(sizeof(int) * (N > 0 ? N : 1));
+    int8_t *DA = (int8_t *)calloc((size_t)s * M * (K > 0 ? K : 1), 1);
+    int8_t *DB = (int8_t *)calloc((size_t)s * N * (K > 0 ? K : 1), 1);
+    double *W = (double *)calloc((size_t)M * N + 1, sizeof(double));
+    for (int i = 0; i < M; ++i) {
+        double m = 0;
+        for (int p = 0; p < K; ++p) m = fmax(m, fabs(A[OPIDX(ta, i, p, lda)]));
+        ea[i] = orc_exp_of(m);
+        for (int p = 0; p < K; ++p) {
+            int8_t d[8];
+            orc_ozaki_digits(A[OPIDX(ta, i, p, lda)], ea[i], s, d);
+            for (int t = 0; t < s; ++t) DA[((size_t)t * M + i) * K + p] = d[t];
+        }
+    }
+    for (int j = 0; j < N; ++j) {
+        double m = 0;
+        for (int p = 0; p < K; ++p) m = fmax(m, fabs(B[OPIDX(tb, p, j, ldb)]));
+        eb[j] = orc_exp_of(m);
+        for (int p = 0; p < K; ++p) {
+            int8_t d[8];
+            orc_ozaki_digits(B[OPIDX(tb, p, j, ldb)], eb[j], s, d);
+            for (int t = 0; t < s; ++t) DB[((size_t)t * N + j) * K + p] = d[t];
+        }
+    }
+    int first = 1;
+    for (int g = s + 1; g >= 2; --g) {
+        int pairs[16][2], np = 0;
+        for (int p = 1; p <= s; ++p) {
+            int q = g - p;
+            if (q >= 1 && q <= s) {
+                pairs[np][0] = p;
+                pairs[np][1] = q;
+                ++np;
+            }
+        }
+        double scale = ldexp(1.0, -7 * g);
+        for (int off = 0; off < np; off += segmax) {
+            int cnt = np - off < segmax ? np - off : segmax;
+#pragma omp parallel for schedule(static)
+            for (int j = 0; j < N; ++j)
+                for (int i = 0; i < M; ++i) {
+                    int64_t acc = 0;
+                    for (int c = 0; c < cnt; ++c) {
+                        const int8_t *da = DA + ((size_t)(pairs[off + c][0] - 1) * M + i) * K;
+                        const int8_t *db = DB + ((size_t)(pairs[off + c][1] - 1) * N + j) * K;
+                        for (int p = 0; p < K; ++p) acc += (int64_t)da[p] * db[p];
+                    }
+                    double add = scale * (double)acc;
+                    double *w = &W[i + (size_t)j * M];
+                    *w = first ? add : *w + add;
+                }
+            first = 0;
+        }
+    }
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < M; ++i) {
+            double v = alpha * ldexp(W[i + (size_t)j * M], ea[i] + eb[j]);
+            double *c = &C[i + (int64_t)j * ldc];
+            *c = (beta == 0.0) ? v : fma(beta, *c, v);
+        }
+    free(ea);
+    free(eb);
+    free(DA);
+    free(DB);
+    free(W);
+}

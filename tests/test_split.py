@@ -1,0 +1,161 @@
+"""Split-precision DGEMM (Ozaki scheme on INT8 MFMA).
+
+No reference counterpart: the algorithm is pinned from its mathematics.
+  * CPU: the oracle's digit split reconstructs its input exactly (rational
+    arithmetic); on integer inputs the whole oracle pipeline equals the exact
+    product; its error against the double-double oracle falls with the
+    number of slices.
+  * GPU: the library's result equals the oracle restatement bit for bit
+    (every split step and every integer product is exact and the FP64 fold
+    order is fixed), and on integer inputs equals the exact product.
+"""
+import sys
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+EPS = np.finfo(np.float64).eps
+
+
+def _op(X, t):
+    return X if t == "N" else X.T
+
+
+def _exact_int(A, B, ta, tb):
+    opA = _op(A, ta).astype(np.int64)
+    opB = _op(B, tb).astype(np.int64)
+    return opA @ opB
+
+
+# ------------------------------------------------------------------ CPU
+def test_digit_split_is_exact(orc):
+    rng = np.random.default_rng(30)
+    for _ in range(3000):
+        x = float(rng.standard_normal() * np.exp(rng.uniform(-40, 40)))
+        e = int(np.frexp(abs(x) * rng.uniform(1.0, 1e6))[1])  # any e with |x| 2^-e < 1
+        for s in (1, 3, 7, 8):
+            d = orc.ozaki_digits(x, e, s)
+            assert all(-127 <= v <= 127 for v in d)
+            assert all(v == 0 or np.sign(v) == np.sign(x) for v in d)
+            rec = sum(Fraction(v) * Fraction(2) ** (-7 * (t + 1)) for t, v in enumerate(d))
+            rem = Fraction(x) * Fraction(2) ** (-e) - rec
+            assert abs(rem) < Fraction(2) ** (-7 * s), (x, e, s)
+            if s == 8 and abs(x) * 2.0 ** -e >= 2.0 ** -3:  # all 53 bits fit in 56
+                assert rem == 0
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+def test_oracle_split_exact_on_14bit_integers(orc, ta, tb):
+    """|a| < 2^14 with row maxima >= 2^13 splits into exactly two digits;
+    with s >= 3 every digit product is kept, so the result is exact."""
+    rng = np.random.default_rng(31)
+    M, N, K = 23, 17, 41
+    A = rng.integers(-(2**14) + 1, 2**14, (M, K) if ta == "N" else (K, M)).astype(np.float64)
+    B = rng.integers(-(2**14) + 1, 2**14, (K, N) if tb == "N" else (N, K)).astype(np.float64)
+    # different scale per row / column exercises the exponents
+    if ta == "N":
+        A *= 2.0 ** rng.integers(-20, 20, (M, 1))
+    else:
+        A *= 2.0 ** rng.integers(-20, 20, (1, M))
+    A, B = np.asfortranarray(A), np.asfortranarray(B)
+    C0 = np.asfortranarray(rng.integers(-50, 50, (M, N)).astype(np.float64))
+    out = C0.copy(order="F")
+    orc.dgemm_ozaki(ta, tb, M, N, K, 1.0, A, A.shape[0], B, B.shape[0], 1.0, out, M, 3)
+    opA = [[Fraction(v) for v in row] for row in _op(A, ta)]
+    opB = [[Fraction(v) for v in row] for row in _op(B, tb)]
+    for i in range(M):
+        for j in range(N):
+            ex = sum(opA[i][p] * opB[p][j] for p in range(K)) + Fraction(C0[i, j])
+            assert out[i, j] == float(ex), (i, j)
+
+
+def test_oracle_split_error_vs_dd_falls_with_slices(orc):
+    M, N, K = 40, 30, 300
+    A = orc.fill_dynrange(M, K, 3, 0, 0.5)
+    B = orc.fill_dynrange(K, N, 3, 1, 0.5)
+    C0 = np.zeros((M, N), order="F")
+    hi, lo = orc.gemm_dd("N", "N", M, N, K, 1.0, A, M, B, K, 0.0, C0, M)
+    scale = np.abs(A) @ np.abs(B)
+    errs = []
+    for s in (3, 4, 5, 6, 7, 8):
+        out = np.zeros((M, N), order="F")
+        orc.dgemm_ozaki("N", "N", M, N, K, 1.0, A, M, B, K, 0.0, out, M, s)
+        errs.append(np.max(np.abs(out - hi) / scale))
+    assert all(errs[i + 1] < errs[i] for i in range(4)), errs
+    # with 8 slices the split product is as accurate as FP64 GEMM allows
+    plain = np.zeros((M, N), order="F")
+    orc.gemm("N", "N", M, N, K, 1.0, A, M, B, K, 0.0, plain, M)
+    assert errs[-1] <= 2 * np.max(np.abs(plain - hi) / scale) + 4 * EPS
+
+
+# ------------------------------------------------------------------ GPU
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("s", [3, 5, 7])
+def test_gpu_split_bit_exact_vs_oracle(gpu, orc, s):
+    bx = gpu
+    M, N, K = 203, 151, 333
+    for ta, tb in [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")]:
+        say("split", s, ta, tb, M, N, K)
+        A = orc.fill_dynrange(M if ta == "N" else K, K if ta == "N" else M, 5, 0, 2.0)
+        B = orc.fill_dynrange(K if tb == "N" else N, N if tb == "N" else K, 5, 1, 2.0)
+        C0 = orc.fill_uniform(M, N, 5, 2)
+        ref = C0.copy(order="F")
+        orc.dgemm_ozaki(ta, tb, M, N, K, 1.25, A, A.shape[0], B, B.shape[0], -0.5, ref, M, s)
+        dA, dB, dC = (bx.DeviceArray.from_host(x) for x in (A, B, C0))
+        bx.dgemm_split_device(ta, tb, M, N, K, 1.25, dA.ptr, A.shape[0], dB.ptr, B.shape[0],
+                              -0.5, dC.ptr, M, s)
+        bx.synchronize()
+        out = dC.download()
+        nbad = int(np.sum(out != ref))
+        assert nbad == 0, f"{nbad} of {M * N} entries differ (max {np.max(np.abs(out - ref))})"
+
+
+@pytest.mark.gpu
+def test_gpu_split_exact_integers_and_beta_zero(gpu, orc):
+    bx = gpu
+    rng = np.random.default_rng(32)
+    M, N, K = 300, 260, 1000
+    A = np.asfortranarray(rng.integers(-(2**14) + 1, 2**14, (M, K)).astype(np.float64))
+    B = np.asfortranarray(rng.integers(-(2**14) + 1, 2**14, (K, N)).astype(np.float64))
+    exact = A.astype(np.int64) @ B.astype(np.int64)
+    dA, dB = bx.DeviceArray.from_host(A), bx.DeviceArray.from_host(B)
+    dC = bx.DeviceArray.from_host(np.full((M, N), np.nan, order="F"))
+    say("split exact integers, beta=0 over NaN")
+    bx.dgemm_split_device("N", "N", M, N, K, 1.0, dA.ptr, M, dB.ptr, K, 0.0, dC.ptr, M, 3)
+    bx.synchronize()
+    assert np.array_equal(dC.download(), exact.astype(np.float64))
+
+
+@pytest.mark.gpu
+def test_gpu_split_large_sampled_blocks(gpu, orc):
+    """2048^3 with a wide dynamic range: sampled 64x64 blocks equal the
+    oracle restatement bit for bit, and the error against the double-double
+    oracle is at the level of FP64 GEMM."""
+    bx = gpu
+    n = 2048
+    A = orc.fill_dynrange(n, n, 7, 0, 2.0)
+    B = orc.fill_dynrange(n, n, 7, 1, 2.0)
+    dA, dB = bx.DeviceArray.from_host(A), bx.DeviceArray.from_host(B)
+    dC = bx.DeviceArray(n, n, np.float64)
+    say("split 2048^3 s=7")
+    bx.dgemm_split_device("N", "N", n, n, n, 1.0, dA.ptr, n, dB.ptr, n, 0.0, dC.ptr, n, 7)
+    bx.synchronize()
+    out = dC.download()
+    rng = np.random.default_rng(33)
+    for _ in range(2):
+        r0, c0 = (int(v) for v in rng.integers(0, n - 64, 2))
+        As = np.asfortranarray(A[r0:r0 + 64])
+        Bs = np.asfortranarray(B[:, c0:c0 + 64])
+        ref = np.zeros((64, 64), order="F")
+        orc.dgemm_ozaki("N", "N", 64, 64, n, 1.0, As, 64, Bs, n, 0.0, ref, 64, 7)
+        blk = out[r0:r0 + 64, c0:c0 + 64]
+        assert np.array_equal(blk, ref)
+        hi, _ = orc.gemm_dd("N", "N", 64, 64, n, 1.0, As, 64, Bs, n, 0.0, ref * 0, 64)
+        scale = np.abs(As) @ np.abs(Bs)
+        assert np.max(np.abs(blk - hi) / scale) <= n * EPS
