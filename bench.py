@@ -12,7 +12,10 @@ Also reported (not `value`):
   * the CPU baseline: OpenBLAS cblas_dgemm -- the reference's CPU BLAS path --
     timed on this host on a bounded sample,
   * extras: the same DGEMM through the cblas_dgemm ABI with host-resident
-    pinned operands (cold and resident-warm), bytes moved vs the lower bound.
+    pinned operands (cold and resident-warm), bytes moved vs the lower bound;
+    split-precision (INT8 Ozaki) vs plain FP64 DGEMM at M=N=K=16384 on inputs
+    (u-0.5)*exp(phi*g), phi in {0.5, 2, 4}, with errors against a
+    double-double reference on sampled 256x256 blocks.
 
 Progress goes to stderr; the single JSON result line goes to stdout.
 """
@@ -166,6 +169,90 @@ def host_resident(bx, n):
     }
 
 
+def _errors(c, ref, scale):
+    d = np.abs(c - ref)
+    nz = np.abs(ref) > 0
+    return {"max_rel": float(np.max(d[nz] / np.abs(ref[nz]))),
+            "max_normwise": float(np.max(d / scale)),
+            "fro_rel": float(np.linalg.norm(d) / np.linalg.norm(ref))}
+
+
+def split_precision(bx, n=16384, phis=(0.5, 2.0, 4.0), slices=(6, 7, 8), blocks=2):
+    """Split-precision vs plain FP64 DGEMM on inputs a = (u-0.5) exp(phi g):
+    effective TFLOP/s (2n^3 / time) and errors against a double-double
+    reference on sampled 256x256 blocks of C."""
+    global np
+    import numpy as np
+
+    dA = bx.DeviceArray(n, n, np.float64)
+    dB = bx.DeviceArray(n, n, np.float64)
+    dC = bx.DeviceArray(n, n, np.float64)
+    flops = 2.0 * n ** 3
+    rng = np.random.default_rng(2024)
+    corners = [tuple(int(v) for v in rng.integers(0, n - 256, 2)) for _ in range(blocks)]
+    ev0, ev1 = bx.Event(), bx.Event()
+
+    def timed(fn):
+        fn()
+        bx.synchronize()
+        ev0.record()
+        fn()
+        ev1.record()
+        ev1.synchronize()
+        return ev0.elapsed_ms(ev1) * 1e-3
+
+    def sample():
+        out = []
+        for (r0, c0) in corners:
+            full = np.empty((256, 256), order="F")
+            # column c of the block lives at dC + (r0 + (c0+c)*n)*8
+            for c in range(256):
+                src = dC.ptr + (r0 + (c0 + c) * n) * 8
+                bx.lib().blasx_memcpy_d2h(full[:, c].ctypes.data, src, 256 * 8)
+            out.append(full)
+        return out
+
+    res = {"n": n, "blocks": f"{blocks} x 256x256, corners {corners}", "per_phi": {}}
+    A = np.zeros((n, n), order="F")
+    B = np.zeros((n, n), order="F")
+    for phi in phis:
+        log(f"split: phi={phi}: generating inputs")
+        bx.fill_dynrange_host(A, 0, 0, phi)
+        bx.fill_dynrange_host(B, 0, 1, phi)
+        dA.upload(A)
+        dB.upload(B)
+        refs, scales = [], []
+        for (r0, c0) in corners:
+            As = np.asfortranarray(A[r0:r0 + 256])
+            Bs = np.asfortranarray(B[:, c0:c0 + 256])
+            refs.append(bx.dd_gemm_block("N", "N", 1.0, As, Bs))
+            scales.append(np.abs(As) @ np.abs(Bs))
+        entry = {}
+        t = timed(lambda: bx.device_gemm("N", "N", n, n, n, 1.0, dA.ptr, n, dB.ptr, n, 0.0,
+                                         dC.ptr, n, np.float64))
+        errs = [_errors(c, r, s) for c, r, s in zip(sample(), refs, scales)]
+        entry["plain"] = {"tflops": round(flops / t / 1e12, 3),
+                          **{k: max(e[k] for e in errs) for k in errs[0]}}
+        log(f"split: phi={phi} plain {entry['plain']}")
+        for s in slices:
+            t = timed(lambda: bx.dgemm_split_device("N", "N", n, n, n, 1.0, dA.ptr, n, dB.ptr,
+                                                    n, 0.0, dC.ptr, n, s))
+            errs = [_errors(c, r, sc) for c, r, sc in zip(sample(), refs, scales)]
+            entry[f"s{s}"] = {"tflops_effective": round(flops / t / 1e12, 3),
+                              "int8_pair_products": s * (s + 1) // 2,
+                              **{k: max(e[k] for e in errs) for k in errs[0]}}
+            log(f"split: phi={phi} s={s} {entry[f's{s}']}")
+        ok = [s for s in slices if entry[f"s{s}"]["max_rel"] <= entry["plain"]["max_rel"]]
+        entry["fewest_slices_at_or_below_plain_max_rel"] = min(ok) if ok else None
+        if ok:
+            best = entry[f"s{min(ok)}"]["tflops_effective"]
+            entry["speedup_at_equal_or_lower_error"] = round(best / entry["plain"]["tflops"], 3)
+        res["per_phi"][str(phi)] = entry
+    for X in (dA, dB, dC):
+        X.free()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,6 +261,7 @@ def main():
     ap.add_argument("--n", type=int, default=32768)
     ap.add_argument("--no-extras", action="store_true", help="headline line only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--split-n", type=int, default=16384, help="size of the split-precision run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build targets one GPU; --gpus {args.gpus} is not supported "
@@ -223,6 +311,10 @@ def main():
             out["host_resident"] = host_resident(bx, n)
         except Exception as e:
             out["host_resident"] = {"error": str(e)}
+        try:
+            out["split_precision"] = split_precision(bx, n=args.split_n)
+        except Exception as e:
+            out["split_precision"] = {"error": str(e)}
     print(json.dumps(out), flush=True)
 
 

@@ -40,6 +40,83 @@ class Stats(C.Structure):
                 ("tiles_generated", C.c_uint64), ("flops", C.c_double)]
 
 
+class GemmDesc(C.Structure):
+    """blasx_gemm_desc_t (include/blasx_ext.h)."""
+    _fields_ = [("dtype", C.c_int), ("transa", C.c_int), ("transb", C.c_int),
+                ("M", C.c_int), ("N", C.c_int), ("K", C.c_int),
+                ("alpha", C.c_double * 2), ("beta", C.c_double * 2),
+                ("A", C.c_void_p), ("lda", C.c_int64), ("B", C.c_void_p), ("ldb", C.c_int64),
+                ("C", C.c_void_p), ("ldc", C.c_int64)]
+
+
+def make_desc(dtype, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C_, ldc):
+    d = GemmDesc()
+    d.dtype = dtype_code(dtype)
+    d.transa, d.transb = TRANS[ta], TRANS[tb]
+    d.M, d.N, d.K = M, N, K
+    d.alpha[0], d.alpha[1] = complex(alpha).real, complex(alpha).imag
+    d.beta[0], d.beta[1] = complex(beta).real, complex(beta).imag
+    d.A, d.lda, d.B, d.ldb, d.C, d.ldc = _addr(A), lda, _addr(B), ldb, _addr(C_), ldc
+    return d
+
+
+def gemm_grouped_device(descs, stream=None):
+    arr = (GemmDesc * len(descs))(*descs)
+    rc = lib().blasx_gemm_grouped_device(arr, len(descs), stream)
+    if rc:
+        raise RuntimeError(f"grouped gemm failed ({rc})")
+
+
+class Server:
+    """GEMM request server (continuous batching); requests carry host arrays."""
+
+    def __init__(self, hbm_bytes, max_batch=256):
+        self.s = lib().blasx_server_create(hbm_bytes, max_batch)
+        if not self.s:
+            raise RuntimeError("blasx_server_create failed")
+        self._keep = {}
+
+    def register_operand(self, a):
+        rc = lib().blasx_server_register_operand(self.s, a.ctypes.data, a.nbytes)
+        if rc:
+            raise RuntimeError(f"register_operand failed ({rc})")
+        self._keep[id(a)] = a
+
+    def submit(self, desc, keep=()):
+        rid = lib().blasx_server_submit(self.s, C.byref(desc))
+        if rid < 0:
+            raise RuntimeError(f"submit failed ({rid})")
+        self._keep[("r", rid)] = keep
+        return rid
+
+    def wait(self, rid):
+        lat = C.c_double()
+        rc = lib().blasx_server_wait(self.s, rid, C.byref(lat))
+        self._keep.pop(("r", rid), None)
+        if rc:
+            raise RuntimeError(f"request {rid} failed ({rc})")
+        return lat.value
+
+    def poll(self, rid):
+        return lib().blasx_server_poll(self.s, rid)
+
+    def counters(self):
+        b, dn, used = C.c_uint64(), C.c_uint64(), C.c_size_t()
+        lib().blasx_server_counters(self.s, C.byref(b), C.byref(dn), C.byref(used))
+        return {"batches": b.value, "done": dn.value, "arena_used": used.value}
+
+    def close(self):
+        if self.s:
+            lib().blasx_server_destroy(self.s)
+            self.s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def lib():
     """Load libblasx.so (raises if it has not been built)."""
     global _lib
@@ -87,11 +164,25 @@ def lib():
                                      _vp]
     L.blasx_dgemm_split_device.argtypes = [i, i, i, i, i, d, _vp, i64, _vp, i64, d, _vp, i64,
                                            i, _vp]
+    L.blasx_gemm_grouped_device.argtypes = [C.POINTER(GemmDesc), i, _vp]
+    L.blasx_server_create.restype = _vp
+    L.blasx_server_create.argtypes = [C.c_size_t, i]
+    L.blasx_server_register_operand.argtypes = [_vp, _vp, C.c_size_t]
+    L.blasx_server_submit.restype = i64
+    L.blasx_server_submit.argtypes = [_vp, C.POINTER(GemmDesc)]
+    L.blasx_server_poll.argtypes = [_vp, i64]
+    L.blasx_server_wait.argtypes = [_vp, i64, C.POINTER(C.c_double)]
+    L.blasx_server_counters.argtypes = [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_size_t)]
+    L.blasx_server_destroy.argtypes = [_vp]
     L.blasx_resident_register.argtypes = [_vp, C.c_size_t]
     L.blasx_resident_unregister.argtypes = [_vp]
     L.blasx_fill_uniform_device.argtypes = [i, _vp, i64, i64, i64, C.c_uint64, C.c_uint32, _vp]
     L.blasx_fill_uniform_host.argtypes = [i, _vp, i64, i64, i64, C.c_uint64, C.c_uint32]
     L.blasx_tune_table_add.argtypes = [i, i, i, i, i]
+    L.blasx_fill_dynrange_host.argtypes = [_vp, i64, i64, i64, C.c_uint64, C.c_uint32, d]
+    L.blasx_dd_gemm_block_host.argtypes = [i, i, i, i, i, d, _vp, i64, _vp, i64, d, _vp, i64,
+                                           _vp, i64]
     # CPU-side test hooks
     L.bxtest_cache_create.restype = _vp
     L.bxtest_cache_create.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t]
@@ -321,6 +412,31 @@ class PinnedArray:
             self.array = None
             lib().blasx_host_free(self.ptr)
             self.ptr = None
+
+
+def fill_dynrange_host(a, seed, mid, phi):
+    """(u - 0.5) * exp(phi * g) inputs of the split-precision workload."""
+    assert a.dtype == np.float64 and a.flags["F_CONTIGUOUS"]
+    rows, cols = a.shape
+    rc = lib().blasx_fill_dynrange_host(a.ctypes.data, rows, cols, rows, seed, mid, phi)
+    if rc:
+        raise RuntimeError(f"blasx_fill_dynrange_host failed ({rc})")
+    return a
+
+
+def dd_gemm_block(ta, tb, alpha, A, B, beta=0.0, C0=None):
+    """Double-double reference of alpha op(A) op(B) + beta C0 (host arrays)."""
+    M = A.shape[0] if ta == "N" else A.shape[1]
+    K = A.shape[1] if ta == "N" else A.shape[0]
+    N = B.shape[1] if tb == "N" else B.shape[0]
+    out = np.zeros((M, N), order="F")
+    c0 = 0 if C0 is None else C0.ctypes.data
+    rc = lib().blasx_dd_gemm_block_host(TRANS[ta], TRANS[tb], M, N, K, alpha, A.ctypes.data,
+                                        A.shape[0], B.ctypes.data, B.shape[0], beta, c0,
+                                        max(M, 1), out.ctypes.data, max(M, 1))
+    if rc:
+        raise RuntimeError(f"blasx_dd_gemm_block_host failed ({rc})")
+    return out
 
 
 def fill_uniform_host(a, seed, mid):

@@ -449,6 +449,11 @@ __device__ __forceinline__ void run_tile(const bxh_problem_t &P, const bxh_seg_t
     gemm_block<E, MAJA, MAJB, CJA, CJB>(P, segs + P.seg_off, unused, tm, tn, smem);
 }
 
+template <typename E>
+__device__ __forceinline__ void grouped_body(const bxh_problem_t *__restrict__ probs, int nprob,
+                                             const bxh_seg_t *__restrict__ segs, int total,
+                                             E *smem);
+
 // Grid-stride over the tiles of many problems (grouped GEMM / tile runtime).
 template <typename E>
 __global__ __launch_bounds__(NT, 2) void k_gemm_grouped(const bxh_problem_t *__restrict__ probs,
@@ -457,6 +462,27 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_grouped(const bxh_problem_t *__r
                                                         int total)
 {
     __shared__ __attribute__((aligned(16))) E smem[Smem<E>::ELEMS];
+    grouped_body<E>(probs, nprob, segs, total, smem);
+}
+
+// Persistent variant for graph replay: the problem count and tile total are
+// read from device memory, so a captured launch is replayed with whatever
+// table the host uploaded before the replay.  counts[0] = nprob,
+// counts[1] = total tiles.
+template <typename E>
+__global__ __launch_bounds__(NT, 2) void k_gemm_grouped_dyn(const bxh_problem_t *__restrict__ probs,
+                                                            const bxh_seg_t *__restrict__ segs,
+                                                            const int *__restrict__ counts)
+{
+    __shared__ __attribute__((aligned(16))) E smem[Smem<E>::ELEMS];
+    grouped_body<E>(probs, counts[0], segs, counts[1], smem);
+}
+
+template <typename E>
+__device__ __forceinline__ void grouped_body(const bxh_problem_t *__restrict__ probs, int nprob,
+                                             const bxh_seg_t *__restrict__ segs, int total,
+                                             E *smem)
+{
     constexpr bool CPLX = sizeof(E) == 16;
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
         int lo = 0, hi = nprob - 1;
@@ -672,6 +698,30 @@ int bxh_gemm_grouped(int dtype, const bxh_problem_t *probs_dev, int nprob,
     case BLASX_Z:
         hipLaunchKernelGGL(k_gemm_grouped<cplx>, dim3(grid), dim3(NT), 0, st, probs_dev, nprob,
                            segs_dev, total_tiles);
+        break;
+    default: return BLASX_EINVAL;
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : fail("grouped gemm launch", e);
+}
+
+int bxh_gemm_grouped_dyn(int dtype, const bxh_problem_t *probs_dev, const bxh_seg_t *segs_dev,
+                         const int *counts_dev, int grid, void *stream)
+{
+    if (grid <= 0) return BLASX_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    switch (dtype) {
+    case BLASX_D:
+        hipLaunchKernelGGL(k_gemm_grouped_dyn<double>, dim3(grid), dim3(NT), 0, st, probs_dev,
+                           segs_dev, counts_dev);
+        break;
+    case BLASX_S:
+        hipLaunchKernelGGL(k_gemm_grouped_dyn<float>, dim3(grid), dim3(NT), 0, st, probs_dev,
+                           segs_dev, counts_dev);
+        break;
+    case BLASX_Z:
+        hipLaunchKernelGGL(k_gemm_grouped_dyn<cplx>, dim3(grid), dim3(NT), 0, st, probs_dev,
+                           segs_dev, counts_dev);
         break;
     default: return BLASX_EINVAL;
     }

@@ -234,6 +234,38 @@ int bxh_memcpy2d_d2h_async(void *dst, size_t dpitch, const void *src, size_t spi
     return 0;
 }
 
+// ---- HIP graphs: capture the work enqueued on a stream, replay it ----
+int bxh_graph_begin(void *s)
+{
+    RT(hipStreamBeginCapture((hipStream_t)s, hipStreamCaptureModeThreadLocal),
+       "hipStreamBeginCapture");
+    return 0;
+}
+
+int bxh_graph_end(void *s, void **exec)
+{
+    hipGraph_t g = nullptr;
+    RT(hipStreamEndCapture((hipStream_t)s, &g), "hipStreamEndCapture");
+    hipGraphExec_t ex = nullptr;
+    hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return rt_fail("hipGraphInstantiate", e);
+    *exec = (void *)ex;
+    return 0;
+}
+
+int bxh_graph_launch(void *exec, void *s)
+{
+    RT(hipGraphLaunch((hipGraphExec_t)exec, (hipStream_t)s), "hipGraphLaunch");
+    return 0;
+}
+
+int bxh_graph_destroy(void *exec)
+{
+    if (exec) RT(hipGraphExecDestroy((hipGraphExec_t)exec), "hipGraphExecDestroy");
+    return 0;
+}
+
 int bxh_memset_async(void *dst, int v, size_t bytes, void *s)
 {
     if (!bytes) return 0;

@@ -79,6 +79,11 @@ int bxh_memcpy2d_h2d_async(void *dst, size_t dpitch, const void *src, size_t spi
 int bxh_memcpy2d_d2h_async(void *dst, size_t dpitch, const void *src, size_t spitch,
                            size_t width, size_t height, void *s);
 int bxh_memset_async(void *dst, int v, size_t bytes, void *s);
+/* HIP graphs: capture what is enqueued on s between begin and end */
+int bxh_graph_begin(void *s);
+int bxh_graph_end(void *s, void **exec);
+int bxh_graph_launch(void *exec, void *s);
+int bxh_graph_destroy(void *exec);
 
 /* ---- kernels ---- */
 /* C = alpha op(A) op(B) + beta C on device pointers; alpha/beta are (re, im) */
@@ -89,6 +94,10 @@ int bxh_gemm(int dtype, int opa, int opb, int m, int n, int k, const double *alp
  * problems share dtype.  grid = 0 picks one block per tile. */
 int bxh_gemm_grouped(int dtype, const bxh_problem_t *probs_dev, int nprob,
                      const bxh_seg_t *segs_dev, int total_tiles, int grid, void *stream);
+/* persistent grouped launch for graph replay: counts_dev[0] = nprob,
+ * counts_dev[1] = total tiles, both read by the kernel */
+int bxh_gemm_grouped_dyn(int dtype, const bxh_problem_t *probs_dev, const bxh_seg_t *segs_dev,
+                         const int *counts_dev, int grid, void *stream);
 /* C = beta * C (beta == 0 writes zeros without reading C) */
 int bxh_scale(int dtype, int m, int n, const double *beta, void *C, int64_t ldc, void *stream);
 int bxh_fill_uniform(int dtype, void *p, int64_t rows, int64_t cols, int64_t ld,

@@ -127,6 +127,11 @@ int bx_tile_gemm(int dt, int opa, int opb, int M, int N, int K, const double *al
 bx_cache_t *bx_global_cache(void);
 int bx_autotune_tile(int dt, int M, int N, int K);
 
+/* grouped GEMM tables (grouped.c) */
+int bx_desc_check(const blasx_gemm_desc_t *d);
+int bx_grouped_fill(const blasx_gemm_desc_t *d, int n, int dtype, bxh_problem_t *probs,
+                    bxh_seg_t *segs, int *nprob);
+
 /* task queue (queue.c): ids task_num-1 .. 0 exactly once across threads */
 typedef struct bx_queue {
     volatile int64_t tail;

@@ -122,6 +122,47 @@ int blasx_dgemm_split_device(int transa, int transb, int M, int N, int K, double
                              const double *dA, int64_t lda, const double *dB, int64_t ldb,
                              double beta, double *dC, int64_t ldc, int slices, void *stream);
 
+/* ---- grouped GEMM -----------------------------------------------------------
+ * No counterpart in the reference (which issues one vendor call per tile
+ * product, blas/blasx_dgemm.c:87-94).  Many independent GEMMs of mixed
+ * shape, transposes and element type (S, D, Z) in one launch per element
+ * type; pointers are HBM addresses.  Asynchronous on `stream`. */
+typedef struct blasx_gemm_desc {
+    int dtype;              /* BLASX_S, BLASX_D or BLASX_Z */
+    int transa, transb;     /* CBLAS enums (111/112/113) */
+    int M, N, K;
+    double alpha[2];        /* (re, im); im ignored for real types */
+    double beta[2];
+    const void *A;
+    int64_t lda;
+    const void *B;
+    int64_t ldb;
+    void *C;
+    int64_t ldc;
+} blasx_gemm_desc_t;
+
+int blasx_gemm_grouped_device(const blasx_gemm_desc_t *descs, int n, void *stream);
+
+/* ---- GEMM request server (continuous batching) ------------------------------
+ * No counterpart in the reference (its ABI is one blocking, non-reentrant
+ * call at a time).  Requests carry HOST pointers (pinned memory gives full
+ * copy overlap); each is admitted when its operands fit in the server's HBM
+ * arena of `hbm_bytes`, joins the next batch, runs in a persistent grouped
+ * launch replayed from a hipGraph, and has C copied back before it is
+ * reported done.  A request that can never fit is rejected (BLASX_ERANGE).
+ * Operands registered with blasx_server_register_operand are uploaded once
+ * and shared by every request whose A or B lies inside them; the caller
+ * must not modify them while the server lives. */
+typedef struct blasx_server blasx_server_t;
+blasx_server_t *blasx_server_create(size_t hbm_bytes, int max_batch);
+int blasx_server_register_operand(blasx_server_t *s, const void *host, size_t bytes);
+int64_t blasx_server_submit(blasx_server_t *s, const blasx_gemm_desc_t *d); /* id or <0 */
+int blasx_server_poll(blasx_server_t *s, int64_t id);       /* 1 done, 0 pending, <0 error */
+int blasx_server_wait(blasx_server_t *s, int64_t id, double *latency_ms);
+int blasx_server_counters(blasx_server_t *s, uint64_t *batches, uint64_t *done,
+                          size_t *arena_used);
+void blasx_server_destroy(blasx_server_t *s);
+
 /* ---- cross-call residency ---------------------------------------------------
  * The reference frees its tile cache after every call (blas/dgemm.c:114).
  * A caller that promises not to modify [ptr, ptr+bytes) until it

@@ -184,3 +184,18 @@ def test_tile_size_heuristic(bx):
     assert L.bxtest_autotune_tile(bx.D, 1024, 1024, 1024) == 512
     assert L.blasx_set_tile_size(bx.D, 4096) != 0   # larger than a cache slot
     assert L.blasx_set_tile_size(bx.D, 300) != 0    # not a supported edge
+
+
+def test_dynrange_host_generator_matches_oracle(bx, orc):
+    a = np.zeros((700, 600), order="F")
+    bx.fill_dynrange_host(a, 9, 4, 2.0)
+    assert np.array_equal(a, orc.fill_dynrange(700, 600, 9, 4, 2.0))
+
+
+def test_dd_block_reference_matches_oracle(bx, orc):
+    A = orc.fill_dynrange(37, 500, 1, 0, 4.0)
+    B = orc.fill_dynrange(29, 500, 1, 1, 4.0)   # used transposed
+    C0 = orc.fill_uniform(37, 29, 1, 2)
+    out = bx.dd_gemm_block("N", "T", 1.5, A, B, -0.25, C0)
+    hi, _ = orc.gemm_dd("N", "T", 37, 29, 500, 1.5, A, 37, B, 29, -0.25, C0, 37)
+    assert np.array_equal(out, hi)
