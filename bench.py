@@ -253,6 +253,121 @@ def split_precision(bx, n=16384, phis=(0.5, 2.0, 4.0), slices=(6, 7, 8), blocks=
     return res
 
 
+def serving(bx, rates=(50, 100, 200, 500, 1000, 2000), nreq=200, budget=200 << 30, seed=7):
+    """Mixed GEMM request stream through the continuous-batching server vs the
+    same trace through the BLAS ABI one call at a time.
+
+    Requests: Poisson arrivals at `rate`/s; M, N, K log-uniform in [64, 8192];
+    60% FP64 / 30% FP32 / 10% complex128; random transposes; 30% use one of 8
+    resident 4096x4096 FP64 B operands (K, N <= 4096, a view into it).
+    Operands are views into pinned host pools (their values do not matter
+    for timing; results are checked by the test-suite), so C outputs of
+    concurrent requests may overlap in host memory."""
+    import numpy as np
+
+    L = bx.lib()
+    L.blasx_finalize()  # release the tile cache of the sections before
+    rng = np.random.default_rng(seed)
+    dts = [np.float64, np.float32, np.complex128]
+    pools = {}
+    for dt in dts:
+        pools[dt] = [bx.PinnedArray(8192, 8192, dt) for _ in range(3)]
+        for i, p in enumerate(pools[dt]):
+            bx.fill_uniform_host(p.array, 11, i)
+    shared = [bx.PinnedArray(4096, 4096, np.float64) for _ in range(8)]
+    for i, p in enumerate(shared):
+        bx.fill_uniform_host(p.array, 12, i)
+
+    def lu():
+        return int(np.exp(rng.uniform(np.log(64), np.log(8192))))
+
+    def make_trace(rate):
+        t, reqs = 0.0, []
+        for _ in range(nreq):
+            t += rng.exponential(1.0 / rate)
+            dt = dts[rng.choice(3, p=[0.6, 0.3, 0.1])]
+            M, N, K = lu(), lu(), lu()
+            cplx = dt == np.complex128
+            ta = str(rng.choice(["N", "T", "C"] if cplx else ["N", "T"]))
+            tb = str(rng.choice(["N", "T", "C"] if cplx else ["N", "T"]))
+            Ap, Bp, Cp = (p.array for p in pools[dt])
+            sh = None
+            if rng.random() < 0.3:
+                dt, Ap, Bp, Cp = np.float64, *(p.array for p in pools[np.float64])
+                N, K, tb = min(N, 4096), min(K, 4096), "N"
+                sh = int(rng.integers(8))
+                Bp = shared[sh].array
+                ta = "N" if ta == "C" else ta
+            reqs.append(dict(t=t, dt=dt, ta=ta, tb=tb, M=M, N=N, K=K, A=Ap, B=Bp, C=Cp,
+                             shared=sh))
+        return reqs
+
+    def desc(r):
+        ld = r["A"].shape[0]
+        return bx.make_desc(r["dt"], r["ta"], r["tb"], r["M"], r["N"], r["K"], 1.0, r["A"], ld,
+                            r["B"], r["B"].shape[0], 1.0, r["C"], r["C"].shape[0])
+
+    def flops(r):
+        return (8.0 if r["dt"] == np.complex128 else 2.0) * r["M"] * r["N"] * r["K"]
+
+    out = {"requests_per_rate": nreq, "budget_bytes": budget, "rates": {}}
+    srv = bx.Server(budget, max_batch=256)
+    try:
+        for p in shared:
+            srv.register_operand(p.array)
+        for rate in rates:
+            trace = make_trace(rate)
+            log(f"serving: rate {rate}/s, {nreq} requests")
+            ids = []
+            t0 = time.perf_counter()
+            for r in trace:
+                while True:
+                    dtw = t0 + r["t"] - time.perf_counter()
+                    if dtw <= 0:
+                        break
+                    time.sleep(min(dtw, 0.001))
+                ids.append(srv.submit(desc(r)))
+            lat = [srv.wait(i) for i in ids]
+            wall = time.perf_counter() - t0
+            fl = sum(flops(r) for r in trace)
+            lat_s = sorted(lat)
+            out["rates"][str(rate)] = {
+                "req_per_s": round(nreq / wall, 2),
+                "tflops": round(fl / wall / 1e12, 3),
+                "p50_ms": round(lat_s[len(lat_s) // 2], 3),
+                "p99_ms": round(lat_s[min(len(lat_s) - 1, int(0.99 * len(lat_s)))], 3),
+            }
+            log(f"serving: rate {rate}: {out['rates'][str(rate)]}")
+        out["server_counters"] = srv.counters()
+    finally:
+        srv.close()
+    # the same request mix, one blocking ABI call at a time (default dispatch)
+    trace = make_trace(1e9)
+    log("serving: serial ABI baseline")
+    t0 = time.perf_counter()
+    lat = []
+    for r in trace:
+        t = time.perf_counter()
+        bx.cblas_gemm(bx.COL_MAJOR, r["ta"], r["tb"], r["M"], r["N"], r["K"], 1.0, r["A"],
+                      r["A"].shape[0], r["B"], r["B"].shape[0], 1.0, r["C"], r["C"].shape[0])
+        lat.append((time.perf_counter() - t) * 1e3)
+    wall = time.perf_counter() - t0
+    lat.sort()
+    out["serial_abi"] = {"req_per_s": round(nreq / wall, 2),
+                         "tflops": round(sum(flops(r) for r in trace) / wall / 1e12, 3),
+                         "p50_ms_service": round(lat[len(lat) // 2], 3),
+                         "p99_ms_service": round(lat[min(len(lat) - 1, int(0.99 * len(lat)))], 3)}
+    log(f"serving: serial ABI {out['serial_abi']}")
+    best = max(v["req_per_s"] for v in out["rates"].values())
+    out["best_req_per_s_vs_serial"] = round(best / out["serial_abi"]["req_per_s"], 3)
+    for dt in dts:
+        for p in pools[dt]:
+            p.free()
+    for p in shared:
+        p.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -262,6 +377,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="headline line only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--split-n", type=int, default=16384, help="size of the split-precision run")
+    ap.add_argument("--serve-requests", type=int, default=200, help="requests per arrival rate")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build targets one GPU; --gpus {args.gpus} is not supported "
@@ -315,6 +431,10 @@ def main():
             out["split_precision"] = split_precision(bx, n=args.split_n)
         except Exception as e:
             out["split_precision"] = {"error": str(e)}
+        try:
+            out["serving"] = serving(bx, nreq=args.serve_requests)
+        except Exception as e:
+            out["serving"] = {"error": str(e)}
     print(json.dumps(out), flush=True)
 
 
