@@ -169,6 +169,50 @@ def host_resident(bx, n):
     }
 
 
+def out_of_core(bx, n=65536, budget_frac=0.75):
+    """DGEMM through cblas_dgemm with pinned host operands whose A+B working
+    set exceeds the tile-cache budget (budget = budget_frac * |A+B|), against
+    the same call with a budget that holds everything.  Reports throughput,
+    the ratio, bytes moved vs the lower bound e(MK + KN + 2MN) and the
+    host-link rates."""
+    import numpy as np
+
+    L = bx.lib()
+    log(f"out-of-core: allocating 3 x {n}^2 FP64 pinned host matrices")
+    mats = [bx.PinnedArray(n, n, np.float64) for _ in range(3)]
+    for i, m in enumerate(mats):
+        bx.fill_uniform_host(m.array, 3, i)
+    A, B, Cm = (m.array for m in mats)
+    flops = 2.0 * n ** 3
+    res = {"n": n}
+    old = L.blasx_get_hbm_budget()
+    try:
+        for name, budget in (("in_core", old), ("out_of_core", int(budget_frac * 2 * A.nbytes))):
+            L.blasx_finalize()
+            L.blasx_set_hbm_budget(budget)
+            bx.reset_stats()
+            t = time.perf_counter()
+            bx.gemm("N", "N", 1.0, A, B, 1.0, Cm)
+            dt = time.perf_counter() - t
+            st = bx.stats()
+            res[name] = {"budget_bytes": budget, "tflops": round(flops / dt / 1e12, 3),
+                         "seconds": round(dt, 3),
+                         "h2d_GBps": round(st["bytes_h2d"] / dt / 1e9, 2),
+                         "d2h_GBps": round(st["bytes_d2h"] / dt / 1e9, 2),
+                         "bytes_over_lower_bound": round((st["bytes_h2d"] + st["bytes_d2h"]) /
+                                                         max(st["bytes_lower_bound"], 1), 3),
+                         "tile_evictions": st["tile_evictions"]}
+            log(f"out-of-core: {name}: {res[name]}")
+    finally:
+        L.blasx_finalize()
+        L.blasx_set_hbm_budget(old)
+        for m in mats:
+            m.free()
+    res["out_of_core_over_in_core"] = round(res["out_of_core"]["tflops"] /
+                                            res["in_core"]["tflops"], 3)
+    return res
+
+
 def _errors(c, ref, scale):
     d = np.abs(c - ref)
     nz = np.abs(ref) > 0
@@ -378,6 +422,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--split-n", type=int, default=16384, help="size of the split-precision run")
     ap.add_argument("--serve-requests", type=int, default=200, help="requests per arrival rate")
+    ap.add_argument("--ooc-n", type=int, default=65536, help="size of the out-of-core run")
+    ap.add_argument("--only", default="", help="comma list of extras to run "
+                    "(host,ooc,split,serve); default all")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build targets one GPU; --gpus {args.gpus} is not supported "
@@ -423,18 +470,18 @@ def main():
         except Exception as e:  # reported, never silently dropped
             out["cpu_baseline"] = {"value": None, "error": str(e)}
     if not args.no_extras:
-        try:
-            out["host_resident"] = host_resident(bx, n)
-        except Exception as e:
-            out["host_resident"] = {"error": str(e)}
-        try:
-            out["split_precision"] = split_precision(bx, n=args.split_n)
-        except Exception as e:
-            out["split_precision"] = {"error": str(e)}
-        try:
-            out["serving"] = serving(bx, nreq=args.serve_requests)
-        except Exception as e:
-            out["serving"] = {"error": str(e)}
+        only = set(x for x in args.only.split(",") if x)
+        sections = [("host", "host_resident", lambda: host_resident(bx, n)),
+                    ("ooc", "out_of_core", lambda: out_of_core(bx, n=args.ooc_n)),
+                    ("split", "split_precision", lambda: split_precision(bx, n=args.split_n)),
+                    ("serve", "serving", lambda: serving(bx, nreq=args.serve_requests))]
+        for key, name, fn in sections:
+            if only and key not in only:
+                continue
+            try:
+                out[name] = fn()
+            except Exception as e:  # reported in the line, never dropped silently
+                out[name] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(out), flush=True)
 
 

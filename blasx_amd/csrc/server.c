@@ -138,6 +138,7 @@ typedef struct sreq {
 typedef struct batch {
     int in_use;
     int64_t *ids;
+    sreq_t *snap;          /* copies of the admitted requests, read while unlocked */
     int n;
     void *ev_h2d, *ev_comp, *ev_done;
     void *desc_host, *desc_dev;
@@ -211,13 +212,15 @@ static int capture_graph(blasx_server_t *s, batch_t *b)
     return rc;
 }
 
-/* copy in, launch and copy out one batch; called with the lock held */
+/* copy in, launch and copy out one batch.  Runs WITHOUT the lock: it reads
+ * only the batch's snapshot of its requests and the batch's own buffers, so
+ * submitters are never blocked behind copies. */
 static int launch_batch(blasx_server_t *s, batch_t *b)
 {
     int rc = 0;
     blasx_gemm_desc_t *dev = (blasx_gemm_desc_t *)malloc(sizeof(blasx_gemm_desc_t) * b->n);
     for (int i = 0; i < b->n && !rc; ++i) {
-        sreq_t *r = &s->req[b->ids[i]];
+        const sreq_t *r = &b->snap[i];
         const blasx_gemm_desc_t *d = &r->d;
         const size_t es = elem(d->dtype);
         int ra, ca, rb, cb;
@@ -264,7 +267,7 @@ static int launch_batch(blasx_server_t *s, batch_t *b)
     if (!rc) rc = bxh_event_record(b->ev_comp, s->s_comp);
     if (!rc) rc = bxh_stream_wait_event(s->s_d2h, b->ev_comp);
     for (int i = 0; i < b->n && !rc; ++i) {
-        sreq_t *r = &s->req[b->ids[i]];
+        const sreq_t *r = &b->snap[i];
         const size_t es = elem(r->d.dtype);
         rc = bxh_memcpy2d_d2h_async(r->d.C, (size_t)r->d.ldc * es, r->dC, (size_t)r->d.M * es,
                                     (size_t)r->d.M * es, r->d.N, s->s_d2h);
@@ -297,11 +300,15 @@ static int admit(blasx_server_t *s, sreq_t *r)
     r->dA = resident_dev(s, d->A, span_bytes(ra, ca, d->lda, es));
     r->dB = resident_dev(s, d->B, span_bytes(rb, cb, d->ldb, es));
     if (!r->dA) {
+        /* owned allocations record a nonzero size (an empty operand still
+         * holds one granule); 0 means "resident, not ours to free" */
         r->bA = (size_t)ra * ca * es;
+        if (r->bA == 0) r->bA = 1;
         if (!(r->dA = arena_alloc(&s->arena, r->bA))) return -1;
     }
     if (!r->dB) {
         r->bB = (size_t)rb * cb * es;
+        if (r->bB == 0) r->bB = 1;
         if (!(r->dB = arena_alloc(&s->arena, r->bB))) {
             arena_free(&s->arena, r->bA ? r->dA : NULL, r->bA);
             return -1;
@@ -350,11 +357,17 @@ static void *scheduler(void *arg)
                 sreq_t *r = &s->req[s->next_admit];
                 if (admit(s, r) != 0) break; /* wait for memory: keep arrival order */
                 r->state = 1;
+                fb->snap[fb->n] = *r;
                 fb->ids[fb->n++] = s->next_admit++;
             }
             if (fb->n > 0) {
                 fb->in_use = 1;
-                if (launch_batch(s, fb) != 0) {
+                /* only this thread launches or retires batches, so the slot
+                 * stays ours while the lock is dropped for the copies */
+                pthread_mutex_unlock(&s->mu);
+                const int lrc = launch_batch(s, fb);
+                pthread_mutex_lock(&s->mu);
+                if (lrc != 0) {
                     bx_log(0, "server: batch launch failed: %s", bxh_last_error());
                     bxh_device_sync();
                     for (int i = 0; i < fb->n; ++i) {
@@ -410,6 +423,7 @@ BX_EXPORT blasx_server_t *blasx_server_create(size_t hbm_bytes, int max_batch)
     for (int k = 0; k < NBATCH && !rc; ++k) {
         batch_t *b = &s->b[k];
         b->ids = (int64_t *)malloc(sizeof(int64_t) * max_batch);
+        b->snap = (sreq_t *)malloc(sizeof(sreq_t) * max_batch);
         rc = bxh_event_create(&b->ev_h2d, 0);
         if (!rc) rc = bxh_event_create(&b->ev_comp, 0);
         if (!rc) rc = bxh_event_create(&b->ev_done, 0);
@@ -529,6 +543,7 @@ BX_EXPORT void blasx_server_destroy(blasx_server_t *s)
         bxh_host_free(b->desc_host);
         bxh_free(b->desc_dev);
         free(b->ids);
+        free(b->snap);
     }
     bxh_stream_destroy(s->s_h2d);
     bxh_stream_destroy(s->s_comp);

@@ -136,9 +136,23 @@ def test_server_alone_equals_in_mix(gpu, orc):
                 q["B"] = np.asfortranarray(p["B"])
                 _check(orc, m, q)
         c = srv.counters()
-        assert c["done"] == 2 * len(probs) and c["batches"] < 2 * len(probs)
+        assert c["done"] == 2 * len(probs)
         assert c["arena_used"] == 600 * 900 * 8 + (-(600 * 900 * 8)) % 256  # only the resident B remains
         assert all(x >= 0 for x in lat)
+        # continuous batching: small requests that arrive while a large one
+        # computes are admitted together into few batches, not one each
+        n = 4096
+        big = dict(dt=np.float64, ta="N", tb="N", M=n, N=n, K=n, alpha=1.0, beta=0.0,
+                   A=orc.fill_uniform(n, n, 3, 0), B=orc.fill_uniform(n, n, 3, 1),
+                   C0=np.zeros((n, n), order="F"))
+        smalls = [_problem(orc, rng, 100 + i) for i in range(24)]
+        before = srv.counters()["batches"]
+        say("server: burst behind a large request")
+        outs, _ = _run_server(bx, srv, [big] + smalls)
+        used = srv.counters()["batches"] - before
+        assert used < len(smalls), used
+        for p, o in zip(smalls, outs[1:]):
+            _check(orc, o, p)
     finally:
         srv.close()
 
