@@ -136,7 +136,11 @@ def test_gpu_split_exact_integers_and_beta_zero(gpu, orc):
 def test_gpu_split_large_sampled_blocks(gpu, orc):
     """2048^3 with a wide dynamic range: sampled 64x64 blocks equal the
     oracle restatement bit for bit, and the error against the double-double
-    oracle is at the level of FP64 GEMM."""
+    oracle stays inside the scheme's bound.  With a = 2^e (sum_p d_p 2^-7p +
+    r), |r| < 2^-7s, and the pairs p+q > s+1 dropped, every product is off by
+    less than 2^(e_i+f_j-7s) (s+2), so
+        |C - C_exact|_ij <= K 2^(e_i+f_j-7s) (s+2) + 16 u (|A||B|)_ij
+    (the last term covers the FP64 folding of the digit groups)."""
     bx = gpu
     n = 2048
     A = orc.fill_dynrange(n, n, 7, 0, 2.0)
@@ -157,5 +161,11 @@ def test_gpu_split_large_sampled_blocks(gpu, orc):
         blk = out[r0:r0 + 64, c0:c0 + 64]
         assert np.array_equal(blk, ref)
         hi, _ = orc.gemm_dd("N", "N", 64, 64, n, 1.0, As, 64, Bs, n, 0.0, ref * 0, 64)
-        scale = np.abs(As) @ np.abs(Bs)
-        assert np.max(np.abs(blk - hi) / scale) <= n * EPS
+        s = 7
+        ea = np.frexp(np.abs(As).max(axis=1))[1].astype(np.float64)  # e_i of each row
+        eb = np.frexp(np.abs(Bs).max(axis=0))[1].astype(np.float64)  # f_j of each column
+        bound = n * (s + 2) * np.exp2(ea[:, None] + eb[None, :] - 7 * s)
+        bound += 16 * EPS * (np.abs(As) @ np.abs(Bs))
+        assert np.all(np.abs(blk - hi) <= bound)
+        # and the bound is not vacuous: it is far below the entries themselves
+        assert np.median(bound / (np.abs(As) @ np.abs(Bs))) < 1e-9
