@@ -239,6 +239,97 @@ __global__ __launch_bounds__(OZ_NT, 2) void k_ozaki_group(const OzSeg *__restric
         }
 }
 
+// Same computation with a 256x256 block tile: 4 waves of 128x128 (8x8
+// MFMA_I32_16X16X64_I8 accumulators, 256 registers, held in AGPRs), one
+// block per CU.  Twice the INT8 operations per byte of L2 traffic of the
+// 128x128 kernel (256 instead of 128 ops/byte) and half the LDS reads per
+// MFMA; used when the padded problem has enough 256x256 tiles to fill the GPU.
+constexpr int OZ_BIG = 256;
+
+__global__ __launch_bounds__(OZ_NT, 1) void k_ozaki_group_big(const OzSeg *__restrict__ segs,
+                                                              int nseg, int kp, int m, int n,
+                                                              double *__restrict__ W, int64_t ldw,
+                                                              double scale, int first)
+{
+    __shared__ __attribute__((aligned(16))) int8_t sA[2][OZ_BIG * OZ_STR];
+    __shared__ __attribute__((aligned(16))) int8_t sB[2][OZ_BIG * OZ_STR];
+    const int tiles_n = (n + OZ_BIG - 1) / OZ_BIG;
+    int b = blockIdx.x;
+    const int nb = gridDim.x;
+    if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3); // contiguous tiles per XCD
+    const int tm = b / tiles_n, tn = b % tiles_n;
+    const int m0 = tm * OZ_BIG, n0 = tn * OZ_BIG;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int l16 = lane & 15, lg = lane >> 4;
+
+    i32x4 acc[8][8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] = i32x4{0, 0, 0, 0};
+
+    // loader: 256 rows x 64 bytes per operand = 1024 x 16 B, four per thread
+    const int lrow = tid >> 2, lk = (tid & 3) * 16;
+    i32x4 ra[4], rb[4];
+    auto gload = [&](int seg, int k0) {
+        const int8_t *pa = segs[seg].A + (int64_t)(m0 + lrow) * kp + k0 + lk;
+        const int8_t *pb = segs[seg].B + (int64_t)(n0 + lrow) * kp + k0 + lk;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            ra[i] = *(const i32x4 *)(pa + (int64_t)(64 * i) * kp);
+            rb[i] = *(const i32x4 *)(pb + (int64_t)(64 * i) * kp);
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            *(i32x4 *)&sA[buf][(lrow + 64 * i) * OZ_STR + lk] = ra[i];
+            *(i32x4 *)&sB[buf][(lrow + 64 * i) * OZ_STR + lk] = rb[i];
+        }
+    };
+
+    const int ktiles = kp / OZ_BK;
+    const int total = nseg * ktiles;
+    gload(0, 0);
+    lstore(0);
+    __syncthreads();
+    for (int it = 0; it < total; ++it) {
+        const int buf = it & 1;
+        const bool more = it + 1 < total;
+        if (more) gload((it + 1) / ktiles, ((it + 1) % ktiles) * OZ_BK);
+        i32x4 fb[8];
+#pragma unroll
+        for (int nb2 = 0; nb2 < 8; ++nb2)
+            fb[nb2] = *(const i32x4 *)&sB[buf][(wn * 128 + nb2 * 16 + l16) * OZ_STR + lg * 16];
+#pragma unroll
+        for (int mb = 0; mb < 8; ++mb) {
+            const i32x4 fa = *(const i32x4 *)&sA[buf][(wm * 128 + mb * 16 + l16) * OZ_STR + lg * 16];
+#pragma unroll
+            for (int nb2 = 0; nb2 < 8; ++nb2)
+                acc[mb][nb2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa, fb[nb2], acc[mb][nb2], 0, 0, 0);
+        }
+        if (more) lstore(buf ^ 1);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int mb = 0; mb < 8; ++mb)
+#pragma unroll
+        for (int nb2 = 0; nb2 < 8; ++nb2) {
+            const int col = n0 + wn * 128 + nb2 * 16 + l16;
+            if (col >= n) continue;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = m0 + wm * 128 + mb * 16 + 4 * lg + v;
+                if (row >= m) continue;
+                double *w = W + row + (int64_t)col * ldw;
+                const double add = scale * (double)acc[mb][nb2][v];
+                *w = first ? add : *w + add;
+            }
+        }
+}
+
 // 4. C = alpha * 2^(e_i + f_j) * W + beta * C
 __global__ void k_ozaki_finish(int m, int n, const double *__restrict__ W, int64_t ldw,
                                const int *__restrict__ ea, const int *__restrict__ eb,
@@ -310,9 +401,16 @@ int bxh_oz_group(const void *segs_dev, int nseg, int kp, int m, int n, int mp, i
 {
     if (nseg <= 0) return 0;
     if (mp % OZ_BM || np % OZ_BN || kp % OZ_BK) return BLASX_EINVAL;
-    const int grid = (mp / OZ_BM) * (np / OZ_BN);
-    hipLaunchKernelGGL(k_ozaki_group, dim3(grid), dim3(OZ_NT), 0, (hipStream_t)stream,
-                       (const OzSeg *)segs_dev, nseg, kp, m, n, W, ldw, scale, first);
+    const int big_tiles = (mp % OZ_BIG == 0 && np % OZ_BIG == 0) ? (mp / OZ_BIG) * (np / OZ_BIG) : 0;
+    if (big_tiles >= 256) { // enough 256x256 tiles to give every CU one
+        hipLaunchKernelGGL(k_ozaki_group_big, dim3(big_tiles), dim3(OZ_NT), 0,
+                           (hipStream_t)stream, (const OzSeg *)segs_dev, nseg, kp, m, n, W, ldw,
+                           scale, first);
+    } else {
+        const int grid = (mp / OZ_BM) * (np / OZ_BN);
+        hipLaunchKernelGGL(k_ozaki_group, dim3(grid), dim3(OZ_NT), 0, (hipStream_t)stream,
+                           (const OzSeg *)segs_dev, nseg, kp, m, n, W, ldw, scale, first);
+    }
     hipError_t r = hipGetLastError();
     return r == hipSuccess ? 0 : oz_fail("ozaki group launch", r);
 }

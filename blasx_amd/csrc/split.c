@@ -62,7 +62,9 @@ int bx_dgemm_split(int opa, int opb, int M, int N, int K, double alpha, const do
         double b2[2] = {beta, 0.0};
         return bxh_scale(BLASX_D, M, N, b2, C, ldc, stream);
     }
-    const int mp = (int)rup(M, bxh_oz_block_m()), np = (int)rup(N, bxh_oz_block_m());
+    /* large problems pad to the 256x256 tile of the big INT8 kernel */
+    const int pad = (M >= 4096 && N >= 4096) ? 2 * bxh_oz_block_m() : bxh_oz_block_m();
+    const int mp = (int)rup(M, pad), np = (int)rup(N, pad);
     const int kp = (int)rup(K, bxh_oz_block_k());
     const int segmax = max_segments(kp);
     if (segmax < 1) return BLASX_ERANGE; /* K above ~133k: int32 products could overflow */

@@ -133,6 +133,33 @@ def test_gpu_split_exact_integers_and_beta_zero(gpu, orc):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "T")])
+def test_gpu_split_big_tile_path_sampled(gpu, orc, ta, tb):
+    """M, N >= 4096 pads to 256 and runs the 256x256 INT8 kernel; ragged M, N
+    and K exercise the padding.  Sampled 48x40 blocks (including the last
+    rows/columns) must equal the oracle restatement bit for bit."""
+    bx = gpu
+    M, N, K = 4100, 4097, 777
+    A = orc.fill_dynrange(M if ta == "N" else K, K if ta == "N" else M, 8, 0, 1.0)
+    B = orc.fill_dynrange(K if tb == "N" else N, N if tb == "N" else K, 8, 1, 1.0)
+    C0 = orc.fill_uniform(M, N, 8, 2)
+    dA, dB, dC = (bx.DeviceArray.from_host(x) for x in (A, B, C0))
+    say("split big tiles", ta, tb, M, N, K)
+    bx.dgemm_split_device(ta, tb, M, N, K, 0.75, dA.ptr, A.shape[0], dB.ptr, B.shape[0], 2.0,
+                          dC.ptr, M, 6)
+    bx.synchronize()
+    out = dC.download()
+    opA = A if ta == "N" else A.T
+    opB = B if tb == "N" else B.T
+    for (r0, c0) in [(0, 0), (M - 48, N - 40), (1999, 3001)]:
+        As = np.asfortranarray(opA[r0:r0 + 48])
+        Bs = np.asfortranarray(opB[:, c0:c0 + 40])
+        ref = np.asfortranarray(C0[r0:r0 + 48, c0:c0 + 40])
+        orc.dgemm_ozaki("N", "N", 48, 40, K, 0.75, As, 48, Bs, K, 2.0, ref, 48, 6)
+        assert np.array_equal(out[r0:r0 + 48, c0:c0 + 40], ref), (r0, c0)
+
+
+@pytest.mark.gpu
 def test_gpu_split_large_sampled_blocks(gpu, orc):
     """2048^3 with a wide dynamic range: sampled 64x64 blocks equal the
     oracle restatement bit for bit, and the error against the double-double
@@ -169,3 +196,20 @@ def test_gpu_split_large_sampled_blocks(gpu, orc):
         assert np.all(np.abs(blk - hi) <= bound)
         # and the bound is not vacuous: it is far below the entries themselves
         assert np.median(bound / (np.abs(As) @ np.abs(Bs))) < 1e-9
+
+
+def test_oracle_split_error_within_bound(orc):
+    """The per-entry bound K 2^(e_i+f_j-7s) (s+2) + 16u(|A||B|)_ij holds for
+    the oracle restatement against the double-double oracle, for every s."""
+    n = 300
+    A = orc.fill_dynrange(40, n, 4, 0, 2.0)
+    B = orc.fill_dynrange(n, 30, 4, 1, 2.0)
+    hi, _ = orc.gemm_dd("N", "N", 40, 30, n, 1.0, A, 40, B, n, 0.0, np.zeros((40, 30), order="F"), 40)
+    ea = np.frexp(np.abs(A).max(axis=1))[1].astype(np.float64)
+    eb = np.frexp(np.abs(B).max(axis=0))[1].astype(np.float64)
+    for s in range(2, 9):
+        out = np.zeros((40, 30), order="F")
+        orc.dgemm_ozaki("N", "N", 40, 30, n, 1.0, A, 40, B, n, 0.0, out, 40, s)
+        bound = n * (s + 2) * np.exp2(ea[:, None] + eb[None, :] - 7 * s)
+        bound += 16 * EPS * (np.abs(A) @ np.abs(B))
+        assert np.all(np.abs(out - hi) <= bound), s
