@@ -412,6 +412,28 @@ def serving(bx, rates=(50, 100, 200, 500, 1000, 2000), nreq=200, budget=200 << 3
     return out
 
 
+def traffic_probe(bx, n):
+    """One DGEMM launch at n^3 and one 1 GiB copy-kernel launch, for the
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes: the copy kernel moves a
+    known 1 GiB each way and calibrates the counters."""
+    import numpy as np
+
+    A = bx.DeviceArray(n, n, np.float64)
+    B = bx.DeviceArray(n, n, np.float64)
+    Cm = bx.DeviceArray(n, n, np.float64)
+    for i, X in enumerate((A, B, Cm)):
+        X.fill_uniform(0, i)
+    bx.synchronize()
+    bx.device_gemm("N", "N", n, n, n, 1.0, A.ptr, n, B.ptr, n, 1.0, Cm.ptr, n, np.float64)
+    bx.synchronize()
+    src = bx.DeviceArray(1 << 17, 1024, np.float64)  # 1 GiB
+    dst = bx.DeviceArray(1 << 17, 1024, np.float64)
+    rc = bx.lib().blasx_device_copy(dst.ptr, src.ptr, 1 << 30, None)
+    bx.synchronize()
+    log(f"traffic probe: one DGEMM {n}^3 and one 1 GiB copy (rc {rc})")
+    print(json.dumps({"traffic_probe": {"n": n, "copy_bytes": 1 << 30}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -423,6 +445,8 @@ def main():
     ap.add_argument("--split-n", type=int, default=16384, help="size of the split-precision run")
     ap.add_argument("--serve-requests", type=int, default=200, help="requests per arrival rate")
     ap.add_argument("--ooc-n", type=int, default=65536, help="size of the out-of-core run")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="one DGEMM + one 1 GiB copy kernel, for rocprofv3 --pmc passes")
     ap.add_argument("--only", default="", help="comma list of extras to run "
                     "(host,ooc,split,serve); default all")
     args = ap.parse_args()
@@ -437,6 +461,9 @@ def main():
         print("bench.py: libblasx reports no usable GPU", file=sys.stderr)
         sys.exit(1)
     n = args.n
+    if args.traffic_probe:
+        traffic_probe(bx, n)
+        return
     wall, avg_ms, flops = headline(bx, n, args.steps, args.warmup)
     value = flops * args.steps / wall / 1e12
     achieved = flops / (avg_ms * 1e-3) / 1e12

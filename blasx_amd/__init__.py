@@ -153,6 +153,7 @@ def lib():
     L.blasx_memcpy_d2h.argtypes = [_vp, _vp, C.c_size_t]
     L.blasx_memset_device.argtypes = [_vp, i, C.c_size_t]
     L.blasx_compute_stream.restype = _vp
+    L.blasx_device_copy.argtypes = [_vp, _vp, C.c_size_t, _vp]
     L.blasx_event_create.restype = _vp
     L.blasx_event_destroy.argtypes = [_vp]
     L.blasx_event_record.argtypes = [_vp, _vp]

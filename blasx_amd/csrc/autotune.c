@@ -7,10 +7,12 @@
  * and otherwise by the heuristic in tile_rt.c.  Table file: one entry per
  * line, "dtype M N K T", nearest (log-size) entry of the same dtype wins.
  */
+#define _GNU_SOURCE
 #include <math.h>
 #include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <time.h>
 
 #include "internal.h"
 
@@ -51,6 +53,70 @@ int bx_tune_lookup(int dt, int M, int N, int K)
     }
     pthread_mutex_unlock(&g_lock);
     return bd < 1.0 ? best : 0; /* only near matches (within ~e in size) */
+}
+
+static double now_s(void)
+{
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return t.tv_sec + t.tv_nsec * 1e-9;
+}
+
+/* Measure the host-operand runtime for each candidate tile edge on
+ * synthetic pinned operands of the given shape (NN, alpha = beta = 1), keep
+ * the fastest in the table (and append it to $BLASX_TUNE_FILE if set).
+ * times_out[i] receives the seconds of candidate 256 << i (0 if skipped). */
+BX_EXPORT int blasx_autotune_tile_size(int dtype, int M, int N, int K, int reps, int *best_T,
+                                       double times_out[4])
+{
+    if (dtype != BLASX_S && dtype != BLASX_D && dtype != BLASX_Z) return BLASX_EINVAL;
+    if (M <= 0 || N <= 0 || K <= 0) return BLASX_EINVAL;
+    if (bx_ensure_init() != 0) return BLASX_ENOGPU;
+    if (reps < 1) reps = 2;
+    const size_t es = bx_esize(dtype);
+    void *A = NULL, *B = NULL, *C = NULL;
+    int rc = bxh_host_malloc(&A, (size_t)M * K * es);
+    if (!rc) rc = bxh_host_malloc(&B, (size_t)K * N * es);
+    if (!rc) rc = bxh_host_malloc(&C, (size_t)M * N * es);
+    if (!rc) rc = blasx_fill_uniform_host(dtype, A, M, K, M, 1, 0);
+    if (!rc) rc = blasx_fill_uniform_host(dtype, B, K, N, K, 1, 1);
+    if (!rc) rc = blasx_fill_uniform_host(dtype, C, M, N, M, 1, 2);
+    const int saved = g_bx.tile_override[dtype];
+    double best = 1e30;
+    int bestT = 0;
+    const double one[2] = {1.0, 0.0};
+    for (int i = 0; i < 4 && !rc; ++i) {
+        const int T = 256 << i;
+        if (times_out) times_out[i] = 0.0;
+        if ((size_t)T * T * es > (size_t)2048 * 2048 * 8) continue; /* larger than a cache slot */
+        g_bx.tile_override[dtype] = T;
+        rc = bx_tile_gemm(dtype, BX_OP_N, BX_OP_N, M, N, K, one, A, M, B, K, one, C, M); /* warm-up */
+        double t0 = now_s();
+        for (int r = 0; r < reps && !rc; ++r)
+            rc = bx_tile_gemm(dtype, BX_OP_N, BX_OP_N, M, N, K, one, A, M, B, K, one, C, M);
+        const double t = (now_s() - t0) / reps;
+        if (times_out) times_out[i] = t;
+        if (!rc && t < best) {
+            best = t;
+            bestT = T;
+        }
+    }
+    g_bx.tile_override[dtype] = saved;
+    bxh_host_free(A);
+    bxh_host_free(B);
+    bxh_host_free(C);
+    if (rc) return rc;
+    if (best_T) *best_T = bestT;
+    blasx_tune_table_add(dtype, M, N, K, bestT);
+    const char *path = getenv("BLASX_TUNE_FILE");
+    if (path && *path) {
+        FILE *f = fopen(path, "a");
+        if (f) {
+            fprintf(f, "%d %d %d %d %d\n", dtype, M, N, K, bestT);
+            fclose(f);
+        }
+    }
+    return 0;
 }
 
 BX_EXPORT int blasx_tune_table_add(int dtype, int M, int N, int K, int T)

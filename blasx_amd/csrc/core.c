@@ -349,6 +349,12 @@ BX_EXPORT int blasx_memset_device(void *dst, int value, size_t bytes)
     return rc ? rc : bxh_stream_sync(g_bx.s_comp);
 }
 
+BX_EXPORT int blasx_device_copy(void *dst, const void *src, size_t bytes, void *stream)
+{
+    if (bx_ensure_init() != 0) return BLASX_ENOGPU;
+    return bxh_copy_kernel(dst, src, bytes, stream ? stream : g_bx.s_comp);
+}
+
 BX_EXPORT int blasx_device_synchronize(void)
 {
     if (bx_ensure_init() != 0) return BLASX_ENOGPU;

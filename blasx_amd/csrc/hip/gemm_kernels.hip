@@ -582,6 +582,15 @@ __global__ void k_fill(int dtype, void *p, int64_t rows, int64_t cols, int64_t l
     }
 }
 
+// plain streaming copy (16-byte vectors); also the reference kernel for
+// calibrating HBM byte counters
+__global__ void k_copy16(const int4 *__restrict__ src, int4 *__restrict__ dst, int64_t n16)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 thread_local char g_err[256];
 
 int fail(const char *what, hipError_t e)
@@ -754,6 +763,19 @@ int bxh_scale(int dtype, int m, int n, const double *beta, void *C, int64_t ldc,
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : fail("scale launch", e);
+}
+
+int bxh_copy_kernel(void *dst, const void *src, size_t bytes, void *stream)
+{
+    if (bytes == 0) return 0;
+    if ((bytes & 15) || (((uintptr_t)dst | (uintptr_t)src) & 15)) return BLASX_EINVAL;
+    const int64_t n16 = (int64_t)(bytes / 16);
+    int64_t grid = (n16 + 255) / 256;
+    if (grid > 65536) grid = 65536;
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream,
+                       (const int4 *)src, (int4 *)dst, n16);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : fail("copy launch", e);
 }
 
 int bxh_fill_uniform(int dtype, void *p, int64_t rows, int64_t cols, int64_t ld, uint64_t seed,

@@ -102,6 +102,8 @@ int bxh_gemm_grouped_dyn(int dtype, const bxh_problem_t *probs_dev, const bxh_se
 int bxh_scale(int dtype, int m, int n, const double *beta, void *C, int64_t ldc, void *stream);
 int bxh_fill_uniform(int dtype, void *p, int64_t rows, int64_t cols, int64_t ld,
                      uint64_t seed, uint32_t id, void *stream);
+/* streaming device copy kernel (16-byte aligned, multiple of 16 bytes) */
+int bxh_copy_kernel(void *dst, const void *src, size_t bytes, void *stream);
 
 #ifdef __cplusplus
 }

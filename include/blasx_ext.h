@@ -77,6 +77,14 @@ int blasx_set_hbm_budget(size_t bytes);
 size_t blasx_get_hbm_budget(void);
 int blasx_set_tile_size(int dtype, int tile);   /* 0 = autotune */
 int blasx_get_tile_size(int dtype, int M, int N, int K);
+/* Autotuner for the tile edge of host-resident GEMMs (the reference fixes it
+ * at compile time, include/blasx.h:5-7): times every candidate edge
+ * (256, 512, 1024, 2048 within a cache slot) on synthetic pinned operands of
+ * the given shape, records the fastest in the table consulted per call (and
+ * appends it to $BLASX_TUNE_FILE).  times_out[i] = seconds for edge 256<<i. */
+int blasx_autotune_tile_size(int dtype, int M, int N, int K, int reps, int *best_T,
+                             double times_out[4]);
+int blasx_tune_table_add(int dtype, int M, int N, int K, int T);
 
 /* ---- memory ---------------------------------------------------------------- */
 void *blasx_host_alloc(size_t bytes);    /* pinned host memory (hipHostMalloc) */
@@ -87,6 +95,8 @@ int blasx_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int blasx_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int blasx_memset_device(void *dst, int value, size_t bytes);
 int blasx_device_synchronize(void);
+/* HBM -> HBM streaming copy by a kernel (16-byte aligned, multiple of 16 bytes) */
+int blasx_device_copy(void *dst, const void *src, size_t bytes, void *stream);
 void *blasx_compute_stream(void);        /* hipStream_t of the library's compute stream */
 /* HIP events, for timing work on a stream (NULL = the compute stream) */
 void *blasx_event_create(void);
