@@ -10,11 +10,11 @@ def test_tune_table_lookup_cpu(bx):
     L = bx.lib()
     L.blasx_tune_table_add.argtypes = [C.c_int] * 5
     # a measured entry overrides the heuristic for nearby shapes only
-    assert L.bxtest_autotune_tile(bx.D, 6000, 6000, 6000) == 2048
-    assert L.blasx_tune_table_add(bx.D, 6000, 6000, 6000, 1024) == 0
-    assert L.bxtest_autotune_tile(bx.D, 6100, 5900, 6000) == 1024
-    assert L.bxtest_autotune_tile(bx.D, 30000, 30000, 30000) == 2048
-    assert L.bxtest_autotune_tile(bx.S, 6000, 6000, 6000) == 2048
+    assert L.bxtest_autotune_tile(bx.D, 10000, 10000, 10000) == 2048  # heuristic
+    assert L.blasx_tune_table_add(bx.D, 10000, 10000, 10000, 512) == 0
+    assert L.bxtest_autotune_tile(bx.D, 10100, 9900, 10000) == 512
+    assert L.bxtest_autotune_tile(bx.D, 40000, 40000, 40000) == 2048
+    assert L.bxtest_autotune_tile(bx.S, 10000, 10000, 10000) == 2048
 
 
 @pytest.mark.gpu
