@@ -159,9 +159,20 @@ def host_resident(bx, n):
     L.blasx_cache_flush()
     for m in mats:
         m.free()
+    # ordinary (pageable) host memory, as most BLAS callers pass it
+    log("host-resident: pageable operands")
+    Ap = bx.fill_uniform_host(np.empty((n, n), order="F"), 0, 0)
+    Bp = bx.fill_uniform_host(np.empty((n, n), order="F"), 0, 1)
+    Cp = bx.fill_uniform_host(np.empty((n, n), order="F"), 0, 2)
+    t = time.perf_counter()
+    bx.gemm("N", "N", 1.0, Ap, Bp, 1.0, Cp)
+    pageable = time.perf_counter() - t
+    log(f"host-resident pageable: {pageable:.3f}s ({flops / pageable / 1e12:.2f} TFLOP/s)")
+    del Ap, Bp, Cp
     return {
         "cold_tflops": round(flops / cold / 1e12, 3),
         "warm_tflops": round(flops / warm / 1e12, 3),
+        "pageable_tflops": round(flops / pageable / 1e12, 3),
         "cold_h2d_GBps": round(st["bytes_h2d"] / cold / 1e9, 2),
         "cold_bytes_over_lower_bound": round((st["bytes_h2d"] + st["bytes_d2h"]) /
                                              max(st["bytes_lower_bound"], 1), 3),
