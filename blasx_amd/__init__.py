@@ -176,6 +176,8 @@ def lib():
     L.blasx_server_counters.argtypes = [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_size_t)]
     L.blasx_server_destroy.argtypes = [_vp]
+    L.blasx_zgemm_split_device.argtypes = [i, i, i, i, i, _vp, _vp, i64, _vp, i64, _vp, _vp,
+                                           i64, i, _vp]
     L.blasx_resident_register.argtypes = [_vp, C.c_size_t]
     L.blasx_resident_unregister.argtypes = [_vp]
     L.blasx_fill_uniform_device.argtypes = [i, _vp, i64, i64, i64, C.c_uint64, C.c_uint32, _vp]
@@ -387,6 +389,16 @@ def dgemm_split_device(ta, tb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, 
                                         beta, dC, ldc, slices, stream)
     if rc:
         raise RuntimeError(f"split dgemm failed ({rc})")
+
+
+def zgemm_split_device(ta, tb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, slices=0,
+                       stream=None):
+    """Split-precision complex128 GEMM on HBM pointers."""
+    al, be = _scalar(Z, alpha), _scalar(Z, beta)
+    rc = lib().blasx_zgemm_split_device(TRANS[ta], TRANS[tb], M, N, K, C.cast(al, _vp), dA, lda,
+                                        dB, ldb, C.cast(be, _vp), dC, ldc, slices, stream)
+    if rc:
+        raise RuntimeError(f"split zgemm failed ({rc})")
 
 
 def synchronize():

@@ -346,6 +346,44 @@ __global__ void k_ozaki_finish(int m, int n, const double *__restrict__ W, int64
     }
 }
 
+// complex support: split a complex matrix (rows x cols, ld) into dense real
+// and imaginary planes (ld = rows); conj negates the imaginary plane
+__global__ void k_deinterleave(const double *__restrict__ Z, int64_t ld, int rows, int cols,
+                               int conj, double *__restrict__ re, double *__restrict__ im)
+{
+    const int64_t total = (int64_t)rows * cols;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % rows), j = (int)(t / rows);
+        const double *z = Z + 2 * (i + (int64_t)j * ld);
+        re[t] = z[0];
+        im[t] = conj ? -z[1] : z[1];
+    }
+}
+
+// C = alpha * (Pr + i Pi) + beta * C with a fixed FMA order
+__global__ void k_zcombine(int m, int n, const double *__restrict__ pr, const double *__restrict__ pi,
+                           double ar, double ai, double br, double bi, int beta_zero,
+                           double *__restrict__ C, int64_t ldc)
+{
+    const int64_t total = (int64_t)m * n;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % m), j = (int)(t / m);
+        double *c = C + 2 * (i + (int64_t)j * ldc);
+        const double tr = fma(ar, pr[t], -(ai * pi[t]));
+        const double ti = fma(ar, pi[t], ai * pr[t]);
+        if (beta_zero) {
+            c[0] = tr;
+            c[1] = ti;
+        } else {
+            const double cr = c[0], cim = c[1];
+            c[0] = fma(br, cr, fma(-bi, cim, tr));
+            c[1] = fma(br, cim, fma(bi, cr, ti));
+        }
+    }
+}
+
 thread_local char g_oz_err[256];
 int oz_fail(const char *what, hipError_t e)
 {
@@ -413,6 +451,33 @@ int bxh_oz_group(const void *segs_dev, int nseg, int kp, int m, int n, int mp, i
     }
     hipError_t r = hipGetLastError();
     return r == hipSuccess ? 0 : oz_fail("ozaki group launch", r);
+}
+
+int bxh_oz_deinterleave(const double *Z, int64_t ld, int rows, int cols, int conj, double *re,
+                        double *im, void *stream)
+{
+    const int64_t total = (int64_t)rows * cols;
+    if (total <= 0) return 0;
+    int grid = (int)((total + 255) / 256);
+    if (grid > 262144) grid = 262144;
+    hipLaunchKernelGGL(k_deinterleave, dim3(grid), dim3(256), 0, (hipStream_t)stream, Z, ld,
+                       rows, cols, conj, re, im);
+    hipError_t r = hipGetLastError();
+    return r == hipSuccess ? 0 : oz_fail("deinterleave launch", r);
+}
+
+int bxh_oz_zcombine(int m, int n, const double *pr, const double *pi, const double *alpha,
+                    const double *beta, double *C, int64_t ldc, void *stream)
+{
+    const int64_t total = (int64_t)m * n;
+    if (total <= 0) return 0;
+    int grid = (int)((total + 255) / 256);
+    if (grid > 262144) grid = 262144;
+    const int bz = beta[0] == 0.0 && beta[1] == 0.0;
+    hipLaunchKernelGGL(k_zcombine, dim3(grid), dim3(256), 0, (hipStream_t)stream, m, n, pr, pi,
+                       alpha[0], alpha[1], beta[0], beta[1], bz, C, ldc);
+    hipError_t r = hipGetLastError();
+    return r == hipSuccess ? 0 : oz_fail("zcombine launch", r);
 }
 
 int bxh_oz_finish(int m, int n, const double *W, int64_t ldw, const int *ea, const int *eb,

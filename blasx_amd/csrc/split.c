@@ -149,6 +149,77 @@ int bx_dgemm_split(int opa, int opb, int M, int N, int K, double alpha, const do
     return rc;
 }
 
+int bxh_oz_deinterleave(const double *Z, int64_t ld, int rows, int cols, int conj, double *re,
+                        double *im, void *stream);
+int bxh_oz_zcombine(int m, int n, const double *pr, const double *pi, const double *alpha,
+                    const double *beta, double *C, int64_t ldc, void *stream);
+
+/* complex split GEMM: four real split products on de-interleaved planes
+ *   Pr = Ar Br - Ai Bi,  Pi = Ar Bi + Ai Br,  C = alpha (Pr + i Pi) + beta C
+ * (a conjugated operand has its imaginary plane negated) */
+static pthread_mutex_t g_zs_lock = PTHREAD_MUTEX_INITIALIZER;
+static struct {
+    void *dev;
+    size_t cap;
+    void *done;
+} g_zs;
+
+int bx_zgemm_split(int opa, int opb, int M, int N, int K, const double *alpha, const double *A,
+                   int64_t lda, const double *B, int64_t ldb, const double *beta, double *C,
+                   int64_t ldc, int s, void *stream)
+{
+    if (M == 0 || N == 0) return 0;
+    if (K == 0 || (alpha[0] == 0.0 && alpha[1] == 0.0))
+        return bxh_scale(BLASX_Z, M, N, beta, C, ldc, stream);
+    const int ra = opa == BX_OP_N ? M : K, ca = opa == BX_OP_N ? K : M;
+    const int rb = opb == BX_OP_N ? K : N, cb = opb == BX_OP_N ? N : K;
+    const size_t na = (size_t)ra * ca, nb = (size_t)rb * cb, nc = (size_t)M * N;
+    const size_t bytes = 8 * (2 * na + 2 * nb + 2 * nc);
+    pthread_mutex_lock(&g_zs_lock);
+    int rc = 0;
+    if (!g_zs.done) rc = bxh_event_create(&g_zs.done, 0);
+    if (!rc && bytes > g_zs.cap) {
+        bxh_event_sync(g_zs.done);
+        bxh_free(g_zs.dev);
+        g_zs.dev = NULL;
+        g_zs.cap = 0;
+        rc = bxh_malloc(&g_zs.dev, bytes);
+        if (!rc) g_zs.cap = bytes;
+    }
+    if (!rc) rc = bxh_event_sync(g_zs.done);
+    double *Ar = (double *)g_zs.dev, *Ai = Ar + na, *Br = Ai + na, *Bi = Br + nb;
+    double *Pr = Bi + nb, *Pi = Pr + nc;
+    const int ropa = opa == BX_OP_N ? BX_OP_N : BX_OP_T, ropb = opb == BX_OP_N ? BX_OP_N : BX_OP_T;
+    if (!rc) rc = bxh_oz_deinterleave(A, lda, ra, ca, opa == BX_OP_C, Ar, Ai, stream);
+    if (!rc) rc = bxh_oz_deinterleave(B, ldb, rb, cb, opb == BX_OP_C, Br, Bi, stream);
+    if (!rc) rc = bx_dgemm_split(ropa, ropb, M, N, K, 1.0, Ar, ra, Br, rb, 0.0, Pr, M, s, stream);
+    if (!rc) rc = bx_dgemm_split(ropa, ropb, M, N, K, -1.0, Ai, ra, Bi, rb, 1.0, Pr, M, s, stream);
+    if (!rc) rc = bx_dgemm_split(ropa, ropb, M, N, K, 1.0, Ar, ra, Bi, rb, 0.0, Pi, M, s, stream);
+    if (!rc) rc = bx_dgemm_split(ropa, ropb, M, N, K, 1.0, Ai, ra, Br, rb, 1.0, Pi, M, s, stream);
+    if (!rc) rc = bxh_oz_zcombine(M, N, Pr, Pi, alpha, beta, C, ldc, stream);
+    if (!rc) rc = bxh_event_record(g_zs.done, stream);
+    pthread_mutex_unlock(&g_zs_lock);
+    return rc;
+}
+
+BX_EXPORT int blasx_zgemm_split_device(int transa, int transb, int M, int N, int K,
+                                       const double *alpha, const double *dA, int64_t lda,
+                                       const double *dB, int64_t ldb, const double *beta,
+                                       double *dC, int64_t ldc, int slices, void *stream)
+{
+    if (bx_ensure_init() != 0) return BLASX_ENOGPU;
+    const int opa = transa == 111 ? BX_OP_N : transa == 112 ? BX_OP_T : transa == 113 ? BX_OP_C : -1;
+    const int opb = transb == 111 ? BX_OP_N : transb == 112 ? BX_OP_T : transb == 113 ? BX_OP_C : -1;
+    if (opa < 0 || opb < 0 || M < 0 || N < 0 || K < 0) return BLASX_EINVAL;
+    int64_t nrowa = opa == BX_OP_N ? M : K, nrowb = opb == BX_OP_N ? K : N;
+    if (lda < (nrowa > 1 ? nrowa : 1) || ldb < (nrowb > 1 ? nrowb : 1) || ldc < (M > 1 ? M : 1))
+        return BLASX_EINVAL;
+    if (slices <= 0) slices = 7;
+    if (slices < 2 || slices > 8) return BLASX_EINVAL;
+    return bx_zgemm_split(opa, opb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, slices,
+                          stream ? stream : g_bx.s_comp);
+}
+
 BX_EXPORT int blasx_dgemm_split_device(int transa, int transb, int M, int N, int K, double alpha,
                                        const double *dA, int64_t lda, const double *dB,
                                        int64_t ldb, double beta, double *dC, int64_t ldc,

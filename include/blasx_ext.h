@@ -131,6 +131,13 @@ int blasx_zgemm_device(int transa, int transb, int M, int N, int K, const double
 int blasx_dgemm_split_device(int transa, int transb, int M, int N, int K, double alpha,
                              const double *dA, int64_t lda, const double *dB, int64_t ldb,
                              double beta, double *dC, int64_t ldc, int slices, void *stream);
+/* complex128 version: four real split products on the real/imaginary
+ * planes (Pr = ArBr - AiBi, Pi = ArBi + AiBr), then C = alpha P + beta C;
+ * alpha/beta are (re, im); trans 113 conjugates */
+int blasx_zgemm_split_device(int transa, int transb, int M, int N, int K, const double *alpha,
+                             const double *dA, int64_t lda, const double *dB, int64_t ldb,
+                             const double *beta, double *dC, int64_t ldc, int slices,
+                             void *stream);
 
 /* ---- grouped GEMM -----------------------------------------------------------
  * No counterpart in the reference (which issues one vendor call per tile

@@ -82,6 +82,9 @@ def lib():
         L.orc_dgemm_ozaki.argtypes = [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, C.c_double,
                                       _dp, C.c_int, _dp, C.c_int, C.c_double, _dp, C.c_int,
                                       C.c_int, C.c_int]
+        L.orc_zgemm_ozaki.argtypes = [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, _dp, _dp,
+                                      C.c_int, _dp, C.c_int, _dp, _dp, C.c_int, C.c_int,
+                                      C.c_int]
         L.orc_flops_gemm.restype = C.c_double
         L.orc_flops_gemm.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
         L.orc_openblas_load.argtypes = [C.c_char_p]
@@ -222,6 +225,17 @@ def dgemm_ozaki(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C_, ldc, s, segmax
     segmax = ozaki_segmax(K) if segmax is None else segmax
     lib().orc_dgemm_ozaki(ta.encode(), tb.encode(), M, N, K, alpha, _ptr(A, _dp), lda,
                           _ptr(B, _dp), ldb, beta, _ptr(C_, _dp), ldc, s, segmax)
+    return C_
+
+
+def zgemm_ozaki(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C_, ldc, s, segmax=None):
+    """CPU restatement of the complex split-precision GEMM; C_ updated in place."""
+    segmax = ozaki_segmax(K) if segmax is None else segmax
+    al = np.array([complex(alpha).real, complex(alpha).imag])
+    be = np.array([complex(beta).real, complex(beta).imag])
+    lib().orc_zgemm_ozaki(ta.encode(), tb.encode(), M, N, K, _ptr(al, _dp),
+                          A.ctypes.data_as(_dp), lda, B.ctypes.data_as(_dp), ldb, _ptr(be, _dp),
+                          C_.ctypes.data_as(_dp), ldc, s, segmax)
     return C_
 
 

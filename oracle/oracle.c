@@ -534,3 +534,59 @@ void orc_dgemm_ozaki(char ta, char tb, int M, int N, int K, double alpha, const 
     free(DB);
     free(W);
 }
+
+/* complex split GEMM restated: planes (conjugate = negated imaginary plane),
+ * Pr = Ar Br - Ai Bi, Pi = Ar Bi + Ai Br by four real split products in
+ * this order, then C = alpha P + beta C with the library's FMA order. */
+void orc_zgemm_ozaki(char ta, char tb, int M, int N, int K, const double *alpha, const double *A,
+                     int lda, const double *B, int ldb, const double *beta, double *C, int ldc,
+                     int s, int segmax)
+{
+    const int ra = ta == 'N' ? M : K, ca = ta == 'N' ? K : M;
+    const int rb = tb == 'N' ? K : N, cb = tb == 'N' ? N : K;
+    double *Ar = (double *)malloc(sizeof(double) * ((size_t)ra * ca + 1));
+    double *Ai = (double *)malloc(sizeof(double) * ((size_t)ra * ca + 1));
+    double *Br = (double *)malloc(sizeof(double) * ((size_t)rb * cb + 1));
+    double *Bi = (double *)malloc(sizeof(double) * ((size_t)rb * cb + 1));
+    double *Pr = (double *)calloc((size_t)M * N + 1, sizeof(double));
+    double *Pi = (double *)calloc((size_t)M * N + 1, sizeof(double));
+    for (int j = 0; j < ca; ++j)
+        for (int i = 0; i < ra; ++i) {
+            const double *z = &A[2 * (i + (int64_t)j * lda)];
+            Ar[i + (size_t)j * ra] = z[0];
+            Ai[i + (size_t)j * ra] = ta == 'C' ? -z[1] : z[1];
+        }
+    for (int j = 0; j < cb; ++j)
+        for (int i = 0; i < rb; ++i) {
+            const double *z = &B[2 * (i + (int64_t)j * ldb)];
+            Br[i + (size_t)j * rb] = z[0];
+            Bi[i + (size_t)j * rb] = tb == 'C' ? -z[1] : z[1];
+        }
+    const char rta = ta == 'N' ? 'N' : 'T', rtb = tb == 'N' ? 'N' : 'T';
+    orc_dgemm_ozaki(rta, rtb, M, N, K, 1.0, Ar, ra, Br, rb, 0.0, Pr, M, s, segmax);
+    orc_dgemm_ozaki(rta, rtb, M, N, K, -1.0, Ai, ra, Bi, rb, 1.0, Pr, M, s, segmax);
+    orc_dgemm_ozaki(rta, rtb, M, N, K, 1.0, Ar, ra, Bi, rb, 0.0, Pi, M, s, segmax);
+    orc_dgemm_ozaki(rta, rtb, M, N, K, 1.0, Ai, ra, Br, rb, 1.0, Pi, M, s, segmax);
+    const int bz = beta[0] == 0.0 && beta[1] == 0.0;
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < M; ++i) {
+            const size_t t = i + (size_t)j * M;
+            double *c = &C[2 * (i + (int64_t)j * ldc)];
+            const double tr = fma(alpha[0], Pr[t], -(alpha[1] * Pi[t]));
+            const double ti = fma(alpha[0], Pi[t], alpha[1] * Pr[t]);
+            if (bz) {
+                c[0] = tr;
+                c[1] = ti;
+            } else {
+                const double cr = c[0], ci = c[1];
+                c[0] = fma(beta[0], cr, fma(-beta[1], ci, tr));
+                c[1] = fma(beta[0], ci, fma(beta[1], cr, ti));
+            }
+        }
+    free(Ar);
+    free(Ai);
+    free(Br);
+    free(Bi);
+    free(Pr);
+    free(Pi);
+}

@@ -213,3 +213,54 @@ def test_oracle_split_error_within_bound(orc):
         bound = n * (s + 2) * np.exp2(ea[:, None] + eb[None, :] - 7 * s)
         bound += 16 * EPS * (np.abs(A) @ np.abs(B))
         assert np.all(np.abs(out - hi) <= bound), s
+
+
+def _cplx_int(rng, shape):
+    return np.asfortranarray(rng.integers(-(2**14) + 1, 2**14, shape)
+                             + 1j * rng.integers(-(2**14) + 1, 2**14, shape))
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("C", "N"), ("T", "C")])
+def test_oracle_zsplit_exact_on_integers(orc, ta, tb):
+    rng = np.random.default_rng(34)
+    M, N, K = 9, 7, 13
+    A = _cplx_int(rng, (M, K) if ta == "N" else (K, M))
+    B = _cplx_int(rng, (K, N) if tb == "N" else (N, K))
+    C0 = _cplx_int(rng, (M, N))
+
+    def op(X, t):
+        return X if t == "N" else (X.T if t == "T" else X.conj().T)
+    exact = (2 - 1j) * (op(A, ta) @ op(B, tb)) + (1 + 1j) * C0
+    out = C0.copy(order="F")
+    orc.zgemm_ozaki(ta, tb, M, N, K, 2 - 1j, A, A.shape[0], B, B.shape[0], 1 + 1j, out, M, 3)
+    assert np.array_equal(out, exact)
+
+
+def test_oracle_zsplit_close_to_dd(orc):
+    M, N, K = 20, 16, 200
+    A = orc.fill_uniform(M, K, 6, 0, dtype=np.complex128)
+    B = orc.fill_uniform(K, N, 6, 1, dtype=np.complex128)
+    C0 = np.zeros((M, N), dtype=np.complex128, order="F")
+    hi, _ = orc.gemm_dd("N", "N", M, N, K, 1.0, A, M, B, K, 0.0, C0, M)
+    out = C0.copy(order="F")
+    orc.zgemm_ozaki("N", "N", M, N, K, 1.0, A, M, B, K, 0.0, out, M, 8)
+    scale = np.abs(A) @ np.abs(B)
+    assert np.max(np.abs(out - hi) / scale) <= 8 * EPS
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("C", "T"), ("T", "C")])
+def test_gpu_zsplit_bit_exact_vs_oracle(gpu, orc, ta, tb):
+    bx = gpu
+    M, N, K = 150, 131, 270
+    A = orc.fill_uniform(M if ta == "N" else K, K if ta == "N" else M, 9, 0, dtype=np.complex128)
+    B = orc.fill_uniform(K if tb == "N" else N, N if tb == "N" else K, 9, 1, dtype=np.complex128)
+    C0 = orc.fill_uniform(M, N, 9, 2, dtype=np.complex128)
+    ref = C0.copy(order="F")
+    orc.zgemm_ozaki(ta, tb, M, N, K, 0.5 + 1j, A, A.shape[0], B, B.shape[0], -1 + 0.25j, ref, M, 6)
+    dA, dB, dC = (bx.DeviceArray.from_host(x) for x in (A, B, C0))
+    say("zsplit", ta, tb, M, N, K)
+    bx.zgemm_split_device(ta, tb, M, N, K, 0.5 + 1j, dA.ptr, A.shape[0], dB.ptr, B.shape[0],
+                          -1 + 0.25j, dC.ptr, M, 6)
+    bx.synchronize()
+    assert np.array_equal(dC.download(), ref)
