@@ -19,6 +19,7 @@ thread_local char g_rt_err[256];
 int rt_fail(const char *what, hipError_t e)
 {
     snprintf(g_rt_err, sizeof g_rt_err, "%s: %s", what, hipGetErrorString(e));
+    (void)hipGetLastError(); /* the failure is reported here; do not let it resurface later */
     return e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation ? BLASX_ENOMEM : BLASX_EHIP;
 }
 

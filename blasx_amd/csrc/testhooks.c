@@ -86,3 +86,11 @@ BX_EXPORT uintptr_t bxtest_cache_slot_key(void *c, int slot)
 {
     return bx_cache_slot_addr_key((bx_cache_t *)c, slot);
 }
+
+void bx_host_copy2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width,
+                    size_t height);
+BX_EXPORT void bxtest_host_copy2d(void *dst, size_t dpitch, const void *src, size_t spitch,
+                                  size_t width, size_t height)
+{
+    bx_host_copy2d(dst, dpitch, src, spitch, width, height);
+}

@@ -125,8 +125,104 @@ static int keep_resident(const bx_tile_key_t *k, void *u)
     return bx_resident_contains(k->addr, end);
 }
 
+/* ---- pageable host memory: pinned staging rings ----
+ * copy-in: the caller's tile is packed into a pinned buffer by the host copy
+ * pool, then DMA'd; a buffer is reused once its previous DMA has finished.
+ * copy-out: DMA into a pinned buffer; the host unpacks it into the caller's
+ * C when the ring wraps around to it, or when the call drains. */
+#define NS_IN 8
+#define NS_OUT 24
+void bx_host_copy2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width,
+                    size_t height);
+
+static struct {
+    int ready;
+    void *in[NS_IN], *ev_in[NS_IN];
+    int in_next;
+    void *out[NS_OUT], *ev_out[NS_OUT];
+    int out_next;
+    struct {
+        char *dst;
+        size_t dpitch, width, cols;
+        int busy;
+    } pend[NS_OUT];
+} g_st;
+
+static int ensure_staging(void)
+{
+    if (g_st.ready) return 0;
+    int rc = 0;
+    for (int i = 0; i < NS_IN && !rc; ++i) {
+        rc = bxh_host_malloc(&g_st.in[i], BX_SLOT_BYTES);
+        if (!rc) rc = bxh_event_create(&g_st.ev_in[i], 0);
+    }
+    for (int i = 0; i < NS_OUT && !rc; ++i) {
+        rc = bxh_host_malloc(&g_st.out[i], BX_SLOT_BYTES);
+        if (!rc) rc = bxh_event_create(&g_st.ev_out[i], 0);
+    }
+    if (!rc) g_st.ready = 1;
+    return rc;
+}
+
+static int copy_in(int pageable, void *dst, size_t dpitch, const void *src, size_t spitch,
+                   size_t width, size_t cols)
+{
+    if (!pageable) return bxh_memcpy2d_h2d_async(dst, dpitch, src, spitch, width, cols, g_bx.s_h2d);
+    const int i = g_st.in_next;
+    g_st.in_next = (i + 1) % NS_IN;
+    int rc = bxh_event_sync(g_st.ev_in[i]); /* its previous DMA has read it */
+    if (rc) return rc;
+    bx_host_copy2d(g_st.in[i], width, src, spitch, width, cols);
+    rc = bxh_memcpy2d_h2d_async(dst, dpitch, g_st.in[i], width, width, cols, g_bx.s_h2d);
+    if (!rc) rc = bxh_event_record(g_st.ev_in[i], g_bx.s_h2d);
+    return rc;
+}
+
+static int finish_out(int i, int discard)
+{
+    if (!g_st.pend[i].busy) return 0;
+    int rc = bxh_event_sync(g_st.ev_out[i]);
+    if (!rc && !discard)
+        bx_host_copy2d(g_st.pend[i].dst, g_st.pend[i].dpitch, g_st.out[i], g_st.pend[i].width,
+                       g_st.pend[i].width, g_st.pend[i].cols);
+    g_st.pend[i].busy = 0;
+    return rc;
+}
+
+static int copy_out(int pageable, void *dst, size_t dpitch, const void *src, size_t spitch,
+                    size_t width, size_t cols)
+{
+    if (!pageable) return bxh_memcpy2d_d2h_async(dst, dpitch, src, spitch, width, cols, g_bx.s_d2h);
+    const int i = g_st.out_next;
+    g_st.out_next = (i + 1) % NS_OUT;
+    int rc = finish_out(i, 0);
+    if (!rc) rc = bxh_memcpy2d_d2h_async(g_st.out[i], width, src, spitch, width, cols, g_bx.s_d2h);
+    if (!rc) rc = bxh_event_record(g_st.ev_out[i], g_bx.s_d2h);
+    if (!rc) {
+        g_st.pend[i].dst = (char *)dst;
+        g_st.pend[i].dpitch = dpitch;
+        g_st.pend[i].width = width;
+        g_st.pend[i].cols = cols;
+        g_st.pend[i].busy = 1;
+    }
+    return rc;
+}
+
+/* unpack every staged C tile, oldest first */
+static int drain_out(int discard)
+{
+    if (!g_st.ready) return 0;
+    int rc = 0;
+    for (int k = 0; k < NS_OUT; ++k) {
+        const int r = finish_out((g_st.out_next + k) % NS_OUT, discard);
+        if (!rc) rc = r;
+    }
+    return rc;
+}
+
 typedef struct plan {
     int dt, es, opa, opb, M, N, K, T;
+    int pg_a, pg_b, pg_c; /* operand in pageable host memory: staged copies */
     int mt, nt, kt, RM, RN;
     const char *A, *B;
     char *C;
@@ -228,8 +324,8 @@ static int run_plan(plan_t *p, bx_cache_t *c)
                     void *cs = bx_cache_slot_ptr(c, cst[d * R + t]);
                     if (!p->beta_zero) {
                         const char *hc = p->C + ((int64_t)i * T + (int64_t)k * T * p->ldc) * es;
-                        rc = bxh_memcpy2d_h2d_async(cs, (size_t)T * es, hc, (size_t)p->ldc * es,
-                                                    (size_t)mc * es, (size_t)nc, g_bx.s_h2d);
+                        rc = copy_in(p->pg_c, cs, (size_t)T * es, hc, (size_t)p->ldc * es,
+                                     (size_t)mc * es, (size_t)nc);
                         if (rc) break;
                         h2d += (uint64_t)mc * nc * es;
                     }
@@ -288,9 +384,9 @@ static int run_plan(plan_t *p, bx_cache_t *c)
                             rc = bxh_stream_wait_event(g_bx.s_h2d, g_rt.ev_comp[ev % DEPTH]);
                     }
                     if (!rc)
-                        rc = bxh_memcpy2d_h2d_async(bx_cache_slot_ptr(c, sa[ii]), (size_t)T * es,
-                                                    (const void *)key.addr, (size_t)p->lda * es,
-                                                    (size_t)rows * es, (size_t)cols, g_bx.s_h2d);
+                        rc = copy_in(p->pg_a, bx_cache_slot_ptr(c, sa[ii]), (size_t)T * es,
+                                     (const void *)key.addr, (size_t)p->lda * es,
+                                     (size_t)rows * es, (size_t)cols);
                     h2d += (uint64_t)rows * cols * es;
                 }
                 for (int kk = 0; kk < nk && !rc; ++kk) {
@@ -323,9 +419,9 @@ static int run_plan(plan_t *p, bx_cache_t *c)
                             rc = bxh_stream_wait_event(g_bx.s_h2d, g_rt.ev_comp[ev % DEPTH]);
                     }
                     if (!rc)
-                        rc = bxh_memcpy2d_h2d_async(bx_cache_slot_ptr(c, sb[kk]), (size_t)T * es,
-                                                    (const void *)key.addr, (size_t)p->ldb * es,
-                                                    (size_t)rows * es, (size_t)cols, g_bx.s_h2d);
+                        rc = copy_in(p->pg_b, bx_cache_slot_ptr(c, sb[kk]), (size_t)T * es,
+                                     (const void *)key.addr, (size_t)p->ldb * es,
+                                     (size_t)rows * es, (size_t)cols);
                     h2d += (uint64_t)rows * cols * es;
                 }
                 if (rc) break;
@@ -366,10 +462,9 @@ static int run_plan(plan_t *p, bx_cache_t *c)
                     const int mc = (p->M - i * T < T) ? p->M - i * T : T;
                     const int nc = (p->N - k * T < T) ? p->N - k * T : T;
                     char *hc = p->C + ((int64_t)i * T + (int64_t)k * T * p->ldc) * es;
-                    rc = bxh_memcpy2d_d2h_async(hc, (size_t)p->ldc * es,
-                                                bx_cache_slot_ptr(c, cst[d * R + t]),
-                                                (size_t)T * es, (size_t)mc * es, (size_t)nc,
-                                                g_bx.s_d2h);
+                    rc = copy_out(p->pg_c, hc, (size_t)p->ldc * es,
+                                  bx_cache_slot_ptr(c, cst[d * R + t]), (size_t)T * es,
+                                  (size_t)mc * es, (size_t)nc);
                     d2h += (uint64_t)mc * nc * es;
                 }
             if (!rc) rc = bxh_event_record(g_rt.ev_d2h[d], g_bx.s_d2h);
@@ -381,6 +476,9 @@ static int run_plan(plan_t *p, bx_cache_t *c)
     int rc3 = bxh_stream_sync(g_bx.s_comp);
     int rc4 = bxh_stream_sync(g_bx.s_d2h);
     if (!rc) rc = rc2 ? rc2 : rc3 ? rc3 : rc4;
+    /* unpack staged C tiles into the caller's pageable C */
+    const int rc5 = drain_out(rc != 0);
+    if (!rc) rc = rc5;
 
     for (int t = 0; t < ntaken; ++t) bx_cache_return_scratch(c, cst[t]);
     free(cst);
@@ -433,11 +531,16 @@ int bx_tile_gemm(int dt, int opa, int opb, int M, int N, int K, const double *al
         p.beta[0] = beta[0];
         p.beta[1] = beta[1];
         p.beta_zero = beta[0] == 0.0 && beta[1] == 0.0;
+        /* DMA needs pinned host memory; anything else goes through staging */
+        p.pg_a = bxh_pointer_is_pinned(A) == 0;
+        p.pg_b = bxh_pointer_is_pinned(B) == 0;
+        p.pg_c = bxh_pointer_is_pinned(C) == 0;
+        if (p.pg_a || p.pg_b || p.pg_c) rc = ensure_staging();
         choose_round_shape(&p, bx_cache_capacity_slots(c));
         bx_cache_reset_rounds(c);
         bx_log(2, "tile gemm dt=%d %dx%dx%d T=%d rounds of %dx%d tiles", dt, M, N, K, T, p.RM,
                p.RN);
-        rc = run_plan(&p, c);
+        if (!rc) rc = run_plan(&p, c);
         if (!rc) {
             BX_STAT_ADD(bytes_lower_bound,
                         (uint64_t)p.es * ((uint64_t)M * K + (uint64_t)K * N +

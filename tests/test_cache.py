@@ -203,6 +203,24 @@ def test_queue_exactly_once_under_threads(bx):
     L.bxtest_queue_destroy(q)
 
 
+def test_host_copy_pool_strided(bx):
+    """The parallel staging copy moves `height` columns of `width` bytes
+    between pitched buffers exactly (multi-threaded above 1 MiB)."""
+    import numpy as np
+
+    L = bx.lib()
+    L.bxtest_host_copy2d.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_size_t,
+                                     C.c_size_t]
+    rng = np.random.default_rng(3)
+    for rows, cols, lds, ldd in [(5, 3, 7, 5), (1000, 700, 1003, 1000), (2048, 513, 4000, 2050)]:
+        src = rng.standard_normal((lds, cols))
+        src = np.asfortranarray(src)
+        dst = np.asfortranarray(np.full((ldd, cols), -7.0))
+        L.bxtest_host_copy2d(dst.ctypes.data, ldd * 8, src.ctypes.data, lds * 8, rows * 8, cols)
+        assert np.array_equal(dst[:rows], src[:rows])
+        assert np.all(dst[rows:] == -7.0)
+
+
 def test_queue_order_matches_reference(bx, orc):
     ref = orc.ref_lib()
     if ref is None:
