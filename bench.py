@@ -197,7 +197,9 @@ def out_of_core(bx, n=65536, budget_frac=0.75):
     res = {"n": n}
     old = L.blasx_get_hbm_budget()
     try:
-        for name, budget in (("in_core", old), ("out_of_core", int(budget_frac * 2 * A.nbytes))):
+        cases = (("in_core", old), ("out_of_core", int(budget_frac * 2 * A.nbytes)),
+                 ("out_of_core_below_B", int(0.4 * 2 * A.nbytes)))
+        for name, budget in cases:
             L.blasx_finalize()
             L.blasx_set_hbm_budget(budget)
             bx.reset_stats()
@@ -219,6 +221,8 @@ def out_of_core(bx, n=65536, budget_frac=0.75):
         del A, B, Cm
     res["out_of_core_over_in_core"] = round(res["out_of_core"]["tflops"] /
                                             res["in_core"]["tflops"], 3)
+    res["below_B_over_in_core"] = round(res["out_of_core_below_B"]["tflops"] /
+                                        res["in_core"]["tflops"], 3)
     return res
 
 

@@ -426,6 +426,12 @@ class PinnedArray:
             lib().blasx_host_free(self.ptr)
             self.ptr = None
 
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
 
 def fill_dynrange_host(a, seed, mid, phi):
     """(u - 0.5) * exp(phi * g) inputs of the split-precision workload."""
