@@ -165,6 +165,9 @@ def lib():
                                      _vp]
     L.blasx_dgemm_split_device.argtypes = [i, i, i, i, i, d, _vp, i64, _vp, i64, d, _vp, i64,
                                            i, _vp]
+    L.blasx_dgemm_crt_device.argtypes = [i, i, i, i, i, d, _vp, i64, _vp, i64, d, _vp, i64, i,
+                                         _vp]
+    L.blasx_crt_bits.argtypes = [i, i]
     L.blasx_gemm_grouped_device.argtypes = [C.POINTER(GemmDesc), i, _vp]
     L.blasx_server_create.restype = _vp
     L.blasx_server_create.argtypes = [C.c_size_t, i]
@@ -389,6 +392,20 @@ def dgemm_split_device(ta, tb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, 
                                         beta, dC, ldc, slices, stream)
     if rc:
         raise RuntimeError(f"split dgemm failed ({rc})")
+
+
+def dgemm_crt_device(ta, tb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, moduli=0,
+                     stream=None):
+    """CRT split (Ozaki scheme II, INT8 MFMA) DGEMM on HBM pointers."""
+    rc = lib().blasx_dgemm_crt_device(TRANS[ta], TRANS[tb], M, N, K, alpha, dA, lda, dB, ldb,
+                                      beta, dC, ldc, moduli, stream)
+    if rc:
+        raise RuntimeError(f"crt dgemm failed ({rc})")
+
+
+def crt_bits(K, moduli=0):
+    """Bits kept per operand entry by the CRT split at this K."""
+    return lib().blasx_crt_bits(K, moduli)
 
 
 def zgemm_split_device(ta, tb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, slices=0,

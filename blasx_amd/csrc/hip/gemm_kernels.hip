@@ -25,6 +25,7 @@
 
 #include "../hiplayer.h"
 #include "blasx_ext.h"
+#include "gmem.h"
 
 namespace {
 
@@ -102,13 +103,13 @@ template <typename E, int VEC, int ROWS, int BK, int PAD, int MAJ, bool CONJ> st
                 const E *p0 = base + (r0 + (tid % TPR) * VEC) + (k0 + tid / TPR) * ld;
                 const int64_t step = (int64_t)RPP * ld;
 #pragma unroll
-                for (int p = 0; p < NL; ++p) r[p] = *(const Vec<E, VEC> *)(p0 + p * step);
+                for (int p = 0; p < NL; ++p) r[p] = gload((const Vec<E, VEC> *)(p0 + p * step));
             } else {
                 constexpr int TPR = BK / VEC, RPP = NT / TPR;
                 const E *p0 = base + (k0 + (tid % TPR) * VEC) + (int64_t)(r0 + tid / TPR) * ld;
                 const int64_t step = (int64_t)RPP * ld;
 #pragma unroll
-                for (int p = 0; p < NL; ++p) r[p] = *(const Vec<E, VEC> *)(p0 + p * step);
+                for (int p = 0; p < NL; ++p) r[p] = gload((const Vec<E, VEC> *)(p0 + p * step));
             }
             if constexpr (CONJ) {
 #pragma unroll
@@ -126,11 +127,11 @@ template <typename E, int VEC, int ROWS, int BK, int PAD, int MAJ, bool CONJ> st
                 const int64_t gk = k0 + kq + p * RPP;
                 const int gr = r0 + rr;
                 if (vec_ok && gk < klim && gr + VEC <= rlim) {
-                    r[p] = *(const Vec<E, VEC> *)(base + gr + gk * ld);
+                    r[p] = gload((const Vec<E, VEC> *)(base + gr + gk * ld));
                 } else {
 #pragma unroll
                     for (int e = 0; e < VEC; ++e)
-                        r[p].e[e] = (gk < klim && gr + e < rlim) ? base[gr + e + gk * ld]
+                        r[p].e[e] = (gk < klim && gr + e < rlim) ? gload(base + gr + e + gk * ld)
                                                                  : zero_of<E>();
                 }
             }
@@ -142,11 +143,11 @@ template <typename E, int VEC, int ROWS, int BK, int PAD, int MAJ, bool CONJ> st
                 const int gr = r0 + rq + p * RPP;
                 const int64_t gk = k0 + kv;
                 if (vec_ok && gr < rlim && gk + VEC <= klim) {
-                    r[p] = *(const Vec<E, VEC> *)(base + gk + (int64_t)gr * ld);
+                    r[p] = gload((const Vec<E, VEC> *)(base + gk + (int64_t)gr * ld));
                 } else {
 #pragma unroll
                     for (int e = 0; e < VEC; ++e)
-                        r[p].e[e] = (gr < rlim && gk + e < klim) ? base[gk + e + (int64_t)gr * ld]
+                        r[p].e[e] = (gr < rlim && gk + e < klim) ? gload(base + gk + e + (int64_t)gr * ld)
                                                                  : zero_of<E>();
                 }
             }
@@ -266,10 +267,21 @@ template <typename E> struct Smem {
     static constexpr int ELEMS = 2 * (A_ELEMS + B_ELEMS);
 };
 
-__device__ __forceinline__ const bxh_seg_t &seg_at(const bxh_seg_t *segs, const bxh_seg_t &seg0,
-                                                    int i)
+// segment i of the table, or the by-value segment when there is no table;
+// fields are read through the global address space one by one (selecting
+// between a table pointer and a local would make every access FLAT)
+__device__ __forceinline__ bxh_seg_t seg_at(const bxh_seg_t *segs, const bxh_seg_t &seg0, int i)
 {
-    return segs ? segs[i] : seg0;
+    if (!segs) return seg0;
+    const __attribute__((address_space(1))) bxh_seg_t *g =
+        (const __attribute__((address_space(1))) bxh_seg_t *)(segs + i);
+    bxh_seg_t s;
+    s.A = g->A;
+    s.B = g->B;
+    s.lda = g->lda;
+    s.ldb = g->ldb;
+    s.k = g->k;
+    return s;
 }
 
 // ---------------------------------------------------------------------------
@@ -309,7 +321,7 @@ __device__ __forceinline__ void gemm_block(const bxh_problem_t &P, const bxh_seg
     }
 
     auto load_both = [&](int si, int64_t k0) {
-        const bxh_seg_t &S = seg_at(segs, seg0, si);
+        const bxh_seg_t S = seg_at(segs, seg0, si);
         const E *A = reinterpret_cast<const E *>(S.A);
         const E *B = reinterpret_cast<const E *>(S.B);
         la.load(A, S.lda, m0, P.m, k0, S.k, aligned16(A, S.lda, sizeof(E)));
@@ -375,11 +387,11 @@ __device__ __forceinline__ void gemm_block(const bxh_problem_t &P, const bxh_seg
                     double orr = ar * sr - ai * sim, oi = ar * sim + ai * sr;
                     cplx *c = Cp + row + (int64_t)col * ldc;
                     if (!P.beta_zero) {
-                        const cplx c0 = *c;
+                        const cplx c0 = gload(c);
                         orr += br * c0.re - bi * c0.im;
                         oi += br * c0.im + bi * c0.re;
                     }
-                    *c = cplx{orr, oi};
+                    gstore(c, cplx{orr, oi});
                 }
             }
     } else {
@@ -401,18 +413,18 @@ __device__ __forceinline__ void gemm_block(const bxh_problem_t &P, const bxh_seg
 #pragma unroll
                             for (int e = 0; e < VEC; ++e) out.e[e] = alpha * accr[h * VEC + e][nb][v];
                         } else {
-                            const Vec<E, VEC> old = *(const Vec<E, VEC> *)c;
+                            const Vec<E, VEC> old = gload((const Vec<E, VEC> *)c);
 #pragma unroll
                             for (int e = 0; e < VEC; ++e)
                                 out.e[e] = alpha * accr[h * VEC + e][nb][v] + beta * old.e[e];
                         }
-                        *(Vec<E, VEC> *)c = out;
+                        gstore((Vec<E, VEC> *)c, out);
                     } else {
 #pragma unroll
                         for (int e = 0; e < VEC; ++e) {
                             if (rbase + e >= P.m) continue;
                             const E s = alpha * accr[h * VEC + e][nb][v];
-                            c[e] = P.beta_zero ? s : s + beta * c[e];
+                            gstore(c + e, P.beta_zero ? s : s + beta * gload(c + e));
                         }
                     }
                 }

@@ -19,9 +19,11 @@
 //  4. C = alpha * 2^(e_i + f_j) * W + beta * C  (k_ozaki_finish).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 
 #include "../hiplayer.h"
 #include "blasx_ext.h"
+#include "gmem.h"
 
 namespace {
 
@@ -46,13 +48,38 @@ __device__ __forceinline__ int exp_of(double amax)
     return e;
 }
 
-__global__ void k_exp_strided(const double *__restrict__ X, int64_t ld, int n, int len,
-                              int *__restrict__ e)
+// mode 0 runs in two passes so that the whole GPU streams X: pass 1 gives
+// block (bx, by) rows i in [64 bx, 64 bx + 64) and the by-th chunk of k,
+// 64 rows x 4 k per step (coalesced 512-byte row segments), and writes the
+// chunk's max |x| to part[by * n + i]; pass 2 takes the max over chunks.
+// fmax over non-negative values is associative, so the result equals the
+// single sequential max (NaN entries are skipped by fmax in both).
+constexpr int EXP_CHUNKS = 32;
+
+__global__ void k_amax_strided(const double *__restrict__ X, int64_t ld, int n, int len,
+                               int chunk, double *__restrict__ part)
+{
+    __shared__ double red[4][64];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + tx;
+    const int k0 = blockIdx.y * chunk, k1 = min(len, k0 + chunk);
+    double m = 0.0;
+    if (i < n)
+        for (int k = k0 + ty; k < k1; k += 4) m = fmax(m, fabs(X[i + (int64_t)k * ld]));
+    red[ty][tx] = m;
+    __syncthreads();
+    if (ty == 0 && i < n)
+        part[(int64_t)blockIdx.y * n + i] =
+            fmax(fmax(red[0][tx], red[1][tx]), fmax(red[2][tx], red[3][tx]));
+}
+
+__global__ void k_exp_parts(const double *__restrict__ part, int chunks, int n,
+                            int *__restrict__ e)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double m = 0.0;
-    for (int k = 0; k < len; ++k) m = fmax(m, fabs(X[i + (int64_t)k * ld]));
+    for (int c = 0; c < chunks; ++c) m = fmax(m, part[(int64_t)c * n + i]);
     e[i] = exp_of(m);
 }
 
@@ -150,29 +177,51 @@ __global__ void k_split_contig(const double *__restrict__ X, int64_t ld, int n, 
 //    to whole block tiles, so loads need no bounds checks.
 //    256 threads = 2x2 waves, wave tile 64x64 = 4x4 MFMA_I32_16X16X64_I8.
 // ---------------------------------------------------------------------------
+// Block tile index -> (tm, tn), walking groups of 8 tile rows column by
+// column.  The blocks of one XCD run a contiguous index range at a time
+// (see the XCD renumbering in the kernels), so those resident together form
+// an 8 x (blocks / 8) patch: per K step they read 8 A tiles and blocks/8 B
+// tiles through their shared L2 instead of 1 A tile and one B tile each.
+__device__ __forceinline__ void group_tile(int t, int tiles_m, int tiles_n, int &tm, int &tn)
+{
+    constexpr int G = 8;
+    const int per_group = G * tiles_n;
+    const int g = t / per_group;
+    const int first = g * G;
+    const int gsz = min(tiles_m - first, G);
+    const int r = t - g * per_group;
+    tm = first + r % gsz;
+    tn = r / gsz;
+}
+
 struct OzSeg {
     const int8_t *A;
     const int8_t *B;
 };
 
-__global__ __launch_bounds__(OZ_NT, 2) void k_ozaki_group(const OzSeg *__restrict__ segs,
-                                                          int nseg, int kp, int m, int n,
-                                                          double *__restrict__ W, int64_t ldw,
-                                                          double scale, int first)
+// block -> (m0, n0) of its BT x BT tile: XCD renumbering, then group_tile
+template <int BT> __device__ __forceinline__ void block_origin(int n, int &m0, int &n0)
 {
-    __shared__ __attribute__((aligned(16))) int8_t sA[2][OZ_BM * OZ_STR];
-    __shared__ __attribute__((aligned(16))) int8_t sB[2][OZ_BN * OZ_STR];
-    const int tiles_n = (n + OZ_BN - 1) / OZ_BN;
+    const int tiles_n = (n + BT - 1) / BT;
     int b = blockIdx.x;
     const int nb = gridDim.x;
     if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3); // contiguous tiles per XCD
-    const int tm = b / tiles_n, tn = b % tiles_n;
-    const int m0 = tm * OZ_BM, n0 = tn * OZ_BN;
+    int tm, tn;
+    group_tile(b, nb / tiles_n, tiles_n, tm, tn);
+    m0 = tm * BT;
+    n0 = tn * BT;
+}
+
+// 128x128 tile: acc[mb][nb] of wave (wm, wn) holds rows
+// m0 + 64 wm + 16 mb + 4 (lane/16) + v, column n0 + 64 wn + 16 nb + lane%16
+__device__ __forceinline__ void tile128(const OzSeg *__restrict__ segs, int nseg, int kp, int m0,
+                                        int n0, i32x4 (&acc)[4][4])
+{
+    __shared__ __attribute__((aligned(16))) int8_t sA[2][OZ_BM * OZ_STR];
+    __shared__ __attribute__((aligned(16))) int8_t sB[2][OZ_BN * OZ_STR];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int l16 = lane & 15, lg = lane >> 4;
-
-    i32x4 acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -181,13 +230,13 @@ __global__ __launch_bounds__(OZ_NT, 2) void k_ozaki_group(const OzSeg *__restric
     // loader: 128 rows x 64 bytes per operand = 512 x 16 B, two per thread
     const int lrow = tid >> 2, lk = (tid & 3) * 16;
     i32x4 ra[2], rb[2];
-    auto gload = [&](int seg, int k0) {
+    auto fetch = [&](int seg, int k0) {
         const int8_t *pa = segs[seg].A + (int64_t)(m0 + lrow) * kp + k0 + lk;
         const int8_t *pb = segs[seg].B + (int64_t)(n0 + lrow) * kp + k0 + lk;
-        ra[0] = *(const i32x4 *)pa;
-        ra[1] = *(const i32x4 *)(pa + (int64_t)64 * kp);
-        rb[0] = *(const i32x4 *)pb;
-        rb[1] = *(const i32x4 *)(pb + (int64_t)64 * kp);
+        ra[0] = gload((const i32x4 *)pa);
+        ra[1] = gload((const i32x4 *)(pa + (int64_t)64 * kp));
+        rb[0] = gload((const i32x4 *)pb);
+        rb[1] = gload((const i32x4 *)(pb + (int64_t)64 * kp));
     };
     auto lstore = [&](int buf) {
         *(i32x4 *)&sA[buf][lrow * OZ_STR + lk] = ra[0];
@@ -198,13 +247,13 @@ __global__ __launch_bounds__(OZ_NT, 2) void k_ozaki_group(const OzSeg *__restric
 
     const int ktiles = kp / OZ_BK;
     const int total = nseg * ktiles;
-    gload(0, 0);
+    fetch(0, 0);
     lstore(0);
     __syncthreads();
     for (int it = 0; it < total; ++it) {
         const int buf = it & 1;
         const bool more = it + 1 < total;
-        if (more) gload((it + 1) / ktiles, ((it + 1) % ktiles) * OZ_BK);
+        if (more) fetch((it + 1) / ktiles, ((it + 1) % ktiles) * OZ_BK);
         i32x4 fa[4], fb[4];
 #pragma unroll
         for (int mb = 0; mb < 4; ++mb)
@@ -221,49 +270,23 @@ __global__ __launch_bounds__(OZ_NT, 2) void k_ozaki_group(const OzSeg *__restric
         __syncthreads();
     }
 
-    // epilogue: output row of accumulator element v is 4*lg + v (as FP32)
-#pragma unroll
-    for (int mb = 0; mb < 4; ++mb)
-#pragma unroll
-        for (int nb2 = 0; nb2 < 4; ++nb2) {
-            const int col = n0 + wn * 64 + nb2 * 16 + l16;
-            if (col >= n) continue;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = m0 + wm * 64 + mb * 16 + 4 * lg + v;
-                if (row >= m) continue;
-                double *w = W + row + (int64_t)col * ldw;
-                const double add = scale * (double)acc[mb][nb2][v];
-                *w = first ? add : *w + add;
-            }
-        }
 }
 
-// Same computation with a 256x256 block tile: 4 waves of 128x128 (8x8
-// MFMA_I32_16X16X64_I8 accumulators, 256 registers, held in AGPRs), one
-// block per CU.  Twice the INT8 operations per byte of L2 traffic of the
-// 128x128 kernel (256 instead of 128 ops/byte) and half the LDS reads per
-// MFMA; used when the padded problem has enough 256x256 tiles to fill the GPU.
+// 256x256 tile: 4 waves of 128x128 (8x8 MFMA_I32_16X16X64_I8 accumulators,
+// 256 registers, held in AGPRs), one block per CU.  Twice the INT8
+// operations per byte of L2 traffic of the 128x128 tile and half the LDS
+// reads per MFMA.  acc[mb][nb] of wave (wm, wn) holds rows
+// m0 + 128 wm + 16 mb + 4 (lane/16) + v, column n0 + 128 wn + 16 nb + lane%16.
 constexpr int OZ_BIG = 256;
 
-__global__ __launch_bounds__(OZ_NT, 1) void k_ozaki_group_big(const OzSeg *__restrict__ segs,
-                                                              int nseg, int kp, int m, int n,
-                                                              double *__restrict__ W, int64_t ldw,
-                                                              double scale, int first)
+__device__ __forceinline__ void tile256(const OzSeg *__restrict__ segs, int nseg, int kp, int m0,
+                                        int n0, i32x4 (&acc)[8][8])
 {
-    __shared__ __attribute__((aligned(16))) int8_t sA[2][OZ_BIG * OZ_STR];
-    __shared__ __attribute__((aligned(16))) int8_t sB[2][OZ_BIG * OZ_STR];
-    const int tiles_n = (n + OZ_BIG - 1) / OZ_BIG;
-    int b = blockIdx.x;
-    const int nb = gridDim.x;
-    if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3); // contiguous tiles per XCD
-    const int tm = b / tiles_n, tn = b % tiles_n;
-    const int m0 = tm * OZ_BIG, n0 = tn * OZ_BIG;
+    __shared__ __attribute__((aligned(16))) int8_t sA[3][OZ_BIG * OZ_STR];
+    __shared__ __attribute__((aligned(16))) int8_t sB[3][OZ_BIG * OZ_STR];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int l16 = lane & 15, lg = lane >> 4;
-
-    i32x4 acc[8][8];
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
@@ -272,13 +295,17 @@ __global__ __launch_bounds__(OZ_NT, 1) void k_ozaki_group_big(const OzSeg *__res
     // loader: 256 rows x 64 bytes per operand = 1024 x 16 B, four per thread
     const int lrow = tid >> 2, lk = (tid & 3) * 16;
     i32x4 ra[4], rb[4];
-    auto gload = [&](int seg, int k0) {
+    const int ktiles = kp / OZ_BK;
+    const int total = nseg * ktiles;
+    // K tile t (over all segments) -> registers
+    auto fetch = [&](int t) {
+        const int seg = t / ktiles, k0 = (t - seg * ktiles) * OZ_BK;
         const int8_t *pa = segs[seg].A + (int64_t)(m0 + lrow) * kp + k0 + lk;
         const int8_t *pb = segs[seg].B + (int64_t)(n0 + lrow) * kp + k0 + lk;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            ra[i] = *(const i32x4 *)(pa + (int64_t)(64 * i) * kp);
-            rb[i] = *(const i32x4 *)(pb + (int64_t)(64 * i) * kp);
+            ra[i] = gload((const i32x4 *)(pa + (int64_t)(64 * i) * kp));
+            rb[i] = gload((const i32x4 *)(pb + (int64_t)(64 * i) * kp));
         }
     };
     auto lstore = [&](int buf) {
@@ -288,46 +315,123 @@ __global__ __launch_bounds__(OZ_NT, 1) void k_ozaki_group_big(const OzSeg *__res
             *(i32x4 *)&sB[buf][(lrow + 64 * i) * OZ_STR + lk] = rb[i];
         }
     };
+    auto rda = [&](int buf, int i) {
+        return *(const i32x4 *)&sA[buf][(wm * 128 + i * 16 + l16) * OZ_STR + lg * 16];
+    };
+    auto rdb = [&](int buf, int i) {
+        return *(const i32x4 *)&sB[buf][(wn * 128 + i * 16 + l16) * OZ_STR + lg * 16];
+    };
 
-    const int ktiles = kp / OZ_BK;
-    const int total = nseg * ktiles;
-    gload(0, 0);
+    // Software pipeline over three LDS buffers (tile t lives in buffer t%3).
+    // Iteration it:
+    //   * loads tile it+2 from HBM/L2 into registers and stages it in buffer
+    //     (it+2)%3 at the end (that buffer's previous tile, it-1, was last
+    //     read in iteration it-1, before its closing barrier);
+    //   * runs the 64 MFMAs of tile it: its 8 B fragments and first A
+    //     fragment are already in registers, the other A fragments are read
+    //     one row block ahead of their MFMAs;
+    //   * reads the B fragments and first A fragment of tile it+1 (staged in
+    //     iteration it-1) between those MFMAs.
+    // One barrier per iteration.  The body has no branches: past the last
+    // tile it re-fetches tile total-1 into a buffer never read again, and
+    // fragments read past the end are never used.
+    i32x4 fb[2][8], fa0[2];
+    fetch(0);
     lstore(0);
+    fetch(total > 1 ? 1 : 0);
+    lstore(1);
     __syncthreads();
-    for (int it = 0; it < total; ++it) {
-        const int buf = it & 1;
-        const bool more = it + 1 < total;
-        if (more) gload((it + 1) / ktiles, ((it + 1) % ktiles) * OZ_BK);
-        i32x4 fb[8];
 #pragma unroll
-        for (int nb2 = 0; nb2 < 8; ++nb2)
-            fb[nb2] = *(const i32x4 *)&sB[buf][(wn * 128 + nb2 * 16 + l16) * OZ_STR + lg * 16];
+    for (int i = 0; i < 8; ++i) fb[0][i] = rdb(0, i);
+    fa0[0] = rda(0, 0);
+    int bcur = 0; // buffer of tile it
+    for (int it = 0; it < total; ++it) {
+        const int bnxt = bcur == 2 ? 0 : bcur + 1, bfree = bnxt == 2 ? 0 : bnxt + 1;
+        fetch(it + 2 < total ? it + 2 : total - 1);
+        i32x4 fa = fa0[0];
 #pragma unroll
         for (int mb = 0; mb < 8; ++mb) {
-            const i32x4 fa = *(const i32x4 *)&sA[buf][(wm * 128 + mb * 16 + l16) * OZ_STR + lg * 16];
+            i32x4 fan;
+            if (mb < 7) fan = rda(bcur, mb + 1);
+            fb[1][mb] = rdb(bnxt, mb);
+            if (mb == 7) fa0[1] = rda(bnxt, 0);
 #pragma unroll
             for (int nb2 = 0; nb2 < 8; ++nb2)
-                acc[mb][nb2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa, fb[nb2], acc[mb][nb2], 0, 0, 0);
+                acc[mb][nb2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa, fb[0][nb2], acc[mb][nb2],
+                                                                     0, 0, 0);
+            if (mb < 7) fa = fan;
         }
-        if (more) lstore(buf ^ 1);
+        lstore(bfree);
+        // issue order: global loads; per row block its LDS reads, then its
+        // 8 MFMAs; the LDS stores between the last row block's MFMAs
+        __builtin_amdgcn_sched_group_barrier(0x020, 8, 0);
+#pragma unroll
+        for (int i = 0; i < 7; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);
+        }
+        __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+        }
         __syncthreads();
+        bcur = bnxt;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) fb[0][i] = fb[1][i];
+        fa0[0] = fa0[1];
     }
 
+}
+
+// W = (first ? 0 : W) + scale * (tile sum); rows >= m / columns >= n skipped
+template <int WT, int NB>
+__device__ __forceinline__ void fold_w(const i32x4 (&acc)[NB][NB], int m0, int n0, int m, int n,
+                                       double *__restrict__ W, int64_t ldw, double scale,
+                                       int first)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lg = lane >> 4;
 #pragma unroll
-    for (int mb = 0; mb < 8; ++mb)
+    for (int mb = 0; mb < NB; ++mb)
 #pragma unroll
-        for (int nb2 = 0; nb2 < 8; ++nb2) {
-            const int col = n0 + wn * 128 + nb2 * 16 + l16;
+        for (int nb2 = 0; nb2 < NB; ++nb2) {
+            const int col = n0 + wn * WT + nb2 * 16 + l16;
             if (col >= n) continue;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = m0 + wm * 128 + mb * 16 + 4 * lg + v;
+                const int row = m0 + wm * WT + mb * 16 + 4 * lg + v;
                 if (row >= m) continue;
                 double *w = W + row + (int64_t)col * ldw;
                 const double add = scale * (double)acc[mb][nb2][v];
                 *w = first ? add : *w + add;
             }
         }
+}
+
+__global__ __launch_bounds__(OZ_NT, 2) void k_ozaki_group(const OzSeg *__restrict__ segs,
+                                                          int nseg, int kp, int m, int n,
+                                                          double *__restrict__ W, int64_t ldw,
+                                                          double scale, int first)
+{
+    int m0, n0;
+    block_origin<OZ_BM>(n, m0, n0);
+    i32x4 acc[4][4];
+    tile128(segs, nseg, kp, m0, n0, acc);
+    fold_w<64, 4>(acc, m0, n0, m, n, W, ldw, scale, first);
+}
+
+__global__ __launch_bounds__(OZ_NT, 1) void k_ozaki_group_big(const OzSeg *__restrict__ segs,
+                                                              int nseg, int kp, int m, int n,
+                                                              double *__restrict__ W, int64_t ldw,
+                                                              double scale, int first)
+{
+    int m0, n0;
+    block_origin<OZ_BIG>(n, m0, n0);
+    i32x4 acc[8][8];
+    tile256(segs, nseg, kp, m0, n0, acc);
+    fold_w<128, 8>(acc, m0, n0, m, n, W, ldw, scale, first);
 }
 
 // 4. C = alpha * 2^(e_i + f_j) * W + beta * C
@@ -384,6 +488,226 @@ __global__ void k_zcombine(int m, int n, const double *__restrict__ pr, const do
     }
 }
 
+// ---------------------------------------------------------------------------
+// CRT split (Ozaki scheme II: Ozaki, Uchino, Imamura 2025).  Entries are
+// scaled to integers A'_ik = trunc(a_ik 2^(beta - e_i)), |A'| < 2^beta; for
+// each modulus m_l (pairwise coprime, <= 256) the symmetric residues
+// a_l = smod(A', m_l) in [-floor(m/2), ceil(m/2) - 1] fit int8, one exact
+// INT8 GEMM per modulus gives R_l = smod(a_l b_l^T, m_l), and the Chinese
+// remainder theorem recovers the integer product (|A'B'| < K 2^(2 beta) <=
+// M/8, M = prod m_l) in floating point:
+//   Q_k = sum_l Wk_l R_l      (w_l = (M/m_l)((M/m_l)^-1 mod m_l) cut into
+//                              40-bit chunks k = 1..4: exact sums)
+//   kq  = rint((Q_1 + Q_2) / M),  D_k = Q_k - kq M_k  (exact)
+//   S   = ((D_1 + D_2) + D_3) + D_4 = A'B' (1 + 3u)
+// and C = alpha 2^(e_i + f_j - 2 beta) S + beta C.
+// ---------------------------------------------------------------------------
+struct CrtMods {
+    int L;
+    int m[20];
+    double rinv[20];  // 1 / m_l, rounded
+    float rinvf[20];
+    uint8_t p2[20][64]; // 2^t mod m_l
+};
+
+// r in [lo - m, hi + m] -> [lo, hi], lo = -floor(m/2), hi = ceil(m/2) - 1
+__host__ __device__ __forceinline__ int smod_fix(int r, int m)
+{
+    const int lo = -(m >> 1), hi = ((m + 1) >> 1) - 1;
+    r = r > hi ? r - m : r;
+    r = r < lo ? r + m : r;
+    return r;
+}
+
+// |x| < 2^22: the float quotient is within 1 of round(x / m)
+__host__ __device__ __forceinline__ int smod_small(int x, int m, float rinvf)
+{
+    const int q = (int)rintf((float)x * rinvf);
+    return smod_fix(x - q * m, m);
+}
+
+// any int32: the double quotient is within 1 of round(x / m)
+__host__ __device__ __forceinline__ int smod_i32(int x, int m, double rinv)
+{
+    const int q = (int)rint((double)x * rinv);
+    return smod_fix(x - q * m, m);
+}
+
+// symmetric residue of an integer-valued double v, |v| < 2^116.  For
+// |v| >= 2^53, v = x 2^t with |x| < 2^53 integral, and the residue is that
+// of smod(x) * (2^t mod m).  For |v| < 2^53, q = rint(v / m) is within 1 of
+// the rounded quotient and v - q m is exact in the FMA.
+__host__ __device__ __forceinline__ int crt_resid(double v, int m, double rinv, float rinvf,
+                                                  const uint8_t *p2)
+{
+    if (!(fabs(v) < 0x1p116)) return 0; // non-finite input: the split needs finite entries
+    int p = 1;
+    if (fabs(v) >= 0x1p53) {
+        const int t = ilogb(v) - 52;
+        v = ldexp(v, -t);
+        p = p2[t];
+    }
+    const double q = rint(v * rinv);
+    int r = smod_fix((int)fma(-q, (double)m, v), m);
+    if (p != 1) r = smod_small(r * p, m, rinvf);
+    return r;
+}
+
+// contiguous (mode 1): x(i,k) = X[k + i*ld]; one thread per 16 consecutive
+// k of one row; all L residue planes (plane stride np*kp)
+__global__ __launch_bounds__(256) void k_crt_split_contig(const double *__restrict__ X, int64_t ld, int n, int len,
+                                   const int *__restrict__ e, int beta, CrtMods md,
+                                   int8_t *__restrict__ D, int np, int kp)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int chunks = kp / 16;
+    if (t >= (int64_t)np * chunks) return;
+    const int i = (int)(t / chunks), k0 = (int)(t % chunks) * 16;
+    double v[16];
+    const int sh = i < n ? beta - e[i] : 0;
+#pragma unroll
+    for (int b = 0; b < 16; ++b)
+        v[b] = (i < n && k0 + b < len) ? trunc(ldexp(X[k0 + b + (int64_t)i * ld], sh)) : 0.0;
+    for (int l = 0; l < md.L; ++l) {
+        uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+            const int r = crt_resid(v[b], md.m[l], md.rinv[l], md.rinvf[l], md.p2[l]);
+            w[b >> 2] |= ((uint32_t)(uint8_t)(int8_t)r) << (8 * (b & 3));
+        }
+        *(i32x4 *)(D + (int64_t)l * np * kp + (int64_t)i * kp + k0) =
+            i32x4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+    }
+}
+
+// strided (mode 0): x(i,k) = X[i + k*ld]; a 64 x 64 tile per block, staged
+// through LDS eight planes at a time so both the FP64 reads (along i) and
+// the int8 writes (along k) are coalesced
+__global__ __launch_bounds__(256) void k_crt_split_strided(const double *__restrict__ X, int64_t ld, int n, int len,
+                                    const int *__restrict__ e, int beta, CrtMods md,
+                                    int8_t *__restrict__ D, int np, int kp)
+{
+    __shared__ int8_t tile[8][SP_T][SP_T + 4];
+    const int i0 = blockIdx.x * SP_T, k0 = blockIdx.y * SP_T;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6; // 64 x 4
+    double v[SP_T / 4];
+    {
+        const int i = i0 + tx;
+        const int sh = i < n ? beta - e[i] : 0;
+#pragma unroll
+        for (int q = 0; q < SP_T / 4; ++q) {
+            const int k = k0 + ty + 4 * q;
+            v[q] = (i < n && k < len) ? trunc(ldexp(X[i + (int64_t)k * ld], sh)) : 0.0;
+        }
+    }
+    const int row = threadIdx.x >> 2, kc = (threadIdx.x & 3) * 16;
+    for (int l0 = 0; l0 < md.L; l0 += 8) {
+        const int cnt = md.L - l0 < 8 ? md.L - l0 : 8;
+        for (int p = 0; p < cnt; ++p) {
+            const int l = l0 + p;
+#pragma unroll
+            for (int q = 0; q < SP_T / 4; ++q)
+                tile[p][tx][ty + 4 * q] =
+                    (int8_t)crt_resid(v[q], md.m[l], md.rinv[l], md.rinvf[l], md.p2[l]);
+        }
+        __syncthreads();
+        const int i = i0 + row;
+        if (i < np)
+            for (int p = 0; p < cnt; ++p) {
+                int8_t *dst = D + (int64_t)(l0 + p) * np * kp + (int64_t)i * kp + k0 + kc;
+                uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+                for (int b = 0; b < 16; ++b)
+                    w[b >> 2] |= ((uint32_t)(uint8_t)tile[p][row][kc + b]) << (8 * (b & 3));
+                *(i32x4 *)dst = i32x4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+            }
+        __syncthreads();
+    }
+}
+
+// residues of one tile sum -> int8 plane (column-major, ld ldr); every lane
+// owns 4 consecutive rows of one column: one 32-bit store each
+template <int WT, int NB>
+__device__ __forceinline__ void store_resid(const i32x4 (&acc)[NB][NB], int m0, int n0, int mod,
+                                            double rinv, int8_t *__restrict__ R, int64_t ldr)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lg = lane >> 4;
+#pragma unroll
+    for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+        for (int nb2 = 0; nb2 < NB; ++nb2) {
+            const int col = n0 + wn * WT + nb2 * 16 + l16;
+            const int row = m0 + wm * WT + mb * 16 + 4 * lg;
+            uint32_t w = 0;
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                w |= ((uint32_t)(uint8_t)(int8_t)smod_i32(acc[mb][nb2][v], mod, rinv)) << (8 * v);
+            *(uint32_t *)(R + row + (int64_t)col * ldr) = w;
+        }
+}
+
+// one INT8 GEMM per modulus (blockIdx.y) over padded operands; R planes are
+// mp x np (ld mp), so no bounds checks
+__global__ __launch_bounds__(OZ_NT, 2) void k_crt_gemm(const OzSeg *__restrict__ segs, int kp,
+                                                       int np, CrtMods md, int8_t *__restrict__ R,
+                                                       int64_t ldr, int64_t plane)
+{
+    int m0, n0;
+    block_origin<OZ_BM>(np, m0, n0);
+    const int l = blockIdx.y;
+    i32x4 acc[4][4];
+    tile128(segs + l, 1, kp, m0, n0, acc);
+    store_resid<64, 4>(acc, m0, n0, md.m[l], md.rinv[l], R + l * plane, ldr);
+}
+
+__global__ __launch_bounds__(OZ_NT, 1) void k_crt_gemm_big(const OzSeg *__restrict__ segs, int kp,
+                                                           int np, CrtMods md,
+                                                           int8_t *__restrict__ R, int64_t ldr,
+                                                           int64_t plane)
+{
+    int m0, n0;
+    block_origin<OZ_BIG>(np, m0, n0);
+    const int l = blockIdx.y;
+    i32x4 acc[8][8];
+    tile256(segs + l, 1, kp, m0, n0, acc);
+    store_resid<128, 8>(acc, m0, n0, md.m[l], md.rinv[l], R + l * plane, ldr);
+}
+
+struct CrtRecon {
+    int L;
+    double W[4][20]; // 40-bit chunks of the CRT weights
+    double Mk[4];    // 40-bit chunks of M
+    double invM;     // 1 / (((M_4 + M_3) + M_2) + M_1)
+};
+
+__global__ void k_crt_recon(int m, int n, const int8_t *__restrict__ R, int64_t ldr,
+                            int64_t plane, CrtRecon c, const int *__restrict__ ea,
+                            const int *__restrict__ eb, int two_beta, double alpha, double beta,
+                            int beta_zero, double *__restrict__ C, int64_t ldc)
+{
+    const int64_t total = (int64_t)m * n;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % m), j = (int)(t / m);
+        const int8_t *r = R + i + (int64_t)j * ldr;
+        double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+        for (int l = 0; l < c.L; ++l) {
+            const double rl = (double)r[l * plane];
+            q0 = fma(c.W[0][l], rl, q0);
+            q1 = fma(c.W[1][l], rl, q1);
+            q2 = fma(c.W[2][l], rl, q2);
+            q3 = fma(c.W[3][l], rl, q3);
+        }
+        const double kq = rint((q0 + q1) * c.invM);
+        const double S = ((fma(-kq, c.Mk[0], q0) + fma(-kq, c.Mk[1], q1)) + fma(-kq, c.Mk[2], q2)) +
+                         fma(-kq, c.Mk[3], q3);
+        const double v = alpha * ldexp(S, ea[i] + eb[j] - two_beta);
+        double *cp = C + i + (int64_t)j * ldc;
+        *cp = beta_zero ? v : fma(beta, *cp, v);
+    }
+}
+
 thread_local char g_oz_err[256];
 int oz_fail(const char *what, hipError_t e)
 {
@@ -402,19 +726,28 @@ int bxh_oz_block_k(void) { return OZ_BK; }
 
 // exponents of the rows of op(X) (n rows, len = reduced extent):
 // contiguous != 0: x(i,k) = X[k + i*ld], else x(i,k) = X[i + k*ld]
+// part: EXP_CHUNKS * n doubles of scratch (contiguous == 0 only)
 int bxh_oz_exponents(const double *X, int64_t ld, int n, int len, int contiguous, int *e,
-                     void *stream)
+                     double *part, void *stream)
 {
     if (n <= 0) return 0;
     hipStream_t st = (hipStream_t)stream;
-    if (contiguous)
+    if (contiguous) {
         hipLaunchKernelGGL(k_exp_contig, dim3(n), dim3(256), 0, st, X, ld, n, len, e);
-    else
-        hipLaunchKernelGGL(k_exp_strided, dim3((n + 255) / 256), dim3(256), 0, st, X, ld, n, len,
+    } else {
+        int chunk = (len + EXP_CHUNKS - 1) / EXP_CHUNKS;
+        chunk = chunk < 4 ? 4 : (chunk + 3) / 4 * 4;
+        const int chunks = len > 0 ? (len + chunk - 1) / chunk : 1;
+        hipLaunchKernelGGL(k_amax_strided, dim3((n + 63) / 64, chunks), dim3(256), 0, st, X, ld,
+                           n, len, chunk, part);
+        hipLaunchKernelGGL(k_exp_parts, dim3((n + 255) / 256), dim3(256), 0, st, part, chunks, n,
                            e);
+    }
     hipError_t r = hipGetLastError();
     return r == hipSuccess ? 0 : oz_fail("exponent launch", r);
 }
+
+int bxh_oz_exp_chunks(void) { return EXP_CHUNKS; }
 
 int bxh_oz_split(const double *X, int64_t ld, int n, int len, int contiguous, const int *e,
                  int s, int8_t *D, int np, int kp, void *stream)
@@ -438,7 +771,7 @@ int bxh_oz_group(const void *segs_dev, int nseg, int kp, int m, int n, int mp, i
                  int64_t ldw, double scale, int first, void *stream)
 {
     if (nseg <= 0) return 0;
-    if (mp % OZ_BM || np % OZ_BN || kp % OZ_BK) return BLASX_EINVAL;
+    if (kp <= 0 || mp % OZ_BM || np % OZ_BN || kp % OZ_BK) return BLASX_EINVAL;
     const int big_tiles = (mp % OZ_BIG == 0 && np % OZ_BIG == 0) ? (mp / OZ_BIG) * (np / OZ_BIG) : 0;
     if (big_tiles >= 256) { // enough 256x256 tiles to give every CU one
         hipLaunchKernelGGL(k_ozaki_group_big, dim3(big_tiles), dim3(OZ_NT), 0,
@@ -452,6 +785,105 @@ int bxh_oz_group(const void *segs_dev, int nseg, int kp, int m, int n, int mp, i
     hipError_t r = hipGetLastError();
     return r == hipSuccess ? 0 : oz_fail("ozaki group launch", r);
 }
+
+// ---- CRT split -------------------------------------------------------------
+// host view of the constant tables (same layout as the kernels' structs)
+struct bxh_crt_mods_host {
+    int L;
+    int m[20];
+    double rinv[20];
+    float rinvf[20];
+    uint8_t p2[20][64];
+};
+struct bxh_crt_recon_host {
+    int L;
+    double W[4][20];
+    double Mk[4];
+    double invM;
+};
+static_assert(sizeof(bxh_crt_mods_host) == sizeof(CrtMods), "layout");
+static_assert(sizeof(bxh_crt_recon_host) == sizeof(CrtRecon), "layout");
+
+static CrtMods crt_mods(int L, const int *m)
+{
+    CrtMods md;
+    memset(&md, 0, sizeof md);
+    md.L = L;
+    for (int l = 0; l < L; ++l) {
+        md.m[l] = m[l];
+        md.rinv[l] = 1.0 / m[l];
+        md.rinvf[l] = 1.0f / (float)m[l];
+        int p = 1 % m[l];
+        for (int t = 0; t < 64; ++t) {
+            md.p2[l][t] = (uint8_t)p;
+            p = (p * 2) % m[l];
+        }
+    }
+    return md;
+}
+
+// residue planes of op(X) rows (n x len, padded np x kp), modes as bxh_oz_split
+int bxh_crt_split(const double *X, int64_t ld, int n, int len, int contiguous, const int *e,
+                  int beta, int L, const int *moduli, int8_t *D, int np, int kp, void *stream)
+{
+    if (L < 1 || L > 20 || np % 64 || kp % 64) return BLASX_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    const CrtMods md = crt_mods(L, moduli);
+    if (contiguous) {
+        const int64_t total = (int64_t)np * (kp / 16);
+        hipLaunchKernelGGL(k_crt_split_contig, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                           st, X, ld, n, len, e, beta, md, D, np, kp);
+    } else {
+        hipLaunchKernelGGL(k_crt_split_strided, dim3(np / SP_T, kp / SP_T), dim3(256), 0, st, X,
+                           ld, n, len, e, beta, md, D, np, kp);
+    }
+    hipError_t r = hipGetLastError();
+    return r == hipSuccess ? 0 : oz_fail("crt split launch", r);
+}
+
+// segs_dev: L (A plane, B plane) pointers; R: L planes of mp x np int8 (ld mp)
+int bxh_crt_gemm(const void *segs_dev, int L, const int *moduli, int kp, int mp, int np,
+                 int8_t *R, void *stream)
+{
+    if (L < 1 || L > 20 || kp <= 0 || mp % OZ_BM || np % OZ_BN || kp % OZ_BK) return BLASX_EINVAL;
+    const CrtMods md = crt_mods(L, moduli);
+    const int64_t plane = (int64_t)mp * np;
+    const int big_tiles = (mp % OZ_BIG == 0 && np % OZ_BIG == 0) ? (mp / OZ_BIG) * (np / OZ_BIG) : 0;
+    if (big_tiles * L >= 256) {
+        hipLaunchKernelGGL(k_crt_gemm_big, dim3(big_tiles, L), dim3(OZ_NT), 0, (hipStream_t)stream,
+                           (const OzSeg *)segs_dev, kp, np, md, R, (int64_t)mp, plane);
+    } else {
+        hipLaunchKernelGGL(k_crt_gemm, dim3((mp / OZ_BM) * (np / OZ_BN), L), dim3(OZ_NT), 0,
+                           (hipStream_t)stream, (const OzSeg *)segs_dev, kp, np, md, R,
+                           (int64_t)mp, plane);
+    }
+    hipError_t r = hipGetLastError();
+    return r == hipSuccess ? 0 : oz_fail("crt gemm launch", r);
+}
+
+int bxh_crt_recon(int m, int n, const int8_t *R, int64_t ldr, int64_t plane, const void *recon,
+                  const int *ea, const int *eb, int beta_bits, double alpha, double beta,
+                  double *C, int64_t ldc, void *stream)
+{
+    if (m <= 0 || n <= 0) return 0;
+    CrtRecon c;
+    memcpy(&c, recon, sizeof c);
+    const int64_t total = (int64_t)m * n;
+    int grid = (int)((total + 255) / 256);
+    if (grid > 262144) grid = 262144;
+    hipLaunchKernelGGL(k_crt_recon, dim3(grid), dim3(256), 0, (hipStream_t)stream, m, n, R, ldr,
+                       plane, c, ea, eb, 2 * beta_bits, alpha, beta, beta == 0.0 ? 1 : 0, C, ldc);
+    hipError_t r = hipGetLastError();
+    return r == hipSuccess ? 0 : oz_fail("crt recon launch", r);
+}
+
+// host-side check of the residue helpers the kernels use (same code, CPU)
+int bxh_crt_resid_host(double v, int m)
+{
+    const CrtMods md = crt_mods(1, &m);
+    return crt_resid(v, m, md.rinv[0], md.rinvf[0], md.p2[0]);
+}
+int bxh_crt_smod_i32_host(int x, int m) { return smod_i32(x, m, 1.0 / m); }
 
 int bxh_oz_deinterleave(const double *Z, int64_t ld, int rows, int cols, int conj, double *re,
                         double *im, void *stream)

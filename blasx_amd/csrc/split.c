@@ -17,7 +17,8 @@
 int bxh_oz_block_m(void);
 int bxh_oz_block_k(void);
 int bxh_oz_exponents(const double *X, int64_t ld, int n, int len, int contiguous, int *e,
-                     void *stream);
+                     double *part, void *stream);
+int bxh_oz_exp_chunks(void);
 int bxh_oz_split(const double *X, int64_t ld, int n, int len, int contiguous, const int *e,
                  int s, int8_t *D, int np, int kp, void *stream);
 int bxh_oz_group(const void *segs_dev, int nseg, int kp, int m, int n, int mp, int np, double *W,
@@ -73,7 +74,8 @@ int bx_dgemm_split(int opa, int opb, int M, int N, int K, double alpha, const do
     const size_t wbytes = (size_t)M * N * sizeof(double);
     const size_t off_b8 = rup(a8, 256), off_w = off_b8 + rup(b8, 256);
     const size_t off_ea = off_w + rup(wbytes, 256), off_eb = off_ea + rup((size_t)M * 4, 256);
-    const size_t total = off_eb + rup((size_t)N * 4, 256);
+    const size_t off_part = off_eb + rup((size_t)N * 4, 256);
+    const size_t total = off_part + (size_t)bxh_oz_exp_chunks() * (M > N ? M : N) * sizeof(double);
 
     pthread_mutex_lock(&g_oz_lock);
     int rc = 0;
@@ -99,12 +101,13 @@ int bx_dgemm_split(int opa, int opb, int M, int N, int K, double alpha, const do
     int8_t *DA = (int8_t *)base, *DB = (int8_t *)(base + off_b8);
     double *W = (double *)(base + off_w);
     int *ea = (int *)(base + off_ea), *eb = (int *)(base + off_eb);
+    double *part = (double *)(base + off_part);
 
     /* exponents and slices: rows of op(A), columns of op(B) */
     const int a_contig = opa != BX_OP_N; /* A^T: row i of op(A) is column i of A */
     const int b_contig = opb == BX_OP_N; /* B: column j of op(B) is column j of B */
-    rc = bxh_oz_exponents(A, lda, M, K, a_contig, ea, stream);
-    if (!rc) rc = bxh_oz_exponents(B, ldb, N, K, b_contig, eb, stream);
+    rc = bxh_oz_exponents(A, lda, M, K, a_contig, ea, part, stream);
+    if (!rc) rc = bxh_oz_exponents(B, ldb, N, K, b_contig, eb, part, stream);
     if (!rc) rc = bxh_oz_split(A, lda, M, K, a_contig, ea, s, DA, mp, kp, stream);
     if (!rc) rc = bxh_oz_split(B, ldb, N, K, b_contig, eb, s, DB, np, kp, stream);
 

@@ -94,3 +94,29 @@ BX_EXPORT void bxtest_host_copy2d(void *dst, size_t dpitch, const void *src, siz
 {
     bx_host_copy2d(dst, dpitch, src, spitch, width, height);
 }
+
+/* CRT split: the host-computed reconstruction constants and the residue
+ * helpers the kernels use (compiled for the host from the same source) */
+typedef struct {
+    int L;
+    double W[4][20];
+    double Mk[4];
+    double invM;
+} crt_recon_view_t;
+void bx_crt_consts(int L, void *c);
+int bxh_crt_resid_host(double v, int m);
+int bxh_crt_smod_i32_host(int x, int m);
+
+BX_EXPORT void bxtest_crt_consts(int L, double *W /* [4][20] */, double *Mk, double *invM)
+{
+    crt_recon_view_t c;
+    bx_crt_consts(L, &c);
+    for (int k = 0; k < 4; ++k) {
+        Mk[k] = c.Mk[k];
+        for (int l = 0; l < 20; ++l) W[k * 20 + l] = c.W[k][l];
+    }
+    *invM = c.invM;
+}
+
+BX_EXPORT int bxtest_crt_resid(double v, int m) { return bxh_crt_resid_host(v, m); }
+BX_EXPORT int bxtest_crt_smod_i32(int x, int m) { return bxh_crt_smod_i32_host(x, m); }

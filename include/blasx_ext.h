@@ -139,6 +139,20 @@ int blasx_zgemm_split_device(int transa, int transb, int M, int N, int K, const 
                              const double *beta, double *dC, int64_t ldc, int slices,
                              void *stream);
 
+/* ---- CRT split DGEMM (Ozaki scheme II on INT8 MFMA) ------------------------
+ * No counterpart in the reference.  Entries of op(A) rows / op(B) columns
+ * are scaled by powers of two to integers of `bits` bits, reduced modulo
+ * `moduli` pairwise coprime moduli <= 256 (2..20; 0 = the fewest reaching
+ * 56 bits at this K), multiplied with one exact INT8 GEMM per modulus and
+ * recombined by the Chinese remainder theorem in FP64.  bits =
+ * floor((log2 prod(moduli) - log2 K - 3) / 2), reported by blasx_crt_bits.
+ * K <= 131071.  Each entry keeps `bits` bits relative to the largest
+ * magnitude in its row (op(B): column).  Inputs must be finite. */
+int blasx_dgemm_crt_device(int transa, int transb, int M, int N, int K, double alpha,
+                           const double *dA, int64_t lda, const double *dB, int64_t ldb,
+                           double beta, double *dC, int64_t ldc, int moduli, void *stream);
+int blasx_crt_bits(int K, int moduli);
+
 /* ---- grouped GEMM -----------------------------------------------------------
  * No counterpart in the reference (which issues one vendor call per tile
  * product, blas/blasx_dgemm.c:87-94).  Many independent GEMMs of mixed

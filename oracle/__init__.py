@@ -85,6 +85,20 @@ def lib():
         L.orc_zgemm_ozaki.argtypes = [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, _dp, _dp,
                                       C.c_int, _dp, C.c_int, _dp, _dp, C.c_int, C.c_int,
                                       C.c_int]
+        L.orc_crt_modulus.argtypes = [C.c_int]
+        L.orc_crt_beta.argtypes = [C.c_int, C.c_int]
+        L.orc_smod.restype = C.c_int64
+        L.orc_smod.argtypes = [C.c_int64, C.c_int64]
+        L.orc_crt_resid.argtypes = [C.c_double, C.c_int]
+        L.orc_crt_digits.argtypes = [C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]
+        L.orc_crt_horner.restype = C.c_double
+        L.orc_crt_horner.argtypes = [C.POINTER(C.c_int), C.c_int]
+        L.orc_crt_consts.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.orc_crt_float.restype = C.c_double
+        L.orc_crt_float.argtypes = [C.POINTER(C.c_int), C.c_int, _dp, _dp, C.c_double]
+        L.orc_dgemm_crt.argtypes = [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, C.c_double,
+                                    _dp, C.c_int, _dp, C.c_int, C.c_double, _dp, C.c_int,
+                                    C.c_int]
         L.orc_flops_gemm.restype = C.c_double
         L.orc_flops_gemm.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
         L.orc_openblas_load.argtypes = [C.c_char_p]
@@ -236,6 +250,63 @@ def zgemm_ozaki(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C_, ldc, s, segmax
     lib().orc_zgemm_ozaki(ta.encode(), tb.encode(), M, N, K, _ptr(al, _dp),
                           A.ctypes.data_as(_dp), lda, B.ctypes.data_as(_dp), ldb, _ptr(be, _dp),
                           C_.ctypes.data_as(_dp), ldc, s, segmax)
+    return C_
+
+
+def crt_moduli(L):
+    """The first L moduli of the CRT split (pairwise coprime, <= 256)."""
+    return [lib().orc_crt_modulus(l) for l in range(L)]
+
+
+def crt_beta(K, L):
+    """Bits kept per operand entry by the CRT split with L moduli."""
+    return lib().orc_crt_beta(K, L)
+
+
+def smod(v, m):
+    return lib().orc_smod(v, m)
+
+
+def crt_resid(v, m):
+    """Symmetric residue of the integer-valued double v modulo m."""
+    return lib().orc_crt_resid(v, m)
+
+
+def crt_digits(r):
+    """Balanced mixed-radix (Garner) digits of residues r[l] mod m_l."""
+    L = len(r)
+    ri = (C.c_int * L)(*r)
+    v = (C.c_int * L)()
+    lib().orc_crt_digits(ri, L, v)
+    return [int(v[l]) for l in range(L)]
+
+
+def crt_horner(v):
+    vi = (C.c_int * len(v))(*v)
+    return lib().orc_crt_horner(vi, len(v))
+
+
+def crt_consts(L):
+    """(Wk[4][L], Mk[4], 1/Md): the 40-bit chunks of the CRT weights and of M."""
+    W = np.zeros(4 * L)
+    Mk = np.zeros(4)
+    inv = C.c_double()
+    lib().orc_crt_consts(L, W.ctypes.data_as(_dp), Mk.ctypes.data_as(_dp), C.byref(inv))
+    return W.reshape(4, L), Mk, inv.value
+
+
+def crt_float(r, consts):
+    """Floating CRT reconstruction of residues r with crt_consts(len(r))."""
+    W, Mk, inv = consts
+    ri = (C.c_int * len(r))(*r)
+    Wc = np.ascontiguousarray(W, dtype=np.float64)
+    return lib().orc_crt_float(ri, len(r), Wc.ctypes.data_as(_dp), Mk.ctypes.data_as(_dp), inv)
+
+
+def dgemm_crt(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C_, ldc, L):
+    """CPU restatement of the CRT split DGEMM; C_ updated in place."""
+    lib().orc_dgemm_crt(ta.encode(), tb.encode(), M, N, K, alpha, _ptr(A, _dp), lda,
+                        _ptr(B, _dp), ldb, beta, _ptr(C_, _dp), ldc, L)
     return C_
 
 

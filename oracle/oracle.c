@@ -590,3 +590,232 @@ void orc_zgemm_ozaki(char ta, char tb, int M, int N, int K, const double *alpha,
     free(Pr);
     free(Pi);
 }
+
+/* ------------------------------------------------------------------ */
+/* CRT split DGEMM (Ozaki scheme II: Ozaki, Uchino, Imamura 2025,      */
+/* integer matrix products modulo small coprime moduli, recombined by   */
+/* the Chinese remainder theorem).  No reference counterpart; restates  */
+/* the method in the library's exact evaluation order:                  */
+/*   moduli m_1..m_L from ORC_CRT_MODS (pairwise coprime, <= 256),       */
+/*   beta = floor((sum log2 m_l - log2 K - 3) / 2 - 2^-20),             */
+/*   e_i, f_j as in the scheme above,                                   */
+/*   A'_ik = trunc(ldexp(op(A)_ik, beta - e_i)),  |A'| < 2^beta,        */
+/*   a_l = smod(A', m_l) with smod(v, m) in [-floor(m/2), ceil(m/2)-1], */
+/*   R_l = smod(a_l b_l^T, m_l)  (exact integer products),              */
+/*   floating CRT: M = prod m_l with bit length E, w_l = (M/m_l) y_l,   */
+/*   y_l = (M/m_l)^-1 mod m_l; w_l and M are cut into 40-bit chunks,    */
+/*   chunk k holding bits [max(0, E-40k), E-40(k-1)), k = 1..4, so that  */
+/*     Q_k = sum_l Wk_l R_l       (exact: |Q_k| < 20 2^47 chunk units)  */
+/*     Md = ((M_4 + M_3) + M_2) + M_1,  kq = rint((Q_1 + Q_2) / Md)     */
+/*        (computed as (Q_1 + Q_2) * (1 / Md)), |kq| < 2^12             */
+/*     D_k = Q_k - kq M_k         (exact: < 2^53 chunk units)           */
+/*     S = ((D_1 + D_2) + D_3) + D_4                                    */
+/*   and C = alpha * ldexp(S, e_i + f_j - 2 beta)  (+ fma(beta, C, .)). */
+/* |A'B'| < K 2^(2 beta) <= M/8, so Q = P + kq M with |P| < M/8 and kq  */
+/* is exact.  The partial sum D_1 + .. + D_j equals P minus the lower  */
+/* chunks, so it is exact whenever it is small, and S = P (1 + 3u) for */
+/* every P, with no cancellation loss.                                  */
+/* Garner's mixed-radix digits (orc_crt_digits) give the exact integer  */
+/* and check the residues independently.                                */
+/* ------------------------------------------------------------------ */
+static const int ORC_CRT_MODS[20] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
+                                     223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
+
+int orc_crt_modulus(int l) { return (l >= 0 && l < 20) ? ORC_CRT_MODS[l] : 0; }
+
+int orc_crt_beta(int K, int L)
+{
+    double lm = 0.0;
+    for (int l = 0; l < L; ++l) lm += log2((double)ORC_CRT_MODS[l]);
+    return (int)floor((lm - log2((double)(K > 0 ? K : 1)) - 3.0) / 2.0 - 0x1p-20);
+}
+
+int64_t orc_smod(int64_t v, int64_t m)
+{
+    int64_t r = v % m;
+    if (r < 0) r += m;
+    if (r >= (m + 1) / 2) r -= m;
+    return r;
+}
+
+/* residue of an integer-valued double of any magnitude, with int64 math */
+int orc_crt_resid(double v, int m)
+{
+    int t = 0;
+    if (fabs(v) >= 0x1p53) {
+        t = ilogb(v) - 52;
+        v = ldexp(v, -t);
+    }
+    int64_t r = orc_smod((int64_t)v, m), p = 1 % m;
+    for (int k = 0; k < t; ++k) p = (p * 2) % m;
+    return (int)orc_smod(r * p, m);
+}
+
+static int orc_inv_mod(int a, int m)
+{
+    a %= m;
+    for (int x = 1; x < m; ++x)
+        if ((a * x) % m == 1) return x;
+    return 0;
+}
+
+/* Garner digits of residues r[0..L-1] (symmetric); returns digits v[] */
+void orc_crt_digits(const int *r, int L, int *v)
+{
+    for (int l = 0; l < L; ++l) {
+        const int m = ORC_CRT_MODS[l];
+        int64_t t = r[l];
+        for (int j = 0; j < l; ++j)
+            t = orc_smod((t - v[j]) * orc_inv_mod(ORC_CRT_MODS[j], m), m);
+        v[l] = (int)orc_smod(t, m);
+    }
+}
+
+double orc_crt_horner(const int *v, int L)
+{
+    double S = (double)v[L - 1];
+    for (int l = L - 2; l >= 0; --l) S = fma(S, (double)ORC_CRT_MODS[l], (double)v[l]);
+    return S;
+}
+
+/* little-endian base-2^32 naturals, 6 limbs (M < 2^160) */
+typedef struct {
+    uint32_t d[6];
+} orc_big;
+
+static void big_mul_small(orc_big *x, uint32_t f)
+{
+    uint64_t carry = 0;
+    for (int i = 0; i < 6; ++i) {
+        uint64_t t = (uint64_t)x->d[i] * f + carry;
+        x->d[i] = (uint32_t)t;
+        carry = t >> 32;
+    }
+}
+
+static void big_div_small(orc_big *x, uint32_t q)
+{
+    uint64_t rem = 0;
+    for (int i = 5; i >= 0; --i) {
+        uint64_t cur = (rem << 32) | x->d[i];
+        x->d[i] = (uint32_t)(cur / q);
+        rem = cur % q;
+    }
+}
+
+static uint32_t big_mod_small(const orc_big *x, uint32_t q)
+{
+    uint64_t rem = 0;
+    for (int i = 5; i >= 0; --i) rem = ((rem << 32) | x->d[i]) % q;
+    return (uint32_t)rem;
+}
+
+static int big_bitlen(const orc_big *x)
+{
+    for (int i = 5; i >= 0; --i)
+        if (x->d[i])
+            for (int b = 31; b >= 0; --b)
+                if (x->d[i] >> b & 1) return 32 * i + b + 1;
+    return 0;
+}
+
+/* bits [lo, hi) of x as an exact double scaled by 2^lo */
+static double big_bits(const orc_big *x, int lo, int hi)
+{
+    double v = 0.0;
+    for (int b = hi - 1; b >= lo; --b) v = v * 2.0 + (double)(x->d[b / 32] >> (b % 32) & 1);
+    return ldexp(v, lo);
+}
+
+/* floating CRT constants: Wk[k*L + l], Mk[k] (k = 0..3), inverse of Md */
+void orc_crt_consts(int L, double *Wk, double *Mk, double *invM)
+{
+    orc_big M = {{1, 0, 0, 0, 0, 0}};
+    for (int l = 0; l < L; ++l) big_mul_small(&M, (uint32_t)ORC_CRT_MODS[l]);
+    const int E = big_bitlen(&M);
+    for (int k = 0; k < 4; ++k) {
+        const int hi = E - 40 * k, lo = E - 40 * (k + 1) > 0 ? E - 40 * (k + 1) : 0;
+        Mk[k] = hi > 0 ? big_bits(&M, lo, hi) : 0.0;
+    }
+    for (int l = 0; l < L; ++l) {
+        const uint32_t m = (uint32_t)ORC_CRT_MODS[l];
+        orc_big w = M;
+        big_div_small(&w, m);                            /* M / m_l */
+        const uint32_t y = (uint32_t)orc_inv_mod((int)big_mod_small(&w, m), (int)m);
+        big_mul_small(&w, y);                            /* w_l = (M/m_l) y_l < M */
+        for (int k = 0; k < 4; ++k) {
+            const int hi = E - 40 * k, lo = E - 40 * (k + 1) > 0 ? E - 40 * (k + 1) : 0;
+            Wk[k * L + l] = hi > 0 ? big_bits(&w, lo, hi) : 0.0;
+        }
+    }
+    const double Md = ((Mk[3] + Mk[2]) + Mk[1]) + Mk[0];
+    *invM = 1.0 / Md;
+}
+
+/* floating CRT of residues r[0..L-1] with the constants above */
+double orc_crt_float(const int *r, int L, const double *Wk, const double *Mk, double invM)
+{
+    double Q[4];
+    for (int k = 0; k < 4; ++k) {
+        double q = 0.0;
+        for (int l = 0; l < L; ++l) q = fma(Wk[k * L + l], (double)r[l], q);
+        Q[k] = q;
+    }
+    const double kq = rint((Q[0] + Q[1]) * invM);
+    double D[4];
+    for (int k = 0; k < 4; ++k) D[k] = fma(-kq, Mk[k], Q[k]);
+    return ((D[0] + D[1]) + D[2]) + D[3];
+}
+
+void orc_dgemm_crt(char ta, char tb, int M, int N, int K, double alpha, const double *A, int lda,
+                   const double *B, int ldb, double beta, double *C, int ldc, int L)
+{
+    const int bb = orc_crt_beta(K, L);
+    double Wk[4 * 20], Mk[4], invM;
+    orc_crt_consts(L, Wk, Mk, &invM);
+    int *ea = (int *)malloc(sizeof(int) * (M > 0 ? M : 1));
+    int *eb = (int *)malloc(sizeof(int) * (N > 0 ? N : 1));
+    int8_t *RA = (int8_t *)calloc((size_t)L * M * (K > 0 ? K : 1), 1);
+    int8_t *RB = (int8_t *)calloc((size_t)L * N * (K > 0 ? K : 1), 1);
+    for (int i = 0; i < M; ++i) {
+        double mx = 0;
+        for (int p = 0; p < K; ++p) mx = fmax(mx, fabs(A[OPIDX(ta, i, p, lda)]));
+        ea[i] = orc_exp_of(mx);
+        for (int p = 0; p < K; ++p) {
+            const double v = trunc(ldexp(A[OPIDX(ta, i, p, lda)], bb - ea[i]));
+            for (int l = 0; l < L; ++l)
+                RA[((size_t)l * M + i) * K + p] = (int8_t)orc_crt_resid(v, ORC_CRT_MODS[l]);
+        }
+    }
+    for (int j = 0; j < N; ++j) {
+        double mx = 0;
+        for (int p = 0; p < K; ++p) mx = fmax(mx, fabs(B[OPIDX(tb, p, j, ldb)]));
+        eb[j] = orc_exp_of(mx);
+        for (int p = 0; p < K; ++p) {
+            const double v = trunc(ldexp(B[OPIDX(tb, p, j, ldb)], bb - eb[j]));
+            for (int l = 0; l < L; ++l)
+                RB[((size_t)l * N + j) * K + p] = (int8_t)orc_crt_resid(v, ORC_CRT_MODS[l]);
+        }
+    }
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < N; ++j) {
+        int r[20];
+        for (int i = 0; i < M; ++i) {
+            for (int l = 0; l < L; ++l) {
+                const int8_t *ra = RA + ((size_t)l * M + i) * K;
+                const int8_t *rb = RB + ((size_t)l * N + j) * K;
+                int64_t acc = 0;
+                for (int p = 0; p < K; ++p) acc += (int64_t)ra[p] * rb[p];
+                r[l] = (int)orc_smod(acc, ORC_CRT_MODS[l]);
+            }
+            const double S = orc_crt_float(r, L, Wk, Mk, invM);
+            const double x = alpha * ldexp(S, ea[i] + eb[j] - 2 * bb);
+            double *c = &C[i + (int64_t)j * ldc];
+            *c = (beta == 0.0) ? x : fma(beta, *c, x);
+        }
+    }
+    free(ea);
+    free(eb);
+    free(RA);
+    free(RB);
+}

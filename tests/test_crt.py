@@ -1,0 +1,297 @@
+"""CRT split DGEMM (Ozaki scheme II on INT8 MFMA).
+
+No reference counterpart: the method is pinned from its mathematics.
+  * CPU: the reconstruction constants of the library and of the oracle
+    equal the exact big-integer values; the residue helpers the kernels use
+    (compiled for the host from the same source) and the oracle's equal
+    Python's exact arithmetic; the oracle's floating CRT recovers integers
+    to 2u at every magnitude; Garner's digits rebuild the exact integer; on
+    integer inputs the oracle pipeline equals the exact product; its error
+    obeys the scheme's bound and falls as moduli are added.
+  * GPU: the library's result equals the oracle restatement bit for bit
+    (residues and integer products are exact, the FP64 reconstruction order
+    is fixed) and equals the exact product on integer inputs.
+"""
+import ctypes as C
+import math
+import random
+import sys
+
+import numpy as np
+import pytest
+
+EPS = np.finfo(np.float64).eps
+MODS = [256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
+        223, 217, 211, 199, 197, 193, 191, 181, 179, 173]
+
+
+def say(*a):
+    print("[crt]", *a)
+    sys.stdout.flush()
+
+
+def _smod(v, m):
+    r = v % m
+    return r - m if r >= (m + 1) // 2 else r
+
+
+def _exact_consts(L):
+    M = math.prod(MODS[:L])
+    E = M.bit_length()
+    ws = [(M // m) * pow(M // m, -1, m) for m in MODS[:L]]
+
+    def chunks(x):
+        out = []
+        for k in range(4):
+            hi, lo = E - 40 * k, max(0, E - 40 * (k + 1))
+            out.append(((x >> lo) & ((1 << (hi - lo)) - 1)) << lo if hi > 0 else 0)
+        return out
+
+    return M, [chunks(w) for w in ws], chunks(M)
+
+
+# ------------------------------------------------------------------ CPU
+
+def test_moduli_pairwise_coprime_and_beta(orc, bx):
+    assert orc.crt_moduli(20) == MODS
+    for i in range(20):
+        for j in range(i):
+            assert math.gcd(MODS[i], MODS[j]) == 1
+    L_ = bx.lib()
+    L_.blasx_crt_bits.argtypes = [C.c_int, C.c_int]
+    for K in (1, 7, 1000, 16384, 131071):
+        for L in range(2, 21):
+            b = orc.crt_beta(K, L)
+            assert L_.blasx_crt_bits(K, L) == b
+            if b >= 1:  # |A'B'| < K 2^(2b) <= M / 8 exactly
+                assert 8 * K * 2 ** (2 * b) <= math.prod(MODS[:L])
+
+
+@pytest.mark.parametrize("L", [2, 5, 8, 13, 16, 20])
+def test_reconstruction_constants_exact(orc, bx, L):
+    M, W, Mk = _exact_consts(L)
+    Lb = bx.lib()
+    Lb.bxtest_crt_consts.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    Wl = np.zeros(80)
+    Ml = np.zeros(4)
+    inv = C.c_double()
+    Lb.bxtest_crt_consts(L, Wl.ctypes.data, Ml.ctypes.data, C.byref(inv))
+    Wo, Mo, invo = orc.crt_consts(L)
+    for k in range(4):
+        assert int(Ml[k]) == Mk[k] and int(Mo[k]) == Mk[k]
+        for l in range(L):
+            assert int(Wl[k * 20 + l]) == W[l][k] and int(Wo[k][l]) == W[l][k]
+    assert sum(Mk) == M
+    md = ((float(Mk[3]) + float(Mk[2])) + float(Mk[1])) + float(Mk[0])
+    assert inv.value == invo == 1.0 / md
+
+
+def test_residue_helpers_match_exact_arithmetic(orc, bx):
+    """crt_resid on integer-valued doubles of every magnitude up to 2^115
+    (the split's |A'| < 2^beta), and smod of int32 products, against
+    Python's exact integers: the library's helpers (host build of the
+    kernel code) and the oracle's independent int64 version."""
+    Lb = bx.lib()
+    Lb.bxtest_crt_resid.argtypes = [C.c_double, C.c_int]
+    Lb.bxtest_crt_smod_i32.argtypes = [C.c_int, C.c_int]
+    rng = random.Random(11)
+    vals = [0.0, 1.0, -1.0, 127.0, -128.0, 2.0 ** 53, -(2.0 ** 53), 2.0 ** 53 - 1, 2.0 ** 115]
+    for _ in range(3000):
+        e = rng.randrange(0, 116)
+        vals.append(float(math.trunc(rng.uniform(-1, 1) * 2.0 ** e)))
+    for v in vals:
+        iv = int(v)
+        for m in MODS:
+            want = _smod(iv, m)
+            assert Lb.bxtest_crt_resid(v, m) == want, (v, m)
+            assert orc.crt_resid(v, m) == want, (v, m)
+    for _ in range(20000):
+        x = rng.randrange(-2 ** 31, 2 ** 31)
+        m = rng.choice(MODS)
+        assert Lb.bxtest_crt_smod_i32(x, m) == _smod(x, m), (x, m)
+    for m in MODS:  # quotient ties and range ends
+        for x in (m // 2, -(m // 2), (m + 1) // 2, 3 * m // 2, -3 * m // 2, 2 ** 31 - 1, -2 ** 31):
+            assert Lb.bxtest_crt_smod_i32(x, m) == _smod(x, m), (x, m)
+
+
+@pytest.mark.parametrize("L", [8, 12, 16, 20])
+def test_floating_crt_and_garner(orc, L):
+    """Every integer |P| < M/8 is recovered to 2u by the floating CRT, and
+    exactly by Garner's digits."""
+    ms = MODS[:L]
+    M = math.prod(ms)
+    consts = orc.crt_consts(L)
+    rng = random.Random(L)
+    for _ in range(1500):
+        P = rng.randrange(-2 ** rng.randrange(0, M.bit_length() - 3), 2 ** 60) \
+            if rng.random() < 0.2 else rng.randrange(-M // 8, M // 8) >> rng.randrange(0, 120)
+        r = [_smod(P, m) for m in ms]
+        S = orc.crt_float(r, consts)
+        assert abs(S - P) <= 2 * EPS / 2 * abs(P) + (0 if P else 0), (P, S)
+        v = orc.crt_digits(r)
+        acc, mult = 0, 1
+        for d, m in zip(v, ms):
+            acc += d * mult
+            mult *= m
+        assert acc == P
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+def test_oracle_exact_on_integer_inputs(orc, ta, tb):
+    """Integers whose row maxima reach 2^20 are kept exactly when beta >= 21
+    (A'_ik = a_ik 2^(beta - e_i) is then an integer), so the result is the
+    exact integer product."""
+    rng = np.random.default_rng(5)
+    M, N, K = 37, 29, 61
+    A = rng.integers(-(2 ** 20) + 1, 2 ** 20, (M, K) if ta == "N" else (K, M)).astype(np.float64)
+    B = rng.integers(-(2 ** 20) + 1, 2 ** 20, (K, N) if tb == "N" else (N, K)).astype(np.float64)
+    A, B = np.asfortranarray(A), np.asfortranarray(B)
+    opA = A if ta == "N" else A.T
+    opB = B if tb == "N" else B.T
+    exact = opA.astype(np.int64) @ opB.astype(np.int64)
+    for L in (8, 12, 20):
+        assert orc.crt_beta(K, L) >= 21
+        Cc = np.full((M, N), np.nan, order="F")
+        orc.dgemm_crt(ta, tb, M, N, K, 1.0, A, A.shape[0], B, B.shape[0], 0.0, Cc, M, L)
+        assert np.array_equal(Cc, exact.astype(np.float64)), L
+
+
+def _crt_bound(As, Bs, beta, alpha=1.0):
+    """|C - C_exact| for C = alpha op(A) op(B): truncation of both operands
+    (|a - A' 2^(e-beta)| < 2^(e-beta)) plus the reconstruction and scaling
+    roundings."""
+    ea = np.frexp(np.abs(As).max(axis=1))[1].astype(np.float64)
+    eb = np.frexp(np.abs(Bs).max(axis=0))[1].astype(np.float64)
+    K = As.shape[1]
+    t = (np.exp2(ea - beta)[:, None] * np.abs(Bs).sum(axis=0)[None, :]
+         + np.exp2(eb - beta)[None, :] * np.abs(As).sum(axis=1)[:, None]
+         + K * np.exp2(ea[:, None] + eb[None, :] - 2 * beta))
+    return abs(alpha) * (t * (1 + 1e-12) + 4 * EPS * (np.abs(As) @ np.abs(Bs)))
+
+
+def test_oracle_error_within_bound_and_falls_with_moduli(orc):
+    M, N, K = 48, 40, 300
+    A = orc.fill_dynrange(M, K, 3, 0, 2.0)
+    B = orc.fill_dynrange(K, N, 3, 1, 2.0)
+    hi, lo = orc.gemm_dd("N", "N", M, N, K, 1.0, A, M, B, K, 0.0, np.zeros((M, N), order="F"), M)
+    errs = []
+    for L in (8, 11, 14, 17, 20):
+        Cc = np.zeros((M, N), order="F")
+        orc.dgemm_crt("N", "N", M, N, K, 1.0, A, M, B, K, 0.0, Cc, M, L)
+        err = np.abs(Cc - hi)
+        assert np.all(err <= _crt_bound(A, B, orc.crt_beta(K, L))), L
+        errs.append(np.max(err / (np.abs(A) @ np.abs(B))))
+    assert all(b < a for a, b in zip(errs, errs[1:])), errs
+    assert errs[-1] < 4 * EPS
+
+
+def test_oracle_alpha_beta_and_zero_rows(orc):
+    """alpha/beta follow C = alpha*P + fma(beta, C); a zero row of A and a
+    zero column of B give exact zeros (times alpha) plus beta*C."""
+    M, N, K = 20, 18, 50
+    A = orc.fill_uniform(M, K, 9, 0)
+    B = orc.fill_uniform(K, N, 9, 1)
+    A[3, :] = 0.0
+    B[:, 5] = 0.0
+    C0 = orc.fill_uniform(M, N, 9, 2)
+    Cc = C0.copy(order="F")
+    orc.dgemm_crt("N", "N", M, N, K, -2.0, A, M, B, K, 0.5, Cc, M, 16)
+    assert np.array_equal(Cc[3, :], 0.5 * C0[3, :])
+    assert np.array_equal(Cc[:, 5], 0.5 * C0[:, 5])
+    ref = -2.0 * (A @ B) + 0.5 * C0
+    assert np.max(np.abs(Cc - ref)) < 1e-13
+
+
+# ------------------------------------------------------------------ GPU
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("L", [8, 14, 20])
+def test_gpu_crt_bit_exact_vs_oracle(gpu, orc, L):
+    bx = gpu
+    M, N, K = 203, 151, 333
+    for ta, tb in [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")]:
+        say("crt", L, ta, tb, M, N, K)
+        A = orc.fill_dynrange(M if ta == "N" else K, K if ta == "N" else M, 5, 0, 2.0)
+        B = orc.fill_dynrange(K if tb == "N" else N, N if tb == "N" else K, 5, 1, 2.0)
+        C0 = orc.fill_uniform(M, N, 5, 2)
+        ref = C0.copy(order="F")
+        orc.dgemm_crt(ta, tb, M, N, K, 1.25, A, A.shape[0], B, B.shape[0], -0.5, ref, M, L)
+        dA, dB, dC = (bx.DeviceArray.from_host(x) for x in (A, B, C0))
+        bx.dgemm_crt_device(ta, tb, M, N, K, 1.25, dA.ptr, A.shape[0], dB.ptr, B.shape[0],
+                            -0.5, dC.ptr, M, L)
+        bx.synchronize()
+        out = dC.download()
+        nbad = int(np.sum(out != ref))
+        assert nbad == 0, f"{nbad} of {M * N} entries differ (max {np.max(np.abs(out - ref))})"
+
+
+@pytest.mark.gpu
+def test_gpu_crt_exact_integers_and_beta_zero(gpu, orc):
+    bx = gpu
+    rng = np.random.default_rng(32)
+    M, N, K = 300, 260, 1000
+    A = np.asfortranarray(rng.integers(-(2 ** 20) + 1, 2 ** 20, (M, K)).astype(np.float64))
+    B = np.asfortranarray(rng.integers(-(2 ** 20) + 1, 2 ** 20, (K, N)).astype(np.float64))
+    exact = A.astype(np.int64) @ B.astype(np.int64)
+    dA, dB = bx.DeviceArray.from_host(A), bx.DeviceArray.from_host(B)
+    dC = bx.DeviceArray.from_host(np.full((M, N), np.nan, order="F"))
+    say("crt exact integers, beta=0 over NaN")
+    bx.dgemm_crt_device("N", "N", M, N, K, 1.0, dA.ptr, M, dB.ptr, K, 0.0, dC.ptr, M, 12)
+    bx.synchronize()
+    assert np.array_equal(dC.download(), exact.astype(np.float64))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "T")])
+def test_gpu_crt_big_tile_path_sampled(gpu, orc, ta, tb):
+    """M, N >= 4096 pads to 256 and runs the 256x256 INT8 kernel; ragged M,
+    N and K exercise the padding.  Sampled blocks (including the last
+    rows/columns) equal the oracle restatement bit for bit."""
+    bx = gpu
+    M, N, K = 4100, 4097, 777
+    A = orc.fill_dynrange(M if ta == "N" else K, K if ta == "N" else M, 8, 0, 1.0)
+    B = orc.fill_dynrange(K if tb == "N" else N, N if tb == "N" else K, 8, 1, 1.0)
+    C0 = orc.fill_uniform(M, N, 8, 2)
+    dA, dB, dC = (bx.DeviceArray.from_host(x) for x in (A, B, C0))
+    say("crt big tiles", ta, tb, M, N, K)
+    bx.dgemm_crt_device(ta, tb, M, N, K, 0.75, dA.ptr, A.shape[0], dB.ptr, B.shape[0], 2.0,
+                        dC.ptr, M, 16)
+    bx.synchronize()
+    out = dC.download()
+    opA = A if ta == "N" else A.T
+    opB = B if tb == "N" else B.T
+    for (r0, c0) in [(0, 0), (M - 48, N - 40), (1999, 3001)]:
+        As = np.asfortranarray(opA[r0:r0 + 48])
+        Bs = np.asfortranarray(opB[:, c0:c0 + 40])
+        ref = np.asfortranarray(C0[r0:r0 + 48, c0:c0 + 40])
+        orc.dgemm_crt("N", "N", 48, 40, K, 0.75, As, 48, Bs, K, 2.0, ref, 48, 16)
+        assert np.array_equal(out[r0:r0 + 48, c0:c0 + 40], ref), (r0, c0)
+
+
+@pytest.mark.gpu
+def test_gpu_crt_large_within_bound(gpu, orc):
+    """2048^3 with a wide dynamic range and the default moduli: sampled
+    blocks equal the oracle bit for bit and stay inside the scheme's error
+    bound against the double-double oracle."""
+    bx = gpu
+    n = 2048
+    A = orc.fill_dynrange(n, n, 7, 0, 2.0)
+    B = orc.fill_dynrange(n, n, 7, 1, 2.0)
+    dA, dB = bx.DeviceArray.from_host(A), bx.DeviceArray.from_host(B)
+    dC = bx.DeviceArray(n, n, np.float64)
+    say("crt 2048^3 default moduli")
+    bx.dgemm_crt_device("N", "N", n, n, n, 1.0, dA.ptr, n, dB.ptr, n, 0.0, dC.ptr, n, 0)
+    bx.synchronize()
+    out = dC.download()
+    L = next(L for L in range(2, 21) if orc.crt_beta(n, L) >= 56)
+    rng = np.random.default_rng(33)
+    for _ in range(2):
+        r0, c0 = (int(v) for v in rng.integers(0, n - 64, 2))
+        As = np.asfortranarray(A[r0:r0 + 64])
+        Bs = np.asfortranarray(B[:, c0:c0 + 64])
+        ref = np.zeros((64, 64), order="F")
+        orc.dgemm_crt("N", "N", 64, 64, n, 1.0, As, 64, Bs, n, 0.0, ref, 64, L)
+        blk = out[r0:r0 + 64, c0:c0 + 64]
+        assert np.array_equal(blk, ref)
+        hi, _ = orc.gemm_dd("N", "N", 64, 64, n, 1.0, As, 64, Bs, n, 0.0, ref * 0, 64)
+        assert np.all(np.abs(blk - hi) <= _crt_bound(As, Bs, orc.crt_beta(n, L)))
