@@ -168,6 +168,8 @@ def lib():
     L.blasx_dgemm_crt_device.argtypes = [i, i, i, i, i, d, _vp, i64, _vp, i64, d, _vp, i64, i,
                                          _vp]
     L.blasx_crt_bits.argtypes = [i, i]
+    L.blasx_zgemm_crt_device.argtypes = [i, i, i, i, i, _vp, _vp, i64, _vp, i64, _vp, _vp, i64,
+                                         i, _vp]
     L.blasx_gemm_grouped_device.argtypes = [C.POINTER(GemmDesc), i, _vp]
     L.blasx_server_create.restype = _vp
     L.blasx_server_create.argtypes = [C.c_size_t, i]
@@ -401,6 +403,16 @@ def dgemm_crt_device(ta, tb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, mo
                                       beta, dC, ldc, moduli, stream)
     if rc:
         raise RuntimeError(f"crt dgemm failed ({rc})")
+
+
+def zgemm_crt_device(ta, tb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, moduli=0,
+                     stream=None):
+    """Complex128 CRT split GEMM (3 INT8 products per modulus) on HBM pointers."""
+    al, be = _scalar(Z, alpha), _scalar(Z, beta)
+    rc = lib().blasx_zgemm_crt_device(TRANS[ta], TRANS[tb], M, N, K, C.cast(al, _vp), dA, lda,
+                                      dB, ldb, C.cast(be, _vp), dC, ldc, moduli, stream)
+    if rc:
+        raise RuntimeError(f"crt zgemm failed ({rc})")
 
 
 def crt_bits(K, moduli=0):

@@ -27,8 +27,17 @@ int bxh_oz_exponents(const double *X, int64_t ld, int n, int len, int contiguous
                      double *part, void *stream);
 int bxh_crt_split(const double *X, int64_t ld, int n, int len, int contiguous, const int *e,
                   int beta, int L, const int *moduli, int8_t *D, int np, int kp, void *stream);
-int bxh_crt_gemm(const void *segs_dev, int L, const int *moduli, int kp, int mp, int np,
+int bxh_crt_gemm(const void *segs_dev, int L, int per, const int *moduli, int kp, int mp, int np,
                  int8_t *R, void *stream);
+int bxh_oz_exponents_z(const double *Z, int64_t ld, int n, int len, int contiguous, int *e,
+                       double *part, void *stream);
+int bxh_zcrt_split(const double *Z, int64_t ld, int n, int len, int contiguous, int conj,
+                   const int *e, int beta, int L, const int *moduli, int8_t *D, int np, int kp,
+                   void *stream);
+int bxh_zcrt_recon(int m, int n, const int8_t *R, int64_t ldr, int64_t plane, const void *recon,
+                   const int *moduli, const int *ea, const int *eb, int beta_bits,
+                   const double *alpha, const double *beta, double *C, int64_t ldc,
+                   void *stream);
 int bxh_crt_recon(int m, int n, const int8_t *R, int64_t ldr, int64_t plane, const void *recon,
                   const int *ea, const int *eb, int beta_bits, double alpha, double beta,
                   double *C, int64_t ldc, void *stream);
@@ -185,7 +194,7 @@ int bx_dgemm_crt(int opa, int opb, int M, int N, int K, double alpha, const doub
     const size_t off_ea = off_r + rup(r8, 256), off_eb = off_ea + rup((size_t)M * 4, 256);
     const size_t off_part = off_eb + rup((size_t)N * 4, 256);
     const size_t total = off_part + (size_t)bxh_oz_exp_chunks() * (M > N ? M : N) * sizeof(double);
-    const size_t segs_bytes = (size_t)CRT_MAX_L * 2 * sizeof(void *);
+    const size_t segs_bytes = (size_t)3 * CRT_MAX_L * 2 * sizeof(void *);
 
     crt_recon_t rc_tab;
     bx_crt_consts(L, &rc_tab);
@@ -228,7 +237,7 @@ int bx_dgemm_crt(int opa, int opb, int M, int N, int K, double alpha, const doub
     }
     if (!rc) rc = bxh_memcpy_h2d_async(g_crt.segs_dev, g_crt.segs_host,
                                        (size_t)L * 2 * sizeof(void *), stream);
-    if (!rc) rc = bxh_crt_gemm(g_crt.segs_dev, L, CRT_MODS, kp, mp, np, R, stream);
+    if (!rc) rc = bxh_crt_gemm(g_crt.segs_dev, L, 1, CRT_MODS, kp, mp, np, R, stream);
     if (!rc)
         rc = bxh_crt_recon(M, N, R, mp, (int64_t)mp * np, &rc_tab, ea, eb, bb, alpha, beta, C,
                            ldc, stream);
@@ -239,6 +248,112 @@ int bx_dgemm_crt(int opa, int opb, int M, int N, int K, double alpha, const doub
         bx_stat_flops(2.0 * M * (double)N * K);
     }
     return rc;
+}
+
+/* complex: 3 products per modulus (Karatsuba, exact in residues), see
+ * hip/ozaki_kernels.hip; beta from 2K since |Re|, |Im| < 2K 2^(2 beta) */
+int bx_crt_default_moduli_z(int K)
+{
+    for (int L = 2; L <= CRT_MAX_L; ++L)
+        if (bx_crt_beta(2 * K, L) >= 56) return L;
+    return CRT_MAX_L;
+}
+
+int bx_zgemm_crt(int opa, int opb, int M, int N, int K, const double *alpha, const double *A,
+                 int64_t lda, const double *B, int64_t ldb, const double *beta, double *C,
+                 int64_t ldc, int L, void *stream)
+{
+    if (M == 0 || N == 0) return 0;
+    if (K == 0 || (alpha[0] == 0.0 && alpha[1] == 0.0))
+        return bxh_scale(BLASX_Z, M, N, beta, C, ldc, stream);
+    if (K > CRT_MAX_K) return BLASX_ERANGE;
+    const int bb = bx_crt_beta(2 * K, L);
+    if (bb < 1) return BLASX_EINVAL;
+    const int pad = (M >= 4096 && N >= 4096) ? 2 * bxh_oz_block_m() : bxh_oz_block_m();
+    const int mp = (int)rup(M, pad), np = (int)rup(N, pad);
+    const int kp = (int)rup(K, bxh_oz_block_k());
+    const int P = 3 * L;
+
+    const size_t a8 = (size_t)P * mp * kp, b8 = (size_t)P * np * kp, r8 = (size_t)P * mp * np;
+    const size_t off_b8 = rup(a8, 256), off_r = off_b8 + rup(b8, 256);
+    const size_t off_ea = off_r + rup(r8, 256), off_eb = off_ea + rup((size_t)M * 4, 256);
+    const size_t off_part = off_eb + rup((size_t)N * 4, 256);
+    const size_t total =
+        off_part + (size_t)2 * bxh_oz_exp_chunks() * (M > N ? M : N) * sizeof(double);
+    const size_t segs_bytes = (size_t)3 * CRT_MAX_L * 2 * sizeof(void *);
+
+    crt_recon_t rc_tab;
+    bx_crt_consts(L, &rc_tab);
+
+    pthread_mutex_lock(&g_crt_lock);
+    int rc = 0;
+    if (!g_crt.done) rc = bxh_event_create(&g_crt.done, 0);
+    if (!rc && !g_crt.segs_host) rc = bxh_host_malloc(&g_crt.segs_host, segs_bytes);
+    if (!rc && !g_crt.segs_dev) rc = bxh_malloc(&g_crt.segs_dev, segs_bytes);
+    if (!rc && total > g_crt.dev_cap) {
+        bxh_event_sync(g_crt.done);
+        bxh_free(g_crt.dev);
+        g_crt.dev = NULL;
+        g_crt.dev_cap = 0;
+        rc = bxh_malloc(&g_crt.dev, total);
+        if (!rc) g_crt.dev_cap = total;
+    }
+    if (rc) {
+        pthread_mutex_unlock(&g_crt_lock);
+        return rc;
+    }
+    bxh_event_sync(g_crt.done);
+
+    char *base = (char *)g_crt.dev;
+    int8_t *DA = (int8_t *)base, *DB = (int8_t *)(base + off_b8), *R = (int8_t *)(base + off_r);
+    int *ea = (int *)(base + off_ea), *eb = (int *)(base + off_eb);
+    double *part = (double *)(base + off_part);
+
+    const int a_contig = opa != BX_OP_N, b_contig = opb == BX_OP_N;
+    rc = bxh_oz_exponents_z(A, lda, M, K, a_contig, ea, part, stream);
+    if (!rc) rc = bxh_oz_exponents_z(B, ldb, N, K, b_contig, eb, part, stream);
+    if (!rc)
+        rc = bxh_zcrt_split(A, lda, M, K, a_contig, opa == BX_OP_C, ea, bb, L, CRT_MODS, DA, mp,
+                            kp, stream);
+    if (!rc)
+        rc = bxh_zcrt_split(B, ldb, N, K, b_contig, opb == BX_OP_C, eb, bb, L, CRT_MODS, DB, np,
+                            kp, stream);
+    const int8_t **tab = (const int8_t **)g_crt.segs_host;
+    for (int p = 0; p < P; ++p) {
+        tab[2 * p] = DA + (size_t)p * mp * kp;
+        tab[2 * p + 1] = DB + (size_t)p * np * kp;
+    }
+    if (!rc) rc = bxh_memcpy_h2d_async(g_crt.segs_dev, g_crt.segs_host,
+                                       (size_t)P * 2 * sizeof(void *), stream);
+    if (!rc) rc = bxh_crt_gemm(g_crt.segs_dev, L, 3, CRT_MODS, kp, mp, np, R, stream);
+    if (!rc)
+        rc = bxh_zcrt_recon(M, N, R, mp, (int64_t)mp * np, &rc_tab, CRT_MODS, ea, eb, bb, alpha,
+                            beta, C, ldc, stream);
+    if (!rc) rc = bxh_event_record(g_crt.done, stream);
+    pthread_mutex_unlock(&g_crt_lock);
+    if (!rc) {
+        BX_STAT_ADD(kernel_launches, 10);
+        bx_stat_flops(8.0 * M * (double)N * K);
+    }
+    return rc;
+}
+
+BX_EXPORT int blasx_zgemm_crt_device(int transa, int transb, int M, int N, int K,
+                                     const double *alpha, const double *dA, int64_t lda,
+                                     const double *dB, int64_t ldb, const double *beta,
+                                     double *dC, int64_t ldc, int moduli, void *stream)
+{
+    if (bx_ensure_init() != 0) return BLASX_ENOGPU;
+    const int opa = transa == 111 ? BX_OP_N : transa == 112 ? BX_OP_T : transa == 113 ? BX_OP_C : -1;
+    const int opb = transb == 111 ? BX_OP_N : transb == 112 ? BX_OP_T : transb == 113 ? BX_OP_C : -1;
+    if (opa < 0 || opb < 0 || M < 0 || N < 0 || K < 0) return BLASX_EINVAL;
+    int64_t nrowa = opa == BX_OP_N ? M : K, nrowb = opb == BX_OP_N ? K : N;
+    if (lda < (nrowa > 1 ? nrowa : 1) || ldb < (nrowb > 1 ? nrowb : 1) || ldc < (M > 1 ? M : 1))
+        return BLASX_EINVAL;
+    if (moduli == 0) moduli = bx_crt_default_moduli_z(K);
+    if (moduli < 2 || moduli > CRT_MAX_L) return BLASX_EINVAL;
+    return bx_zgemm_crt(opa, opb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, moduli,
+                        stream ? stream : g_bx.s_comp);
 }
 
 BX_EXPORT int blasx_dgemm_crt_device(int transa, int transb, int M, int N, int K, double alpha,

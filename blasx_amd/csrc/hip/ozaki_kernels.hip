@@ -73,13 +73,17 @@ __global__ void k_amax_strided(const double *__restrict__ X, int64_t ld, int n, 
             fmax(fmax(red[0][tx], red[1][tx]), fmax(red[2][tx], red[3][tx]));
 }
 
-__global__ void k_exp_parts(const double *__restrict__ part, int chunks, int n,
+// per = 2: rows 2i and 2i+1 of the scan (real and imaginary parts of a
+// complex row) give one exponent
+__global__ void k_exp_parts(const double *__restrict__ part, int chunks, int n, int per,
                             int *__restrict__ e)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const int64_t rows = (int64_t)n * per;
     double m = 0.0;
-    for (int c = 0; c < chunks; ++c) m = fmax(m, part[(int64_t)c * n + i]);
+    for (int c = 0; c < chunks; ++c)
+        for (int r = 0; r < per; ++r) m = fmax(m, part[c * rows + (int64_t)i * per + r]);
     e[i] = exp_of(m);
 }
 
@@ -625,6 +629,96 @@ __global__ __launch_bounds__(256) void k_crt_split_strided(const double *__restr
     }
 }
 
+// complex operands: planes 3l (real part), 3l+1 (imaginary part), 3l+2
+// (their sum) of modulus l, from one shared exponent per row; conj negates
+// the imaginary part.  Plane p: c = p % 3 of modulus p / 3.
+__device__ __forceinline__ int zcrt_plane_resid(double vr, double vi, int p, const CrtMods &md)
+{
+    const int l = p / 3, c = p - 3 * l, m = md.m[l];
+    const int r = c == 1 ? 0 : crt_resid(vr, m, md.rinv[l], md.rinvf[l], md.p2[l]);
+    const int q = c == 0 ? 0 : crt_resid(vi, m, md.rinv[l], md.rinvf[l], md.p2[l]);
+    return c == 2 ? smod_fix(r + q, m) : r + q;
+}
+
+// contiguous (mode 1): z(i,k) = Z[k + i*ld]; one thread per 16 k of one row
+__global__ __launch_bounds__(256) void k_zcrt_split_contig(const double *__restrict__ Z,
+                                                           int64_t ld, int n, int len, int conj,
+                                                           const int *__restrict__ e, int beta,
+                                                           CrtMods md, int8_t *__restrict__ D,
+                                                           int np, int kp)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int chunks = kp / 16;
+    if (t >= (int64_t)np * chunks) return;
+    const int i = (int)(t / chunks), k0 = (int)(t % chunks) * 16;
+    double vr[16], vi[16];
+    const int sh = i < n ? beta - e[i] : 0;
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+        const bool in = i < n && k0 + b < len;
+        const double *z = Z + 2 * (k0 + b + (int64_t)i * ld);
+        vr[b] = in ? trunc(ldexp(z[0], sh)) : 0.0;
+        vi[b] = in ? trunc(ldexp(conj ? -z[1] : z[1], sh)) : 0.0;
+    }
+    for (int p = 0; p < 3 * md.L; ++p) {
+        uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int b = 0; b < 16; ++b)
+            w[b >> 2] |= ((uint32_t)(uint8_t)(int8_t)zcrt_plane_resid(vr[b], vi[b], p, md))
+                         << (8 * (b & 3));
+        *(i32x4 *)(D + (int64_t)p * np * kp + (int64_t)i * kp + k0) =
+            i32x4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+    }
+}
+
+// strided (mode 0): z(i,k) = Z[i + k*ld]; 64 x 64 tiles through LDS, eight
+// planes at a time
+__global__ __launch_bounds__(256) void k_zcrt_split_strided(const double *__restrict__ Z,
+                                                            int64_t ld, int n, int len, int conj,
+                                                            const int *__restrict__ e, int beta,
+                                                            CrtMods md, int8_t *__restrict__ D,
+                                                            int np, int kp)
+{
+    __shared__ int8_t tile[8][SP_T][SP_T + 4];
+    const int i0 = blockIdx.x * SP_T, k0 = blockIdx.y * SP_T;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6; // 64 x 4
+    double vr[SP_T / 4], vi[SP_T / 4];
+    {
+        const int i = i0 + tx;
+        const int sh = i < n ? beta - e[i] : 0;
+#pragma unroll
+        for (int q = 0; q < SP_T / 4; ++q) {
+            const int k = k0 + ty + 4 * q;
+            const bool in = i < n && k < len;
+            const double *z = Z + 2 * (i + (int64_t)k * ld);
+            vr[q] = in ? trunc(ldexp(z[0], sh)) : 0.0;
+            vi[q] = in ? trunc(ldexp(conj ? -z[1] : z[1], sh)) : 0.0;
+        }
+    }
+    const int row = threadIdx.x >> 2, kc = (threadIdx.x & 3) * 16;
+    const int planes = 3 * md.L;
+    for (int p0 = 0; p0 < planes; p0 += 8) {
+        const int cnt = planes - p0 < 8 ? planes - p0 : 8;
+        for (int p = 0; p < cnt; ++p) {
+#pragma unroll
+            for (int q = 0; q < SP_T / 4; ++q)
+                tile[p][tx][ty + 4 * q] = (int8_t)zcrt_plane_resid(vr[q], vi[q], p0 + p, md);
+        }
+        __syncthreads();
+        const int i = i0 + row;
+        if (i < np)
+            for (int p = 0; p < cnt; ++p) {
+                int8_t *dst = D + (int64_t)(p0 + p) * np * kp + (int64_t)i * kp + k0 + kc;
+                uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+                for (int b = 0; b < 16; ++b)
+                    w[b >> 2] |= ((uint32_t)(uint8_t)tile[p][row][kc + b]) << (8 * (b & 3));
+                *(i32x4 *)dst = i32x4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+            }
+        __syncthreads();
+    }
+}
+
 // residues of one tile sum -> int8 plane (column-major, ld ldr); every lane
 // owns 4 consecutive rows of one column: one 32-bit store each
 template <int WT, int NB>
@@ -649,29 +743,31 @@ __device__ __forceinline__ void store_resid(const i32x4 (&acc)[NB][NB], int m0, 
 
 // one INT8 GEMM per modulus (blockIdx.y) over padded operands; R planes are
 // mp x np (ld mp), so no bounds checks
+// product p = blockIdx.y uses modulus p / per (per = 3 for complex planes)
 __global__ __launch_bounds__(OZ_NT, 2) void k_crt_gemm(const OzSeg *__restrict__ segs, int kp,
-                                                       int np, CrtMods md, int8_t *__restrict__ R,
-                                                       int64_t ldr, int64_t plane)
+                                                       int np, int per, CrtMods md,
+                                                       int8_t *__restrict__ R, int64_t ldr,
+                                                       int64_t plane)
 {
     int m0, n0;
     block_origin<OZ_BM>(np, m0, n0);
-    const int l = blockIdx.y;
+    const int p = blockIdx.y, l = p / per;
     i32x4 acc[4][4];
-    tile128(segs + l, 1, kp, m0, n0, acc);
-    store_resid<64, 4>(acc, m0, n0, md.m[l], md.rinv[l], R + l * plane, ldr);
+    tile128(segs + p, 1, kp, m0, n0, acc);
+    store_resid<64, 4>(acc, m0, n0, md.m[l], md.rinv[l], R + p * plane, ldr);
 }
 
 __global__ __launch_bounds__(OZ_NT, 1) void k_crt_gemm_big(const OzSeg *__restrict__ segs, int kp,
-                                                           int np, CrtMods md,
+                                                           int np, int per, CrtMods md,
                                                            int8_t *__restrict__ R, int64_t ldr,
                                                            int64_t plane)
 {
     int m0, n0;
     block_origin<OZ_BIG>(np, m0, n0);
-    const int l = blockIdx.y;
+    const int p = blockIdx.y, l = p / per;
     i32x4 acc[8][8];
-    tile256(segs + l, 1, kp, m0, n0, acc);
-    store_resid<128, 8>(acc, m0, n0, md.m[l], md.rinv[l], R + l * plane, ldr);
+    tile256(segs + p, 1, kp, m0, n0, acc);
+    store_resid<128, 8>(acc, m0, n0, md.m[l], md.rinv[l], R + p * plane, ldr);
 }
 
 struct CrtRecon {
@@ -708,6 +804,56 @@ __global__ void k_crt_recon(int m, int n, const int8_t *__restrict__ R, int64_t 
     }
 }
 
+// complex: per modulus Re = smod(R1 - R2), Im = smod(R3 - R1 - R2) from the
+// three product planes 3l, 3l+1, 3l+2; CRT of both; C = alpha x + beta C
+// with the complex FMA order of k_zcombine
+__global__ void k_zcrt_recon(int m, int n, const int8_t *__restrict__ R, int64_t ldr,
+                             int64_t plane, CrtRecon c, CrtMods md, const int *__restrict__ ea,
+                             const int *__restrict__ eb, int two_beta, double ar, double ai,
+                             double br, double bi, int beta_zero, double *__restrict__ C,
+                             int64_t ldc)
+{
+    const int64_t total = (int64_t)m * n;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % m), j = (int)(t / m);
+        const int8_t *r = R + i + (int64_t)j * ldr;
+        double q[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+        for (int l = 0; l < c.L; ++l) {
+            const int mod = md.m[l];
+            const int r1 = r[(3 * l) * plane], r2 = r[(3 * l + 1) * plane],
+                      r3 = r[(3 * l + 2) * plane];
+            const double vre = (double)smod_small(r1 - r2, mod, md.rinvf[l]);
+            const double vim = (double)smod_small(r3 - r1 - r2, mod, md.rinvf[l]);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                q[0][k] = fma(c.W[k][l], vre, q[0][k]);
+                q[1][k] = fma(c.W[k][l], vim, q[1][k]);
+            }
+        }
+        double x[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const double kq = rint((q[h][0] + q[h][1]) * c.invM);
+            const double S = ((fma(-kq, c.Mk[0], q[h][0]) + fma(-kq, c.Mk[1], q[h][1])) +
+                              fma(-kq, c.Mk[2], q[h][2])) +
+                             fma(-kq, c.Mk[3], q[h][3]);
+            x[h] = ldexp(S, ea[i] + eb[j] - two_beta);
+        }
+        const double tr = fma(ar, x[0], -(ai * x[1]));
+        const double ti = fma(ar, x[1], ai * x[0]);
+        double *cp = C + 2 * (i + (int64_t)j * ldc);
+        if (beta_zero) {
+            cp[0] = tr;
+            cp[1] = ti;
+        } else {
+            const double cr = cp[0], cim = cp[1];
+            cp[0] = fma(br, cr, fma(-bi, cim, tr));
+            cp[1] = fma(br, cim, fma(bi, cr, ti));
+        }
+    }
+}
+
 thread_local char g_oz_err[256];
 int oz_fail(const char *what, hipError_t e)
 {
@@ -741,8 +887,29 @@ int bxh_oz_exponents(const double *X, int64_t ld, int n, int len, int contiguous
         hipLaunchKernelGGL(k_amax_strided, dim3((n + 63) / 64, chunks), dim3(256), 0, st, X, ld,
                            n, len, chunk, part);
         hipLaunchKernelGGL(k_exp_parts, dim3((n + 255) / 256), dim3(256), 0, st, part, chunks, n,
-                           e);
+                           1, e);
     }
+    hipError_t r = hipGetLastError();
+    return r == hipSuccess ? 0 : oz_fail("exponent launch", r);
+}
+
+// complex rows (interleaved re/im, ld in complex elements): one exponent of
+// max(|re|, |im|) per row; part: EXP_CHUNKS * 2n doubles (contiguous == 0)
+int bxh_oz_exponents_z(const double *Z, int64_t ld, int n, int len, int contiguous, int *e,
+                       double *part, void *stream)
+{
+    if (n <= 0) return 0;
+    if (contiguous) // row i: 2*len consecutive doubles
+        return bxh_oz_exponents(Z, 2 * ld, n, 2 * len, 1, e, part, stream);
+    hipStream_t st = (hipStream_t)stream;
+    int chunk = (len + EXP_CHUNKS - 1) / EXP_CHUNKS;
+    chunk = chunk < 4 ? 4 : (chunk + 3) / 4 * 4;
+    const int chunks = len > 0 ? (len + chunk - 1) / chunk : 1;
+    // 2n real rows (re of row i at 2i, im at 2i+1), column stride 2*ld doubles
+    hipLaunchKernelGGL(k_amax_strided, dim3((2 * n + 63) / 64, chunks), dim3(256), 0, st, Z,
+                       2 * ld, 2 * n, len, chunk, part);
+    hipLaunchKernelGGL(k_exp_parts, dim3((n + 255) / 256), dim3(256), 0, st, part, chunks, n, 2,
+                       e);
     hipError_t r = hipGetLastError();
     return r == hipSuccess ? 0 : oz_fail("exponent launch", r);
 }
@@ -841,20 +1008,25 @@ int bxh_crt_split(const double *X, int64_t ld, int n, int len, int contiguous, c
     return r == hipSuccess ? 0 : oz_fail("crt split launch", r);
 }
 
-// segs_dev: L (A plane, B plane) pointers; R: L planes of mp x np int8 (ld mp)
-int bxh_crt_gemm(const void *segs_dev, int L, const int *moduli, int kp, int mp, int np,
+// segs_dev: per*L (A plane, B plane) pointers; R: per*L planes of mp x np
+// int8 (ld mp); product p uses modulus p / per
+int bxh_crt_gemm(const void *segs_dev, int L, int per, const int *moduli, int kp, int mp, int np,
                  int8_t *R, void *stream)
 {
-    if (L < 1 || L > 20 || kp <= 0 || mp % OZ_BM || np % OZ_BN || kp % OZ_BK) return BLASX_EINVAL;
+    if (L < 1 || L > 20 || per < 1 || per > 3 || kp <= 0 || mp % OZ_BM || np % OZ_BN ||
+        kp % OZ_BK)
+        return BLASX_EINVAL;
     const CrtMods md = crt_mods(L, moduli);
     const int64_t plane = (int64_t)mp * np;
+    const int nprod = per * L;
     const int big_tiles = (mp % OZ_BIG == 0 && np % OZ_BIG == 0) ? (mp / OZ_BIG) * (np / OZ_BIG) : 0;
-    if (big_tiles * L >= 256) {
-        hipLaunchKernelGGL(k_crt_gemm_big, dim3(big_tiles, L), dim3(OZ_NT), 0, (hipStream_t)stream,
-                           (const OzSeg *)segs_dev, kp, np, md, R, (int64_t)mp, plane);
+    if (big_tiles * nprod >= 256) {
+        hipLaunchKernelGGL(k_crt_gemm_big, dim3(big_tiles, nprod), dim3(OZ_NT), 0,
+                           (hipStream_t)stream, (const OzSeg *)segs_dev, kp, np, per, md, R,
+                           (int64_t)mp, plane);
     } else {
-        hipLaunchKernelGGL(k_crt_gemm, dim3((mp / OZ_BM) * (np / OZ_BN), L), dim3(OZ_NT), 0,
-                           (hipStream_t)stream, (const OzSeg *)segs_dev, kp, np, md, R,
+        hipLaunchKernelGGL(k_crt_gemm, dim3((mp / OZ_BM) * (np / OZ_BN), nprod), dim3(OZ_NT), 0,
+                           (hipStream_t)stream, (const OzSeg *)segs_dev, kp, np, per, md, R,
                            (int64_t)mp, plane);
     }
     hipError_t r = hipGetLastError();
@@ -875,6 +1047,46 @@ int bxh_crt_recon(int m, int n, const int8_t *R, int64_t ldr, int64_t plane, con
                        plane, c, ea, eb, 2 * beta_bits, alpha, beta, beta == 0.0 ? 1 : 0, C, ldc);
     hipError_t r = hipGetLastError();
     return r == hipSuccess ? 0 : oz_fail("crt recon launch", r);
+}
+
+// complex operand rows: 3L planes (re, im, re + im per modulus)
+int bxh_zcrt_split(const double *Z, int64_t ld, int n, int len, int contiguous, int conj,
+                   const int *e, int beta, int L, const int *moduli, int8_t *D, int np, int kp,
+                   void *stream)
+{
+    if (L < 1 || L > 20 || np % 64 || kp % 64) return BLASX_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    const CrtMods md = crt_mods(L, moduli);
+    if (contiguous) {
+        const int64_t total = (int64_t)np * (kp / 16);
+        hipLaunchKernelGGL(k_zcrt_split_contig, dim3((unsigned)((total + 255) / 256)), dim3(256),
+                           0, st, Z, ld, n, len, conj, e, beta, md, D, np, kp);
+    } else {
+        hipLaunchKernelGGL(k_zcrt_split_strided, dim3(np / SP_T, kp / SP_T), dim3(256), 0, st, Z,
+                           ld, n, len, conj, e, beta, md, D, np, kp);
+    }
+    hipError_t r = hipGetLastError();
+    return r == hipSuccess ? 0 : oz_fail("zcrt split launch", r);
+}
+
+int bxh_zcrt_recon(int m, int n, const int8_t *R, int64_t ldr, int64_t plane, const void *recon,
+                   const int *moduli, const int *ea, const int *eb, int beta_bits,
+                   const double *alpha, const double *beta, double *C, int64_t ldc,
+                   void *stream)
+{
+    if (m <= 0 || n <= 0) return 0;
+    CrtRecon c;
+    memcpy(&c, recon, sizeof c);
+    const CrtMods md = crt_mods(c.L, moduli);
+    const int64_t total = (int64_t)m * n;
+    int grid = (int)((total + 255) / 256);
+    if (grid > 262144) grid = 262144;
+    const int bz = beta[0] == 0.0 && beta[1] == 0.0;
+    hipLaunchKernelGGL(k_zcrt_recon, dim3(grid), dim3(256), 0, (hipStream_t)stream, m, n, R, ldr,
+                       plane, c, md, ea, eb, 2 * beta_bits, alpha[0], alpha[1], beta[0], beta[1],
+                       bz, C, ldc);
+    hipError_t r = hipGetLastError();
+    return r == hipSuccess ? 0 : oz_fail("zcrt recon launch", r);
 }
 
 // host-side check of the residue helpers the kernels use (same code, CPU)

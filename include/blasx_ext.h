@@ -152,6 +152,15 @@ int blasx_dgemm_crt_device(int transa, int transb, int M, int N, int K, double a
                            const double *dA, int64_t lda, const double *dB, int64_t ldb,
                            double beta, double *dC, int64_t ldc, int moduli, void *stream);
 int blasx_crt_bits(int K, int moduli);
+/* complex128 version: one exponent per op(A) row / op(B) column for both
+ * parts, three exact INT8 products per modulus on the residues of Re, Im
+ * and Re + Im (Karatsuba, exact in modular arithmetic), so 3 * moduli INT8
+ * GEMMs; bits = blasx_crt_bits(2K, moduli); alpha/beta are (re, im);
+ * trans 113 conjugates */
+int blasx_zgemm_crt_device(int transa, int transb, int M, int N, int K, const double *alpha,
+                           const double *dA, int64_t lda, const double *dB, int64_t ldb,
+                           const double *beta, double *dC, int64_t ldc, int moduli,
+                           void *stream);
 
 /* ---- grouped GEMM -----------------------------------------------------------
  * No counterpart in the reference (which issues one vendor call per tile

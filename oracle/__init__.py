@@ -96,6 +96,8 @@ def lib():
         L.orc_crt_consts.argtypes = [C.c_int, _dp, _dp, _dp]
         L.orc_crt_float.restype = C.c_double
         L.orc_crt_float.argtypes = [C.POINTER(C.c_int), C.c_int, _dp, _dp, C.c_double]
+        L.orc_zgemm_crt.argtypes = [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, _dp, _dp,
+                                    C.c_int, _dp, C.c_int, _dp, _dp, C.c_int, C.c_int]
         L.orc_dgemm_crt.argtypes = [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, C.c_double,
                                     _dp, C.c_int, _dp, C.c_int, C.c_double, _dp, C.c_int,
                                     C.c_int]
@@ -307,6 +309,16 @@ def dgemm_crt(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C_, ldc, L):
     """CPU restatement of the CRT split DGEMM; C_ updated in place."""
     lib().orc_dgemm_crt(ta.encode(), tb.encode(), M, N, K, alpha, _ptr(A, _dp), lda,
                         _ptr(B, _dp), ldb, beta, _ptr(C_, _dp), ldc, L)
+    return C_
+
+
+def zgemm_crt(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C_, ldc, L):
+    """CPU restatement of the complex CRT split GEMM; C_ updated in place."""
+    al = np.array([complex(alpha).real, complex(alpha).imag])
+    be = np.array([complex(beta).real, complex(beta).imag])
+    lib().orc_zgemm_crt(ta.encode(), tb.encode(), M, N, K, _ptr(al, _dp),
+                        A.ctypes.data_as(_dp), lda, B.ctypes.data_as(_dp), ldb, _ptr(be, _dp),
+                        C_.ctypes.data_as(_dp), ldc, L)
     return C_
 
 

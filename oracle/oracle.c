@@ -819,3 +819,92 @@ void orc_dgemm_crt(char ta, char tb, int M, int N, int K, double alpha, const do
     free(RA);
     free(RB);
 }
+
+/* complex CRT split restated: op(A) rows / op(B) columns share one exponent
+ * for both parts (of max(|re|, |im|)), beta = orc_crt_beta(2K, L) since
+ * |Re P|, |Im P| < 2K 2^(2 beta); per modulus three exact products on the
+ * residue planes (Karatsuba, exact in modular arithmetic):
+ *   R1 = ar br, R2 = ai bi, R3 = (ar + ai)(br + bi)   (each smod m),
+ *   Re = smod(R1 - R2), Im = smod(R3 - R1 - R2),
+ * floating CRT of Re and Im, scaling by 2^(e_i + f_j - 2 beta), then
+ * C = alpha x + beta C with the library's complex FMA order. */
+void orc_zgemm_crt(char ta, char tb, int M, int N, int K, const double *alpha, const double *A,
+                   int lda, const double *B, int ldb, const double *beta, double *C, int ldc,
+                   int L)
+{
+    const int bb = orc_crt_beta(2 * K, L);
+    double Wk[4 * 20], Mk[4], invM;
+    orc_crt_consts(L, Wk, Mk, &invM);
+    const size_t pa = (size_t)M * (K > 0 ? K : 1), pb = (size_t)N * (K > 0 ? K : 1);
+    int8_t *RA = (int8_t *)calloc((size_t)3 * L * pa, 1);
+    int8_t *RB = (int8_t *)calloc((size_t)3 * L * pb, 1);
+    int *ea = (int *)malloc(sizeof(int) * (M > 0 ? M : 1));
+    int *eb = (int *)malloc(sizeof(int) * (N > 0 ? N : 1));
+    for (int side = 0; side < 2; ++side) {
+        const int rows = side == 0 ? M : N;
+        const char t = side == 0 ? ta : tb;
+        const double *X = side == 0 ? A : B;
+        const int ld = side == 0 ? lda : ldb;
+        int8_t *R = side == 0 ? RA : RB;
+        int *e = side == 0 ? ea : eb;
+        const size_t plane = side == 0 ? pa : pb;
+        for (int i = 0; i < rows; ++i) {
+            double mx = 0;
+            for (int p = 0; p < K; ++p) {
+                const int64_t o = side == 0 ? OPIDX(t, i, p, ld) : OPIDX(t, p, i, ld);
+                mx = fmax(mx, fmax(fabs(X[2 * o]), fabs(X[2 * o + 1])));
+            }
+            e[i] = orc_exp_of(mx);
+            for (int p = 0; p < K; ++p) {
+                const int64_t o = side == 0 ? OPIDX(t, i, p, ld) : OPIDX(t, p, i, ld);
+                const double re = X[2 * o], im = t == 'C' ? -X[2 * o + 1] : X[2 * o + 1];
+                const double vr = trunc(ldexp(re, bb - e[i])), vi = trunc(ldexp(im, bb - e[i]));
+                for (int l = 0; l < L; ++l) {
+                    const int m = ORC_CRT_MODS[l];
+                    const int r = orc_crt_resid(vr, m), q = orc_crt_resid(vi, m);
+                    R[(size_t)(3 * l) * plane + (size_t)i * K + p] = (int8_t)r;
+                    R[(size_t)(3 * l + 1) * plane + (size_t)i * K + p] = (int8_t)q;
+                    R[(size_t)(3 * l + 2) * plane + (size_t)i * K + p] = (int8_t)orc_smod(r + q, m);
+                }
+            }
+        }
+    }
+    const int bz = beta[0] == 0.0 && beta[1] == 0.0;
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < N; ++j) {
+        int rre[20], rim[20];
+        for (int i = 0; i < M; ++i) {
+            for (int l = 0; l < L; ++l) {
+                const int m = ORC_CRT_MODS[l];
+                int64_t P[3];
+                for (int c = 0; c < 3; ++c) {
+                    const int8_t *ra = RA + (size_t)(3 * l + c) * pa + (size_t)i * K;
+                    const int8_t *rb = RB + (size_t)(3 * l + c) * pb + (size_t)j * K;
+                    int64_t acc = 0;
+                    for (int p = 0; p < K; ++p) acc += (int64_t)ra[p] * rb[p];
+                    P[c] = orc_smod(acc, m);
+                }
+                rre[l] = (int)orc_smod(P[0] - P[1], m);
+                rim[l] = (int)orc_smod(P[2] - P[0] - P[1], m);
+            }
+            const int sh = ea[i] + eb[j] - 2 * bb;
+            const double xr = ldexp(orc_crt_float(rre, L, Wk, Mk, invM), sh);
+            const double xi = ldexp(orc_crt_float(rim, L, Wk, Mk, invM), sh);
+            const double tr = fma(alpha[0], xr, -(alpha[1] * xi));
+            const double ti = fma(alpha[0], xi, alpha[1] * xr);
+            double *c = &C[2 * (i + (int64_t)j * ldc)];
+            if (bz) {
+                c[0] = tr;
+                c[1] = ti;
+            } else {
+                const double cr = c[0], ci = c[1];
+                c[0] = fma(beta[0], cr, fma(-beta[1], ci, tr));
+                c[1] = fma(beta[0], ci, fma(beta[1], cr, ti));
+            }
+        }
+    }
+    free(RA);
+    free(RB);
+    free(ea);
+    free(eb);
+}

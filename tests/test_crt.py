@@ -295,3 +295,87 @@ def test_gpu_crt_large_within_bound(gpu, orc):
         assert np.array_equal(blk, ref)
         hi, _ = orc.gemm_dd("N", "N", 64, 64, n, 1.0, As, 64, Bs, n, 0.0, ref * 0, 64)
         assert np.all(np.abs(blk - hi) <= _crt_bound(As, Bs, orc.crt_beta(n, L)))
+
+
+# ------------------------------------------------------------------ complex
+
+def _cplx_int(rng, shape, bits):
+    return np.asfortranarray(rng.integers(-(2 ** bits) + 1, 2 ** bits, shape)
+                             + 1j * rng.integers(-(2 ** bits) + 1, 2 ** bits, shape))
+
+
+def _zop(X, t):
+    return X if t == "N" else (X.T if t == "T" else X.conj().T)
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("C", "N"), ("T", "C"), ("N", "T")])
+def test_oracle_zcrt_exact_on_integers(orc, ta, tb):
+    """Gaussian integers whose row maxima reach 2^18 are kept exactly
+    (beta(2K, L) >= 19), and the three-product Karatsuba form is exact in
+    residues, so the result is the exact complex product."""
+    rng = np.random.default_rng(35)
+    M, N, K = 23, 19, 41
+    A = _cplx_int(rng, (M, K) if ta == "N" else (K, M), 18)
+    B = _cplx_int(rng, (K, N) if tb == "N" else (N, K), 18)
+    C0 = _cplx_int(rng, (M, N), 18)
+    exact = (2 - 1j) * (_zop(A, ta) @ _zop(B, tb)) + (1 + 1j) * C0
+    for L in (10, 16):
+        assert orc.crt_beta(2 * K, L) >= 19
+        out = C0.copy(order="F")
+        orc.zgemm_crt(ta, tb, M, N, K, 2 - 1j, A, A.shape[0], B, B.shape[0], 1 + 1j, out, M, L)
+        assert np.array_equal(out, exact), L
+
+
+def test_oracle_zcrt_close_to_dd(orc):
+    M, N, K = 20, 16, 200
+    A = orc.fill_uniform(M, K, 6, 0, dtype=np.complex128)
+    B = orc.fill_uniform(K, N, 6, 1, dtype=np.complex128)
+    C0 = np.zeros((M, N), dtype=np.complex128, order="F")
+    hi, _ = orc.gemm_dd("N", "N", M, N, K, 1.0, A, M, B, K, 0.0, C0, M)
+    scale = np.abs(A) @ np.abs(B)
+    errs = []
+    for L in (10, 14, 18):
+        out = C0.copy(order="F")
+        orc.zgemm_crt("N", "N", M, N, K, 1.0, A, M, B, K, 0.0, out, M, L)
+        errs.append(np.max(np.abs(out - hi) / scale))
+    assert errs[0] > errs[1] > errs[2]
+    assert errs[2] <= 4 * EPS
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("C", "T"), ("T", "C")])
+def test_gpu_zcrt_bit_exact_vs_oracle(gpu, orc, ta, tb):
+    bx = gpu
+    M, N, K = 150, 131, 270
+    A = orc.fill_uniform(M if ta == "N" else K, K if ta == "N" else M, 9, 0, dtype=np.complex128)
+    B = orc.fill_uniform(K if tb == "N" else N, N if tb == "N" else K, 9, 1, dtype=np.complex128)
+    C0 = orc.fill_uniform(M, N, 9, 2, dtype=np.complex128)
+    ref = C0.copy(order="F")
+    orc.zgemm_crt(ta, tb, M, N, K, 0.5 + 1j, A, A.shape[0], B, B.shape[0], -1 + 0.25j, ref, M, 15)
+    dA, dB, dC = (bx.DeviceArray.from_host(x) for x in (A, B, C0))
+    say("zcrt", ta, tb, M, N, K)
+    bx.zgemm_crt_device(ta, tb, M, N, K, 0.5 + 1j, dA.ptr, A.shape[0], dB.ptr, B.shape[0],
+                        -1 + 0.25j, dC.ptr, M, 15)
+    bx.synchronize()
+    assert np.array_equal(dC.download(), ref)
+
+
+@pytest.mark.gpu
+def test_gpu_zcrt_big_tile_path_sampled(gpu, orc):
+    bx = gpu
+    M, N, K = 4097, 4100, 300
+    A = orc.fill_uniform(M, K, 10, 0, dtype=np.complex128)
+    B = orc.fill_uniform(N, K, 10, 1, dtype=np.complex128)  # op(B) = B^H
+    dA, dB = bx.DeviceArray.from_host(A), bx.DeviceArray.from_host(B)
+    dC = bx.DeviceArray(M, N, np.complex128)
+    say("zcrt big tiles N C", M, N, K)
+    bx.zgemm_crt_device("N", "C", M, N, K, 1.0, dA.ptr, M, dB.ptr, N, 0.0, dC.ptr, M, 0)
+    bx.synchronize()
+    out = dC.download()
+    L = next(L for L in range(2, 21) if orc.crt_beta(2 * K, L) >= 56)
+    for (r0, c0) in [(0, 0), (M - 40, N - 36), (2048, 1111)]:
+        As = np.asfortranarray(A[r0:r0 + 40])
+        Bs = np.asfortranarray(B[c0:c0 + 36])
+        ref = np.zeros((40, 36), dtype=np.complex128, order="F")
+        orc.zgemm_crt("N", "C", 40, 36, K, 1.0, As, 40, Bs, 36, 0.0, ref, 40, L)
+        assert np.array_equal(out[r0:r0 + 40, c0:c0 + 36], ref), (r0, c0)
