@@ -324,6 +324,91 @@ def split_precision(bx, n=16384, phis=(0.5, 2.0, 4.0), slices=(6, 7, 8),
     return res
 
 
+def zsplit_precision(bx, n=8192, phi=0.5, slices=(6, 7), blocks=2):
+    """Complex128: plain ZGEMM (four FP64 MFMAs per complex MAC) vs the digit
+    split (4 real split products) and the CRT split (3 INT8 products per
+    modulus); effective TFLOP/s = 8n^3 / time, errors on sampled 128x128
+    blocks against double-double references of the real and imaginary parts
+    (Re C = [Ar, -Ai] [Br; Bi], Im C = [Ar, Ai] [Bi; Br], one rounding each)."""
+    import numpy as np
+
+    flops = 8.0 * n ** 3
+    A = np.empty((n, n), dtype=np.complex128, order="F")
+    B = np.empty((n, n), dtype=np.complex128, order="F")
+    plane = np.empty((n, n), order="F")
+    for X, ids in ((A, (0, 1)), (B, (2, 3))):
+        bx.fill_dynrange_host(plane, 11, ids[0], phi)
+        X.real = plane
+        bx.fill_dynrange_host(plane, 11, ids[1], phi)
+        X.imag = plane
+    del plane
+    dA, dB = bx.DeviceArray.from_host(A), bx.DeviceArray.from_host(B)
+    dC = bx.DeviceArray(n, n, np.complex128)
+    rng = np.random.default_rng(2025)
+    corners = [tuple(int(v) for v in rng.integers(0, n - 128, 2)) for _ in range(blocks)]
+    refs, scales = [], []
+    for (r0, c0) in corners:
+        As, Bs = A[r0:r0 + 128], B[:, c0:c0 + 128]
+        Ar, Ai = np.asfortranarray(As.real), np.asfortranarray(As.imag)
+        Br, Bi = np.asfortranarray(Bs.real), np.asfortranarray(Bs.imag)
+        re = bx.dd_gemm_block("N", "N", 1.0, np.asfortranarray(np.hstack([Ar, -Ai])),
+                              np.asfortranarray(np.vstack([Br, Bi])))
+        im = bx.dd_gemm_block("N", "N", 1.0, np.asfortranarray(np.hstack([Ar, Ai])),
+                              np.asfortranarray(np.vstack([Bi, Br])))
+        refs.append(re + 1j * im)
+        scales.append(np.abs(As) @ np.abs(Bs))
+    del A, B
+    ev0, ev1 = bx.Event(), bx.Event()
+
+    def timed(fn):
+        fn()
+        bx.synchronize()
+        ev0.record()
+        fn()
+        ev1.record()
+        ev1.synchronize()
+        return ev0.elapsed_ms(ev1) * 1e-3
+
+    def errors():
+        out = []
+        for (r0, c0), ref, sc in zip(corners, refs, scales):
+            blk = np.empty((128, 128), dtype=np.complex128, order="F")
+            for c in range(128):
+                src = dC.ptr + (r0 + (c0 + c) * n) * 16
+                bx.lib().blasx_memcpy_d2h(blk[:, c].ctypes.data, src, 128 * 16)
+            out.append(_errors(blk, ref, sc))
+        return {k: max(e[k] for e in out) for k in out[0]}
+
+    res = {"n": n, "phi": phi, "blocks": f"{blocks} x 128x128, corners {corners}"}
+    t = timed(lambda: bx.device_gemm("N", "N", n, n, n, 1.0, dA.ptr, n, dB.ptr, n, 0.0, dC.ptr,
+                                     n, np.complex128))
+    res["plain"] = {"tflops": round(flops / t / 1e12, 3), **errors()}
+    log(f"zsplit: plain {res['plain']}")
+    for s in slices:
+        t = timed(lambda: bx.zgemm_split_device("N", "N", n, n, n, 1.0, dA.ptr, n, dB.ptr, n,
+                                                0.0, dC.ptr, n, s))
+        res[f"s{s}"] = {"tflops_effective": round(flops / t / 1e12, 3),
+                        "int8_products": 4 * s * (s + 1) // 2, **errors()}
+        log(f"zsplit: s={s} {res[f's{s}']}")
+    L0 = next(L for L in range(2, 21) if bx.crt_bits(2 * n, L) >= 56)
+    for L in (L0 - 2, L0, L0 + 2):
+        t = timed(lambda: bx.zgemm_crt_device("N", "N", n, n, n, 1.0, dA.ptr, n, dB.ptr, n, 0.0,
+                                              dC.ptr, n, L))
+        res[f"crt{L}"] = {"tflops_effective": round(flops / t / 1e12, 3), "int8_products": 3 * L,
+                          "bits": bx.crt_bits(2 * n, L), **errors()}
+        log(f"zsplit: crt L={L} {res[f'crt{L}']}")
+    ok = [k for k in res if k.startswith(("s", "crt")) and isinstance(res[k], dict)
+          and res[k]["max_rel"] <= res["plain"]["max_rel"]]
+    if ok:
+        best = max(ok, key=lambda k: res[k]["tflops_effective"])
+        res["fastest_at_or_below_plain_max_rel"] = best
+        res["speedup_at_equal_or_lower_error"] = round(
+            res[best]["tflops_effective"] / res["plain"]["tflops"], 3)
+    for X in (dA, dB, dC):
+        X.free()
+    return res
+
+
 def serving(bx, rates=(50, 100, 200, 500, 1000, 2000), nreq=200, budget=200 << 30, seed=7):
     """Mixed GEMM request stream through the continuous-batching server vs the
     same trace through the BLAS ABI one call at a time.
@@ -475,7 +560,7 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="one DGEMM + one 1 GiB copy kernel, for rocprofv3 --pmc passes")
     ap.add_argument("--only", default="", help="comma list of extras to run "
-                    "(host,ooc,split,serve); default all")
+                    "(host,ooc,split,zsplit,serve); default all")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build targets one GPU; --gpus {args.gpus} is not supported "
@@ -528,6 +613,7 @@ def main():
         sections = [("host", "host_resident", lambda: host_resident(bx, n)),
                     ("ooc", "out_of_core", lambda: out_of_core(bx, n=args.ooc_n)),
                     ("split", "split_precision", lambda: split_precision(bx, n=args.split_n)),
+                    ("zsplit", "zsplit_precision", lambda: zsplit_precision(bx)),
                     ("serve", "serving", lambda: serving(bx, nreq=args.serve_requests))]
         for key, name, fn in sections:
             if only and key not in only:

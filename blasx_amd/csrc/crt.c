@@ -186,7 +186,7 @@ int bx_dgemm_crt(int opa, int opb, int M, int N, int K, double alpha, const doub
     if (bb < 1) return BLASX_EINVAL; /* too few moduli for this K */
     const int pad = (M >= 4096 && N >= 4096) ? 2 * bxh_oz_block_m() : bxh_oz_block_m();
     const int mp = (int)rup(M, pad), np = (int)rup(N, pad);
-    const int kp = (int)rup(K, bxh_oz_block_k());
+    const int kp = (int)rup(K, 2 * bxh_oz_block_k()); /* an even number of K tiles */
 
     const size_t a8 = (size_t)L * mp * kp, b8 = (size_t)L * np * kp;
     const size_t r8 = (size_t)L * mp * np;
@@ -271,7 +271,7 @@ int bx_zgemm_crt(int opa, int opb, int M, int N, int K, const double *alpha, con
     if (bb < 1) return BLASX_EINVAL;
     const int pad = (M >= 4096 && N >= 4096) ? 2 * bxh_oz_block_m() : bxh_oz_block_m();
     const int mp = (int)rup(M, pad), np = (int)rup(N, pad);
-    const int kp = (int)rup(K, bxh_oz_block_k());
+    const int kp = (int)rup(K, 2 * bxh_oz_block_k()); /* an even number of K tiles */
     const int P = 3 * L;
 
     const size_t a8 = (size_t)P * mp * kp, b8 = (size_t)P * np * kp, r8 = (size_t)P * mp * np;

@@ -30,8 +30,29 @@ namespace {
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int OZ_BM = 128, OZ_BN = 128, OZ_BK = 64;
-constexpr int OZ_STR = OZ_BK + 16; // LDS row stride in bytes: conflict-free 16-byte reads
+// LDS layout of a staged K tile: row r (of the M or N extent), 16-byte
+// chunk c (k = 16c .. 16c+15) at byte lds_at(r, c): unpadded 64-byte rows
+// with the chunk index XORed by bits 1-2 of the row.  Measured on gfx950
+// with the LDS bank-conflict counter, this is conflict-free both for the
+// MFMA fragment reads (ds_read_b128, lane -> row lane%16, chunk lane/16) and
+// for the staging writes (lane -> row lane/4, chunk lane%4); a padded
+// 80-byte row stride conflicts 2-way on the reads and 3-way on the writes.
+constexpr int OZ_STR = OZ_BK;
+__device__ __forceinline__ int lds_at(int r, int c) { return r * OZ_STR + ((c ^ ((r >> 1) & 3)) << 4); }
 constexpr int OZ_NT = 256;
+
+// Slice / residue planes are stored as panels: for each block of OZ_PR = 128
+// rows and each K tile of OZ_BK = 64 bytes, the 128 x 64 piece is one
+// contiguous 8 KB chunk (row-major inside), chunks ordered by row block, then
+// K tile.  A block's K-tile load is then a contiguous, fully used stretch
+// of cache lines instead of 128 half-used lines kp bytes apart.
+constexpr int OZ_PR = 128;
+constexpr int OZ_CHUNK = OZ_PR * OZ_BK; // 8 KB
+__host__ __device__ __forceinline__ int64_t panel_off(int i, int k, int kp)
+{
+    return ((int64_t)(i / OZ_PR) * (kp / OZ_BK) + k / OZ_BK) * OZ_CHUNK + (i % OZ_PR) * OZ_BK +
+           k % OZ_BK;
+}
 
 // ---------------------------------------------------------------------------
 // 1. exponents: e[i] = smallest integer with max_k |x(i,k)| * 2^-e < 1
@@ -137,7 +158,7 @@ __global__ void k_split_strided(const double *__restrict__ X, int64_t ld, int n,
     const int i = i0 + row;
     if (i >= np) return;
     for (int p = 0; p < s; ++p) {
-        int8_t *dst = D + (int64_t)p * np * kp + (int64_t)i * kp + k0 + kc;
+        int8_t *dst = D + (int64_t)p * np * kp + panel_off(i, k0 + kc, kp);
         uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int b = 0; b < 16; ++b)
@@ -170,7 +191,7 @@ __global__ void k_split_contig(const double *__restrict__ X, int64_t ld, int n, 
             r[b] -= tq;
             w[b >> 2] |= ((uint32_t)(uint8_t)(int8_t)(int)tq) << (8 * (b & 3));
         }
-        *(i32x4 *)(D + (int64_t)p * np * kp + (int64_t)i * kp + k0) =
+        *(i32x4 *)(D + (int64_t)p * np * kp + panel_off(i, k0, kp)) =
             i32x4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
     }
 }
@@ -234,19 +255,20 @@ __device__ __forceinline__ void tile128(const OzSeg *__restrict__ segs, int nseg
     // loader: 128 rows x 64 bytes per operand = 512 x 16 B, two per thread
     const int lrow = tid >> 2, lk = (tid & 3) * 16;
     i32x4 ra[2], rb[2];
+    // one contiguous 8 KB panel chunk per operand: rows lrow and lrow + 64
     auto fetch = [&](int seg, int k0) {
-        const int8_t *pa = segs[seg].A + (int64_t)(m0 + lrow) * kp + k0 + lk;
-        const int8_t *pb = segs[seg].B + (int64_t)(n0 + lrow) * kp + k0 + lk;
+        const int8_t *pa = segs[seg].A + panel_off(m0, k0, kp) + tid * 16;
+        const int8_t *pb = segs[seg].B + panel_off(n0, k0, kp) + tid * 16;
         ra[0] = gload((const i32x4 *)pa);
-        ra[1] = gload((const i32x4 *)(pa + (int64_t)64 * kp));
+        ra[1] = gload((const i32x4 *)(pa + OZ_CHUNK / 2));
         rb[0] = gload((const i32x4 *)pb);
-        rb[1] = gload((const i32x4 *)(pb + (int64_t)64 * kp));
+        rb[1] = gload((const i32x4 *)(pb + OZ_CHUNK / 2));
     };
     auto lstore = [&](int buf) {
-        *(i32x4 *)&sA[buf][lrow * OZ_STR + lk] = ra[0];
-        *(i32x4 *)&sA[buf][(lrow + 64) * OZ_STR + lk] = ra[1];
-        *(i32x4 *)&sB[buf][lrow * OZ_STR + lk] = rb[0];
-        *(i32x4 *)&sB[buf][(lrow + 64) * OZ_STR + lk] = rb[1];
+        *(i32x4 *)&sA[buf][lds_at(lrow, lk >> 4)] = ra[0];
+        *(i32x4 *)&sA[buf][lds_at(lrow + 64, lk >> 4)] = ra[1];
+        *(i32x4 *)&sB[buf][lds_at(lrow, lk >> 4)] = rb[0];
+        *(i32x4 *)&sB[buf][lds_at(lrow + 64, lk >> 4)] = rb[1];
     };
 
     const int ktiles = kp / OZ_BK;
@@ -261,10 +283,10 @@ __device__ __forceinline__ void tile128(const OzSeg *__restrict__ segs, int nseg
         i32x4 fa[4], fb[4];
 #pragma unroll
         for (int mb = 0; mb < 4; ++mb)
-            fa[mb] = *(const i32x4 *)&sA[buf][(wm * 64 + mb * 16 + l16) * OZ_STR + lg * 16];
+            fa[mb] = *(const i32x4 *)&sA[buf][lds_at(wm * 64 + mb * 16 + l16, lg)];
 #pragma unroll
         for (int nb2 = 0; nb2 < 4; ++nb2)
-            fb[nb2] = *(const i32x4 *)&sB[buf][(wn * 64 + nb2 * 16 + l16) * OZ_STR + lg * 16];
+            fb[nb2] = *(const i32x4 *)&sB[buf][lds_at(wn * 64 + nb2 * 16 + l16, lg)];
 #pragma unroll
         for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
@@ -276,136 +298,123 @@ __device__ __forceinline__ void tile128(const OzSeg *__restrict__ segs, int nseg
 
 }
 
-// 256x256 tile: 4 waves of 128x128 (8x8 MFMA_I32_16X16X64_I8 accumulators,
-// 256 registers, held in AGPRs), one block per CU.  Twice the INT8
-// operations per byte of L2 traffic of the 128x128 tile and half the LDS
-// reads per MFMA.  acc[mb][nb] of wave (wm, wn) holds rows
-// m0 + 128 wm + 16 mb + 4 (lane/16) + v, column n0 + 128 wn + 16 nb + lane%16.
+// 256x256 tile, 8 waves (two per SIMD) of 128x64: 32 MFMA_I32_16X16X64_I8
+// accumulators (128 AGPRs) per wave.  One wave per SIMD cannot keep the INT8
+// matrix core busy on its own (a pure-MFMA loop reaches 2.7 POP/s with one
+// wave per SIMD and 4.4 with two, of 5.0), so the tile is split over two
+// waves per SIMD, which also hide each other's LDS waits.  Twice the INT8
+// operations per byte of L2 traffic of the 128x128 tile.  acc[mb][nb] of wave
+// (wm, wn) = (wave & 1, wave >> 1) holds rows m0 + 128 wm + 16 mb +
+// 4 (lane/16) + v, column n0 + 64 wn + 16 nb + lane%16.
 constexpr int OZ_BIG = 256;
+constexpr int OZ_NT_BIG = 512;
+
+template <int V> struct Const {
+    static constexpr int value = V;
+};
 
 __device__ __forceinline__ void tile256(const OzSeg *__restrict__ segs, int nseg, int kp, int m0,
-                                        int n0, i32x4 (&acc)[8][8])
+                                        int n0, i32x4 (&acc)[8][4])
 {
-    __shared__ __attribute__((aligned(16))) int8_t sA[3][OZ_BIG * OZ_STR];
-    __shared__ __attribute__((aligned(16))) int8_t sB[3][OZ_BIG * OZ_STR];
+    __shared__ __attribute__((aligned(16))) int8_t sA[2][OZ_BIG * OZ_STR];
+    __shared__ __attribute__((aligned(16))) int8_t sB[2][OZ_BIG * OZ_STR];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int l16 = lane & 15, lg = lane >> 4;
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
-        for (int j = 0; j < 8; ++j) acc[i][j] = i32x4{0, 0, 0, 0};
+        for (int j = 0; j < 4; ++j) acc[i][j] = i32x4{0, 0, 0, 0};
 
-    // loader: 256 rows x 64 bytes per operand = 1024 x 16 B, four per thread
+    // loader: 256 rows x 64 bytes per operand = 1024 x 16 B, two per thread,
+    // in two register sets (tiles t and t+1 in flight)
     const int lrow = tid >> 2, lk = (tid & 3) * 16;
-    i32x4 ra[4], rb[4];
-    const int ktiles = kp / OZ_BK;
+    i32x4 ra[2][2], rb[2][2];
+    const int ktiles = kp / OZ_BK; // even (the host pads K to 128)
     const int total = nseg * ktiles;
-    // K tile t (over all segments) -> registers
-    auto fetch = [&](int t) {
+    // K tile t (over all segments) -> register set `set`: two 8 KB panel
+    // chunks per operand (row blocks m0/128 and m0/128 + 1), one 16-byte
+    // piece of each per thread: rows lrow and lrow + 128
+    auto fetch = [&](int t, const int set) {
         const int seg = t / ktiles, k0 = (t - seg * ktiles) * OZ_BK;
-        const int8_t *pa = segs[seg].A + (int64_t)(m0 + lrow) * kp + k0 + lk;
-        const int8_t *pb = segs[seg].B + (int64_t)(n0 + lrow) * kp + k0 + lk;
+        const int8_t *pa = segs[seg].A + panel_off(m0, k0, kp) + tid * 16;
+        const int8_t *pb = segs[seg].B + panel_off(n0, k0, kp) + tid * 16;
+        const int64_t next_rows = (int64_t)ktiles * OZ_CHUNK; // the following row block
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            ra[i] = gload((const i32x4 *)(pa + (int64_t)(64 * i) * kp));
-            rb[i] = gload((const i32x4 *)(pb + (int64_t)(64 * i) * kp));
+        for (int i = 0; i < 2; ++i) {
+            ra[set][i] = gload((const i32x4 *)(pa + i * next_rows));
+            rb[set][i] = gload((const i32x4 *)(pb + i * next_rows));
         }
     };
-    auto lstore = [&](int buf) {
+    auto lstore = [&](int buf, const int set) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            *(i32x4 *)&sA[buf][(lrow + 64 * i) * OZ_STR + lk] = ra[i];
-            *(i32x4 *)&sB[buf][(lrow + 64 * i) * OZ_STR + lk] = rb[i];
+        for (int i = 0; i < 2; ++i) {
+            *(i32x4 *)&sA[buf][lds_at(lrow + 128 * i, lk >> 4)] = ra[set][i];
+            *(i32x4 *)&sB[buf][lds_at(lrow + 128 * i, lk >> 4)] = rb[set][i];
         }
     };
-    auto rda = [&](int buf, int i) {
-        return *(const i32x4 *)&sA[buf][(wm * 128 + i * 16 + l16) * OZ_STR + lg * 16];
-    };
-    auto rdb = [&](int buf, int i) {
-        return *(const i32x4 *)&sB[buf][(wn * 128 + i * 16 + l16) * OZ_STR + lg * 16];
-    };
-
-    // Software pipeline over three LDS buffers (tile t lives in buffer t%3).
-    // Iteration it:
-    //   * loads tile it+2 from HBM/L2 into registers and stages it in buffer
-    //     (it+2)%3 at the end (that buffer's previous tile, it-1, was last
-    //     read in iteration it-1, before its closing barrier);
-    //   * runs the 64 MFMAs of tile it: its 8 B fragments and first A
-    //     fragment are already in registers, the other A fragments are read
-    //     one row block ahead of their MFMAs;
-    //   * reads the B fragments and first A fragment of tile it+1 (staged in
-    //     iteration it-1) between those MFMAs.
-    // One barrier per iteration.  The body has no branches: past the last
-    // tile it re-fetches tile total-1 into a buffer never read again, and
-    // fragments read past the end are never used.
-    i32x4 fb[2][8], fa0[2];
-    fetch(0);
-    lstore(0);
-    fetch(total > 1 ? 1 : 0);
-    lstore(1);
-    __syncthreads();
+    // Double-buffered LDS with global loads two K tiles ahead: step t (LDS
+    // buffer and register set t % 2) issues the loads of tile t+2 into the
+    // register set tile t used, reads this tile's 12 MFMA fragments (LDS
+    // returns them in order, so the first MFMAs start after 5 reads), runs
+    // 32 MFMAs, stores tile t+1 (loaded a whole step earlier) into the other
+    // buffer and ends with the one barrier.  The loop runs two steps per
+    // trip with fixed register sets and has no branches: past the end it
+    // re-loads tile total-1 into a buffer that is not read again.
+    auto step = [&](const int t, auto set_c) {
+        constexpr int S = decltype(set_c)::value;
+        fetch(t + 2 < total ? t + 2 : total - 1, S);
+        i32x4 fa[8], fb[4];
+        fa[0] = *(const i32x4 *)&sA[S][lds_at(wm * 128 + l16, lg)];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) fb[0][i] = rdb(0, i);
-    fa0[0] = rda(0, 0);
-    int bcur = 0; // buffer of tile it
-    for (int it = 0; it < total; ++it) {
-        const int bnxt = bcur == 2 ? 0 : bcur + 1, bfree = bnxt == 2 ? 0 : bnxt + 1;
-        fetch(it + 2 < total ? it + 2 : total - 1);
-        i32x4 fa = fa0[0];
+        for (int nb2 = 0; nb2 < 4; ++nb2)
+            fb[nb2] = *(const i32x4 *)&sB[S][lds_at(wn * 64 + nb2 * 16 + l16, lg)];
 #pragma unroll
-        for (int mb = 0; mb < 8; ++mb) {
-            i32x4 fan;
-            if (mb < 7) fan = rda(bcur, mb + 1);
-            fb[1][mb] = rdb(bnxt, mb);
-            if (mb == 7) fa0[1] = rda(bnxt, 0);
+        for (int mb = 1; mb < 8; ++mb)
+            fa[mb] = *(const i32x4 *)&sA[S][lds_at(wm * 128 + mb * 16 + l16, lg)];
 #pragma unroll
-            for (int nb2 = 0; nb2 < 8; ++nb2)
-                acc[mb][nb2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa, fb[0][nb2], acc[mb][nb2],
+        for (int mb = 0; mb < 8; ++mb)
+#pragma unroll
+            for (int nb2 = 0; nb2 < 4; ++nb2)
+                acc[mb][nb2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[mb], fb[nb2], acc[mb][nb2],
                                                                      0, 0, 0);
-            if (mb < 7) fa = fan;
-        }
-        lstore(bfree);
-        // issue order: global loads; per row block its LDS reads, then its
-        // 8 MFMAs; the LDS stores between the last row block's MFMAs
-        __builtin_amdgcn_sched_group_barrier(0x020, 8, 0);
-#pragma unroll
-        for (int i = 0; i < 7; ++i) {
-            __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
-            __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);
-        }
-        __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
-        }
+        // keep that order: otherwise the scheduler reuses one register for
+        // every fa and waits for each read right before its MFMAs
+        __builtin_amdgcn_sched_group_barrier(0x020, 4, 0);  // 4 global loads
+        __builtin_amdgcn_sched_group_barrier(0x100, 12, 0); // 12 LDS reads
+        __builtin_amdgcn_sched_group_barrier(0x008, 32, 0); // then 32 MFMAs
+        lstore(S ^ 1, S ^ 1);
         __syncthreads();
-        bcur = bnxt;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) fb[0][i] = fb[1][i];
-        fa0[0] = fa0[1];
+    };
+    fetch(0, 0);
+    lstore(0, 0);
+    fetch(total > 1 ? 1 : 0, 1);
+    __syncthreads();
+    for (int t = 0; t < total; t += 2) {
+        step(t, Const<0>{});
+        step(t + 1, Const<1>{});
     }
-
 }
 
-// W = (first ? 0 : W) + scale * (tile sum); rows >= m / columns >= n skipped
-template <int WT, int NB>
-__device__ __forceinline__ void fold_w(const i32x4 (&acc)[NB][NB], int m0, int n0, int m, int n,
+// W = (first ? 0 : W) + scale * (tile sum); rows >= m / columns >= n skipped.
+// Wave (wm, wn) = (wave & 1, wave >> 1) owns rows m0 + WTM wm + 16 mb + ...,
+// columns n0 + WTN wn + 16 nb + lane % 16.
+template <int WTM, int WTN, int NBM, int NBN>
+__device__ __forceinline__ void fold_w(const i32x4 (&acc)[NBM][NBN], int m0, int n0, int m, int n,
                                        double *__restrict__ W, int64_t ldw, double scale,
                                        int first)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lg = lane >> 4;
 #pragma unroll
-    for (int mb = 0; mb < NB; ++mb)
+    for (int mb = 0; mb < NBM; ++mb)
 #pragma unroll
-        for (int nb2 = 0; nb2 < NB; ++nb2) {
-            const int col = n0 + wn * WT + nb2 * 16 + l16;
+        for (int nb2 = 0; nb2 < NBN; ++nb2) {
+            const int col = n0 + wn * WTN + nb2 * 16 + l16;
             if (col >= n) continue;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = m0 + wm * WT + mb * 16 + 4 * lg + v;
+                const int row = m0 + wm * WTM + mb * 16 + 4 * lg + v;
                 if (row >= m) continue;
                 double *w = W + row + (int64_t)col * ldw;
                 const double add = scale * (double)acc[mb][nb2][v];
@@ -423,19 +432,20 @@ __global__ __launch_bounds__(OZ_NT, 2) void k_ozaki_group(const OzSeg *__restric
     block_origin<OZ_BM>(n, m0, n0);
     i32x4 acc[4][4];
     tile128(segs, nseg, kp, m0, n0, acc);
-    fold_w<64, 4>(acc, m0, n0, m, n, W, ldw, scale, first);
+    fold_w<64, 64, 4, 4>(acc, m0, n0, m, n, W, ldw, scale, first);
 }
 
-__global__ __launch_bounds__(OZ_NT, 1) void k_ozaki_group_big(const OzSeg *__restrict__ segs,
-                                                              int nseg, int kp, int m, int n,
-                                                              double *__restrict__ W, int64_t ldw,
-                                                              double scale, int first)
+__global__ __launch_bounds__(OZ_NT_BIG, 1) void k_ozaki_group_big(const OzSeg *__restrict__ segs,
+                                                                  int nseg, int kp, int m, int n,
+                                                                  double *__restrict__ W,
+                                                                  int64_t ldw, double scale,
+                                                                  int first)
 {
     int m0, n0;
     block_origin<OZ_BIG>(n, m0, n0);
-    i32x4 acc[8][8];
+    i32x4 acc[8][4];
     tile256(segs, nseg, kp, m0, n0, acc);
-    fold_w<128, 8>(acc, m0, n0, m, n, W, ldw, scale, first);
+    fold_w<128, 64, 8, 4>(acc, m0, n0, m, n, W, ldw, scale, first);
 }
 
 // 4. C = alpha * 2^(e_i + f_j) * W + beta * C
@@ -579,7 +589,7 @@ __global__ __launch_bounds__(256) void k_crt_split_contig(const double *__restri
             const int r = crt_resid(v[b], md.m[l], md.rinv[l], md.rinvf[l], md.p2[l]);
             w[b >> 2] |= ((uint32_t)(uint8_t)(int8_t)r) << (8 * (b & 3));
         }
-        *(i32x4 *)(D + (int64_t)l * np * kp + (int64_t)i * kp + k0) =
+        *(i32x4 *)(D + (int64_t)l * np * kp + panel_off(i, k0, kp)) =
             i32x4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
     }
 }
@@ -618,7 +628,7 @@ __global__ __launch_bounds__(256) void k_crt_split_strided(const double *__restr
         const int i = i0 + row;
         if (i < np)
             for (int p = 0; p < cnt; ++p) {
-                int8_t *dst = D + (int64_t)(l0 + p) * np * kp + (int64_t)i * kp + k0 + kc;
+                int8_t *dst = D + (int64_t)(l0 + p) * np * kp + panel_off(i, k0 + kc, kp);
                 uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
                 for (int b = 0; b < 16; ++b)
@@ -666,7 +676,7 @@ __global__ __launch_bounds__(256) void k_zcrt_split_contig(const double *__restr
         for (int b = 0; b < 16; ++b)
             w[b >> 2] |= ((uint32_t)(uint8_t)(int8_t)zcrt_plane_resid(vr[b], vi[b], p, md))
                          << (8 * (b & 3));
-        *(i32x4 *)(D + (int64_t)p * np * kp + (int64_t)i * kp + k0) =
+        *(i32x4 *)(D + (int64_t)p * np * kp + panel_off(i, k0, kp)) =
             i32x4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
     }
 }
@@ -708,7 +718,7 @@ __global__ __launch_bounds__(256) void k_zcrt_split_strided(const double *__rest
         const int i = i0 + row;
         if (i < np)
             for (int p = 0; p < cnt; ++p) {
-                int8_t *dst = D + (int64_t)(p0 + p) * np * kp + (int64_t)i * kp + k0 + kc;
+                int8_t *dst = D + (int64_t)(p0 + p) * np * kp + panel_off(i, k0 + kc, kp);
                 uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
                 for (int b = 0; b < 16; ++b)
@@ -721,18 +731,19 @@ __global__ __launch_bounds__(256) void k_zcrt_split_strided(const double *__rest
 
 // residues of one tile sum -> int8 plane (column-major, ld ldr); every lane
 // owns 4 consecutive rows of one column: one 32-bit store each
-template <int WT, int NB>
-__device__ __forceinline__ void store_resid(const i32x4 (&acc)[NB][NB], int m0, int n0, int mod,
-                                            double rinv, int8_t *__restrict__ R, int64_t ldr)
+template <int WTM, int WTN, int NBM, int NBN>
+__device__ __forceinline__ void store_resid(const i32x4 (&acc)[NBM][NBN], int m0, int n0,
+                                            int mod, double rinv, int8_t *__restrict__ R,
+                                            int64_t ldr)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lg = lane >> 4;
 #pragma unroll
-    for (int mb = 0; mb < NB; ++mb)
+    for (int mb = 0; mb < NBM; ++mb)
 #pragma unroll
-        for (int nb2 = 0; nb2 < NB; ++nb2) {
-            const int col = n0 + wn * WT + nb2 * 16 + l16;
-            const int row = m0 + wm * WT + mb * 16 + 4 * lg;
+        for (int nb2 = 0; nb2 < NBN; ++nb2) {
+            const int col = n0 + wn * WTN + nb2 * 16 + l16;
+            const int row = m0 + wm * WTM + mb * 16 + 4 * lg;
             uint32_t w = 0;
 #pragma unroll
             for (int v = 0; v < 4; ++v)
@@ -754,20 +765,20 @@ __global__ __launch_bounds__(OZ_NT, 2) void k_crt_gemm(const OzSeg *__restrict__
     const int p = blockIdx.y, l = p / per;
     i32x4 acc[4][4];
     tile128(segs + p, 1, kp, m0, n0, acc);
-    store_resid<64, 4>(acc, m0, n0, md.m[l], md.rinv[l], R + p * plane, ldr);
+    store_resid<64, 64, 4, 4>(acc, m0, n0, md.m[l], md.rinv[l], R + p * plane, ldr);
 }
 
-__global__ __launch_bounds__(OZ_NT, 1) void k_crt_gemm_big(const OzSeg *__restrict__ segs, int kp,
-                                                           int np, int per, CrtMods md,
-                                                           int8_t *__restrict__ R, int64_t ldr,
-                                                           int64_t plane)
+__global__ __launch_bounds__(OZ_NT_BIG, 1) void k_crt_gemm_big(const OzSeg *__restrict__ segs,
+                                                               int kp, int np, int per, CrtMods md,
+                                                               int8_t *__restrict__ R, int64_t ldr,
+                                                               int64_t plane)
 {
     int m0, n0;
     block_origin<OZ_BIG>(np, m0, n0);
     const int p = blockIdx.y, l = p / per;
-    i32x4 acc[8][8];
+    i32x4 acc[8][4];
     tile256(segs + p, 1, kp, m0, n0, acc);
-    store_resid<128, 8>(acc, m0, n0, md.m[l], md.rinv[l], R + p * plane, ldr);
+    store_resid<128, 64, 8, 4>(acc, m0, n0, md.m[l], md.rinv[l], R + p * plane, ldr);
 }
 
 struct CrtRecon {
@@ -939,9 +950,12 @@ int bxh_oz_group(const void *segs_dev, int nseg, int kp, int m, int n, int mp, i
 {
     if (nseg <= 0) return 0;
     if (kp <= 0 || mp % OZ_BM || np % OZ_BN || kp % OZ_BK) return BLASX_EINVAL;
-    const int big_tiles = (mp % OZ_BIG == 0 && np % OZ_BIG == 0) ? (mp / OZ_BIG) * (np / OZ_BIG) : 0;
+    // the 256x256 engine runs K tiles in pairs
+    const int big_tiles = (mp % OZ_BIG == 0 && np % OZ_BIG == 0 && (kp / OZ_BK) % 2 == 0)
+                              ? (mp / OZ_BIG) * (np / OZ_BIG)
+                              : 0;
     if (big_tiles >= 256) { // enough 256x256 tiles to give every CU one
-        hipLaunchKernelGGL(k_ozaki_group_big, dim3(big_tiles), dim3(OZ_NT), 0,
+        hipLaunchKernelGGL(k_ozaki_group_big, dim3(big_tiles), dim3(OZ_NT_BIG), 0,
                            (hipStream_t)stream, (const OzSeg *)segs_dev, nseg, kp, m, n, W, ldw,
                            scale, first);
     } else {
@@ -1019,9 +1033,12 @@ int bxh_crt_gemm(const void *segs_dev, int L, int per, const int *moduli, int kp
     const CrtMods md = crt_mods(L, moduli);
     const int64_t plane = (int64_t)mp * np;
     const int nprod = per * L;
-    const int big_tiles = (mp % OZ_BIG == 0 && np % OZ_BIG == 0) ? (mp / OZ_BIG) * (np / OZ_BIG) : 0;
+    // the 256x256 engine runs K tiles in pairs
+    const int big_tiles = (mp % OZ_BIG == 0 && np % OZ_BIG == 0 && (kp / OZ_BK) % 2 == 0)
+                              ? (mp / OZ_BIG) * (np / OZ_BIG)
+                              : 0;
     if (big_tiles * nprod >= 256) {
-        hipLaunchKernelGGL(k_crt_gemm_big, dim3(big_tiles, nprod), dim3(OZ_NT), 0,
+        hipLaunchKernelGGL(k_crt_gemm_big, dim3(big_tiles, nprod), dim3(OZ_NT_BIG), 0,
                            (hipStream_t)stream, (const OzSeg *)segs_dev, kp, np, per, md, R,
                            (int64_t)mp, plane);
     } else {

@@ -10,10 +10,10 @@
  *   * admission: queued requests are admitted in arrival order while their
  *     operands fit in the arena; a request larger than the whole arena is
  *     rejected at submit, so admission never deadlocks;
- *   * batching: everything admitted since the last launch forms the next
- *     batch; up to NBATCH batches are in flight, so requests that arrive
- *     while others compute are copied in during that compute and launched
- *     as soon as a batch slot frees up;
+ *   * batching: what was admitted since the last launch forms the next
+ *     batch, closed at BATCH_BYTES of copies; up to NBATCH batches are in
+ *     flight, so requests that arrive while others compute are copied in
+ *     during that compute and launched as soon as a batch slot frees up;
  *   * launch: each batch slot owns a captured hipGraph of three persistent
  *     grouped kernels (S, D, Z) that read their problem tables from that
  *     slot's device buffer; a launch uploads the tables and replays the
@@ -34,6 +34,10 @@
 
 #define NBATCH 3
 #define MAXRES 64
+/* a batch closes once its copies reach this many bytes, so that under load
+ * several batches are in flight and one batch's copies overlap another's
+ * compute (one huge batch would serialise copy-in, compute and copy-out) */
+#define BATCH_BYTES ((size_t)384 << 20)
 
 static double now_s(void)
 {
@@ -281,6 +285,16 @@ static int launch_batch(blasx_server_t *s, batch_t *b)
     return rc;
 }
 
+/* host<->device bytes of one request if none of its operands is resident */
+static size_t copy_bytes(const blasx_gemm_desc_t *d)
+{
+    const size_t es = elem(d->dtype);
+    int ra, ca, rb, cb;
+    extents(d, &ra, &ca, &rb, &cb);
+    const int bz = d->beta[0] == 0.0 && d->beta[1] == 0.0;
+    return es * ((size_t)ra * ca + (size_t)rb * cb + (size_t)(bz ? 1 : 2) * d->M * d->N);
+}
+
 static void release(blasx_server_t *s, sreq_t *r)
 {
     arena_free(&s->arena, r->bA ? r->dA : NULL, r->bA);
@@ -353,9 +367,13 @@ static void *scheduler(void *arg)
             if (!s->b[k].in_use) fb = &s->b[k];
         if (fb && s->next_admit < s->nreq) {
             fb->n = 0;
+            size_t bytes = 0;
             while (s->next_admit < s->nreq && fb->n < s->max_batch) {
                 sreq_t *r = &s->req[s->next_admit];
+                const size_t rb = copy_bytes(&r->d);
+                if (fb->n > 0 && bytes + rb > BATCH_BYTES) break;
                 if (admit(s, r) != 0) break; /* wait for memory: keep arrival order */
+                bytes += rb;
                 r->state = 1;
                 fb->snap[fb->n] = *r;
                 fb->ids[fb->n++] = s->next_admit++;

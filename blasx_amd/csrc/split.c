@@ -66,7 +66,7 @@ int bx_dgemm_split(int opa, int opb, int M, int N, int K, double alpha, const do
     /* large problems pad to the 256x256 tile of the big INT8 kernel */
     const int pad = (M >= 4096 && N >= 4096) ? 2 * bxh_oz_block_m() : bxh_oz_block_m();
     const int mp = (int)rup(M, pad), np = (int)rup(N, pad);
-    const int kp = (int)rup(K, bxh_oz_block_k());
+    const int kp = (int)rup(K, 2 * bxh_oz_block_k()); /* an even number of K tiles */
     const int segmax = max_segments(kp);
     if (segmax < 1) return BLASX_ERANGE; /* K above ~133k: int32 products could overflow */
 

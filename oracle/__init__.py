@@ -230,9 +230,11 @@ def ozaki_digits(x, e, s):
     return [int(d[t]) for t in range(s)]
 
 
-def ozaki_segmax(K, block_k=64):
-    """Slice pairs per exact int32 launch: nseg * Kp * 127^2 < 2^31."""
-    kp = (K + block_k - 1) // block_k * block_k
+def ozaki_segmax(K, k_align=128):
+    """Slice pairs per exact int32 launch: nseg * Kp * 127^2 < 2^31, with Kp
+    the K extent the library pads to (a multiple of 128: pairs of 64-byte
+    K tiles)."""
+    kp = (K + k_align - 1) // k_align * k_align
     return (2**31 - 1) // (kp * 127 * 127)
 
 
