@@ -521,7 +521,7 @@ struct CrtMods {
     int m[20];
     double rinv[20];  // 1 / m_l, rounded
     float rinvf[20];
-    uint8_t p2[20][64]; // 2^t mod m_l
+    int c26[20];      // 2^26 mod m_l
 };
 
 // r in [lo - m, hi + m] -> [lo, hi], lo = -floor(m/2), hi = ceil(m/2) - 1
@@ -547,24 +547,28 @@ __host__ __device__ __forceinline__ int smod_i32(int x, int m, double rinv)
     return smod_fix(x - q * m, m);
 }
 
-// symmetric residue of an integer-valued double v, |v| < 2^116.  For
-// |v| >= 2^53, v = x 2^t with |x| < 2^53 integral, and the residue is that
-// of smod(x) * (2^t mod m).  For |v| < 2^53, q = rint(v / m) is within 1 of
-// the rounded quotient and v - q m is exact in the FMA.
-__host__ __device__ __forceinline__ int crt_resid(double v, int m, double rinv, float rinvf,
-                                                  const uint8_t *p2)
+// An integer-valued double v, |v| < 2^79 (the split's |A'| < 2^beta, beta
+// <= 77), as v = h 2^26 + l with h = trunc(v 2^-26) and l = v - h 2^26 both
+// exact (|h| < 2^53, |l| < 2^26).  Non-finite inputs are taken as zero.
+struct Split26 {
+    double h, l;
+};
+__host__ __device__ __forceinline__ Split26 split26(double v)
 {
-    if (!(fabs(v) < 0x1p116)) return 0; // non-finite input: the split needs finite entries
-    int p = 1;
-    if (fabs(v) >= 0x1p53) {
-        const int t = ilogb(v) - 52;
-        v = ldexp(v, -t);
-        p = p2[t];
-    }
-    const double q = rint(v * rinv);
-    int r = smod_fix((int)fma(-q, (double)m, v), m);
-    if (p != 1) r = smod_small(r * p, m, rinvf);
-    return r;
+    if (!(fabs(v) < 0x1p79)) v = 0.0;
+    const double h = trunc(v * 0x1p-26);
+    return Split26{h, v - h * 0x1p26};
+}
+
+// symmetric residue of v = h 2^26 + l: smod(h) (2^26 mod m) + smod(l),
+// reduced once more.  For |x| < 2^53, q = rint(x / m) is within 1 of the
+// rounded quotient and x - q m is exact in the FMA; no branch, no table.
+__host__ __device__ __forceinline__ int crt_resid(Split26 v, int m, double rinv, float rinvf,
+                                                  int c26)
+{
+    const double qh = rint(v.h * rinv), ql = rint(v.l * rinv);
+    const int rh = (int)fma(-qh, (double)m, v.h), rl = (int)fma(-ql, (double)m, v.l);
+    return smod_small(rh * c26 + rl, m, rinvf);
 }
 
 // contiguous (mode 1): x(i,k) = X[k + i*ld]; one thread per 16 consecutive
@@ -577,16 +581,17 @@ __global__ __launch_bounds__(256) void k_crt_split_contig(const double *__restri
     const int chunks = kp / 16;
     if (t >= (int64_t)np * chunks) return;
     const int i = (int)(t / chunks), k0 = (int)(t % chunks) * 16;
-    double v[16];
+    Split26 v[16];
     const int sh = i < n ? beta - e[i] : 0;
 #pragma unroll
     for (int b = 0; b < 16; ++b)
-        v[b] = (i < n && k0 + b < len) ? trunc(ldexp(X[k0 + b + (int64_t)i * ld], sh)) : 0.0;
+        v[b] = split26((i < n && k0 + b < len) ? trunc(ldexp(X[k0 + b + (int64_t)i * ld], sh))
+                                               : 0.0);
     for (int l = 0; l < md.L; ++l) {
         uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int b = 0; b < 16; ++b) {
-            const int r = crt_resid(v[b], md.m[l], md.rinv[l], md.rinvf[l], md.p2[l]);
+            const int r = crt_resid(v[b], md.m[l], md.rinv[l], md.rinvf[l], md.c26[l]);
             w[b >> 2] |= ((uint32_t)(uint8_t)(int8_t)r) << (8 * (b & 3));
         }
         *(i32x4 *)(D + (int64_t)l * np * kp + panel_off(i, k0, kp)) =
@@ -594,59 +599,44 @@ __global__ __launch_bounds__(256) void k_crt_split_contig(const double *__restri
     }
 }
 
-// strided (mode 0): x(i,k) = X[i + k*ld]; a 64 x 64 tile per block, staged
-// through LDS eight planes at a time so both the FP64 reads (along i) and
-// the int8 writes (along k) are coalesced
-__global__ __launch_bounds__(256) void k_crt_split_strided(const double *__restrict__ X, int64_t ld, int n, int len,
-                                    const int *__restrict__ e, int beta, CrtMods md,
-                                    int8_t *__restrict__ D, int np, int kp)
+// strided (mode 0): x(i,k) = X[i + k*ld]; block = 64 rows x 64 k, thread =
+// one row x 16 consecutive k: the FP64 reads are coalesced along i across
+// the wave, and each plane gets one 16-byte store per thread
+__global__ __launch_bounds__(256) void k_crt_split_strided(const double *__restrict__ X, int64_t ld,
+                                                           int n, int len,
+                                                           const int *__restrict__ e, int beta,
+                                                           CrtMods md, int8_t *__restrict__ D,
+                                                           int np, int kp)
 {
-    __shared__ int8_t tile[8][SP_T][SP_T + 4];
-    const int i0 = blockIdx.x * SP_T, k0 = blockIdx.y * SP_T;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6; // 64 x 4
-    double v[SP_T / 4];
-    {
-        const int i = i0 + tx;
-        const int sh = i < n ? beta - e[i] : 0;
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int k0 = blockIdx.y * 64 + (threadIdx.x >> 6) * 16;
+    if (i >= np) return;
+    Split26 v[16];
+    const int sh = i < n ? beta - e[i] : 0;
 #pragma unroll
-        for (int q = 0; q < SP_T / 4; ++q) {
-            const int k = k0 + ty + 4 * q;
-            v[q] = (i < n && k < len) ? trunc(ldexp(X[i + (int64_t)k * ld], sh)) : 0.0;
+    for (int b = 0; b < 16; ++b)
+        v[b] = split26((i < n && k0 + b < len) ? trunc(ldexp(X[i + (int64_t)(k0 + b) * ld], sh))
+                                               : 0.0);
+    for (int l = 0; l < md.L; ++l) {
+        uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+            const int r = crt_resid(v[b], md.m[l], md.rinv[l], md.rinvf[l], md.c26[l]);
+            w[b >> 2] |= ((uint32_t)(uint8_t)(int8_t)r) << (8 * (b & 3));
         }
-    }
-    const int row = threadIdx.x >> 2, kc = (threadIdx.x & 3) * 16;
-    for (int l0 = 0; l0 < md.L; l0 += 8) {
-        const int cnt = md.L - l0 < 8 ? md.L - l0 : 8;
-        for (int p = 0; p < cnt; ++p) {
-            const int l = l0 + p;
-#pragma unroll
-            for (int q = 0; q < SP_T / 4; ++q)
-                tile[p][tx][ty + 4 * q] =
-                    (int8_t)crt_resid(v[q], md.m[l], md.rinv[l], md.rinvf[l], md.p2[l]);
-        }
-        __syncthreads();
-        const int i = i0 + row;
-        if (i < np)
-            for (int p = 0; p < cnt; ++p) {
-                int8_t *dst = D + (int64_t)(l0 + p) * np * kp + panel_off(i, k0 + kc, kp);
-                uint32_t w[4] = {0, 0, 0, 0};
-#pragma unroll
-                for (int b = 0; b < 16; ++b)
-                    w[b >> 2] |= ((uint32_t)(uint8_t)tile[p][row][kc + b]) << (8 * (b & 3));
-                *(i32x4 *)dst = i32x4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
-            }
-        __syncthreads();
+        *(i32x4 *)(D + (int64_t)l * np * kp + panel_off(i, k0, kp)) =
+            i32x4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
     }
 }
 
 // complex operands: planes 3l (real part), 3l+1 (imaginary part), 3l+2
 // (their sum) of modulus l, from one shared exponent per row; conj negates
 // the imaginary part.  Plane p: c = p % 3 of modulus p / 3.
-__device__ __forceinline__ int zcrt_plane_resid(double vr, double vi, int p, const CrtMods &md)
+__device__ __forceinline__ int zcrt_plane_resid(Split26 vr, Split26 vi, int p, const CrtMods &md)
 {
     const int l = p / 3, c = p - 3 * l, m = md.m[l];
-    const int r = c == 1 ? 0 : crt_resid(vr, m, md.rinv[l], md.rinvf[l], md.p2[l]);
-    const int q = c == 0 ? 0 : crt_resid(vi, m, md.rinv[l], md.rinvf[l], md.p2[l]);
+    const int r = c == 1 ? 0 : crt_resid(vr, m, md.rinv[l], md.rinvf[l], md.c26[l]);
+    const int q = c == 0 ? 0 : crt_resid(vi, m, md.rinv[l], md.rinvf[l], md.c26[l]);
     return c == 2 ? smod_fix(r + q, m) : r + q;
 }
 
@@ -661,14 +651,14 @@ __global__ __launch_bounds__(256) void k_zcrt_split_contig(const double *__restr
     const int chunks = kp / 16;
     if (t >= (int64_t)np * chunks) return;
     const int i = (int)(t / chunks), k0 = (int)(t % chunks) * 16;
-    double vr[16], vi[16];
+    Split26 vr[16], vi[16];
     const int sh = i < n ? beta - e[i] : 0;
 #pragma unroll
     for (int b = 0; b < 16; ++b) {
         const bool in = i < n && k0 + b < len;
         const double *z = Z + 2 * (k0 + b + (int64_t)i * ld);
-        vr[b] = in ? trunc(ldexp(z[0], sh)) : 0.0;
-        vi[b] = in ? trunc(ldexp(conj ? -z[1] : z[1], sh)) : 0.0;
+        vr[b] = split26(in ? trunc(ldexp(z[0], sh)) : 0.0);
+        vi[b] = split26(in ? trunc(ldexp(conj ? -z[1] : z[1], sh)) : 0.0);
     }
     for (int p = 0; p < 3 * md.L; ++p) {
         uint32_t w[4] = {0, 0, 0, 0};
@@ -681,51 +671,33 @@ __global__ __launch_bounds__(256) void k_zcrt_split_contig(const double *__restr
     }
 }
 
-// strided (mode 0): z(i,k) = Z[i + k*ld]; 64 x 64 tiles through LDS, eight
-// planes at a time
+// strided (mode 0): z(i,k) = Z[i + k*ld]; thread = one row x 16 k, as above
 __global__ __launch_bounds__(256) void k_zcrt_split_strided(const double *__restrict__ Z,
                                                             int64_t ld, int n, int len, int conj,
                                                             const int *__restrict__ e, int beta,
                                                             CrtMods md, int8_t *__restrict__ D,
                                                             int np, int kp)
 {
-    __shared__ int8_t tile[8][SP_T][SP_T + 4];
-    const int i0 = blockIdx.x * SP_T, k0 = blockIdx.y * SP_T;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6; // 64 x 4
-    double vr[SP_T / 4], vi[SP_T / 4];
-    {
-        const int i = i0 + tx;
-        const int sh = i < n ? beta - e[i] : 0;
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int k0 = blockIdx.y * 64 + (threadIdx.x >> 6) * 16;
+    if (i >= np) return;
+    Split26 vr[16], vi[16];
+    const int sh = i < n ? beta - e[i] : 0;
 #pragma unroll
-        for (int q = 0; q < SP_T / 4; ++q) {
-            const int k = k0 + ty + 4 * q;
-            const bool in = i < n && k < len;
-            const double *z = Z + 2 * (i + (int64_t)k * ld);
-            vr[q] = in ? trunc(ldexp(z[0], sh)) : 0.0;
-            vi[q] = in ? trunc(ldexp(conj ? -z[1] : z[1], sh)) : 0.0;
-        }
+    for (int b = 0; b < 16; ++b) {
+        const bool in = i < n && k0 + b < len;
+        const double *z = Z + 2 * (i + (int64_t)(k0 + b) * ld);
+        vr[b] = split26(in ? trunc(ldexp(z[0], sh)) : 0.0);
+        vi[b] = split26(in ? trunc(ldexp(conj ? -z[1] : z[1], sh)) : 0.0);
     }
-    const int row = threadIdx.x >> 2, kc = (threadIdx.x & 3) * 16;
-    const int planes = 3 * md.L;
-    for (int p0 = 0; p0 < planes; p0 += 8) {
-        const int cnt = planes - p0 < 8 ? planes - p0 : 8;
-        for (int p = 0; p < cnt; ++p) {
+    for (int p = 0; p < 3 * md.L; ++p) {
+        uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
-            for (int q = 0; q < SP_T / 4; ++q)
-                tile[p][tx][ty + 4 * q] = (int8_t)zcrt_plane_resid(vr[q], vi[q], p0 + p, md);
-        }
-        __syncthreads();
-        const int i = i0 + row;
-        if (i < np)
-            for (int p = 0; p < cnt; ++p) {
-                int8_t *dst = D + (int64_t)(p0 + p) * np * kp + panel_off(i, k0 + kc, kp);
-                uint32_t w[4] = {0, 0, 0, 0};
-#pragma unroll
-                for (int b = 0; b < 16; ++b)
-                    w[b >> 2] |= ((uint32_t)(uint8_t)tile[p][row][kc + b]) << (8 * (b & 3));
-                *(i32x4 *)dst = i32x4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
-            }
-        __syncthreads();
+        for (int b = 0; b < 16; ++b)
+            w[b >> 2] |= ((uint32_t)(uint8_t)(int8_t)zcrt_plane_resid(vr[b], vi[b], p, md))
+                         << (8 * (b & 3));
+        *(i32x4 *)(D + (int64_t)p * np * kp + panel_off(i, k0, kp)) =
+            i32x4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
     }
 }
 
@@ -788,30 +760,46 @@ struct CrtRecon {
     double invM;     // 1 / (((M_4 + M_3) + M_2) + M_1)
 };
 
+// one thread per 4 consecutive rows of one column (one 32-bit load per
+// plane; the planes are mp x np with mp a multiple of 128, so the 4 bytes
+// are always inside the plane)
+__device__ __forceinline__ double crt_float(const double (&q)[4], const CrtRecon &c)
+{
+    const double kq = rint((q[0] + q[1]) * c.invM);
+    return ((fma(-kq, c.Mk[0], q[0]) + fma(-kq, c.Mk[1], q[1])) + fma(-kq, c.Mk[2], q[2])) +
+           fma(-kq, c.Mk[3], q[3]);
+}
+
 __global__ void k_crt_recon(int m, int n, const int8_t *__restrict__ R, int64_t ldr,
                             int64_t plane, CrtRecon c, const int *__restrict__ ea,
                             const int *__restrict__ eb, int two_beta, double alpha, double beta,
                             int beta_zero, double *__restrict__ C, int64_t ldc)
 {
-    const int64_t total = (int64_t)m * n;
+    const int m4 = (m + 3) / 4;
+    const int64_t total = (int64_t)m4 * n;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
          t += (int64_t)gridDim.x * blockDim.x) {
-        const int i = (int)(t % m), j = (int)(t / m);
-        const int8_t *r = R + i + (int64_t)j * ldr;
-        double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+        const int i0 = 4 * (int)(t % m4), j = (int)(t / m4);
+        const int8_t *r = R + i0 + (int64_t)j * ldr;
+        double q[4][4] = {};
         for (int l = 0; l < c.L; ++l) {
-            const double rl = (double)r[l * plane];
-            q0 = fma(c.W[0][l], rl, q0);
-            q1 = fma(c.W[1][l], rl, q1);
-            q2 = fma(c.W[2][l], rl, q2);
-            q3 = fma(c.W[3][l], rl, q3);
+            const uint32_t w = gload((const uint32_t *)(r + l * plane));
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const double rl = (double)(int8_t)(w >> (8 * v));
+#pragma unroll
+                for (int k = 0; k < 4; ++k) q[v][k] = fma(c.W[k][l], rl, q[v][k]);
+            }
         }
-        const double kq = rint((q0 + q1) * c.invM);
-        const double S = ((fma(-kq, c.Mk[0], q0) + fma(-kq, c.Mk[1], q1)) + fma(-kq, c.Mk[2], q2)) +
-                         fma(-kq, c.Mk[3], q3);
-        const double v = alpha * ldexp(S, ea[i] + eb[j] - two_beta);
-        double *cp = C + i + (int64_t)j * ldc;
-        *cp = beta_zero ? v : fma(beta, *cp, v);
+        const int fj = eb[j] - two_beta;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = i0 + v;
+            if (i >= m) break;
+            const double x = alpha * ldexp(crt_float(q[v], c), ea[i] + fj);
+            double *cp = C + i + (int64_t)j * ldc;
+            *cp = beta_zero ? x : fma(beta, *cp, x);
+        }
     }
 }
 
@@ -824,43 +812,49 @@ __global__ void k_zcrt_recon(int m, int n, const int8_t *__restrict__ R, int64_t
                              double br, double bi, int beta_zero, double *__restrict__ C,
                              int64_t ldc)
 {
-    const int64_t total = (int64_t)m * n;
+    const int m4 = (m + 3) / 4;
+    const int64_t total = (int64_t)m4 * n;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
          t += (int64_t)gridDim.x * blockDim.x) {
-        const int i = (int)(t % m), j = (int)(t / m);
-        const int8_t *r = R + i + (int64_t)j * ldr;
-        double q[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+        const int i0 = 4 * (int)(t % m4), j = (int)(t / m4);
+        const int8_t *r = R + i0 + (int64_t)j * ldr;
+        double q[4][2][4] = {};
         for (int l = 0; l < c.L; ++l) {
             const int mod = md.m[l];
-            const int r1 = r[(3 * l) * plane], r2 = r[(3 * l + 1) * plane],
-                      r3 = r[(3 * l + 2) * plane];
-            const double vre = (double)smod_small(r1 - r2, mod, md.rinvf[l]);
-            const double vim = (double)smod_small(r3 - r1 - r2, mod, md.rinvf[l]);
+            const uint32_t w1 = gload((const uint32_t *)(r + (3 * l) * plane));
+            const uint32_t w2 = gload((const uint32_t *)(r + (3 * l + 1) * plane));
+            const uint32_t w3 = gload((const uint32_t *)(r + (3 * l + 2) * plane));
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                q[0][k] = fma(c.W[k][l], vre, q[0][k]);
-                q[1][k] = fma(c.W[k][l], vim, q[1][k]);
+            for (int v = 0; v < 4; ++v) {
+                const int r1 = (int8_t)(w1 >> (8 * v)), r2 = (int8_t)(w2 >> (8 * v)),
+                          r3 = (int8_t)(w3 >> (8 * v));
+                const double vre = (double)smod_small(r1 - r2, mod, md.rinvf[l]);
+                const double vim = (double)smod_small(r3 - r1 - r2, mod, md.rinvf[l]);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    q[v][0][k] = fma(c.W[k][l], vre, q[v][0][k]);
+                    q[v][1][k] = fma(c.W[k][l], vim, q[v][1][k]);
+                }
             }
         }
-        double x[2];
+        const int fj = eb[j] - two_beta;
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const double kq = rint((q[h][0] + q[h][1]) * c.invM);
-            const double S = ((fma(-kq, c.Mk[0], q[h][0]) + fma(-kq, c.Mk[1], q[h][1])) +
-                              fma(-kq, c.Mk[2], q[h][2])) +
-                             fma(-kq, c.Mk[3], q[h][3]);
-            x[h] = ldexp(S, ea[i] + eb[j] - two_beta);
-        }
-        const double tr = fma(ar, x[0], -(ai * x[1]));
-        const double ti = fma(ar, x[1], ai * x[0]);
-        double *cp = C + 2 * (i + (int64_t)j * ldc);
-        if (beta_zero) {
-            cp[0] = tr;
-            cp[1] = ti;
-        } else {
-            const double cr = cp[0], cim = cp[1];
-            cp[0] = fma(br, cr, fma(-bi, cim, tr));
-            cp[1] = fma(br, cim, fma(bi, cr, ti));
+        for (int v = 0; v < 4; ++v) {
+            const int i = i0 + v;
+            if (i >= m) break;
+            const int sh = ea[i] + fj;
+            const double x0 = ldexp(crt_float(q[v][0], c), sh), x1 = ldexp(crt_float(q[v][1], c), sh);
+            const double tr = fma(ar, x0, -(ai * x1));
+            const double ti = fma(ar, x1, ai * x0);
+            double *cp = C + 2 * (i + (int64_t)j * ldc);
+            if (beta_zero) {
+                cp[0] = tr;
+                cp[1] = ti;
+            } else {
+                const double cr = cp[0], cim = cp[1];
+                cp[0] = fma(br, cr, fma(-bi, cim, tr));
+                cp[1] = fma(br, cim, fma(bi, cr, ti));
+            }
         }
     }
 }
@@ -974,7 +968,7 @@ struct bxh_crt_mods_host {
     int m[20];
     double rinv[20];
     float rinvf[20];
-    uint8_t p2[20][64];
+    int c26[20];
 };
 struct bxh_crt_recon_host {
     int L;
@@ -994,11 +988,7 @@ static CrtMods crt_mods(int L, const int *m)
         md.m[l] = m[l];
         md.rinv[l] = 1.0 / m[l];
         md.rinvf[l] = 1.0f / (float)m[l];
-        int p = 1 % m[l];
-        for (int t = 0; t < 64; ++t) {
-            md.p2[l][t] = (uint8_t)p;
-            p = (p * 2) % m[l];
-        }
+        md.c26[l] = (int)((1u << 26) % (unsigned)m[l]);
     }
     return md;
 }
@@ -1015,8 +1005,8 @@ int bxh_crt_split(const double *X, int64_t ld, int n, int len, int contiguous, c
         hipLaunchKernelGGL(k_crt_split_contig, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
                            st, X, ld, n, len, e, beta, md, D, np, kp);
     } else {
-        hipLaunchKernelGGL(k_crt_split_strided, dim3(np / SP_T, kp / SP_T), dim3(256), 0, st, X,
-                           ld, n, len, e, beta, md, D, np, kp);
+        hipLaunchKernelGGL(k_crt_split_strided, dim3(np / 64, kp / 64), dim3(256), 0, st, X, ld,
+                           n, len, e, beta, md, D, np, kp);
     }
     hipError_t r = hipGetLastError();
     return r == hipSuccess ? 0 : oz_fail("crt split launch", r);
@@ -1057,7 +1047,7 @@ int bxh_crt_recon(int m, int n, const int8_t *R, int64_t ldr, int64_t plane, con
     if (m <= 0 || n <= 0) return 0;
     CrtRecon c;
     memcpy(&c, recon, sizeof c);
-    const int64_t total = (int64_t)m * n;
+    const int64_t total = (int64_t)((m + 3) / 4) * n; // 4 rows per thread
     int grid = (int)((total + 255) / 256);
     if (grid > 262144) grid = 262144;
     hipLaunchKernelGGL(k_crt_recon, dim3(grid), dim3(256), 0, (hipStream_t)stream, m, n, R, ldr,
@@ -1079,8 +1069,8 @@ int bxh_zcrt_split(const double *Z, int64_t ld, int n, int len, int contiguous, 
         hipLaunchKernelGGL(k_zcrt_split_contig, dim3((unsigned)((total + 255) / 256)), dim3(256),
                            0, st, Z, ld, n, len, conj, e, beta, md, D, np, kp);
     } else {
-        hipLaunchKernelGGL(k_zcrt_split_strided, dim3(np / SP_T, kp / SP_T), dim3(256), 0, st, Z,
-                           ld, n, len, conj, e, beta, md, D, np, kp);
+        hipLaunchKernelGGL(k_zcrt_split_strided, dim3(np / 64, kp / 64), dim3(256), 0, st, Z, ld,
+                           n, len, conj, e, beta, md, D, np, kp);
     }
     hipError_t r = hipGetLastError();
     return r == hipSuccess ? 0 : oz_fail("zcrt split launch", r);
@@ -1095,7 +1085,7 @@ int bxh_zcrt_recon(int m, int n, const int8_t *R, int64_t ldr, int64_t plane, co
     CrtRecon c;
     memcpy(&c, recon, sizeof c);
     const CrtMods md = crt_mods(c.L, moduli);
-    const int64_t total = (int64_t)m * n;
+    const int64_t total = (int64_t)((m + 3) / 4) * n; // 4 rows per thread
     int grid = (int)((total + 255) / 256);
     if (grid > 262144) grid = 262144;
     const int bz = beta[0] == 0.0 && beta[1] == 0.0;
@@ -1110,7 +1100,7 @@ int bxh_zcrt_recon(int m, int n, const int8_t *R, int64_t ldr, int64_t plane, co
 int bxh_crt_resid_host(double v, int m)
 {
     const CrtMods md = crt_mods(1, &m);
-    return crt_resid(v, m, md.rinv[0], md.rinvf[0], md.p2[0]);
+    return crt_resid(split26(v), m, md.rinv[0], md.rinvf[0], md.c26[0]);
 }
 int bxh_crt_smod_i32_host(int x, int m) { return smod_i32(x, m, 1.0 / m); }
 

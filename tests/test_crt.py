@@ -87,17 +87,18 @@ def test_reconstruction_constants_exact(orc, bx, L):
 
 
 def test_residue_helpers_match_exact_arithmetic(orc, bx):
-    """crt_resid on integer-valued doubles of every magnitude up to 2^115
-    (the split's |A'| < 2^beta), and smod of int32 products, against
-    Python's exact integers: the library's helpers (host build of the
-    kernel code) and the oracle's independent int64 version."""
+    """crt_resid on integer-valued doubles of every magnitude up to 2^78
+    (the split's |A'| < 2^beta, beta <= 77), and smod of int32 products,
+    against Python's exact integers: the library's helpers (host build of
+    the kernel code) and the oracle's independent int64 version."""
     Lb = bx.lib()
     Lb.bxtest_crt_resid.argtypes = [C.c_double, C.c_int]
     Lb.bxtest_crt_smod_i32.argtypes = [C.c_int, C.c_int]
     rng = random.Random(11)
-    vals = [0.0, 1.0, -1.0, 127.0, -128.0, 2.0 ** 53, -(2.0 ** 53), 2.0 ** 53 - 1, 2.0 ** 115]
+    vals = [0.0, 1.0, -1.0, 127.0, -128.0, 2.0 ** 53, -(2.0 ** 53), 2.0 ** 53 - 1, 2.0 ** 26,
+            2.0 ** 26 - 1, -(2.0 ** 26), 2.0 ** 78, -(2.0 ** 78) + 2.0 ** 30]
     for _ in range(3000):
-        e = rng.randrange(0, 116)
+        e = rng.randrange(0, 79)
         vals.append(float(math.trunc(rng.uniform(-1, 1) * 2.0 ** e)))
     for v in vals:
         iv = int(v)
