@@ -546,6 +546,24 @@ def traffic_probe(bx, n):
     print(json.dumps({"traffic_probe": {"n": n, "copy_bytes": 1 << 30}}), flush=True)
 
 
+def crt_probe(bx, n, moduli):
+    """One CRT split DGEMM launch sequence at n^3 (NN, operands generated in
+    HBM), for rocprofv3 kernel-trace and counter passes."""
+    import numpy as np
+
+    A = bx.DeviceArray(n, n, np.float64)
+    B = bx.DeviceArray(n, n, np.float64)
+    Cm = bx.DeviceArray(n, n, np.float64)
+    A.fill_uniform(0, 0)
+    B.fill_uniform(0, 1)
+    bx.synchronize()
+    bx.dgemm_crt_device("N", "N", n, n, n, 1.0, A.ptr, n, B.ptr, n, 0.0, Cm.ptr, n, moduli)
+    bx.synchronize()
+    log(f"crt probe: one CRT DGEMM {n}^3 with {moduli} moduli")
+    print(json.dumps({"crt_probe": {"n": n, "moduli": moduli,
+                                    "bits": bx.crt_bits(n, moduli)}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -559,6 +577,8 @@ def main():
     ap.add_argument("--ooc-n", type=int, default=65536, help="size of the out-of-core run")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="one DGEMM + one 1 GiB copy kernel, for rocprofv3 --pmc passes")
+    ap.add_argument("--crt-probe", type=int, default=0, metavar="MODULI",
+                    help="one CRT split DGEMM at --n with MODULI moduli, for rocprofv3 passes")
     ap.add_argument("--only", default="", help="comma list of extras to run "
                     "(host,ooc,split,zsplit,serve); default all")
     args = ap.parse_args()
@@ -575,6 +595,9 @@ def main():
     n = args.n
     if args.traffic_probe:
         traffic_probe(bx, n)
+        return
+    if args.crt_probe:
+        crt_probe(bx, n, args.crt_probe)
         return
     wall, avg_ms, flops = headline(bx, n, args.steps, args.warmup)
     value = flops * args.steps / wall / 1e12
