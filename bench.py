@@ -180,25 +180,36 @@ def host_resident(bx, n):
     }
 
 
-def out_of_core(bx, n=65536, budget_frac=0.75):
-    """DGEMM through cblas_dgemm with pinned host operands whose A+B working
-    set exceeds the tile-cache budget (budget = budget_frac * |A+B|), against
-    the same call with a budget that holds everything.  Reports throughput,
-    the ratio, bytes moved vs the lower bound e(MK + KN + 2MN) and the
-    host-link rates."""
+def out_of_core(bx, n=65536, budget_frac=0.75, host_limit=160 << 30):
+    """DGEMM through cblas_dgemm with pageable host operands whose working set
+    exceeds the tile-cache budget (budget = budget_frac * operand bytes),
+    against the same call with a budget that holds everything.  Reports
+    throughput, the ratio, bytes moved vs the lower bound e(MK + KN + 2MN) and
+    the host-link rates.
+
+    When three n x n matrices exceed `host_limit` bytes of host memory (n =
+    131072: 3 x 128 GiB), B is the same host matrix as A (C = A A + C, a
+    legal BLAS call: both operands are read-only), so the run needs 2 x 128
+    GiB; the tile cache then keys both operands' tiles on one matrix, and the
+    budgets are fractions of that one matrix."""
     import numpy as np
 
     L = bx.lib()
+    alias = 3 * n * n * 8 > host_limit
     # ordinary host memory: 3 x 32 GiB of pinned memory is more than the
     # driver grants one process here; the runtime stages pageable operands
-    log(f"out-of-core: allocating 3 x {n}^2 FP64 host matrices")
-    A, B, Cm = (bx.fill_uniform_host(np.empty((n, n), order="F"), 3, i) for i in range(3))
+    log(f"out-of-core: allocating {2 if alias else 3} x {n}^2 FP64 host matrices"
+        + (" (B aliases A)" if alias else ""))
+    A = bx.fill_uniform_host(np.empty((n, n), order="F"), 3, 0)
+    B = A if alias else bx.fill_uniform_host(np.empty((n, n), order="F"), 3, 1)
+    Cm = bx.fill_uniform_host(np.empty((n, n), order="F"), 3, 2)
+    operand_bytes = A.nbytes * (1 if alias else 2)
     flops = 2.0 * n ** 3
-    res = {"n": n}
+    res = {"n": n, "b_aliases_a": alias, "operand_bytes": operand_bytes}
     old = L.blasx_get_hbm_budget()
     try:
-        cases = (("in_core", old), ("out_of_core", int(budget_frac * 2 * A.nbytes)),
-                 ("out_of_core_below_B", int(0.4 * 2 * A.nbytes)))
+        cases = (("in_core", old), ("out_of_core", int(budget_frac * operand_bytes)),
+                 ("out_of_core_below_B", int(0.4 * operand_bytes)))
         for name, budget in cases:
             L.blasx_finalize()
             L.blasx_set_hbm_budget(budget)
@@ -564,6 +575,20 @@ def crt_probe(bx, n, moduli):
                                     "bits": bx.crt_bits(n, moduli)}}), flush=True)
 
 
+def peaks(bx):
+    """Measured MFMA and HBM ceilings of this GPU (blasx_measure_peaks)."""
+    import ctypes as C
+
+    out = (C.c_double * 4)()
+    rc = bx.lib().blasx_measure_peaks(out)
+    if rc:
+        raise RuntimeError(f"blasx_measure_peaks failed ({rc})")
+    res = {"fp64_mfma_tflops": round(out[0], 2), "fp32_mfma_tflops": round(out[1], 2),
+           "int8_mfma_tops": round(out[2], 1), "hbm_copy_GBps": round(out[3], 1)}
+    log(f"peaks: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -577,6 +602,8 @@ def main():
     ap.add_argument("--ooc-n", type=int, default=65536, help="size of the out-of-core run")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="one DGEMM + one 1 GiB copy kernel, for rocprofv3 --pmc passes")
+    ap.add_argument("--peaks", action="store_true",
+                    help="measure the MFMA and HBM ceilings of this GPU and exit")
     ap.add_argument("--crt-probe", type=int, default=0, metavar="MODULI",
                     help="one CRT split DGEMM at --n with MODULI moduli, for rocprofv3 passes")
     ap.add_argument("--only", default="", help="comma list of extras to run "
@@ -595,6 +622,9 @@ def main():
     n = args.n
     if args.traffic_probe:
         traffic_probe(bx, n)
+        return
+    if args.peaks:
+        print(json.dumps({"peaks": peaks(bx)}), flush=True)
         return
     if args.crt_probe:
         crt_probe(bx, n, args.crt_probe)

@@ -168,6 +168,7 @@ def lib():
     L.blasx_dgemm_crt_device.argtypes = [i, i, i, i, i, d, _vp, i64, _vp, i64, d, _vp, i64, i,
                                          _vp]
     L.blasx_crt_bits.argtypes = [i, i]
+    L.blasx_measure_peaks.argtypes = [C.POINTER(C.c_double)]
     L.blasx_zgemm_crt_device.argtypes = [i, i, i, i, i, _vp, _vp, i64, _vp, i64, _vp, _vp, i64,
                                          i, _vp]
     L.blasx_gemm_grouped_device.argtypes = [C.POINTER(GemmDesc), i, _vp]

@@ -355,6 +355,14 @@ BX_EXPORT int blasx_device_copy(void *dst, const void *src, size_t bytes, void *
     return bxh_copy_kernel(dst, src, bytes, stream ? stream : g_bx.s_comp);
 }
 
+int bxh_measure_peaks(int cus, size_t copy_bytes, double *out, void *stream);
+
+BX_EXPORT int blasx_measure_peaks(double *out)
+{
+    if (bx_ensure_init() != 0) return BLASX_ENOGPU;
+    return bxh_measure_peaks(g_bx.cus > 0 ? g_bx.cus : 256, (size_t)4 << 30, out, g_bx.s_comp);
+}
+
 BX_EXPORT int blasx_device_synchronize(void)
 {
     if (bx_ensure_init() != 0) return BLASX_ENOGPU;

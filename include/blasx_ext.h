@@ -97,6 +97,11 @@ int blasx_memset_device(void *dst, int value, size_t bytes);
 int blasx_device_synchronize(void);
 /* HBM -> HBM streaming copy by a kernel (16-byte aligned, multiple of 16 bytes) */
 int blasx_device_copy(void *dst, const void *src, size_t bytes, void *stream);
+/* Measured ceilings of this GPU: out[0] FP64 MFMA TFLOP/s, out[1] FP32 MFMA
+ * TFLOP/s, out[2] INT8 MFMA TOP/s (register-only MFMA loops, 16 independent
+ * accumulators per wave, best of 4/8/16 waves per CU), out[3] HBM GB/s
+ * (read + write of a 4 GiB device copy).  Takes a few hundred ms. */
+int blasx_measure_peaks(double *out);
 void *blasx_compute_stream(void);        /* hipStream_t of the library's compute stream */
 /* HIP events, for timing work on a stream (NULL = the compute stream) */
 void *blasx_event_create(void);
