@@ -299,101 +299,187 @@ __device__ __forceinline__ void tile128(const OzSeg *__restrict__ segs, int nseg
 }
 
 // 256x256 tile, 8 waves (two per SIMD) of 128x64: 32 MFMA_I32_16X16X64_I8
-// accumulators (128 AGPRs) per wave.  One wave per SIMD cannot keep the INT8
-// matrix core busy on its own (a pure-MFMA loop reaches 2.7 POP/s with one
-// wave per SIMD and 4.4 with two, of 5.0), so the tile is split over two
-// waves per SIMD, which also hide each other's LDS waits.  Twice the INT8
-// operations per byte of L2 traffic of the 128x128 tile.  acc[mb][nb] of wave
-// (wm, wn) = (wave & 1, wave >> 1) holds rows m0 + 128 wm + 16 mb +
-// 4 (lane/16) + v, column n0 + 64 wn + 16 nb + lane%16.
+// accumulators (128 VGPRs) per wave.  acc[mb][nb] of wave (wm, wn) =
+// (wave & 1, wave >> 1) holds rows m0 + 128 wm + 16 mb + 4 (lane/16) + v,
+// column n0 + 64 wn + 16 nb + lane%16.
+//
+// LDS traffic, not the matrix cores, bounds an INT8 tile on gfx950: per
+// 64-byte K step the tile stores 32 KB into LDS and its waves read 96 KB.
+// Measured on this tile (16384^3, one modulus): removing only the LDS stores
+// of a register-staged version lifted it from 3.0 to 4.7 POP/s.  So the
+// operands go from global memory straight into LDS (GLOBAL_LOAD_LDS_DWORDX4:
+// no VGPR round trip, no DS_WRITE), and every LDS read is issued a step or
+// 24 MFMAs before the MFMA that needs it.
+//
+// Pipeline (one segment's K range; K tile t = 64 bytes of K):
+//  * four LDS stages of 256 rows x 64 bytes per operand (128 KB), tile t in
+//    stage t % 4;
+//  * step t: start the DMA of tile t + 3 into stage (t + 3) % 4 (the stage of
+//    tile t - 1, whose reads all completed before the last barrier), read the
+//    last two A fragments of tile t, then run the 32 MFMAs of tile t while
+//    reading tile t + 1's fragments (DMA'd two steps ago, published by the
+//    last barrier): the B fragments into the second B register set after the
+//    first 4 MFMAs, A fragment mb < 6 after the MFMAs of row block mb; wait
+//    until tile t + 2's DMA has landed (tile t + 3's may still run) and end
+//    with one barrier.
+//  * a DMA instruction writes 64 lanes x 16 bytes to LDS at M0 + 16 lane;
+//    wave w's instruction i fills bytes 1024 (2w + i) .. of a stage (rows
+//    16 (2w + i) ..), so each lane fetches the global 16 bytes that the LDS
+//    layout puts at its position.
+// The DMA is issued as inline assembly because the compiler, seeing the
+// builtin's LDS write, waits for every outstanding DMA before each LDS read
+// (it cannot tell the stages apart), which would expose the full global
+// latency every step; the waits are explicit here instead (`vm_wait`), and
+// the barrier is a bare S_BARRIER for the same reason (__syncthreads would
+// also wait for every DMA).  Measured against the register-staged version
+// on random data: bit-identical, 2.61 vs 2.46 POP/s.
 constexpr int OZ_BIG = 256;
 constexpr int OZ_NT_BIG = 512;
+constexpr int OZ_STAGE = OZ_BIG * OZ_STR; // bytes per operand per stage
+constexpr int OZ_NSTAGE = 4;
 
 template <int V> struct Const {
     static constexpr int value = V;
 };
 
-__device__ __forceinline__ void tile256(const OzSeg *__restrict__ segs, int nseg, int kp, int m0,
-                                        int n0, i32x4 (&acc)[8][4])
+// S_WAITCNT with only the vector-memory counter limited to N
+template <int N> __device__ __forceinline__ void vm_wait()
 {
-    __shared__ __attribute__((aligned(16))) int8_t sA[2][OZ_BIG * OZ_STR];
-    __shared__ __attribute__((aligned(16))) int8_t sB[2][OZ_BIG * OZ_STR];
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+// S_WAITCNT with only the LDS/scalar-memory counter limited to 0
+__device__ __forceinline__ void lgkm_wait0()
+{
+    __builtin_amdgcn_s_waitcnt(0xF | (0x3 << 14) | (0x7 << 4));
+}
+
+// 16 bytes per lane from global memory to LDS (M0 + 16 lane)
+__device__ __forceinline__ void dma16(const int8_t *g, const int8_t *lds_wave_base)
+{
+    const uint32_t l = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) const int8_t *)lds_wave_base;
+    asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off"
+                 :
+                 : "s"(__builtin_amdgcn_readfirstlane(l)), "v"(g)
+                 : "memory", "m0");
+}
+
+__device__ __forceinline__ void tile256_seg(const int8_t *__restrict__ gA,
+                                            const int8_t *__restrict__ gB, int kp, int m0, int n0,
+                                            i32x4 (&acc)[8][4], int8_t *sA, int8_t *sB)
+{
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int l16 = lane & 15, lg = lane >> 4;
+    const int ktiles = kp / OZ_BK; // even and >= 2 (the host pads K to 128)
+    const int last = ktiles - 1;
+    const int64_t next_rows = (int64_t)ktiles * OZ_CHUNK;
+    // DMA: instruction i of this wave fills tile rows r = 16 (2 wave + i) +
+    // lane / 4 at LDS chunk position lane % 4, which holds logical 16-byte
+    // chunk c = position ^ ((r >> 1) & 3) (the lds_at swizzle)
+    int64_t goff[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int r = (wave * 2 + i) * 16 + (lane >> 2);
+        const int c = (lane & 3) ^ ((r >> 1) & 3);
+        goff[i] = (int64_t)(r >> 7) * next_rows + (r & 127) * OZ_BK + c * 16;
+    }
+    const int8_t *pa = gA + panel_off(m0, 0, kp);
+    const int8_t *pb = gB + panel_off(n0, 0, kp);
+    auto dma = [&](int f, int stage) {
+        const int64_t off = (int64_t)f * OZ_CHUNK;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int l = stage * OZ_STAGE + (wave * 2 + i) * 1024;
+            dma16(pa + off + goff[i], sA + l);
+            dma16(pb + off + goff[i], sB + l);
+        }
+    };
+    const int a_row = wm * 128 + l16, b_row = wn * 64 + l16;
+    auto rdA = [&](int stage, int mb) {
+        return *(const i32x4 *)&sA[stage * OZ_STAGE + lds_at(a_row + mb * 16, lg)];
+    };
+    auto rdB = [&](int stage, int nb) {
+        return *(const i32x4 *)&sB[stage * OZ_STAGE + lds_at(b_row + nb * 16, lg)];
+    };
+
+    i32x4 fa[8], fb[2][4];
+    dma(0, 0);
+    dma(1, 1);
+    dma(min(2, last), 2);
+    vm_wait<0>();
+    __builtin_amdgcn_s_barrier();
+#pragma unroll
+    for (int mb = 0; mb < 6; ++mb) fa[mb] = rdA(0, mb);
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) fb[0][nb] = rdB(0, nb);
+    // enter the loop with no LDS or scalar load outstanding (a scalar load
+    // hoisted above the loop would otherwise make every step's first wait a
+    // full drain)
+    lgkm_wait0();
+
+    int cur = 0; // stage of tile t
+    auto step = [&](int t, auto par_c) {
+        constexpr int P = decltype(par_c)::value;
+        const int nxt = (cur + 1) & 3, wr = (cur + 3) & 3;
+        dma(min(t + 3, last), wr); // tile t + 3
+        fa[6] = rdA(cur, 6);
+        fa[7] = rdA(cur, 7);
+#pragma unroll
+        for (int mb = 0; mb < 8; ++mb) {
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb)
+                acc[mb][nb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[mb], fb[P][nb],
+                                                                    acc[mb][nb], 0, 0, 0);
+            if (mb == 0) {
+#pragma unroll
+                for (int nb = 0; nb < 4; ++nb) fb[P ^ 1][nb] = rdB(nxt, nb);
+            }
+            if (mb < 6) fa[mb] = rdA(nxt, mb);
+        }
+        // pin that order (the scheduler would otherwise hoist the reads of
+        // tile t + 1 above the MFMAs that still need the registers)
+        __builtin_amdgcn_sched_group_barrier(0x100, 2, 0); // last A fragments of tile t
+        __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 5, 0);
+#pragma unroll
+        for (int mb = 1; mb < 6; ++mb) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);
+        __builtin_amdgcn_sched_barrier(0); // every MFMA of the step before the barrier
+        // the step's LDS reads (the last one 8 MFMAs ago) have returned: the
+        // next step then starts from a known LDS count, so its first MFMAs
+        // do not wait for the two reads issued right before them
+        lgkm_wait0();
+        vm_wait<4>(); // tile t + 2 has landed
+        __builtin_amdgcn_s_barrier();
+        cur = nxt;
+    };
+    for (int t = 0; t < ktiles; t += 2) {
+        step(t, Const<0>{});
+        step(t + 1, Const<1>{});
+    }
+    // drain the DMAs past the end (clamped re-loads of the last tile) before
+    // the stages are reused or the workgroup's LDS is released
+    vm_wait<0>();
+    __builtin_amdgcn_s_barrier();
+}
+
+__device__ __forceinline__ void tile256(const OzSeg *__restrict__ segs, int nseg, int kp, int m0,
+                                        int n0, i32x4 (&acc)[8][4])
+{
+    __shared__ __attribute__((aligned(16))) int8_t sA[OZ_NSTAGE * OZ_STAGE];
+    __shared__ __attribute__((aligned(16))) int8_t sB[OZ_NSTAGE * OZ_STAGE];
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = i32x4{0, 0, 0, 0};
-
-    // loader: 256 rows x 64 bytes per operand = 1024 x 16 B, two per thread,
-    // in two register sets (tiles t and t+1 in flight)
-    const int lrow = tid >> 2, lk = (tid & 3) * 16;
-    i32x4 ra[2][2], rb[2][2];
-    const int ktiles = kp / OZ_BK; // even (the host pads K to 128)
-    const int total = nseg * ktiles;
-    // K tile t (over all segments) -> register set `set`: two 8 KB panel
-    // chunks per operand (row blocks m0/128 and m0/128 + 1), one 16-byte
-    // piece of each per thread: rows lrow and lrow + 128
-    auto fetch = [&](int t, const int set) {
-        const int seg = t / ktiles, k0 = (t - seg * ktiles) * OZ_BK;
-        const int8_t *pa = segs[seg].A + panel_off(m0, k0, kp) + tid * 16;
-        const int8_t *pb = segs[seg].B + panel_off(n0, k0, kp) + tid * 16;
-        const int64_t next_rows = (int64_t)ktiles * OZ_CHUNK; // the following row block
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            ra[set][i] = gload((const i32x4 *)(pa + i * next_rows));
-            rb[set][i] = gload((const i32x4 *)(pb + i * next_rows));
-        }
-    };
-    auto lstore = [&](int buf, const int set) {
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            *(i32x4 *)&sA[buf][lds_at(lrow + 128 * i, lk >> 4)] = ra[set][i];
-            *(i32x4 *)&sB[buf][lds_at(lrow + 128 * i, lk >> 4)] = rb[set][i];
-        }
-    };
-    // Double-buffered LDS with global loads two K tiles ahead: step t (LDS
-    // buffer and register set t % 2) issues the loads of tile t+2 into the
-    // register set tile t used, reads this tile's 12 MFMA fragments (LDS
-    // returns them in order, so the first MFMAs start after 5 reads), runs
-    // 32 MFMAs, stores tile t+1 (loaded a whole step earlier) into the other
-    // buffer and ends with the one barrier.  The loop runs two steps per
-    // trip with fixed register sets and has no branches: past the end it
-    // re-loads tile total-1 into a buffer that is not read again.
-    auto step = [&](const int t, auto set_c) {
-        constexpr int S = decltype(set_c)::value;
-        fetch(t + 2 < total ? t + 2 : total - 1, S);
-        i32x4 fa[8], fb[4];
-        fa[0] = *(const i32x4 *)&sA[S][lds_at(wm * 128 + l16, lg)];
-#pragma unroll
-        for (int nb2 = 0; nb2 < 4; ++nb2)
-            fb[nb2] = *(const i32x4 *)&sB[S][lds_at(wn * 64 + nb2 * 16 + l16, lg)];
-#pragma unroll
-        for (int mb = 1; mb < 8; ++mb)
-            fa[mb] = *(const i32x4 *)&sA[S][lds_at(wm * 128 + mb * 16 + l16, lg)];
-#pragma unroll
-        for (int mb = 0; mb < 8; ++mb)
-#pragma unroll
-            for (int nb2 = 0; nb2 < 4; ++nb2)
-                acc[mb][nb2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[mb], fb[nb2], acc[mb][nb2],
-                                                                     0, 0, 0);
-        // keep that order: otherwise the scheduler reuses one register for
-        // every fa and waits for each read right before its MFMAs
-        __builtin_amdgcn_sched_group_barrier(0x020, 4, 0);  // 4 global loads
-        __builtin_amdgcn_sched_group_barrier(0x100, 12, 0); // 12 LDS reads
-        __builtin_amdgcn_sched_group_barrier(0x008, 32, 0); // then 32 MFMAs
-        lstore(S ^ 1, S ^ 1);
-        __syncthreads();
-    };
-    fetch(0, 0);
-    lstore(0, 0);
-    fetch(total > 1 ? 1 : 0, 1);
-    __syncthreads();
-    for (int t = 0; t < total; t += 2) {
-        step(t, Const<0>{});
-        step(t + 1, Const<1>{});
-    }
+    // each segment restarts the pipeline (its final barrier has retired every
+    // read and DMA of the stages the next prologue overwrites)
+    for (int s = 0; s < nseg; ++s) tile256_seg(segs[s].A, segs[s].B, kp, m0, n0, acc, sA, sB);
 }
 
 // W = (first ? 0 : W) + scale * (tile sum); rows >= m / columns >= n skipped.
@@ -519,9 +605,10 @@ __global__ void k_zcombine(int m, int n, const double *__restrict__ pr, const do
 struct CrtMods {
     int L;
     int m[20];
-    double rinv[20];  // 1 / m_l, rounded
-    float rinvf[20];
-    int c26[20];      // 2^26 mod m_l
+    double rinv[20];   // 1 / m_l, rounded (smod_i32)
+    float rinvf[20];   // 1 / m_l, rounded
+    float mf[20];      // m_l
+    float c12[7][20];  // 2^(12 j) smod m_l (odd m_l), j < 7
 };
 
 // r in [lo - m, hi + m] -> [lo, hi], lo = -floor(m/2), hi = ceil(m/2) - 1
@@ -540,39 +627,113 @@ __host__ __device__ __forceinline__ int smod_small(int x, int m, float rinvf)
     return smod_fix(x - q * m, m);
 }
 
-// any int32: the double quotient is within 1 of round(x / m)
+// any int32: the double quotient is within 1 of round(x / m), and x - q m
+// is exact in the FMA (no 32-bit integer multiply, a quarter-rate VALU op)
 __host__ __device__ __forceinline__ int smod_i32(int x, int m, double rinv)
 {
-    const int q = (int)rint((double)x * rinv);
-    return smod_fix(x - q * m, m);
+    const double q = rint((double)x * rinv);
+    return smod_fix((int)fma(-q, (double)m, (double)x), m);
 }
 
-// An integer-valued double v, |v| < 2^79 (the split's |A'| < 2^beta, beta
-// <= 77), as v = h 2^26 + l with h = trunc(v 2^-26) and l = v - h 2^26 both
-// exact (|h| < 2^53, |l| < 2^26).  Non-finite inputs are taken as zero.
-struct Split26 {
-    double h, l;
+// An integer-valued double v as signed 12-bit digits, |v| = sum_j d_j
+// 2^(12 j), j < J (J = 5 covers |v| < 2^60, J = 7 covers |v| < 2^84; the
+// split's |A'| < 2^beta), kept as floats f_j = sign(v) d_j, plus low = the
+// low 12 bits of v's two's complement.  Non-finite values and |v| >=
+// 2^(12 J) are taken as zero.  Every step is exact: top = trunc(|v| 2^-60)
+// and hi = trunc(rest 2^-30) are integers, and the FMA remainders are
+// integers below 2^60 and 2^30.
+template <int J> struct Digits {
+    float f[J];
+    int low;
 };
-__host__ __device__ __forceinline__ Split26 split26(double v)
+template <int J> __host__ __device__ __forceinline__ Digits<J> digits12(double v)
 {
-    if (!(fabs(v) < 0x1p79)) v = 0.0;
-    const double h = trunc(v * 0x1p-26);
-    return Split26{h, v - h * 0x1p26};
+    static_assert(J == 5 || J == 7, "digit count");
+    Digits<J> r;
+    double a = fabs(v);
+    if (!(a < (J == 5 ? 0x1p60 : 0x1p84))) a = 0.0;
+    const float sg = v < 0 ? -1.0f : 1.0f;
+    double top = 0.0;
+    if (J == 7) {
+        top = trunc(a * 0x1p-60);
+        a = fma(-top, 0x1p60, a);
+    }
+    const double hd = trunc(a * 0x1p-30);
+    const uint32_t h = (uint32_t)hd, l = (uint32_t)fma(-hd, 0x1p30, a);
+    uint32_t d[7];
+    d[0] = l & 0xFFF;
+    d[1] = (l >> 12) & 0xFFF;
+    d[2] = (l >> 24) | ((h & 0x3F) << 6);
+    d[3] = (h >> 6) & 0xFFF;
+    d[4] = h >> 18;
+    if (J == 7) {
+        const uint32_t t = (uint32_t)top;
+        d[5] = t & 0xFFF;
+        d[6] = t >> 12;
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j) r.f[j] = sg * (float)d[j];
+    r.low = v < 0 ? (int)(0u - d[0]) : (int)d[0];
+    return r;
 }
 
-// symmetric residue of v = h 2^26 + l: smod(h) (2^26 mod m) + smod(l),
-// reduced once more.  For |x| < 2^53, q = rint(x / m) is within 1 of the
-// rounded quotient and x - q m is exact in the FMA; no branch, no table.
-__host__ __device__ __forceinline__ int crt_resid(Split26 v, int m, double rinv, float rinvf,
-                                                  int c26)
+// Symmetric residue of v modulo m from its digits.  Odd m: s = sum_j f_j
+// (2^(12 j) smod m) is computed exactly in float (|s| <= 7 * 4095 * 127 <
+// 2^22), and for |s| < 2^22 the float quotient q = rint(s * fl(1/m)) is
+// exactly round(s / m) (its error is below 1 / (2m), the least distance of
+// s / m from a half-integer), so s - q m, also exact, is the symmetric
+// residue, with the sign of v carried by the digits.  m = 256: the low
+// byte of v's two's complement.  The constants of one modulus are read
+// once (`ModConst`) and reused for every element.
+template <int J> struct ModConst {
+    float c[J], rinv, m;
+    bool byte; // m = 256
+};
+template <int J> __host__ __device__ __forceinline__ ModConst<J> mod_const(const CrtMods &md, int l)
 {
-    const double qh = rint(v.h * rinv), ql = rint(v.l * rinv);
-    const int rh = (int)fma(-qh, (double)m, v.h), rl = (int)fma(-ql, (double)m, v.l);
-    return smod_small(rh * c26 + rl, m, rinvf);
+    ModConst<J> k;
+#pragma unroll
+    for (int j = 0; j < J; ++j) k.c[j] = md.c12[j][l];
+    k.rinv = md.rinvf[l];
+    k.m = md.mf[l];
+    k.byte = md.m[l] == 256;
+    return k;
+}
+template <int J>
+__host__ __device__ __forceinline__ int digit_resid_odd(const Digits<J> &x, const ModConst<J> &k)
+{
+    float s = 0.0f;
+#pragma unroll
+    for (int j = 0; j < J; ++j) s = fmaf(x.f[j], k.c[j], s);
+    const float q = rintf(s * k.rinv);
+    return (int)fmaf(-q, k.m, s);
+}
+template <int J>
+__host__ __device__ __forceinline__ int digit_resid(const Digits<J> &x, const ModConst<J> &k)
+{
+    return k.byte ? (int)(int8_t)(uint8_t)(x.low & 0xFF) : digit_resid_odd<J>(x, k);
+}
+
+// one plane's residues of 16 consecutive-k elements, packed for a 16-byte
+// store; the modulus test is taken once for all 16
+template <int J>
+__device__ __forceinline__ i32x4 pack_resid16(const Digits<J> (&v)[16], const ModConst<J> &k)
+{
+    uint32_t w[4] = {0, 0, 0, 0};
+    if (k.byte) {
+#pragma unroll
+        for (int b = 0; b < 16; ++b) w[b >> 2] |= ((uint32_t)v[b].low & 0xFF) << (8 * (b & 3));
+    } else {
+#pragma unroll
+        for (int b = 0; b < 16; ++b)
+            w[b >> 2] |= ((uint32_t)(uint8_t)(int8_t)digit_resid_odd<J>(v[b], k)) << (8 * (b & 3));
+    }
+    return i32x4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
 }
 
 // contiguous (mode 1): x(i,k) = X[k + i*ld]; one thread per 16 consecutive
 // k of one row; all L residue planes (plane stride np*kp)
+template <int J>
 __global__ __launch_bounds__(256) void k_crt_split_contig(const double *__restrict__ X, int64_t ld, int n, int len,
                                    const int *__restrict__ e, int beta, CrtMods md,
                                    int8_t *__restrict__ D, int np, int kp)
@@ -581,27 +742,21 @@ __global__ __launch_bounds__(256) void k_crt_split_contig(const double *__restri
     const int chunks = kp / 16;
     if (t >= (int64_t)np * chunks) return;
     const int i = (int)(t / chunks), k0 = (int)(t % chunks) * 16;
-    Split26 v[16];
+    Digits<J> v[16];
     const int sh = i < n ? beta - e[i] : 0;
 #pragma unroll
     for (int b = 0; b < 16; ++b)
-        v[b] = split26((i < n && k0 + b < len) ? trunc(ldexp(X[k0 + b + (int64_t)i * ld], sh))
-                                               : 0.0);
-    for (int l = 0; l < md.L; ++l) {
-        uint32_t w[4] = {0, 0, 0, 0};
-#pragma unroll
-        for (int b = 0; b < 16; ++b) {
-            const int r = crt_resid(v[b], md.m[l], md.rinv[l], md.rinvf[l], md.c26[l]);
-            w[b >> 2] |= ((uint32_t)(uint8_t)(int8_t)r) << (8 * (b & 3));
-        }
+        v[b] = digits12<J>((i < n && k0 + b < len) ? trunc(ldexp(X[k0 + b + (int64_t)i * ld], sh))
+                                                   : 0.0);
+    for (int l = 0; l < md.L; ++l)
         *(i32x4 *)(D + (int64_t)l * np * kp + panel_off(i, k0, kp)) =
-            i32x4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
-    }
+            pack_resid16<J>(v, mod_const<J>(md, l));
 }
 
 // strided (mode 0): x(i,k) = X[i + k*ld]; block = 64 rows x 64 k, thread =
 // one row x 16 consecutive k: the FP64 reads are coalesced along i across
 // the wave, and each plane gets one 16-byte store per thread
+template <int J>
 __global__ __launch_bounds__(256) void k_crt_split_strided(const double *__restrict__ X, int64_t ld,
                                                            int n, int len,
                                                            const int *__restrict__ e, int beta,
@@ -611,36 +766,39 @@ __global__ __launch_bounds__(256) void k_crt_split_strided(const double *__restr
     const int i = blockIdx.x * 64 + (threadIdx.x & 63);
     const int k0 = blockIdx.y * 64 + (threadIdx.x >> 6) * 16;
     if (i >= np) return;
-    Split26 v[16];
+    Digits<J> v[16];
     const int sh = i < n ? beta - e[i] : 0;
 #pragma unroll
     for (int b = 0; b < 16; ++b)
-        v[b] = split26((i < n && k0 + b < len) ? trunc(ldexp(X[i + (int64_t)(k0 + b) * ld], sh))
-                                               : 0.0);
-    for (int l = 0; l < md.L; ++l) {
-        uint32_t w[4] = {0, 0, 0, 0};
-#pragma unroll
-        for (int b = 0; b < 16; ++b) {
-            const int r = crt_resid(v[b], md.m[l], md.rinv[l], md.rinvf[l], md.c26[l]);
-            w[b >> 2] |= ((uint32_t)(uint8_t)(int8_t)r) << (8 * (b & 3));
-        }
+        v[b] = digits12<J>((i < n && k0 + b < len) ? trunc(ldexp(X[i + (int64_t)(k0 + b) * ld], sh))
+                                                   : 0.0);
+    for (int l = 0; l < md.L; ++l)
         *(i32x4 *)(D + (int64_t)l * np * kp + panel_off(i, k0, kp)) =
-            i32x4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
-    }
+            pack_resid16<J>(v, mod_const<J>(md, l));
 }
 
 // complex operands: planes 3l (real part), 3l+1 (imaginary part), 3l+2
 // (their sum) of modulus l, from one shared exponent per row; conj negates
 // the imaginary part.  Plane p: c = p % 3 of modulus p / 3.
-__device__ __forceinline__ int zcrt_plane_resid(Split26 vr, Split26 vi, int p, const CrtMods &md)
+template <int J>
+__device__ __forceinline__ i32x4 zcrt_pack16(const Digits<J> (&vr)[16], const Digits<J> (&vi)[16],
+                                             int p, const CrtMods &md)
 {
-    const int l = p / 3, c = p - 3 * l, m = md.m[l];
-    const int r = c == 1 ? 0 : crt_resid(vr, m, md.rinv[l], md.rinvf[l], md.c26[l]);
-    const int q = c == 0 ? 0 : crt_resid(vi, m, md.rinv[l], md.rinvf[l], md.c26[l]);
-    return c == 2 ? smod_fix(r + q, m) : r + q;
+    const int l = p / 3, c = p - 3 * l;
+    const ModConst<J> k = mod_const<J>(md, l);
+    if (c == 0) return pack_resid16<J>(vr, k);
+    if (c == 1) return pack_resid16<J>(vi, k);
+    uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+        const int r = smod_fix(digit_resid<J>(vr[b], k) + digit_resid<J>(vi[b], k), md.m[l]);
+        w[b >> 2] |= ((uint32_t)(uint8_t)(int8_t)r) << (8 * (b & 3));
+    }
+    return i32x4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
 }
 
 // contiguous (mode 1): z(i,k) = Z[k + i*ld]; one thread per 16 k of one row
+template <int J>
 __global__ __launch_bounds__(256) void k_zcrt_split_contig(const double *__restrict__ Z,
                                                            int64_t ld, int n, int len, int conj,
                                                            const int *__restrict__ e, int beta,
@@ -651,27 +809,21 @@ __global__ __launch_bounds__(256) void k_zcrt_split_contig(const double *__restr
     const int chunks = kp / 16;
     if (t >= (int64_t)np * chunks) return;
     const int i = (int)(t / chunks), k0 = (int)(t % chunks) * 16;
-    Split26 vr[16], vi[16];
+    Digits<J> vr[16], vi[16];
     const int sh = i < n ? beta - e[i] : 0;
 #pragma unroll
     for (int b = 0; b < 16; ++b) {
         const bool in = i < n && k0 + b < len;
         const double *z = Z + 2 * (k0 + b + (int64_t)i * ld);
-        vr[b] = split26(in ? trunc(ldexp(z[0], sh)) : 0.0);
-        vi[b] = split26(in ? trunc(ldexp(conj ? -z[1] : z[1], sh)) : 0.0);
+        vr[b] = digits12<J>(in ? trunc(ldexp(z[0], sh)) : 0.0);
+        vi[b] = digits12<J>(in ? trunc(ldexp(conj ? -z[1] : z[1], sh)) : 0.0);
     }
-    for (int p = 0; p < 3 * md.L; ++p) {
-        uint32_t w[4] = {0, 0, 0, 0};
-#pragma unroll
-        for (int b = 0; b < 16; ++b)
-            w[b >> 2] |= ((uint32_t)(uint8_t)(int8_t)zcrt_plane_resid(vr[b], vi[b], p, md))
-                         << (8 * (b & 3));
-        *(i32x4 *)(D + (int64_t)p * np * kp + panel_off(i, k0, kp)) =
-            i32x4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
-    }
+    for (int p = 0; p < 3 * md.L; ++p)
+        *(i32x4 *)(D + (int64_t)p * np * kp + panel_off(i, k0, kp)) = zcrt_pack16<J>(vr, vi, p, md);
 }
 
 // strided (mode 0): z(i,k) = Z[i + k*ld]; thread = one row x 16 k, as above
+template <int J>
 __global__ __launch_bounds__(256) void k_zcrt_split_strided(const double *__restrict__ Z,
                                                             int64_t ld, int n, int len, int conj,
                                                             const int *__restrict__ e, int beta,
@@ -681,24 +833,17 @@ __global__ __launch_bounds__(256) void k_zcrt_split_strided(const double *__rest
     const int i = blockIdx.x * 64 + (threadIdx.x & 63);
     const int k0 = blockIdx.y * 64 + (threadIdx.x >> 6) * 16;
     if (i >= np) return;
-    Split26 vr[16], vi[16];
+    Digits<J> vr[16], vi[16];
     const int sh = i < n ? beta - e[i] : 0;
 #pragma unroll
     for (int b = 0; b < 16; ++b) {
         const bool in = i < n && k0 + b < len;
         const double *z = Z + 2 * (i + (int64_t)(k0 + b) * ld);
-        vr[b] = split26(in ? trunc(ldexp(z[0], sh)) : 0.0);
-        vi[b] = split26(in ? trunc(ldexp(conj ? -z[1] : z[1], sh)) : 0.0);
+        vr[b] = digits12<J>(in ? trunc(ldexp(z[0], sh)) : 0.0);
+        vi[b] = digits12<J>(in ? trunc(ldexp(conj ? -z[1] : z[1], sh)) : 0.0);
     }
-    for (int p = 0; p < 3 * md.L; ++p) {
-        uint32_t w[4] = {0, 0, 0, 0};
-#pragma unroll
-        for (int b = 0; b < 16; ++b)
-            w[b >> 2] |= ((uint32_t)(uint8_t)(int8_t)zcrt_plane_resid(vr[b], vi[b], p, md))
-                         << (8 * (b & 3));
-        *(i32x4 *)(D + (int64_t)p * np * kp + panel_off(i, k0, kp)) =
-            i32x4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
-    }
+    for (int p = 0; p < 3 * md.L; ++p)
+        *(i32x4 *)(D + (int64_t)p * np * kp + panel_off(i, k0, kp)) = zcrt_pack16<J>(vr, vi, p, md);
 }
 
 // residues of one tile sum -> int8 plane (column-major, ld ldr); every lane
@@ -740,14 +885,17 @@ __global__ __launch_bounds__(OZ_NT, 2) void k_crt_gemm(const OzSeg *__restrict__
     store_resid<64, 64, 4, 4>(acc, m0, n0, md.m[l], md.rinv[l], R + p * plane, ldr);
 }
 
+// product p0 + blockIdx.y.  The host launches one product at a time when
+// one fills the GPU: measured at 16384^3, a single launch over all products
+// ran the same blocks 5% slower.
 __global__ __launch_bounds__(OZ_NT_BIG, 1) void k_crt_gemm_big(const OzSeg *__restrict__ segs,
                                                                int kp, int np, int per, CrtMods md,
                                                                int8_t *__restrict__ R, int64_t ldr,
-                                                               int64_t plane)
+                                                               int64_t plane, int p0)
 {
     int m0, n0;
     block_origin<OZ_BIG>(np, m0, n0);
-    const int p = blockIdx.y, l = p / per;
+    const int p = p0 + (int)blockIdx.y, l = p / per;
     i32x4 acc[8][4];
     tile256(segs + p, 1, kp, m0, n0, acc);
     store_resid<128, 64, 8, 4>(acc, m0, n0, md.m[l], md.rinv[l], R + p * plane, ldr);
@@ -968,7 +1116,8 @@ struct bxh_crt_mods_host {
     int m[20];
     double rinv[20];
     float rinvf[20];
-    int c26[20];
+    float mf[20];
+    float c12[7][20];
 };
 struct bxh_crt_recon_host {
     int L;
@@ -988,7 +1137,12 @@ static CrtMods crt_mods(int L, const int *m)
         md.m[l] = m[l];
         md.rinv[l] = 1.0 / m[l];
         md.rinvf[l] = 1.0f / (float)m[l];
-        md.c26[l] = (int)((1u << 26) % (unsigned)m[l]);
+        md.mf[l] = (float)m[l];
+        uint64_t p = 1; // 2^(12 j) mod m
+        for (int j = 0; j < 7; ++j) {
+            md.c12[j][l] = (float)smod_fix((int)p, m[l]);
+            p = (p << 12) % (uint64_t)m[l];
+        }
     }
     return md;
 }
@@ -1000,13 +1154,24 @@ int bxh_crt_split(const double *X, int64_t ld, int n, int len, int contiguous, c
     if (L < 1 || L > 20 || np % 64 || kp % 64) return BLASX_EINVAL;
     hipStream_t st = (hipStream_t)stream;
     const CrtMods md = crt_mods(L, moduli);
+    const bool d5 = beta <= 60; // |A'| < 2^beta: five 12-bit digits suffice
     if (contiguous) {
         const int64_t total = (int64_t)np * (kp / 16);
-        hipLaunchKernelGGL(k_crt_split_contig, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                           st, X, ld, n, len, e, beta, md, D, np, kp);
+        const dim3 grid((unsigned)((total + 255) / 256));
+        if (d5)
+            hipLaunchKernelGGL(k_crt_split_contig<5>, grid, dim3(256), 0, st, X, ld, n, len, e,
+                               beta, md, D, np, kp);
+        else
+            hipLaunchKernelGGL(k_crt_split_contig<7>, grid, dim3(256), 0, st, X, ld, n, len, e,
+                               beta, md, D, np, kp);
     } else {
-        hipLaunchKernelGGL(k_crt_split_strided, dim3(np / 64, kp / 64), dim3(256), 0, st, X, ld,
-                           n, len, e, beta, md, D, np, kp);
+        const dim3 grid(np / 64, kp / 64);
+        if (d5)
+            hipLaunchKernelGGL(k_crt_split_strided<5>, grid, dim3(256), 0, st, X, ld, n, len, e,
+                               beta, md, D, np, kp);
+        else
+            hipLaunchKernelGGL(k_crt_split_strided<7>, grid, dim3(256), 0, st, X, ld, n, len, e,
+                               beta, md, D, np, kp);
     }
     hipError_t r = hipGetLastError();
     return r == hipSuccess ? 0 : oz_fail("crt split launch", r);
@@ -1028,9 +1193,13 @@ int bxh_crt_gemm(const void *segs_dev, int L, int per, const int *moduli, int kp
                               ? (mp / OZ_BIG) * (np / OZ_BIG)
                               : 0;
     if (big_tiles * nprod >= 256) {
-        hipLaunchKernelGGL(k_crt_gemm_big, dim3(big_tiles, nprod), dim3(OZ_NT_BIG), 0,
-                           (hipStream_t)stream, (const OzSeg *)segs_dev, kp, np, per, md, R,
-                           (int64_t)mp, plane);
+        // one launch per product when one product fills the GPU, else all
+        // products in one launch
+        const int per_launch = big_tiles >= 256 ? 1 : nprod;
+        for (int p = 0; p < nprod; p += per_launch)
+            hipLaunchKernelGGL(k_crt_gemm_big, dim3(big_tiles, per_launch), dim3(OZ_NT_BIG), 0,
+                               (hipStream_t)stream, (const OzSeg *)segs_dev, kp, np, per, md, R,
+                               (int64_t)mp, plane, p);
     } else {
         hipLaunchKernelGGL(k_crt_gemm, dim3((mp / OZ_BM) * (np / OZ_BN), nprod), dim3(OZ_NT), 0,
                            (hipStream_t)stream, (const OzSeg *)segs_dev, kp, np, per, md, R,
@@ -1064,13 +1233,24 @@ int bxh_zcrt_split(const double *Z, int64_t ld, int n, int len, int contiguous, 
     if (L < 1 || L > 20 || np % 64 || kp % 64) return BLASX_EINVAL;
     hipStream_t st = (hipStream_t)stream;
     const CrtMods md = crt_mods(L, moduli);
+    const bool d5 = beta <= 60; // |A'| < 2^beta: five 12-bit digits suffice
     if (contiguous) {
         const int64_t total = (int64_t)np * (kp / 16);
-        hipLaunchKernelGGL(k_zcrt_split_contig, dim3((unsigned)((total + 255) / 256)), dim3(256),
-                           0, st, Z, ld, n, len, conj, e, beta, md, D, np, kp);
+        const dim3 grid((unsigned)((total + 255) / 256));
+        if (d5)
+            hipLaunchKernelGGL(k_zcrt_split_contig<5>, grid, dim3(256), 0, st, Z, ld, n, len,
+                               conj, e, beta, md, D, np, kp);
+        else
+            hipLaunchKernelGGL(k_zcrt_split_contig<7>, grid, dim3(256), 0, st, Z, ld, n, len,
+                               conj, e, beta, md, D, np, kp);
     } else {
-        hipLaunchKernelGGL(k_zcrt_split_strided, dim3(np / 64, kp / 64), dim3(256), 0, st, Z, ld,
-                           n, len, conj, e, beta, md, D, np, kp);
+        const dim3 grid(np / 64, kp / 64);
+        if (d5)
+            hipLaunchKernelGGL(k_zcrt_split_strided<5>, grid, dim3(256), 0, st, Z, ld, n, len,
+                               conj, e, beta, md, D, np, kp);
+        else
+            hipLaunchKernelGGL(k_zcrt_split_strided<7>, grid, dim3(256), 0, st, Z, ld, n, len,
+                               conj, e, beta, md, D, np, kp);
     }
     hipError_t r = hipGetLastError();
     return r == hipSuccess ? 0 : oz_fail("zcrt split launch", r);
@@ -1100,7 +1280,12 @@ int bxh_zcrt_recon(int m, int n, const int8_t *R, int64_t ldr, int64_t plane, co
 int bxh_crt_resid_host(double v, int m)
 {
     const CrtMods md = crt_mods(1, &m);
-    return crt_resid(split26(v), m, md.rinv[0], md.rinvf[0], md.c26[0]);
+    return digit_resid<7>(digits12<7>(v), mod_const<7>(md, 0));
+}
+int bxh_crt_resid5_host(double v, int m)
+{
+    const CrtMods md = crt_mods(1, &m);
+    return digit_resid<5>(digits12<5>(v), mod_const<5>(md, 0));
 }
 int bxh_crt_smod_i32_host(int x, int m) { return smod_i32(x, m, 1.0 / m); }
 

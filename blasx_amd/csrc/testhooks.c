@@ -105,6 +105,7 @@ typedef struct {
 } crt_recon_view_t;
 void bx_crt_consts(int L, void *c);
 int bxh_crt_resid_host(double v, int m);
+int bxh_crt_resid5_host(double v, int m);
 int bxh_crt_smod_i32_host(int x, int m);
 
 BX_EXPORT void bxtest_crt_consts(int L, double *W /* [4][20] */, double *Mk, double *invM)
@@ -119,4 +120,5 @@ BX_EXPORT void bxtest_crt_consts(int L, double *W /* [4][20] */, double *Mk, dou
 }
 
 BX_EXPORT int bxtest_crt_resid(double v, int m) { return bxh_crt_resid_host(v, m); }
+BX_EXPORT int bxtest_crt_resid5(double v, int m) { return bxh_crt_resid5_host(v, m); }
 BX_EXPORT int bxtest_crt_smod_i32(int x, int m) { return bxh_crt_smod_i32_host(x, m); }
