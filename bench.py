@@ -246,7 +246,7 @@ def _errors(c, ref, scale):
 
 
 def split_precision(bx, n=16384, phis=(0.5, 2.0, 4.0), slices=(6, 7, 8),
-                    moduli=(14, 16, 18, 20), blocks=2):
+                    moduli=(14, 15, 16, 17, 18, 20), blocks=2):
     """Split-precision vs plain FP64 DGEMM on inputs a = (u-0.5) exp(phi g):
     effective TFLOP/s (2n^3 / time) and errors against a double-double
     reference on sampled 256x256 blocks of C, for the digit split (s slices,
@@ -402,7 +402,7 @@ def zsplit_precision(bx, n=8192, phi=0.5, slices=(6, 7), blocks=2):
                         "int8_products": 4 * s * (s + 1) // 2, **errors()}
         log(f"zsplit: s={s} {res[f's{s}']}")
     L0 = next(L for L in range(2, 21) if bx.crt_bits(2 * n, L) >= 56)
-    for L in (L0 - 2, L0, L0 + 2):
+    for L in range(L0 - 3, L0 + 1):
         t = timed(lambda: bx.zgemm_crt_device("N", "N", n, n, n, 1.0, dA.ptr, n, dB.ptr, n, 0.0,
                                               dC.ptr, n, L))
         res[f"crt{L}"] = {"tflops_effective": round(flops / t / 1e12, 3), "int8_products": 3 * L,

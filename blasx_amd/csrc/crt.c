@@ -3,10 +3,16 @@
  * the reference; the method and its exactness argument are stated in
  * hip/ozaki_kernels.hip.
  *
- * One INT8 GEMM per modulus instead of one per digit pair: L moduli keep
- * beta = floor((log2 M - log2 K - 3) / 2) bits of every entry (M = product
- * of the moduli), e.g. K = 16384: L = 16 -> 54 bits, L = 18 -> 61 bits,
- * where the digit scheme needs s(s+1)/2 = 36 products for 56 bits.
+ * One INT8 GEMM per modulus instead of one per digit pair.  Rows of op(A)
+ * and columns of op(B) are scaled by their 2-norms (Cauchy-Schwarz: |sum_k
+ * A'_ik B'_kj| <= ||A'_i|| ||B'_j|| < 2^(2 gamma) <= M / 8, gamma =
+ * floor((log2 M - 3) / 2), M = product of the moduli), so every entry keeps
+ * gamma bits relative to its row's 2-norm.  In the worst case (all entries
+ * of a row equal in magnitude) that is beta = floor((log2 M - log2 K - 3) /
+ * 2) bits relative to the row maximum (blasx_crt_bits), e.g. K = 16384:
+ * L = 16 -> 54 bits, L = 18 -> 61 bits, where the digit scheme needs
+ * s(s+1)/2 = 36 products for 56 bits; rows whose RMS is below their maximum
+ * keep log2(max / RMS) bits more.
  *
  * Workspace (HBM, grown on demand and kept): L residue planes of op(A)
  * (Mp x Kp) and of op(B)^T (Np x Kp), L int8 planes of the residue products
@@ -23,14 +29,14 @@
 int bxh_oz_block_m(void);
 int bxh_oz_block_k(void);
 int bxh_oz_exp_chunks(void);
-int bxh_oz_exponents(const double *X, int64_t ld, int n, int len, int contiguous, int *e,
-                     double *part, void *stream);
+int bxh_oz_exponents(const double *X, int64_t ld, int n, int len, int contiguous, int norm,
+                     int *e, double *part, void *stream);
 int bxh_crt_split(const double *X, int64_t ld, int n, int len, int contiguous, const int *e,
                   int beta, int L, const int *moduli, int8_t *D, int np, int kp, void *stream);
 int bxh_crt_gemm(const void *segs_dev, int L, int per, const int *moduli, int kp, int mp, int np,
                  int8_t *R, void *stream);
-int bxh_oz_exponents_z(const double *Z, int64_t ld, int n, int len, int contiguous, int *e,
-                       double *part, void *stream);
+int bxh_oz_exponents_z(const double *Z, int64_t ld, int n, int len, int contiguous, int norm,
+                       int *e, double *part, void *stream);
 int bxh_zcrt_split(const double *Z, int64_t ld, int n, int len, int contiguous, int conj,
                    const int *e, int beta, int L, const int *moduli, int8_t *D, int np, int kp,
                    void *stream);
@@ -58,6 +64,15 @@ typedef struct {
     double invM;
 } crt_recon_t;
 
+/* scaling bits relative to the row 2-norm */
+int bx_crt_gamma(int L)
+{
+    double lm = 0.0;
+    for (int l = 0; l < L; ++l) lm += log2((double)CRT_MODS[l]);
+    return (int)floor((lm - 3.0) / 2.0 - 0x1p-20);
+}
+
+/* worst-case bits relative to the row maximum */
 int bx_crt_beta(int K, int L)
 {
     double lm = 0.0;
@@ -182,8 +197,7 @@ int bx_dgemm_crt(int opa, int opb, int M, int N, int K, double alpha, const doub
         return bxh_scale(BLASX_D, M, N, b2, C, ldc, stream);
     }
     if (K > CRT_MAX_K) return BLASX_ERANGE;
-    const int bb = bx_crt_beta(K, L);
-    if (bb < 1) return BLASX_EINVAL; /* too few moduli for this K */
+    const int bb = bx_crt_gamma(L);
     const int pad = (M >= 4096 && N >= 4096) ? 2 * bxh_oz_block_m() : bxh_oz_block_m();
     const int mp = (int)rup(M, pad), np = (int)rup(N, pad);
     const int kp = (int)rup(K, 2 * bxh_oz_block_k()); /* an even number of K tiles */
@@ -226,8 +240,8 @@ int bx_dgemm_crt(int opa, int opb, int M, int N, int K, double alpha, const doub
 
     const int a_contig = opa != BX_OP_N; /* A^T: row i of op(A) is column i of A */
     const int b_contig = opb == BX_OP_N; /* B: column j of op(B) is column j of B */
-    rc = bxh_oz_exponents(A, lda, M, K, a_contig, ea, part, stream);
-    if (!rc) rc = bxh_oz_exponents(B, ldb, N, K, b_contig, eb, part, stream);
+    rc = bxh_oz_exponents(A, lda, M, K, a_contig, 1, ea, part, stream);
+    if (!rc) rc = bxh_oz_exponents(B, ldb, N, K, b_contig, 1, eb, part, stream);
     if (!rc) rc = bxh_crt_split(A, lda, M, K, a_contig, ea, bb, L, CRT_MODS, DA, mp, kp, stream);
     if (!rc) rc = bxh_crt_split(B, ldb, N, K, b_contig, eb, bb, L, CRT_MODS, DB, np, kp, stream);
     const int8_t **tab = (const int8_t **)g_crt.segs_host;
@@ -251,7 +265,7 @@ int bx_dgemm_crt(int opa, int opb, int M, int N, int K, double alpha, const doub
 }
 
 /* complex: 3 products per modulus (Karatsuba, exact in residues), see
- * hip/ozaki_kernels.hip; beta from 2K since |Re|, |Im| < 2K 2^(2 beta) */
+ * hip/ozaki_kernels.hip; the worst-case bits count 2K real terms */
 int bx_crt_default_moduli_z(int K)
 {
     for (int L = 2; L <= CRT_MAX_L; ++L)
@@ -267,8 +281,8 @@ int bx_zgemm_crt(int opa, int opb, int M, int N, int K, const double *alpha, con
     if (K == 0 || (alpha[0] == 0.0 && alpha[1] == 0.0))
         return bxh_scale(BLASX_Z, M, N, beta, C, ldc, stream);
     if (K > CRT_MAX_K) return BLASX_ERANGE;
-    const int bb = bx_crt_beta(2 * K, L);
-    if (bb < 1) return BLASX_EINVAL;
+    /* |Re P|, |Im P| <= ||a_i|| ||b_j|| over the 2K real parts */
+    const int bb = bx_crt_gamma(L);
     const int pad = (M >= 4096 && N >= 4096) ? 2 * bxh_oz_block_m() : bxh_oz_block_m();
     const int mp = (int)rup(M, pad), np = (int)rup(N, pad);
     const int kp = (int)rup(K, 2 * bxh_oz_block_k()); /* an even number of K tiles */
@@ -310,8 +324,8 @@ int bx_zgemm_crt(int opa, int opb, int M, int N, int K, const double *alpha, con
     double *part = (double *)(base + off_part);
 
     const int a_contig = opa != BX_OP_N, b_contig = opb == BX_OP_N;
-    rc = bxh_oz_exponents_z(A, lda, M, K, a_contig, ea, part, stream);
-    if (!rc) rc = bxh_oz_exponents_z(B, ldb, N, K, b_contig, eb, part, stream);
+    rc = bxh_oz_exponents_z(A, lda, M, K, a_contig, 1, ea, part, stream);
+    if (!rc) rc = bxh_oz_exponents_z(B, ldb, N, K, b_contig, 1, eb, part, stream);
     if (!rc)
         rc = bxh_zcrt_split(A, lda, M, K, a_contig, opa == BX_OP_C, ea, bb, L, CRT_MODS, DA, mp,
                             kp, stream);

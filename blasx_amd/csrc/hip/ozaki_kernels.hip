@@ -55,28 +55,52 @@ __host__ __device__ __forceinline__ int64_t panel_off(int i, int k, int kp)
 }
 
 // ---------------------------------------------------------------------------
-// 1. exponents: e[i] = smallest integer with max_k |x(i,k)| * 2^-e < 1
+// 1. exponents, two kinds (NORM = 0 / 1):
+//    max:  e[i] = smallest integer with max_k |x(i,k)| * 2^-e < 1
+//          (the digit scheme: every scaled entry in (-1, 1));
+//    norm: e[i] = smallest integer with ss_i (1 + 2^-30) < 4^e, ss_i =
+//          sum_k x(i,k)^2 as computed (the CRT scheme's Cauchy-Schwarz
+//          scaling: ||x_i||_2 < 2^e, so every entry too; the factor covers
+//          the rounding of the computed sum for K < 2^22).  From the
+//          exponent k of ss_i (1 + 2^-30) (frexp: below 2^k, at least
+//          2^(k-1)), e = ceil(k / 2), with no square root.
 //    mode 0: x(i,k) = X[i + k*ld]  (reduce along the strided dimension:
 //            one thread per i, coalesced across threads)
 //    mode 1: x(i,k) = X[k + i*ld]  (reduce along the contiguous dimension:
 //            one block per i)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int exp_of(double amax)
+__host__ __device__ __forceinline__ int exp_of(double amax)
 {
-    if (amax == 0.0 || !isfinite(amax)) return 0;
+    if (amax == 0.0 || !__builtin_isfinite(amax)) return 0;
     int e;
     frexp(amax, &e); // amax = m * 2^e, m in [0.5, 1)
     return e;
 }
 
+__host__ __device__ __forceinline__ int norm_exp_of(double ss)
+{
+    const int k = exp_of(ss * (1.0 + 0x1p-30));
+    return (k + 1) >> 1; // ceil(k / 2), k of either sign
+}
+
+template <int NORM> __device__ __forceinline__ double red_op(double acc, double x)
+{
+    return NORM ? fma(x, x, acc) : fmax(acc, fabs(x));
+}
+template <int NORM> __device__ __forceinline__ double red_join(double a, double b)
+{
+    return NORM ? a + b : fmax(a, b);
+}
+
 // mode 0 runs in two passes so that the whole GPU streams X: pass 1 gives
 // block (bx, by) rows i in [64 bx, 64 bx + 64) and the by-th chunk of k,
 // 64 rows x 4 k per step (coalesced 512-byte row segments), and writes the
-// chunk's max |x| to part[by * n + i]; pass 2 takes the max over chunks.
-// fmax over non-negative values is associative, so the result equals the
-// single sequential max (NaN entries are skipped by fmax in both).
+// chunk's max |x| (sum of squares) to part[by * n + i]; pass 2 combines the
+// chunks.  fmax over non-negative values is associative, so the max equals
+// the single sequential max (NaN entries are skipped by fmax in both).
 constexpr int EXP_CHUNKS = 32;
 
+template <int NORM>
 __global__ void k_amax_strided(const double *__restrict__ X, int64_t ld, int n, int len,
                                int chunk, double *__restrict__ part)
 {
@@ -86,16 +110,18 @@ __global__ void k_amax_strided(const double *__restrict__ X, int64_t ld, int n, 
     const int k0 = blockIdx.y * chunk, k1 = min(len, k0 + chunk);
     double m = 0.0;
     if (i < n)
-        for (int k = k0 + ty; k < k1; k += 4) m = fmax(m, fabs(X[i + (int64_t)k * ld]));
+        for (int k = k0 + ty; k < k1; k += 4) m = red_op<NORM>(m, X[i + (int64_t)k * ld]);
     red[ty][tx] = m;
     __syncthreads();
     if (ty == 0 && i < n)
         part[(int64_t)blockIdx.y * n + i] =
-            fmax(fmax(red[0][tx], red[1][tx]), fmax(red[2][tx], red[3][tx]));
+            red_join<NORM>(red_join<NORM>(red[0][tx], red[1][tx]),
+                           red_join<NORM>(red[2][tx], red[3][tx]));
 }
 
 // per = 2: rows 2i and 2i+1 of the scan (real and imaginary parts of a
 // complex row) give one exponent
+template <int NORM>
 __global__ void k_exp_parts(const double *__restrict__ part, int chunks, int n, int per,
                             int *__restrict__ e)
 {
@@ -104,10 +130,11 @@ __global__ void k_exp_parts(const double *__restrict__ part, int chunks, int n, 
     const int64_t rows = (int64_t)n * per;
     double m = 0.0;
     for (int c = 0; c < chunks; ++c)
-        for (int r = 0; r < per; ++r) m = fmax(m, part[c * rows + (int64_t)i * per + r]);
-    e[i] = exp_of(m);
+        for (int r = 0; r < per; ++r) m = red_join<NORM>(m, part[c * rows + (int64_t)i * per + r]);
+    e[i] = NORM ? norm_exp_of(m) : exp_of(m);
 }
 
+template <int NORM>
 __global__ void k_exp_contig(const double *__restrict__ X, int64_t ld, int n, int len,
                              int *__restrict__ e)
 {
@@ -115,14 +142,15 @@ __global__ void k_exp_contig(const double *__restrict__ X, int64_t ld, int n, in
     const int i = blockIdx.x;
     double m = 0.0;
     const double *row = X + (int64_t)i * ld;
-    for (int k = threadIdx.x; k < len; k += blockDim.x) m = fmax(m, fabs(row[k]));
+    for (int k = threadIdx.x; k < len; k += blockDim.x) m = red_op<NORM>(m, row[k]);
     red[threadIdx.x] = m;
     __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        if ((int)threadIdx.x < s)
+            red[threadIdx.x] = red_join<NORM>(red[threadIdx.x], red[threadIdx.x + s]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) e[i] = exp_of(red[0]);
+    if (threadIdx.x == 0) e[i] = NORM ? norm_exp_of(red[0]) : exp_of(red[0]);
 }
 
 // ---------------------------------------------------------------------------
@@ -355,14 +383,17 @@ __device__ __forceinline__ void lgkm_wait0()
     __builtin_amdgcn_s_waitcnt(0xF | (0x3 << 14) | (0x7 << 4));
 }
 
-// 16 bytes per lane from global memory to LDS (M0 + 16 lane)
+// 16 bytes per lane from global memory to LDS (M0 + 16 lane).  M0 is a
+// reserved register the compiler does not allocate (a clobber of it is
+// ignored); nothing else in the kernels that call this uses M0 (checked in
+// their gfx950 assembly: every M0 write is this statement's).
 __device__ __forceinline__ void dma16(const int8_t *g, const int8_t *lds_wave_base)
 {
     const uint32_t l = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) const int8_t *)lds_wave_base;
     asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off"
                  :
                  : "s"(__builtin_amdgcn_readfirstlane(l)), "v"(g)
-                 : "memory", "m0");
+                 : "memory");
 }
 
 __device__ __forceinline__ void tile256_seg(const int8_t *__restrict__ gA,
@@ -636,8 +667,8 @@ __host__ __device__ __forceinline__ int smod_i32(int x, int m, double rinv)
 }
 
 // An integer-valued double v as signed 12-bit digits, |v| = sum_j d_j
-// 2^(12 j), j < J (J = 5 covers |v| < 2^60, J = 7 covers |v| < 2^84; the
-// split's |A'| < 2^beta), kept as floats f_j = sign(v) d_j, plus low = the
+// 2^(12 j), j < J (J = 5, 6, 7 cover |v| < 2^60, 2^72, 2^84; the split's
+// |A'| < 2^gamma), kept as floats f_j = sign(v) d_j, plus low = the
 // low 12 bits of v's two's complement.  Non-finite values and |v| >=
 // 2^(12 J) are taken as zero.  Every step is exact: top = trunc(|v| 2^-60)
 // and hi = trunc(rest 2^-30) are integers, and the FMA remainders are
@@ -648,13 +679,13 @@ template <int J> struct Digits {
 };
 template <int J> __host__ __device__ __forceinline__ Digits<J> digits12(double v)
 {
-    static_assert(J == 5 || J == 7, "digit count");
+    static_assert(J >= 5 && J <= 7, "digit count");
     Digits<J> r;
     double a = fabs(v);
-    if (!(a < (J == 5 ? 0x1p60 : 0x1p84))) a = 0.0;
+    if (!(a < (J == 5 ? 0x1p60 : J == 6 ? 0x1p72 : 0x1p84))) a = 0.0;
     const float sg = v < 0 ? -1.0f : 1.0f;
     double top = 0.0;
-    if (J == 7) {
+    if (J > 5) {
         top = trunc(a * 0x1p-60);
         a = fma(-top, 0x1p60, a);
     }
@@ -666,7 +697,7 @@ template <int J> __host__ __device__ __forceinline__ Digits<J> digits12(double v
     d[2] = (l >> 24) | ((h & 0x3F) << 6);
     d[3] = (h >> 6) & 0xFFF;
     d[4] = h >> 18;
-    if (J == 7) {
+    if (J > 5) {
         const uint32_t t = (uint32_t)top;
         d[5] = t & 0xFFF;
         d[6] = t >> 12;
@@ -1016,6 +1047,27 @@ int oz_fail(const char *what, hipError_t e)
 
 } // namespace
 
+namespace {
+template <int NORM>
+void exp_launch(const double *X, int64_t ld, int n, int len, int contiguous, int per,
+                       int *e, double *part, hipStream_t st)
+{
+    if (contiguous) {
+        hipLaunchKernelGGL(k_exp_contig<NORM>, dim3(n), dim3(256), 0, st, X, ld, n, len, e);
+        return;
+    }
+    int chunk = (len + EXP_CHUNKS - 1) / EXP_CHUNKS;
+    chunk = chunk < 4 ? 4 : (chunk + 3) / 4 * 4;
+    const int chunks = len > 0 ? (len + chunk - 1) / chunk : 1;
+    // per = 2: 2n real rows (re of row i at 2i, im at 2i+1)
+    hipLaunchKernelGGL(k_amax_strided<NORM>, dim3((per * n + 63) / 64, chunks), dim3(256), 0, st,
+                       X, ld, per * n, len, chunk, part);
+    hipLaunchKernelGGL(k_exp_parts<NORM>, dim3((n + 255) / 256), dim3(256), 0, st, part, chunks, n,
+                       per, e);
+}
+
+} // namespace
+
 extern "C" {
 
 const char *bxh_ozaki_error(void) { return g_oz_err; }
@@ -1026,43 +1078,34 @@ int bxh_oz_block_k(void) { return OZ_BK; }
 // exponents of the rows of op(X) (n rows, len = reduced extent):
 // contiguous != 0: x(i,k) = X[k + i*ld], else x(i,k) = X[i + k*ld]
 // part: EXP_CHUNKS * n doubles of scratch (contiguous == 0 only)
-int bxh_oz_exponents(const double *X, int64_t ld, int n, int len, int contiguous, int *e,
-                     double *part, void *stream)
+// norm = 0: max exponents (digit scheme); 1: 2-norm exponents (CRT scheme)
+int bxh_oz_exponents(const double *X, int64_t ld, int n, int len, int contiguous, int norm,
+                     int *e, double *part, void *stream)
 {
     if (n <= 0) return 0;
     hipStream_t st = (hipStream_t)stream;
-    if (contiguous) {
-        hipLaunchKernelGGL(k_exp_contig, dim3(n), dim3(256), 0, st, X, ld, n, len, e);
-    } else {
-        int chunk = (len + EXP_CHUNKS - 1) / EXP_CHUNKS;
-        chunk = chunk < 4 ? 4 : (chunk + 3) / 4 * 4;
-        const int chunks = len > 0 ? (len + chunk - 1) / chunk : 1;
-        hipLaunchKernelGGL(k_amax_strided, dim3((n + 63) / 64, chunks), dim3(256), 0, st, X, ld,
-                           n, len, chunk, part);
-        hipLaunchKernelGGL(k_exp_parts, dim3((n + 255) / 256), dim3(256), 0, st, part, chunks, n,
-                           1, e);
-    }
+    if (norm)
+        exp_launch<1>(X, ld, n, len, contiguous, 1, e, part, st);
+    else
+        exp_launch<0>(X, ld, n, len, contiguous, 1, e, part, st);
     hipError_t r = hipGetLastError();
     return r == hipSuccess ? 0 : oz_fail("exponent launch", r);
 }
 
-// complex rows (interleaved re/im, ld in complex elements): one exponent of
-// max(|re|, |im|) per row; part: EXP_CHUNKS * 2n doubles (contiguous == 0)
-int bxh_oz_exponents_z(const double *Z, int64_t ld, int n, int len, int contiguous, int *e,
-                       double *part, void *stream)
+// complex rows (interleaved re/im, ld in complex elements): one exponent per
+// row from max(|re|, |im|) (norm = 0) or sum(re^2 + im^2) (norm = 1); part:
+// EXP_CHUNKS * 2n doubles (contiguous == 0)
+int bxh_oz_exponents_z(const double *Z, int64_t ld, int n, int len, int contiguous, int norm,
+                       int *e, double *part, void *stream)
 {
     if (n <= 0) return 0;
     if (contiguous) // row i: 2*len consecutive doubles
-        return bxh_oz_exponents(Z, 2 * ld, n, 2 * len, 1, e, part, stream);
+        return bxh_oz_exponents(Z, 2 * ld, n, 2 * len, 1, norm, e, part, stream);
     hipStream_t st = (hipStream_t)stream;
-    int chunk = (len + EXP_CHUNKS - 1) / EXP_CHUNKS;
-    chunk = chunk < 4 ? 4 : (chunk + 3) / 4 * 4;
-    const int chunks = len > 0 ? (len + chunk - 1) / chunk : 1;
-    // 2n real rows (re of row i at 2i, im at 2i+1), column stride 2*ld doubles
-    hipLaunchKernelGGL(k_amax_strided, dim3((2 * n + 63) / 64, chunks), dim3(256), 0, st, Z,
-                       2 * ld, 2 * n, len, chunk, part);
-    hipLaunchKernelGGL(k_exp_parts, dim3((n + 255) / 256), dim3(256), 0, st, part, chunks, n, 2,
-                       e);
+    if (norm)
+        exp_launch<1>(Z, 2 * ld, n, len, 0, 2, e, part, st);
+    else
+        exp_launch<0>(Z, 2 * ld, n, len, 0, 2, e, part, st);
     hipError_t r = hipGetLastError();
     return r == hipSuccess ? 0 : oz_fail("exponent launch", r);
 }
@@ -1154,24 +1197,25 @@ int bxh_crt_split(const double *X, int64_t ld, int n, int len, int contiguous, c
     if (L < 1 || L > 20 || np % 64 || kp % 64) return BLASX_EINVAL;
     hipStream_t st = (hipStream_t)stream;
     const CrtMods md = crt_mods(L, moduli);
-    const bool d5 = beta <= 60; // |A'| < 2^beta: five 12-bit digits suffice
+    // |A'| < 2^beta: the fewest 12-bit digits that cover beta bits
+    const int J = beta <= 60 ? 5 : beta <= 72 ? 6 : 7;
     if (contiguous) {
         const int64_t total = (int64_t)np * (kp / 16);
         const dim3 grid((unsigned)((total + 255) / 256));
-        if (d5)
-            hipLaunchKernelGGL(k_crt_split_contig<5>, grid, dim3(256), 0, st, X, ld, n, len, e,
-                               beta, md, D, np, kp);
-        else
-            hipLaunchKernelGGL(k_crt_split_contig<7>, grid, dim3(256), 0, st, X, ld, n, len, e,
-                               beta, md, D, np, kp);
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, X, ld, n, len, e, beta, md, D, np, kp);
+        };
+        if (J == 5) go(k_crt_split_contig<5>);
+        else if (J == 6) go(k_crt_split_contig<6>);
+        else go(k_crt_split_contig<7>);
     } else {
         const dim3 grid(np / 64, kp / 64);
-        if (d5)
-            hipLaunchKernelGGL(k_crt_split_strided<5>, grid, dim3(256), 0, st, X, ld, n, len, e,
-                               beta, md, D, np, kp);
-        else
-            hipLaunchKernelGGL(k_crt_split_strided<7>, grid, dim3(256), 0, st, X, ld, n, len, e,
-                               beta, md, D, np, kp);
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, X, ld, n, len, e, beta, md, D, np, kp);
+        };
+        if (J == 5) go(k_crt_split_strided<5>);
+        else if (J == 6) go(k_crt_split_strided<6>);
+        else go(k_crt_split_strided<7>);
     }
     hipError_t r = hipGetLastError();
     return r == hipSuccess ? 0 : oz_fail("crt split launch", r);
@@ -1233,24 +1277,27 @@ int bxh_zcrt_split(const double *Z, int64_t ld, int n, int len, int contiguous, 
     if (L < 1 || L > 20 || np % 64 || kp % 64) return BLASX_EINVAL;
     hipStream_t st = (hipStream_t)stream;
     const CrtMods md = crt_mods(L, moduli);
-    const bool d5 = beta <= 60; // |A'| < 2^beta: five 12-bit digits suffice
+    // |A'| < 2^beta: the fewest 12-bit digits that cover beta bits
+    const int J = beta <= 60 ? 5 : beta <= 72 ? 6 : 7;
     if (contiguous) {
         const int64_t total = (int64_t)np * (kp / 16);
         const dim3 grid((unsigned)((total + 255) / 256));
-        if (d5)
-            hipLaunchKernelGGL(k_zcrt_split_contig<5>, grid, dim3(256), 0, st, Z, ld, n, len,
-                               conj, e, beta, md, D, np, kp);
-        else
-            hipLaunchKernelGGL(k_zcrt_split_contig<7>, grid, dim3(256), 0, st, Z, ld, n, len,
-                               conj, e, beta, md, D, np, kp);
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, Z, ld, n, len, conj, e, beta, md, D,
+                               np, kp);
+        };
+        if (J == 5) go(k_zcrt_split_contig<5>);
+        else if (J == 6) go(k_zcrt_split_contig<6>);
+        else go(k_zcrt_split_contig<7>);
     } else {
         const dim3 grid(np / 64, kp / 64);
-        if (d5)
-            hipLaunchKernelGGL(k_zcrt_split_strided<5>, grid, dim3(256), 0, st, Z, ld, n, len,
-                               conj, e, beta, md, D, np, kp);
-        else
-            hipLaunchKernelGGL(k_zcrt_split_strided<7>, grid, dim3(256), 0, st, Z, ld, n, len,
-                               conj, e, beta, md, D, np, kp);
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, Z, ld, n, len, conj, e, beta, md, D,
+                               np, kp);
+        };
+        if (J == 5) go(k_zcrt_split_strided<5>);
+        else if (J == 6) go(k_zcrt_split_strided<6>);
+        else go(k_zcrt_split_strided<7>);
     }
     hipError_t r = hipGetLastError();
     return r == hipSuccess ? 0 : oz_fail("zcrt split launch", r);
@@ -1282,6 +1329,12 @@ int bxh_crt_resid_host(double v, int m)
     const CrtMods md = crt_mods(1, &m);
     return digit_resid<7>(digits12<7>(v), mod_const<7>(md, 0));
 }
+int bxh_crt_resid6_host(double v, int m)
+{
+    const CrtMods md = crt_mods(1, &m);
+    return digit_resid<6>(digits12<6>(v), mod_const<6>(md, 0));
+}
+int bxh_crt_norm_exp_host(double ss) { return norm_exp_of(ss); }
 int bxh_crt_resid5_host(double v, int m)
 {
     const CrtMods md = crt_mods(1, &m);

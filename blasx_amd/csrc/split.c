@@ -16,8 +16,8 @@
 
 int bxh_oz_block_m(void);
 int bxh_oz_block_k(void);
-int bxh_oz_exponents(const double *X, int64_t ld, int n, int len, int contiguous, int *e,
-                     double *part, void *stream);
+int bxh_oz_exponents(const double *X, int64_t ld, int n, int len, int contiguous, int norm,
+                     int *e, double *part, void *stream);
 int bxh_oz_exp_chunks(void);
 int bxh_oz_split(const double *X, int64_t ld, int n, int len, int contiguous, const int *e,
                  int s, int8_t *D, int np, int kp, void *stream);
@@ -106,8 +106,8 @@ int bx_dgemm_split(int opa, int opb, int M, int N, int K, double alpha, const do
     /* exponents and slices: rows of op(A), columns of op(B) */
     const int a_contig = opa != BX_OP_N; /* A^T: row i of op(A) is column i of A */
     const int b_contig = opb == BX_OP_N; /* B: column j of op(B) is column j of B */
-    rc = bxh_oz_exponents(A, lda, M, K, a_contig, ea, part, stream);
-    if (!rc) rc = bxh_oz_exponents(B, ldb, N, K, b_contig, eb, part, stream);
+    rc = bxh_oz_exponents(A, lda, M, K, a_contig, 0, ea, part, stream);
+    if (!rc) rc = bxh_oz_exponents(B, ldb, N, K, b_contig, 0, eb, part, stream);
     if (!rc) rc = bxh_oz_split(A, lda, M, K, a_contig, ea, s, DA, mp, kp, stream);
     if (!rc) rc = bxh_oz_split(B, ldb, N, K, b_contig, eb, s, DB, np, kp, stream);
 

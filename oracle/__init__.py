@@ -87,6 +87,8 @@ def lib():
                                       C.c_int]
         L.orc_crt_modulus.argtypes = [C.c_int]
         L.orc_crt_beta.argtypes = [C.c_int, C.c_int]
+        L.orc_crt_gamma.argtypes = [C.c_int]
+        L.orc_norm_exp.argtypes = [C.c_double]
         L.orc_smod.restype = C.c_int64
         L.orc_smod.argtypes = [C.c_int64, C.c_int64]
         L.orc_crt_resid.argtypes = [C.c_double, C.c_int]
@@ -263,8 +265,22 @@ def crt_moduli(L):
 
 
 def crt_beta(K, L):
-    """Bits kept per operand entry by the CRT split with L moduli."""
+    """Worst-case bits kept per operand entry, relative to its row's
+    maximum, by the CRT split with L moduli (all entries of a row equal in
+    magnitude)."""
     return lib().orc_crt_beta(K, L)
+
+
+def crt_gamma(L):
+    """Scaling bits of the CRT split with L moduli, relative to the row's
+    2-norm: 2^(2 gamma + 3) <= prod m_l."""
+    return lib().orc_crt_gamma(L)
+
+
+def norm_exp(ss):
+    """Smallest e with ss (1 + 2^-30) < 4^e (the row exponent from a sum of
+    squares)."""
+    return lib().orc_norm_exp(ss)
 
 
 def smod(v, m):

@@ -597,9 +597,10 @@ void orc_zgemm_ozaki(char ta, char tb, int M, int N, int K, const double *alpha,
 /* the Chinese remainder theorem).  No reference counterpart; restates  */
 /* the method in the library's exact evaluation order:                  */
 /*   moduli m_1..m_L from ORC_CRT_MODS (pairwise coprime, <= 256),       */
-/*   beta = floor((sum log2 m_l - log2 K - 3) / 2 - 2^-20),             */
-/*   e_i, f_j as in the scheme above,                                   */
-/*   A'_ik = trunc(ldexp(op(A)_ik, beta - e_i)),  |A'| < 2^beta,        */
+/*   beta = gamma = floor((sum log2 m_l - 3) / 2 - 2^-20),              */
+/*   e_i = orc_norm_exp(sum_k op(A)_ik^2): ||op(A)_i||_2 < 2^e_i, and   */
+/*   f_j likewise for the columns of op(B) (Cauchy-Schwarz scaling),    */
+/*   A'_ik = trunc(ldexp(op(A)_ik, beta - e_i)),  ||A'_i|| < 2^beta,    */
 /*   a_l = smod(A', m_l) with smod(v, m) in [-floor(m/2), ceil(m/2)-1], */
 /*   R_l = smod(a_l b_l^T, m_l)  (exact integer products),              */
 /*   floating CRT: M = prod m_l with bit length E, w_l = (M/m_l) y_l,   */
@@ -611,7 +612,9 @@ void orc_zgemm_ozaki(char ta, char tb, int M, int N, int K, const double *alpha,
 /*     D_k = Q_k - kq M_k         (exact: < 2^53 chunk units)           */
 /*     S = ((D_1 + D_2) + D_3) + D_4                                    */
 /*   and C = alpha * ldexp(S, e_i + f_j - 2 beta)  (+ fma(beta, C, .)). */
-/* |A'B'| < K 2^(2 beta) <= M/8, so Q = P + kq M with |P| < M/8 and kq  */
+/* |A'B'| <= ||A'_i|| ||B'_j|| < 2^(2 beta) <= M/8 (worst-case bits    */
+/* relative to the row maximum: orc_crt_beta), so Q = P + kq M with    */
+/* |P| < M/8 and kq                                                     */
 /* is exact.  The partial sum D_1 + .. + D_j equals P minus the lower  */
 /* chunks, so it is exact whenever it is small, and S = P (1 + 3u) for */
 /* every P, with no cancellation loss.                                  */
@@ -622,6 +625,25 @@ static const int ORC_CRT_MODS[20] = {256, 255, 253, 251, 247, 241, 239, 233, 229
                                      223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
 
 int orc_crt_modulus(int l) { return (l >= 0 && l < 20) ? ORC_CRT_MODS[l] : 0; }
+
+/* scaling bits relative to the row 2-norm: 2^(2 gamma) <= M / 8 */
+int orc_crt_gamma(int L)
+{
+    double lm = 0.0;
+    for (int l = 0; l < L; ++l) lm += log2((double)ORC_CRT_MODS[l]);
+    return (int)floor((lm - 3.0) / 2.0 - 0x1p-20);
+}
+
+/* smallest e with ss (1 + 2^-30) < 4^e: from the binary exponent k of
+ * ss (1 + 2^-30) (below 2^k, at least 2^(k-1)), e = ceil(k / 2) */
+int orc_norm_exp(double ss)
+{
+    const double u = ss * (1.0 + 0x1p-30);
+    if (u == 0.0 || !isfinite(u)) return 0;
+    int k;
+    frexp(u, &k);
+    return (k + 1) >> 1;
+}
 
 int orc_crt_beta(int K, int L)
 {
@@ -770,7 +792,7 @@ double orc_crt_float(const int *r, int L, const double *Wk, const double *Mk, do
 void orc_dgemm_crt(char ta, char tb, int M, int N, int K, double alpha, const double *A, int lda,
                    const double *B, int ldb, double beta, double *C, int ldc, int L)
 {
-    const int bb = orc_crt_beta(K, L);
+    const int bb = orc_crt_gamma(L);
     double Wk[4 * 20], Mk[4], invM;
     orc_crt_consts(L, Wk, Mk, &invM);
     int *ea = (int *)malloc(sizeof(int) * (M > 0 ? M : 1));
@@ -778,9 +800,9 @@ void orc_dgemm_crt(char ta, char tb, int M, int N, int K, double alpha, const do
     int8_t *RA = (int8_t *)calloc((size_t)L * M * (K > 0 ? K : 1), 1);
     int8_t *RB = (int8_t *)calloc((size_t)L * N * (K > 0 ? K : 1), 1);
     for (int i = 0; i < M; ++i) {
-        double mx = 0;
-        for (int p = 0; p < K; ++p) mx = fmax(mx, fabs(A[OPIDX(ta, i, p, lda)]));
-        ea[i] = orc_exp_of(mx);
+        double ss = 0;
+        for (int p = 0; p < K; ++p) ss = fma(A[OPIDX(ta, i, p, lda)], A[OPIDX(ta, i, p, lda)], ss);
+        ea[i] = orc_norm_exp(ss);
         for (int p = 0; p < K; ++p) {
             const double v = trunc(ldexp(A[OPIDX(ta, i, p, lda)], bb - ea[i]));
             for (int l = 0; l < L; ++l)
@@ -788,9 +810,9 @@ void orc_dgemm_crt(char ta, char tb, int M, int N, int K, double alpha, const do
         }
     }
     for (int j = 0; j < N; ++j) {
-        double mx = 0;
-        for (int p = 0; p < K; ++p) mx = fmax(mx, fabs(B[OPIDX(tb, p, j, ldb)]));
-        eb[j] = orc_exp_of(mx);
+        double ss = 0;
+        for (int p = 0; p < K; ++p) ss = fma(B[OPIDX(tb, p, j, ldb)], B[OPIDX(tb, p, j, ldb)], ss);
+        eb[j] = orc_norm_exp(ss);
         for (int p = 0; p < K; ++p) {
             const double v = trunc(ldexp(B[OPIDX(tb, p, j, ldb)], bb - eb[j]));
             for (int l = 0; l < L; ++l)
@@ -821,8 +843,9 @@ void orc_dgemm_crt(char ta, char tb, int M, int N, int K, double alpha, const do
 }
 
 /* complex CRT split restated: op(A) rows / op(B) columns share one exponent
- * for both parts (of max(|re|, |im|)), beta = orc_crt_beta(2K, L) since
- * |Re P|, |Im P| < 2K 2^(2 beta); per modulus three exact products on the
+ * for both parts (of the 2-norm over the 2K real parts), beta =
+ * orc_crt_gamma(L) since |Re P|, |Im P| <= ||a_i|| ||b_j|| < 2^(2 beta)
+ * (Cauchy-Schwarz over 2K terms); per modulus three exact products on the
  * residue planes (Karatsuba, exact in modular arithmetic):
  *   R1 = ar br, R2 = ai bi, R3 = (ar + ai)(br + bi)   (each smod m),
  *   Re = smod(R1 - R2), Im = smod(R3 - R1 - R2),
@@ -832,7 +855,7 @@ void orc_zgemm_crt(char ta, char tb, int M, int N, int K, const double *alpha, c
                    int lda, const double *B, int ldb, const double *beta, double *C, int ldc,
                    int L)
 {
-    const int bb = orc_crt_beta(2 * K, L);
+    const int bb = orc_crt_gamma(L);
     double Wk[4 * 20], Mk[4], invM;
     orc_crt_consts(L, Wk, Mk, &invM);
     const size_t pa = (size_t)M * (K > 0 ? K : 1), pb = (size_t)N * (K > 0 ? K : 1);
@@ -849,12 +872,12 @@ void orc_zgemm_crt(char ta, char tb, int M, int N, int K, const double *alpha, c
         int *e = side == 0 ? ea : eb;
         const size_t plane = side == 0 ? pa : pb;
         for (int i = 0; i < rows; ++i) {
-            double mx = 0;
+            double ss = 0;
             for (int p = 0; p < K; ++p) {
                 const int64_t o = side == 0 ? OPIDX(t, i, p, ld) : OPIDX(t, p, i, ld);
-                mx = fmax(mx, fmax(fabs(X[2 * o]), fabs(X[2 * o + 1])));
+                ss = fma(X[2 * o + 1], X[2 * o + 1], fma(X[2 * o], X[2 * o], ss));
             }
-            e[i] = orc_exp_of(mx);
+            e[i] = orc_norm_exp(ss);
             for (int p = 0; p < K; ++p) {
                 const int64_t o = side == 0 ? OPIDX(t, i, p, ld) : OPIDX(t, p, i, ld);
                 const double re = X[2 * o], im = t == 'C' ? -X[2 * o + 1] : X[2 * o + 1];

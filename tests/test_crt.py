@@ -14,6 +14,7 @@ No reference counterpart: the method is pinned from its mathematics.
 """
 import ctypes as C
 import math
+from fractions import Fraction
 import random
 import sys
 
@@ -63,8 +64,13 @@ def test_moduli_pairwise_coprime_and_beta(orc, bx):
         for L in range(2, 21):
             b = orc.crt_beta(K, L)
             assert L_.blasx_crt_bits(K, L) == b
-            if b >= 1:  # |A'B'| < K 2^(2b) <= M / 8 exactly
+            if b >= 1:  # worst case: |A'B'| < K 2^(2b) <= M / 8 exactly
                 assert 8 * K * 2 ** (2 * b) <= math.prod(MODS[:L])
+    for L in range(2, 21):
+        # scaling by row 2-norms: |A'B'| <= ||A'_i|| ||B'_j|| < 2^(2g) <= M / 8,
+        # and g is the largest such integer
+        g = orc.crt_gamma(L)
+        assert 8 * 2 ** (2 * g) <= math.prod(MODS[:L]) < 8 * 2 ** (2 * g + 2)
 
 
 @pytest.mark.parametrize("L", [2, 5, 8, 13, 16, 20])
@@ -97,6 +103,7 @@ def test_residue_helpers_match_exact_arithmetic(orc, bx):
     Lb = bx.lib()
     Lb.bxtest_crt_resid.argtypes = [C.c_double, C.c_int]
     Lb.bxtest_crt_resid5.argtypes = [C.c_double, C.c_int]
+    Lb.bxtest_crt_resid6.argtypes = [C.c_double, C.c_int]
     Lb.bxtest_crt_smod_i32.argtypes = [C.c_int, C.c_int]
     rng = random.Random(11)
     vals = [0.0, -0.0, 1.0, -1.0, 127.0, -128.0, 2.0 ** 53, -(2.0 ** 53), 2.0 ** 53 - 1, 2.0 ** 26,
@@ -115,11 +122,48 @@ def test_residue_helpers_match_exact_arithmetic(orc, bx):
             want = _smod(iv, m)
             assert Lb.bxtest_crt_resid(v, m) == want, (v, m)
             assert Lb.bxtest_crt_resid5(v, m) == (want if abs(v) < 2.0 ** 60 else 0), (v, m)
+            assert Lb.bxtest_crt_resid6(v, m) == (want if abs(v) < 2.0 ** 72 else 0), (v, m)
             if abs(v) < 2.0 ** 79:
                 assert orc.crt_resid(v, m) == want, (v, m)
     for v in (float("inf"), -float("inf"), float("nan"), 2.0 ** 84, -(2.0 ** 90)):
         for m in MODS:
             assert Lb.bxtest_crt_resid(v, m) == 0 and Lb.bxtest_crt_resid5(v, m) == 0, (v, m)
+            assert Lb.bxtest_crt_resid6(v, m) == 0, (v, m)
+
+
+def _norm_exp_exact(ss):
+    """Smallest integer e with fl(ss * (1 + 2^-30)) < 4^e, exactly."""
+    u = ss * (1.0 + 2.0 ** -30)
+    if u == 0.0 or not math.isfinite(u):
+        return 0
+    q = Fraction(u)
+    e = -600
+    while Fraction(4) ** e <= q:
+        e += 1
+    while e > -600 and Fraction(4) ** (e - 1) > q:
+        e -= 1
+    return e
+
+
+def test_norm_exponent_matches_definition(orc, bx):
+    """The row exponent of the CRT scaling (smallest e with ss (1 + 2^-30) <
+    4^e, from the binary exponent alone): the library's host build of the
+    kernel code and the oracle against exact rational arithmetic, on powers
+    of 4 and 2 and their neighbours, tiny and huge sums, zero and inf."""
+    Lb = bx.lib()
+    Lb.bxtest_crt_norm_exp.argtypes = [C.c_double]
+    rng = random.Random(5)
+    vals = [0.0, float("inf"), 5e-324, 1e-300, 1e300]
+    for k in range(-60, 61):
+        for b in (2.0 ** k, 4.0 ** (k // 2)):
+            vals += [b, b * (1 - 2 ** -52), b * (1 + 2 ** -52), b * (1 - 2 ** -31), b * (1 - 2 ** -29)]
+    vals += [rng.uniform(0, 1) * 2.0 ** rng.randrange(-200, 200) for _ in range(2000)]
+    for ss in vals:
+        want = _norm_exp_exact(ss)
+        assert Lb.bxtest_crt_norm_exp(ss) == want, ss
+        assert orc.norm_exp(ss) == want, ss
+        if 0 < ss < float("inf"):  # the row norm is below 2^e
+            assert math.sqrt(ss) < 2.0 ** want
     for _ in range(20000):
         x = rng.randrange(-2 ** 31, 2 ** 31)
         m = rng.choice(MODS)
@@ -164,19 +208,23 @@ def test_oracle_exact_on_integer_inputs(orc, ta, tb):
     opA = A if ta == "N" else A.T
     opB = B if tb == "N" else B.T
     exact = opA.astype(np.int64) @ opB.astype(np.int64)
+    emax = max(max(orc.norm_exp(float(np.sum(opA * opA, axis=1).max())), 0),
+               max(orc.norm_exp(float(np.sum(opB * opB, axis=0).max())), 0))
     for L in (8, 12, 20):
-        assert orc.crt_beta(K, L) >= 21
+        assert orc.crt_gamma(L) >= emax  # A' = a 2^(gamma - e) is an integer
         Cc = np.full((M, N), np.nan, order="F")
         orc.dgemm_crt(ta, tb, M, N, K, 1.0, A, A.shape[0], B, B.shape[0], 0.0, Cc, M, L)
         assert np.array_equal(Cc, exact.astype(np.float64)), L
 
 
-def _crt_bound(As, Bs, beta, alpha=1.0):
-    """|C - C_exact| for C = alpha op(A) op(B): truncation of both operands
-    (|a - A' 2^(e-beta)| < 2^(e-beta)) plus the reconstruction and scaling
-    roundings."""
-    ea = np.frexp(np.abs(As).max(axis=1))[1].astype(np.float64)
-    eb = np.frexp(np.abs(Bs).max(axis=0))[1].astype(np.float64)
+def _crt_bound(orc, As, Bs, beta, alpha=1.0):
+    """|C - C_exact| for C = alpha op(A) op(B) with gamma = beta scaling bits:
+    truncation of both operands (|a - A' 2^(e-beta)| < 2^(e-beta), e the
+    row's 2-norm exponent, taken one higher here so that a different
+    summation order of the sum of squares cannot tighten it) plus the
+    reconstruction and scaling roundings."""
+    ea = 1.0 + np.array([orc.norm_exp(float(v)) for v in np.sum(As * As, axis=1)])
+    eb = 1.0 + np.array([orc.norm_exp(float(v)) for v in np.sum(Bs * Bs, axis=0)])
     K = As.shape[1]
     t = (np.exp2(ea - beta)[:, None] * np.abs(Bs).sum(axis=0)[None, :]
          + np.exp2(eb - beta)[None, :] * np.abs(As).sum(axis=1)[:, None]
@@ -194,7 +242,7 @@ def test_oracle_error_within_bound_and_falls_with_moduli(orc):
         Cc = np.zeros((M, N), order="F")
         orc.dgemm_crt("N", "N", M, N, K, 1.0, A, M, B, K, 0.0, Cc, M, L)
         err = np.abs(Cc - hi)
-        assert np.all(err <= _crt_bound(A, B, orc.crt_beta(K, L))), L
+        assert np.all(err <= _crt_bound(orc, A, B, orc.crt_gamma(L))), L
         errs.append(np.max(err / (np.abs(A) @ np.abs(B))))
     assert all(b < a for a, b in zip(errs, errs[1:])), errs
     assert errs[-1] < 4 * EPS
@@ -309,7 +357,7 @@ def test_gpu_crt_large_within_bound(gpu, orc):
         blk = out[r0:r0 + 64, c0:c0 + 64]
         assert np.array_equal(blk, ref)
         hi, _ = orc.gemm_dd("N", "N", 64, 64, n, 1.0, As, 64, Bs, n, 0.0, ref * 0, 64)
-        assert np.all(np.abs(blk - hi) <= _crt_bound(As, Bs, orc.crt_beta(n, L)))
+        assert np.all(np.abs(blk - hi) <= _crt_bound(orc, As, Bs, orc.crt_gamma(L)))
 
 
 # ------------------------------------------------------------------ complex
@@ -334,8 +382,11 @@ def test_oracle_zcrt_exact_on_integers(orc, ta, tb):
     B = _cplx_int(rng, (K, N) if tb == "N" else (N, K), 18)
     C0 = _cplx_int(rng, (M, N), 18)
     exact = (2 - 1j) * (_zop(A, ta) @ _zop(B, tb)) + (1 + 1j) * C0
+    oa, ob = _zop(A, ta), _zop(B, tb)
+    emax = max(orc.norm_exp(float(np.sum(np.abs(oa) ** 2, axis=1).max())),
+               orc.norm_exp(float(np.sum(np.abs(ob) ** 2, axis=0).max())))
     for L in (10, 16):
-        assert orc.crt_beta(2 * K, L) >= 19
+        assert orc.crt_gamma(L) >= emax  # both parts scale to integers
         out = C0.copy(order="F")
         orc.zgemm_crt(ta, tb, M, N, K, 2 - 1j, A, A.shape[0], B, B.shape[0], 1 + 1j, out, M, L)
         assert np.array_equal(out, exact), L
