@@ -121,7 +121,53 @@ def cpu_baseline():
             "sample": (f"OpenBLAS cblas_dgemm (the reference's CPU BLAS path, dlopen+dlsym as in "
                        f"blas/blasx_internal.c:26-36) DGEMM M=N=K={n} NN alpha=1 beta=1, same "
                        f"seeded generator; median of {len(times)} calls after 1 warm-up; "
-                       f"{os.path.basename(info['path'])} {info['config'][:40]}")}
+                       f"{os.path.basename(info['path'])} {info['config'][:40]}"),
+            "cfg0": cfg0_cpu_and_parity(oracle, threads)}
+
+
+def cfg0_cpu_and_parity(oracle, threads):
+    """The reference's CPU-runnable case: DGEMM M=N=K=1024 NN, alpha=1.5,
+    beta=0.5, uniform[-1,1) seeds 0 (ids 0, 1, 2), ld=1024.  OpenBLAS
+    GFLOP/s (median of >= 3 calls after a warm-up), and the same inputs
+    through this library's cblas_dgemm (1024 > 1000 routes to the GPU, as in
+    the reference), both checked against the oracle's naive triple loop:
+    relative Frobenius error ||C - C_naive|| / (|alpha| ||A|| ||B|| + |beta|
+    ||C0||)."""
+    import numpy as np
+
+    import blasx_amd as bx
+
+    n, alpha, beta = 1024, 1.5, 0.5
+    A = oracle.fill_uniform(n, n, 0, 0)
+    B = oracle.fill_uniform(n, n, 0, 1)
+    C0 = oracle.fill_uniform(n, n, 0, 2)
+    ref = C0.copy(order="F")
+    oracle.gemm("N", "N", n, n, n, alpha, A, n, B, n, beta, ref, n)
+    scale = abs(alpha) * np.linalg.norm(A) * np.linalg.norm(B) + abs(beta) * np.linalg.norm(C0)
+    times, Cc = [], None
+    for r in range(6):
+        Cc = C0.copy(order="F")
+        t = time.perf_counter()
+        oracle.openblas_gemm("N", "N", n, n, n, alpha, A, n, B, n, beta, Cc, n)
+        if r:
+            times.append(time.perf_counter() - t)
+    cpu_err = float(np.linalg.norm(Cc - ref) / scale)
+    gt = []
+    for r in range(4):
+        Cg = C0.copy(order="F")
+        t = time.perf_counter()
+        bx.gemm("N", "N", alpha, A, B, beta, Cg)
+        if r:
+            gt.append(time.perf_counter() - t)
+    gpu_err = float(np.linalg.norm(Cg - ref) / scale)
+    flops = 2.0 * n ** 3
+    res = {"n": n, "cpu_gflops": round(flops / statistics.median(times) / 1e9, 1),
+           "cpu_threads": threads, "cpu_rel_fro_err": cpu_err,
+           "abi_gflops": round(flops / statistics.median(gt) / 1e9, 1),
+           "abi_path": "GPU (1024 > 1000, host operands, pageable)",
+           "abi_rel_fro_err": gpu_err, "tolerance": n * 2.0 ** -53}
+    log(f"cfg0: {res}")
+    return res
 
 
 def host_resident(bx, n):
@@ -178,6 +224,106 @@ def host_resident(bx, n):
                                              max(st["bytes_lower_bound"], 1), 3),
         "warm_h2d_bytes": st2["bytes_h2d"],
     }
+
+
+def matrix_inhbm(bx, n=32768):
+    """The kernel on operands resident in HBM for every element type and
+    transpose the ABI serves: SGEMM and DGEMM NN/NT/TN/TT, ZGEMM NN/NT/TN/TT/
+    CN/NC/CC, M=N=K=n, alpha=beta=1, seeded uniform operands generated on
+    the device.  One warm-up per type, one timed call per variant (HIP
+    events on the compute stream); TFLOP/s and the fraction of the
+    published FP32 (157.3) / FP64 (78.6) matrix peak."""
+    import numpy as np
+
+    peaks = {np.float32: 157.3, np.float64: 78.6, np.complex128: 78.6}
+    variants = {np.float32: ("NN", "NT", "TN", "TT"), np.float64: ("NN", "NT", "TN", "TT"),
+                np.complex128: ("NN", "NT", "TN", "TT", "CN", "NC", "CC")}
+    res = {}
+    ev0, ev1 = bx.Event(), bx.Event()
+    for dt in (np.float32, np.float64, np.complex128):
+        name = {np.float32: "sgemm", np.float64: "dgemm", np.complex128: "zgemm"}[dt]
+        X = [bx.DeviceArray(n, n, dt) for _ in range(3)]
+        for i, x in enumerate(X):
+            x.fill_uniform(0, i)
+        bx.synchronize()
+        flops = (8.0 if dt == np.complex128 else 2.0) * n ** 3
+
+        def call(t):
+            bx.device_gemm(t[0], t[1], n, n, n, 1.0, X[0].ptr, n, X[1].ptr, n, 1.0, X[2].ptr, n,
+                           dt)
+
+        call("NN")
+        bx.synchronize()
+        for t in variants[dt]:
+            ev0.record()
+            call(t)
+            ev1.record()
+            ev1.synchronize()
+            ms = ev0.elapsed_ms(ev1)
+            tf = flops / (ms * 1e-3) / 1e12
+            res[f"{name}_{t}"] = {"tflops": round(tf, 2), "frac_of_peak": round(tf / peaks[dt], 4)}
+            log(f"matrix: {name} {t} {n}^3: {ms:.1f} ms, {tf:.2f} TFLOP/s")
+        for x in X:
+            x.free()
+    return res
+
+
+def host_resident_sz(bx, n=32768):
+    """SGEMM and ZGEMM NN through the BLAS ABI with host operands (the DGEMM
+    case is host_resident): cold call (empty tile cache) and warm call (A
+    and B registered resident), bytes moved vs the lower bound
+    e (MK + KN + 2MN).  Pinned memory where the driver grants it, else
+    pageable (reported)."""
+    import numpy as np
+
+    L = bx.lib()
+    res = {}
+    for dt in (np.float32, np.complex128):
+        name = "sgemm" if dt == np.float32 else "zgemm"
+        flops = (8.0 if dt == np.complex128 else 2.0) * n ** 3
+        pinned = []
+        try:
+            pinned = [bx.PinnedArray(n, n, dt) for _ in range(3)]
+            mats = [p.array for p in pinned]
+            kind = "pinned"
+        except MemoryError:
+            for p in pinned:
+                p.free()
+            pinned = []
+            mats = [np.empty((n, n), dtype=dt, order="F") for _ in range(3)]
+            kind = "pageable"
+        log(f"host-resident {name}: {kind} 3 x {n}^2")
+        for i, m in enumerate(mats):
+            bx.fill_uniform_host(m, 0, i)
+        A, B, Cm = mats
+        L.blasx_cache_flush()
+        bx.reset_stats()
+        t = time.perf_counter()
+        bx.gemm("N", "N", 1.0, A, B, 1.0, Cm)
+        cold = time.perf_counter() - t
+        st = bx.stats()
+        L.blasx_resident_register(A.ctypes.data, A.nbytes)
+        L.blasx_resident_register(B.ctypes.data, B.nbytes)
+        bx.gemm("N", "N", 1.0, A, B, 1.0, Cm)
+        bx.reset_stats()
+        t = time.perf_counter()
+        bx.gemm("N", "N", 1.0, A, B, 1.0, Cm)
+        warm = time.perf_counter() - t
+        st2 = bx.stats()
+        L.blasx_resident_unregister(A.ctypes.data)
+        L.blasx_resident_unregister(B.ctypes.data)
+        L.blasx_cache_flush()
+        res[name] = {"memory": kind, "cold_tflops": round(flops / cold / 1e12, 3),
+                     "warm_tflops": round(flops / warm / 1e12, 3),
+                     "cold_h2d_GBps": round(st["bytes_h2d"] / cold / 1e9, 2),
+                     "cold_bytes_over_lower_bound": round(
+                         (st["bytes_h2d"] + st["bytes_d2h"]) / max(st["bytes_lower_bound"], 1), 3),
+                     "warm_h2d_bytes": st2["bytes_h2d"]}
+        log(f"host-resident {name}: {res[name]}")
+        for p in pinned:
+            p.free()
+        del A, B, Cm, mats
+    return res
 
 
 def out_of_core(bx, n=65536, budget_frac=0.75, host_limit=160 << 30):
@@ -607,7 +753,7 @@ def main():
     ap.add_argument("--crt-probe", type=int, default=0, metavar="MODULI",
                     help="one CRT split DGEMM at --n with MODULI moduli, for rocprofv3 passes")
     ap.add_argument("--only", default="", help="comma list of extras to run "
-                    "(host,ooc,split,zsplit,serve); default all")
+                    "(matrix,host,hostsz,ooc,split,zsplit,serve); default all")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build targets one GPU; --gpus {args.gpus} is not supported "
@@ -663,7 +809,9 @@ def main():
             out["cpu_baseline"] = {"value": None, "error": str(e)}
     if not args.no_extras:
         only = set(x for x in args.only.split(",") if x)
-        sections = [("host", "host_resident", lambda: host_resident(bx, n)),
+        sections = [("matrix", "matrix_inhbm", lambda: matrix_inhbm(bx, n)),
+                    ("host", "host_resident", lambda: host_resident(bx, n)),
+                    ("hostsz", "host_resident_sz", lambda: host_resident_sz(bx, n)),
                     ("ooc", "out_of_core", lambda: out_of_core(bx, n=args.ooc_n)),
                     ("split", "split_precision", lambda: split_precision(bx, n=args.split_n)),
                     ("zsplit", "zsplit_precision", lambda: zsplit_precision(bx)),
