@@ -181,7 +181,7 @@ template <typename E, int VEC, int ROWS, int BK, int PAD, int MAJ, bool CONJ> st
     }
 };
 
-__device__ __forceinline__ bool aligned16(const void *p, int64_t ld, int esize)
+__host__ __device__ __forceinline__ bool aligned16(const void *p, int64_t ld, int esize)
 {
     return ((reinterpret_cast<uintptr_t>(p) & 15) == 0) && (((ld * esize) & 15) == 0);
 }
@@ -449,6 +449,163 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_plain(bxh_problem_t P, bxh_seg_t
     gemm_block<E, MAJA, MAJB, CJA, CJB>(P, nullptr, S, tm, tn, smem);
 }
 
+// ---------------------------------------------------------------------------
+// FP64 GEMM with global -> LDS DMA staging, for problems without edge tiles
+// (M, N multiples of 128, K a multiple of 16, 16-byte aligned operands).
+//
+// Why: measured on the 32768^3 NN problem, the register-staged kernel above
+// ran at 67.3 TFLOP/s, 75.5 with its LDS stores removed and 75.9 with MFMAs
+// and LDS reads only: staging each K tile through VGPRs and DS_WRITEs costs
+// 11%.  Here every operand byte goes from global memory straight to LDS
+// (GLOBAL_LOAD_LDS_DWORDX4: 16 bytes per lane at M0 + 16 lane), which
+// constrains the LDS layouts to what one instruction writes, 1 KB runs:
+//  * an operand whose rows are contiguous in memory (MAJ_MN: op(A) = A,
+//    op(B) = B^T) is staged [k][row], rows of 128 doubles padded to 136 (a
+//    K row is one 1 KB run); the MFMA reads one double per lane;
+//  * an operand with K contiguous (MAJ_K) is staged [row][k], 16 doubles per
+//    row as eight 16-byte chunks, chunk c at position c ^ ((row >> 1) & 7)
+//    (the 16 lanes of a fragment read then cover all 64 banks); one 16-byte
+//    read gives a lane the operands of two consecutive K steps.
+// K values are assigned to the MFMA's lane groups so that both layouts
+// serve them: K step s, lane group lg uses k = 2 (4 (s / 2) + lg) + s % 2.
+// Two LDS stages (35 KB each at most), two blocks per CU; step t starts the
+// DMA of tile t + 1 into the other stage (read last in step t - 1, before
+// the last barrier), runs the 64 MFMAs of tile t, waits for its own DMAs
+// and ends with one barrier.  The DMA is inline assembly, and the waits are
+// explicit (`s_waitcnt vmcnt(0)`, bare `s_barrier`): with the builtin the
+// compiler would also wait for the in-flight DMA before every LDS read.
+// ---------------------------------------------------------------------------
+namespace dma64 {
+constexpr int BM = 128, BK = 16, STR_MN = 136;
+template <int MAJ> constexpr int stage_doubles() { return MAJ == MAJ_MN ? BK * STR_MN : BM * BK; }
+
+// element (r, k) of a MAJ_K stage, in doubles
+__device__ __forceinline__ int pos_k(int r, int k)
+{
+    return r * BK + ((((k >> 1) ^ ((r >> 1) & 7))) << 1) + (k & 1);
+}
+
+// 16 bytes per lane from global memory to LDS (M0 + 16 lane).  M0 is a
+// reserved register the compiler does not allocate; nothing else in this
+// kernel uses M0 (checked in its gfx950 assembly).
+__device__ __forceinline__ void dma16(const double *g, const double *lds_wave_base)
+{
+    const uint32_t l =
+        (uint32_t)(uintptr_t)(__attribute__((address_space(3))) const double *)lds_wave_base;
+    asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off"
+                 :
+                 : "s"(__builtin_amdgcn_readfirstlane(l)), "v"(g)
+                 : "memory");
+}
+
+__device__ __forceinline__ void wait_dma()
+{
+    // vmcnt(0), the other counters at their maximum
+    __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xF << 8));
+}
+
+// this wave's four DMA instructions for one operand's K tile k0 (rows r0..)
+template <int MAJ>
+__device__ __forceinline__ void dma_tile(const double *X, int64_t ld, int r0, int64_t k0,
+                                         double *stage, int wave, int lane)
+{
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int q = wave * 4 + i; // 1 KB run of the stage
+        if constexpr (MAJ == MAJ_MN) {
+            // K row q: doubles 2 lane, 2 lane + 1 of rows r0 .. r0 + 127
+            dma16(X + (r0 + 2 * lane) + (k0 + q) * ld, stage + q * STR_MN);
+        } else {
+            // rows 8q .. 8q + 7, lane -> row 8q + lane / 8, position lane % 8
+            const int r = 8 * q + (lane >> 3);
+            const int c = (lane & 7) ^ ((r >> 1) & 7);
+            dma16(X + (k0 + 2 * c) + (int64_t)(r0 + r) * ld, stage + q * 128);
+        }
+    }
+}
+
+// the operand value lane (l16, lg) feeds block b of K step s
+template <int MAJ>
+__device__ __forceinline__ double frag(const double *stage, int row, int s, int lg)
+{
+    const int k = 2 * (4 * (s >> 1) + lg) + (s & 1);
+    if constexpr (MAJ == MAJ_MN)
+        return stage[k * STR_MN + row];
+    else
+        return stage[pos_k(row, k)];
+}
+} // namespace dma64
+
+template <int MAJA, int MAJB>
+__global__ __launch_bounds__(NT, 2) void k_dgemm_dma(bxh_problem_t P, bxh_seg_t S)
+{
+    using namespace dma64;
+    constexpr int SA = stage_doubles<MAJA>(), SB = stage_doubles<MAJB>();
+    __shared__ __attribute__((aligned(16))) double smem[2 * (SA + SB)];
+    int b = blockIdx.x;
+    const int nb = gridDim.x;
+    if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+    int tm, tn;
+    tile_coords(b, P.tiles_m, P.tiles_n, tm, tn);
+    const int m0 = tm * BM, n0 = tn * BM;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lg = lane >> 4;
+    const double *A = (const double *)S.A, *B = (const double *)S.B;
+    const int ktiles = (int)(S.k / BK);
+
+    d4_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = d4_t{0, 0, 0, 0};
+
+    auto stageA = [&](int buf) { return smem + buf * SA; };
+    auto stageB = [&](int buf) { return smem + 2 * SA + buf * SB; };
+    auto issue = [&](int t, int buf) {
+        dma_tile<MAJA>(A, S.lda, m0, (int64_t)t * BK, stageA(buf), wave, lane);
+        dma_tile<MAJB>(B, S.ldb, n0, (int64_t)t * BK, stageB(buf), wave, lane);
+    };
+    issue(0, 0);
+    wait_dma();
+    __builtin_amdgcn_s_barrier();
+    for (int t = 0; t < ktiles; ++t) {
+        const int buf = t & 1;
+        if (t + 1 < ktiles) issue(t + 1, buf ^ 1);
+        const double *As = stageA(buf), *Bs = stageB(buf);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            double a[4], bb[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                a[i] = frag<MAJA>(As, wm * 64 + i * 16 + l16, s, lg);
+                bb[i] = frag<MAJB>(Bs, wn * 64 + i * 16 + l16, s, lg);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], bb[j], acc[i][j]);
+        }
+        wait_dma();
+        __builtin_amdgcn_s_barrier();
+    }
+
+    double *Cp = (double *)P.C;
+    const double alpha = P.alpha_re, beta = P.beta_re;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = n0 + wn * 64 + j * 16 + l16;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = m0 + wm * 64 + i * 16 + Cfg<double>::out_row(lg, v);
+                double *c = Cp + row + (int64_t)col * P.ldc;
+                const double x = alpha * acc[i][j][v];
+                gstore(c, P.beta_zero ? x : x + beta * gload(c));
+            }
+        }
+}
+
 template <typename E, int MAJA, int MAJB, bool CJA, bool CJB>
 __device__ __forceinline__ void run_tile(const bxh_problem_t &P, const bxh_seg_t *segs, int local,
                                          E *smem)
@@ -620,10 +777,37 @@ int launch_plain(const bxh_problem_t &P, const bxh_seg_t &S, hipStream_t st)
     return e == hipSuccess ? 0 : fail("gemm launch", e);
 }
 
+template <int MAJA, int MAJB>
+int launch_dgemm_dma(const bxh_problem_t &P, const bxh_seg_t &S, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_dgemm_dma<MAJA, MAJB>), dim3(P.tiles_m * P.tiles_n), dim3(NT), 0, st, P,
+                       S);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : fail("dgemm dma launch", e);
+}
+
+// the DMA-staged FP64 kernel's conditions: no edge tiles, 16-byte aligned
+// operands and leading dimensions
+bool dgemm_dma_ok(const bxh_problem_t &P, const bxh_seg_t &S)
+{
+    return P.m % 128 == 0 && P.n % 128 == 0 && S.k >= 16 && S.k % 16 == 0 &&
+           aligned16(S.A, S.lda, 8) && aligned16(S.B, S.ldb, 8);
+}
+
 template <typename E> int dispatch_real(int opa, int opb, const bxh_problem_t &P, const bxh_seg_t &S,
                                         hipStream_t st)
 {
     const int ta = opa != 0, tb = opb != 0;
+    if constexpr (sizeof(E) == 8) {
+        if (dgemm_dma_ok(P, S)) {
+            switch (ta * 2 + tb) {
+            case 0: return launch_dgemm_dma<MAJ_MN, MAJ_K>(P, S, st);
+            case 1: return launch_dgemm_dma<MAJ_MN, MAJ_MN>(P, S, st);
+            case 2: return launch_dgemm_dma<MAJ_K, MAJ_K>(P, S, st);
+            default: return launch_dgemm_dma<MAJ_K, MAJ_MN>(P, S, st);
+            }
+        }
+    }
     switch (ta * 2 + tb) {
     case 0: return launch_plain<E, MAJ_MN, MAJ_K, false, false>(P, S, st);
     case 1: return launch_plain<E, MAJ_MN, MAJ_MN, false, false>(P, S, st);

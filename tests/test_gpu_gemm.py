@@ -87,6 +87,42 @@ def test_device_dgemm_shapes_integer_exact(gpu, orc, shape):
         assert np.array_equal(dC.download(), ref), (ta, tb, shape)
 
 
+@pytest.mark.parametrize("ta,tb", TRANS_REAL)
+def test_device_dgemm_dma_path(gpu, orc, ta, tb):
+    """Problems without edge tiles (M, N multiples of 128, K of 16, aligned
+    operands) take the DMA-staged FP64 kernel: padded leading dimensions,
+    several K tiles and a grid that is not a multiple of 8 blocks; bit-exact
+    on integer inputs, within the tolerance on uniform inputs, and beta = 0
+    ignores NaN in C."""
+    bx = gpu
+    rng = np.random.default_rng(31)
+    for (M, N, K, pad) in ((384, 640, 1040, 2), (1024, 512, 64, 0)):
+        say("device dgemm dma", ta, tb, M, N, K, pad)
+        ra, ca = (M, K) if ta == "N" else (K, M)
+        rb, cb = (K, N) if tb == "N" else (N, K)
+        Ai = np.asfortranarray(rng.integers(-256, 257, (ra + pad, ca)), dtype=np.float64)
+        Bi = np.asfortranarray(rng.integers(-256, 257, (rb + pad, cb)), dtype=np.float64)
+        C0 = np.asfortranarray(rng.integers(-1000, 1001, (M, N)), dtype=np.float64)
+        ref = C0.copy(order="F")
+        orc.gemm(ta, tb, M, N, K, 3.0, Ai, ra + pad, Bi, rb + pad, -2.0, ref, M)
+        dA, dB, dC = (bx.DeviceArray.from_host(x) for x in (Ai, Bi, C0))
+        bx.device_gemm(ta, tb, M, N, K, 3.0, dA.ptr, ra + pad, dB.ptr, rb + pad, -2.0, dC.ptr, M,
+                       np.float64)
+        bx.synchronize()
+        assert np.array_equal(dC.download(), ref), (ta, tb, M, N, K)
+        A = _mat(orc, ra, ca, np.float64, 3, 0)
+        B = _mat(orc, rb, cb, np.float64, 3, 1)
+        Cn = np.full((M, N), np.nan, order="F")
+        ref = np.zeros((M, N), order="F")
+        orc.gemm(ta, tb, M, N, K, 0.75, A, ra, B, rb, 0.0, ref, M)
+        dA, dB, dC = (bx.DeviceArray.from_host(x) for x in (A, B, Cn))
+        bx.device_gemm(ta, tb, M, N, K, 0.75, dA.ptr, ra, dB.ptr, rb, 0.0, dC.ptr, M, np.float64)
+        bx.synchronize()
+        out = dC.download()
+        assert not np.isnan(out).any()
+        _check(orc, out, ref, 0.75, A, B, 0.0, np.zeros((M, N)), K, np.float64, f"{ta}{tb} dma")
+
+
 def test_device_sgemm_integer_exact(gpu, orc):
     bx = gpu
     rng = np.random.default_rng(11)
