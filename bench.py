@@ -799,7 +799,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3),
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
-                     "kernel": "k_gemm_plain<double,MN,K> (FP64 MFMA 16x16x4)",
+                     "kernel": "k_dgemm_dma<MN,K> (FP64 MFMA 16x16x4, global->LDS DMA staging)",
                      "launch_ms": round(avg_ms, 3)},
     }
     if not args.no_cpu:
