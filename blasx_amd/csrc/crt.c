@@ -5,12 +5,12 @@
  *
  * One INT8 GEMM per modulus instead of one per digit pair.  Rows of op(A)
  * and columns of op(B) are scaled by their 2-norms (Cauchy-Schwarz: |sum_k
- * A'_ik B'_kj| <= ||A'_i|| ||B'_j|| < 2^(2 gamma) <= M / 8, gamma =
- * floor((log2 M - 3) / 2), M = product of the moduli), so every entry keeps
+ * A'_ik B'_kj| <= ||A'_i|| ||B'_j|| < 2^(2 gamma) < M / 2, gamma =
+ * floor((log2 M - 1) / 2), M = product of the moduli), so every entry keeps
  * gamma bits relative to its row's 2-norm.  In the worst case (all entries
- * of a row equal in magnitude) that is beta = floor((log2 M - log2 K - 3) /
+ * of a row equal in magnitude) that is beta = floor((log2 M - log2 K - 1) /
  * 2) bits relative to the row maximum (blasx_crt_bits), e.g. K = 16384:
- * L = 16 -> 54 bits, L = 18 -> 61 bits, where the digit scheme needs
+ * L = 16 -> 55 bits, L = 18 -> 62 bits, where the digit scheme needs
  * s(s+1)/2 = 36 products for 56 bits; rows whose RMS is below their maximum
  * keep log2(max / RMS) bits more.
  *
@@ -69,7 +69,7 @@ int bx_crt_gamma(int L)
 {
     double lm = 0.0;
     for (int l = 0; l < L; ++l) lm += log2((double)CRT_MODS[l]);
-    return (int)floor((lm - 3.0) / 2.0 - 0x1p-20);
+    return (int)floor((lm - 1.0) / 2.0 - 0x1p-20);
 }
 
 /* worst-case bits relative to the row maximum */
@@ -77,7 +77,7 @@ int bx_crt_beta(int K, int L)
 {
     double lm = 0.0;
     for (int l = 0; l < L; ++l) lm += log2((double)CRT_MODS[l]);
-    return (int)floor((lm - log2((double)(K > 0 ? K : 1)) - 3.0) / 2.0 - 0x1p-20);
+    return (int)floor((lm - log2((double)(K > 0 ? K : 1)) - 1.0) / 2.0 - 0x1p-20);
 }
 
 /* ---- naturals below 2^192 as 12 little-endian 16-bit digits in uint32 --- */

@@ -621,15 +621,17 @@ __global__ void k_zcombine(int m, int n, const double *__restrict__ pr, const do
 
 // ---------------------------------------------------------------------------
 // CRT split (Ozaki scheme II: Ozaki, Uchino, Imamura 2025).  Entries are
-// scaled to integers A'_ik = trunc(a_ik 2^(beta - e_i)), |A'| < 2^beta; for
-// each modulus m_l (pairwise coprime, <= 256) the symmetric residues
-// a_l = smod(A', m_l) in [-floor(m/2), ceil(m/2) - 1] fit int8, one exact
-// INT8 GEMM per modulus gives R_l = smod(a_l b_l^T, m_l), and the Chinese
-// remainder theorem recovers the integer product (|A'B'| < K 2^(2 beta) <=
-// M/8, M = prod m_l) in floating point:
+// scaled to integers A'_ik = trunc(a_ik 2^(beta - e_i)) with ||A'_i||_2 <
+// 2^beta (e_i from the row's 2-norm); for each modulus m_l (pairwise
+// coprime, <= 256) the symmetric residues a_l = smod(A', m_l) in
+// [-floor(m/2), ceil(m/2) - 1] fit int8, one exact INT8 GEMM per modulus
+// gives R_l = smod(a_l b_l^T, m_l), and the Chinese remainder theorem
+// recovers the integer product (|A'B'| < 2^(2 beta) < M/2 by Cauchy-Schwarz,
+// M = prod m_l, beta = floor((log2 M - 1) / 2)) in floating point:
 //   Q_k = sum_l Wk_l R_l      (w_l = (M/m_l)((M/m_l)^-1 mod m_l) cut into
 //                              40-bit chunks k = 1..4: exact sums)
-//   kq  = rint((Q_1 + Q_2) / M),  D_k = Q_k - kq M_k  (exact)
+//   kq  = rint((Q_1 + Q_2) / M)  (within 2^-38 of Q/M = kq + P/M, |P/M| <
+//         1/2 - 2^-21: exact),  D_k = Q_k - kq M_k  (exact)
 //   S   = ((D_1 + D_2) + D_3) + D_4 = A'B' (1 + 3u)
 // and C = alpha 2^(e_i + f_j - 2 beta) S + beta C.
 // ---------------------------------------------------------------------------

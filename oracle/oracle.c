@@ -597,7 +597,7 @@ void orc_zgemm_ozaki(char ta, char tb, int M, int N, int K, const double *alpha,
 /* the Chinese remainder theorem).  No reference counterpart; restates  */
 /* the method in the library's exact evaluation order:                  */
 /*   moduli m_1..m_L from ORC_CRT_MODS (pairwise coprime, <= 256),       */
-/*   beta = gamma = floor((sum log2 m_l - 3) / 2 - 2^-20),              */
+/*   beta = gamma = floor((sum log2 m_l - 1) / 2 - 2^-20),              */
 /*   e_i = orc_norm_exp(sum_k op(A)_ik^2): ||op(A)_i||_2 < 2^e_i, and   */
 /*   f_j likewise for the columns of op(B) (Cauchy-Schwarz scaling),    */
 /*   A'_ik = trunc(ldexp(op(A)_ik, beta - e_i)),  ||A'_i|| < 2^beta,    */
@@ -612,9 +612,10 @@ void orc_zgemm_ozaki(char ta, char tb, int M, int N, int K, const double *alpha,
 /*     D_k = Q_k - kq M_k         (exact: < 2^53 chunk units)           */
 /*     S = ((D_1 + D_2) + D_3) + D_4                                    */
 /*   and C = alpha * ldexp(S, e_i + f_j - 2 beta)  (+ fma(beta, C, .)). */
-/* |A'B'| <= ||A'_i|| ||B'_j|| < 2^(2 beta) <= M/8 (worst-case bits    */
+/* |A'B'| <= ||A'_i|| ||B'_j|| < 2^(2 beta) < M/2 (worst-case bits     */
 /* relative to the row maximum: orc_crt_beta), so Q = P + kq M with    */
-/* |P| < M/8 and kq                                                     */
+/* |P/M| < 1/2 - 2^-21 while (Q_1 + Q_2)/Md is within 2^-38 of Q/M:    */
+/* kq                                                                   */
 /* is exact.  The partial sum D_1 + .. + D_j equals P minus the lower  */
 /* chunks, so it is exact whenever it is small, and S = P (1 + 3u) for */
 /* every P, with no cancellation loss.                                  */
@@ -626,12 +627,14 @@ static const int ORC_CRT_MODS[20] = {256, 255, 253, 251, 247, 241, 239, 233, 229
 
 int orc_crt_modulus(int l) { return (l >= 0 && l < 20) ? ORC_CRT_MODS[l] : 0; }
 
-/* scaling bits relative to the row 2-norm: 2^(2 gamma) <= M / 8 */
+/* scaling bits relative to the row 2-norm: 2^(2 gamma + 1) < M (M is
+ * never a power of two, and the 2^-20 keeps 2^(2 gamma + 1) below
+ * M (1 - 2^-20)) */
 int orc_crt_gamma(int L)
 {
     double lm = 0.0;
     for (int l = 0; l < L; ++l) lm += log2((double)ORC_CRT_MODS[l]);
-    return (int)floor((lm - 3.0) / 2.0 - 0x1p-20);
+    return (int)floor((lm - 1.0) / 2.0 - 0x1p-20);
 }
 
 /* smallest e with ss (1 + 2^-30) < 4^e: from the binary exponent k of
@@ -649,7 +652,7 @@ int orc_crt_beta(int K, int L)
 {
     double lm = 0.0;
     for (int l = 0; l < L; ++l) lm += log2((double)ORC_CRT_MODS[l]);
-    return (int)floor((lm - log2((double)(K > 0 ? K : 1)) - 3.0) / 2.0 - 0x1p-20);
+    return (int)floor((lm - log2((double)(K > 0 ? K : 1)) - 1.0) / 2.0 - 0x1p-20);
 }
 
 int64_t orc_smod(int64_t v, int64_t m)

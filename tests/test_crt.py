@@ -64,13 +64,16 @@ def test_moduli_pairwise_coprime_and_beta(orc, bx):
         for L in range(2, 21):
             b = orc.crt_beta(K, L)
             assert L_.blasx_crt_bits(K, L) == b
-            if b >= 1:  # worst case: |A'B'| < K 2^(2b) <= M / 8 exactly
-                assert 8 * K * 2 ** (2 * b) <= math.prod(MODS[:L])
+            if b >= 1:  # worst case: |A'B'| < K 2^(2b) < M / 2 exactly
+                assert 2 * K * 2 ** (2 * b) < math.prod(MODS[:L])
     for L in range(2, 21):
-        # scaling by row 2-norms: |A'B'| <= ||A'_i|| ||B'_j|| < 2^(2g) <= M / 8,
-        # and g is the largest such integer
+        # scaling by row 2-norms: |A'B'| <= ||A'_i|| ||B'_j|| < 2^(2g) < M / 2
+        # with room for the floating quotient (2^(2g+1) < M (1 - 2^-20)), and
+        # g is the largest such integer
         g = orc.crt_gamma(L)
-        assert 8 * 2 ** (2 * g) <= math.prod(MODS[:L]) < 8 * 2 ** (2 * g + 2)
+        M = math.prod(MODS[:L])
+        assert 2 ** (2 * g + 1) * (2 ** 20) < M * (2 ** 20 - 1)
+        assert M < 2 * 2 ** (2 * g + 2)
 
 
 @pytest.mark.parametrize("L", [2, 5, 8, 13, 16, 20])
@@ -175,18 +178,26 @@ def test_norm_exponent_matches_definition(orc, bx):
 
 @pytest.mark.parametrize("L", [8, 12, 16, 20])
 def test_floating_crt_and_garner(orc, L):
-    """Every integer |P| < M/8 is recovered to 2u by the floating CRT, and
-    exactly by Garner's digits."""
+    """Every integer |P| < 2^(2 gamma) (< M/2) is recovered to 2u by the
+    floating CRT, and exactly by Garner's digits: the range ends, values
+    within 2^-19 of M/2, and random magnitudes."""
     ms = MODS[:L]
     M = math.prod(ms)
+    lim = 2 ** (2 * orc.crt_gamma(L))  # the scaling's bound on |A'B'|
     consts = orc.crt_consts(L)
     rng = random.Random(L)
-    for _ in range(1500):
-        P = rng.randrange(-2 ** rng.randrange(0, M.bit_length() - 3), 2 ** 60) \
-            if rng.random() < 0.2 else rng.randrange(-M // 8, M // 8) >> rng.randrange(0, 120)
+    ends = [lim - 1, -(lim - 1), lim // 2, -(lim // 3), 1, -1, 0]
+    for i in range(1500):
+        if i < len(ends):
+            P = ends[i]
+        elif rng.random() < 0.2:
+            P = rng.randrange(-2 ** rng.randrange(0, M.bit_length() - 1), 2 ** 60)
+            P = max(-(lim - 1), min(lim - 1, P))
+        else:
+            P = rng.randrange(-(lim - 1), lim) >> rng.randrange(0, 120)
         r = [_smod(P, m) for m in ms]
         S = orc.crt_float(r, consts)
-        assert abs(S - P) <= 2 * EPS / 2 * abs(P) + (0 if P else 0), (P, S)
+        assert abs(S - P) <= 2 * EPS / 2 * abs(P), (P, S)
         v = orc.crt_digits(r)
         acc, mult = 0, 1
         for d, m in zip(v, ms):
