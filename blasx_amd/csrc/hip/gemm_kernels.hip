@@ -536,22 +536,37 @@ __device__ __forceinline__ double frag(const double *stage, int row, int s, int 
 }
 } // namespace dma64
 
+namespace dma64 {
+// LDS the DMA block needs at most (doubles): two stages of the larger layout
+constexpr int SMEM_DOUBLES = 2 * 2 * (BK * STR_MN);
+
+// every segment of P has K a multiple of 16 (or is empty) and 16-byte aligned
+// operands, and P has no edge tiles: the block below can run it
+__device__ __forceinline__ bool eligible(const bxh_problem_t &P, const bxh_seg_t *segs,
+                                         const bxh_seg_t &seg0)
+{
+    if (P.m % BM || P.n % BM || P.nseg < 1) return false;
+    for (int i = 0; i < P.nseg; ++i) {
+        const bxh_seg_t S = seg_at(segs, seg0, i);
+        if (S.k <= 0) continue;
+        if (S.k % BK || !aligned16(S.A, S.lda, 8) || !aligned16(S.B, S.ldb, 8)) return false;
+    }
+    return true;
+}
+} // namespace dma64
+
+// One 128x128 block tile (tm, tn) of P over all its segments (a plain GEMM
+// has one, the tile runtime one per cached K tile), operands DMA-staged; the
+// K tiles of consecutive segments run as one pipeline.
 template <int MAJA, int MAJB>
-__global__ __launch_bounds__(NT, 2) void k_dgemm_dma(bxh_problem_t P, bxh_seg_t S)
+__device__ __forceinline__ void dma64_block(const bxh_problem_t &P, const bxh_seg_t *segs,
+                                            const bxh_seg_t &seg0, int tm, int tn, double *smem)
 {
     using namespace dma64;
     constexpr int SA = stage_doubles<MAJA>(), SB = stage_doubles<MAJB>();
-    __shared__ __attribute__((aligned(16))) double smem[2 * (SA + SB)];
-    int b = blockIdx.x;
-    const int nb = gridDim.x;
-    if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
-    int tm, tn;
-    tile_coords(b, P.tiles_m, P.tiles_n, tm, tn);
     const int m0 = tm * BM, n0 = tn * BM;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lg = lane >> 4;
-    const double *A = (const double *)S.A, *B = (const double *)S.B;
-    const int ktiles = (int)(S.k / BK);
 
     d4_t acc[4][4];
 #pragma unroll
@@ -561,16 +576,44 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm_dma(bxh_problem_t P, bxh_seg_t 
 
     auto stageA = [&](int buf) { return smem + buf * SA; };
     auto stageB = [&](int buf) { return smem + 2 * SA + buf * SB; };
-    auto issue = [&](int t, int buf) {
-        dma_tile<MAJA>(A, S.lda, m0, (int64_t)t * BK, stageA(buf), wave, lane);
-        dma_tile<MAJB>(B, S.ldb, n0, (int64_t)t * BK, stageB(buf), wave, lane);
+    // The K tile being DMA'd is tile k0 / 16 of segment si, whose fields are
+    // kept in `cur` (the table is read once per segment).  advance() moves to
+    // the next 16 of K, else to the first non-empty later segment; si reaches
+    // P.nseg when there is none.
+    int si = 0;
+    int64_t k0 = 0;
+    bxh_seg_t cur = seg_at(segs, seg0, 0);
+    auto next_nonempty = [&]() {
+        while (si < P.nseg) {
+            cur = seg_at(segs, seg0, si);
+            if (cur.k > 0) return;
+            ++si;
+        }
     };
-    issue(0, 0);
-    wait_dma();
-    __builtin_amdgcn_s_barrier();
-    for (int t = 0; t < ktiles; ++t) {
-        const int buf = t & 1;
-        if (t + 1 < ktiles) issue(t + 1, buf ^ 1);
+    auto advance = [&]() {
+        k0 += BK;
+        if (k0 < cur.k) return;
+        k0 = 0;
+        ++si;
+        next_nonempty();
+    };
+    auto issue = [&](int buf) {
+        dma_tile<MAJA>((const double *)cur.A, cur.lda, m0, k0, stageA(buf), wave, lane);
+        dma_tile<MAJB>((const double *)cur.B, cur.ldb, n0, k0, stageB(buf), wave, lane);
+    };
+    next_nonempty();
+    const bool any = si < P.nseg;
+    if (any) {
+        issue(0);
+        wait_dma();
+        __builtin_amdgcn_s_barrier();
+    }
+    int buf = 0;
+    bool more = any; // the tile in stage `buf` is valid
+    while (more) {
+        advance();
+        more = si < P.nseg;
+        if (more) issue(buf ^ 1);
         const double *As = stageA(buf), *Bs = stageB(buf);
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
@@ -587,6 +630,7 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm_dma(bxh_problem_t P, bxh_seg_t 
         }
         wait_dma();
         __builtin_amdgcn_s_barrier();
+        buf ^= 1;
     }
 
     double *Cp = (double *)P.C;
@@ -606,6 +650,18 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm_dma(bxh_problem_t P, bxh_seg_t 
         }
 }
 
+template <int MAJA, int MAJB>
+__global__ __launch_bounds__(NT, 2) void k_dgemm_dma(bxh_problem_t P, bxh_seg_t S)
+{
+    __shared__ __attribute__((aligned(16))) double smem[dma64::SMEM_DOUBLES];
+    int b = blockIdx.x;
+    const int nb = gridDim.x;
+    if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+    int tm, tn;
+    tile_coords(b, P.tiles_m, P.tiles_n, tm, tn);
+    dma64_block<MAJA, MAJB>(P, nullptr, S, tm, tn, smem);
+}
+
 template <typename E, int MAJA, int MAJB, bool CJA, bool CJB>
 __device__ __forceinline__ void run_tile(const bxh_problem_t &P, const bxh_seg_t *segs, int local,
                                          E *smem)
@@ -618,41 +674,56 @@ __device__ __forceinline__ void run_tile(const bxh_problem_t &P, const bxh_seg_t
     gemm_block<E, MAJA, MAJB, CJA, CJB>(P, segs + P.seg_off, unused, tm, tn, smem);
 }
 
-template <typename E>
+// MODE 0: every problem on the register-staged block; for FP64 MODE 1 runs
+// only the problems the DMA-staged block can take (dma64::eligible) and
+// MODE 2 only the others: two launches, so that neither kernel carries both
+// blocks' registers (one kernel with both spilled).
+template <typename E, int MODE>
 __device__ __forceinline__ void grouped_body(const bxh_problem_t *__restrict__ probs, int nprob,
                                              const bxh_seg_t *__restrict__ segs, int total,
                                              E *smem);
 
 // Grid-stride over the tiles of many problems (grouped GEMM / tile runtime).
-template <typename E>
+// LDS of a grouped kernel: the register-staged block's, and for FP64 also
+// the DMA-staged block's
+template <typename E> constexpr int grouped_smem_elems()
+{
+    return sizeof(E) == 8 && dma64::SMEM_DOUBLES > Smem<E>::ELEMS ? dma64::SMEM_DOUBLES
+                                                                  : Smem<E>::ELEMS;
+}
+
+template <typename E, int MODE>
 __global__ __launch_bounds__(NT, 2) void k_gemm_grouped(const bxh_problem_t *__restrict__ probs,
                                                         int nprob,
                                                         const bxh_seg_t *__restrict__ segs,
                                                         int total)
 {
-    __shared__ __attribute__((aligned(16))) E smem[Smem<E>::ELEMS];
-    grouped_body<E>(probs, nprob, segs, total, smem);
+    __shared__ __attribute__((aligned(16))) E smem[MODE == 1 ? grouped_smem_elems<E>()
+                                                              : Smem<E>::ELEMS];
+    grouped_body<E, MODE>(probs, nprob, segs, total, smem);
 }
 
 // Persistent variant for graph replay: the problem count and tile total are
 // read from device memory, so a captured launch is replayed with whatever
 // table the host uploaded before the replay.  counts[0] = nprob,
 // counts[1] = total tiles.
-template <typename E>
+template <typename E, int MODE>
 __global__ __launch_bounds__(NT, 2) void k_gemm_grouped_dyn(const bxh_problem_t *__restrict__ probs,
                                                             const bxh_seg_t *__restrict__ segs,
                                                             const int *__restrict__ counts)
 {
-    __shared__ __attribute__((aligned(16))) E smem[Smem<E>::ELEMS];
-    grouped_body<E>(probs, counts[0], segs, counts[1], smem);
+    __shared__ __attribute__((aligned(16))) E smem[MODE == 1 ? grouped_smem_elems<E>()
+                                                              : Smem<E>::ELEMS];
+    grouped_body<E, MODE>(probs, counts[0], segs, counts[1], smem);
 }
 
-template <typename E>
+template <typename E, int MODE>
 __device__ __forceinline__ void grouped_body(const bxh_problem_t *__restrict__ probs, int nprob,
                                              const bxh_seg_t *__restrict__ segs, int total,
                                              E *smem)
 {
     constexpr bool CPLX = sizeof(E) == 16;
+    static_assert(MODE == 0 || sizeof(E) == 8, "the DMA split is FP64 only");
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
         int lo = 0, hi = nprob - 1;
         while (lo < hi) {
@@ -678,11 +749,30 @@ __device__ __forceinline__ void grouped_body(const bxh_problem_t *__restrict__ p
             }
         } else {
             const int ta = P.opa != 0, tb = P.opb != 0;
-            switch (ta * 2 + tb) {
-            case 0: run_tile<E, MAJ_MN, MAJ_K, false, false>(P, segs, local, smem); break;
-            case 1: run_tile<E, MAJ_MN, MAJ_MN, false, false>(P, segs, local, smem); break;
-            case 2: run_tile<E, MAJ_K, MAJ_K, false, false>(P, segs, local, smem); break;
-            default: run_tile<E, MAJ_K, MAJ_MN, false, false>(P, segs, local, smem); break;
+            if constexpr (MODE != 0) {
+                const bxh_seg_t unused = {nullptr, nullptr, 0, 0, 0};
+                const bxh_seg_t *ps = segs + P.seg_off;
+                if (dma64::eligible(P, ps, unused) != (MODE == 1)) continue; // the other launch's
+                if constexpr (MODE == 1) {
+                    int tm, tn;
+                    tile_coords(local, P.tiles_m, P.tiles_n, tm, tn);
+                    double *sd = reinterpret_cast<double *>(smem);
+                    switch (ta * 2 + tb) {
+                    case 0: dma64_block<MAJ_MN, MAJ_K>(P, ps, unused, tm, tn, sd); break;
+                    case 1: dma64_block<MAJ_MN, MAJ_MN>(P, ps, unused, tm, tn, sd); break;
+                    case 2: dma64_block<MAJ_K, MAJ_K>(P, ps, unused, tm, tn, sd); break;
+                    default: dma64_block<MAJ_K, MAJ_MN>(P, ps, unused, tm, tn, sd); break;
+                    }
+                    continue;
+                }
+            }
+            if constexpr (MODE != 1) {
+                switch (ta * 2 + tb) {
+                case 0: run_tile<E, MAJ_MN, MAJ_K, false, false>(P, segs, local, smem); break;
+                case 1: run_tile<E, MAJ_MN, MAJ_MN, false, false>(P, segs, local, smem); break;
+                case 2: run_tile<E, MAJ_K, MAJ_K, false, false>(P, segs, local, smem); break;
+                default: run_tile<E, MAJ_K, MAJ_MN, false, false>(P, segs, local, smem); break;
+                }
             }
         }
     }
@@ -893,16 +983,18 @@ int bxh_gemm_grouped(int dtype, const bxh_problem_t *probs_dev, int nprob,
     hipStream_t st = (hipStream_t)stream;
     switch (dtype) {
     case BLASX_D:
-        hipLaunchKernelGGL(k_gemm_grouped<double>, dim3(grid), dim3(NT), 0, st, probs_dev, nprob,
-                           segs_dev, total_tiles);
+        hipLaunchKernelGGL((k_gemm_grouped<double, 1>), dim3(grid), dim3(NT), 0, st, probs_dev,
+                           nprob, segs_dev, total_tiles);
+        hipLaunchKernelGGL((k_gemm_grouped<double, 2>), dim3(grid), dim3(NT), 0, st, probs_dev,
+                           nprob, segs_dev, total_tiles);
         break;
     case BLASX_S:
-        hipLaunchKernelGGL(k_gemm_grouped<float>, dim3(grid), dim3(NT), 0, st, probs_dev, nprob,
-                           segs_dev, total_tiles);
+        hipLaunchKernelGGL((k_gemm_grouped<float, 0>), dim3(grid), dim3(NT), 0, st, probs_dev,
+                           nprob, segs_dev, total_tiles);
         break;
     case BLASX_Z:
-        hipLaunchKernelGGL(k_gemm_grouped<cplx>, dim3(grid), dim3(NT), 0, st, probs_dev, nprob,
-                           segs_dev, total_tiles);
+        hipLaunchKernelGGL((k_gemm_grouped<cplx, 0>), dim3(grid), dim3(NT), 0, st, probs_dev,
+                           nprob, segs_dev, total_tiles);
         break;
     default: return BLASX_EINVAL;
     }
@@ -917,15 +1009,17 @@ int bxh_gemm_grouped_dyn(int dtype, const bxh_problem_t *probs_dev, const bxh_se
     hipStream_t st = (hipStream_t)stream;
     switch (dtype) {
     case BLASX_D:
-        hipLaunchKernelGGL(k_gemm_grouped_dyn<double>, dim3(grid), dim3(NT), 0, st, probs_dev,
-                           segs_dev, counts_dev);
+        hipLaunchKernelGGL((k_gemm_grouped_dyn<double, 1>), dim3(grid), dim3(NT), 0, st,
+                           probs_dev, segs_dev, counts_dev);
+        hipLaunchKernelGGL((k_gemm_grouped_dyn<double, 2>), dim3(grid), dim3(NT), 0, st,
+                           probs_dev, segs_dev, counts_dev);
         break;
     case BLASX_S:
-        hipLaunchKernelGGL(k_gemm_grouped_dyn<float>, dim3(grid), dim3(NT), 0, st, probs_dev,
-                           segs_dev, counts_dev);
+        hipLaunchKernelGGL((k_gemm_grouped_dyn<float, 0>), dim3(grid), dim3(NT), 0, st,
+                           probs_dev, segs_dev, counts_dev);
         break;
     case BLASX_Z:
-        hipLaunchKernelGGL(k_gemm_grouped_dyn<cplx>, dim3(grid), dim3(NT), 0, st, probs_dev,
+        hipLaunchKernelGGL((k_gemm_grouped_dyn<cplx, 0>), dim3(grid), dim3(NT), 0, st, probs_dev,
                            segs_dev, counts_dev);
         break;
     default: return BLASX_EINVAL;
