@@ -662,6 +662,137 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm_dma(bxh_problem_t P, bxh_seg_t 
     dma64_block<MAJA, MAJB>(P, nullptr, S, tm, tn, smem);
 }
 
+// ---------------------------------------------------------------------------
+// ZGEMM with global -> LDS DMA staging (M a multiple of 128, N of 64, K of 8;
+// complex128 operands are always 16-byte aligned).  Same scheme as the FP64
+// kernel; a complex element is exactly one DMA lane's 16 bytes and one
+// 16-byte fragment read, so K steps map to lane groups directly (step s,
+// group lg: k = 4 s + lg).  A row-contiguous operand is staged [k][row]
+// unpadded (a fragment read is 16 lanes x 16 bytes = one 256-byte LDS pass);
+// a K-contiguous one [row][k], 8 elements per row, element k of row r at
+// position k ^ ((r >> 1) & 7).  Conjugation is a sign flip of the imaginary
+// part as the fragment is read.  4M: four FP64 MFMAs per complex MAC.
+// ---------------------------------------------------------------------------
+namespace dmaz {
+constexpr int BM = 128, BN = 64, BK = 8;
+template <int MAJ, int ROWS> constexpr int stage_elems() { return BK * ROWS; }
+
+template <int MAJ, int ROWS>
+__device__ __forceinline__ void dma_tile(const cplx *X, int64_t ld, int r0, int64_t k0, cplx *stage,
+                                         int wave, int lane)
+{
+    constexpr int RUNS = ROWS * BK / 64; // 1 KB runs per stage
+#pragma unroll
+    for (int i = 0; i < RUNS / 4; ++i) {
+        const int q = wave * (RUNS / 4) + i;
+        if constexpr (MAJ == MAJ_MN) {
+            // K row q / (ROWS / 64), elements 64 (q % (ROWS / 64)) + lane
+            constexpr int PER = ROWS / 64;
+            const int kk = q / PER, rr = (q % PER) * 64 + lane;
+            dma64::dma16((const double *)(X + (r0 + rr) + (k0 + kk) * ld),
+                         (const double *)(stage + kk * ROWS + (q % PER) * 64));
+        } else {
+            // rows 8q .. 8q + 7, lane -> row 8q + lane / 8, position lane % 8
+            const int r = 8 * q + (lane >> 3);
+            const int k = (lane & 7) ^ ((r >> 1) & 7);
+            dma64::dma16((const double *)(X + (k0 + k) + (int64_t)(r0 + r) * ld),
+                         (const double *)(stage + q * 64));
+        }
+    }
+}
+
+template <int MAJ, int ROWS, bool CJ>
+__device__ __forceinline__ cplx frag(const cplx *stage, int row, int s, int lg)
+{
+    const int k = 4 * s + lg;
+    const cplx v = MAJ == MAJ_MN ? stage[k * ROWS + row]
+                                 : stage[row * BK + (k ^ ((row >> 1) & 7))];
+    return CJ ? cplx{v.re, -v.im} : v;
+}
+} // namespace dmaz
+
+template <int MAJA, int MAJB, bool CJA, bool CJB>
+__global__ __launch_bounds__(NT, 2) void k_zgemm_dma(bxh_problem_t P, bxh_seg_t S)
+{
+    using namespace dmaz;
+    constexpr int SA = stage_elems<MAJA, BM>(), SB = stage_elems<MAJB, BN>();
+    __shared__ __attribute__((aligned(16))) cplx smem[2 * (SA + SB)];
+    int b = blockIdx.x;
+    const int nb = gridDim.x;
+    if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+    int tm, tn;
+    tile_coords(b, P.tiles_m, P.tiles_n, tm, tn);
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lg = lane >> 4;
+    const cplx *A = (const cplx *)S.A, *B = (const cplx *)S.B;
+    const int ktiles = (int)(S.k / BK);
+
+    d4_t acc[4][2][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j][0] = acc[i][j][1] = d4_t{0, 0, 0, 0};
+
+    auto stageA = [&](int buf) { return smem + buf * SA; };
+    auto stageB = [&](int buf) { return smem + 2 * SA + buf * SB; };
+    auto issue = [&](int t, int buf) {
+        dma_tile<MAJA, BM>(A, S.lda, m0, (int64_t)t * BK, stageA(buf), wave, lane);
+        dma_tile<MAJB, BN>(B, S.ldb, n0, (int64_t)t * BK, stageB(buf), wave, lane);
+    };
+    issue(0, 0);
+    dma64::wait_dma();
+    __builtin_amdgcn_s_barrier();
+    for (int t = 0; t < ktiles; ++t) {
+        const int buf = t & 1;
+        if (t + 1 < ktiles) issue(t + 1, buf ^ 1);
+        const cplx *As = stageA(buf), *Bs = stageB(buf);
+#pragma unroll
+        for (int s = 0; s < BK / 4; ++s) {
+            cplx a[4], bb[2];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = frag<MAJA, BM, CJA>(As, wm * 64 + i * 16 + l16, s, lg);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) bb[j] = frag<MAJB, BN, CJB>(Bs, wn * 32 + j * 16 + l16, s, lg);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const double nai = -a[i].im;
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    acc[i][j][0] = mfma(a[i].re, bb[j].re, acc[i][j][0]);
+                    acc[i][j][1] = mfma(a[i].re, bb[j].im, acc[i][j][1]);
+                    acc[i][j][0] = mfma(nai, bb[j].im, acc[i][j][0]);
+                    acc[i][j][1] = mfma(a[i].im, bb[j].re, acc[i][j][1]);
+                }
+            }
+        }
+        dma64::wait_dma();
+        __builtin_amdgcn_s_barrier();
+    }
+
+    cplx *Cp = (cplx *)P.C;
+    const double ar = P.alpha_re, ai = P.alpha_im, br = P.beta_re, bi = P.beta_im;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int col = n0 + wn * 32 + j * 16 + l16;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = m0 + wm * 64 + i * 16 + Cfg<cplx>::out_row(lg, v);
+                const double sr = acc[i][j][0][v], sim = acc[i][j][1][v];
+                double orr = ar * sr - ai * sim, oi = ar * sim + ai * sr;
+                cplx *c = Cp + row + (int64_t)col * P.ldc;
+                if (!P.beta_zero) {
+                    const cplx c0 = gload(c);
+                    orr += br * c0.re - bi * c0.im;
+                    oi += br * c0.im + bi * c0.re;
+                }
+                gstore(c, cplx{orr, oi});
+            }
+        }
+}
+
 template <typename E, int MAJA, int MAJB, bool CJA, bool CJB>
 __device__ __forceinline__ void run_tile(const bxh_problem_t &P, const bxh_seg_t *segs, int local,
                                          E *smem)
@@ -906,8 +1037,32 @@ template <typename E> int dispatch_real(int opa, int opb, const bxh_problem_t &P
     }
 }
 
+template <int MAJA, int MAJB, bool CJA, bool CJB>
+int launch_zgemm_dma(const bxh_problem_t &P, const bxh_seg_t &S, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_zgemm_dma<MAJA, MAJB, CJA, CJB>), dim3(P.tiles_m * P.tiles_n), dim3(NT),
+                       0, st, P, S);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : fail("zgemm dma launch", e);
+}
+
 int dispatch_cplx(int opa, int opb, const bxh_problem_t &P, const bxh_seg_t &S, hipStream_t st)
 {
+    // the DMA-staged kernel: no edge tiles, 16-byte aligned operands
+    if (P.m % dmaz::BM == 0 && P.n % dmaz::BN == 0 && S.k >= dmaz::BK && S.k % dmaz::BK == 0 &&
+        aligned16(S.A, S.lda, 16) && aligned16(S.B, S.ldb, 16)) {
+        switch (opa * 3 + opb) {
+        case 0: return launch_zgemm_dma<MAJ_MN, MAJ_K, false, false>(P, S, st);
+        case 1: return launch_zgemm_dma<MAJ_MN, MAJ_MN, false, false>(P, S, st);
+        case 2: return launch_zgemm_dma<MAJ_MN, MAJ_MN, false, true>(P, S, st);
+        case 3: return launch_zgemm_dma<MAJ_K, MAJ_K, false, false>(P, S, st);
+        case 4: return launch_zgemm_dma<MAJ_K, MAJ_MN, false, false>(P, S, st);
+        case 5: return launch_zgemm_dma<MAJ_K, MAJ_MN, false, true>(P, S, st);
+        case 6: return launch_zgemm_dma<MAJ_K, MAJ_K, true, false>(P, S, st);
+        case 7: return launch_zgemm_dma<MAJ_K, MAJ_MN, true, false>(P, S, st);
+        default: return launch_zgemm_dma<MAJ_K, MAJ_MN, true, true>(P, S, st);
+        }
+    }
     switch (opa * 3 + opb) {
     case 0: return launch_plain<cplx, MAJ_MN, MAJ_K, false, false>(P, S, st);
     case 1: return launch_plain<cplx, MAJ_MN, MAJ_MN, false, false>(P, S, st);

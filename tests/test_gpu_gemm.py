@@ -161,6 +161,44 @@ def test_device_zgemm_integer_exact_conj(gpu, orc):
         assert np.array_equal(dC.download(), exact), (ta, tb)
 
 
+@pytest.mark.parametrize("ta,tb", TRANS_CPLX)
+def test_device_zgemm_dma_path(gpu, orc, ta, tb):
+    """ZGEMMs without edge tiles (M a multiple of 128, N of 64, K of 8) take
+    the DMA-staged kernel: every transpose and conjugate pair, padded leading
+    dimensions, several K tiles; bit-exact on Gaussian-integer inputs, within
+    the tolerance on uniform inputs, and beta = 0 ignores NaN in C."""
+    bx = gpu
+    rng = np.random.default_rng(41)
+    M, N, K, pad = 256, 192, 72, 3
+    ra, ca = (M, K) if ta == "N" else (K, M)
+    rb, cb = (K, N) if tb == "N" else (N, K)
+
+    def ci(shape):
+        return np.asfortranarray(rng.integers(-64, 65, shape) + 1j * rng.integers(-64, 65, shape))
+    say("device zgemm dma", ta, tb, M, N, K)
+    A, B, C0 = ci((ra + pad, ca)), ci((rb + pad, cb)), ci((M, N))
+    al, be = 2 - 1j, 1 + 3j
+    exact = al * (_op(A[:ra], ta) @ _op(B[:rb], tb)) + be * C0
+    dA, dB, dC = (bx.DeviceArray.from_host(x) for x in (A, B, C0))
+    bx.device_gemm(ta, tb, M, N, K, al, dA.ptr, ra + pad, dB.ptr, rb + pad, be, dC.ptr, M,
+                   np.complex128)
+    bx.synchronize()
+    assert np.array_equal(dC.download(), exact), (ta, tb)
+    A = _mat(orc, ra, ca, np.complex128, 4, 0)
+    B = _mat(orc, rb, cb, np.complex128, 4, 1)
+    Cn = np.full((M, N), np.nan + 1j * np.nan, order="F")
+    ref = np.zeros((M, N), dtype=np.complex128, order="F")
+    orc.gemm(ta, tb, M, N, K, 0.5 + 0.25j, A, ra, B, rb, 0.0, ref, M)
+    dA, dB, dC = (bx.DeviceArray.from_host(x) for x in (A, B, Cn))
+    bx.device_gemm(ta, tb, M, N, K, 0.5 + 0.25j, dA.ptr, ra, dB.ptr, rb, 0.0, dC.ptr, M,
+                   np.complex128)
+    bx.synchronize()
+    out = dC.download()
+    assert not np.isnan(out).any()
+    _check(orc, out, ref, 0.5 + 0.25j, A, B, 0.0, np.zeros((M, N), dtype=np.complex128), K,
+           np.complex128, f"{ta}{tb} zdma")
+
+
 def test_device_beta_zero_ignores_nan(gpu, orc):
     bx = gpu
     M, N, K = 300, 200, 100
