@@ -793,6 +793,131 @@ __global__ __launch_bounds__(NT, 2) void k_zgemm_dma(bxh_problem_t P, bxh_seg_t 
         }
 }
 
+// ---------------------------------------------------------------------------
+// SGEMM with global -> LDS DMA staging (M, N multiples of 128, K of 32,
+// 16-byte aligned operands).  Same scheme as the FP64 kernel; a DMA lane
+// carries four floats:
+//  * a row-contiguous operand is staged [k][row], 128 floats per K row, its
+//    16-byte chunks (four rows) at position chunk ^ (4 ((k >> 2) & 3)) so
+//    that the four lane groups of a fragment read (K rows 4 apart) fall on
+//    different banks; one DMA instruction fills two K rows;
+//  * a K-contiguous operand is staged [row][k], 32 floats per row as eight
+//    16-byte chunks (four K values), chunk c at position c ^ ((row >> 1) & 7);
+//    one 16-byte read gives a lane four K steps.
+// K step s, lane group lg: k = 4 (lg + 4 (s / 4)) + s % 4.
+// ---------------------------------------------------------------------------
+namespace dmas {
+constexpr int BM = 128, BK = 32;
+constexpr int STAGE = BM * BK; // floats per operand per stage (both layouts)
+
+__device__ __forceinline__ int pos_mn(int row, int k)
+{
+    return k * BM + ((((row >> 2) ^ (((k >> 2) & 3) << 2))) << 2) + (row & 3);
+}
+__device__ __forceinline__ int pos_k(int row, int k)
+{
+    return row * BK + ((((k >> 2) ^ ((row >> 1) & 7))) << 2) + (k & 3);
+}
+
+template <int MAJ>
+__device__ __forceinline__ void dma_tile(const float *X, int64_t ld, int r0, int64_t k0,
+                                         float *stage, int wave, int lane)
+{
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int q = wave * 4 + i; // 1 KB run of the stage
+        if constexpr (MAJ == MAJ_MN) {
+            // K rows 2q, 2q + 1; lane -> K row 2q + lane / 32, position lane % 32
+            const int k = 2 * q + (lane >> 5), p = lane & 31;
+            const int chunk = p ^ (((k >> 2) & 3) << 2);
+            dma64::dma16((const double *)(X + (r0 + 4 * chunk) + (k0 + k) * ld),
+                         (const double *)(stage + q * 256));
+        } else {
+            // rows 8q .. 8q + 7, lane -> row 8q + lane / 8, position lane % 8
+            const int r = 8 * q + (lane >> 3);
+            const int c = (lane & 7) ^ ((r >> 1) & 7);
+            dma64::dma16((const double *)(X + (k0 + 4 * c) + (int64_t)(r0 + r) * ld),
+                         (const double *)(stage + q * 256));
+        }
+    }
+}
+
+template <int MAJ> __device__ __forceinline__ float frag(const float *stage, int row, int s, int lg)
+{
+    const int k = 4 * (lg + 4 * (s >> 2)) + (s & 3);
+    return MAJ == MAJ_MN ? stage[pos_mn(row, k)] : stage[pos_k(row, k)];
+}
+} // namespace dmas
+
+template <int MAJA, int MAJB>
+__global__ __launch_bounds__(NT, 2) void k_sgemm_dma(bxh_problem_t P, bxh_seg_t S)
+{
+    using namespace dmas;
+    __shared__ __attribute__((aligned(16))) float smem[2 * 2 * STAGE];
+    int b = blockIdx.x;
+    const int nb = gridDim.x;
+    if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+    int tm, tn;
+    tile_coords(b, P.tiles_m, P.tiles_n, tm, tn);
+    const int m0 = tm * BM, n0 = tn * BM;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lg = lane >> 4;
+    const float *A = (const float *)S.A, *B = (const float *)S.B;
+    const int ktiles = (int)(S.k / BK);
+
+    f4_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{0, 0, 0, 0};
+
+    auto stageA = [&](int buf) { return smem + buf * STAGE; };
+    auto stageB = [&](int buf) { return smem + 2 * STAGE + buf * STAGE; };
+    auto issue = [&](int t, int buf) {
+        dma_tile<MAJA>(A, S.lda, m0, (int64_t)t * BK, stageA(buf), wave, lane);
+        dma_tile<MAJB>(B, S.ldb, n0, (int64_t)t * BK, stageB(buf), wave, lane);
+    };
+    issue(0, 0);
+    dma64::wait_dma();
+    __builtin_amdgcn_s_barrier();
+    for (int t = 0; t < ktiles; ++t) {
+        const int buf = t & 1;
+        if (t + 1 < ktiles) issue(t + 1, buf ^ 1);
+        const float *As = stageA(buf), *Bs = stageB(buf);
+#pragma unroll
+        for (int s = 0; s < BK / 4; ++s) {
+            float a[4], bb[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                a[i] = frag<MAJA>(As, wm * 64 + i * 16 + l16, s, lg);
+                bb[i] = frag<MAJB>(Bs, wn * 64 + i * 16 + l16, s, lg);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], bb[j], acc[i][j]);
+        }
+        dma64::wait_dma();
+        __builtin_amdgcn_s_barrier();
+    }
+
+    float *Cp = (float *)P.C;
+    const float alpha = (float)P.alpha_re, beta = (float)P.beta_re;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = n0 + wn * 64 + j * 16 + l16;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = m0 + wm * 64 + i * 16 + Cfg<float>::out_row(lg, v);
+                float *c = Cp + row + (int64_t)col * P.ldc;
+                const float x = alpha * acc[i][j][v];
+                gstore(c, P.beta_zero ? x : x + beta * gload(c));
+            }
+        }
+}
+
 template <typename E, int MAJA, int MAJB, bool CJA, bool CJB>
 __device__ __forceinline__ void run_tile(const bxh_problem_t &P, const bxh_seg_t *segs, int local,
                                          E *smem)
@@ -1015,10 +1140,30 @@ bool dgemm_dma_ok(const bxh_problem_t &P, const bxh_seg_t &S)
            aligned16(S.A, S.lda, 8) && aligned16(S.B, S.ldb, 8);
 }
 
+template <int MAJA, int MAJB>
+int launch_sgemm_dma(const bxh_problem_t &P, const bxh_seg_t &S, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_sgemm_dma<MAJA, MAJB>), dim3(P.tiles_m * P.tiles_n), dim3(NT), 0, st, P,
+                       S);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : fail("sgemm dma launch", e);
+}
+
 template <typename E> int dispatch_real(int opa, int opb, const bxh_problem_t &P, const bxh_seg_t &S,
                                         hipStream_t st)
 {
     const int ta = opa != 0, tb = opb != 0;
+    if constexpr (sizeof(E) == 4) {
+        if (P.m % dmas::BM == 0 && P.n % dmas::BM == 0 && S.k >= dmas::BK &&
+            S.k % dmas::BK == 0 && aligned16(S.A, S.lda, 4) && aligned16(S.B, S.ldb, 4)) {
+            switch (ta * 2 + tb) {
+            case 0: return launch_sgemm_dma<MAJ_MN, MAJ_K>(P, S, st);
+            case 1: return launch_sgemm_dma<MAJ_MN, MAJ_MN>(P, S, st);
+            case 2: return launch_sgemm_dma<MAJ_K, MAJ_K>(P, S, st);
+            default: return launch_sgemm_dma<MAJ_K, MAJ_MN>(P, S, st);
+            }
+        }
+    }
     if constexpr (sizeof(E) == 8) {
         if (dgemm_dma_ok(P, S)) {
             switch (ta * 2 + tb) {

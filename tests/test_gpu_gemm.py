@@ -161,6 +161,43 @@ def test_device_zgemm_integer_exact_conj(gpu, orc):
         assert np.array_equal(dC.download(), exact), (ta, tb)
 
 
+@pytest.mark.parametrize("ta,tb", TRANS_REAL)
+def test_device_sgemm_dma_path(gpu, orc, ta, tb):
+    """SGEMMs without edge tiles (M, N multiples of 128, K of 32, aligned
+    operands) take the DMA-staged FP32 kernel: padded leading dimensions,
+    several K tiles; bit-exact on small-integer inputs (every partial sum is
+    exact in FP32), within the tolerance on uniform inputs, and beta = 0
+    ignores NaN in C."""
+    bx = gpu
+    rng = np.random.default_rng(43)
+    M, N, K, pad = 256, 384, 96, 4
+    ra, ca = (M, K) if ta == "N" else (K, M)
+    rb, cb = (K, N) if tb == "N" else (N, K)
+    say("device sgemm dma", ta, tb, M, N, K)
+    A = np.asfortranarray(rng.integers(-16, 17, (ra + pad, ca)), dtype=np.float32)
+    B = np.asfortranarray(rng.integers(-16, 17, (rb + pad, cb)), dtype=np.float32)
+    C0 = np.asfortranarray(rng.integers(-64, 65, (M, N)), dtype=np.float32)
+    exact = (3 * (_op(A[:ra], ta).astype(np.int64) @ _op(B[:rb], tb).astype(np.int64))
+             - 2 * C0.astype(np.int64)).astype(np.float32)
+    dA, dB, dC = (bx.DeviceArray.from_host(x) for x in (A, B, C0))
+    bx.device_gemm(ta, tb, M, N, K, 3.0, dA.ptr, ra + pad, dB.ptr, rb + pad, -2.0, dC.ptr, M,
+                   np.float32)
+    bx.synchronize()
+    assert np.array_equal(dC.download(), exact), (ta, tb)
+    A = _mat(orc, ra, ca, np.float32, 5, 0)
+    B = _mat(orc, rb, cb, np.float32, 5, 1)
+    Cn = np.full((M, N), np.nan, dtype=np.float32, order="F")
+    ref = np.zeros((M, N), dtype=np.float32, order="F")
+    orc.gemm(ta, tb, M, N, K, 0.75, A, ra, B, rb, 0.0, ref, M, acc64=True)
+    dA, dB, dC = (bx.DeviceArray.from_host(x) for x in (A, B, Cn))
+    bx.device_gemm(ta, tb, M, N, K, 0.75, dA.ptr, ra, dB.ptr, rb, 0.0, dC.ptr, M, np.float32)
+    bx.synchronize()
+    out = dC.download()
+    assert not np.isnan(out).any()
+    _check(orc, out, ref, 0.75, A, B, 0.0, np.zeros((M, N), dtype=np.float32), K, np.float32,
+           f"{ta}{tb} sdma")
+
+
 @pytest.mark.parametrize("ta,tb", TRANS_CPLX)
 def test_device_zgemm_dma_path(gpu, orc, ta, tb):
     """ZGEMMs without edge tiles (M a multiple of 128, N of 64, K of 8) take
