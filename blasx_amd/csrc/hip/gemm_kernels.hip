@@ -712,85 +712,19 @@ __device__ __forceinline__ cplx frag(const cplx *stage, int row, int s, int lg)
 } // namespace dmaz
 
 template <int MAJA, int MAJB, bool CJA, bool CJB>
+__device__ __forceinline__ void dmaz_block(const bxh_problem_t &P, const bxh_seg_t *segs,
+                                           const bxh_seg_t &seg0, int tm, int tn, cplx *smem);
+
+template <int MAJA, int MAJB, bool CJA, bool CJB>
 __global__ __launch_bounds__(NT, 2) void k_zgemm_dma(bxh_problem_t P, bxh_seg_t S)
 {
-    using namespace dmaz;
-    constexpr int SA = stage_elems<MAJA, BM>(), SB = stage_elems<MAJB, BN>();
-    __shared__ __attribute__((aligned(16))) cplx smem[2 * (SA + SB)];
+    __shared__ __attribute__((aligned(16))) cplx smem[2 * (dmaz::BM + dmaz::BN) * dmaz::BK];
     int b = blockIdx.x;
     const int nb = gridDim.x;
     if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
     int tm, tn;
     tile_coords(b, P.tiles_m, P.tiles_n, tm, tn);
-    const int m0 = tm * BM, n0 = tn * BN;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lg = lane >> 4;
-    const cplx *A = (const cplx *)S.A, *B = (const cplx *)S.B;
-    const int ktiles = (int)(S.k / BK);
-
-    d4_t acc[4][2][2];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j][0] = acc[i][j][1] = d4_t{0, 0, 0, 0};
-
-    auto stageA = [&](int buf) { return smem + buf * SA; };
-    auto stageB = [&](int buf) { return smem + 2 * SA + buf * SB; };
-    auto issue = [&](int t, int buf) {
-        dma_tile<MAJA, BM>(A, S.lda, m0, (int64_t)t * BK, stageA(buf), wave, lane);
-        dma_tile<MAJB, BN>(B, S.ldb, n0, (int64_t)t * BK, stageB(buf), wave, lane);
-    };
-    issue(0, 0);
-    dma64::wait_dma();
-    __builtin_amdgcn_s_barrier();
-    for (int t = 0; t < ktiles; ++t) {
-        const int buf = t & 1;
-        if (t + 1 < ktiles) issue(t + 1, buf ^ 1);
-        const cplx *As = stageA(buf), *Bs = stageB(buf);
-#pragma unroll
-        for (int s = 0; s < BK / 4; ++s) {
-            cplx a[4], bb[2];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = frag<MAJA, BM, CJA>(As, wm * 64 + i * 16 + l16, s, lg);
-#pragma unroll
-            for (int j = 0; j < 2; ++j) bb[j] = frag<MAJB, BN, CJB>(Bs, wn * 32 + j * 16 + l16, s, lg);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const double nai = -a[i].im;
-#pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    acc[i][j][0] = mfma(a[i].re, bb[j].re, acc[i][j][0]);
-                    acc[i][j][1] = mfma(a[i].re, bb[j].im, acc[i][j][1]);
-                    acc[i][j][0] = mfma(nai, bb[j].im, acc[i][j][0]);
-                    acc[i][j][1] = mfma(a[i].im, bb[j].re, acc[i][j][1]);
-                }
-            }
-        }
-        dma64::wait_dma();
-        __builtin_amdgcn_s_barrier();
-    }
-
-    cplx *Cp = (cplx *)P.C;
-    const double ar = P.alpha_re, ai = P.alpha_im, br = P.beta_re, bi = P.beta_im;
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int col = n0 + wn * 32 + j * 16 + l16;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = m0 + wm * 64 + i * 16 + Cfg<cplx>::out_row(lg, v);
-                const double sr = acc[i][j][0][v], sim = acc[i][j][1][v];
-                double orr = ar * sr - ai * sim, oi = ar * sim + ai * sr;
-                cplx *c = Cp + row + (int64_t)col * P.ldc;
-                if (!P.beta_zero) {
-                    const cplx c0 = gload(c);
-                    orr += br * c0.re - bi * c0.im;
-                    oi += br * c0.im + bi * c0.re;
-                }
-                gstore(c, cplx{orr, oi});
-            }
-        }
+    dmaz_block<MAJA, MAJB, CJA, CJB>(P, nullptr, S, tm, tn, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -849,21 +783,64 @@ template <int MAJ> __device__ __forceinline__ float frag(const float *stage, int
 }
 } // namespace dmas
 
+// the segment walk shared by the DMA-staged blocks: the K tile being DMA'd
+// is tile k0 / BK of segment si, whose fields are kept in `cur` (the table
+// is read once per segment); advance() moves to the next BK of K, else to
+// the first non-empty later segment, and si reaches nseg when there is none
+template <int BK> struct SegWalk {
+    const bxh_seg_t *segs;
+    const bxh_seg_t &seg0;
+    int nseg, si;
+    int64_t k0;
+    bxh_seg_t cur;
+    __device__ __forceinline__ SegWalk(const bxh_seg_t *s, const bxh_seg_t &s0, int n)
+        : segs(s), seg0(s0), nseg(n), si(0), k0(0)
+    {
+        next_nonempty();
+    }
+    __device__ __forceinline__ void next_nonempty()
+    {
+        while (si < nseg) {
+            cur = seg_at(segs, seg0, si);
+            if (cur.k > 0) return;
+            ++si;
+        }
+    }
+    __device__ __forceinline__ bool valid() const { return si < nseg; }
+    __device__ __forceinline__ void advance()
+    {
+        k0 += BK;
+        if (k0 < cur.k) return;
+        k0 = 0;
+        ++si;
+        next_nonempty();
+    }
+};
+
+// every segment of P has K a multiple of bk (or is empty) and 16-byte
+// aligned operands, and P has no edge tiles (bm x bn blocks)
+__device__ __forceinline__ bool dma_eligible(const bxh_problem_t &P, const bxh_seg_t *segs,
+                                             const bxh_seg_t &seg0, int bm, int bn, int bk,
+                                             int esize)
+{
+    if (P.m % bm || P.n % bn || P.nseg < 1) return false;
+    for (int i = 0; i < P.nseg; ++i) {
+        const bxh_seg_t S = seg_at(segs, seg0, i);
+        if (S.k <= 0) continue;
+        if (S.k % bk || !aligned16(S.A, S.lda, esize) || !aligned16(S.B, S.ldb, esize))
+            return false;
+    }
+    return true;
+}
+
 template <int MAJA, int MAJB>
-__global__ __launch_bounds__(NT, 2) void k_sgemm_dma(bxh_problem_t P, bxh_seg_t S)
+__device__ __forceinline__ void dmas_block(const bxh_problem_t &P, const bxh_seg_t *segs,
+                                           const bxh_seg_t &seg0, int tm, int tn, float *smem)
 {
     using namespace dmas;
-    __shared__ __attribute__((aligned(16))) float smem[2 * 2 * STAGE];
-    int b = blockIdx.x;
-    const int nb = gridDim.x;
-    if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
-    int tm, tn;
-    tile_coords(b, P.tiles_m, P.tiles_n, tm, tn);
     const int m0 = tm * BM, n0 = tn * BM;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lg = lane >> 4;
-    const float *A = (const float *)S.A, *B = (const float *)S.B;
-    const int ktiles = (int)(S.k / BK);
 
     f4_t acc[4][4];
 #pragma unroll
@@ -873,16 +850,22 @@ __global__ __launch_bounds__(NT, 2) void k_sgemm_dma(bxh_problem_t P, bxh_seg_t 
 
     auto stageA = [&](int buf) { return smem + buf * STAGE; };
     auto stageB = [&](int buf) { return smem + 2 * STAGE + buf * STAGE; };
-    auto issue = [&](int t, int buf) {
-        dma_tile<MAJA>(A, S.lda, m0, (int64_t)t * BK, stageA(buf), wave, lane);
-        dma_tile<MAJB>(B, S.ldb, n0, (int64_t)t * BK, stageB(buf), wave, lane);
+    SegWalk<BK> w(segs, seg0, P.nseg);
+    auto issue = [&](int buf) {
+        dma_tile<MAJA>((const float *)w.cur.A, w.cur.lda, m0, w.k0, stageA(buf), wave, lane);
+        dma_tile<MAJB>((const float *)w.cur.B, w.cur.ldb, n0, w.k0, stageB(buf), wave, lane);
     };
-    issue(0, 0);
-    dma64::wait_dma();
-    __builtin_amdgcn_s_barrier();
-    for (int t = 0; t < ktiles; ++t) {
-        const int buf = t & 1;
-        if (t + 1 < ktiles) issue(t + 1, buf ^ 1);
+    bool more = w.valid();
+    if (more) {
+        issue(0);
+        dma64::wait_dma();
+        __builtin_amdgcn_s_barrier();
+    }
+    int buf = 0;
+    while (more) {
+        w.advance();
+        more = w.valid();
+        if (more) issue(buf ^ 1);
         const float *As = stageA(buf), *Bs = stageB(buf);
 #pragma unroll
         for (int s = 0; s < BK / 4; ++s) {
@@ -899,6 +882,7 @@ __global__ __launch_bounds__(NT, 2) void k_sgemm_dma(bxh_problem_t P, bxh_seg_t 
         }
         dma64::wait_dma();
         __builtin_amdgcn_s_barrier();
+        buf ^= 1;
     }
 
     float *Cp = (float *)P.C;
@@ -918,6 +902,100 @@ __global__ __launch_bounds__(NT, 2) void k_sgemm_dma(bxh_problem_t P, bxh_seg_t 
         }
 }
 
+template <int MAJA, int MAJB>
+__global__ __launch_bounds__(NT, 2) void k_sgemm_dma(bxh_problem_t P, bxh_seg_t S)
+{
+    __shared__ __attribute__((aligned(16))) float smem[2 * 2 * dmas::STAGE];
+    int b = blockIdx.x;
+    const int nb = gridDim.x;
+    if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+    int tm, tn;
+    tile_coords(b, P.tiles_m, P.tiles_n, tm, tn);
+    dmas_block<MAJA, MAJB>(P, nullptr, S, tm, tn, smem);
+}
+
+template <int MAJA, int MAJB, bool CJA, bool CJB>
+__device__ __forceinline__ void dmaz_block(const bxh_problem_t &P, const bxh_seg_t *segs,
+                                           const bxh_seg_t &seg0, int tm, int tn, cplx *smem)
+{
+    using namespace dmaz;
+    constexpr int SA = stage_elems<MAJA, BM>(), SB = stage_elems<MAJB, BN>();
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lg = lane >> 4;
+
+    d4_t acc[4][2][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j][0] = acc[i][j][1] = d4_t{0, 0, 0, 0};
+
+    auto stageA = [&](int buf) { return smem + buf * SA; };
+    auto stageB = [&](int buf) { return smem + 2 * SA + buf * SB; };
+    SegWalk<BK> w(segs, seg0, P.nseg);
+    auto issue = [&](int buf) {
+        dma_tile<MAJA, BM>((const cplx *)w.cur.A, w.cur.lda, m0, w.k0, stageA(buf), wave, lane);
+        dma_tile<MAJB, BN>((const cplx *)w.cur.B, w.cur.ldb, n0, w.k0, stageB(buf), wave, lane);
+    };
+    bool more = w.valid();
+    if (more) {
+        issue(0);
+        dma64::wait_dma();
+        __builtin_amdgcn_s_barrier();
+    }
+    int buf = 0;
+    while (more) {
+        w.advance();
+        more = w.valid();
+        if (more) issue(buf ^ 1);
+        const cplx *As = stageA(buf), *Bs = stageB(buf);
+#pragma unroll
+        for (int s = 0; s < BK / 4; ++s) {
+            cplx a[4], bb[2];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = frag<MAJA, BM, CJA>(As, wm * 64 + i * 16 + l16, s, lg);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) bb[j] = frag<MAJB, BN, CJB>(Bs, wn * 32 + j * 16 + l16, s, lg);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const double nai = -a[i].im;
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    acc[i][j][0] = mfma(a[i].re, bb[j].re, acc[i][j][0]);
+                    acc[i][j][1] = mfma(a[i].re, bb[j].im, acc[i][j][1]);
+                    acc[i][j][0] = mfma(nai, bb[j].im, acc[i][j][0]);
+                    acc[i][j][1] = mfma(a[i].im, bb[j].re, acc[i][j][1]);
+                }
+            }
+        }
+        dma64::wait_dma();
+        __builtin_amdgcn_s_barrier();
+        buf ^= 1;
+    }
+
+    cplx *Cp = (cplx *)P.C;
+    const double ar = P.alpha_re, ai = P.alpha_im, br = P.beta_re, bi = P.beta_im;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int col = n0 + wn * 32 + j * 16 + l16;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = m0 + wm * 64 + i * 16 + Cfg<cplx>::out_row(lg, v);
+                const double sr = acc[i][j][0][v], sim = acc[i][j][1][v];
+                double orr = ar * sr - ai * sim, oi = ar * sim + ai * sr;
+                cplx *c = Cp + row + (int64_t)col * P.ldc;
+                if (!P.beta_zero) {
+                    const cplx c0 = gload(c);
+                    orr += br * c0.re - bi * c0.im;
+                    oi += br * c0.im + bi * c0.re;
+                }
+                gstore(c, cplx{orr, oi});
+            }
+        }
+}
+
 template <typename E, int MAJA, int MAJB, bool CJA, bool CJB>
 __device__ __forceinline__ void run_tile(const bxh_problem_t &P, const bxh_seg_t *segs, int local,
                                          E *smem)
@@ -930,22 +1008,24 @@ __device__ __forceinline__ void run_tile(const bxh_problem_t &P, const bxh_seg_t
     gemm_block<E, MAJA, MAJB, CJA, CJB>(P, segs + P.seg_off, unused, tm, tn, smem);
 }
 
-// MODE 0: every problem on the register-staged block; for FP64 MODE 1 runs
-// only the problems the DMA-staged block can take (dma64::eligible) and
-// MODE 2 only the others: two launches, so that neither kernel carries both
-// blocks' registers (one kernel with both spilled).
+// MODE 1 runs only the problems the DMA-staged block of the type can take
+// (no edge tiles, aligned segments) and MODE 2 only the others: two
+// launches, so that neither kernel carries both blocks' registers (one
+// kernel with both spilled).
 template <typename E, int MODE>
 __device__ __forceinline__ void grouped_body(const bxh_problem_t *__restrict__ probs, int nprob,
                                              const bxh_seg_t *__restrict__ segs, int total,
                                              E *smem);
 
 // Grid-stride over the tiles of many problems (grouped GEMM / tile runtime).
-// LDS of a grouped kernel: the register-staged block's, and for FP64 also
-// the DMA-staged block's
-template <typename E> constexpr int grouped_smem_elems()
+// LDS of a grouped kernel: MODE 1 runs the DMA-staged block of its type,
+// the other modes the register-staged block
+template <typename E, int MODE> constexpr int grouped_smem_elems()
 {
-    return sizeof(E) == 8 && dma64::SMEM_DOUBLES > Smem<E>::ELEMS ? dma64::SMEM_DOUBLES
-                                                                  : Smem<E>::ELEMS;
+    if (MODE != 1) return Smem<E>::ELEMS;
+    if (sizeof(E) == 8) return dma64::SMEM_DOUBLES;
+    if (sizeof(E) == 4) return 2 * 2 * dmas::STAGE;
+    return 2 * (dmaz::BM + dmaz::BN) * dmaz::BK;
 }
 
 template <typename E, int MODE>
@@ -954,8 +1034,7 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_grouped(const bxh_problem_t *__r
                                                         const bxh_seg_t *__restrict__ segs,
                                                         int total)
 {
-    __shared__ __attribute__((aligned(16))) E smem[MODE == 1 ? grouped_smem_elems<E>()
-                                                              : Smem<E>::ELEMS];
+    __shared__ __attribute__((aligned(16))) E smem[grouped_smem_elems<E, MODE>()];
     grouped_body<E, MODE>(probs, nprob, segs, total, smem);
 }
 
@@ -968,8 +1047,7 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_grouped_dyn(const bxh_problem_t 
                                                             const bxh_seg_t *__restrict__ segs,
                                                             const int *__restrict__ counts)
 {
-    __shared__ __attribute__((aligned(16))) E smem[MODE == 1 ? grouped_smem_elems<E>()
-                                                              : Smem<E>::ELEMS];
+    __shared__ __attribute__((aligned(16))) E smem[grouped_smem_elems<E, MODE>()];
     grouped_body<E, MODE>(probs, counts[0], segs, counts[1], smem);
 }
 
@@ -979,7 +1057,8 @@ __device__ __forceinline__ void grouped_body(const bxh_problem_t *__restrict__ p
                                              E *smem)
 {
     constexpr bool CPLX = sizeof(E) == 16;
-    static_assert(MODE == 0 || sizeof(E) == 8, "the DMA split is FP64 only");
+    static_assert(MODE == 1 || MODE == 2, "grouped mode");
+    const bxh_seg_t unused = {nullptr, nullptr, 0, 0, 0};
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
         int lo = 0, hi = nprob - 1;
         while (lo < hi) {
@@ -991,7 +1070,49 @@ __device__ __forceinline__ void grouped_body(const bxh_problem_t *__restrict__ p
         }
         const bxh_problem_t P = probs[lo];
         const int local = t - P.tile_start;
-        if constexpr (CPLX) {
+        const bxh_seg_t *ps = segs + P.seg_off;
+        const bool dma = sizeof(E) == 8    ? dma_eligible(P, ps, unused, 128, 128, 16, 8)
+                         : sizeof(E) == 4 ? dma_eligible(P, ps, unused, 128, 128, 32, 4)
+                                          : dma_eligible(P, ps, unused, 128, 64, 8, 16);
+        if (dma != (MODE == 1)) continue; // the other launch's tile
+        if constexpr (MODE == 1) {
+            // block tiles of the DMA-staged blocks equal the register-staged ones
+            int tm, tn;
+            tile_coords(local, P.tiles_m, P.tiles_n, tm, tn);
+            if constexpr (CPLX) {
+                cplx *sz = reinterpret_cast<cplx *>(smem);
+                switch (P.opa * 3 + P.opb) {
+                case 0: dmaz_block<MAJ_MN, MAJ_K, false, false>(P, ps, unused, tm, tn, sz); break;
+                case 1: dmaz_block<MAJ_MN, MAJ_MN, false, false>(P, ps, unused, tm, tn, sz); break;
+                case 2: dmaz_block<MAJ_MN, MAJ_MN, false, true>(P, ps, unused, tm, tn, sz); break;
+                case 3: dmaz_block<MAJ_K, MAJ_K, false, false>(P, ps, unused, tm, tn, sz); break;
+                case 4: dmaz_block<MAJ_K, MAJ_MN, false, false>(P, ps, unused, tm, tn, sz); break;
+                case 5: dmaz_block<MAJ_K, MAJ_MN, false, true>(P, ps, unused, tm, tn, sz); break;
+                case 6: dmaz_block<MAJ_K, MAJ_K, true, false>(P, ps, unused, tm, tn, sz); break;
+                case 7: dmaz_block<MAJ_K, MAJ_MN, true, false>(P, ps, unused, tm, tn, sz); break;
+                default: dmaz_block<MAJ_K, MAJ_MN, true, true>(P, ps, unused, tm, tn, sz); break;
+                }
+            } else {
+                const int ta = P.opa != 0, tb = P.opb != 0;
+                if constexpr (sizeof(E) == 8) {
+                    double *sd = reinterpret_cast<double *>(smem);
+                    switch (ta * 2 + tb) {
+                    case 0: dma64_block<MAJ_MN, MAJ_K>(P, ps, unused, tm, tn, sd); break;
+                    case 1: dma64_block<MAJ_MN, MAJ_MN>(P, ps, unused, tm, tn, sd); break;
+                    case 2: dma64_block<MAJ_K, MAJ_K>(P, ps, unused, tm, tn, sd); break;
+                    default: dma64_block<MAJ_K, MAJ_MN>(P, ps, unused, tm, tn, sd); break;
+                    }
+                } else {
+                    float *sf = reinterpret_cast<float *>(smem);
+                    switch (ta * 2 + tb) {
+                    case 0: dmas_block<MAJ_MN, MAJ_K>(P, ps, unused, tm, tn, sf); break;
+                    case 1: dmas_block<MAJ_MN, MAJ_MN>(P, ps, unused, tm, tn, sf); break;
+                    case 2: dmas_block<MAJ_K, MAJ_K>(P, ps, unused, tm, tn, sf); break;
+                    default: dmas_block<MAJ_K, MAJ_MN>(P, ps, unused, tm, tn, sf); break;
+                    }
+                }
+            }
+        } else if constexpr (CPLX) {
             switch (P.opa * 3 + P.opb) {
             case 0: run_tile<E, MAJ_MN, MAJ_K, false, false>(P, segs, local, smem); break;
             case 1: run_tile<E, MAJ_MN, MAJ_MN, false, false>(P, segs, local, smem); break;
@@ -1005,30 +1126,11 @@ __device__ __forceinline__ void grouped_body(const bxh_problem_t *__restrict__ p
             }
         } else {
             const int ta = P.opa != 0, tb = P.opb != 0;
-            if constexpr (MODE != 0) {
-                const bxh_seg_t unused = {nullptr, nullptr, 0, 0, 0};
-                const bxh_seg_t *ps = segs + P.seg_off;
-                if (dma64::eligible(P, ps, unused) != (MODE == 1)) continue; // the other launch's
-                if constexpr (MODE == 1) {
-                    int tm, tn;
-                    tile_coords(local, P.tiles_m, P.tiles_n, tm, tn);
-                    double *sd = reinterpret_cast<double *>(smem);
-                    switch (ta * 2 + tb) {
-                    case 0: dma64_block<MAJ_MN, MAJ_K>(P, ps, unused, tm, tn, sd); break;
-                    case 1: dma64_block<MAJ_MN, MAJ_MN>(P, ps, unused, tm, tn, sd); break;
-                    case 2: dma64_block<MAJ_K, MAJ_K>(P, ps, unused, tm, tn, sd); break;
-                    default: dma64_block<MAJ_K, MAJ_MN>(P, ps, unused, tm, tn, sd); break;
-                    }
-                    continue;
-                }
-            }
-            if constexpr (MODE != 1) {
-                switch (ta * 2 + tb) {
-                case 0: run_tile<E, MAJ_MN, MAJ_K, false, false>(P, segs, local, smem); break;
-                case 1: run_tile<E, MAJ_MN, MAJ_MN, false, false>(P, segs, local, smem); break;
-                case 2: run_tile<E, MAJ_K, MAJ_K, false, false>(P, segs, local, smem); break;
-                default: run_tile<E, MAJ_K, MAJ_MN, false, false>(P, segs, local, smem); break;
-                }
+            switch (ta * 2 + tb) {
+            case 0: run_tile<E, MAJ_MN, MAJ_K, false, false>(P, segs, local, smem); break;
+            case 1: run_tile<E, MAJ_MN, MAJ_MN, false, false>(P, segs, local, smem); break;
+            case 2: run_tile<E, MAJ_K, MAJ_K, false, false>(P, segs, local, smem); break;
+            default: run_tile<E, MAJ_K, MAJ_MN, false, false>(P, segs, local, smem); break;
             }
         }
     }
@@ -1289,11 +1391,15 @@ int bxh_gemm_grouped(int dtype, const bxh_problem_t *probs_dev, int nprob,
                            nprob, segs_dev, total_tiles);
         break;
     case BLASX_S:
-        hipLaunchKernelGGL((k_gemm_grouped<float, 0>), dim3(grid), dim3(NT), 0, st, probs_dev,
+        hipLaunchKernelGGL((k_gemm_grouped<float, 1>), dim3(grid), dim3(NT), 0, st, probs_dev,
+                           nprob, segs_dev, total_tiles);
+        hipLaunchKernelGGL((k_gemm_grouped<float, 2>), dim3(grid), dim3(NT), 0, st, probs_dev,
                            nprob, segs_dev, total_tiles);
         break;
     case BLASX_Z:
-        hipLaunchKernelGGL((k_gemm_grouped<cplx, 0>), dim3(grid), dim3(NT), 0, st, probs_dev,
+        hipLaunchKernelGGL((k_gemm_grouped<cplx, 1>), dim3(grid), dim3(NT), 0, st, probs_dev,
+                           nprob, segs_dev, total_tiles);
+        hipLaunchKernelGGL((k_gemm_grouped<cplx, 2>), dim3(grid), dim3(NT), 0, st, probs_dev,
                            nprob, segs_dev, total_tiles);
         break;
     default: return BLASX_EINVAL;
@@ -1315,12 +1421,16 @@ int bxh_gemm_grouped_dyn(int dtype, const bxh_problem_t *probs_dev, const bxh_se
                            probs_dev, segs_dev, counts_dev);
         break;
     case BLASX_S:
-        hipLaunchKernelGGL((k_gemm_grouped_dyn<float, 0>), dim3(grid), dim3(NT), 0, st,
+        hipLaunchKernelGGL((k_gemm_grouped_dyn<float, 1>), dim3(grid), dim3(NT), 0, st,
+                           probs_dev, segs_dev, counts_dev);
+        hipLaunchKernelGGL((k_gemm_grouped_dyn<float, 2>), dim3(grid), dim3(NT), 0, st,
                            probs_dev, segs_dev, counts_dev);
         break;
     case BLASX_Z:
-        hipLaunchKernelGGL((k_gemm_grouped_dyn<cplx, 0>), dim3(grid), dim3(NT), 0, st, probs_dev,
-                           segs_dev, counts_dev);
+        hipLaunchKernelGGL((k_gemm_grouped_dyn<cplx, 1>), dim3(grid), dim3(NT), 0, st,
+                           probs_dev, segs_dev, counts_dev);
+        hipLaunchKernelGGL((k_gemm_grouped_dyn<cplx, 2>), dim3(grid), dim3(NT), 0, st,
+                           probs_dev, segs_dev, counts_dev);
         break;
     default: return BLASX_EINVAL;
     }
