@@ -103,6 +103,29 @@ int bx_cpublas_load(void)
     return ok ? 0 : BLASX_ECPU;
 }
 
+/* Resolve one routine of the CPU BLAS by its LP64 name ("cblas_dgemv",
+ * "dgemv_"); the ILP64 build numpy bundles names it scipy_<name>64_.  NULL
+ * (with a message) when the CPU BLAS or the routine is missing, or when the
+ * lookup would land in this library. */
+void *bx_cpublas_sym(const char *name)
+{
+    if (bx_cpublas_load() != 0) return NULL;
+    char buf[160];
+    if (g_ilp64)
+        snprintf(buf, sizeof buf, "scipy_%s64_", name);
+    else
+        snprintf(buf, sizeof buf, "%s", name);
+    void *f = dlsym(g_handle, buf);
+    Dl_info self, other;
+    if (f && dladdr((void *)&bx_cpublas_sym, &self) && dladdr(f, &other) &&
+        self.dli_fbase == other.dli_fbase)
+        f = NULL;
+    if (!f) bx_log(0, "CPU BLAS %s has no %s", g_path, buf);
+    return f;
+}
+
+int bx_cpublas_ilp64(void) { return g_ilp64; }
+
 BX_EXPORT const char *blasx_cpublas_path(void)
 {
     bx_cpublas_load();

@@ -63,6 +63,8 @@ void bx_stat_flops(double f);
 
 /* ---------------- CPU BLAS (cpublas.c) ---------------- */
 int bx_cpublas_load(void);
+void *bx_cpublas_sym(const char *lp64_name);
+int bx_cpublas_ilp64(void);
 int bx_cpu_gemm(int dt, int order, int ta, int tb, int M, int N, int K, const void *alpha,
                 const void *A, int lda, const void *B, int ldb, const void *beta, void *C,
                 int ldc);

@@ -14,10 +14,13 @@
 extern "C" {
 #endif
 
-/* CBLAS enums, same values as the reference's include/cblas.h:11-12 */
+/* CBLAS enums, same values as the reference's include/cblas.h:10-14 */
 #ifndef CBLAS_H
 enum CBLAS_ORDER { CblasRowMajor = 101, CblasColMajor = 102 };
 enum CBLAS_TRANSPOSE { CblasNoTrans = 111, CblasTrans = 112, CblasConjTrans = 113 };
+enum CBLAS_UPLO { CblasUpper = 121, CblasLower = 122 };
+enum CBLAS_DIAG { CblasNonUnit = 131, CblasUnit = 132 };
+enum CBLAS_SIDE { CblasLeft = 141, CblasRight = 142 };
 #endif
 
 /* ---- CBLAS entry points -------------------------------------------------
@@ -70,6 +73,7 @@ void xerbla_(const char *srname, const int *info);
 }
 #endif
 
+#include "blasx_blas.h"
 #include "blasx_ext.h"
 
 #endif /* BLASX_H */
