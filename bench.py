@@ -226,6 +226,134 @@ def host_resident(bx, n):
     }
 
 
+def _syrk_call(L, name, uplo, trans, n, k, alpha, A, lda, beta, Cm, ldc):
+    import ctypes as C
+
+    fn = getattr(L, "cblas_" + name)
+    fn.restype = None
+    if name == "zsyrk":
+        al = (C.c_double * 2)(alpha, 0.0)
+        be = (C.c_double * 2)(beta, 0.0)
+        a_arg, b_arg = C.cast(al, C.c_void_p), C.cast(be, C.c_void_p)
+    else:
+        a_arg, b_arg = C.c_double(alpha), C.c_double(beta)
+    fn(102, uplo, trans, n, k, a_arg, C.c_void_p(A), lda, b_arg, C.c_void_p(Cm), ldc)
+
+
+def rank_k(bx, n=32768):
+    """Rank-k updates through the ABI: DSYRK (lower, NoTrans, alpha = beta = 1,
+    N = K = n) with operands in HBM (one grouped launch) and host-resident
+    (tile runtime over the triangle's C tiles, cold and warm), ZHERK in HBM,
+    and OpenBLAS DSYRK on the host for comparison.  FLOPs: N(N+1)K real,
+    4N(N+1)K complex."""
+    import ctypes as C
+
+    import numpy as np
+
+    L = bx.lib()
+    out = {}
+    flops = float(n) * (n + 1) * n
+    dA = bx.DeviceArray(n, n, np.float64)
+    dC = bx.DeviceArray(n, n, np.float64)
+    dA.fill_uniform(0, 0)
+    dC.fill_uniform(0, 2)
+    bx.synchronize()
+    _syrk_call(L, "dsyrk", 122, 111, n, n, 1.0, dA.ptr, n, 1.0, dC.ptr, n)  # warm-up
+    times = []
+    for _ in range(3):
+        t = time.perf_counter()
+        _syrk_call(L, "dsyrk", 122, 111, n, n, 1.0, dA.ptr, n, 1.0, dC.ptr, n)
+        times.append(time.perf_counter() - t)
+    best = min(times)
+    log(f"dsyrk in HBM N=K={n}: {best:.3f}s ({flops / best / 1e12:.2f} TFLOP/s, "
+        f"{flops / best / 1e12 / FP64_PEAK_TFLOPS:.3f} of FP64 peak)")
+    out["dsyrk_inhbm_tflops"] = round(flops / best / 1e12, 3)
+    dA.free()
+    dC.free()
+
+    nz, kz = n // 2, n
+    zfl = 4.0 * nz * (nz + 1) * kz
+    dA = bx.DeviceArray(nz, kz, np.complex128)
+    dC = bx.DeviceArray(nz, nz, np.complex128)
+    dA.fill_uniform(0, 0)
+    dC.fill_uniform(0, 2)
+    bx.synchronize()
+    _syrk_call(L, "zherk", 122, 111, nz, kz, 1.0, dA.ptr, nz, 1.0, dC.ptr, nz)
+    times = []
+    for _ in range(3):
+        t = time.perf_counter()
+        _syrk_call(L, "zherk", 122, 111, nz, kz, 1.0, dA.ptr, nz, 1.0, dC.ptr, nz)
+        times.append(time.perf_counter() - t)
+    best = min(times)
+    log(f"zherk in HBM N={nz} K={kz}: {best:.3f}s ({zfl / best / 1e12:.2f} TFLOP/s)")
+    out["zherk_inhbm_tflops"] = round(zfl / best / 1e12, 3)
+    out["zherk_shape"] = [nz, kz]
+    dA.free()
+    dC.free()
+
+    log(f"dsyrk host-resident: allocating 2 x {n}^2 FP64 pinned host matrices")
+    mats = [bx.PinnedArray(n, n, np.float64) for _ in range(2)]
+    bx.fill_uniform_host(mats[0].array, 0, 0)
+    bx.fill_uniform_host(mats[1].array, 0, 2)
+    A, Cm = mats[0].array, mats[1].array
+    L.blasx_cache_flush()
+    bx.reset_stats()
+    t = time.perf_counter()
+    _syrk_call(L, "dsyrk", 122, 111, n, n, 1.0, A.ctypes.data, n, 1.0, Cm.ctypes.data, n)
+    cold = time.perf_counter() - t
+    st = bx.stats()
+    L.blasx_resident_register(A.ctypes.data, A.nbytes)
+    _syrk_call(L, "dsyrk", 122, 111, n, n, 1.0, A.ctypes.data, n, 1.0, Cm.ctypes.data, n)
+    t = time.perf_counter()
+    _syrk_call(L, "dsyrk", 122, 111, n, n, 1.0, A.ctypes.data, n, 1.0, Cm.ctypes.data, n)
+    warm = time.perf_counter() - t
+    L.blasx_resident_unregister(A.ctypes.data)
+    L.blasx_cache_flush()
+    for m in mats:
+        m.free()
+    log(f"dsyrk host-resident: cold {cold:.3f}s ({flops / cold / 1e12:.2f} TFLOP/s), "
+        f"warm {warm:.3f}s ({flops / warm / 1e12:.2f} TFLOP/s), cold H2D "
+        f"{st['bytes_h2d'] / 1e9:.1f} GB D2H {st['bytes_d2h'] / 1e9:.1f} GB")
+    out["dsyrk_host_cold_tflops"] = round(flops / cold / 1e12, 3)
+    out["dsyrk_host_warm_tflops"] = round(flops / warm / 1e12, 3)
+    out["dsyrk_host_cold_bytes_over_lower_bound"] = round(
+        (st["bytes_h2d"] + st["bytes_d2h"]) / max(st["bytes_lower_bound"], 1), 3)
+
+    # OpenBLAS DSYRK on the host cores, a bounded sample (the reference's path)
+    L.blasx_cpublas_path.restype = C.c_char_p
+    path = L.blasx_cpublas_path().decode()
+    cpu = C.CDLL(path)
+    threads = os.cpu_count() or 1
+    env_thr = os.environ.get("OMP_NUM_THREADS")
+    if env_thr and env_thr.isdigit():
+        threads = min(threads, int(env_thr))
+    ilp = not hasattr(cpu, "cblas_dsyrk")
+    fn = cpu.scipy_cblas_dsyrk64_ if ilp else cpu.cblas_dsyrk
+    it = C.c_int64 if ilp else C.c_int
+    if ilp and hasattr(cpu, "scipy_openblas_set_num_threads64_"):
+        cpu.scipy_openblas_set_num_threads64_(C.c_int(threads))
+    ns = 8192
+    As = bx.fill_uniform_host(np.empty((ns, ns), order="F"), 0, 0)
+    Cs = bx.fill_uniform_host(np.empty((ns, ns), order="F"), 0, 2)
+    args = (C.c_int(102), C.c_int(122), C.c_int(111), it(ns), it(ns), C.c_double(1.0),
+            C.c_void_p(As.ctypes.data), it(ns), C.c_double(1.0), C.c_void_p(Cs.ctypes.data),
+            it(ns))
+    fn(*args)
+    ct = []
+    for _ in range(3):
+        t = time.perf_counter()
+        fn(*args)
+        ct.append(time.perf_counter() - t)
+    cflops = float(ns) * (ns + 1) * ns
+    med = statistics.median(ct)
+    log(f"OpenBLAS dsyrk N=K={ns} on {threads} threads: {med:.3f}s "
+        f"({cflops / med / 1e12:.3f} TFLOP/s)")
+    out["cpu_dsyrk"] = {"value": round(cflops / med / 1e12, 4), "unit": "TFLOP/s",
+                        "cores": threads, "sample": f"OpenBLAS cblas_dsyrk N=K={ns} lower NoTrans, "
+                        "median of 3 after 1 warm-up"}
+    return out
+
+
 def matrix_inhbm(bx, n=32768):
     """The kernel on operands resident in HBM for every element type and
     transpose the ABI serves: SGEMM and DGEMM NN/NT/TN/TT, ZGEMM NN/NT/TN/TT/
@@ -753,7 +881,7 @@ def main():
     ap.add_argument("--crt-probe", type=int, default=0, metavar="MODULI",
                     help="one CRT split DGEMM at --n with MODULI moduli, for rocprofv3 passes")
     ap.add_argument("--only", default="", help="comma list of extras to run "
-                    "(matrix,host,hostsz,ooc,split,zsplit,serve); default all")
+                    "(matrix,host,hostsz,ooc,split,zsplit,serve,syrk); default all")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build targets one GPU; --gpus {args.gpus} is not supported "
@@ -815,7 +943,8 @@ def main():
                     ("ooc", "out_of_core", lambda: out_of_core(bx, n=args.ooc_n)),
                     ("split", "split_precision", lambda: split_precision(bx, n=args.split_n)),
                     ("zsplit", "zsplit_precision", lambda: zsplit_precision(bx)),
-                    ("serve", "serving", lambda: serving(bx, nreq=args.serve_requests))]
+                    ("serve", "serving", lambda: serving(bx, nreq=args.serve_requests)),
+                    ("syrk", "rank_k", lambda: rank_k(bx, n))]
         for key, name, fn in sections:
             if only and key not in only:
                 continue

@@ -1,6 +1,7 @@
 #!/usr/bin/env python3
 """Generate the CPU-BLAS forwarders of libblasx: every CBLAS and Fortran-77
-level-1/2/3 routine other than the GEMMs, each forwarded to the dlopen'ed CPU
+level-1/2/3 routine other than the GEMMs and the GPU rank-k updates (ON_GPU
+below), each forwarded to the dlopen'ed CPU
 BLAS through the library handle, as the reference forwards them
 (blas/blasx_globalpointers.c, one wrapper per routine in blas/*.c, e.g.
 blas/dgemv.c:9-36).  Writes csrc/forward.c and ../include/blasx_blas.h.
@@ -366,6 +367,10 @@ class Routine:
         return "%s %s(%s)" % (rt, self.fname, params)
 
 
+# run on the GPU by syrk.c (declared in blasx.h), not forwarded
+ON_GPU = {"ssyrk", "dsyrk", "zsyrk", "zherk"}
+
+
 def routines():
     out = []
     for names, ret, sig, f77 in L1:
@@ -374,7 +379,8 @@ def routines():
     for group in (L2, L3):
         for precs, stem, sig in group:
             for p in precs:
-                out.append(Routine(p + stem, p, "VOID", sig, "call"))
+                if p + stem not in ON_GPU:
+                    out.append(Routine(p + stem, p, "VOID", sig, "call"))
     return out
 
 
@@ -465,7 +471,8 @@ def generate():
     cw("/* forward.c -- generated by gen_forward.py; do not edit.")
     cw(" *")
     cw(" * CPU-BLAS forwarders for every CBLAS and Fortran-77 level-1/2/3 routine")
-    cw(" * except the GEMMs: each resolves its symbol in the dlopen'ed CPU BLAS once")
+    cw(" * except the GEMMs and {s,d,z}syrk, zherk (GPU, syrk.c): each resolves its")
+    cw(" * symbol in the dlopen'ed CPU BLAS once")
     cw(" * (through the library handle, never RTLD_NEXT) and calls it, widening")
     cw(" * integers to int64_t when that BLAS is ILP64.  The reference forwards the")
     cw(" * same routines the same way (blas/blasx_globalpointers.c, blas/dgemv.c:9-36).")
@@ -526,10 +533,11 @@ def generate():
     hw("/* blasx_blas.h -- generated by blasx_amd/csrc/gen_forward.py; do not edit.")
     hw(" *")
     hw(" * The CBLAS and Fortran-77 level-1/2/3 routines libblasx exports besides the")
-    hw(" * GEMMs (declared in blasx.h).  Each forwards to the CPU BLAS, as the")
-    hw(" * reference's wrappers do; the comment before each names the reference")
-    hw(" * routine it replaces.  Integers are LP64 int, matrices column-major unless")
-    hw(" * an Order argument says otherwise, complex values interleaved (re, im). */")
+    hw(" * GEMMs and the GPU rank-k updates (declared in blasx.h).  Each forwards to")
+    hw(" * the CPU BLAS, as the reference's wrappers do; the comment before each")
+    hw(" * names the reference routine it replaces.  Integers are LP64 int, matrices")
+    hw(" * column-major unless an Order argument says otherwise, complex values")
+    hw(" * interleaved (re, im). */")
     hw("#ifndef BLASX_BLAS_H")
     hw("#define BLASX_BLAS_H")
     hw("")

@@ -287,6 +287,12 @@ __device__ __forceinline__ bxh_seg_t seg_at(const bxh_seg_t *segs, const bxh_seg
 // ---------------------------------------------------------------------------
 // One block tile of one problem.
 // ---------------------------------------------------------------------------
+// whether a rank-k update writes element (row, col) of its diagonal block
+__device__ __forceinline__ bool tri_keep(int tri, int row, int col)
+{
+    return (!(tri & BX_TRI_LOWER) || row >= col) && (!(tri & BX_TRI_UPPER) || row <= col);
+}
+
 template <typename E, int MAJA, int MAJB, bool CJA, bool CJB>
 __device__ __forceinline__ void gemm_block(const bxh_problem_t &P, const bxh_seg_t *segs,
                                            const bxh_seg_t &seg0, int tm, int tn, E *smem)
@@ -367,10 +373,12 @@ __device__ __forceinline__ void gemm_block(const bxh_problem_t &P, const bxh_seg
         buf ^= 1;
     }
 
-    // ---- epilogue: C = alpha * acc + beta * C ----
+    // ---- epilogue: C = alpha * acc + beta * C (on the stored triangle of a
+    // rank-k update's diagonal block only) ----
     E *Cp = reinterpret_cast<E *>(P.C);
     const int64_t ldc = P.ldc;
     const int l16 = lane & 15, lg = lane >> 4;
+    const int tri = P.tri, toff = P.tri_off;
     if constexpr (CPLX) {
         const double ar = P.alpha_re, ai = P.alpha_im, br = P.beta_re, bi = P.beta_im;
 #pragma unroll
@@ -382,7 +390,7 @@ __device__ __forceinline__ void gemm_block(const bxh_problem_t &P, const bxh_seg
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int row = m0 + wm * 64 + mb * 16 + Cfg<cplx>::out_row(lg, v);
-                    if (row >= P.m) continue;
+                    if (row >= P.m || !tri_keep(tri, row + toff, col)) continue;
                     const double sr = accz[mb][nb][0][v], sim = accz[mb][nb][1][v];
                     double orr = ar * sr - ai * sim, oi = ar * sim + ai * sr;
                     cplx *c = Cp + row + (int64_t)col * ldc;
@@ -391,6 +399,7 @@ __device__ __forceinline__ void gemm_block(const bxh_problem_t &P, const bxh_seg
                         orr += br * c0.re - bi * c0.im;
                         oi += br * c0.im + bi * c0.re;
                     }
+                    if ((tri & BX_TRI_HERM) && row + toff == col) oi = 0.0;
                     gstore(c, cplx{orr, oi});
                 }
             }
@@ -407,7 +416,7 @@ __device__ __forceinline__ void gemm_block(const bxh_problem_t &P, const bxh_seg
                 for (int v = 0; v < 4; ++v) {
                     const int rbase = m0 + wm * 64 + h * 16 * VEC + VEC * C_::out_row(lg, v);
                     E *c = Cp + rbase + (int64_t)col * ldc;
-                    if (cvec && rbase + VEC <= P.m) {
+                    if (cvec && rbase + VEC <= P.m && tri == 0) {
                         Vec<E, VEC> out;
                         if (P.beta_zero) {
 #pragma unroll
@@ -422,7 +431,7 @@ __device__ __forceinline__ void gemm_block(const bxh_problem_t &P, const bxh_seg
                     } else {
 #pragma unroll
                         for (int e = 0; e < VEC; ++e) {
-                            if (rbase + e >= P.m) continue;
+                            if (rbase + e >= P.m || !tri_keep(tri, rbase + e + toff, col)) continue;
                             const E s = alpha * accr[h * VEC + e][nb][v];
                             gstore(c + e, P.beta_zero ? s : s + beta * gload(c + e));
                         }
@@ -539,20 +548,6 @@ __device__ __forceinline__ double frag(const double *stage, int row, int s, int 
 namespace dma64 {
 // LDS the DMA block needs at most (doubles): two stages of the larger layout
 constexpr int SMEM_DOUBLES = 2 * 2 * (BK * STR_MN);
-
-// every segment of P has K a multiple of 16 (or is empty) and 16-byte aligned
-// operands, and P has no edge tiles: the block below can run it
-__device__ __forceinline__ bool eligible(const bxh_problem_t &P, const bxh_seg_t *segs,
-                                         const bxh_seg_t &seg0)
-{
-    if (P.m % BM || P.n % BM || P.nseg < 1) return false;
-    for (int i = 0; i < P.nseg; ++i) {
-        const bxh_seg_t S = seg_at(segs, seg0, i);
-        if (S.k <= 0) continue;
-        if (S.k % BK || !aligned16(S.A, S.lda, 8) || !aligned16(S.B, S.ldb, 8)) return false;
-    }
-    return true;
-}
 } // namespace dma64
 
 // One 128x128 block tile (tm, tn) of P over all its segments (a plain GEMM
@@ -818,12 +813,14 @@ template <int BK> struct SegWalk {
 };
 
 // every segment of P has K a multiple of bk (or is empty) and 16-byte
-// aligned operands, and P has no edge tiles (bm x bn blocks)
+// aligned operands, and P has no edge tiles (bm x bn blocks) and no triangle
+// mask
 __device__ __forceinline__ bool dma_eligible(const bxh_problem_t &P, const bxh_seg_t *segs,
                                              const bxh_seg_t &seg0, int bm, int bn, int bk,
                                              int esize)
 {
-    if (P.m % bm || P.n % bn || P.nseg < 1) return false;
+    // triangle-masked blocks (rank-k diagonal blocks) take the register-staged path
+    if (P.m % bm || P.n % bn || P.nseg < 1 || P.tri) return false;
     for (int i = 0; i < P.nseg; ++i) {
         const bxh_seg_t S = seg_at(segs, seg0, i);
         if (S.k <= 0) continue;
@@ -1137,23 +1134,26 @@ __device__ __forceinline__ void grouped_body(const bxh_problem_t *__restrict__ p
 }
 
 template <typename E>
-__global__ void k_scale(int m, int n, E beta, int beta_zero, E *C, int64_t ldc)
+__global__ void k_scale(int m, int n, E beta, int beta_zero, int tri, E *C, int64_t ldc)
 {
     const int64_t total = (int64_t)m * n;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = i % m, c = i / m;
+        if (!tri_keep(tri, (int)r, (int)c)) continue;
         E *p = C + r + c * ldc;
         *p = beta_zero ? E(0) : beta * *p;
     }
 }
 
-__global__ void k_scale_z(int m, int n, double br, double bi, int beta_zero, cplx *C, int64_t ldc)
+__global__ void k_scale_z(int m, int n, double br, double bi, int beta_zero, int tri, cplx *C,
+                          int64_t ldc)
 {
     const int64_t total = (int64_t)m * n;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = i % m, c = i / m;
+        if (!tri_keep(tri, (int)r, (int)c)) continue;
         cplx *p = C + r + c * ldc;
         if (beta_zero) {
             *p = cplx{0.0, 0.0};
@@ -1161,6 +1161,7 @@ __global__ void k_scale_z(int m, int n, double br, double bi, int beta_zero, cpl
             const cplx v = *p;
             *p = cplx{br * v.re - bi * v.im, br * v.im + bi * v.re};
         }
+        if ((tri & BX_TRI_HERM) && r == c) p->im = 0.0;
     }
 }
 
@@ -1362,6 +1363,8 @@ int bxh_gemm(int dtype, int opa, int opb, int m, int n, int k, const double *alp
     P.beta_re = beta[0];
     P.beta_im = dtype == BLASX_Z ? beta[1] : 0.0;
     P.beta_zero = (P.beta_re == 0.0 && P.beta_im == 0.0);
+    P.tri = 0;
+    P.tri_off = 0;
     bxh_seg_t S;
     S.A = A;
     S.B = B;
@@ -1440,6 +1443,12 @@ int bxh_gemm_grouped_dyn(int dtype, const bxh_problem_t *probs_dev, const bxh_se
 
 int bxh_scale(int dtype, int m, int n, const double *beta, void *C, int64_t ldc, void *stream)
 {
+    return bxh_scale_tri(dtype, m, n, 0, beta, C, ldc, stream);
+}
+
+int bxh_scale_tri(int dtype, int m, int n, int tri, const double *beta, void *C, int64_t ldc,
+                  void *stream)
+{
     if (m <= 0 || n <= 0) return 0;
     const int64_t total = (int64_t)m * n;
     int grid = (int)((total + 255) / 256);
@@ -1448,16 +1457,16 @@ int bxh_scale(int dtype, int m, int n, const double *beta, void *C, int64_t ldc,
     const int bz = (beta[0] == 0.0 && (dtype != BLASX_Z || beta[1] == 0.0));
     switch (dtype) {
     case BLASX_D:
-        hipLaunchKernelGGL(k_scale<double>, dim3(grid), dim3(256), 0, st, m, n, beta[0], bz,
+        hipLaunchKernelGGL(k_scale<double>, dim3(grid), dim3(256), 0, st, m, n, beta[0], bz, tri,
                            (double *)C, ldc);
         break;
     case BLASX_S:
         hipLaunchKernelGGL(k_scale<float>, dim3(grid), dim3(256), 0, st, m, n, (float)beta[0], bz,
-                           (float *)C, ldc);
+                           tri, (float *)C, ldc);
         break;
     case BLASX_Z:
         hipLaunchKernelGGL(k_scale_z, dim3(grid), dim3(256), 0, st, m, n, beta[0], beta[1], bz,
-                           (cplx *)C, ldc);
+                           tri, (cplx *)C, ldc);
         break;
     default: return BLASX_EINVAL;
     }

@@ -38,9 +38,18 @@ typedef struct bxh_problem {
     int32_t tiles_m, tiles_n;    /* block tiles of this problem */
     int32_t tile_start;          /* prefix sum of tiles_m*tiles_n over earlier problems */
     int32_t beta_zero;           /* C is not read when set */
+    /* rank-k updates (SYRK/HERK): with BX_TRI_LOWER only C elements with
+     * row + tri_off >= col are written, with BX_TRI_UPPER those with
+     * row + tri_off <= col (row, col relative to C); BX_TRI_HERM also zeroes
+     * the imaginary part where row + tri_off == col.  0: every element. */
+    int32_t tri, tri_off;
     double alpha_re, alpha_im;
     double beta_re, beta_im;
 } bxh_problem_t;
+
+#define BX_TRI_LOWER 1
+#define BX_TRI_UPPER 2
+#define BX_TRI_HERM 4
 
 /* block-tile edges per element type (for computing tiles_m/tiles_n) */
 int bxh_tile_m(int dtype);
@@ -100,6 +109,9 @@ int bxh_gemm_grouped_dyn(int dtype, const bxh_problem_t *probs_dev, const bxh_se
                          const int *counts_dev, int grid, void *stream);
 /* C = beta * C (beta == 0 writes zeros without reading C) */
 int bxh_scale(int dtype, int m, int n, const double *beta, void *C, int64_t ldc, void *stream);
+/* the same on the triangle tri names (BX_TRI_* bits), square n x n C */
+int bxh_scale_tri(int dtype, int m, int n, int tri, const double *beta, void *C, int64_t ldc,
+                  void *stream);
 int bxh_fill_uniform(int dtype, void *p, int64_t rows, int64_t cols, int64_t ld,
                      uint64_t seed, uint32_t id, void *stream);
 /* streaming device copy kernel (16-byte aligned, multiple of 16 bytes) */

@@ -229,7 +229,18 @@ typedef struct plan {
     int64_t lda, ldb, ldc;
     double alpha[2], beta[2];
     int beta_zero;
+    /* rank-k update (SYRK/HERK, B = A): only C tiles on the stored triangle
+     * run, diagonal tiles masked (BX_TRI_* bits); 0 for a GEMM */
+    int tri;
 } plan_t;
+
+/* whether C tile (i, k) lies on the stored triangle of a rank-k update */
+static int tile_on_tri(const plan_t *p, int i, int k)
+{
+    if (p->tri & BX_TRI_LOWER) return k <= i;
+    if (p->tri & BX_TRI_UPPER) return k >= i;
+    return 1;
+}
 
 static void choose_round_shape(plan_t *p, int cap_slots)
 {
@@ -277,10 +288,18 @@ static int run_plan(plan_t *p, bx_cache_t *c)
     int *cst = (int *)malloc(sizeof(int) * DEPTH * R);
     int *sa = (int *)malloc(sizeof(int) * p->RM);
     int *sb = (int *)malloc(sizeof(int) * p->RN);
-    if (!cst || !sa || !sb) {
+    /* this round's C tiles (row, column within the round block) and which
+     * operand tiles they use */
+    int *tii = (int *)malloc(sizeof(int) * R), *tkk = (int *)malloc(sizeof(int) * R);
+    char *use_a = (char *)malloc(p->RM), *use_b = (char *)malloc(p->RN);
+    if (!cst || !sa || !sb || !tii || !tkk || !use_a || !use_b) {
         free(cst);
         free(sa);
         free(sb);
+        free(tii);
+        free(tkk);
+        free(use_a);
+        free(use_b);
         return BLASX_ENOMEM;
     }
     int ntaken = 0;
@@ -296,8 +315,24 @@ static int run_plan(plan_t *p, bx_cache_t *c)
     const int nbm = (p->mt + p->RM - 1) / p->RM, nbn = (p->nt + p->RN - 1) / p->RN;
     int64_t round = 0;
     for (int bi = 0; bi < nbm && !rc; ++bi) {
-        for (int q = 0; q < nbn && !rc; ++q, ++round) {
+        for (int q = 0; q < nbn && !rc; ++q) {
             const int bj = (bi & 1) ? nbn - 1 - q : q; /* alternate direction */
+            const int i0 = bi * p->RM, i1 = (i0 + p->RM < p->mt) ? i0 + p->RM : p->mt;
+            const int k0 = bj * p->RN, k1 = (k0 + p->RN < p->nt) ? k0 + p->RN : p->nt;
+            const int ni = i1 - i0, nk = k1 - k0;
+            /* the round's C tiles: all of the block, or those on the triangle */
+            int ntile = 0;
+            memset(use_a, 0, ni);
+            memset(use_b, 0, nk);
+            for (int ii = 0; ii < ni; ++ii)
+                for (int kk = 0; kk < nk; ++kk)
+                    if (tile_on_tri(p, i0 + ii, k0 + kk)) {
+                        tii[ntile] = ii;
+                        tkk[ntile] = kk;
+                        use_a[ii] = use_b[kk] = 1;
+                        ++ntile;
+                    }
+            if (ntile == 0) continue; /* a block off the triangle: no round */
             const int d = (int)(round % DEPTH);
             if (round >= DEPTH) {
                 /* at most DEPTH rounds in flight: round-DEPTH has computed */
@@ -306,55 +341,58 @@ static int run_plan(plan_t *p, bx_cache_t *c)
                 if ((rc = bxh_stream_wait_event(g_bx.s_h2d, g_rt.ev_d2h[d]))) break;
                 if ((rc = bxh_stream_wait_event(g_bx.s_comp, g_rt.ev_d2h[d]))) break;
             }
-            const int i0 = bi * p->RM, i1 = (i0 + p->RM < p->mt) ? i0 + p->RM : p->mt;
-            const int k0 = bj * p->RN, k1 = (k0 + p->RN < p->nt) ? k0 + p->RN : p->nt;
-            const int ni = i1 - i0, nk = k1 - k0;
             char *desc = (char *)g_rt.desc_host[d];
             bxh_problem_t *prob = (bxh_problem_t *)desc;
             bxh_seg_t *seg = (bxh_seg_t *)(desc + seg_off_bytes);
 
             /* C tiles in, problem descriptors */
             int tiles_total = 0;
-            for (int ii = 0; ii < ni; ++ii)
-                for (int kk = 0; kk < nk; ++kk) {
-                    const int t = ii * nk + kk;
-                    const int i = i0 + ii, k = k0 + kk;
-                    const int mc = (p->M - i * T < T) ? p->M - i * T : T;
-                    const int nc = (p->N - k * T < T) ? p->N - k * T : T;
-                    void *cs = bx_cache_slot_ptr(c, cst[d * R + t]);
-                    if (!p->beta_zero) {
-                        const char *hc = p->C + ((int64_t)i * T + (int64_t)k * T * p->ldc) * es;
-                        rc = copy_in(p->pg_c, cs, (size_t)T * es, hc, (size_t)p->ldc * es,
-                                     (size_t)mc * es, (size_t)nc);
-                        if (rc) break;
-                        h2d += (uint64_t)mc * nc * es;
-                    }
-                    bxh_problem_t *P = &prob[t];
-                    memset(P, 0, sizeof *P);
-                    P->C = cs;
-                    P->ldc = T;
-                    P->m = mc;
-                    P->n = nc;
-                    P->nseg = p->kt;
-                    P->seg_off = t * p->kt;
-                    P->opa = p->opa;
-                    P->opb = p->opb;
-                    P->tiles_m = (mc + bm - 1) / bm;
-                    P->tiles_n = (nc + bn - 1) / bn;
-                    P->tile_start = tiles_total;
-                    tiles_total += P->tiles_m * P->tiles_n;
-                    P->beta_zero = p->beta_zero;
-                    P->alpha_re = p->alpha[0];
-                    P->alpha_im = p->alpha[1];
-                    P->beta_re = p->beta[0];
-                    P->beta_im = p->beta[1];
+            for (int t = 0; t < ntile; ++t) {
+                const int ii = tii[t], kk = tkk[t];
+                const int i = i0 + ii, k = k0 + kk;
+                const int mc = (p->M - i * T < T) ? p->M - i * T : T;
+                const int nc = (p->N - k * T < T) ? p->N - k * T : T;
+                void *cs = bx_cache_slot_ptr(c, cst[d * R + t]);
+                /* a masked diagonal tile goes back whole: its other
+                 * triangle must hold the caller's values, so it is read
+                 * even when beta == 0 */
+                const int diag = p->tri && i == k;
+                if (!p->beta_zero || diag) {
+                    const char *hc = p->C + ((int64_t)i * T + (int64_t)k * T * p->ldc) * es;
+                    rc = copy_in(p->pg_c, cs, (size_t)T * es, hc, (size_t)p->ldc * es,
+                                 (size_t)mc * es, (size_t)nc);
+                    if (rc) break;
+                    h2d += (uint64_t)mc * nc * es;
                 }
+                bxh_problem_t *P = &prob[t];
+                memset(P, 0, sizeof *P);
+                P->C = cs;
+                P->ldc = T;
+                P->m = mc;
+                P->n = nc;
+                P->nseg = p->kt;
+                P->seg_off = t * p->kt;
+                P->opa = p->opa;
+                P->opb = p->opb;
+                P->tiles_m = (mc + bm - 1) / bm;
+                P->tiles_n = (nc + bn - 1) / bn;
+                P->tile_start = tiles_total;
+                tiles_total += P->tiles_m * P->tiles_n;
+                P->beta_zero = p->beta_zero;
+                P->alpha_re = p->alpha[0];
+                P->alpha_im = p->alpha[1];
+                P->beta_re = p->beta[0];
+                P->beta_im = p->beta[1];
+                P->tri = diag ? p->tri : 0;
+                P->tri_off = 0;
+            }
             if (rc) break;
 
             /* operand tiles of every K step, through the cache */
             for (int j = 0; j < p->kt && !rc; ++j) {
                 const int kext = (p->K - j * T < T) ? p->K - j * T : T;
                 for (int ii = 0; ii < ni && !rc; ++ii) {
+                    if (!use_a[ii]) continue;
                     const int i = i0 + ii;
                     const int mc = (p->M - i * T < T) ? p->M - i * T : T;
                     int64_t r0, c0;
@@ -390,6 +428,7 @@ static int run_plan(plan_t *p, bx_cache_t *c)
                     h2d += (uint64_t)rows * cols * es;
                 }
                 for (int kk = 0; kk < nk && !rc; ++kk) {
+                    if (!use_b[kk]) continue;
                     const int k = k0 + kk;
                     const int nc = (p->N - k * T < T) ? p->N - k * T : T;
                     int64_t r0, c0;
@@ -425,20 +464,19 @@ static int run_plan(plan_t *p, bx_cache_t *c)
                     h2d += (uint64_t)rows * cols * es;
                 }
                 if (rc) break;
-                for (int ii = 0; ii < ni; ++ii)
-                    for (int kk = 0; kk < nk; ++kk) {
-                        bxh_seg_t *S = &seg[(ii * nk + kk) * p->kt + j];
-                        S->A = bx_cache_slot_ptr(c, sa[ii]);
-                        S->B = bx_cache_slot_ptr(c, sb[kk]);
-                        S->lda = T;
-                        S->ldb = T;
-                        S->k = kext;
-                    }
+                for (int t = 0; t < ntile; ++t) {
+                    bxh_seg_t *S = &seg[t * p->kt + j];
+                    S->A = bx_cache_slot_ptr(c, sa[tii[t]]);
+                    S->B = bx_cache_slot_ptr(c, sb[tkk[t]]);
+                    S->lda = T;
+                    S->ldb = T;
+                    S->k = kext;
+                }
             }
             if (rc) break;
 
             /* compute after this round's copies; descriptors ride the compute stream */
-            const int nprob = ni * nk;
+            const int nprob = ntile;
             if ((rc = bxh_event_record(g_rt.ev_h2d[d], g_bx.s_h2d))) break;
             if ((rc = bxh_stream_wait_event(g_bx.s_comp, g_rt.ev_h2d[d]))) break;
             if ((rc = bxh_memcpy_h2d_async(g_rt.desc_dev[d], g_rt.desc_host[d],
@@ -455,19 +493,18 @@ static int run_plan(plan_t *p, bx_cache_t *c)
 
             /* C tiles out */
             if ((rc = bxh_stream_wait_event(g_bx.s_d2h, g_rt.ev_comp[d]))) break;
-            for (int ii = 0; ii < ni && !rc; ++ii)
-                for (int kk = 0; kk < nk && !rc; ++kk) {
-                    const int t = ii * nk + kk;
-                    const int i = i0 + ii, k = k0 + kk;
-                    const int mc = (p->M - i * T < T) ? p->M - i * T : T;
-                    const int nc = (p->N - k * T < T) ? p->N - k * T : T;
-                    char *hc = p->C + ((int64_t)i * T + (int64_t)k * T * p->ldc) * es;
-                    rc = copy_out(p->pg_c, hc, (size_t)p->ldc * es,
-                                  bx_cache_slot_ptr(c, cst[d * R + t]), (size_t)T * es,
-                                  (size_t)mc * es, (size_t)nc);
-                    d2h += (uint64_t)mc * nc * es;
-                }
+            for (int t = 0; t < ntile && !rc; ++t) {
+                const int i = i0 + tii[t], k = k0 + tkk[t];
+                const int mc = (p->M - i * T < T) ? p->M - i * T : T;
+                const int nc = (p->N - k * T < T) ? p->N - k * T : T;
+                char *hc = p->C + ((int64_t)i * T + (int64_t)k * T * p->ldc) * es;
+                rc = copy_out(p->pg_c, hc, (size_t)p->ldc * es,
+                              bx_cache_slot_ptr(c, cst[d * R + t]), (size_t)T * es,
+                              (size_t)mc * es, (size_t)nc);
+                d2h += (uint64_t)mc * nc * es;
+            }
             if (!rc) rc = bxh_event_record(g_rt.ev_d2h[d], g_bx.s_d2h);
+            ++round;
         }
     }
 
@@ -484,6 +521,10 @@ static int run_plan(plan_t *p, bx_cache_t *c)
     free(cst);
     free(sa);
     free(sb);
+    free(tii);
+    free(tkk);
+    free(use_a);
+    free(use_b);
     bx_cache_invalidate(c, keep_resident, NULL);
 
     BX_STAT_ADD(bytes_h2d, h2d);
@@ -494,9 +535,9 @@ static int run_plan(plan_t *p, bx_cache_t *c)
     return rc;
 }
 
-int bx_tile_gemm(int dt, int opa, int opb, int M, int N, int K, const double *alpha,
-                 const void *A, int64_t lda, const void *B, int64_t ldb, const double *beta,
-                 void *C, int64_t ldc)
+static int tile_call(int dt, int opa, int opb, int M, int N, int K, const double *alpha,
+                     const void *A, int64_t lda, const void *B, int64_t ldb, const double *beta,
+                     void *C, int64_t ldc, int tri)
 {
     if (bx_ensure_init() != 0) return BLASX_ENOGPU;
     pthread_mutex_lock(&g_bx.gpu_lock);
@@ -514,6 +555,7 @@ int bx_tile_gemm(int dt, int opa, int opb, int M, int N, int K, const double *al
         p.M = M;
         p.N = N;
         p.K = K;
+        p.tri = tri;
         int T = g_bx.tile_override[dt] ? g_bx.tile_override[dt] : bx_autotune_tile(dt, M, N, K);
         while ((size_t)T * T * p.es > bx_cache_slot_bytes(c)) T /= 2;
         p.T = T;
@@ -538,16 +580,45 @@ int bx_tile_gemm(int dt, int opa, int opb, int M, int N, int K, const double *al
         if (p.pg_a || p.pg_b || p.pg_c) rc = ensure_staging();
         choose_round_shape(&p, bx_cache_capacity_slots(c));
         bx_cache_reset_rounds(c);
-        bx_log(2, "tile gemm dt=%d %dx%dx%d T=%d rounds of %dx%d tiles", dt, M, N, K, T, p.RM,
-               p.RN);
+        bx_log(2, "tile %s dt=%d %dx%dx%d T=%d rounds of %dx%d tiles", tri ? "syrk" : "gemm", dt,
+               M, N, K, T, p.RM, p.RN);
         if (!rc) rc = run_plan(&p, c);
         if (!rc) {
-            BX_STAT_ADD(bytes_lower_bound,
-                        (uint64_t)p.es * ((uint64_t)M * K + (uint64_t)K * N +
-                                          (uint64_t)M * N * (p.beta_zero ? 1 : 2)));
-            bx_stat_flops((dt == BLASX_Z ? 8.0 : 2.0) * M * (double)N * K);
+            const double cm = dt == BLASX_Z ? 4.0 : 1.0; /* real flops per multiply-add / 2 */
+            if (tri) {
+                /* A read once, the stored triangle of C read and written */
+                const uint64_t tri_elems = (uint64_t)N * (N + 1) / 2;
+                BX_STAT_ADD(bytes_lower_bound,
+                            (uint64_t)p.es * ((uint64_t)N * K + tri_elems * (p.beta_zero ? 1 : 2)));
+                bx_stat_flops(cm * (double)N * (N + 1) * K);
+            } else {
+                BX_STAT_ADD(bytes_lower_bound,
+                            (uint64_t)p.es * ((uint64_t)M * K + (uint64_t)K * N +
+                                              (uint64_t)M * N * (p.beta_zero ? 1 : 2)));
+                bx_stat_flops(2.0 * cm * M * (double)N * K);
+            }
         }
     }
     pthread_mutex_unlock(&g_bx.gpu_lock);
     return rc;
+}
+
+int bx_tile_gemm(int dt, int opa, int opb, int M, int N, int K, const double *alpha,
+                 const void *A, int64_t lda, const void *B, int64_t ldb, const double *beta,
+                 void *C, int64_t ldc)
+{
+    return tile_call(dt, opa, opb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, 0);
+}
+
+/* C = alpha op(A) op(A)^T + beta C (op(A)^H for HERK, tri & BX_TRI_HERM) on
+ * the triangle tri names, for host-resident operands: the GEMM tile plan
+ * restricted to C tiles on that triangle, B = A, so each A tile is fetched
+ * once per round for both of its roles.  The reference's dsyrk forwards to
+ * the CPU BLAS (blas/dsyrk.c:46-58); its tile helpers for a GPU version enumerate
+ * the same triangle (blas/blasx_mem_control.c:3-22, 189-219). */
+int bx_tile_syrk(int dt, int tri, int opa, int N, int K, const double *alpha, const void *A,
+                 int64_t lda, const double *beta, void *C, int64_t ldc)
+{
+    const int opb = opa != BX_OP_N ? BX_OP_N : (tri & BX_TRI_HERM) ? BX_OP_C : BX_OP_T;
+    return tile_call(dt, opa, opb, N, N, K, alpha, A, lda, A, lda, beta, C, ldc, tri);
 }

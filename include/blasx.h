@@ -66,6 +66,37 @@ void zgemm_(const char *transa, const char *transb, const int *m, const int *n, 
             const void *alpha, const void *A, const int *lda, const void *B, const int *ldb,
             const void *beta, void *C, const int *ldc);
 
+/* ---- rank-k updates on the GPU (C = alpha op(A) op(A)^T + beta C on one
+ * triangle; op(A)^H and real alpha, beta for HERK) ---------------------------
+ * replaces cblas_ssyrk, blas/ssyrk.c:15 */
+void cblas_ssyrk(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                 const enum CBLAS_TRANSPOSE Trans, const int N, const int K, const float alpha,
+                 const float *A, const int lda, const float beta, float *C, const int ldc);
+/* replaces cblas_dsyrk, blas/dsyrk.c:15 */
+void cblas_dsyrk(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                 const enum CBLAS_TRANSPOSE Trans, const int N, const int K, const double alpha,
+                 const double *A, const int lda, const double beta, double *C, const int ldc);
+/* replaces cblas_zsyrk, blas/zsyrk.c:15 */
+void cblas_zsyrk(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                 const enum CBLAS_TRANSPOSE Trans, const int N, const int K, const void *alpha,
+                 const void *A, const int lda, const void *beta, void *C, const int ldc);
+/* replaces cblas_zherk, blas/zherk.c:15 */
+void cblas_zherk(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                 const enum CBLAS_TRANSPOSE Trans, const int N, const int K, const double alpha,
+                 const void *A, const int lda, const double beta, void *C, const int ldc);
+/* replaces ssyrk_, blas/ssyrk.c:76 */
+void ssyrk_(const char *uplo, const char *trans, const int *n, const int *k, const float *alpha,
+            const float *A, const int *lda, const float *beta, float *C, const int *ldc);
+/* replaces dsyrk_, blas/dsyrk.c:76 */
+void dsyrk_(const char *uplo, const char *trans, const int *n, const int *k, const double *alpha,
+            const double *A, const int *lda, const double *beta, double *C, const int *ldc);
+/* replaces zsyrk_, blas/zsyrk.c:76 */
+void zsyrk_(const char *uplo, const char *trans, const int *n, const int *k, const void *alpha,
+            const void *A, const int *lda, const void *beta, void *C, const int *ldc);
+/* replaces zherk_, blas/zherk.c:82 */
+void zherk_(const char *uplo, const char *trans, const int *n, const int *k, const double *alpha,
+            const void *A, const int *lda, const double *beta, void *C, const int *ldc);
+
 /* replaces xerbla_, blas/error.c:5-8.  Weak: a caller (LAPACK) may override it. */
 void xerbla_(const char *srname, const int *info);
 
