@@ -1072,6 +1072,15 @@ __device__ __forceinline__ void grouped_body(const bxh_problem_t *__restrict__ p
                          : sizeof(E) == 4 ? dma_eligible(P, ps, unused, 128, 128, 32, 4)
                                           : dma_eligible(P, ps, unused, 128, 64, 8, 16);
         if (dma != (MODE == 1)) continue; // the other launch's tile
+        if (P.tri) {
+            // a block tile of a rank-k diagonal block wholly off the stored
+            // triangle writes nothing: skip it before any load
+            int tm, tn;
+            tile_coords(local, P.tiles_m, P.tiles_n, tm, tn);
+            const int r0 = tm * Cfg<E>::BM + P.tri_off, c0 = tn * Cfg<E>::BN;
+            if ((P.tri & BX_TRI_LOWER) && r0 + Cfg<E>::BM - 1 < c0) continue;
+            if ((P.tri & BX_TRI_UPPER) && r0 > c0 + Cfg<E>::BN - 1) continue;
+        }
         if constexpr (MODE == 1) {
             // block tiles of the DMA-staged blocks equal the register-staged ones
             int tm, tn;
