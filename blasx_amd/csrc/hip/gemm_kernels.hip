@@ -553,7 +553,9 @@ constexpr int SMEM_DOUBLES = 2 * 2 * (BK * STR_MN);
 // One 128x128 block tile (tm, tn) of P over all its segments (a plain GEMM
 // has one, the tile runtime one per cached K tile), operands DMA-staged; the
 // K tiles of consecutive segments run as one pipeline.
-template <int MAJA, int MAJB>
+// MASK: P may be a rank-k diagonal block (P.tri), whose epilogue writes only
+// the stored triangle; the plain kernels instantiate it without.
+template <int MAJA, int MAJB, bool MASK = false>
 __device__ __forceinline__ void dma64_block(const bxh_problem_t &P, const bxh_seg_t *segs,
                                             const bxh_seg_t &seg0, int tm, int tn, double *smem)
 {
@@ -638,6 +640,8 @@ __device__ __forceinline__ void dma64_block(const bxh_problem_t &P, const bxh_se
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int row = m0 + wm * 64 + i * 16 + Cfg<double>::out_row(lg, v);
+                if constexpr (MASK)
+                    if (!tri_keep(P.tri, row + P.tri_off, col)) continue;
                 double *c = Cp + row + (int64_t)col * P.ldc;
                 const double x = alpha * acc[i][j][v];
                 gstore(c, P.beta_zero ? x : x + beta * gload(c));
@@ -706,7 +710,7 @@ __device__ __forceinline__ cplx frag(const cplx *stage, int row, int s, int lg)
 }
 } // namespace dmaz
 
-template <int MAJA, int MAJB, bool CJA, bool CJB>
+template <int MAJA, int MAJB, bool CJA, bool CJB, bool MASK = false>
 __device__ __forceinline__ void dmaz_block(const bxh_problem_t &P, const bxh_seg_t *segs,
                                            const bxh_seg_t &seg0, int tm, int tn, cplx *smem);
 
@@ -813,14 +817,12 @@ template <int BK> struct SegWalk {
 };
 
 // every segment of P has K a multiple of bk (or is empty) and 16-byte
-// aligned operands, and P has no edge tiles (bm x bn blocks) and no triangle
-// mask
+// aligned operands, and P has no edge tiles (bm x bn blocks)
 __device__ __forceinline__ bool dma_eligible(const bxh_problem_t &P, const bxh_seg_t *segs,
                                              const bxh_seg_t &seg0, int bm, int bn, int bk,
                                              int esize)
 {
-    // triangle-masked blocks (rank-k diagonal blocks) take the register-staged path
-    if (P.m % bm || P.n % bn || P.nseg < 1 || P.tri) return false;
+    if (P.m % bm || P.n % bn || P.nseg < 1) return false;
     for (int i = 0; i < P.nseg; ++i) {
         const bxh_seg_t S = seg_at(segs, seg0, i);
         if (S.k <= 0) continue;
@@ -830,7 +832,7 @@ __device__ __forceinline__ bool dma_eligible(const bxh_problem_t &P, const bxh_s
     return true;
 }
 
-template <int MAJA, int MAJB>
+template <int MAJA, int MAJB, bool MASK = false>
 __device__ __forceinline__ void dmas_block(const bxh_problem_t &P, const bxh_seg_t *segs,
                                            const bxh_seg_t &seg0, int tm, int tn, float *smem)
 {
@@ -892,6 +894,8 @@ __device__ __forceinline__ void dmas_block(const bxh_problem_t &P, const bxh_seg
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int row = m0 + wm * 64 + i * 16 + Cfg<float>::out_row(lg, v);
+                if constexpr (MASK)
+                    if (!tri_keep(P.tri, row + P.tri_off, col)) continue;
                 float *c = Cp + row + (int64_t)col * P.ldc;
                 const float x = alpha * acc[i][j][v];
                 gstore(c, P.beta_zero ? x : x + beta * gload(c));
@@ -911,7 +915,7 @@ __global__ __launch_bounds__(NT, 2) void k_sgemm_dma(bxh_problem_t P, bxh_seg_t 
     dmas_block<MAJA, MAJB>(P, nullptr, S, tm, tn, smem);
 }
 
-template <int MAJA, int MAJB, bool CJA, bool CJB>
+template <int MAJA, int MAJB, bool CJA, bool CJB, bool MASK>
 __device__ __forceinline__ void dmaz_block(const bxh_problem_t &P, const bxh_seg_t *segs,
                                            const bxh_seg_t &seg0, int tm, int tn, cplx *smem)
 {
@@ -980,6 +984,8 @@ __device__ __forceinline__ void dmaz_block(const bxh_problem_t &P, const bxh_seg
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int row = m0 + wm * 64 + i * 16 + Cfg<cplx>::out_row(lg, v);
+                if constexpr (MASK)
+                    if (!tri_keep(P.tri, row + P.tri_off, col)) continue;
                 const double sr = acc[i][j][0][v], sim = acc[i][j][1][v];
                 double orr = ar * sr - ai * sim, oi = ar * sim + ai * sr;
                 cplx *c = Cp + row + (int64_t)col * P.ldc;
@@ -988,6 +994,8 @@ __device__ __forceinline__ void dmaz_block(const bxh_problem_t &P, const bxh_seg
                     orr += br * c0.re - bi * c0.im;
                     oi += br * c0.im + bi * c0.re;
                 }
+                if constexpr (MASK)
+                    if ((P.tri & BX_TRI_HERM) && row + P.tri_off == col) oi = 0.0;
                 gstore(c, cplx{orr, oi});
             }
         }
@@ -1088,33 +1096,33 @@ __device__ __forceinline__ void grouped_body(const bxh_problem_t *__restrict__ p
             if constexpr (CPLX) {
                 cplx *sz = reinterpret_cast<cplx *>(smem);
                 switch (P.opa * 3 + P.opb) {
-                case 0: dmaz_block<MAJ_MN, MAJ_K, false, false>(P, ps, unused, tm, tn, sz); break;
-                case 1: dmaz_block<MAJ_MN, MAJ_MN, false, false>(P, ps, unused, tm, tn, sz); break;
-                case 2: dmaz_block<MAJ_MN, MAJ_MN, false, true>(P, ps, unused, tm, tn, sz); break;
-                case 3: dmaz_block<MAJ_K, MAJ_K, false, false>(P, ps, unused, tm, tn, sz); break;
-                case 4: dmaz_block<MAJ_K, MAJ_MN, false, false>(P, ps, unused, tm, tn, sz); break;
-                case 5: dmaz_block<MAJ_K, MAJ_MN, false, true>(P, ps, unused, tm, tn, sz); break;
-                case 6: dmaz_block<MAJ_K, MAJ_K, true, false>(P, ps, unused, tm, tn, sz); break;
-                case 7: dmaz_block<MAJ_K, MAJ_MN, true, false>(P, ps, unused, tm, tn, sz); break;
-                default: dmaz_block<MAJ_K, MAJ_MN, true, true>(P, ps, unused, tm, tn, sz); break;
+                case 0: dmaz_block<MAJ_MN, MAJ_K, false, false, true>(P, ps, unused, tm, tn, sz); break;
+                case 1: dmaz_block<MAJ_MN, MAJ_MN, false, false, true>(P, ps, unused, tm, tn, sz); break;
+                case 2: dmaz_block<MAJ_MN, MAJ_MN, false, true, true>(P, ps, unused, tm, tn, sz); break;
+                case 3: dmaz_block<MAJ_K, MAJ_K, false, false, true>(P, ps, unused, tm, tn, sz); break;
+                case 4: dmaz_block<MAJ_K, MAJ_MN, false, false, true>(P, ps, unused, tm, tn, sz); break;
+                case 5: dmaz_block<MAJ_K, MAJ_MN, false, true, true>(P, ps, unused, tm, tn, sz); break;
+                case 6: dmaz_block<MAJ_K, MAJ_K, true, false, true>(P, ps, unused, tm, tn, sz); break;
+                case 7: dmaz_block<MAJ_K, MAJ_MN, true, false, true>(P, ps, unused, tm, tn, sz); break;
+                default: dmaz_block<MAJ_K, MAJ_MN, true, true, true>(P, ps, unused, tm, tn, sz); break;
                 }
             } else {
                 const int ta = P.opa != 0, tb = P.opb != 0;
                 if constexpr (sizeof(E) == 8) {
                     double *sd = reinterpret_cast<double *>(smem);
                     switch (ta * 2 + tb) {
-                    case 0: dma64_block<MAJ_MN, MAJ_K>(P, ps, unused, tm, tn, sd); break;
-                    case 1: dma64_block<MAJ_MN, MAJ_MN>(P, ps, unused, tm, tn, sd); break;
-                    case 2: dma64_block<MAJ_K, MAJ_K>(P, ps, unused, tm, tn, sd); break;
-                    default: dma64_block<MAJ_K, MAJ_MN>(P, ps, unused, tm, tn, sd); break;
+                    case 0: dma64_block<MAJ_MN, MAJ_K, true>(P, ps, unused, tm, tn, sd); break;
+                    case 1: dma64_block<MAJ_MN, MAJ_MN, true>(P, ps, unused, tm, tn, sd); break;
+                    case 2: dma64_block<MAJ_K, MAJ_K, true>(P, ps, unused, tm, tn, sd); break;
+                    default: dma64_block<MAJ_K, MAJ_MN, true>(P, ps, unused, tm, tn, sd); break;
                     }
                 } else {
                     float *sf = reinterpret_cast<float *>(smem);
                     switch (ta * 2 + tb) {
-                    case 0: dmas_block<MAJ_MN, MAJ_K>(P, ps, unused, tm, tn, sf); break;
-                    case 1: dmas_block<MAJ_MN, MAJ_MN>(P, ps, unused, tm, tn, sf); break;
-                    case 2: dmas_block<MAJ_K, MAJ_K>(P, ps, unused, tm, tn, sf); break;
-                    default: dmas_block<MAJ_K, MAJ_MN>(P, ps, unused, tm, tn, sf); break;
+                    case 0: dmas_block<MAJ_MN, MAJ_K, true>(P, ps, unused, tm, tn, sf); break;
+                    case 1: dmas_block<MAJ_MN, MAJ_MN, true>(P, ps, unused, tm, tn, sf); break;
+                    case 2: dmas_block<MAJ_K, MAJ_K, true>(P, ps, unused, tm, tn, sf); break;
+                    default: dmas_block<MAJ_K, MAJ_MN, true>(P, ps, unused, tm, tn, sf); break;
                     }
                 }
             }
