@@ -81,10 +81,6 @@ static const char *dt_name(int dt)
     return dt == BLASX_S ? "SGEMM " : dt == BLASX_D ? "DGEMM " : dt == BLASX_Z ? "ZGEMM " : "CGEMM ";
 }
 
-int bx_device_gemm(int dt, int transa, int transb, int M, int N, int K, const double *alpha,
-                   const void *A, int64_t lda, const void *B, int64_t ldb, const double *beta,
-                   void *C, int64_t ldc, void *stream);
-
 static void gemm_entry(int dt, int order, int TransA, int TransB, int M, int N, int K,
                        const void *alpha, const void *A, int lda, const void *B, int ldb,
                        const void *beta, void *C, int ldc)

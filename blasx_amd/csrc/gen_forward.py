@@ -1,16 +1,17 @@
 #!/usr/bin/env python3
 """Generate the CPU-BLAS forwarders of libblasx: every CBLAS and Fortran-77
-level-1/2/3 routine other than the GEMMs and the GPU rank-k updates (ON_GPU
-below), each forwarded to the dlopen'ed CPU
-BLAS through the library handle, as the reference forwards them
-(blas/blasx_globalpointers.c, one wrapper per routine in blas/*.c, e.g.
-blas/dgemv.c:9-36).  Writes csrc/forward.c and ../include/blasx_blas.h.
+level-1/2/3 routine other than the GEMMs and the GPU level-3 routines (ON_GPU
+below), each forwarded to the dlopen'ed CPU BLAS through the library handle,
+as the reference forwards them (blas/blasx_globalpointers.c, one wrapper per
+routine in blas/*.c, e.g. blas/dgemv.c:9-36).
 
 The CPU BLAS may be LP64 (libopenblas.so: cblas_dgemv, dgemv_) or the ILP64
 build numpy bundles (scipy_cblas_dgemv64_, scipy_dgemv_64_); the forwarders
 widen every integer argument to int64_t for the latter.  Argument checking,
 xerbla numbering and RowMajor handling are the CPU BLAS's own (netlib
-semantics).  Run from anywhere: python3 gen_forward.py [outdir-for-tests]
+semantics).  Writes csrc/forward.c, csrc/cpu_forward.h (the CPU routines
+behind the GPU entry points) and ../include/blasx_blas.h.
+Run from anywhere: python3 gen_forward.py [outdir-for-tests]
 """
 import os
 import sys
@@ -367,8 +368,10 @@ class Routine:
         return "%s %s(%s)" % (rt, self.fname, params)
 
 
-# run on the GPU by syrk.c (declared in blasx.h), not forwarded
-ON_GPU = {"ssyrk", "dsyrk", "zsyrk", "zherk"}
+# run on the GPU by syrk.c and level3.c (declared in blasx.h): no public
+# forwarder, but an internal bx_cpu_<name> for their CPU-sized calls
+ON_GPU = {"ssyrk", "dsyrk", "zsyrk", "zherk", "ssymm", "dsymm", "zsymm", "zhemm", "strmm",
+          "dtrmm", "ztrmm", "ssyr2k", "dsyr2k", "zsyr2k"}
 
 
 def routines():
@@ -379,8 +382,7 @@ def routines():
     for group in (L2, L3):
         for precs, stem, sig in group:
             for p in precs:
-                if p + stem not in ON_GPU:
-                    out.append(Routine(p + stem, p, "VOID", sig, "call"))
+                out.append(Routine(p + stem, p, "VOID", sig, "call"))
     return out
 
 
@@ -392,9 +394,14 @@ def emit_cblas(r, w):
     rt = ret_type(r.ret, r.p)
     names = ", ".join(a.name for a in r.args)
     fail = "" if rt == "void" else " 0"
+    internal = r.base in ON_GPU
     w("/* %s: %s */" % (r.cname, cite(r.cname)))
     params = ", ".join(decl(a.cblas_type(), a.name) for a in r.args)
-    w("static %s fw_%s(%s)" % (rt, r.base, params))
+    if internal:
+        # the CPU BLAS's own routine, for the GPU entry point's CPU-sized calls
+        w("%s bx_cpu_%s(%s)" % (rt, r.base, params))
+    else:
+        w("static %s fw_%s(%s)" % (rt, r.base, params))
     w("{")
     w("    static void *slot;")
     w("    void *f = fwd_sym(&slot, \"%s\");" % r.cname)
@@ -410,6 +417,9 @@ def emit_cblas(r, w):
     else:
         w("    %s((%s)f)(%s);" % (ret, fn_type(rt, lp), names))
     w("}")
+    if internal:
+        w("")
+        return
     w("BX_EXPORT %s" % r.cblas_proto())
     w("{")
     w("    %sfw_%s(%s);" % (ret, r.base, names))
@@ -471,8 +481,9 @@ def generate():
     cw("/* forward.c -- generated by gen_forward.py; do not edit.")
     cw(" *")
     cw(" * CPU-BLAS forwarders for every CBLAS and Fortran-77 level-1/2/3 routine")
-    cw(" * except the GEMMs and {s,d,z}syrk, zherk (GPU, syrk.c): each resolves its")
-    cw(" * symbol in the dlopen'ed CPU BLAS once")
+    cw(" * except the GEMMs and the GPU level-3 routines (syrk.c, level3.c; for those")
+    cw(" * only an internal bx_cpu_* form): each resolves its symbol in the dlopen'ed")
+    cw(" * CPU BLAS once")
     cw(" * (through the library handle, never RTLD_NEXT) and calls it, widening")
     cw(" * integers to int64_t when that BLAS is ILP64.  The reference forwards the")
     cw(" * same routines the same way (blas/blasx_globalpointers.c, blas/dgemv.c:9-36).")
@@ -484,6 +495,7 @@ def generate():
     cw("#include <stdint.h>")
     cw("")
     cw("#include \"blasx.h\"")
+    cw("#include \"cpu_forward.h\"")
     cw("#include \"internal.h\"")
     cw("")
     cw("static void *fwd_sym(void **slot, const char *name)")
@@ -497,7 +509,7 @@ def generate():
     cw("")
     for r in rs:
         emit_cblas(r, cw)
-        if r.f77:
+        if r.f77 and r.base not in ON_GPU:
             emit_f77(r, cw)
     for name, p, sig in F77_ONLY:
         args = parse(sig, p)
@@ -533,7 +545,7 @@ def generate():
     hw("/* blasx_blas.h -- generated by blasx_amd/csrc/gen_forward.py; do not edit.")
     hw(" *")
     hw(" * The CBLAS and Fortran-77 level-1/2/3 routines libblasx exports besides the")
-    hw(" * GEMMs and the GPU rank-k updates (declared in blasx.h).  Each forwards to")
+    hw(" * GEMMs and the GPU level-3 routines (declared in blasx.h).  Each forwards to")
     hw(" * the CPU BLAS, as the reference's wrappers do; the comment before each")
     hw(" * names the reference routine it replaces.  Integers are LP64 int, matrices")
     hw(" * column-major unless an Order argument says otherwise, complex values")
@@ -562,6 +574,8 @@ def generate():
     hw("#endif")
     hw("")
     for r in rs:
+        if r.base in ON_GPU:
+            continue
         hw("/* %s */" % cite(r.cname))
         hw(r.cblas_proto() + ";")
         if r.f77:
@@ -583,21 +597,37 @@ def generate():
     hw("#endif")
     hw("")
     hw("#endif /* BLASX_BLAS_H */")
-    return "\n".join(c) + "\n", "\n".join(h) + "\n"
+    g = ["/* cpu_forward.h -- generated by gen_forward.py; do not edit.",
+         " *",
+         " * The CPU BLAS's own routines behind the GPU entry points (syrk.c,",
+         " * level3.c), for their CPU-sized calls: LP64 CBLAS signatures, forwarded",
+         " * like every other routine in forward.c.  Internal to libblasx. */",
+         "#ifndef BX_CPU_FORWARD_H",
+         "#define BX_CPU_FORWARD_H",
+         "",
+         "#include \"blasx.h\"",
+         ""]
+    for r in rs:
+        if r.base in ON_GPU:
+            params = ", ".join(decl(a.cblas_type(), a.name) for a in r.args)
+            g.append("%s bx_cpu_%s(%s);" % (ret_type(r.ret, r.p), r.base, params))
+    g += ["", "#endif /* BX_CPU_FORWARD_H */"]
+    return "\n".join(c) + "\n", "\n".join(h) + "\n", "\n".join(g) + "\n"
 
 
 def main():
-    csrc, hdr = generate()
+    csrc, hdr, internal = generate()
     if len(sys.argv) > 1:
         outc = os.path.join(sys.argv[1], "forward.c")
         outh = os.path.join(sys.argv[1], "blasx_blas.h")
+        outi = os.path.join(sys.argv[1], "cpu_forward.h")
     else:
         outc = os.path.join(HERE, "forward.c")
         outh = os.path.join(HERE, "..", "..", "include", "blasx_blas.h")
-    with open(outc, "w") as f:
-        f.write(csrc)
-    with open(outh, "w") as f:
-        f.write(hdr)
+        outi = os.path.join(HERE, "cpu_forward.h")
+    for path, text in ((outc, csrc), (outh, hdr), (outi, internal)):
+        with open(path, "w") as f:
+            f.write(text)
 
 
 if __name__ == "__main__":

@@ -87,6 +87,8 @@ int bxh_memcpy2d_h2d_async(void *dst, size_t dpitch, const void *src, size_t spi
                            size_t width, size_t height, void *s);
 int bxh_memcpy2d_d2h_async(void *dst, size_t dpitch, const void *src, size_t spitch,
                            size_t width, size_t height, void *s);
+int bxh_memcpy2d_d2d_async(void *dst, size_t dpitch, const void *src, size_t spitch,
+                           size_t width, size_t height, void *s);
 int bxh_memset_async(void *dst, int v, size_t bytes, void *s);
 /* HIP graphs: capture what is enqueued on s between begin and end */
 int bxh_graph_begin(void *s);
@@ -114,6 +116,15 @@ int bxh_scale_tri(int dtype, int m, int n, int tri, const double *beta, void *C,
                   void *stream);
 int bxh_fill_uniform(int dtype, void *p, int64_t rows, int64_t cols, int64_t ld,
                      uint64_t seed, uint32_t id, void *stream);
+/* materialise an n x n operand in place from its stored triangle (lower != 0:
+ * the lower one): BX_FILL_SYM mirrors it, BX_FILL_HERM mirrors it conjugated
+ * and zeroes the diagonal's imaginary part, BX_FILL_TRI zeroes the other
+ * triangle, BX_FILL_UNIT also sets the diagonal to one */
+#define BX_FILL_SYM 0
+#define BX_FILL_HERM 1
+#define BX_FILL_TRI 2
+#define BX_FILL_UNIT 3
+int bxh_tri_fill(int dtype, int n, int lower, int mode, void *A, int64_t lda, void *stream);
 /* streaming device copy kernel (16-byte aligned, multiple of 16 bytes) */
 int bxh_copy_kernel(void *dst, const void *src, size_t bytes, void *stream);
 

@@ -129,8 +129,13 @@ int bx_tile_gemm(int dt, int opa, int opb, int M, int N, int K, const double *al
 int bx_tile_syrk(int dt, int tri, int opa, int N, int K, const double *alpha, const void *A,
                  int64_t lda, const double *beta, void *C, int64_t ldc);
 int bx_device_syrk(int dt, int tri, int opa, int N, int K, const double *alpha, const void *A,
-                   int64_t lda, const double *beta, void *C, int64_t ldc, void *stream);
+                   int64_t lda, const void *B, int64_t ldb, const double *beta, void *C,
+                   int64_t ldc, void *stream);
 bx_cache_t *bx_global_cache(void);
+/* C = alpha op(A) op(B) + beta C on HBM operands (transa, transb: CBLAS enums) */
+int bx_device_gemm(int dt, int transa, int transb, int M, int N, int K, const double *alpha,
+                   const void *A, int64_t lda, const void *B, int64_t ldb, const double *beta,
+                   void *C, int64_t ldc, void *stream);
 int bx_autotune_tile(int dt, int M, int N, int K);
 
 /* grouped GEMM tables (grouped.c) */

@@ -1,0 +1,570 @@
+/*
+ * level3.c -- SYMM, HEMM, TRMM and SYR2K on the GPU, and their f77 forms:
+ *   cblas_{s,d,z}symm, cblas_zhemm  C = alpha A B + beta C (Side Left) or
+ *                                   alpha B A + beta C (Right), A symmetric
+ *                                   (Hermitian) given by one triangle
+ *   cblas_{s,d,z}trmm               B = alpha op(A) B (Left) or alpha B op(A),
+ *                                   A triangular, optionally unit diagonal
+ *   cblas_{s,d,z}syr2k              C = alpha op(A) op(B)^T + alpha op(B) op(A)^T
+ *                                   + beta C on one triangle
+ *
+ * Replaces blas/{s,d,z}symm.c:16-95, blas/zhemm.c:16-101,
+ * blas/{s,d,z}trmm.c:17-90 and blas/{s,d,z}syr2k.c:16-110.  The reference
+ * validates (blas/dsymm.c:29-36, blas/dtrmm.c:32-40, blas/dsyr2k.c:43-49) and
+ * then always forwards to the CPU BLAS: both branches of its dispatch choose
+ * the CPU (blas/dsymm.c:44-47, blas/dtrmm.c:48-51, blas/dsyr2k.c:57-60).
+ * Kept: the validation order and info numbers (lowest wins), the xerbla_
+ * names, the "an extent above 1000 (500 complex)" size test.
+ *
+ * On the GPU:
+ *   SYMM/HEMM and TRMM are one GEMM on an operand materialised in HBM
+ *   (bxh_tri_fill mirrors or zeroes the other triangle: O(n^2) bytes against
+ *   the GEMM's O(n^2 m) FLOPs).  TRMM writes a separate output that is then
+ *   copied over B.
+ *   SYR2K is the rank-k grouped launch with two K segments per problem
+ *   (bx_device_syrk).
+ * Host operands are copied whole to HBM for the call: these routines run in
+ * core, and a call whose operands do not fit goes to the CPU BLAS.  Operands
+ * already in HBM are used in place.
+ *
+ * Deviations (netlib semantics): RowMajor is computed; ZSYR2K rejects
+ * ConjTrans (info 2); f77 argument errors report the lowest parameter number.
+ */
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "blasx.h"
+#include "cpu_forward.h"
+#include "internal.h"
+
+/* ------------------------------------------------------------ device work */
+typedef struct {
+    void *buf[4];
+    int n;
+} scratch_t;
+
+static int scratch_alloc(scratch_t *s, void **p, size_t bytes)
+{
+    if (bytes < 16) bytes = 16;
+    if (bxh_malloc(p, bytes) != 0) {
+        /* the tile cache keeps its HBM slab between calls: release it, retry */
+        bx_cache_t *c = bx_global_cache();
+        if (!c) return BLASX_ENOMEM;
+        bx_cache_invalidate_all(c);
+        bx_cache_free_memory(c);
+        if (bxh_malloc(p, bytes) != 0) return BLASX_ENOMEM;
+    }
+    s->buf[s->n++] = *p;
+    return 0;
+}
+
+static void scratch_free(scratch_t *s)
+{
+    for (int i = 0; i < s->n; ++i) bxh_free(s->buf[i]);
+    s->n = 0;
+}
+
+/* the caller's rows x cols matrix as a device matrix: itself when it is in
+ * HBM, else a dense copy (ld = rows), uploaded when `upload` */
+static int dev_view(scratch_t *s, const void *p, int rows, int cols, int ld, int es, int upload,
+                    void **dev, int64_t *dld)
+{
+    if (bxh_pointer_is_pinned(p) == 2) {
+        *dev = (void *)p;
+        *dld = ld;
+        return 0;
+    }
+    const int rc = scratch_alloc(s, dev, (size_t)rows * cols * es);
+    *dld = rows > 1 ? rows : 1;
+    if (rc || !upload) return rc;
+    return bxh_memcpy2d_h2d_async(*dev, (size_t)rows * es, p, (size_t)ld * es, (size_t)rows * es,
+                                  (size_t)cols, g_bx.s_comp);
+}
+
+/* a private dense copy of the n x n operand (host or HBM), ld = n */
+static int dev_copy(scratch_t *s, const void *p, int n, int ld, int es, void **dev)
+{
+    const int rc = scratch_alloc(s, dev, (size_t)n * n * es);
+    if (rc) return rc;
+    const size_t w = (size_t)n * es;
+    if (bxh_pointer_is_pinned(p) == 2)
+        return bxh_memcpy2d_d2d_async(*dev, w, p, (size_t)ld * es, w, (size_t)n, g_bx.s_comp);
+    return bxh_memcpy2d_h2d_async(*dev, w, p, (size_t)ld * es, w, (size_t)n, g_bx.s_comp);
+}
+
+/* the device result (rows x cols, leading dimension dld) into the caller's matrix */
+static int dev_store(void *p, int ld, const void *dev, int64_t dld, int rows, int cols, int es)
+{
+    if (p == dev) return 0;
+    const size_t w = (size_t)rows * es;
+    if (bxh_pointer_is_pinned(p) == 2)
+        return bxh_memcpy2d_d2d_async(p, (size_t)ld * es, dev, (size_t)dld * es, w, (size_t)cols,
+                                      g_bx.s_comp);
+    return bxh_memcpy2d_d2h_async(p, (size_t)ld * es, dev, (size_t)dld * es, w, (size_t)cols,
+                                  g_bx.s_comp);
+}
+
+static int finish(scratch_t *s, int rc)
+{
+    const int rs = bxh_stream_sync(g_bx.s_comp);
+    scratch_free(s);
+    pthread_mutex_unlock(&g_bx.gpu_lock);
+    return rc ? rc : rs;
+}
+
+static int gpu_symm(int dt, int herm, int side, int uplo, int M, int N, const double *al,
+                    const void *A, int lda, const void *B, int ldb, const double *be, void *C,
+                    int ldc)
+{
+    const int es = bx_esize(dt), na = side == CblasLeft ? M : N;
+    const int bz = be[0] == 0.0 && be[1] == 0.0;
+    scratch_t s = {{0}, 0};
+    void *dA, *dB, *dC;
+    int64_t lB, lC;
+    pthread_mutex_lock(&g_bx.gpu_lock);
+    int rc = dev_copy(&s, A, na, lda, es, &dA);
+    if (!rc)
+        rc = bxh_tri_fill(dt, na, uplo == CblasLower, herm ? BX_FILL_HERM : BX_FILL_SYM, dA, na,
+                          g_bx.s_comp);
+    if (!rc) rc = dev_view(&s, B, M, N, ldb, es, 1, &dB, &lB);
+    if (!rc) rc = dev_view(&s, C, M, N, ldc, es, !bz, &dC, &lC);
+    if (!rc)
+        rc = side == CblasLeft ? bx_device_gemm(dt, CblasNoTrans, CblasNoTrans, M, N, M, al, dA,
+                                                na, dB, lB, be, dC, lC, g_bx.s_comp)
+                               : bx_device_gemm(dt, CblasNoTrans, CblasNoTrans, M, N, N, al, dB,
+                                                lB, dA, na, be, dC, lC, g_bx.s_comp);
+    if (!rc) rc = dev_store(C, ldc, dC, lC, M, N, es);
+    return finish(&s, rc);
+}
+
+static int gpu_trmm(int dt, int side, int uplo, int trans, int diag, int M, int N,
+                    const double *al, const void *A, int lda, void *B, int ldb)
+{
+    const int es = bx_esize(dt), na = side == CblasLeft ? M : N;
+    const double zero[2] = {0.0, 0.0};
+    scratch_t s = {{0}, 0};
+    void *dT, *dB, *dOut;
+    int64_t lB;
+    pthread_mutex_lock(&g_bx.gpu_lock);
+    int rc = dev_copy(&s, A, na, lda, es, &dT);
+    if (!rc)
+        rc = bxh_tri_fill(dt, na, uplo == CblasLower, diag == CblasUnit ? BX_FILL_UNIT : BX_FILL_TRI,
+                          dT, na, g_bx.s_comp);
+    if (!rc) rc = dev_view(&s, B, M, N, ldb, es, 1, &dB, &lB);
+    if (!rc) rc = scratch_alloc(&s, &dOut, (size_t)M * N * es);
+    if (!rc)
+        rc = side == CblasLeft ? bx_device_gemm(dt, trans, CblasNoTrans, M, N, M, al, dT, na, dB,
+                                                lB, zero, dOut, M, g_bx.s_comp)
+                               : bx_device_gemm(dt, CblasNoTrans, trans, M, N, N, al, dB, lB, dT,
+                                                na, zero, dOut, M, g_bx.s_comp);
+    if (!rc) rc = dev_store(B, ldb, dOut, M, M, N, es);
+    return finish(&s, rc);
+}
+
+static int gpu_syr2k(int dt, int uplo, int trans, int N, int K, const double *al, const void *A,
+                     int lda, const void *B, int ldb, const double *be, void *C, int ldc)
+{
+    const int es = bx_esize(dt);
+    const int rows = trans == CblasNoTrans ? N : K, cols = trans == CblasNoTrans ? K : N;
+    scratch_t s = {{0}, 0};
+    void *dA, *dB, *dC;
+    int64_t lA, lB, lC;
+    pthread_mutex_lock(&g_bx.gpu_lock);
+    int rc = dev_view(&s, A, rows, cols, lda, es, 1, &dA, &lA);
+    if (!rc) rc = dev_view(&s, B, rows, cols, ldb, es, 1, &dB, &lB);
+    /* C goes back whole, so its other triangle is uploaded even when beta == 0 */
+    if (!rc) rc = dev_view(&s, C, N, N, ldc, es, 1, &dC, &lC);
+    if (!rc)
+        rc = bx_device_syrk(dt, uplo == CblasLower ? BX_TRI_LOWER : BX_TRI_UPPER,
+                            trans == CblasNoTrans ? BX_OP_N : BX_OP_T, N, K, al, dA, lA, dB, lB,
+                            be, dC, lC, g_bx.s_comp);
+    if (!rc) rc = dev_store(C, ldc, dC, lC, N, N, es);
+    return finish(&s, rc);
+}
+
+/* ------------------------------------------------------------------ ABI */
+static void scalar2(int dt, const void *v, double out[2])
+{
+    if (dt == BLASX_S)
+        out[0] = *(const float *)v, out[1] = 0;
+    else if (dt == BLASX_D)
+        out[0] = *(const double *)v, out[1] = 0;
+    else
+        out[0] = ((const double *)v)[0], out[1] = ((const double *)v)[1];
+}
+
+static int flip_side(int s) { return s == CblasLeft ? CblasRight : s == CblasRight ? CblasLeft : s; }
+static int flip_uplo(int u) { return u == CblasUpper ? CblasLower : u == CblasLower ? CblasUpper : u; }
+static int side_ok(int s) { return s == CblasLeft || s == CblasRight; }
+static int uplo_ok(int u) { return u == CblasUpper || u == CblasLower; }
+static int max1(int v) { return v > 1 ? v : 1; }
+
+static int gpu_sized(int dt, int a, int b)
+{
+    bx_config();
+    const int thr = dt == BLASX_Z ? g_bx.min_dim_cplx : g_bx.min_dim_real;
+    return a > thr || b > thr;
+}
+
+/* after a GPU attempt: done, or the CPU BLAS (operands beyond HBM, or the
+ * BLASX_CPU_FALLBACK opt-in), or a loud stop */
+static int gpu_outcome(const char *name, int rc)
+{
+    if (rc == 0) {
+        BX_STAT_ADD(calls_gpu, 1);
+        return 0;
+    }
+    if (rc == BLASX_ENOMEM) {
+        bx_log(1, "%s: operands do not fit in HBM; running on the CPU BLAS", name);
+        return 1;
+    }
+    if (g_bx.cpu_fallback) {
+        bx_log(0, "%s: GPU path failed (%d: %s); BLASX_CPU_FALLBACK=1 -> CPU BLAS", name, rc,
+               bxh_last_error());
+        BX_STAT_ADD(cpu_fallbacks, 1);
+        return 1;
+    }
+    bx_fatal("%s: GPU path unavailable or failed (code %d: %s). Set BLASX_CPU_FALLBACK=1 to "
+             "run GPU-sized calls on the CPU BLAS instead.",
+             name, rc, bxh_last_error());
+}
+
+static void need_cpublas(const char *name)
+{
+    if (bx_cpublas_load() != 0)
+        bx_fatal("%s: no CPU BLAS could be loaded (set BLASX_CPUBLAS)", name);
+    BX_STAT_ADD(calls_cpu, 1);
+}
+
+/* ---- SYMM / HEMM ---- */
+static void symm_entry(int dt, int herm, int order, int Side, int Uplo, int M, int N,
+                       const void *alpha, const void *A, int lda, const void *B, int ldb,
+                       const void *beta, void *C, int ldc)
+{
+    const char *name = herm ? "ZHEMM" /* blas/zhemm.c:14 */
+                            : dt == BLASX_S ? "SSYMM " : dt == BLASX_D ? "DSYMM " : "ZSYMM ";
+    int side = Side, uplo = Uplo, m = M, n = N, info = 0;
+    if (order == CblasRowMajor) {
+        /* a row-major C is the column-major C^T = B^T A^T: the other side and
+         * triangle, M and N exchanged (A^T is A, or conj(A) for HEMM, which is
+         * what the column-major view of the stored triangle holds) */
+        side = flip_side(Side);
+        uplo = flip_uplo(Uplo);
+        m = N;
+        n = M;
+    } else if (order != CblasColMajor) {
+        xerbla_(name, &info);
+        return;
+    }
+    const int nrowa = side == CblasLeft ? m : n;
+    if (!side_ok(side)) info = 1;
+    else if (!uplo_ok(uplo)) info = 2;
+    else if (m < 0) info = 3;
+    else if (n < 0) info = 4;
+    else if (lda < max1(nrowa)) info = 7;
+    else if (ldb < max1(m)) info = 9;
+    else if (ldc < max1(m)) info = 12;
+    if (info) {
+        xerbla_(name, &info);
+        return;
+    }
+    if (m == 0 || n == 0) return;
+    double al[2], be[2];
+    scalar2(dt, alpha, al);
+    scalar2(dt, beta, be);
+    if (al[0] == 0.0 && al[1] == 0.0 && be[0] == 1.0 && be[1] == 0.0) return;
+    if (gpu_sized(dt, m, n)) {
+        int rc = bx_ensure_init();
+        if (!rc) rc = gpu_symm(dt, herm, side, uplo, m, n, al, A, lda, B, ldb, be, C, ldc);
+        if (!gpu_outcome(name, rc)) return;
+    }
+    need_cpublas(name);
+    if (herm)
+        bx_cpu_zhemm(order, Side, Uplo, M, N, alpha, A, lda, B, ldb, beta, C, ldc);
+    else if (dt == BLASX_S)
+        bx_cpu_ssymm(order, Side, Uplo, M, N, *(const float *)alpha, A, lda, B, ldb,
+                     *(const float *)beta, C, ldc);
+    else if (dt == BLASX_D)
+        bx_cpu_dsymm(order, Side, Uplo, M, N, *(const double *)alpha, A, lda, B, ldb,
+                     *(const double *)beta, C, ldc);
+    else
+        bx_cpu_zsymm(order, Side, Uplo, M, N, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+/* ---- TRMM ---- */
+static void trmm_entry(int dt, int order, int Side, int Uplo, int TransA, int Diag, int M, int N,
+                       const void *alpha, const void *A, int lda, void *B, int ldb)
+{
+    const char *name = dt == BLASX_S ? "STRMM " : dt == BLASX_D ? "DTRMM " : "ZTRMM ";
+    int side = Side, uplo = Uplo, m = M, n = N, info = 0;
+    if (order == CblasRowMajor) {
+        /* B^T = op(A)^T B^T (or B^T op(A)^T): the other side, and A's stored
+         * triangle read column-major is the other triangle of A^T */
+        side = flip_side(Side);
+        uplo = flip_uplo(Uplo);
+        m = N;
+        n = M;
+    } else if (order != CblasColMajor) {
+        xerbla_(name, &info);
+        return;
+    }
+    const int nrowa = side == CblasLeft ? m : n;
+    if (!side_ok(side)) info = 1;
+    else if (!uplo_ok(uplo)) info = 2;
+    else if (TransA != CblasNoTrans && TransA != CblasTrans && TransA != CblasConjTrans) info = 3;
+    else if (Diag != CblasNonUnit && Diag != CblasUnit) info = 4;
+    else if (m < 0) info = 5;
+    else if (n < 0) info = 6;
+    else if (lda < max1(nrowa)) info = 9;
+    else if (ldb < max1(m)) info = 11;
+    if (info) {
+        xerbla_(name, &info);
+        return;
+    }
+    if (m == 0 || n == 0) return;
+    double al[2];
+    scalar2(dt, alpha, al);
+    if (gpu_sized(dt, m, n)) {
+        int rc = bx_ensure_init();
+        if (!rc) rc = gpu_trmm(dt, side, uplo, TransA, Diag, m, n, al, A, lda, B, ldb);
+        if (!gpu_outcome(name, rc)) return;
+    }
+    need_cpublas(name);
+    if (dt == BLASX_S)
+        bx_cpu_strmm(order, Side, Uplo, TransA, Diag, M, N, *(const float *)alpha, A, lda, B, ldb);
+    else if (dt == BLASX_D)
+        bx_cpu_dtrmm(order, Side, Uplo, TransA, Diag, M, N, *(const double *)alpha, A, lda, B,
+                     ldb);
+    else
+        bx_cpu_ztrmm(order, Side, Uplo, TransA, Diag, M, N, alpha, A, lda, B, ldb);
+}
+
+/* ---- SYR2K ---- */
+static int syr2k_trans_ok(int dt, int t)
+{
+    if (t == CblasNoTrans || t == CblasTrans) return 1;
+    return dt != BLASX_Z && t == CblasConjTrans; /* C means T for real data */
+}
+
+static void syr2k_entry(int dt, int order, int Uplo, int Trans, int N, int K, const void *alpha,
+                        const void *A, int lda, const void *B, int ldb, const void *beta, void *C,
+                        int ldc)
+{
+    const char *name = dt == BLASX_S ? "SSYR2K" : dt == BLASX_D ? "DSYR2K" : "ZSYR2K";
+    int uplo = Uplo, trans = Trans, info = 0;
+    if (order == CblasRowMajor) {
+        uplo = flip_uplo(Uplo);
+        if (syr2k_trans_ok(dt, Trans)) trans = Trans == CblasNoTrans ? CblasTrans : CblasNoTrans;
+    } else if (order != CblasColMajor) {
+        xerbla_(name, &info);
+        return;
+    }
+    const int nrowa = trans == CblasNoTrans ? N : K;
+    if (!uplo_ok(uplo)) info = 1;
+    else if (!syr2k_trans_ok(dt, trans)) info = 2;
+    else if (N < 0) info = 3;
+    else if (K < 0) info = 4;
+    else if (lda < max1(nrowa)) info = 7;
+    else if (ldb < max1(nrowa)) info = 9;
+    else if (ldc < max1(N)) info = 12;
+    if (info) {
+        xerbla_(name, &info);
+        return;
+    }
+    if (N == 0) return;
+    double al[2], be[2];
+    scalar2(dt, alpha, al);
+    scalar2(dt, beta, be);
+    const int no_product = K == 0 || (al[0] == 0.0 && al[1] == 0.0);
+    if (no_product && be[0] == 1.0 && be[1] == 0.0) return;
+    const int in_hbm = bxh_pointer_is_pinned(C) == 2;
+    if (gpu_sized(dt, N, K) && (!no_product || in_hbm)) {
+        int rc = bx_ensure_init();
+        if (!rc) rc = gpu_syr2k(dt, uplo, trans, N, K, al, A, lda, B, ldb, be, C, ldc);
+        if (!gpu_outcome(name, rc)) return;
+    }
+    need_cpublas(name);
+    if (dt == BLASX_S)
+        bx_cpu_ssyr2k(order, Uplo, Trans, N, K, *(const float *)alpha, A, lda, B, ldb,
+                      *(const float *)beta, C, ldc);
+    else if (dt == BLASX_D)
+        bx_cpu_dsyr2k(order, Uplo, Trans, N, K, *(const double *)alpha, A, lda, B, ldb,
+                      *(const double *)beta, C, ldc);
+    else
+        bx_cpu_zsyr2k(order, Uplo, Trans, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+/* ------------------------------------------------------------ CBLAS */
+BX_EXPORT void cblas_ssymm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                           const enum CBLAS_UPLO Uplo, const int M, const int N, const float alpha,
+                           const float *A, const int lda, const float *B, const int ldb,
+                           const float beta, float *C, const int ldc)
+{
+    symm_entry(BLASX_S, 0, Order, Side, Uplo, M, N, &alpha, A, lda, B, ldb, &beta, C, ldc);
+}
+
+BX_EXPORT void cblas_dsymm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                           const enum CBLAS_UPLO Uplo, const int M, const int N,
+                           const double alpha, const double *A, const int lda, const double *B,
+                           const int ldb, const double beta, double *C, const int ldc)
+{
+    symm_entry(BLASX_D, 0, Order, Side, Uplo, M, N, &alpha, A, lda, B, ldb, &beta, C, ldc);
+}
+
+BX_EXPORT void cblas_zsymm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                           const enum CBLAS_UPLO Uplo, const int M, const int N, const void *alpha,
+                           const void *A, const int lda, const void *B, const int ldb,
+                           const void *beta, void *C, const int ldc)
+{
+    symm_entry(BLASX_Z, 0, Order, Side, Uplo, M, N, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+BX_EXPORT void cblas_zhemm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                           const enum CBLAS_UPLO Uplo, const int M, const int N, const void *alpha,
+                           const void *A, const int lda, const void *B, const int ldb,
+                           const void *beta, void *C, const int ldc)
+{
+    symm_entry(BLASX_Z, 1, Order, Side, Uplo, M, N, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+BX_EXPORT void cblas_strmm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                           const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TransA,
+                           const enum CBLAS_DIAG Diag, const int M, const int N, const float alpha,
+                           const float *A, const int lda, float *B, const int ldb)
+{
+    trmm_entry(BLASX_S, Order, Side, Uplo, TransA, Diag, M, N, &alpha, A, lda, B, ldb);
+}
+
+BX_EXPORT void cblas_dtrmm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                           const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TransA,
+                           const enum CBLAS_DIAG Diag, const int M, const int N,
+                           const double alpha, const double *A, const int lda, double *B,
+                           const int ldb)
+{
+    trmm_entry(BLASX_D, Order, Side, Uplo, TransA, Diag, M, N, &alpha, A, lda, B, ldb);
+}
+
+BX_EXPORT void cblas_ztrmm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                           const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TransA,
+                           const enum CBLAS_DIAG Diag, const int M, const int N, const void *alpha,
+                           const void *A, const int lda, void *B, const int ldb)
+{
+    trmm_entry(BLASX_Z, Order, Side, Uplo, TransA, Diag, M, N, alpha, A, lda, B, ldb);
+}
+
+BX_EXPORT void cblas_ssyr2k(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                            const enum CBLAS_TRANSPOSE Trans, const int N, const int K,
+                            const float alpha, const float *A, const int lda, const float *B,
+                            const int ldb, const float beta, float *C, const int ldc)
+{
+    syr2k_entry(BLASX_S, Order, Uplo, Trans, N, K, &alpha, A, lda, B, ldb, &beta, C, ldc);
+}
+
+BX_EXPORT void cblas_dsyr2k(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                            const enum CBLAS_TRANSPOSE Trans, const int N, const int K,
+                            const double alpha, const double *A, const int lda, const double *B,
+                            const int ldb, const double beta, double *C, const int ldc)
+{
+    syr2k_entry(BLASX_D, Order, Uplo, Trans, N, K, &alpha, A, lda, B, ldb, &beta, C, ldc);
+}
+
+BX_EXPORT void cblas_zsyr2k(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                            const enum CBLAS_TRANSPOSE Trans, const int N, const int K,
+                            const void *alpha, const void *A, const int lda, const void *B,
+                            const int ldb, const void *beta, void *C, const int ldc)
+{
+    syr2k_entry(BLASX_Z, Order, Uplo, Trans, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+/* ------------------------------------------------------------ f77 */
+static int f77_side(const char *c, int *out)
+{
+    switch (*c) {
+    case 'L': case 'l': *out = CblasLeft; return 0;
+    case 'R': case 'r': *out = CblasRight; return 0;
+    default: return -1;
+    }
+}
+
+static int f77_uplo(const char *c, int *out)
+{
+    switch (*c) {
+    case 'U': case 'u': *out = CblasUpper; return 0;
+    case 'L': case 'l': *out = CblasLower; return 0;
+    default: return -1;
+    }
+}
+
+static int f77_trans(const char *c, int *out)
+{
+    switch (*c) {
+    case 'N': case 'n': *out = CblasNoTrans; return 0;
+    case 'T': case 't': *out = CblasTrans; return 0;
+    case 'C': case 'c': *out = CblasConjTrans; return 0;
+    default: return -1;
+    }
+}
+
+static int f77_diag(const char *c, int *out)
+{
+    switch (*c) {
+    case 'N': case 'n': *out = CblasNonUnit; return 0;
+    case 'U': case 'u': *out = CblasUnit; return 0;
+    default: return -1;
+    }
+}
+
+/* a bad character argument: xerbla_ with its position (blas/dsymm.c:83-84,
+ * blas/dtrmm.c:76-80, blas/dsyr2k.c:97-98; the lowest position wins) */
+static int f77_bad(const char *name, int info)
+{
+    xerbla_(name, &info);
+    return 1;
+}
+
+#define F77_SYMM(NAME, DT, HERM, TNAME, T)                                                     \
+    BX_EXPORT void NAME(const char *side, const char *uplo, const int *m, const int *n,         \
+                        const T *alpha, const T *A, const int *lda, const T *B, const int *ldb, \
+                        const T *beta, T *C, const int *ldc)                                    \
+    {                                                                                          \
+        int s, u;                                                                              \
+        if (f77_side(side, &s) && f77_bad(TNAME, 1)) return;                                   \
+        if (f77_uplo(uplo, &u) && f77_bad(TNAME, 2)) return;                                   \
+        symm_entry(DT, HERM, CblasColMajor, s, u, *m, *n, alpha, A, *lda, B, *ldb, beta, C,    \
+                   *ldc);                                                                      \
+    }
+F77_SYMM(ssymm_, BLASX_S, 0, "SSYMM ", float)
+F77_SYMM(dsymm_, BLASX_D, 0, "DSYMM ", double)
+F77_SYMM(zsymm_, BLASX_Z, 0, "ZSYMM ", void)
+F77_SYMM(zhemm_, BLASX_Z, 1, "ZHEMM", void)
+
+#define F77_TRMM(NAME, DT, TNAME, T)                                                           \
+    BX_EXPORT void NAME(const char *side, const char *uplo, const char *transa,                \
+                        const char *diag, const int *m, const int *n, const T *alpha,           \
+                        const T *A, const int *lda, T *B, const int *ldb)                       \
+    {                                                                                          \
+        int s, u, t, d;                                                                        \
+        if (f77_side(side, &s) && f77_bad(TNAME, 1)) return;                                   \
+        if (f77_uplo(uplo, &u) && f77_bad(TNAME, 2)) return;                                   \
+        if (f77_trans(transa, &t) && f77_bad(TNAME, 3)) return;                                \
+        if (f77_diag(diag, &d) && f77_bad(TNAME, 4)) return;                                   \
+        trmm_entry(DT, CblasColMajor, s, u, t, d, *m, *n, alpha, A, *lda, B, *ldb);            \
+    }
+F77_TRMM(strmm_, BLASX_S, "STRMM ", float)
+F77_TRMM(dtrmm_, BLASX_D, "DTRMM ", double)
+F77_TRMM(ztrmm_, BLASX_Z, "ZTRMM ", void)
+
+#define F77_SYR2K(NAME, DT, TNAME, T)                                                          \
+    BX_EXPORT void NAME(const char *uplo, const char *trans, const int *n, const int *k,        \
+                        const T *alpha, const T *A, const int *lda, const T *B, const int *ldb, \
+                        const T *beta, T *C, const int *ldc)                                    \
+    {                                                                                          \
+        int u, t;                                                                              \
+        if (f77_uplo(uplo, &u) && f77_bad(TNAME, 1)) return;                                   \
+        if (f77_trans(trans, &t) && f77_bad(TNAME, 2)) return;                                 \
+        syr2k_entry(DT, CblasColMajor, u, t, *n, *k, alpha, A, *lda, B, *ldb, beta, C, *ldc);  \
+    }
+F77_SYR2K(ssyr2k_, BLASX_S, "SSYR2K", float)
+F77_SYR2K(dsyr2k_, BLASX_D, "DSYR2K", double)
+F77_SYR2K(zsyr2k_, BLASX_Z, "ZSYR2K", void)

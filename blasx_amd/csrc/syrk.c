@@ -29,6 +29,7 @@
 #include <string.h>
 
 #include "blasx.h"
+#include "cpu_forward.h"
 #include "internal.h"
 
 /* ---------------------------------------------------------------- device */
@@ -40,9 +41,10 @@ static struct {
     size_t cap;
 } g_sbuf;
 
-static void add_problem(bxh_problem_t *P, bxh_seg_t *S, int idx, int dt, void *C, int64_t ldc,
-                        int m, int n, const void *a, const void *b, int64_t lda, int K, int opa,
-                        int opb, const double *alpha, const double *beta, int tri, int *total)
+/* one problem (C block m x n) with its K segments seg[0..nseg) */
+static void add_problem(bxh_problem_t *P, bxh_seg_t *S, int idx, int nseg, const bxh_seg_t *seg,
+                        int dt, void *C, int64_t ldc, int m, int n, int opa, int opb,
+                        const double *alpha, const double *beta, int tri, int *total)
 {
     const int bm = bxh_tile_m(dt), bn = bxh_tile_n(dt);
     memset(P, 0, sizeof *P);
@@ -50,8 +52,8 @@ static void add_problem(bxh_problem_t *P, bxh_seg_t *S, int idx, int dt, void *C
     P->ldc = ldc;
     P->m = m;
     P->n = n;
-    P->nseg = 1;
-    P->seg_off = idx;
+    P->nseg = nseg;
+    P->seg_off = idx * nseg;
     P->opa = opa;
     P->opb = opb;
     P->tiles_m = (m + bm - 1) / bm;
@@ -65,17 +67,15 @@ static void add_problem(bxh_problem_t *P, bxh_seg_t *S, int idx, int dt, void *C
     P->beta_zero = beta[0] == 0.0 && beta[1] == 0.0;
     P->tri = tri;
     P->tri_off = 0;
-    S->A = a;
-    S->B = b;
-    S->lda = lda;
-    S->ldb = lda;
-    S->k = K;
+    for (int i = 0; i < nseg; ++i) S[i] = seg[i];
 }
 
 /* operands, C and the launch on `stream` (NULL: the library's compute
- * stream); asynchronous, like the other device entry points */
+ * stream); asynchronous, like the other device entry points.  With B (SYR2K)
+ * C = alpha op(A) op(B)^T + alpha op(B) op(A)^T + beta C on the triangle. */
 int bx_device_syrk(int dt, int tri, int opa, int N, int K, const double *alpha, const void *A,
-                   int64_t lda, const double *beta, void *C, int64_t ldc, void *stream)
+                   int64_t lda, const void *B, int64_t ldb, const double *beta, void *C,
+                   int64_t ldc, void *stream)
 {
     if (bx_ensure_init() != 0) return BLASX_ENOGPU;
     void *st = stream ? stream : g_bx.s_comp;
@@ -85,9 +85,9 @@ int bx_device_syrk(int dt, int tri, int opa, int N, int K, const double *alpha, 
 
     const int es = bx_esize(dt);
     const int opb = opa != BX_OP_N ? BX_OP_N : (tri & BX_TRI_HERM) ? BX_OP_C : BX_OP_T;
-    const int npan = (N + PANEL - 1) / PANEL, maxp = 2 * npan;
+    const int npan = (N + PANEL - 1) / PANEL, maxp = 2 * npan, nseg = B ? 2 : 1;
     const size_t prob_bytes = ((size_t)maxp * sizeof(bxh_problem_t) + 255) & ~(size_t)255;
-    const size_t need = prob_bytes + (size_t)maxp * sizeof(bxh_seg_t);
+    const size_t need = prob_bytes + (size_t)maxp * nseg * sizeof(bxh_seg_t);
     int rc = 0;
     pthread_mutex_lock(&g_slock);
     if (!g_sbuf.done) rc = bxh_event_create(&g_sbuf.done, 0);
@@ -104,30 +104,42 @@ int bx_device_syrk(int dt, int tri, int opa, int N, int K, const double *alpha, 
     if (!rc) {
         bxh_problem_t *probs = (bxh_problem_t *)g_sbuf.host;
         bxh_seg_t *segs = (bxh_seg_t *)((char *)g_sbuf.host + prob_bytes);
-        const char *a = (const char *)A;
+        const char *a = (const char *)A, *b = B ? (const char *)B : (const char *)A;
         char *c = (char *)C;
         /* row x of op(A) starts at element x of A (op N) or at column x (T, C) */
         const int64_t step = (opa == BX_OP_N ? 1 : lda) * es;
+        const int64_t stepb = (opa == BX_OP_N ? 1 : ldb) * es;
         int np = 0, total = 0;
         for (int r0 = 0; r0 < N; r0 += PANEL) {
             const int rows = N - r0 < PANEL ? N - r0 : PANEL;
-            add_problem(&probs[np], &segs[np], np, dt, c + (r0 + r0 * ldc) * es, ldc, rows, rows,
-                        a + r0 * step, a + r0 * step, lda, K, opa, opb, alpha, beta, tri, &total);
-            ++np;
-            if (tri & BX_TRI_LOWER) {
-                if (r0 > 0) {
-                    add_problem(&probs[np], &segs[np], np, dt, c + (int64_t)r0 * es, ldc, rows, r0,
-                                a + r0 * step, a, lda, K, opa, opb, alpha, beta, 0, &total);
-                    ++np;
+            /* rows r0.. of op(C) against columns c0.. (count n) */
+            int c0 = r0, n = rows, with_tri = tri;
+            for (int part = 0; part < 2; ++part) {
+                if (part == 1) {
+                    with_tri = 0;
+                    if (tri & BX_TRI_LOWER)
+                        c0 = 0, n = r0;
+                    else
+                        c0 = r0 + rows, n = N - c0;
+                    if (n <= 0) break;
                 }
-            } else {
-                const int c0 = r0 + rows;
-                if (c0 < N) {
-                    add_problem(&probs[np], &segs[np], np, dt, c + (r0 + (int64_t)c0 * ldc) * es,
-                                ldc, rows, N - c0, a + r0 * step, a + c0 * step, lda, K, opa, opb,
-                                alpha, beta, 0, &total);
-                    ++np;
-                }
+                /* term 1: op(A) rows r0.. x op(B) rows c0..; term 2 (SYR2K): op(B)
+                 * rows r0.. x op(A) rows c0..; SYRK has B = A and one term */
+                bxh_seg_t sg[2];
+                sg[0].A = a + r0 * step;
+                sg[0].lda = lda;
+                sg[0].B = B ? b + c0 * stepb : a + c0 * step;
+                sg[0].ldb = B ? ldb : lda;
+                sg[0].k = K;
+                sg[1].A = b + r0 * stepb;
+                sg[1].lda = ldb;
+                sg[1].B = a + c0 * step;
+                sg[1].ldb = lda;
+                sg[1].k = K;
+                add_problem(&probs[np], &segs[np * nseg], np, nseg, sg, dt,
+                            c + (r0 + (int64_t)c0 * ldc) * es, ldc, rows, n, opa, opb, alpha,
+                            beta, with_tri, &total);
+                ++np;
             }
         }
         char *dv = (char *)g_sbuf.dev;
@@ -138,7 +150,7 @@ int bx_device_syrk(int dt, int tri, int opa, int N, int K, const double *alpha, 
         if (!rc) rc = bxh_event_record(g_sbuf.done, st);
         if (!rc) {
             BX_STAT_ADD(kernel_launches, 1);
-            bx_stat_flops((dt == BLASX_Z ? 4.0 : 1.0) * (double)N * (N + 1) * K);
+            bx_stat_flops(nseg * (dt == BLASX_Z ? 4.0 : 1.0) * (double)N * (N + 1) * K);
         }
     }
     pthread_mutex_unlock(&g_slock);
@@ -171,50 +183,23 @@ static void scalar2(int dt, int herk, const void *v, double out[2])
     }
 }
 
-typedef void (*syrk_s_t)(int, int, int, int, int, float, const void *, int, float, void *, int);
-typedef void (*syrk_d_t)(int, int, int, int, int, double, const void *, int, double, void *, int);
-typedef void (*syrk_z_t)(int, int, int, int, int, const void *, const void *, int, const void *,
-                         void *, int);
-typedef void (*syrk_s64_t)(int, int, int, int64_t, int64_t, float, const void *, int64_t, float,
-                           void *, int64_t);
-typedef void (*syrk_d64_t)(int, int, int, int64_t, int64_t, double, const void *, int64_t, double,
-                           void *, int64_t);
-typedef void (*syrk_z64_t)(int, int, int, int64_t, int64_t, const void *, const void *, int64_t,
-                           const void *, void *, int64_t);
-
 /* the CPU BLAS's own routine, Order passed through, as the reference does
  * (blas/dsyrk.c:58) */
 static int cpu_syrk(int dt, int herk, int order, int uplo, int trans, int N, int K,
                     const void *alpha, const void *A, int lda, const void *beta, void *C, int ldc)
 {
-    static void *slot[4];
-    const int which = herk ? 3 : dt == BLASX_S ? 0 : dt == BLASX_D ? 1 : 2;
-    static const char *names[4] = {"cblas_ssyrk", "cblas_dsyrk", "cblas_zsyrk", "cblas_zherk"};
-    void *f = __atomic_load_n(&slot[which], __ATOMIC_ACQUIRE);
-    if (!f) {
-        f = bx_cpublas_sym(names[which]);
-        if (!f) return BLASX_ECPU;
-        __atomic_store_n(&slot[which], f, __ATOMIC_RELEASE);
-    }
-    const int ilp = bx_cpublas_ilp64();
-    if (dt == BLASX_S) {
-        const float a = *(const float *)alpha, b = *(const float *)beta;
-        if (ilp)
-            ((syrk_s64_t)f)(order, uplo, trans, N, K, a, A, lda, b, C, ldc);
-        else
-            ((syrk_s_t)f)(order, uplo, trans, N, K, a, A, lda, b, C, ldc);
-    } else if (dt == BLASX_D || herk) {
-        const double a = *(const double *)alpha, b = *(const double *)beta;
-        if (ilp)
-            ((syrk_d64_t)f)(order, uplo, trans, N, K, a, A, lda, b, C, ldc);
-        else
-            ((syrk_d_t)f)(order, uplo, trans, N, K, a, A, lda, b, C, ldc);
-    } else {
-        if (ilp)
-            ((syrk_z64_t)f)(order, uplo, trans, N, K, alpha, A, lda, beta, C, ldc);
-        else
-            ((syrk_z_t)f)(order, uplo, trans, N, K, alpha, A, lda, beta, C, ldc);
-    }
+    if (bx_cpublas_load() != 0) return BLASX_ECPU;
+    if (herk)
+        bx_cpu_zherk(order, uplo, trans, N, K, *(const double *)alpha, A, lda,
+                     *(const double *)beta, C, ldc);
+    else if (dt == BLASX_S)
+        bx_cpu_ssyrk(order, uplo, trans, N, K, *(const float *)alpha, A, lda,
+                     *(const float *)beta, C, ldc);
+    else if (dt == BLASX_D)
+        bx_cpu_dsyrk(order, uplo, trans, N, K, *(const double *)alpha, A, lda,
+                     *(const double *)beta, C, ldc);
+    else
+        bx_cpu_zsyrk(order, uplo, trans, N, K, alpha, A, lda, beta, C, ldc);
     BX_STAT_ADD(calls_cpu, 1);
     return 0;
 }
@@ -267,7 +252,7 @@ static void syrk_entry(int dt, int herk, int order, int Uplo, int Trans, int N, 
     if (rc == 0) {
         if (bxh_pointer_is_pinned(C) == 2 && (no_product || bxh_pointer_is_pinned(A) == 2)) {
             /* operands already in HBM: run in place */
-            rc = bx_device_syrk(dt, tri, opa, N, K, al, A, lda, be, C, ldc, NULL);
+            rc = bx_device_syrk(dt, tri, opa, N, K, al, A, lda, NULL, 0, be, C, ldc, NULL);
             if (rc == 0) rc = bxh_stream_sync(g_bx.s_comp);
         } else if (no_product) {
             /* C = beta C on the triangle: host work, no product to run */

@@ -97,6 +97,90 @@ void zsyrk_(const char *uplo, const char *trans, const int *n, const int *k, con
 void zherk_(const char *uplo, const char *trans, const int *n, const int *k, const double *alpha,
             const void *A, const int *lda, const double *beta, void *C, const int *ldc);
 
+/* ---- symmetric / Hermitian / triangular products and rank-2k updates on
+ * the GPU (one GEMM on an operand materialised in HBM; SYR2K as a two-term
+ * rank-k launch) ------------------------------------------------------------
+ * replace cblas_ssymm blas/ssymm.c:16, cblas_dsymm blas/dsymm.c:16,
+ * cblas_zsymm blas/zsymm.c:16, cblas_zhemm blas/zhemm.c:16 */
+void cblas_ssymm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const int M, const int N, const float alpha,
+                 const float *A, const int lda, const float *B, const int ldb, const float beta,
+                 float *C, const int ldc);
+void cblas_dsymm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const int M, const int N, const double alpha,
+                 const double *A, const int lda, const double *B, const int ldb,
+                 const double beta, double *C, const int ldc);
+void cblas_zsymm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const int M, const int N, const void *alpha,
+                 const void *A, const int lda, const void *B, const int ldb, const void *beta,
+                 void *C, const int ldc);
+void cblas_zhemm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const int M, const int N, const void *alpha,
+                 const void *A, const int lda, const void *B, const int ldb, const void *beta,
+                 void *C, const int ldc);
+/* replace cblas_strmm blas/strmm.c:17, cblas_dtrmm blas/dtrmm.c:17,
+ * cblas_ztrmm blas/ztrmm.c:17 */
+void cblas_strmm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TransA,
+                 const enum CBLAS_DIAG Diag, const int M, const int N, const float alpha,
+                 const float *A, const int lda, float *B, const int ldb);
+void cblas_dtrmm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TransA,
+                 const enum CBLAS_DIAG Diag, const int M, const int N, const double alpha,
+                 const double *A, const int lda, double *B, const int ldb);
+void cblas_ztrmm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TransA,
+                 const enum CBLAS_DIAG Diag, const int M, const int N, const void *alpha,
+                 const void *A, const int lda, void *B, const int ldb);
+/* replace cblas_ssyr2k blas/ssyr2k.c:16, cblas_dsyr2k blas/dsyr2k.c:16,
+ * cblas_zsyr2k blas/zsyr2k.c:16 */
+void cblas_ssyr2k(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                  const enum CBLAS_TRANSPOSE Trans, const int N, const int K, const float alpha,
+                  const float *A, const int lda, const float *B, const int ldb, const float beta,
+                  float *C, const int ldc);
+void cblas_dsyr2k(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                  const enum CBLAS_TRANSPOSE Trans, const int N, const int K, const double alpha,
+                  const double *A, const int lda, const double *B, const int ldb,
+                  const double beta, double *C, const int ldc);
+void cblas_zsyr2k(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                  const enum CBLAS_TRANSPOSE Trans, const int N, const int K, const void *alpha,
+                  const void *A, const int lda, const void *B, const int ldb, const void *beta,
+                  void *C, const int ldc);
+/* replace ssymm_ blas/ssymm.c:74, dsymm_ blas/dsymm.c:74, zsymm_ blas/zsymm.c:74,
+ * zhemm_ blas/zhemm.c:80 */
+void ssymm_(const char *side, const char *uplo, const int *m, const int *n, const float *alpha,
+            const float *A, const int *lda, const float *B, const int *ldb, const float *beta,
+            float *C, const int *ldc);
+void dsymm_(const char *side, const char *uplo, const int *m, const int *n, const double *alpha,
+            const double *A, const int *lda, const double *B, const int *ldb, const double *beta,
+            double *C, const int *ldc);
+void zsymm_(const char *side, const char *uplo, const int *m, const int *n, const void *alpha,
+            const void *A, const int *lda, const void *B, const int *ldb, const void *beta,
+            void *C, const int *ldc);
+void zhemm_(const char *side, const char *uplo, const int *m, const int *n, const void *alpha,
+            const void *A, const int *lda, const void *B, const int *ldb, const void *beta,
+            void *C, const int *ldc);
+/* replace strmm_ blas/strmm.c:67, dtrmm_ blas/dtrmm.c:67, ztrmm_ blas/ztrmm.c:67 */
+void strmm_(const char *side, const char *uplo, const char *transa, const char *diag,
+            const int *m, const int *n, const float *alpha, const float *A, const int *lda,
+            float *B, const int *ldb);
+void dtrmm_(const char *side, const char *uplo, const char *transa, const char *diag,
+            const int *m, const int *n, const double *alpha, const double *A, const int *lda,
+            double *B, const int *ldb);
+void ztrmm_(const char *side, const char *uplo, const char *transa, const char *diag,
+            const int *m, const int *n, const void *alpha, const void *A, const int *lda,
+            void *B, const int *ldb);
+/* replace ssyr2k_ blas/ssyr2k.c:88, dsyr2k_ blas/dsyr2k.c:88, zsyr2k_ blas/zsyr2k.c:88 */
+void ssyr2k_(const char *uplo, const char *trans, const int *n, const int *k, const float *alpha,
+             const float *A, const int *lda, const float *B, const int *ldb, const float *beta,
+             float *C, const int *ldc);
+void dsyr2k_(const char *uplo, const char *trans, const int *n, const int *k, const double *alpha,
+             const double *A, const int *lda, const double *B, const int *ldb,
+             const double *beta, double *C, const int *ldc);
+void zsyr2k_(const char *uplo, const char *trans, const int *n, const int *k, const void *alpha,
+             const void *A, const int *lda, const void *B, const int *ldb, const void *beta,
+             void *C, const int *ldc);
+
 /* replaces xerbla_, blas/error.c:5-8.  Weak: a caller (LAPACK) may override it. */
 void xerbla_(const char *srname, const int *info);
 

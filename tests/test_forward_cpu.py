@@ -615,10 +615,12 @@ def test_trmm_trsm(api, p, side, uplo, trans):
 
 
 def test_generated_sources_are_current():
-    """forward.c and blasx_blas.h are what gen_forward.py writes."""
+    """forward.c, cpu_forward.h and blasx_blas.h are what gen_forward.py writes."""
     with tempfile.TemporaryDirectory() as d:
         subprocess.run([sys.executable, os.path.join(ROOT, "blasx_amd", "csrc", "gen_forward.py"), d],
                        check=True)
-        for got, want in (("forward.c", os.path.join(ROOT, "blasx_amd", "csrc", "forward.c")),
+        csrc = os.path.join(ROOT, "blasx_amd", "csrc")
+        for got, want in (("forward.c", os.path.join(csrc, "forward.c")),
+                          ("cpu_forward.h", os.path.join(csrc, "cpu_forward.h")),
                           ("blasx_blas.h", os.path.join(ROOT, "include", "blasx_blas.h"))):
             assert open(os.path.join(d, got)).read() == open(want).read(), want
