@@ -354,6 +354,46 @@ def rank_k(bx, n=32768):
     return out
 
 
+def level3_inhbm(bx, n=32768):
+    """DSYMM (Left, Lower), DTRMM (Left, Lower, NoTrans, NonUnit) and DSYR2K
+    (Lower, NoTrans) through the BLAS ABI on operands in HBM, M = N (= K) = n.
+    FLOPs: SYMM 2M^2N, TRMM M^2N, SYR2K 2N(N+1)K."""
+    import ctypes as C
+
+    import numpy as np
+
+    L = bx.lib()
+    out = {}
+    mats = [bx.DeviceArray(n, n, np.float64) for _ in range(3)]
+    for i, m in enumerate(mats):
+        m.fill_uniform(1, i)
+    bx.synchronize()
+    dA, dB, dC = (m.ptr for m in mats)
+    V, D = C.c_void_p, C.c_double
+    calls = {
+        "dsymm": (2.0 * n ** 3, lambda: L.cblas_dsymm(102, 141, 122, n, n, D(1.0), V(dA), n, V(dB),
+                                                     n, D(1.0), V(dC), n)),
+        "dtrmm": (float(n) ** 3, lambda: L.cblas_dtrmm(102, 141, 122, 111, 131, n, n, D(1.0),
+                                                      V(dA), n, V(dB), n)),
+        "dsyr2k": (2.0 * n * (n + 1) * n, lambda: L.cblas_dsyr2k(102, 122, 111, n, n, D(1.0),
+                                                               V(dA), n, V(dB), n, D(1.0),
+                                                               V(dC), n)),
+    }
+    for name, (flops, fn) in calls.items():
+        fn()  # warm-up
+        times = []
+        for _ in range(2):
+            t = time.perf_counter()
+            fn()
+            times.append(time.perf_counter() - t)
+        best = min(times)
+        log(f"{name} in HBM n={n}: {best:.3f}s ({flops / best / 1e12:.2f} TFLOP/s)")
+        out[name + "_tflops"] = round(flops / best / 1e12, 3)
+    for m in mats:
+        m.free()
+    return out
+
+
 def matrix_inhbm(bx, n=32768):
     """The kernel on operands resident in HBM for every element type and
     transpose the ABI serves: SGEMM and DGEMM NN/NT/TN/TT, ZGEMM NN/NT/TN/TT/
@@ -881,7 +921,7 @@ def main():
     ap.add_argument("--crt-probe", type=int, default=0, metavar="MODULI",
                     help="one CRT split DGEMM at --n with MODULI moduli, for rocprofv3 passes")
     ap.add_argument("--only", default="", help="comma list of extras to run "
-                    "(matrix,host,hostsz,ooc,split,zsplit,serve,syrk); default all")
+                    "(matrix,host,hostsz,ooc,split,zsplit,serve,syrk,level3); default all")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build targets one GPU; --gpus {args.gpus} is not supported "
@@ -944,7 +984,8 @@ def main():
                     ("split", "split_precision", lambda: split_precision(bx, n=args.split_n)),
                     ("zsplit", "zsplit_precision", lambda: zsplit_precision(bx)),
                     ("serve", "serving", lambda: serving(bx, nreq=args.serve_requests)),
-                    ("syrk", "rank_k", lambda: rank_k(bx, n))]
+                    ("syrk", "rank_k", lambda: rank_k(bx, n)),
+                    ("level3", "level3_inhbm", lambda: level3_inhbm(bx, n))]
         for key, name, fn in sections:
             if only and key not in only:
                 continue

@@ -17,10 +17,11 @@
  * names, the "an extent above 1000 (500 complex)" size test.
  *
  * On the GPU:
- *   SYMM/HEMM and TRMM are one GEMM on an operand materialised in HBM
- *   (bxh_tri_fill mirrors or zeroes the other triangle: O(n^2) bytes against
- *   the GEMM's O(n^2 m) FLOPs).  TRMM writes a separate output that is then
- *   copied over B.
+ *   SYMM/HEMM are one GEMM on an operand materialised in HBM (bxh_tri_fill
+ *   mirrors the stored triangle: O(n^2) bytes against the GEMM's O(n^2 m)
+ *   FLOPs).  TRMM materialises the triangular matrix the same way and runs
+ *   one grouped launch of panel GEMMs, each over the K range where op(A) is
+ *   nonzero (half a full GEMM's work), into an output then copied over B.
  *   SYR2K is the rank-k grouped launch with two K segments per problem
  *   (bx_device_syrk).
  * Host operands are copied whole to HBM for the call: these routines run in
@@ -138,11 +139,12 @@ static int gpu_symm(int dt, int herm, int side, int uplo, int M, int N, const do
     return finish(&s, rc);
 }
 
+enum { TRMM_PANEL = 256 };
+
 static int gpu_trmm(int dt, int side, int uplo, int trans, int diag, int M, int N,
                     const double *al, const void *A, int lda, void *B, int ldb)
 {
     const int es = bx_esize(dt), na = side == CblasLeft ? M : N;
-    const double zero[2] = {0.0, 0.0};
     scratch_t s = {{0}, 0};
     void *dT, *dB, *dOut;
     int64_t lB;
@@ -153,13 +155,57 @@ static int gpu_trmm(int dt, int side, int uplo, int trans, int diag, int M, int 
                           dT, na, g_bx.s_comp);
     if (!rc) rc = dev_view(&s, B, M, N, ldb, es, 1, &dB, &lB);
     if (!rc) rc = scratch_alloc(&s, &dOut, (size_t)M * N * es);
-    if (!rc)
-        rc = side == CblasLeft ? bx_device_gemm(dt, trans, CblasNoTrans, M, N, M, al, dT, na, dB,
-                                                lB, zero, dOut, M, g_bx.s_comp)
-                               : bx_device_gemm(dt, CblasNoTrans, trans, M, N, N, al, dB, lB, dT,
-                                                na, zero, dOut, M, g_bx.s_comp);
+    /* op(T) is triangular, so each 256-wide panel of the output (rows for
+     * Left, columns for Right) needs only the K range where op(T) is nonzero:
+     * one grouped launch of panel GEMMs does half a full GEMM's work */
+    const int npan = (na + TRMM_PANEL - 1) / TRMM_PANEL;
+    blasx_gemm_desc_t *d = rc ? NULL : (blasx_gemm_desc_t *)calloc(npan, sizeof *d);
+    if (!rc && !d) rc = BLASX_ENOMEM;
+    if (!rc) {
+        const int op_lower = (uplo == CblasLower) == (trans == CblasNoTrans);
+        const char *t = (const char *)dT, *b = (const char *)dB;
+        char *o = (char *)dOut;
+        for (int i = 0; i < npan; ++i) {
+            const int x0 = i * TRMM_PANEL, w = na - x0 < TRMM_PANEL ? na - x0 : TRMM_PANEL;
+            blasx_gemm_desc_t *q = &d[i];
+            q->dtype = dt;
+            q->alpha[0] = al[0];
+            q->alpha[1] = al[1];
+            int k0, k1;
+            if (side == CblasLeft) {
+                /* out[x0.., :] = alpha op(T)[x0.., k0:k1] B[k0:k1, :] */
+                k0 = op_lower ? 0 : x0;
+                k1 = op_lower ? x0 + w : M;
+                q->transa = trans;
+                q->transb = CblasNoTrans;
+                q->M = w, q->N = N, q->K = k1 - k0;
+                /* op(T)[x0, k0] is T[x0, k0] (N) or T[k0, x0] (T, C) */
+                q->A = t + (trans == CblasNoTrans ? x0 + (int64_t)k0 * na : k0 + (int64_t)x0 * na) * es;
+                q->lda = na;
+                q->B = b + (int64_t)k0 * es;
+                q->ldb = lB;
+                q->C = o + (int64_t)x0 * es;
+            } else {
+                /* out[:, x0..] = alpha B[:, k0:k1] op(T)[k0:k1, x0..] */
+                k0 = op_lower ? x0 : 0;
+                k1 = op_lower ? N : x0 + w;
+                q->transa = CblasNoTrans;
+                q->transb = trans;
+                q->M = M, q->N = w, q->K = k1 - k0;
+                q->A = b + (int64_t)k0 * lB * es;
+                q->lda = lB;
+                q->B = t + (trans == CblasNoTrans ? k0 + (int64_t)x0 * na : x0 + (int64_t)k0 * na) * es;
+                q->ldb = na;
+                q->C = o + (int64_t)x0 * M * es;
+            }
+            q->ldc = M;
+        }
+        rc = blasx_gemm_grouped_device(d, npan, g_bx.s_comp);
+    }
     if (!rc) rc = dev_store(B, ldb, dOut, M, M, N, es);
-    return finish(&s, rc);
+    rc = finish(&s, rc); /* the launch has consumed the descriptors */
+    free(d);
+    return rc;
 }
 
 static int gpu_syr2k(int dt, int uplo, int trans, int N, int K, const double *al, const void *A,
