@@ -355,9 +355,10 @@ def rank_k(bx, n=32768):
 
 
 def level3_inhbm(bx, n=32768):
-    """DSYMM (Left, Lower), DTRMM (Left, Lower, NoTrans, NonUnit) and DSYR2K
-    (Lower, NoTrans) through the BLAS ABI on operands in HBM, M = N (= K) = n.
-    FLOPs: SYMM 2M^2N, TRMM M^2N, SYR2K 2N(N+1)K."""
+    """DSYMM (Left, Lower), DTRMM (Left, Lower, NoTrans, NonUnit), DSYR2K
+    (Lower, NoTrans) and DTRSM (Left, Lower, NoTrans, Unit) through the BLAS
+    ABI on operands in HBM, M = N (= K) = n.  FLOPs: SYMM 2M^2N, TRMM and TRSM
+    M^2N, SYR2K 2N(N+1)K."""
     import ctypes as C
 
     import numpy as np
@@ -378,6 +379,10 @@ def level3_inhbm(bx, n=32768):
         "dsyr2k": (2.0 * n * (n + 1) * n, lambda: L.cblas_dsyr2k(102, 122, 111, n, n, D(1.0),
                                                                V(dA), n, V(dB), n, D(1.0),
                                                                V(dC), n)),
+        # the solve's values are not inspected (A is not conditioned for it); the
+        # time does not depend on them
+        "dtrsm": (float(n) ** 3, lambda: L.cblas_dtrsm(102, 141, 122, 111, 132, n, n, D(1.0),
+                                                      V(dA), n, V(dB), n)),
     }
     for name, (flops, fn) in calls.items():
         fn()  # warm-up

@@ -125,6 +125,14 @@ int bxh_fill_uniform(int dtype, void *p, int64_t rows, int64_t cols, int64_t ld,
 #define BX_FILL_TRI 2
 #define BX_FILL_UNIT 3
 int bxh_tri_fill(int dtype, int n, int lower, int mode, void *A, int64_t lda, void *stream);
+/* out (cols x rows) = in^T (in^H with conj, complex only) */
+int bxh_transpose(int dtype, int rows, int cols, const void *in, int64_t ldi, void *out, int64_t ldo,
+                  int conj, void *stream);
+/* solve T X = X in place for an m x m triangular T (lower or upper) with
+ * m <= bxh_trsm_leaf_nb(dtype) */
+int bxh_trsm_leaf_nb(int dtype);
+int bxh_trsm_leaf(int dtype, int lower, int m, int n, const void *T, int64_t ldt, void *X,
+                  int64_t ldx, void *stream);
 /* streaming device copy kernel (16-byte aligned, multiple of 16 bytes) */
 int bxh_copy_kernel(void *dst, const void *src, size_t bytes, void *stream);
 

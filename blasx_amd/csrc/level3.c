@@ -1,15 +1,17 @@
 /*
- * level3.c -- SYMM, HEMM, TRMM and SYR2K on the GPU, and their f77 forms:
+ * level3.c -- SYMM, HEMM, TRMM, TRSM and SYR2K on the GPU, and their f77 forms:
  *   cblas_{s,d,z}symm, cblas_zhemm  C = alpha A B + beta C (Side Left) or
  *                                   alpha B A + beta C (Right), A symmetric
  *                                   (Hermitian) given by one triangle
  *   cblas_{s,d,z}trmm               B = alpha op(A) B (Left) or alpha B op(A),
  *                                   A triangular, optionally unit diagonal
+ *   cblas_{s,d,z}trsm               B = alpha op(A)^-1 B (Left) or alpha B op(A)^-1
  *   cblas_{s,d,z}syr2k              C = alpha op(A) op(B)^T + alpha op(B) op(A)^T
  *                                   + beta C on one triangle
  *
  * Replaces blas/{s,d,z}symm.c:16-95, blas/zhemm.c:16-101,
- * blas/{s,d,z}trmm.c:17-90 and blas/{s,d,z}syr2k.c:16-110.  The reference
+ * blas/{s,d,z}trmm.c:17-90, blas/{s,d,z}trsm.c:16-101 and
+ * blas/{s,d,z}syr2k.c:16-110.  The reference
  * validates (blas/dsymm.c:29-36, blas/dtrmm.c:32-40, blas/dsyr2k.c:43-49) and
  * then always forwards to the CPU BLAS: both branches of its dispatch choose
  * the CPU (blas/dsymm.c:44-47, blas/dtrmm.c:48-51, blas/dsyr2k.c:57-60).
@@ -22,6 +24,11 @@
  *   FLOPs).  TRMM materialises the triangular matrix the same way and runs
  *   one grouped launch of panel GEMMs, each over the K range where op(A) is
  *   nonzero (half a full GEMM's work), into an output then copied over B.
+ *   TRSM solves op(A) X = alpha B recursively: op(A) materialised (transposed
+ *   when asked), the triangle split in halves, one half solved, the other
+ *   updated by one GEMM over the solved half, down to 64-row (32 complex)
+ *   diagonal blocks solved by substitution (bxh_trsm_leaf).  Side Right runs
+ *   as the Left solve of the transposed problem.
  *   SYR2K is the rank-k grouped launch with two K segments per problem
  *   (bx_device_syrk).
  * Host operands are copied whole to HBM for the call: these routines run in
@@ -41,7 +48,7 @@
 
 /* ------------------------------------------------------------ device work */
 typedef struct {
-    void *buf[4];
+    void *buf[6];
     int n;
 } scratch_t;
 
@@ -206,6 +213,74 @@ static int gpu_trmm(int dt, int side, int uplo, int trans, int diag, int M, int 
     rc = finish(&s, rc); /* the launch has consumed the descriptors */
     free(d);
     return rc;
+}
+
+/* X = T^-1 X in place (m x n), T triangular m x m: split T in halves at a
+ * multiple of the leaf size, solve one half, update the other with one GEMM
+ * (K = the solved half), recurse; leaves are substitution kernels */
+static int trsm_rec(int dt, int lower, const char *T, int64_t ldt, char *X, int64_t ldx, int m,
+                    int n, int es)
+{
+    const int nb = bxh_trsm_leaf_nb(dt);
+    if (m <= nb) return bxh_trsm_leaf(dt, lower, m, n, T, ldt, X, ldx, g_bx.s_comp);
+    const int m1 = (m / 2 + nb - 1) / nb * nb, m2 = m - m1;
+    const double minus_one[2] = {-1.0, 0.0}, one[2] = {1.0, 0.0};
+    const char *T22 = T + (m1 + (int64_t)m1 * ldt) * es;
+    char *X2 = X + (int64_t)m1 * es;
+    int rc;
+    if (lower) {
+        /* [T11 0; T21 T22] [X1; X2]: X1 first, then X2 -= T21 X1 */
+        if ((rc = trsm_rec(dt, 1, T, ldt, X, ldx, m1, n, es))) return rc;
+        if ((rc = bx_device_gemm(dt, CblasNoTrans, CblasNoTrans, m2, n, m1, minus_one,
+                                 T + (int64_t)m1 * es, ldt, X, ldx, one, X2, ldx, g_bx.s_comp)))
+            return rc;
+        return trsm_rec(dt, 1, T22, ldt, X2, ldx, m2, n, es);
+    }
+    /* [T11 T12; 0 T22]: X2 first, then X1 -= T12 X2 */
+    if ((rc = trsm_rec(dt, 0, T22, ldt, X2, ldx, m2, n, es))) return rc;
+    if ((rc = bx_device_gemm(dt, CblasNoTrans, CblasNoTrans, m1, n, m2, minus_one,
+                             T + (int64_t)m1 * ldt * es, ldt, X2, ldx, one, X, ldx, g_bx.s_comp)))
+        return rc;
+    return trsm_rec(dt, 0, T, ldt, X, ldx, m1, n, es);
+}
+
+static int gpu_trsm(int dt, int side, int uplo, int trans, int diag, int M, int N,
+                    const double *al, const void *A, int lda, void *B, int ldb)
+{
+    const int es = bx_esize(dt), na = side == CblasLeft ? M : N;
+    const int alpha_one = al[0] == 1.0 && al[1] == 0.0;
+    scratch_t s = {{0}, 0};
+    void *dA, *dT, *dB, *dX;
+    int64_t lB;
+    pthread_mutex_lock(&g_bx.gpu_lock);
+    /* T = op(A), dense and triangular (lower when lowerT) */
+    int rc = dev_copy(&s, A, na, lda, es, &dA);
+    int lowerT = uplo == CblasLower;
+    dT = dA;
+    if (!rc && trans != CblasNoTrans) {
+        rc = scratch_alloc(&s, &dT, (size_t)na * na * es);
+        if (!rc) rc = bxh_transpose(dt, na, na, dA, na, dT, na, trans == CblasConjTrans, g_bx.s_comp);
+        lowerT = !lowerT;
+    }
+    if (!rc && diag == CblasUnit) rc = bxh_tri_fill(dt, na, lowerT, BX_FILL_UNIT, dT, na, g_bx.s_comp);
+    if (!rc) rc = dev_view(&s, B, M, N, ldb, es, 1, &dB, &lB);
+    if (!rc && side == CblasLeft) {
+        /* op(A) X = alpha B, solved in place */
+        if (!alpha_one) rc = bxh_scale(dt, M, N, al, dB, lB, g_bx.s_comp);
+        if (!rc) rc = trsm_rec(dt, lowerT, dT, na, dB, lB, M, N, es);
+    } else if (!rc) {
+        /* X op(A) = alpha B  <=>  op(A)^T X^T = alpha B^T: solve on B^T with T^T */
+        void *dTt;
+        rc = scratch_alloc(&s, &dTt, (size_t)na * na * es);
+        if (!rc) rc = bxh_transpose(dt, na, na, dT, na, dTt, na, 0, g_bx.s_comp);
+        if (!rc) rc = scratch_alloc(&s, &dX, (size_t)N * M * es);
+        if (!rc) rc = bxh_transpose(dt, M, N, dB, lB, dX, N, 0, g_bx.s_comp);
+        if (!rc && !alpha_one) rc = bxh_scale(dt, N, M, al, dX, N, g_bx.s_comp);
+        if (!rc) rc = trsm_rec(dt, !lowerT, dTt, na, dX, N, N, M, es);
+        if (!rc) rc = bxh_transpose(dt, N, M, dX, N, dB, lB, 0, g_bx.s_comp);
+    }
+    if (!rc) rc = dev_store(B, ldb, dB, lB, M, N, es);
+    return finish(&s, rc);
 }
 
 static int gpu_syr2k(int dt, int uplo, int trans, int N, int K, const double *al, const void *A,
@@ -386,6 +461,54 @@ static void trmm_entry(int dt, int order, int Side, int Uplo, int TransA, int Di
         bx_cpu_ztrmm(order, Side, Uplo, TransA, Diag, M, N, alpha, A, lda, B, ldb);
 }
 
+/* ---- TRSM ---- */
+static void trsm_entry(int dt, int order, int Side, int Uplo, int TransA, int Diag, int M, int N,
+                       const void *alpha, const void *A, int lda, void *B, int ldb)
+{
+    const char *name = dt == BLASX_S ? "STRSM " : dt == BLASX_D ? "DTRSM " : "ZTRSM ";
+    int side = Side, uplo = Uplo, m = M, n = N, info = 0;
+    if (order == CblasRowMajor) {
+        /* the same mapping as TRMM: B^T op(A)^T or op(A)^T B^T on the
+         * column-major view, the other side and triangle */
+        side = flip_side(Side);
+        uplo = flip_uplo(Uplo);
+        m = N;
+        n = M;
+    } else if (order != CblasColMajor) {
+        xerbla_(name, &info);
+        return;
+    }
+    const int nrowa = side == CblasLeft ? m : n;
+    if (!side_ok(side)) info = 1;
+    else if (!uplo_ok(uplo)) info = 2;
+    else if (TransA != CblasNoTrans && TransA != CblasTrans && TransA != CblasConjTrans) info = 3;
+    else if (Diag != CblasNonUnit && Diag != CblasUnit) info = 4;
+    else if (m < 0) info = 5;
+    else if (n < 0) info = 6;
+    else if (lda < max1(nrowa)) info = 9;
+    else if (ldb < max1(m)) info = 11;
+    if (info) {
+        xerbla_(name, &info);
+        return;
+    }
+    if (m == 0 || n == 0) return;
+    double al[2];
+    scalar2(dt, alpha, al);
+    if (gpu_sized(dt, m, n)) {
+        int rc = bx_ensure_init();
+        if (!rc) rc = gpu_trsm(dt, side, uplo, TransA, Diag, m, n, al, A, lda, B, ldb);
+        if (!gpu_outcome(name, rc)) return;
+    }
+    need_cpublas(name);
+    if (dt == BLASX_S)
+        bx_cpu_strsm(order, Side, Uplo, TransA, Diag, M, N, *(const float *)alpha, A, lda, B, ldb);
+    else if (dt == BLASX_D)
+        bx_cpu_dtrsm(order, Side, Uplo, TransA, Diag, M, N, *(const double *)alpha, A, lda, B,
+                     ldb);
+    else
+        bx_cpu_ztrsm(order, Side, Uplo, TransA, Diag, M, N, alpha, A, lda, B, ldb);
+}
+
 /* ---- SYR2K ---- */
 static int syr2k_trans_ok(int dt, int t)
 {
@@ -499,6 +622,31 @@ BX_EXPORT void cblas_ztrmm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE S
     trmm_entry(BLASX_Z, Order, Side, Uplo, TransA, Diag, M, N, alpha, A, lda, B, ldb);
 }
 
+BX_EXPORT void cblas_strsm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                           const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TransA,
+                           const enum CBLAS_DIAG Diag, const int M, const int N, const float alpha,
+                           const float *A, const int lda, float *B, const int ldb)
+{
+    trsm_entry(BLASX_S, Order, Side, Uplo, TransA, Diag, M, N, &alpha, A, lda, B, ldb);
+}
+
+BX_EXPORT void cblas_dtrsm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                           const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TransA,
+                           const enum CBLAS_DIAG Diag, const int M, const int N,
+                           const double alpha, const double *A, const int lda, double *B,
+                           const int ldb)
+{
+    trsm_entry(BLASX_D, Order, Side, Uplo, TransA, Diag, M, N, &alpha, A, lda, B, ldb);
+}
+
+BX_EXPORT void cblas_ztrsm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                           const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TransA,
+                           const enum CBLAS_DIAG Diag, const int M, const int N, const void *alpha,
+                           const void *A, const int lda, void *B, const int ldb)
+{
+    trsm_entry(BLASX_Z, Order, Side, Uplo, TransA, Diag, M, N, alpha, A, lda, B, ldb);
+}
+
 BX_EXPORT void cblas_ssyr2k(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
                             const enum CBLAS_TRANSPOSE Trans, const int N, const int K,
                             const float alpha, const float *A, const int lda, const float *B,
@@ -600,6 +748,22 @@ F77_SYMM(zhemm_, BLASX_Z, 1, "ZHEMM", void)
 F77_TRMM(strmm_, BLASX_S, "STRMM ", float)
 F77_TRMM(dtrmm_, BLASX_D, "DTRMM ", double)
 F77_TRMM(ztrmm_, BLASX_Z, "ZTRMM ", void)
+
+#define F77_TRSM(NAME, DT, TNAME, T)                                                           \
+    BX_EXPORT void NAME(const char *side, const char *uplo, const char *transa,                \
+                        const char *diag, const int *m, const int *n, const T *alpha,           \
+                        const T *A, const int *lda, T *B, const int *ldb)                       \
+    {                                                                                          \
+        int s, u, t, d;                                                                        \
+        if (f77_side(side, &s) && f77_bad(TNAME, 1)) return;                                   \
+        if (f77_uplo(uplo, &u) && f77_bad(TNAME, 2)) return;                                   \
+        if (f77_trans(transa, &t) && f77_bad(TNAME, 3)) return;                                \
+        if (f77_diag(diag, &d) && f77_bad(TNAME, 4)) return;                                   \
+        trsm_entry(DT, CblasColMajor, s, u, t, d, *m, *n, alpha, A, *lda, B, *ldb);            \
+    }
+F77_TRSM(strsm_, BLASX_S, "STRSM ", float)
+F77_TRSM(dtrsm_, BLASX_D, "DTRSM ", double)
+F77_TRSM(ztrsm_, BLASX_Z, "ZTRSM ", void)
 
 #define F77_SYR2K(NAME, DT, TNAME, T)                                                          \
     BX_EXPORT void NAME(const char *uplo, const char *trans, const int *n, const int *k,        \

@@ -132,6 +132,20 @@ void cblas_ztrmm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
                  const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TransA,
                  const enum CBLAS_DIAG Diag, const int M, const int N, const void *alpha,
                  const void *A, const int lda, void *B, const int ldb);
+/* replace cblas_strsm blas/strsm.c:16, cblas_dtrsm blas/dtrsm.c:16,
+ * cblas_ztrsm blas/ztrsm.c:16 */
+void cblas_strsm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TransA,
+                 const enum CBLAS_DIAG Diag, const int M, const int N, const float alpha,
+                 const float *A, const int lda, float *B, const int ldb);
+void cblas_dtrsm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TransA,
+                 const enum CBLAS_DIAG Diag, const int M, const int N, const double alpha,
+                 const double *A, const int lda, double *B, const int ldb);
+void cblas_ztrsm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TransA,
+                 const enum CBLAS_DIAG Diag, const int M, const int N, const void *alpha,
+                 const void *A, const int lda, void *B, const int ldb);
 /* replace cblas_ssyr2k blas/ssyr2k.c:16, cblas_dsyr2k blas/dsyr2k.c:16,
  * cblas_zsyr2k blas/zsyr2k.c:16 */
 void cblas_ssyr2k(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
@@ -168,6 +182,16 @@ void dtrmm_(const char *side, const char *uplo, const char *transa, const char *
             const int *m, const int *n, const double *alpha, const double *A, const int *lda,
             double *B, const int *ldb);
 void ztrmm_(const char *side, const char *uplo, const char *transa, const char *diag,
+            const int *m, const int *n, const void *alpha, const void *A, const int *lda,
+            void *B, const int *ldb);
+/* replace strsm_ blas/strsm.c:78, dtrsm_ blas/dtrsm.c:78, ztrsm_ blas/ztrsm.c:78 */
+void strsm_(const char *side, const char *uplo, const char *transa, const char *diag,
+            const int *m, const int *n, const float *alpha, const float *A, const int *lda,
+            float *B, const int *ldb);
+void dtrsm_(const char *side, const char *uplo, const char *transa, const char *diag,
+            const int *m, const int *n, const double *alpha, const double *A, const int *lda,
+            double *B, const int *ldb);
+void ztrsm_(const char *side, const char *uplo, const char *transa, const char *diag,
             const int *m, const int *n, const void *alpha, const void *A, const int *lda,
             void *B, const int *ldb);
 /* replace ssyr2k_ blas/ssyr2k.c:88, dsyr2k_ blas/dsyr2k.c:88, zsyr2k_ blas/zsyr2k.c:88 */

@@ -70,7 +70,7 @@ def call(L, entry, routine, dt, *args):
 
 # argument kinds after Order: e enum, i int, s scalar, m matrix
 KINDS = {"symm": "eeiismimismi", "hemm": "eeiismimismi",
-         "trmm": "eeeeiismimi", "syr2k": "eeiismimismi"}
+         "trmm": "eeeeiismimi", "trsm": "eeeeiismimi", "syr2k": "eeiismimismi"}
 
 
 def full_sym(A, uplo, herm):
@@ -112,6 +112,32 @@ def ref_trmm(orc, side, uplo, trans, diag, M, N, alpha, A, B0):
     else:
         orc.gemm("N", t, M, N, N, alpha, B0, M, T, N, 0.0, out, M)
     return out
+
+
+def trsm_residual(orc, side, uplo, trans, diag, M, N, alpha, A, X, B0):
+    """backward error of a TRSM result X: ||op(T) X - alpha B|| (Left) or
+    ||X op(T) - alpha B|| (Right) over ||op(T)|| ||X|| + |alpha| ||B||,
+    products by the oracle's naive GEMM"""
+    T = full_tri(A, uplo, diag)
+    t = {NOTR: "N", TR: "T", CTR: "C"}[trans]
+    if X.dtype.kind != "c" and t == "C":
+        t = "T"
+    R = np.asfortranarray(B0.copy())
+    if side == LEFT:
+        orc.gemm(t, "N", M, N, M, 1.0, T, M, X, M, -alpha, R, M)
+    else:
+        orc.gemm("N", t, M, N, N, 1.0, X, M, T, N, -alpha, R, M)
+    den = np.linalg.norm(T.ravel()) * np.linalg.norm(X.ravel()) + \
+        abs(alpha) * np.linalg.norm(B0.ravel())
+    return np.linalg.norm(R.ravel()) / den
+
+
+def trsm_operand(A, na):
+    """a well-conditioned triangular operand from uniform entries: the
+    off-diagonal scaled by 1/n, the diagonal moved near 2"""
+    T = np.asfortranarray(A / na)
+    T[np.arange(na), np.arange(na)] += 2.0
+    return T
 
 
 def ref_syr2k(orc, uplo, trans, N, K, alpha, A, B, beta, C0):

@@ -14,7 +14,8 @@ import numpy as np
 import pytest
 
 from level3_common import (COL, CTR, LEFT, LO, NONUNIT, NOTR, RIGHT, ROW, TR, UNIT, UP, call,
-                           full_sym, full_tri, ref_symm, ref_syr2k, ref_trmm)
+                           full_sym, full_tri, ref_symm, ref_syr2k, ref_trmm, trsm_operand,
+                           trsm_residual)
 
 pytestmark = pytest.mark.gpu
 
@@ -217,3 +218,62 @@ def test_gpu_level3_exact_integers(gpu, orc):
     assert np.array_equal(dC.download(), ref)
     for d in (dA, dB, dC):
         d.free()
+
+
+@pytest.mark.parametrize("dt", TRMM_TYPES)
+@pytest.mark.parametrize("where", ["host", "device"])
+def test_gpu_trsm(gpu, orc, dt, where):
+    """backward error of the solve, c n eps, on a well-conditioned operand"""
+    bx = gpu
+    L = bx.lib()
+    M, N = _sizes(dt)
+    alpha = _sc(dt, 1.25, 0.75)
+    transes = (NOTR, TR, CTR) if dt != np.float32 else (NOTR, TR)
+    for side in (LEFT, RIGHT):
+        for uplo in (UP, LO):
+            for trans in transes:
+                for diag in (NONUNIT, UNIT):
+                    say("trsm", np.dtype(dt).name, where, side, uplo, trans, diag)
+                    na = M if side == LEFT else N
+                    A0, B0 = _mats(orc, dt, 8, (na, na), (M, N))
+                    A = trsm_operand(A0, na)
+                    if where == "host":
+                        X = B0.copy(order="F")
+                        call(L, "cblas", PFX[dt] + "trsm", dt, COL, side, uplo, trans, diag, M, N,
+                             alpha, A, na, X, M)
+                    else:
+                        dA, dB = _dev(bx, A), _dev(bx, B0)
+                        call(L, "cblas", PFX[dt] + "trsm", dt, COL, side, uplo, trans, diag, M,
+                             N, alpha, dA.ptr, na, dB.ptr, M)
+                        X = dB.download()
+                        dA.free()
+                        dB.free()
+                    err = trsm_residual(orc, side, uplo, trans, diag, M, N, alpha, A, X, B0)
+                    tol = (2 if np.dtype(dt).kind == "c" else 1) * na * EPS[dt]
+                    assert err <= tol, f"trsm backward error {err:.3e} > {tol:.3e}"
+
+
+def test_gpu_trsm_exact_integers_and_row_major(gpu, orc):
+    """unit triangular integer T and integer X: B = T X is exact, so the solve
+    must give X back bit for bit (ragged leaves, both triangles, both sides,
+    RowMajor)"""
+    bx = gpu
+    L = bx.lib()
+    rng = np.random.default_rng(12)
+    M, N = 1100, 96
+    for side in (LEFT, RIGHT):
+        for uplo in (LO, UP):
+            na = M if side == LEFT else N
+            A = np.asfortranarray(rng.integers(-3, 4, (na, na)).astype(np.float64))
+            Xi = np.asfortranarray(rng.integers(-8, 9, (M, N)).astype(np.float64))
+            T = full_tri(A, uplo, UNIT)
+            B = np.asfortranarray(T @ Xi if side == LEFT else Xi @ T)
+            say("exact dtrsm", side, uplo)
+            X = B.copy(order="F")
+            call(L, "cblas", "dtrsm", np.float64, COL, side, uplo, NOTR, UNIT, M, N, 1.0, A, na,
+                 X, M)
+            assert np.array_equal(X, Xi)
+            X = B.copy(order="F")
+            call(L, "cblas", "dtrsm", np.float64, ROW, RIGHT if side == LEFT else LEFT,
+                 UP if uplo == LO else LO, NOTR, UNIT, N, M, 1.0, A, na, X, M)
+            assert np.array_equal(X, Xi), "row-major"

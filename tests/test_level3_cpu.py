@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from level3_common import (COL, CTR, LEFT, LO, NONUNIT, NOTR, RIGHT, ROW, TR, UNIT, UP, call,
-                           ref_symm, ref_syr2k, ref_trmm)
+                           ref_symm, ref_syr2k, ref_trmm, trsm_operand, trsm_residual)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -75,6 +75,14 @@ ERRS = [
      "DSYMM ", 2),
     ("call(L, 'f77', 'dtrmm', np.float64, LEFT, UP, NOTR, 0, 2, 2, 1.0, A, 2, B, 2)",
      "DTRMM ", 4),
+    ("call(L, 'cblas', 'dtrsm', np.float64, COL, 0, UP, NOTR, UNIT, 2, 2, 1.0, A, 2, B, 2)",
+     "DTRSM ", 1),
+    ("call(L, 'cblas', 'ztrsm', np.complex128, COL, LEFT, UP, NOTR, UNIT, -1, 2, 1.0, A, 2, B, 2)",
+     "ZTRSM ", 5),
+    ("call(L, 'cblas', 'dtrsm', np.float64, COL, RIGHT, UP, NOTR, UNIT, 2, 3, 1.0, A, 2, B, 2)",
+     "DTRSM ", 9),
+    ("call(L, 'f77', 'strsm', np.float32, LEFT, UP, 0, UNIT, 2, 2, 1.0, A, 2, B, 2)",
+     "STRSM ", 3),
 ]
 
 
@@ -167,3 +175,31 @@ def test_syr2k_cpu_sized(bx, orc, dt, uplo, trans):
                  TR if trans == NOTR else NOTR, N, K, alpha, A, lda, B, lda, beta, out, N)
         assert np.max(np.abs(out[m] - ref[m])) <= _tol(dt) * 8, entry
         assert np.array_equal(out[~m].view(np.uint8), C0[~m].view(np.uint8)), entry
+
+
+@pytest.mark.parametrize("dt", DTS)
+@pytest.mark.parametrize("side", [LEFT, RIGHT])
+@pytest.mark.parametrize("uplo", [UP, LO])
+def test_trsm_cpu_sized(bx, orc, dt, side, uplo):
+    rng = np.random.default_rng(6)
+    M, N = 19, 13
+    na = M if side == LEFT else N
+    A = trsm_operand(_rnd(rng, dt, na, na), na)
+    B0 = _rnd(rng, dt, M, N)
+    alpha = _scal(dt, 1.5, 0.5)
+    L = bx.lib()
+    for trans in (NOTR, TR, CTR):
+        for diag in (NONUNIT, UNIT):
+            for entry in ("cblas", "f77", "row"):
+                X = B0.copy(order="F")
+                if entry == "cblas":
+                    call(L, "cblas", PFX[dt] + "trsm", dt, COL, side, uplo, trans, diag, M, N,
+                         alpha, A, na, X, M)
+                elif entry == "f77":
+                    call(L, "f77", PFX[dt] + "trsm", dt, side, uplo, trans, diag, M, N, alpha, A,
+                         na, X, M)
+                else:
+                    call(L, "cblas", PFX[dt] + "trsm", dt, ROW, RIGHT if side == LEFT else LEFT,
+                         LO if uplo == UP else UP, trans, diag, N, M, alpha, A, na, X, M)
+                err = trsm_residual(orc, side, uplo, trans, diag, M, N, alpha, A, X, B0)
+                assert err <= _tol(dt), (trans, diag, entry, err)
