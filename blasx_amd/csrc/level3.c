@@ -10,7 +10,7 @@
  *                                   + beta C on one triangle
  *
  * Replaces blas/{s,d,z}symm.c:16-95, blas/zhemm.c:16-101,
- * blas/{s,d,z}trmm.c:17-90, blas/{s,d,z}trsm.c:16-101 and
+ * blas/{s,d,z}trmm.c:17-90, blas/{s,d,z}trsm.c:16-100 and
  * blas/{s,d,z}syr2k.c:16-110.  The reference
  * validates (blas/dsymm.c:29-36, blas/dtrmm.c:32-40, blas/dsyr2k.c:43-49) and
  * then always forwards to the CPU BLAS: both branches of its dispatch choose
