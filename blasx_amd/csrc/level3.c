@@ -8,10 +8,12 @@
  *   cblas_{s,d,z}trsm               B = alpha op(A)^-1 B (Left) or alpha B op(A)^-1
  *   cblas_{s,d,z}syr2k              C = alpha op(A) op(B)^T + alpha op(B) op(A)^T
  *                                   + beta C on one triangle
+ *   cblas_zher2k                    the Hermitian form: op(B)^H, conj(alpha) on the
+ *                                   second term, beta real
  *
  * Replaces blas/{s,d,z}symm.c:16-95, blas/zhemm.c:16-101,
  * blas/{s,d,z}trmm.c:17-90, blas/{s,d,z}trsm.c:16-100 and
- * blas/{s,d,z}syr2k.c:16-110.  The reference
+ * blas/{s,d,z}syr2k.c:16-110 and blas/zher2k.c:15-118.  The reference
  * validates (blas/dsymm.c:29-36, blas/dtrmm.c:32-40, blas/dsyr2k.c:43-49) and
  * then always forwards to the CPU BLAS: both branches of its dispatch choose
  * the CPU (blas/dsymm.c:44-47, blas/dtrmm.c:48-51, blas/dsyr2k.c:57-60).
@@ -30,13 +32,15 @@
  *   diagonal blocks solved by substitution (bxh_trsm_leaf).  Side Right runs
  *   as the Left solve of the transposed problem.
  *   SYR2K is the rank-k grouped launch with two K segments per problem
- *   (bx_device_syrk).
+ *   (bx_device_syrk); HER2K the same with a rescaled copy of B in the
+ *   second term.
  * Host operands are copied whole to HBM for the call: these routines run in
  * core, and a call whose operands do not fit goes to the CPU BLAS.  Operands
  * already in HBM are used in place.
  *
  * Deviations (netlib semantics): RowMajor is computed; ZSYR2K rejects
- * ConjTrans (info 2); f77 argument errors report the lowest parameter number.
+ * ConjTrans (info 2); ZHER2K reports itself as "ZHER2K"; f77 argument errors
+ * report the lowest parameter number.
  */
 #include <pthread.h>
 #include <stdlib.h>
@@ -304,6 +308,44 @@ static int gpu_syr2k(int dt, int uplo, int trans, int N, int K, const double *al
     return finish(&s, rc);
 }
 
+/* HER2K: C = alpha op(A) op(B)^H + conj(alpha) op(B) op(A)^H + beta C, beta
+ * real, the diagonal real.  The grouped problem has one alpha, so the second
+ * term reads a copy of B scaled by s, |s| = 1: conj(alpha)/alpha for NoTrans
+ * (alpha (sB) A^H = conj(alpha) B A^H) and alpha/conj(alpha) for ConjTrans
+ * (alpha (sB)^H A = conj(alpha) B^H A). */
+static int gpu_her2k(int uplo, int trans, int N, int K, const double *al, const void *A, int lda,
+                     const void *B, int ldb, double beta, void *C, int ldc)
+{
+    const int es = 16;
+    const int rows = trans == CblasNoTrans ? N : K, cols = trans == CblasNoTrans ? K : N;
+    const double be[2] = {beta, 0.0};
+    scratch_t s = {{0}, 0};
+    void *dA, *dB, *dB2 = NULL, *dC;
+    int64_t lA, lB, lC;
+    pthread_mutex_lock(&g_bx.gpu_lock);
+    int rc = dev_view(&s, A, rows, cols, lda, es, 1, &dA, &lA);
+    if (!rc) rc = dev_view(&s, B, rows, cols, ldb, es, 1, &dB, &lB);
+    const int product = K > 0 && (al[0] != 0.0 || al[1] != 0.0);
+    if (!rc && product) {
+        const double m2 = al[0] * al[0] + al[1] * al[1];
+        /* conj(a)/a = conj(a)^2/|a|^2; a/conj(a) = a^2/|a|^2 */
+        const double re = (al[0] * al[0] - al[1] * al[1]) / m2, im = 2.0 * al[0] * al[1] / m2;
+        const double sc[2] = {re, trans == CblasNoTrans ? -im : im};
+        rc = scratch_alloc(&s, &dB2, (size_t)rows * cols * es);
+        if (!rc)
+            rc = bxh_memcpy2d_d2d_async(dB2, (size_t)rows * es, dB, (size_t)lB * es,
+                                        (size_t)rows * es, (size_t)cols, g_bx.s_comp);
+        if (!rc) rc = bxh_scale(BLASX_Z, rows, cols, sc, dB2, rows, g_bx.s_comp);
+    }
+    if (!rc) rc = dev_view(&s, C, N, N, ldc, es, 1, &dC, &lC);
+    if (!rc)
+        rc = bx_device_syr2k(BLASX_Z, (uplo == CblasLower ? BX_TRI_LOWER : BX_TRI_UPPER) | BX_TRI_HERM,
+                             trans == CblasNoTrans ? BX_OP_N : BX_OP_C, N, K, al, dA, lA, dB, lB,
+                             dB2 ? dB2 : dB, dB2 ? rows : lB, be, dC, lC, g_bx.s_comp);
+    if (!rc) rc = dev_store(C, ldc, dC, lC, N, N, es);
+    return finish(&s, rc);
+}
+
 /* ------------------------------------------------------------------ ABI */
 static void scalar2(int dt, const void *v, double out[2])
 {
@@ -564,6 +606,53 @@ static void syr2k_entry(int dt, int order, int Uplo, int Trans, int N, int K, co
         bx_cpu_zsyr2k(order, Uplo, Trans, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
+/* ---- HER2K ---- */
+static void her2k_entry(int order, int Uplo, int Trans, int N, int K, const void *alpha,
+                        const void *A, int lda, const void *B, int ldb, double beta, void *C,
+                        int ldc)
+{
+    /* netlib's name; the reference's wrapper reports "ZHERK" (blas/zher2k.c:13) */
+    const char *name = "ZHER2K";
+    int uplo = Uplo, trans = Trans, info = 0;
+    double al[2];
+    scalar2(BLASX_Z, alpha, al);
+    if (order == CblasRowMajor) {
+        /* C^T = conj(alpha) B'^H A' + alpha A'^H B' on the column-major views
+         * A' = A^T, B' = B^T: the other triangle, the other transposition and
+         * the conjugate alpha */
+        uplo = flip_uplo(Uplo);
+        if (Trans == CblasNoTrans || Trans == CblasConjTrans)
+            trans = Trans == CblasNoTrans ? CblasConjTrans : CblasNoTrans;
+        al[1] = -al[1];
+    } else if (order != CblasColMajor) {
+        xerbla_(name, &info);
+        return;
+    }
+    const int nrowa = trans == CblasNoTrans ? N : K;
+    if (!uplo_ok(uplo)) info = 1;
+    else if (trans != CblasNoTrans && trans != CblasConjTrans) info = 2;
+    else if (N < 0) info = 3;
+    else if (K < 0) info = 4;
+    else if (lda < max1(nrowa)) info = 7;
+    else if (ldb < max1(nrowa)) info = 9;
+    else if (ldc < max1(N)) info = 12;
+    if (info) {
+        xerbla_(name, &info);
+        return;
+    }
+    if (N == 0) return;
+    const int no_product = K == 0 || (al[0] == 0.0 && al[1] == 0.0);
+    if (no_product && beta == 1.0) return;
+    const int in_hbm = bxh_pointer_is_pinned(C) == 2;
+    if (gpu_sized(BLASX_Z, N, K) && (!no_product || in_hbm)) {
+        int rc = bx_ensure_init();
+        if (!rc) rc = gpu_her2k(uplo, trans, N, K, al, A, lda, B, ldb, beta, C, ldc);
+        if (!gpu_outcome(name, rc)) return;
+    }
+    need_cpublas(name);
+    bx_cpu_zher2k(order, Uplo, Trans, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
 /* ------------------------------------------------------------ CBLAS */
 BX_EXPORT void cblas_ssymm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
                            const enum CBLAS_UPLO Uplo, const int M, const int N, const float alpha,
@@ -645,6 +734,14 @@ BX_EXPORT void cblas_ztrsm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE S
                            const void *A, const int lda, void *B, const int ldb)
 {
     trsm_entry(BLASX_Z, Order, Side, Uplo, TransA, Diag, M, N, alpha, A, lda, B, ldb);
+}
+
+BX_EXPORT void cblas_zher2k(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                            const enum CBLAS_TRANSPOSE Trans, const int N, const int K,
+                            const void *alpha, const void *A, const int lda, const void *B,
+                            const int ldb, const double beta, void *C, const int ldc)
+{
+    her2k_entry(Order, Uplo, Trans, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
 BX_EXPORT void cblas_ssyr2k(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
@@ -775,6 +872,16 @@ F77_TRSM(ztrsm_, BLASX_Z, "ZTRSM ", void)
         if (f77_trans(trans, &t) && f77_bad(TNAME, 2)) return;                                 \
         syr2k_entry(DT, CblasColMajor, u, t, *n, *k, alpha, A, *lda, B, *ldb, beta, C, *ldc);  \
     }
+BX_EXPORT void zher2k_(const char *uplo, const char *trans, const int *n, const int *k,
+                       const void *alpha, const void *A, const int *lda, const void *B,
+                       const int *ldb, const double *beta, void *C, const int *ldc)
+{
+    int u, t;
+    if (f77_uplo(uplo, &u) && f77_bad("ZHER2K", 1)) return;
+    if (f77_trans(trans, &t) && f77_bad("ZHER2K", 2)) return;
+    her2k_entry(CblasColMajor, u, t, *n, *k, alpha, A, *lda, B, *ldb, *beta, C, *ldc);
+}
+
 F77_SYR2K(ssyr2k_, BLASX_S, "SSYR2K", float)
 F77_SYR2K(dsyr2k_, BLASX_D, "DSYR2K", double)
 F77_SYR2K(zsyr2k_, BLASX_Z, "ZSYR2K", void)

@@ -77,6 +77,16 @@ int bx_device_syrk(int dt, int tri, int opa, int N, int K, const double *alpha, 
                    int64_t lda, const void *B, int64_t ldb, const double *beta, void *C,
                    int64_t ldc, void *stream)
 {
+    return bx_device_syr2k(dt, tri, opa, N, K, alpha, A, lda, B, ldb, B, ldb, beta, C, ldc,
+                           stream);
+}
+
+/* the same with the second term's op(B) rows taken from B2 (HER2K passes a
+ * copy of B scaled so that one alpha serves both terms) */
+int bx_device_syr2k(int dt, int tri, int opa, int N, int K, const double *alpha, const void *A,
+                    int64_t lda, const void *B, int64_t ldb, const void *B2, int64_t ldb2,
+                    const double *beta, void *C, int64_t ldc, void *stream)
+{
     if (bx_ensure_init() != 0) return BLASX_ENOGPU;
     void *st = stream ? stream : g_bx.s_comp;
     if (N <= 0) return 0;
@@ -105,10 +115,12 @@ int bx_device_syrk(int dt, int tri, int opa, int N, int K, const double *alpha, 
         bxh_problem_t *probs = (bxh_problem_t *)g_sbuf.host;
         bxh_seg_t *segs = (bxh_seg_t *)((char *)g_sbuf.host + prob_bytes);
         const char *a = (const char *)A, *b = B ? (const char *)B : (const char *)A;
+        const char *b2 = B2 ? (const char *)B2 : b;
         char *c = (char *)C;
         /* row x of op(A) starts at element x of A (op N) or at column x (T, C) */
         const int64_t step = (opa == BX_OP_N ? 1 : lda) * es;
         const int64_t stepb = (opa == BX_OP_N ? 1 : ldb) * es;
+        const int64_t stepb2 = (opa == BX_OP_N ? 1 : ldb2) * es;
         int np = 0, total = 0;
         for (int r0 = 0; r0 < N; r0 += PANEL) {
             const int rows = N - r0 < PANEL ? N - r0 : PANEL;
@@ -131,8 +143,8 @@ int bx_device_syrk(int dt, int tri, int opa, int N, int K, const double *alpha, 
                 sg[0].B = B ? b + c0 * stepb : a + c0 * step;
                 sg[0].ldb = B ? ldb : lda;
                 sg[0].k = K;
-                sg[1].A = b + r0 * stepb;
-                sg[1].lda = ldb;
+                sg[1].A = b2 + r0 * stepb2;
+                sg[1].lda = B2 ? ldb2 : ldb;
                 sg[1].B = a + c0 * step;
                 sg[1].ldb = lda;
                 sg[1].k = K;

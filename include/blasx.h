@@ -160,6 +160,15 @@ void cblas_zsyr2k(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
                   const enum CBLAS_TRANSPOSE Trans, const int N, const int K, const void *alpha,
                   const void *A, const int lda, const void *B, const int ldb, const void *beta,
                   void *C, const int ldc);
+/* replaces cblas_zher2k, blas/zher2k.c:15 */
+void cblas_zher2k(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                  const enum CBLAS_TRANSPOSE Trans, const int N, const int K, const void *alpha,
+                  const void *A, const int lda, const void *B, const int ldb, const double beta,
+                  void *C, const int ldc);
+/* replaces zher2k_, blas/zher2k.c:89 */
+void zher2k_(const char *uplo, const char *trans, const int *n, const int *k, const void *alpha,
+             const void *A, const int *lda, const void *B, const int *ldb, const double *beta,
+             void *C, const int *ldc);
 /* replace ssymm_ blas/ssymm.c:74, dsymm_ blas/dsymm.c:74, zsymm_ blas/zsymm.c:74,
  * zhemm_ blas/zhemm.c:80 */
 void ssymm_(const char *side, const char *uplo, const int *m, const int *n, const float *alpha,

@@ -63,6 +63,10 @@ def call(L, entry, routine, dt, *args):
             r, x = _sc(dt, a, f77=entry == "f77")
             keep.append(x)
             out.append(r)
+        elif kind == "r":  # a real double scalar (HER2K's beta)
+            x = C.c_double(a)
+            keep.append(x)
+            out.append(C.byref(x) if entry == "f77" else x)
         else:
             out.append(_p(a))
     fn(*out)
@@ -70,7 +74,8 @@ def call(L, entry, routine, dt, *args):
 
 # argument kinds after Order: e enum, i int, s scalar, m matrix
 KINDS = {"symm": "eeiismimismi", "hemm": "eeiismimismi",
-         "trmm": "eeeeiismimi", "trsm": "eeeeiismimi", "syr2k": "eeiismimismi"}
+         "trmm": "eeeeiismimi", "trsm": "eeeeiismimi", "syr2k": "eeiismimismi",
+         "her2k": "eeiismimirmi"}
 
 
 def full_sym(A, uplo, herm):
@@ -149,4 +154,20 @@ def ref_syr2k(orc, uplo, trans, N, K, alpha, A, B, beta, C0):
     out = C0.copy(order="F")
     m = np.tril(np.ones((N, N), bool)) if uplo == LO else np.triu(np.ones((N, N), bool))
     out[m] = full[m]
+    return out, m
+
+
+def ref_her2k(orc, uplo, trans, N, K, alpha, A, B, beta, C0):
+    """alpha op(A) op(B)^H + conj(alpha) op(B) op(A)^H + beta C on the stored
+    triangle, the diagonal real"""
+    ta, tb = ("N", "C") if trans == NOTR else ("C", "N")
+    lda = A.shape[0]
+    full = np.asfortranarray(C0.copy())
+    orc.gemm(ta, tb, N, N, K, alpha, A, lda, B, lda, beta, full, N)
+    orc.gemm(ta, tb, N, N, K, np.conj(alpha), B, lda, A, lda, 1.0, full, N)
+    out = C0.copy(order="F")
+    m = np.tril(np.ones((N, N), bool)) if uplo == LO else np.triu(np.ones((N, N), bool))
+    out[m] = full[m]
+    d = np.arange(N)
+    out[d, d] = out[d, d].real
     return out, m

@@ -14,8 +14,8 @@ import numpy as np
 import pytest
 
 from level3_common import (COL, CTR, LEFT, LO, NONUNIT, NOTR, RIGHT, ROW, TR, UNIT, UP, call,
-                           full_sym, full_tri, ref_symm, ref_syr2k, ref_trmm, trsm_operand,
-                           trsm_residual)
+                           full_sym, full_tri, ref_her2k, ref_symm, ref_syr2k, ref_trmm,
+                           trsm_operand, trsm_residual)
 
 pytestmark = pytest.mark.gpu
 
@@ -277,3 +277,43 @@ def test_gpu_trsm_exact_integers_and_row_major(gpu, orc):
             call(L, "cblas", "dtrsm", np.float64, ROW, RIGHT if side == LEFT else LEFT,
                  UP if uplo == LO else LO, NOTR, UNIT, N, M, 1.0, A, na, X, M)
             assert np.array_equal(X, Xi), "row-major"
+
+
+@pytest.mark.parametrize("where", ["host", "device"])
+def test_gpu_her2k(gpu, orc, where):
+    bx = gpu
+    L = bx.lib()
+    dt = np.complex128
+    N, K = 600, 320
+    alpha, beta = 0.75 - 0.5j, 0.5
+    for uplo in (UP, LO):
+        for trans in (NOTR, CTR):
+            say("her2k", where, uplo, trans)
+            shape = (N, K) if trans == NOTR else (K, N)
+            A, B, C0 = _mats(orc, dt, 10, shape, shape, (N, N))
+            ref, m = ref_her2k(orc, uplo, trans, N, K, alpha, A, B, beta, C0)
+            lda = shape[0]
+            if where == "host":
+                out = C0.copy(order="F")
+                call(L, "cblas", "zher2k", dt, COL, uplo, trans, N, K, alpha, A, lda, B, lda,
+                     beta, out, N)
+            else:
+                dA, dB, dC = _dev(bx, A), _dev(bx, B), _dev(bx, C0)
+                call(L, "cblas", "zher2k", dt, COL, uplo, trans, N, K, alpha, dA.ptr, lda,
+                     dB.ptr, lda, beta, dC.ptr, N)
+                out = dC.download()
+                for d in (dA, dB, dC):
+                    d.free()
+            _check(out, ref, 2 * abs(alpha) * _fro(A) * _fro(B) + abs(beta) * _fro(C0), K, dt,
+                   "her2k", mask=m)
+            assert np.array_equal(out[~m].view(np.uint8), C0[~m].view(np.uint8))
+            d = np.arange(N)
+            assert not np.any(out[d, d].imag), "diagonal not real"
+    # RowMajor over the same bytes
+    A, B, C0 = _mats(orc, dt, 11, (N, K), (N, K), (N, N))
+    ref, m = ref_her2k(orc, LO, NOTR, N, K, alpha, A, B, beta, C0)
+    out = C0.copy(order="F")
+    say("her2k row-major")
+    call(L, "cblas", "zher2k", dt, ROW, UP, CTR, N, K, np.conj(alpha), A, N, B, N, beta, out, N)
+    _check(out, ref, 2 * abs(alpha) * _fro(A) * _fro(B) + abs(beta) * _fro(C0), K, dt,
+           "her2k row-major", mask=m)

@@ -11,7 +11,8 @@ import numpy as np
 import pytest
 
 from level3_common import (COL, CTR, LEFT, LO, NONUNIT, NOTR, RIGHT, ROW, TR, UNIT, UP, call,
-                           ref_symm, ref_syr2k, ref_trmm, trsm_operand, trsm_residual)
+                           ref_her2k, ref_symm, ref_syr2k, ref_trmm, trsm_operand,
+                           trsm_residual)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -83,6 +84,10 @@ ERRS = [
      "DTRSM ", 9),
     ("call(L, 'f77', 'strsm', np.float32, LEFT, UP, 0, UNIT, 2, 2, 1.0, A, 2, B, 2)",
      "STRSM ", 3),
+    ("call(L, 'cblas', 'zher2k', np.complex128, COL, UP, TR, 2, 2, 1.0, A, 2, B, 2, 0.0, Cm, 2)",
+     "ZHER2K", 2),
+    ("call(L, 'cblas', 'zher2k', np.complex128, COL, UP, NOTR, 3, 2, 1.0, A, 2, B, 3, 0.0, Cm, 3)",
+     "ZHER2K", 7),
 ]
 
 
@@ -203,3 +208,32 @@ def test_trsm_cpu_sized(bx, orc, dt, side, uplo):
                          LO if uplo == UP else UP, trans, diag, N, M, alpha, A, na, X, M)
                 err = trsm_residual(orc, side, uplo, trans, diag, M, N, alpha, A, X, B0)
                 assert err <= _tol(dt), (trans, diag, entry, err)
+
+
+@pytest.mark.parametrize("uplo", [UP, LO])
+@pytest.mark.parametrize("trans", [NOTR, CTR])
+def test_her2k_cpu_sized(bx, orc, uplo, trans):
+    rng = np.random.default_rng(7)
+    dt = np.complex128
+    N, K = 21, 15
+    shape = (N, K) if trans == NOTR else (K, N)
+    A, B, C0 = _rnd(rng, dt, *shape), _rnd(rng, dt, *shape), _rnd(rng, dt, N, N)
+    alpha, beta = 0.5 + 0.75j, -0.5
+    ref, m = ref_her2k(orc, uplo, trans, N, K, alpha, A, B, beta, C0)
+    L = bx.lib()
+    lda = shape[0]
+    for entry in ("cblas", "f77", "row"):
+        out = C0.copy(order="F")
+        if entry == "cblas":
+            call(L, "cblas", "zher2k", dt, COL, uplo, trans, N, K, alpha, A, lda, B, lda, beta,
+                 out, N)
+        elif entry == "f77":
+            call(L, "f77", "zher2k", dt, uplo, trans, N, K, alpha, A, lda, B, lda, beta, out, N)
+        else:
+            # the same bytes read row-major: the other triangle and transposition,
+            # conj(alpha) (C^T = conj(alpha) B'^H A' + alpha A'^H B')
+            call(L, "cblas", "zher2k", dt, ROW, LO if uplo == UP else UP,
+                 CTR if trans == NOTR else NOTR, N, K, np.conj(alpha), A, lda, B, lda, beta,
+                 out, N)
+        assert np.max(np.abs(out[m] - ref[m])) <= 1e-12 * 8, entry
+        assert np.array_equal(out[~m].view(np.uint8), C0[~m].view(np.uint8)), entry
