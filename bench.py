@@ -356,9 +356,9 @@ def rank_k(bx, n=32768):
 
 def level3_inhbm(bx, n=32768):
     """DSYMM (Left, Lower), DTRMM (Left, Lower, NoTrans, NonUnit), DSYR2K
-    (Lower, NoTrans) and DTRSM (Left, Lower, NoTrans, Unit) through the BLAS
-    ABI on operands in HBM, M = N (= K) = n.  FLOPs: SYMM 2M^2N, TRMM and TRSM
-    M^2N, SYR2K 2N(N+1)K."""
+    (Lower, NoTrans), DTRSM (Left, Lower, NoTrans, Unit) and CGEMM (NN)
+    through the BLAS ABI on operands in HBM, M = N (= K) = n.  FLOPs: SYMM
+    2M^2N, TRMM and TRSM M^2N, SYR2K 2N(N+1)K, CGEMM 8MNK."""
     import ctypes as C
 
     import numpy as np
@@ -395,6 +395,28 @@ def level3_inhbm(bx, n=32768):
         log(f"{name} in HBM n={n}: {best:.3f}s ({flops / best / 1e12:.2f} TFLOP/s)")
         out[name + "_tflops"] = round(flops / best / 1e12, 3)
     for m in mats:
+        m.free()
+    # CGEMM (complex64) on the FP32 matrix cores: 8 n^3 real FLOPs
+    cm = [bx.DeviceArray(n, n, np.complex64) for _ in range(3)]
+    for i, m in enumerate(cm):
+        # an n x n complex64 matrix is a 2n x n float32 matrix: (re, im) pairs
+        rc = L.blasx_fill_uniform_device(bx.S, m.ptr, 2 * n, n, 2 * n, 2, i, None)
+        if rc:
+            raise RuntimeError(f"blasx_fill_uniform_device failed ({rc})")
+    bx.synchronize()
+    al = (C.c_float * 2)(1.0, 0.0)
+    fn = lambda: L.cblas_cgemm(102, 111, 111, n, n, n, al, V(cm[0].ptr), n, V(cm[1].ptr), n,
+                               al, V(cm[2].ptr), n)
+    fn()
+    times = []
+    for _ in range(2):
+        t = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t)
+    best = min(times)
+    log(f"cgemm in HBM n={n}: {best:.3f}s ({8.0 * n ** 3 / best / 1e12:.2f} TFLOP/s)")
+    out["cgemm_tflops"] = round(8.0 * n ** 3 / best / 1e12, 3)
+    for m in cm:
         m.free()
     return out
 

@@ -7,9 +7,10 @@
  * reference: validation order and Fortran info numbers (blas/dgemm.c:46-53),
  * the xerbla_ message, the f77 trans-character mapping
  * (blas/blasx_internal.c:161-174), the size dispatch "any of M,N,K above
- * 1000 (S, D) / 500 (Z) goes to the GPU, else the CPU BLAS"
- * (blas/dgemm.c:61-64, blas/zgemm.c:59-62), CGEMM always on the CPU
- * (blas/cgemm.c:59-60), blocking semantics.
+ * 1000 (S, D) / 500 (Z, C) goes to the GPU, else the CPU BLAS"
+ * (blas/dgemm.c:61-64, blas/zgemm.c:59-62), blocking semantics.  CGEMM,
+ * which the reference keeps on the CPU (blas/cgemm.c:59-60), runs on the GPU
+ * too, as one real GEMM on the FP32 matrix cores (bx_gpu_cgemm, level3.c).
  *
  * Deliberate deviations (netlib/OpenBLAS CBLAS semantics, where the
  * reference's GPU path is wrong): RowMajor is computed as
@@ -131,7 +132,7 @@ static void gemm_entry(int dt, int order, int TransA, int TransB, int M, int N, 
 
     bx_config();
     const int thr = (dt == BLASX_Z || dt == BLASX_C) ? g_bx.min_dim_cplx : g_bx.min_dim_real;
-    const int gpu_sized = dt != BLASX_C && (m > thr || n > thr || K > thr);
+    const int gpu_sized = m > thr || n > thr || K > thr;
     if (!gpu_sized) {
         if (bx_cpu_gemm(dt, order, TransA, TransB, M, N, K, alpha, A, lda, B, ldb, beta, C,
                         ldc) != 0)
@@ -143,8 +144,18 @@ static void gemm_entry(int dt, int order, int TransA, int TransB, int M, int N, 
     int rc = bx_ensure_init();
     if (rc == 0) {
         const int opa = op_code(dt, ta), opb = op_code(dt, tb);
-        if (bxh_pointer_is_pinned(a) == 2 && bxh_pointer_is_pinned(b) == 2 &&
-            bxh_pointer_is_pinned(C) == 2) {
+        if (dt == BLASX_C) {
+            rc = bx_gpu_cgemm(ta, tb, m, n, K, al, a, la, b, lb, be, C, ldc);
+            if (rc == BLASX_ENOMEM) {
+                /* operands beyond HBM: the CPU BLAS, as for any routine that runs in core */
+                bx_log(1, "%s: operands do not fit in HBM; running on the CPU BLAS", name);
+                if (bx_cpu_gemm(dt, order, TransA, TransB, M, N, K, alpha, A, lda, B, ldb, beta,
+                                C, ldc) != 0)
+                    bx_fatal("%s: no CPU BLAS could be loaded (set BLASX_CPUBLAS)", name);
+                return;
+            }
+        } else if (bxh_pointer_is_pinned(a) == 2 && bxh_pointer_is_pinned(b) == 2 &&
+                   bxh_pointer_is_pinned(C) == 2) {
             /* operands already in HBM: run in place */
             rc = bx_device_gemm(dt, ta, tb, m, n, K, al, a, la, b, lb, be, C, ldc, NULL);
             if (rc == 0) rc = bxh_stream_sync(g_bx.s_comp);

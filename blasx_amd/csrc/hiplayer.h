@@ -133,6 +133,13 @@ int bxh_transpose(int dtype, int rows, int cols, const void *in, int64_t ldi, vo
 int bxh_trsm_leaf_nb(int dtype);
 int bxh_trsm_leaf(int dtype, int lower, int m, int n, const void *T, int64_t ldt, void *X,
                   int64_t ldx, void *stream);
+/* complex64 GEMM as one real GEMM: Ap = [Re op(A), Im op(A)] (M x 2K),
+ * Bp = [[Re, Im], [-Im, Re]] of op(B) (2K x 2N), op 0 N / 1 T / 2 C; then
+ * C = alpha (P[:, :N] + i P[:, N:]) + beta C (P NULL: C = beta C) */
+int bxh_cpack_a(int M, int K, int op, const void *A, int64_t lda, float *Ap, void *stream);
+int bxh_cpack_b(int K, int N, int op, const void *B, int64_t ldb, float *Bp, void *stream);
+int bxh_ccombine(int M, int N, const float *P, int64_t ldp, const double *alpha,
+                 const double *beta, void *C, int64_t ldc, void *stream);
 /* streaming device copy kernel (16-byte aligned, multiple of 16 bytes) */
 int bxh_copy_kernel(void *dst, const void *src, size_t bytes, void *stream);
 

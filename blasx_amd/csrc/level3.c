@@ -38,6 +38,9 @@
  * core, and a call whose operands do not fit goes to the CPU BLAS.  Operands
  * already in HBM are used in place.
  *
+ * CGEMM (complex64, the ABI in abi.c) also runs here: one real GEMM on the
+ * FP32 matrix cores over packed operands (bx_gpu_cgemm).
+ *
  * Deviations (netlib semantics): RowMajor is computed; ZSYR2K rejects
  * ConjTrans (info 2); ZHER2K reports itself as "ZHER2K"; f77 argument errors
  * report the lowest parameter number.
@@ -343,6 +346,46 @@ static int gpu_her2k(int uplo, int trans, int N, int K, const double *al, const 
                              trans == CblasNoTrans ? BX_OP_N : BX_OP_C, N, K, al, dA, lA, dB, lB,
                              dB2 ? dB2 : dB, dB2 ? rows : lB, be, dC, lC, g_bx.s_comp);
     if (!rc) rc = dev_store(C, ldc, dC, lC, N, N, es);
+    return finish(&s, rc);
+}
+
+/* CGEMM (complex64) on the FP32 matrix cores: one real GEMM of M x 2N x 2K
+ * on packed operands (bxh_cpack_a/b), then C = alpha P + beta C.  Operands
+ * and C as for the other routines here (in HBM in place, else copied); the
+ * transposes are CBLAS enums. */
+int bx_gpu_cgemm(int transa, int transb, int M, int N, int K, const double *al, const void *A,
+                 int lda, const void *B, int ldb, const double *be, void *C, int ldc)
+{
+    const int es = 8;
+    const int ra = transa == CblasNoTrans ? M : K, ca = transa == CblasNoTrans ? K : M;
+    const int rb = transb == CblasNoTrans ? K : N, cb = transb == CblasNoTrans ? N : K;
+    const int opa = transa == CblasNoTrans ? 0 : transa == CblasTrans ? 1 : 2;
+    const int opb = transb == CblasNoTrans ? 0 : transb == CblasTrans ? 1 : 2;
+    const int bz = be[0] == 0.0 && be[1] == 0.0;
+    const int product = K > 0 && (al[0] != 0.0 || al[1] != 0.0);
+    const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
+    scratch_t s = {{0}, 0};
+    void *dA = NULL, *dB = NULL, *dC, *Ap, *Bp, *P = NULL;
+    int64_t lA, lB, lC;
+    pthread_mutex_lock(&g_bx.gpu_lock);
+    int rc = 0;
+    if (product) {
+        rc = dev_view(&s, A, ra, ca, lda, es, 1, &dA, &lA);
+        if (!rc) rc = dev_view(&s, B, rb, cb, ldb, es, 1, &dB, &lB);
+    }
+    if (!rc) rc = dev_view(&s, C, M, N, ldc, es, !bz, &dC, &lC);
+    if (!rc && product) {
+        rc = scratch_alloc(&s, &Ap, (size_t)M * 2 * K * sizeof(float));
+        if (!rc) rc = scratch_alloc(&s, &Bp, (size_t)4 * K * N * sizeof(float));
+        if (!rc) rc = scratch_alloc(&s, &P, (size_t)M * 2 * N * sizeof(float));
+        if (!rc) rc = bxh_cpack_a(M, K, opa, dA, lA, (float *)Ap, g_bx.s_comp);
+        if (!rc) rc = bxh_cpack_b(K, N, opb, dB, lB, (float *)Bp, g_bx.s_comp);
+        if (!rc)
+            rc = bx_device_gemm(BLASX_S, CblasNoTrans, CblasNoTrans, M, 2 * N, 2 * K, one, Ap, M,
+                                Bp, 2 * (int64_t)K, zero, P, M, g_bx.s_comp);
+    }
+    if (!rc) rc = bxh_ccombine(M, N, (const float *)P, M, al, be, dC, lC, g_bx.s_comp);
+    if (!rc) rc = dev_store(C, ldc, dC, lC, M, N, es);
     return finish(&s, rc);
 }
 

@@ -423,3 +423,78 @@ def test_resident_operand_reused_across_calls(gpu, orc):
     finally:
         assert L.blasx_resident_unregister(A.ctypes.data) == 0
         L.blasx_cache_flush()
+
+
+# ------------------------------------------------------------------ CGEMM
+def _c64(orc, rows, cols, seed, mid):
+    return np.asfortranarray(orc.fill_uniform(rows, cols, seed, mid, dtype=np.complex128)
+                             .astype(np.complex64))
+
+
+@pytest.mark.parametrize("where", ["host", "device"])
+def test_abi_cgemm_on_fp32_matrix_cores(gpu, orc, where):
+    """complex64 through cblas_cgemm above the complex threshold: one real GEMM
+    on the FP32 matrix cores over packed operands, every transpose/conjugate
+    pair, against the oracle's naive complex GEMM on the same (exactly
+    widened) inputs; tolerance 2 K eps(float32)"""
+    bx = gpu
+    M, N, K = 600, 520, 300
+    alpha, beta = 1.5 - 0.5j, 0.5 + 0.25j
+    eps = np.finfo(np.float32).eps
+    for ta, tb in TRANS_CPLX:
+        say("cgemm", where, ta, tb, M, N, K)
+        A = _c64(orc, M if ta == "N" else K, K if ta == "N" else M, 3, 0)
+        B = _c64(orc, K if tb == "N" else N, N if tb == "N" else K, 3, 1)
+        C0 = _c64(orc, M, N, 3, 2)
+        ref = C0.astype(np.complex128).copy(order="F")
+        orc.gemm(ta, tb, M, N, K, alpha, A.astype(np.complex128), A.shape[0],
+                 B.astype(np.complex128), B.shape[0], beta, ref, M)
+        bx.reset_stats()
+        if where == "host":
+            out = C0.copy(order="F")
+            bx.cblas_gemm(bx.COL_MAJOR, ta, tb, M, N, K, alpha, A, A.shape[0], B, B.shape[0],
+                          beta, out, M)
+        else:
+            import ctypes as C
+            dA, dB, dC = (bx.DeviceArray.from_host(x) for x in (A, B, C0))
+            al = (C.c_float * 2)(alpha.real, alpha.imag)
+            be = (C.c_float * 2)(beta.real, beta.imag)
+            fn = bx.lib().cblas_cgemm
+            fn(bx.COL_MAJOR, bx.TRANS[ta], bx.TRANS[tb], M, N, K, C.cast(al, C.c_void_p),
+               C.c_void_p(dA.ptr), A.shape[0], C.c_void_p(dB.ptr), B.shape[0],
+               C.cast(be, C.c_void_p), C.c_void_p(dC.ptr), M)
+            out = dC.download()
+            for d in (dA, dB, dC):
+                d.free()
+        assert bx.stats()["calls_gpu"] == 1
+        err = orc.relerr_fro(out.astype(np.complex128), ref, alpha, A.astype(np.complex128),
+                             B.astype(np.complex128), beta, C0.astype(np.complex128))
+        assert err <= 2 * K * eps, f"cgemm {ta}{tb}: rel err {err:.3e}"
+
+
+def test_abi_cgemm_exact_integers_beta_zero_row_major(gpu, orc):
+    """small complex integers: every FP32 partial sum is exact, so the result
+    equals the naive loop bit for bit; beta = 0 ignores NaN in C; RowMajor is
+    the transposed column-major problem"""
+    bx = gpu
+    rng = np.random.default_rng(13)
+    M, N, K = 640, 576, 320
+
+    def ints(*shape):
+        return np.asfortranarray((rng.integers(-4, 5, shape) + 1j * rng.integers(-4, 5, shape))
+                                 .astype(np.complex64))
+
+    A, B = ints(M, K), ints(N, K)
+    ref = np.zeros((M, N), np.complex128, order="F")
+    orc.gemm("N", "C", M, N, K, 2 - 1j, A.astype(np.complex128), M, B.astype(np.complex128), N,
+             0.0, ref, M)
+    out = np.full((M, N), np.nan + 1j * np.nan, np.complex64, order="F")
+    say("cgemm exact N C beta=0")
+    bx.cblas_gemm(bx.COL_MAJOR, "N", "C", M, N, K, 2 - 1j, A, M, B, N, 0.0, out, M)
+    assert np.array_equal(out.astype(np.complex128), ref)
+    # row-major: C (M x N, ldc = N) is the column-major N x M matrix C^T
+    out2 = np.zeros((N, M), np.complex64, order="F")
+    say("cgemm row-major")
+    bx.cblas_gemm(bx.ROW_MAJOR, "N", "T", M, N, K, 2 - 1j, np.ascontiguousarray(A), K,
+                  np.ascontiguousarray(B.conj()), K, 0.0, out2, N)
+    assert np.array_equal(out2.T.astype(np.complex128), ref)
