@@ -36,7 +36,8 @@ void cblas_dgemm(const enum CBLAS_ORDER Order, const enum CBLAS_TRANSPOSE TransA
                  const double alpha, const double *A, const int lda, const double *B,
                  const int ldb, const double beta, double *C, const int ldc);
 
-/* replaces cblas_cgemm, blas/cgemm.c:16-142 (CPU forward, as in the reference) */
+/* replaces cblas_cgemm, blas/cgemm.c:16-142 (the reference keeps it on the CPU;
+ * here above 500 it runs as one real GEMM on the FP32 matrix cores) */
 void cblas_cgemm(const enum CBLAS_ORDER Order, const enum CBLAS_TRANSPOSE TransA,
                  const enum CBLAS_TRANSPOSE TransB, const int M, const int N, const int K,
                  const void *alpha, const void *A, const int lda, const void *B,
@@ -57,7 +58,7 @@ void sgemm_(const char *transa, const char *transb, const int *m, const int *n, 
 void dgemm_(const char *transa, const char *transb, const int *m, const int *n, const int *k,
             const double *alpha, const double *A, const int *lda, const double *B,
             const int *ldb, const double *beta, double *C, const int *ldc);
-/* replaces cgemm_, blas/cgemm.c (CPU forward) */
+/* replaces cgemm_, blas/cgemm.c:119 */
 void cgemm_(const char *transa, const char *transb, const int *m, const int *n, const int *k,
             const void *alpha, const void *A, const int *lda, const void *B, const int *ldb,
             const void *beta, void *C, const int *ldc);
