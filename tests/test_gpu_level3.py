@@ -317,3 +317,57 @@ def test_gpu_her2k(gpu, orc, where):
     call(L, "cblas", "zher2k", dt, ROW, UP, CTR, N, K, np.conj(alpha), A, N, B, N, beta, out, N)
     _check(out, ref, 2 * abs(alpha) * _fro(A) * _fro(B) + abs(beta) * _fro(C0), K, dt,
            "her2k row-major", mask=m)
+
+
+def test_gpu_level3_edges(gpu, orc):
+    """shapes at the edges of the decompositions: a one-column right-hand
+    side, a last TRSM leaf of one row (M = 1025), a one-row Right-side solve
+    with N = 1100, SYMM sized for the GPU by N alone, alpha = 0 for TRMM / TRSM
+    and beta = 0 over NaN for SYMM, SYR2K with K = 0 in HBM"""
+    bx = gpu
+    L = bx.lib()
+    dt = np.float64
+    # TRSM: one column, M = 1025 (leaves of 64 ... and a last leaf of 1 row)
+    for M, N, side in ((1025, 1, LEFT), (1, 1100, RIGHT)):
+        na = M if side == LEFT else N
+        A0, B0 = _mats(orc, dt, 20, (na, na), (M, N))
+        A = trsm_operand(A0, na)
+        for uplo in (LO, UP):
+            say("trsm edge", M, N, side, uplo)
+            X = B0.copy(order="F")
+            call(L, "cblas", "dtrsm", dt, COL, side, uplo, NOTR, NONUNIT, M, N, 1.0, A, na, X, M)
+            err = trsm_residual(orc, side, uplo, NOTR, NONUNIT, M, N, 1.0, A, X, B0)
+            assert err <= na * EPS[dt], err
+    # SYMM: M small, N above the threshold (Side Right: A is N x N)
+    M, N = 40, 1100
+    A, B, C0 = _mats(orc, dt, 21, (N, N), (M, N), (M, N))
+    C0[:, :] = np.nan
+    ref = ref_symm(orc, RIGHT, UP, False, M, N, 2.0, A, B, 0.0, np.zeros_like(C0))
+    out = C0.copy(order="F")
+    say("symm edge beta=0 NaN")
+    call(L, "cblas", "dsymm", dt, COL, RIGHT, UP, M, N, 2.0, A, N, B, M, 0.0, out, M)
+    _check(out, ref, 2.0 * _fro(full_sym(A, UP, False)) * _fro(B), N, dt, "symm edge")
+    # alpha = 0: B becomes zero (TRMM, TRSM), A not needed to be meaningful
+    M, N = 1100, 64
+    A, B0 = _mats(orc, dt, 22, (M, M), (M, N))
+    for r in ("dtrmm", "dtrsm"):
+        say(r, "alpha=0")
+        dA, dB = _dev(bx, A), _dev(bx, B0)
+        call(L, "cblas", r, dt, COL, LEFT, LO, NOTR, UNIT, M, N, 0.0, dA.ptr, M, dB.ptr, M)
+        assert not np.any(dB.download()), r
+        dA.free()
+        dB.free()
+    # SYR2K with K = 0 in HBM: C = beta C on the triangle only
+    N = 1100
+    C0 = orc.fill_uniform(N, N, 23, 0, dtype=dt)
+    dC = _dev(bx, C0)
+    dummy = _dev(bx, np.zeros((N, 1), order="F"))
+    say("syr2k K=0 device")
+    call(L, "cblas", "dsyr2k", dt, COL, LO, NOTR, N, 0, 1.0, dummy.ptr, N, dummy.ptr, N, 0.5,
+         dC.ptr, N)
+    out = dC.download()
+    m = np.tril(np.ones((N, N), bool))
+    assert np.array_equal(out[m], 0.5 * C0[m])
+    assert np.array_equal(out[~m], C0[~m])
+    dC.free()
+    dummy.free()
