@@ -498,12 +498,15 @@ def generate():
     cw("#include \"cpu_forward.h\"")
     cw("#include \"internal.h\"")
     cw("")
+    cw("/* a routine the CPU BLAS lacks is looked up (and reported) once */")
+    cw("#define FWD_MISSING ((void *)-1)")
+    cw("")
     cw("static void *fwd_sym(void **slot, const char *name)")
     cw("{")
     cw("    void *f = __atomic_load_n(slot, __ATOMIC_ACQUIRE);")
-    cw("    if (f) return f;")
+    cw("    if (f) return f == FWD_MISSING ? NULL : f;")
     cw("    f = bx_cpublas_sym(name);")
-    cw("    if (f) __atomic_store_n(slot, f, __ATOMIC_RELEASE);")
+    cw("    __atomic_store_n(slot, f ? f : FWD_MISSING, __ATOMIC_RELEASE);")
     cw("    return f;")
     cw("}")
     cw("")

@@ -63,9 +63,11 @@ static int scratch_alloc(scratch_t *s, void **p, size_t bytes)
 {
     if (bytes < 16) bytes = 16;
     if (bxh_malloc(p, bytes) != 0) {
-        /* the tile cache keeps its HBM slab between calls: release it, retry */
+        /* the tile cache keeps its HBM slab between calls: release it, retry
+         * (tiles kept resident across calls are dropped with it) */
         bx_cache_t *c = bx_global_cache();
         if (!c) return BLASX_ENOMEM;
+        bx_log(1, "releasing the tile cache's HBM for a %zu-byte operand", bytes);
         bx_cache_invalidate_all(c);
         bx_cache_free_memory(c);
         if (bxh_malloc(p, bytes) != 0) return BLASX_ENOMEM;
