@@ -371,7 +371,8 @@ class Routine:
 # run on the GPU by syrk.c and level3.c (declared in blasx.h): no public
 # forwarder, but an internal bx_cpu_<name> for their CPU-sized calls
 ON_GPU = {"ssyrk", "dsyrk", "zsyrk", "zherk", "ssymm", "dsymm", "zsymm", "zhemm", "strmm",
-          "dtrmm", "ztrmm", "strsm", "dtrsm", "ztrsm", "ssyr2k", "dsyr2k", "zsyr2k", "zher2k"}
+          "dtrmm", "ztrmm", "strsm", "dtrsm", "ztrsm", "ssyr2k", "dsyr2k", "zsyr2k", "zher2k",
+          "csyrk", "cherk", "csymm", "chemm", "ctrmm", "ctrsm", "csyr2k", "cher2k"}
 
 
 def routines():

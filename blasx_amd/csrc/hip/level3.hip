@@ -15,35 +15,57 @@
 
 namespace {
 
-struct z16 {
-    double re, im;
+// complex elements: T is double (complex128) or float (complex64)
+template <typename T> struct cx {
+    T re, im;
 };
+using z16 = cx<double>;
+using c8 = cx<float>;
 
-template <typename E> __device__ __forceinline__ E conj_of(E v) { return v; }
-template <> __device__ __forceinline__ z16 conj_of(z16 v) { return z16{v.re, -v.im}; }
-template <typename E> __device__ __forceinline__ E real_of(E v) { return v; }
-template <> __device__ __forceinline__ z16 real_of(z16 v) { return z16{v.re, 0.0}; }
-template <typename E> __device__ __forceinline__ E zero_of() { return E(0); }
-template <> __device__ __forceinline__ z16 zero_of() { return z16{0.0, 0.0}; }
-template <typename E> __device__ __forceinline__ E one_of() { return E(1); }
-template <> __device__ __forceinline__ z16 one_of() { return z16{1.0, 0.0}; }
+template <typename E> struct is_cx { static constexpr bool value = false; };
+template <typename T> struct is_cx<cx<T>> { static constexpr bool value = true; };
+
+template <typename E> __device__ __forceinline__ E conj_of(E v)
+{
+    if constexpr (is_cx<E>::value) return E{v.re, -v.im};
+    else return v;
+}
+template <typename E> __device__ __forceinline__ E real_of(E v)
+{
+    if constexpr (is_cx<E>::value) return E{v.re, 0};
+    else return v;
+}
+template <typename E> __device__ __forceinline__ E zero_of()
+{
+    if constexpr (is_cx<E>::value) return E{0, 0};
+    else return E(0);
+}
+template <typename E> __device__ __forceinline__ E one_of()
+{
+    if constexpr (is_cx<E>::value) return E{1, 0};
+    else return E(1);
+}
 
 // acc - a * b, and a / b, for the substitution
-template <typename E> __device__ __forceinline__ E sub_mul(E acc, E a, E b) { return acc - a * b; }
-template <> __device__ __forceinline__ z16 sub_mul(z16 acc, z16 a, z16 b)
+template <typename E> __device__ __forceinline__ E sub_mul(E acc, E a, E b)
 {
-    return z16{acc.re - (a.re * b.re - a.im * b.im), acc.im - (a.re * b.im + a.im * b.re)};
+    if constexpr (is_cx<E>::value)
+        return E{acc.re - (a.re * b.re - a.im * b.im), acc.im - (a.re * b.im + a.im * b.re)};
+    else return acc - a * b;
 }
-template <typename E> __device__ __forceinline__ E divide(E a, E b) { return a / b; }
 // complex division by Smith's method (Smith 1962), which avoids overflow in |b|^2
-template <> __device__ __forceinline__ z16 divide(z16 a, z16 b)
+template <typename E> __device__ __forceinline__ E divide(E a, E b)
 {
-    if (fabs(b.re) >= fabs(b.im)) {
-        const double r = b.im / b.re, d = b.re + b.im * r;
-        return z16{(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+    if constexpr (is_cx<E>::value) {
+        if (fabs(b.re) >= fabs(b.im)) {
+            const auto r = b.im / b.re, d = b.re + b.im * r;
+            return E{(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+        }
+        const auto r = b.re / b.im, d = b.im + b.re * r;
+        return E{(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+    } else {
+        return a / b;
     }
-    const double r = b.re / b.im, d = b.im + b.re * r;
-    return z16{(a.re * r + a.im) / d, (a.im * r - a.re) / d};
 }
 
 constexpr int TB = 32;
@@ -160,6 +182,24 @@ __global__ __launch_bounds__(NB) void k_trsm_leaf(int lower, int m, int n, const
     }
 }
 
+// dst = src on the triangle tri names (BX_TRI_* bits; row >= col for lower,
+// row <= col for upper), the other triangle of dst untouched; BX_TRI_HERM
+// also zeroes the diagonal's imaginary part
+template <typename E>
+__global__ void k_tri_copy(int n, int tri, const E *__restrict__ src, int64_t lds,
+                           E *__restrict__ dst, int64_t ldd)
+{
+    const int64_t total = (int64_t)n * n;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(t % n), c = (int)(t / n);
+        if ((tri & BX_TRI_LOWER) ? r < c : r > c) continue;
+        E v = src[r + (int64_t)c * lds];
+        if ((tri & BX_TRI_HERM) && r == c) v = real_of(v);
+        dst[r + (int64_t)c * ldd] = v;
+    }
+}
+
 template <typename E> int launch_fill(int n, int lower, int mode, void *A, int64_t lda, hipStream_t st)
 {
     const int nt = (n + TB - 1) / TB;
@@ -180,6 +220,7 @@ int bxh_tri_fill(int dtype, int n, int lower, int mode, void *A, int64_t lda, vo
     case BLASX_S: return launch_fill<float>(n, lower, mode, A, lda, st);
     case BLASX_D: return launch_fill<double>(n, lower, mode, A, lda, st);
     case BLASX_Z: return launch_fill<z16>(n, lower, mode, A, lda, st);
+    case BLASX_C: return launch_fill<c8>(n, lower, mode, A, lda, st);
     default: return BLASX_EINVAL;
     }
 }
@@ -203,12 +244,17 @@ int bxh_transpose(int dtype, int rows, int cols, const void *in, int64_t ldi, vo
         hipLaunchKernelGGL(k_transpose<z16>, grid, dim3(256), 0, st, rows, cols, (const z16 *)in,
                            ldi, (z16 *)out, ldo, conj);
         break;
+    case BLASX_C:
+        hipLaunchKernelGGL(k_transpose<c8>, grid, dim3(256), 0, st, rows, cols, (const c8 *)in,
+                           ldi, (c8 *)out, ldo, conj);
+        break;
     default: return BLASX_EINVAL;
     }
     return hipGetLastError() == hipSuccess ? 0 : BLASX_EHIP;
 }
 
 int bxh_trsm_leaf_nb(int dtype) { return dtype == BLASX_Z ? 32 : 64; }
+
 
 int bxh_trsm_leaf(int dtype, int lower, int m, int n, const void *T, int64_t ldt, void *X,
                   int64_t ldx, void *stream)
@@ -230,6 +276,40 @@ int bxh_trsm_leaf(int dtype, int lower, int m, int n, const void *T, int64_t ldt
     case BLASX_Z:
         hipLaunchKernelGGL((k_trsm_leaf<z16, 32>), grid, dim3(32), 0, st, lower, m, n,
                            (const z16 *)T, ldt, (z16 *)X, ldx);
+        break;
+    case BLASX_C:
+        hipLaunchKernelGGL((k_trsm_leaf<c8, 64>), grid, dim3(64), 0, st, lower, m, n,
+                           (const c8 *)T, ldt, (c8 *)X, ldx);
+        break;
+    default: return BLASX_EINVAL;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : BLASX_EHIP;
+}
+
+int bxh_tri_copy(int dtype, int n, int tri, const void *src, int64_t lds, void *dst, int64_t ldd,
+                 void *stream)
+{
+    if (n <= 0) return 0;
+    const int64_t total = (int64_t)n * n;
+    int grid = (int)((total + 255) / 256);
+    if (grid > 65536) grid = 65536;
+    hipStream_t st = (hipStream_t)stream;
+    switch (dtype) {
+    case BLASX_S:
+        hipLaunchKernelGGL(k_tri_copy<float>, dim3(grid), dim3(256), 0, st, n, tri,
+                           (const float *)src, lds, (float *)dst, ldd);
+        break;
+    case BLASX_D:
+        hipLaunchKernelGGL(k_tri_copy<double>, dim3(grid), dim3(256), 0, st, n, tri,
+                           (const double *)src, lds, (double *)dst, ldd);
+        break;
+    case BLASX_Z:
+        hipLaunchKernelGGL(k_tri_copy<z16>, dim3(grid), dim3(256), 0, st, n, tri,
+                           (const z16 *)src, lds, (z16 *)dst, ldd);
+        break;
+    case BLASX_C:
+        hipLaunchKernelGGL(k_tri_copy<c8>, dim3(grid), dim3(256), 0, st, n, tri, (const c8 *)src,
+                           lds, (c8 *)dst, ldd);
         break;
     default: return BLASX_EINVAL;
     }

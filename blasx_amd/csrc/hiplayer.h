@@ -128,6 +128,10 @@ int bxh_tri_fill(int dtype, int n, int lower, int mode, void *A, int64_t lda, vo
 /* out (cols x rows) = in^T (in^H with conj, complex only) */
 int bxh_transpose(int dtype, int rows, int cols, const void *in, int64_t ldi, void *out, int64_t ldo,
                   int conj, void *stream);
+/* dst = src on the triangle tri names (BX_TRI_* bits), n x n; BX_TRI_HERM also
+ * zeroes the diagonal's imaginary part */
+int bxh_tri_copy(int dtype, int n, int tri, const void *src, int64_t lds, void *dst, int64_t ldd,
+                 void *stream);
 /* solve T X = X in place for an m x m triangular T (lower or upper) with
  * m <= bxh_trsm_leaf_nb(dtype) */
 int bxh_trsm_leaf_nb(int dtype);

@@ -137,6 +137,8 @@ int bx_device_syr2k(int dt, int tri, int opa, int N, int K, const double *alpha,
 bx_cache_t *bx_global_cache(void);
 int bx_gpu_cgemm(int transa, int transb, int M, int N, int K, const double *al, const void *A,
                  int lda, const void *B, int ldb, const double *be, void *C, int ldc);
+int bx_gpu_crank(int herm, int uplo, int trans, int N, int K, const double *al, const void *A,
+                 int lda, const void *B, int ldb, const double *be, void *C, int ldc);
 /* C = alpha op(A) op(B) + beta C on HBM operands (transa, transb: CBLAS enums) */
 int bx_device_gemm(int dt, int transa, int transb, int M, int N, int K, const double *alpha,
                    const void *A, int64_t lda, const void *B, int64_t ldb, const double *beta,

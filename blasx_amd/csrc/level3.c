@@ -54,13 +54,17 @@
 #include "internal.h"
 
 /* ------------------------------------------------------------ device work */
+/* the HBM buffers of one call; the most any routine here takes is 8 (complex64
+ * TRSM, Right side: A, op(A), B, three packing buffers, T^T, B^T) */
+enum { SCRATCH_MAX = 12 };
 typedef struct {
-    void *buf[6];
+    void *buf[SCRATCH_MAX];
     int n;
 } scratch_t;
 
 static int scratch_alloc(scratch_t *s, void **p, size_t bytes)
 {
+    if (s->n >= SCRATCH_MAX) return BLASX_EINVAL;
     if (bytes < 16) bytes = 16;
     if (bxh_malloc(p, bytes) != 0) {
         /* the tile cache keeps its HBM slab between calls: release it, retry
@@ -130,6 +134,117 @@ static int finish(scratch_t *s, int rc)
     return rc ? rc : rs;
 }
 
+/* ---- complex64 on the FP32 matrix cores ----
+ * C = alpha op(A) op(B) + beta C as one real GEMM of M x 2N x 2K over the
+ * packed operands [Re op(A), Im op(A)] and [[Re, Im], [-Im, Re]] of op(B)
+ * (bxh_cpack_a/b), then C = alpha P + beta C (bxh_ccombine).  The packing
+ * buffers come from the caller, sized for the largest product it runs. */
+typedef struct {
+    float *Ap, *Bp, *P;
+    size_t nAp, nBp, nP; /* floats */
+} cwork_t;
+
+static int cwork_alloc(scratch_t *s, cwork_t *w, size_t nAp, size_t nBp, size_t nP)
+{
+    int rc = scratch_alloc(s, (void **)&w->Ap, nAp * sizeof(float));
+    if (!rc) rc = scratch_alloc(s, (void **)&w->Bp, nBp * sizeof(float));
+    if (!rc) rc = scratch_alloc(s, (void **)&w->P, nP * sizeof(float));
+    w->nAp = nAp, w->nBp = nBp, w->nP = nP;
+    return rc;
+}
+
+static int cgemm_dev(cwork_t *w, int transa, int transb, int M, int N, int K, const double *al,
+                     const void *A, int64_t lda, const void *B, int64_t ldb, const double *be,
+                     void *C, int64_t ldc)
+{
+    const int opa = transa == CblasNoTrans ? 0 : transa == CblasTrans ? 1 : 2;
+    const int opb = transb == CblasNoTrans ? 0 : transb == CblasTrans ? 1 : 2;
+    const int product = K > 0 && (al[0] != 0.0 || al[1] != 0.0);
+    const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
+    int rc = 0;
+    if (product) {
+        if ((size_t)M * 2 * K > w->nAp || (size_t)4 * K * N > w->nBp || (size_t)M * 2 * N > w->nP)
+            return BLASX_EINVAL;
+        rc = bxh_cpack_a(M, K, opa, A, lda, w->Ap, g_bx.s_comp);
+        if (!rc) rc = bxh_cpack_b(K, N, opb, B, ldb, w->Bp, g_bx.s_comp);
+        if (!rc)
+            rc = bx_device_gemm(BLASX_S, CblasNoTrans, CblasNoTrans, M, 2 * N, 2 * K, one, w->Ap,
+                                M, w->Bp, 2 * (int64_t)K, zero, w->P, M, g_bx.s_comp);
+    }
+    if (!rc) rc = bxh_ccombine(M, N, product ? w->P : NULL, M, al, be, C, ldc, g_bx.s_comp);
+    return rc;
+}
+
+/* one GEMM on HBM operands of any element type */
+static int gemm_any(int dt, cwork_t *w, int ta, int tb, int M, int N, int K, const double *al,
+                    const void *A, int64_t lda, const void *B, int64_t ldb, const double *be,
+                    void *C, int64_t ldc)
+{
+    if (dt == BLASX_C) return cgemm_dev(w, ta, tb, M, N, K, al, A, lda, B, ldb, be, C, ldc);
+    return bx_device_gemm(dt, ta, tb, M, N, K, al, A, lda, B, ldb, be, C, ldc, g_bx.s_comp);
+}
+
+/* CGEMM through the ABI (abi.c): operands and C as for the routines here */
+int bx_gpu_cgemm(int transa, int transb, int M, int N, int K, const double *al, const void *A,
+                 int lda, const void *B, int ldb, const double *be, void *C, int ldc)
+{
+    const int es = 8;
+    const int ra = transa == CblasNoTrans ? M : K, ca = transa == CblasNoTrans ? K : M;
+    const int rb = transb == CblasNoTrans ? K : N, cb = transb == CblasNoTrans ? N : K;
+    const int bz = be[0] == 0.0 && be[1] == 0.0;
+    const int product = K > 0 && (al[0] != 0.0 || al[1] != 0.0);
+    scratch_t s = {{0}, 0};
+    cwork_t w = {0};
+    void *dA = NULL, *dB = NULL, *dC;
+    int64_t lA = 1, lB = 1, lC;
+    pthread_mutex_lock(&g_bx.gpu_lock);
+    int rc = 0;
+    if (product) {
+        rc = dev_view(&s, A, ra, ca, lda, es, 1, &dA, &lA);
+        if (!rc) rc = dev_view(&s, B, rb, cb, ldb, es, 1, &dB, &lB);
+        if (!rc) rc = cwork_alloc(&s, &w, (size_t)M * 2 * K, (size_t)4 * K * N, (size_t)M * 2 * N);
+    }
+    if (!rc) rc = dev_view(&s, C, M, N, ldc, es, !bz, &dC, &lC);
+    if (!rc) rc = cgemm_dev(&w, transa, transb, M, N, K, al, dA, lA, dB, lB, be, dC, lC);
+    if (!rc) rc = dev_store(C, ldc, dC, lC, M, N, es);
+    return finish(&s, rc);
+}
+
+/* CSYRK, CHERK (B NULL), CSYR2K and CHER2K: the full N x N product into a copy
+ * of C, then its stored triangle copied back (HERK/HER2K: real diagonal).
+ * Twice the FLOPs of a triangle-only update, on the FP32 matrix cores. */
+int bx_gpu_crank(int herm, int uplo, int trans, int N, int K, const double *al, const void *A,
+                 int lda, const void *B, int ldb, const double *be, void *C, int ldc)
+{
+    const int es = 8;
+    const int rows = trans == CblasNoTrans ? N : K, cols = trans == CblasNoTrans ? K : N;
+    const int conjt = herm ? CblasConjTrans : CblasTrans;
+    const int ta = trans == CblasNoTrans ? CblasNoTrans : conjt;
+    const int tb = trans == CblasNoTrans ? conjt : CblasNoTrans;
+    const double one[2] = {1.0, 0.0};
+    /* the second term of HER2K carries conj(alpha) */
+    const double al2[2] = {al[0], herm ? -al[1] : al[1]};
+    const int tri = (uplo == CblasLower ? BX_TRI_LOWER : BX_TRI_UPPER) | (herm ? BX_TRI_HERM : 0);
+    scratch_t s = {{0}, 0};
+    cwork_t w = {0};
+    void *dA, *dB = NULL, *dC, *dF;
+    int64_t lA, lB = 1, lC;
+    pthread_mutex_lock(&g_bx.gpu_lock);
+    int rc = dev_view(&s, A, rows, cols, lda, es, 1, &dA, &lA);
+    if (!rc && B) rc = dev_view(&s, B, rows, cols, ldb, es, 1, &dB, &lB);
+    if (!rc) rc = dev_view(&s, C, N, N, ldc, es, 1, &dC, &lC);
+    if (!rc) rc = scratch_alloc(&s, &dF, (size_t)N * N * es);
+    if (!rc)
+        rc = bxh_memcpy2d_d2d_async(dF, (size_t)N * es, dC, (size_t)lC * es, (size_t)N * es,
+                                    (size_t)N, g_bx.s_comp);
+    if (!rc) rc = cwork_alloc(&s, &w, (size_t)N * 2 * K, (size_t)4 * K * N, (size_t)N * 2 * N);
+    if (!rc) rc = cgemm_dev(&w, ta, tb, N, N, K, al, dA, lA, B ? dB : dA, B ? lB : lA, be, dF, N);
+    if (!rc && B) rc = cgemm_dev(&w, ta, tb, N, N, K, al2, dB, lB, dA, lA, one, dF, N);
+    if (!rc) rc = bxh_tri_copy(BLASX_C, N, tri, dF, N, dC, lC, g_bx.s_comp);
+    if (!rc) rc = dev_store(C, ldc, dC, lC, N, N, es);
+    return finish(&s, rc);
+}
+
 static int gpu_symm(int dt, int herm, int side, int uplo, int M, int N, const double *al,
                     const void *A, int lda, const void *B, int ldb, const double *be, void *C,
                     int ldc)
@@ -146,11 +261,14 @@ static int gpu_symm(int dt, int herm, int side, int uplo, int M, int N, const do
                           g_bx.s_comp);
     if (!rc) rc = dev_view(&s, B, M, N, ldb, es, 1, &dB, &lB);
     if (!rc) rc = dev_view(&s, C, M, N, ldc, es, !bz, &dC, &lC);
+    cwork_t w = {0};
+    if (!rc && dt == BLASX_C)
+        rc = cwork_alloc(&s, &w, (size_t)M * 2 * na, (size_t)4 * na * N, (size_t)M * 2 * N);
     if (!rc)
-        rc = side == CblasLeft ? bx_device_gemm(dt, CblasNoTrans, CblasNoTrans, M, N, M, al, dA,
-                                                na, dB, lB, be, dC, lC, g_bx.s_comp)
-                               : bx_device_gemm(dt, CblasNoTrans, CblasNoTrans, M, N, N, al, dB,
-                                                lB, dA, na, be, dC, lC, g_bx.s_comp);
+        rc = side == CblasLeft ? gemm_any(dt, &w, CblasNoTrans, CblasNoTrans, M, N, M, al, dA, na,
+                                          dB, lB, be, dC, lC)
+                               : gemm_any(dt, &w, CblasNoTrans, CblasNoTrans, M, N, N, al, dB, lB,
+                                          dA, na, be, dC, lC);
     if (!rc) rc = dev_store(C, ldc, dC, lC, M, N, es);
     return finish(&s, rc);
 }
@@ -171,6 +289,19 @@ static int gpu_trmm(int dt, int side, int uplo, int trans, int diag, int M, int 
                           dT, na, g_bx.s_comp);
     if (!rc) rc = dev_view(&s, B, M, N, ldb, es, 1, &dB, &lB);
     if (!rc) rc = scratch_alloc(&s, &dOut, (size_t)M * N * es);
+    if (dt == BLASX_C) {
+        /* complex64: one full product on the FP32 matrix cores */
+        const double zero[2] = {0.0, 0.0};
+        cwork_t w = {0};
+        if (!rc) rc = cwork_alloc(&s, &w, (size_t)M * 2 * na, (size_t)4 * na * N, (size_t)M * 2 * N);
+        if (!rc)
+            rc = side == CblasLeft ? gemm_any(dt, &w, trans, CblasNoTrans, M, N, M, al, dT, na, dB,
+                                              lB, zero, dOut, M)
+                                   : gemm_any(dt, &w, CblasNoTrans, trans, M, N, N, al, dB, lB, dT,
+                                              na, zero, dOut, M);
+        if (!rc) rc = dev_store(B, ldb, dOut, M, M, N, es);
+        return finish(&s, rc);
+    }
     /* op(T) is triangular, so each 256-wide panel of the output (rows for
      * Left, columns for Right) needs only the K range where op(T) is nonzero:
      * one grouped launch of panel GEMMs does half a full GEMM's work */
@@ -227,8 +358,8 @@ static int gpu_trmm(int dt, int side, int uplo, int trans, int diag, int M, int 
 /* X = T^-1 X in place (m x n), T triangular m x m: split T in halves at a
  * multiple of the leaf size, solve one half, update the other with one GEMM
  * (K = the solved half), recurse; leaves are substitution kernels */
-static int trsm_rec(int dt, int lower, const char *T, int64_t ldt, char *X, int64_t ldx, int m,
-                    int n, int es)
+static int trsm_rec(int dt, cwork_t *w, int lower, const char *T, int64_t ldt, char *X,
+                    int64_t ldx, int m, int n, int es)
 {
     const int nb = bxh_trsm_leaf_nb(dt);
     if (m <= nb) return bxh_trsm_leaf(dt, lower, m, n, T, ldt, X, ldx, g_bx.s_comp);
@@ -239,18 +370,28 @@ static int trsm_rec(int dt, int lower, const char *T, int64_t ldt, char *X, int6
     int rc;
     if (lower) {
         /* [T11 0; T21 T22] [X1; X2]: X1 first, then X2 -= T21 X1 */
-        if ((rc = trsm_rec(dt, 1, T, ldt, X, ldx, m1, n, es))) return rc;
-        if ((rc = bx_device_gemm(dt, CblasNoTrans, CblasNoTrans, m2, n, m1, minus_one,
-                                 T + (int64_t)m1 * es, ldt, X, ldx, one, X2, ldx, g_bx.s_comp)))
+        if ((rc = trsm_rec(dt, w, 1, T, ldt, X, ldx, m1, n, es))) return rc;
+        if ((rc = gemm_any(dt, w, CblasNoTrans, CblasNoTrans, m2, n, m1, minus_one,
+                           T + (int64_t)m1 * es, ldt, X, ldx, one, X2, ldx)))
             return rc;
-        return trsm_rec(dt, 1, T22, ldt, X2, ldx, m2, n, es);
+        return trsm_rec(dt, w, 1, T22, ldt, X2, ldx, m2, n, es);
     }
     /* [T11 T12; 0 T22]: X2 first, then X1 -= T12 X2 */
-    if ((rc = trsm_rec(dt, 0, T22, ldt, X2, ldx, m2, n, es))) return rc;
-    if ((rc = bx_device_gemm(dt, CblasNoTrans, CblasNoTrans, m1, n, m2, minus_one,
-                             T + (int64_t)m1 * ldt * es, ldt, X2, ldx, one, X, ldx, g_bx.s_comp)))
+    if ((rc = trsm_rec(dt, w, 0, T22, ldt, X2, ldx, m2, n, es))) return rc;
+    if ((rc = gemm_any(dt, w, CblasNoTrans, CblasNoTrans, m1, n, m2, minus_one,
+                       T + (int64_t)m1 * ldt * es, ldt, X2, ldx, one, X, ldx)))
         return rc;
-    return trsm_rec(dt, 0, T, ldt, X, ldx, m1, n, es);
+    return trsm_rec(dt, w, 0, T, ldt, X, ldx, m1, n, es);
+}
+
+/* C = alpha C for any element type */
+static int scale_any(int dt, int m, int n, const double *al, void *C, int64_t ldc)
+{
+    if (dt == BLASX_C) {
+        const double zero[2] = {0.0, 0.0};
+        return bxh_ccombine(m, n, NULL, 0, zero, al, C, ldc, g_bx.s_comp);
+    }
+    return bxh_scale(dt, m, n, al, C, ldc, g_bx.s_comp);
 }
 
 static int gpu_trsm(int dt, int side, int uplo, int trans, int diag, int M, int N,
@@ -273,10 +414,15 @@ static int gpu_trsm(int dt, int side, int uplo, int trans, int diag, int M, int 
     }
     if (!rc && diag == CblasUnit) rc = bxh_tri_fill(dt, na, lowerT, BX_FILL_UNIT, dT, na, g_bx.s_comp);
     if (!rc) rc = dev_view(&s, B, M, N, ldb, es, 1, &dB, &lB);
+    /* complex64 updates: packing buffers for the largest one (2 m1 m2 <= na^2) */
+    cwork_t w = {0};
+    const int nn = side == CblasLeft ? N : M;
+    if (!rc && dt == BLASX_C)
+        rc = cwork_alloc(&s, &w, (size_t)na * na, (size_t)4 * na * nn, (size_t)2 * na * nn);
     if (!rc && side == CblasLeft) {
         /* op(A) X = alpha B, solved in place */
-        if (!alpha_one) rc = bxh_scale(dt, M, N, al, dB, lB, g_bx.s_comp);
-        if (!rc) rc = trsm_rec(dt, lowerT, dT, na, dB, lB, M, N, es);
+        if (!alpha_one) rc = scale_any(dt, M, N, al, dB, lB);
+        if (!rc) rc = trsm_rec(dt, &w, lowerT, dT, na, dB, lB, M, N, es);
     } else if (!rc) {
         /* X op(A) = alpha B  <=>  op(A)^T X^T = alpha B^T: solve on B^T with T^T */
         void *dTt;
@@ -284,8 +430,8 @@ static int gpu_trsm(int dt, int side, int uplo, int trans, int diag, int M, int 
         if (!rc) rc = bxh_transpose(dt, na, na, dT, na, dTt, na, 0, g_bx.s_comp);
         if (!rc) rc = scratch_alloc(&s, &dX, (size_t)N * M * es);
         if (!rc) rc = bxh_transpose(dt, M, N, dB, lB, dX, N, 0, g_bx.s_comp);
-        if (!rc && !alpha_one) rc = bxh_scale(dt, N, M, al, dX, N, g_bx.s_comp);
-        if (!rc) rc = trsm_rec(dt, !lowerT, dTt, na, dX, N, N, M, es);
+        if (!rc && !alpha_one) rc = scale_any(dt, N, M, al, dX, N);
+        if (!rc) rc = trsm_rec(dt, &w, !lowerT, dTt, na, dX, N, N, M, es);
         if (!rc) rc = bxh_transpose(dt, N, M, dX, N, dB, lB, 0, g_bx.s_comp);
     }
     if (!rc) rc = dev_store(B, ldb, dB, lB, M, N, es);
@@ -351,46 +497,6 @@ static int gpu_her2k(int uplo, int trans, int N, int K, const double *al, const 
     return finish(&s, rc);
 }
 
-/* CGEMM (complex64) on the FP32 matrix cores: one real GEMM of M x 2N x 2K
- * on packed operands (bxh_cpack_a/b), then C = alpha P + beta C.  Operands
- * and C as for the other routines here (in HBM in place, else copied); the
- * transposes are CBLAS enums. */
-int bx_gpu_cgemm(int transa, int transb, int M, int N, int K, const double *al, const void *A,
-                 int lda, const void *B, int ldb, const double *be, void *C, int ldc)
-{
-    const int es = 8;
-    const int ra = transa == CblasNoTrans ? M : K, ca = transa == CblasNoTrans ? K : M;
-    const int rb = transb == CblasNoTrans ? K : N, cb = transb == CblasNoTrans ? N : K;
-    const int opa = transa == CblasNoTrans ? 0 : transa == CblasTrans ? 1 : 2;
-    const int opb = transb == CblasNoTrans ? 0 : transb == CblasTrans ? 1 : 2;
-    const int bz = be[0] == 0.0 && be[1] == 0.0;
-    const int product = K > 0 && (al[0] != 0.0 || al[1] != 0.0);
-    const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
-    scratch_t s = {{0}, 0};
-    void *dA = NULL, *dB = NULL, *dC, *Ap, *Bp, *P = NULL;
-    int64_t lA, lB, lC;
-    pthread_mutex_lock(&g_bx.gpu_lock);
-    int rc = 0;
-    if (product) {
-        rc = dev_view(&s, A, ra, ca, lda, es, 1, &dA, &lA);
-        if (!rc) rc = dev_view(&s, B, rb, cb, ldb, es, 1, &dB, &lB);
-    }
-    if (!rc) rc = dev_view(&s, C, M, N, ldc, es, !bz, &dC, &lC);
-    if (!rc && product) {
-        rc = scratch_alloc(&s, &Ap, (size_t)M * 2 * K * sizeof(float));
-        if (!rc) rc = scratch_alloc(&s, &Bp, (size_t)4 * K * N * sizeof(float));
-        if (!rc) rc = scratch_alloc(&s, &P, (size_t)M * 2 * N * sizeof(float));
-        if (!rc) rc = bxh_cpack_a(M, K, opa, dA, lA, (float *)Ap, g_bx.s_comp);
-        if (!rc) rc = bxh_cpack_b(K, N, opb, dB, lB, (float *)Bp, g_bx.s_comp);
-        if (!rc)
-            rc = bx_device_gemm(BLASX_S, CblasNoTrans, CblasNoTrans, M, 2 * N, 2 * K, one, Ap, M,
-                                Bp, 2 * (int64_t)K, zero, P, M, g_bx.s_comp);
-    }
-    if (!rc) rc = bxh_ccombine(M, N, (const float *)P, M, al, be, dC, lC, g_bx.s_comp);
-    if (!rc) rc = dev_store(C, ldc, dC, lC, M, N, es);
-    return finish(&s, rc);
-}
-
 /* ------------------------------------------------------------------ ABI */
 static void scalar2(int dt, const void *v, double out[2])
 {
@@ -398,9 +504,13 @@ static void scalar2(int dt, const void *v, double out[2])
         out[0] = *(const float *)v, out[1] = 0;
     else if (dt == BLASX_D)
         out[0] = *(const double *)v, out[1] = 0;
+    else if (dt == BLASX_C)
+        out[0] = ((const float *)v)[0], out[1] = ((const float *)v)[1];
     else
         out[0] = ((const double *)v)[0], out[1] = ((const double *)v)[1];
 }
+
+static int is_cplx(int dt) { return dt == BLASX_Z || dt == BLASX_C; }
 
 static int flip_side(int s) { return s == CblasLeft ? CblasRight : s == CblasRight ? CblasLeft : s; }
 static int flip_uplo(int u) { return u == CblasUpper ? CblasLower : u == CblasLower ? CblasUpper : u; }
@@ -411,7 +521,7 @@ static int max1(int v) { return v > 1 ? v : 1; }
 static int gpu_sized(int dt, int a, int b)
 {
     bx_config();
-    const int thr = dt == BLASX_Z ? g_bx.min_dim_cplx : g_bx.min_dim_real;
+    const int thr = is_cplx(dt) ? g_bx.min_dim_cplx : g_bx.min_dim_real;
     return a > thr || b > thr;
 }
 
@@ -450,8 +560,10 @@ static void symm_entry(int dt, int herm, int order, int Side, int Uplo, int M, i
                        const void *alpha, const void *A, int lda, const void *B, int ldb,
                        const void *beta, void *C, int ldc)
 {
-    const char *name = herm ? "ZHEMM" /* blas/zhemm.c:14 */
-                            : dt == BLASX_S ? "SSYMM " : dt == BLASX_D ? "DSYMM " : "ZSYMM ";
+    /* blas/zhemm.c:14 and blas/chemm.c:14 name themselves without the trailing blank */
+    const char *name = herm ? (dt == BLASX_C ? "CHEMM" : "ZHEMM")
+                       : dt == BLASX_S ? "SSYMM " : dt == BLASX_D ? "DSYMM "
+                       : dt == BLASX_C ? "CSYMM " : "ZSYMM ";
     int side = Side, uplo = Uplo, m = M, n = N, info = 0;
     if (order == CblasRowMajor) {
         /* a row-major C is the column-major C^T = B^T A^T: the other side and
@@ -488,8 +600,12 @@ static void symm_entry(int dt, int herm, int order, int Side, int Uplo, int M, i
         if (!gpu_outcome(name, rc)) return;
     }
     need_cpublas(name);
-    if (herm)
+    if (herm && dt == BLASX_C)
+        bx_cpu_chemm(order, Side, Uplo, M, N, alpha, A, lda, B, ldb, beta, C, ldc);
+    else if (herm)
         bx_cpu_zhemm(order, Side, Uplo, M, N, alpha, A, lda, B, ldb, beta, C, ldc);
+    else if (dt == BLASX_C)
+        bx_cpu_csymm(order, Side, Uplo, M, N, alpha, A, lda, B, ldb, beta, C, ldc);
     else if (dt == BLASX_S)
         bx_cpu_ssymm(order, Side, Uplo, M, N, *(const float *)alpha, A, lda, B, ldb,
                      *(const float *)beta, C, ldc);
@@ -504,7 +620,10 @@ static void symm_entry(int dt, int herm, int order, int Side, int Uplo, int M, i
 static void trmm_entry(int dt, int order, int Side, int Uplo, int TransA, int Diag, int M, int N,
                        const void *alpha, const void *A, int lda, void *B, int ldb)
 {
-    const char *name = dt == BLASX_S ? "STRMM " : dt == BLASX_D ? "DTRMM " : "ZTRMM ";
+    const char *name = dt == BLASX_S   ? "STRMM "
+                       : dt == BLASX_D ? "DTRMM "
+                       : dt == BLASX_C ? "CTRMM "
+                                       : "ZTRMM ";
     int side = Side, uplo = Uplo, m = M, n = N, info = 0;
     if (order == CblasRowMajor) {
         /* B^T = op(A)^T B^T (or B^T op(A)^T): the other side, and A's stored
@@ -544,6 +663,8 @@ static void trmm_entry(int dt, int order, int Side, int Uplo, int TransA, int Di
     else if (dt == BLASX_D)
         bx_cpu_dtrmm(order, Side, Uplo, TransA, Diag, M, N, *(const double *)alpha, A, lda, B,
                      ldb);
+    else if (dt == BLASX_C)
+        bx_cpu_ctrmm(order, Side, Uplo, TransA, Diag, M, N, alpha, A, lda, B, ldb);
     else
         bx_cpu_ztrmm(order, Side, Uplo, TransA, Diag, M, N, alpha, A, lda, B, ldb);
 }
@@ -552,7 +673,10 @@ static void trmm_entry(int dt, int order, int Side, int Uplo, int TransA, int Di
 static void trsm_entry(int dt, int order, int Side, int Uplo, int TransA, int Diag, int M, int N,
                        const void *alpha, const void *A, int lda, void *B, int ldb)
 {
-    const char *name = dt == BLASX_S ? "STRSM " : dt == BLASX_D ? "DTRSM " : "ZTRSM ";
+    const char *name = dt == BLASX_S   ? "STRSM "
+                       : dt == BLASX_D ? "DTRSM "
+                       : dt == BLASX_C ? "CTRSM "
+                                       : "ZTRSM ";
     int side = Side, uplo = Uplo, m = M, n = N, info = 0;
     if (order == CblasRowMajor) {
         /* the same mapping as TRMM: B^T op(A)^T or op(A)^T B^T on the
@@ -592,6 +716,8 @@ static void trsm_entry(int dt, int order, int Side, int Uplo, int TransA, int Di
     else if (dt == BLASX_D)
         bx_cpu_dtrsm(order, Side, Uplo, TransA, Diag, M, N, *(const double *)alpha, A, lda, B,
                      ldb);
+    else if (dt == BLASX_C)
+        bx_cpu_ctrsm(order, Side, Uplo, TransA, Diag, M, N, alpha, A, lda, B, ldb);
     else
         bx_cpu_ztrsm(order, Side, Uplo, TransA, Diag, M, N, alpha, A, lda, B, ldb);
 }
@@ -600,14 +726,17 @@ static void trsm_entry(int dt, int order, int Side, int Uplo, int TransA, int Di
 static int syr2k_trans_ok(int dt, int t)
 {
     if (t == CblasNoTrans || t == CblasTrans) return 1;
-    return dt != BLASX_Z && t == CblasConjTrans; /* C means T for real data */
+    return !is_cplx(dt) && t == CblasConjTrans; /* C means T for real data */
 }
 
 static void syr2k_entry(int dt, int order, int Uplo, int Trans, int N, int K, const void *alpha,
                         const void *A, int lda, const void *B, int ldb, const void *beta, void *C,
                         int ldc)
 {
-    const char *name = dt == BLASX_S ? "SSYR2K" : dt == BLASX_D ? "DSYR2K" : "ZSYR2K";
+    const char *name = dt == BLASX_S   ? "SSYR2K"
+                       : dt == BLASX_D ? "DSYR2K"
+                       : dt == BLASX_C ? "CSYR2K"
+                                       : "ZSYR2K";
     int uplo = Uplo, trans = Trans, info = 0;
     if (order == CblasRowMajor) {
         uplo = flip_uplo(Uplo);
@@ -637,7 +766,10 @@ static void syr2k_entry(int dt, int order, int Uplo, int Trans, int N, int K, co
     const int in_hbm = bxh_pointer_is_pinned(C) == 2;
     if (gpu_sized(dt, N, K) && (!no_product || in_hbm)) {
         int rc = bx_ensure_init();
-        if (!rc) rc = gpu_syr2k(dt, uplo, trans, N, K, al, A, lda, B, ldb, be, C, ldc);
+        if (!rc)
+            rc = dt == BLASX_C
+                     ? bx_gpu_crank(0, uplo, trans, N, K, al, A, lda, B, ldb, be, C, ldc)
+                     : gpu_syr2k(dt, uplo, trans, N, K, al, A, lda, B, ldb, be, C, ldc);
         if (!gpu_outcome(name, rc)) return;
     }
     need_cpublas(name);
@@ -647,20 +779,22 @@ static void syr2k_entry(int dt, int order, int Uplo, int Trans, int N, int K, co
     else if (dt == BLASX_D)
         bx_cpu_dsyr2k(order, Uplo, Trans, N, K, *(const double *)alpha, A, lda, B, ldb,
                       *(const double *)beta, C, ldc);
+    else if (dt == BLASX_C)
+        bx_cpu_csyr2k(order, Uplo, Trans, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
     else
         bx_cpu_zsyr2k(order, Uplo, Trans, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
 /* ---- HER2K ---- */
-static void her2k_entry(int order, int Uplo, int Trans, int N, int K, const void *alpha,
+static void her2k_entry(int dt, int order, int Uplo, int Trans, int N, int K, const void *alpha,
                         const void *A, int lda, const void *B, int ldb, double beta, void *C,
                         int ldc)
 {
-    /* netlib's name; the reference's wrapper reports "ZHERK" (blas/zher2k.c:13) */
-    const char *name = "ZHER2K";
+    /* netlib's names; the reference's zher2k wrapper reports "ZHERK" (blas/zher2k.c:13) */
+    const char *name = dt == BLASX_C ? "CHER2K" : "ZHER2K";
     int uplo = Uplo, trans = Trans, info = 0;
     double al[2];
-    scalar2(BLASX_Z, alpha, al);
+    scalar2(dt, alpha, al);
     if (order == CblasRowMajor) {
         /* C^T = conj(alpha) B'^H A' + alpha A'^H B' on the column-major views
          * A' = A^T, B' = B^T: the other triangle, the other transposition and
@@ -689,13 +823,20 @@ static void her2k_entry(int order, int Uplo, int Trans, int N, int K, const void
     const int no_product = K == 0 || (al[0] == 0.0 && al[1] == 0.0);
     if (no_product && beta == 1.0) return;
     const int in_hbm = bxh_pointer_is_pinned(C) == 2;
-    if (gpu_sized(BLASX_Z, N, K) && (!no_product || in_hbm)) {
+    if (gpu_sized(dt, N, K) && (!no_product || in_hbm)) {
         int rc = bx_ensure_init();
-        if (!rc) rc = gpu_her2k(uplo, trans, N, K, al, A, lda, B, ldb, beta, C, ldc);
+        if (!rc) {
+            const double be[2] = {beta, 0.0};
+            rc = dt == BLASX_C ? bx_gpu_crank(1, uplo, trans, N, K, al, A, lda, B, ldb, be, C, ldc)
+                               : gpu_her2k(uplo, trans, N, K, al, A, lda, B, ldb, beta, C, ldc);
+        }
         if (!gpu_outcome(name, rc)) return;
     }
     need_cpublas(name);
-    bx_cpu_zher2k(order, Uplo, Trans, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    if (dt == BLASX_C)
+        bx_cpu_cher2k(order, Uplo, Trans, N, K, alpha, A, lda, B, ldb, (float)beta, C, ldc);
+    else
+        bx_cpu_zher2k(order, Uplo, Trans, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
 /* ------------------------------------------------------------ CBLAS */
@@ -786,7 +927,55 @@ BX_EXPORT void cblas_zher2k(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO 
                             const void *alpha, const void *A, const int lda, const void *B,
                             const int ldb, const double beta, void *C, const int ldc)
 {
-    her2k_entry(Order, Uplo, Trans, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    her2k_entry(BLASX_Z, Order, Uplo, Trans, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+BX_EXPORT void cblas_cher2k(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                            const enum CBLAS_TRANSPOSE Trans, const int N, const int K,
+                            const void *alpha, const void *A, const int lda, const void *B,
+                            const int ldb, const float beta, void *C, const int ldc)
+{
+    her2k_entry(BLASX_C, Order, Uplo, Trans, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+BX_EXPORT void cblas_csyr2k(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                            const enum CBLAS_TRANSPOSE Trans, const int N, const int K,
+                            const void *alpha, const void *A, const int lda, const void *B,
+                            const int ldb, const void *beta, void *C, const int ldc)
+{
+    syr2k_entry(BLASX_C, Order, Uplo, Trans, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+BX_EXPORT void cblas_csymm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                           const enum CBLAS_UPLO Uplo, const int M, const int N, const void *alpha,
+                           const void *A, const int lda, const void *B, const int ldb,
+                           const void *beta, void *C, const int ldc)
+{
+    symm_entry(BLASX_C, 0, Order, Side, Uplo, M, N, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+BX_EXPORT void cblas_chemm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                           const enum CBLAS_UPLO Uplo, const int M, const int N, const void *alpha,
+                           const void *A, const int lda, const void *B, const int ldb,
+                           const void *beta, void *C, const int ldc)
+{
+    symm_entry(BLASX_C, 1, Order, Side, Uplo, M, N, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+BX_EXPORT void cblas_ctrmm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                           const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TransA,
+                           const enum CBLAS_DIAG Diag, const int M, const int N, const void *alpha,
+                           const void *A, const int lda, void *B, const int ldb)
+{
+    trmm_entry(BLASX_C, Order, Side, Uplo, TransA, Diag, M, N, alpha, A, lda, B, ldb);
+}
+
+BX_EXPORT void cblas_ctrsm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                           const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TransA,
+                           const enum CBLAS_DIAG Diag, const int M, const int N, const void *alpha,
+                           const void *A, const int lda, void *B, const int ldb)
+{
+    trsm_entry(BLASX_C, Order, Side, Uplo, TransA, Diag, M, N, alpha, A, lda, B, ldb);
 }
 
 BX_EXPORT void cblas_ssyr2k(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
@@ -924,9 +1113,24 @@ BX_EXPORT void zher2k_(const char *uplo, const char *trans, const int *n, const 
     int u, t;
     if (f77_uplo(uplo, &u) && f77_bad("ZHER2K", 1)) return;
     if (f77_trans(trans, &t) && f77_bad("ZHER2K", 2)) return;
-    her2k_entry(CblasColMajor, u, t, *n, *k, alpha, A, *lda, B, *ldb, *beta, C, *ldc);
+    her2k_entry(BLASX_Z, CblasColMajor, u, t, *n, *k, alpha, A, *lda, B, *ldb, *beta, C, *ldc);
 }
 
+BX_EXPORT void cher2k_(const char *uplo, const char *trans, const int *n, const int *k,
+                       const void *alpha, const void *A, const int *lda, const void *B,
+                       const int *ldb, const float *beta, void *C, const int *ldc)
+{
+    int u, t;
+    if (f77_uplo(uplo, &u) && f77_bad("CHER2K", 1)) return;
+    if (f77_trans(trans, &t) && f77_bad("CHER2K", 2)) return;
+    her2k_entry(BLASX_C, CblasColMajor, u, t, *n, *k, alpha, A, *lda, B, *ldb, *beta, C, *ldc);
+}
+
+F77_SYMM(csymm_, BLASX_C, 0, "CSYMM ", void)
+F77_SYMM(chemm_, BLASX_C, 1, "CHEMM", void)
+F77_TRMM(ctrmm_, BLASX_C, "CTRMM ", void)
+F77_TRSM(ctrsm_, BLASX_C, "CTRSM ", void)
+F77_SYR2K(csyr2k_, BLASX_C, "CSYR2K", void)
 F77_SYR2K(ssyr2k_, BLASX_S, "SSYR2K", float)
 F77_SYR2K(dsyr2k_, BLASX_D, "DSYR2K", double)
 F77_SYR2K(zsyr2k_, BLASX_Z, "ZSYR2K", void)

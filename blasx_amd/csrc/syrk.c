@@ -1,21 +1,23 @@
 /*
  * syrk.c -- symmetric and Hermitian rank-k updates on the GPU:
- * cblas_{s,d,z}syrk, cblas_zherk and their f77 forms,
+ * cblas_{s,d,c,z}syrk, cblas_{c,z}herk and their f77 forms,
  *   C = alpha op(A) op(A)^T + beta C   (op(A)^H and real alpha, beta for HERK)
  * on the triangle Uplo names.
  *
- * Replaces blas/ssyrk.c:15-95, blas/dsyrk.c:15-95, blas/zsyrk.c:15-95 and
- * blas/zherk.c:15-108, which validate and then always forward to the CPU
+ * Replaces blas/ssyrk.c:15-95, blas/dsyrk.c:15-95, blas/csyrk.c:15-95,
+ * blas/zsyrk.c:15-95, blas/cherk.c:15-108 and blas/zherk.c:15-108, which validate and then always forward to the CPU
  * BLAS (blas/dsyrk.c:46-49: both dispatch branches choose the CPU).  Kept
  * from them: the validation order and info numbers (blas/dsyrk.c:33-38,
  * lowest number wins), the xerbla_ names, the "N or K above 1000" size test
- * the reference computes (blas/dsyrk.c:48), CSYRK and CHERK on the CPU.  The
+ * the reference computes (blas/dsyrk.c:48; 500 for complex types).  The
  * GPU runs what the reference's unused triangular tile helpers describe
  * (blas/blasx_mem_control.c:3-22, 189-219): only the C tiles on the stored
  * triangle, the diagonal tiles masked.
  *
- * Host operands go through the tile runtime (bx_tile_syrk); operands in HBM
- * run as one grouped launch here: for each 256-row panel of C, the
+ * CSYRK and CHERK (complex64) run on the FP32 matrix cores as a full product
+ * whose stored triangle is copied back (bx_gpu_crank, level3.c).  For the
+ * other types, host operands go through the tile runtime (bx_tile_syrk) and
+ * operands in HBM run as one grouped launch here: for each 256-row panel of C, the
  * off-diagonal part is a plain GEMM problem (eligible for the DMA-staged
  * block) and the diagonal 256 x 256 block a masked one.
  *
@@ -172,22 +174,24 @@ int bx_device_syr2k(int dt, int tri, int opa, int N, int K, const double *alpha,
 /* ------------------------------------------------------------------ ABI */
 static const char *syrk_name(int dt, int herk)
 {
-    if (herk) return "ZHERK"; /* blas/zherk.c:13 */
-    return dt == BLASX_S ? "SSYRK " : dt == BLASX_D ? "DSYRK " : "ZSYRK ";
+    if (herk) return dt == BLASX_C ? "CHERK" : "ZHERK"; /* blas/zherk.c:13, blas/cherk.c:13 */
+    return dt == BLASX_S ? "SSYRK " : dt == BLASX_D ? "DSYRK " : dt == BLASX_C ? "CSYRK " : "ZSYRK ";
 }
 
 static int trans_ok(int dt, int herk, int t)
 {
     if (t == CblasNoTrans) return 1;
     if (herk) return t == CblasConjTrans;
-    if (dt == BLASX_Z) return t == CblasTrans;
+    if (dt == BLASX_Z || dt == BLASX_C) return t == CblasTrans;
     return t == CblasTrans || t == CblasConjTrans; /* C means T for real data */
 }
 
 static void scalar2(int dt, int herk, const void *v, double out[2])
 {
-    if (dt == BLASX_S) {
+    if (dt == BLASX_S || (dt == BLASX_C && herk)) {
         out[0] = *(const float *)v, out[1] = 0;
+    } else if (dt == BLASX_C) {
+        out[0] = ((const float *)v)[0], out[1] = ((const float *)v)[1];
     } else if (dt == BLASX_D || herk) {
         out[0] = *(const double *)v, out[1] = 0;
     } else {
@@ -201,9 +205,14 @@ static int cpu_syrk(int dt, int herk, int order, int uplo, int trans, int N, int
                     const void *alpha, const void *A, int lda, const void *beta, void *C, int ldc)
 {
     if (bx_cpublas_load() != 0) return BLASX_ECPU;
-    if (herk)
+    if (herk && dt == BLASX_C)
+        bx_cpu_cherk(order, uplo, trans, N, K, *(const float *)alpha, A, lda,
+                     *(const float *)beta, C, ldc);
+    else if (herk)
         bx_cpu_zherk(order, uplo, trans, N, K, *(const double *)alpha, A, lda,
                      *(const double *)beta, C, ldc);
+    else if (dt == BLASX_C)
+        bx_cpu_csyrk(order, uplo, trans, N, K, alpha, A, lda, beta, C, ldc);
     else if (dt == BLASX_S)
         bx_cpu_ssyrk(order, uplo, trans, N, K, *(const float *)alpha, A, lda,
                      *(const float *)beta, C, ldc);
@@ -251,7 +260,7 @@ static void syrk_entry(int dt, int herk, int order, int Uplo, int Trans, int N, 
     if (no_product && be[0] == 1.0 && be[1] == 0.0) return;
 
     bx_config();
-    const int thr = dt == BLASX_Z ? g_bx.min_dim_cplx : g_bx.min_dim_real;
+    const int thr = (dt == BLASX_Z || dt == BLASX_C) ? g_bx.min_dim_cplx : g_bx.min_dim_real;
     if (!(N > thr || K > thr)) {
         if (cpu_syrk(dt, herk, order, Uplo, Trans, N, K, alpha, A, lda, beta, C, ldc) != 0)
             bx_fatal("%s: no CPU BLAS could be loaded for a CPU-sized call (set BLASX_CPUBLAS)",
@@ -261,7 +270,20 @@ static void syrk_entry(int dt, int herk, int order, int Uplo, int Trans, int N, 
     const int tri = (uplo == CblasLower ? BX_TRI_LOWER : BX_TRI_UPPER) | (herk ? BX_TRI_HERM : 0);
     const int opa = trans == CblasNoTrans ? BX_OP_N : herk ? BX_OP_C : BX_OP_T;
     int rc = bx_ensure_init();
-    if (rc == 0) {
+    if (rc == 0 && dt == BLASX_C) {
+        /* complex64 on the FP32 matrix cores (level3.c) */
+        rc = bx_gpu_crank(herk, uplo, trans, N, K, al, A, lda, NULL, 0, be, C, ldc);
+        if (rc == BLASX_ENOMEM) {
+            bx_log(1, "%s: operands do not fit in HBM; running on the CPU BLAS", name);
+            if (cpu_syrk(dt, herk, order, Uplo, Trans, N, K, alpha, A, lda, beta, C, ldc) != 0)
+                bx_fatal("%s: no CPU BLAS could be loaded (set BLASX_CPUBLAS)", name);
+            return;
+        }
+        if (rc == 0) {
+            BX_STAT_ADD(calls_gpu, 1);
+            return;
+        }
+    } else if (rc == 0) {
         if (bxh_pointer_is_pinned(C) == 2 && (no_product || bxh_pointer_is_pinned(A) == 2)) {
             /* operands already in HBM: run in place */
             rc = bx_device_syrk(dt, tri, opa, N, K, al, A, lda, NULL, 0, be, C, ldc, NULL);
@@ -324,6 +346,22 @@ BX_EXPORT void cblas_zherk(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO U
     syrk_entry(BLASX_Z, 1, Order, Uplo, Trans, N, K, &alpha, A, lda, &beta, C, ldc);
 }
 
+BX_EXPORT void cblas_csyrk(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                           const enum CBLAS_TRANSPOSE Trans, const int N, const int K,
+                           const void *alpha, const void *A, const int lda, const void *beta,
+                           void *C, const int ldc)
+{
+    syrk_entry(BLASX_C, 0, Order, Uplo, Trans, N, K, alpha, A, lda, beta, C, ldc);
+}
+
+BX_EXPORT void cblas_cherk(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                           const enum CBLAS_TRANSPOSE Trans, const int N, const int K,
+                           const float alpha, const void *A, const int lda, const float beta,
+                           void *C, const int ldc)
+{
+    syrk_entry(BLASX_C, 1, Order, Uplo, Trans, N, K, &alpha, A, lda, &beta, C, ldc);
+}
+
 /* f77: info 1 for a bad UPLO, 2 for a bad TRANS character
  * (blas/dsyrk.c:83-85), then the ColMajor CBLAS path */
 static int f77_syrk_args(const char *name, const char *uplo, const char *trans, int *u, int *t)
@@ -379,4 +417,22 @@ BX_EXPORT void zherk_(const char *uplo, const char *trans, const int *n, const i
     int u, t;
     if (f77_syrk_args("ZHERK", uplo, trans, &u, &t)) return;
     syrk_entry(BLASX_Z, 1, CblasColMajor, u, t, *n, *k, alpha, A, *lda, beta, C, *ldc);
+}
+
+BX_EXPORT void csyrk_(const char *uplo, const char *trans, const int *n, const int *k,
+                      const void *alpha, const void *A, const int *lda, const void *beta, void *C,
+                      const int *ldc)
+{
+    int u, t;
+    if (f77_syrk_args("CSYRK ", uplo, trans, &u, &t)) return;
+    syrk_entry(BLASX_C, 0, CblasColMajor, u, t, *n, *k, alpha, A, *lda, beta, C, *ldc);
+}
+
+BX_EXPORT void cherk_(const char *uplo, const char *trans, const int *n, const int *k,
+                      const float *alpha, const void *A, const int *lda, const float *beta,
+                      void *C, const int *ldc)
+{
+    int u, t;
+    if (f77_syrk_args("CHERK", uplo, trans, &u, &t)) return;
+    syrk_entry(BLASX_C, 1, CblasColMajor, u, t, *n, *k, alpha, A, *lda, beta, C, *ldc);
 }

@@ -215,6 +215,69 @@ void zsyr2k_(const char *uplo, const char *trans, const int *n, const int *k, co
              const void *A, const int *lda, const void *B, const int *ldb, const void *beta,
              void *C, const int *ldc);
 
+/* ---- complex64 forms of the above, on the FP32 matrix cores (one real GEMM
+ * over packed operands; rank updates as a full product whose stored triangle
+ * is copied back) ------------------------------------------------------------
+ * replace cblas_csyrk blas/csyrk.c:15, cblas_cherk blas/cherk.c:15,
+ * cblas_csymm blas/csymm.c:16, cblas_chemm blas/chemm.c:16,
+ * cblas_ctrmm blas/ctrmm.c:17, cblas_ctrsm blas/ctrsm.c:16,
+ * cblas_csyr2k blas/csyr2k.c:16, cblas_cher2k blas/cher2k.c:15 */
+void cblas_csyrk(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                 const enum CBLAS_TRANSPOSE Trans, const int N, const int K, const void *alpha,
+                 const void *A, const int lda, const void *beta, void *C, const int ldc);
+void cblas_cherk(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                 const enum CBLAS_TRANSPOSE Trans, const int N, const int K, const float alpha,
+                 const void *A, const int lda, const float beta, void *C, const int ldc);
+void cblas_csymm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const int M, const int N, const void *alpha,
+                 const void *A, const int lda, const void *B, const int ldb, const void *beta,
+                 void *C, const int ldc);
+void cblas_chemm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const int M, const int N, const void *alpha,
+                 const void *A, const int lda, const void *B, const int ldb, const void *beta,
+                 void *C, const int ldc);
+void cblas_ctrmm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TransA,
+                 const enum CBLAS_DIAG Diag, const int M, const int N, const void *alpha,
+                 const void *A, const int lda, void *B, const int ldb);
+void cblas_ctrsm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TransA,
+                 const enum CBLAS_DIAG Diag, const int M, const int N, const void *alpha,
+                 const void *A, const int lda, void *B, const int ldb);
+void cblas_csyr2k(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                  const enum CBLAS_TRANSPOSE Trans, const int N, const int K, const void *alpha,
+                  const void *A, const int lda, const void *B, const int ldb, const void *beta,
+                  void *C, const int ldc);
+void cblas_cher2k(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                  const enum CBLAS_TRANSPOSE Trans, const int N, const int K, const void *alpha,
+                  const void *A, const int lda, const void *B, const int ldb, const float beta,
+                  void *C, const int ldc);
+/* replace csyrk_ blas/csyrk.c:76, cherk_ blas/cherk.c:82, csymm_ blas/csymm.c:74,
+ * chemm_ blas/chemm.c:80, ctrmm_ blas/ctrmm.c:67, ctrsm_ blas/ctrsm.c:78,
+ * csyr2k_ blas/csyr2k.c:88, cher2k_ blas/cher2k.c:89 */
+void csyrk_(const char *uplo, const char *trans, const int *n, const int *k, const void *alpha,
+            const void *A, const int *lda, const void *beta, void *C, const int *ldc);
+void cherk_(const char *uplo, const char *trans, const int *n, const int *k, const float *alpha,
+            const void *A, const int *lda, const float *beta, void *C, const int *ldc);
+void csymm_(const char *side, const char *uplo, const int *m, const int *n, const void *alpha,
+            const void *A, const int *lda, const void *B, const int *ldb, const void *beta,
+            void *C, const int *ldc);
+void chemm_(const char *side, const char *uplo, const int *m, const int *n, const void *alpha,
+            const void *A, const int *lda, const void *B, const int *ldb, const void *beta,
+            void *C, const int *ldc);
+void ctrmm_(const char *side, const char *uplo, const char *transa, const char *diag,
+            const int *m, const int *n, const void *alpha, const void *A, const int *lda,
+            void *B, const int *ldb);
+void ctrsm_(const char *side, const char *uplo, const char *transa, const char *diag,
+            const int *m, const int *n, const void *alpha, const void *A, const int *lda,
+            void *B, const int *ldb);
+void csyr2k_(const char *uplo, const char *trans, const int *n, const int *k, const void *alpha,
+             const void *A, const int *lda, const void *B, const int *ldb, const void *beta,
+             void *C, const int *ldc);
+void cher2k_(const char *uplo, const char *trans, const int *n, const int *k, const void *alpha,
+             const void *A, const int *lda, const void *B, const int *ldb, const float *beta,
+             void *C, const int *ldc);
+
 /* replaces xerbla_, blas/error.c:5-8.  Weak: a caller (LAPACK) may override it. */
 void xerbla_(const char *srname, const int *info);
 

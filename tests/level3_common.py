@@ -15,7 +15,12 @@ NONUNIT, UNIT = 131, 132
 LEFT, RIGHT = 141, 142
 CH = {NOTR: b"N", TR: b"T", CTR: b"C", UP: b"U", LO: b"L", NONUNIT: b"N", UNIT: b"U",
       LEFT: b"L", RIGHT: b"R"}
-PFX = {np.float32: "s", np.float64: "d", np.complex128: "z"}
+PFX = {np.float32: "s", np.float64: "d", np.complex128: "z", np.complex64: "c"}
+
+
+def up(x):
+    """complex64 widened exactly to complex128 for the oracle's complex GEMM"""
+    return np.asfortranarray(x.astype(np.complex128)) if x.dtype == np.complex64 else x
 
 
 def _sc(dt, v, f77=False):
@@ -23,6 +28,9 @@ def _sc(dt, v, f77=False):
         x = C.c_float(v)
     elif dt == np.float64:
         x = C.c_double(v)
+    elif dt == np.complex64:
+        x = (C.c_float * 2)(complex(v).real, complex(v).imag)
+        return C.cast(x, C.c_void_p), x
     else:
         x = (C.c_double * 2)(complex(v).real, complex(v).imag)
         return C.cast(x, C.c_void_p), x
@@ -63,8 +71,8 @@ def call(L, entry, routine, dt, *args):
             r, x = _sc(dt, a, f77=entry == "f77")
             keep.append(x)
             out.append(r)
-        elif kind == "r":  # a real double scalar (HER2K's beta)
-            x = C.c_double(a)
+        elif kind == "r":  # a real scalar (HER2K's beta): float for complex64
+            x = C.c_float(a) if dt == np.complex64 else C.c_double(a)
             keep.append(x)
             out.append(C.byref(x) if entry == "f77" else x)
         else:
@@ -97,6 +105,7 @@ def full_tri(A, uplo, diag):
 
 
 def ref_symm(orc, side, uplo, herm, M, N, alpha, A, B, beta, C0):
+    A, B, C0 = up(A), up(B), up(C0)
     F = full_sym(A, uplo, herm)
     out = np.asfortranarray(C0.copy())
     if side == LEFT:
@@ -107,6 +116,7 @@ def ref_symm(orc, side, uplo, herm, M, N, alpha, A, B, beta, C0):
 
 
 def ref_trmm(orc, side, uplo, trans, diag, M, N, alpha, A, B0):
+    A, B0 = up(A), up(B0)
     T = full_tri(A, uplo, diag)
     t = {NOTR: "N", TR: "T", CTR: "C"}[trans]
     if B0.dtype.kind != "c" and t == "C":
@@ -123,6 +133,7 @@ def trsm_residual(orc, side, uplo, trans, diag, M, N, alpha, A, X, B0):
     """backward error of a TRSM result X: ||op(T) X - alpha B|| (Left) or
     ||X op(T) - alpha B|| (Right) over ||op(T)|| ||X|| + |alpha| ||B||,
     products by the oracle's naive GEMM"""
+    A, X, B0 = up(A), up(X), up(B0)
     T = full_tri(A, uplo, diag)
     t = {NOTR: "N", TR: "T", CTR: "C"}[trans]
     if X.dtype.kind != "c" and t == "C":
@@ -146,6 +157,7 @@ def trsm_operand(A, na):
 
 
 def ref_syr2k(orc, uplo, trans, N, K, alpha, A, B, beta, C0):
+    A, B, C0 = up(A), up(B), up(C0)
     ta, tb = ("N", "T") if trans == NOTR else ("T", "N")
     lda = A.shape[0]
     full = np.asfortranarray(C0.copy())
@@ -160,6 +172,7 @@ def ref_syr2k(orc, uplo, trans, N, K, alpha, A, B, beta, C0):
 def ref_her2k(orc, uplo, trans, N, K, alpha, A, B, beta, C0):
     """alpha op(A) op(B)^H + conj(alpha) op(B) op(A)^H + beta C on the stored
     triangle, the diagonal real"""
+    A, B, C0 = up(A), up(B), up(C0)
     ta, tb = ("N", "C") if trans == NOTR else ("C", "N")
     lda = A.shape[0]
     full = np.asfortranarray(C0.copy())

@@ -16,13 +16,17 @@ CH = {NOTR: b"N", TR: b"T", CTR: b"C", UP: b"U", LO: b"L"}
 
 def name_of(dt, herk):
     if herk:
-        return "zherk"
-    return {np.float32: "ssyrk", np.float64: "dsyrk", np.complex128: "zsyrk"}[dt]
+        return "cherk" if dt == np.complex64 else "zherk"
+    return {np.float32: "ssyrk", np.float64: "dsyrk", np.complex128: "zsyrk",
+            np.complex64: "csyrk"}[dt]
 
 
 def _scal(dt, herk, v, f77=False):
-    if dt == np.float32:
+    if dt == np.float32 or (dt == np.complex64 and herk):
         x = C.c_float(v)
+    elif dt == np.complex64:
+        x = (C.c_float * 2)(complex(v).real, complex(v).imag)
+        return C.cast(x, C.c_void_p), x
     elif dt == np.float64 or herk:
         x = C.c_double(v)
     else:
@@ -64,6 +68,8 @@ def reference(orc, dt, herk, uplo, trans, N, K, alpha, A, lda, beta, C0):
     """The oracle's naive GEMM with B = A, kept on the stored triangle."""
     ta = "N" if trans == NOTR else ("C" if herk else "T")
     tb = ("C" if herk else "T") if trans == NOTR else "N"
+    if A.dtype == np.complex64:  # widened exactly for the oracle's complex GEMM
+        A, C0 = np.asfortranarray(A.astype(np.complex128)), np.asfortranarray(C0.astype(np.complex128))
     full = np.asfortranarray(C0.copy())
     orc.gemm(ta, tb, N, N, K, alpha, A, lda, A, lda, beta, full, N)
     out = C0.copy(order="F")

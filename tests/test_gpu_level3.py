@@ -20,8 +20,8 @@ from level3_common import (COL, CTR, LEFT, LO, NONUNIT, NOTR, RIGHT, ROW, TR, UN
 pytestmark = pytest.mark.gpu
 
 EPS = {np.float64: np.finfo(np.float64).eps, np.float32: np.finfo(np.float32).eps,
-       np.complex128: np.finfo(np.float64).eps}
-PFX = {np.float32: "s", np.float64: "d", np.complex128: "z"}
+       np.complex128: np.finfo(np.float64).eps, np.complex64: np.finfo(np.float32).eps}
+PFX = {np.float32: "s", np.float64: "d", np.complex128: "z", np.complex64: "c"}
 
 
 def say(*a):
@@ -30,6 +30,9 @@ def say(*a):
 
 
 def _mats(orc, dt, seed, *shapes):
+    if dt == np.complex64:  # the complex128 generator, rounded
+        return [np.asfortranarray(orc.fill_uniform(r, c, seed, i, dtype=np.complex128)
+                                  .astype(np.complex64)) for i, (r, c) in enumerate(shapes)]
     return [orc.fill_uniform(r, c, seed, i, dtype=dt) for i, (r, c) in enumerate(shapes)]
 
 
@@ -58,7 +61,7 @@ def _dev(bx, a):
 
 
 SYMM_CASES = [(np.float64, "dsymm"), (np.float32, "ssymm"), (np.complex128, "zsymm"),
-              (np.complex128, "zhemm")]
+              (np.complex128, "zhemm"), (np.complex64, "csymm"), (np.complex64, "chemm")]
 
 
 @pytest.mark.parametrize("dt,r", SYMM_CASES)
@@ -73,7 +76,7 @@ def test_gpu_symm_hemm(gpu, orc, dt, r, where):
             say(r, where, side, uplo, M, N)
             na = M if side == LEFT else N
             A, B, C0 = _mats(orc, dt, 3, (na, na), (M, N), (M, N))
-            ref = ref_symm(orc, side, uplo, r == "zhemm", M, N, alpha, A, B, beta, C0)
+            ref = ref_symm(orc, side, uplo, r.endswith("hemm"), M, N, alpha, A, B, beta, C0)
             bx.reset_stats()
             if where == "host":
                 out = C0.copy(order="F")
@@ -89,12 +92,12 @@ def test_gpu_symm_hemm(gpu, orc, dt, r, where):
                 for d in (dA, dB, dC):
                     d.free()
             assert bx.stats()["calls_gpu"] == 1
-            F = full_sym(A, uplo, r == "zhemm")
+            F = full_sym(A, uplo, r.endswith("hemm"))
             _check(out, ref, abs(alpha) * _fro(F) * _fro(B) + abs(beta) * _fro(C0), na, dt,
                    f"{r} {where}")
 
 
-TRMM_TYPES = [np.float64, np.float32, np.complex128]
+TRMM_TYPES = [np.float64, np.float32, np.complex128, np.complex64]
 
 
 @pytest.mark.parametrize("dt", TRMM_TYPES)
@@ -128,7 +131,7 @@ def test_gpu_trmm(gpu, orc, dt, where):
                     _check(out, ref, abs(alpha) * _fro(T) * _fro(B0), na, dt, "trmm")
 
 
-@pytest.mark.parametrize("dt", [np.float64, np.float32, np.complex128])
+@pytest.mark.parametrize("dt", [np.float64, np.float32, np.complex128, np.complex64])
 @pytest.mark.parametrize("where", ["host", "device"])
 def test_gpu_syr2k(gpu, orc, dt, where):
     bx = gpu
@@ -279,11 +282,11 @@ def test_gpu_trsm_exact_integers_and_row_major(gpu, orc):
             assert np.array_equal(X, Xi), "row-major"
 
 
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
 @pytest.mark.parametrize("where", ["host", "device"])
-def test_gpu_her2k(gpu, orc, where):
+def test_gpu_her2k(gpu, orc, dt, where):
     bx = gpu
     L = bx.lib()
-    dt = np.complex128
     N, K = 600, 320
     alpha, beta = 0.75 - 0.5j, 0.5
     for uplo in (UP, LO):
@@ -295,11 +298,11 @@ def test_gpu_her2k(gpu, orc, where):
             lda = shape[0]
             if where == "host":
                 out = C0.copy(order="F")
-                call(L, "cblas", "zher2k", dt, COL, uplo, trans, N, K, alpha, A, lda, B, lda,
+                call(L, "cblas", PFX[dt] + "her2k", dt, COL, uplo, trans, N, K, alpha, A, lda, B, lda,
                      beta, out, N)
             else:
                 dA, dB, dC = _dev(bx, A), _dev(bx, B), _dev(bx, C0)
-                call(L, "cblas", "zher2k", dt, COL, uplo, trans, N, K, alpha, dA.ptr, lda,
+                call(L, "cblas", PFX[dt] + "her2k", dt, COL, uplo, trans, N, K, alpha, dA.ptr, lda,
                      dB.ptr, lda, beta, dC.ptr, N)
                 out = dC.download()
                 for d in (dA, dB, dC):
@@ -314,7 +317,7 @@ def test_gpu_her2k(gpu, orc, where):
     ref, m = ref_her2k(orc, LO, NOTR, N, K, alpha, A, B, beta, C0)
     out = C0.copy(order="F")
     say("her2k row-major")
-    call(L, "cblas", "zher2k", dt, ROW, UP, CTR, N, K, np.conj(alpha), A, N, B, N, beta, out, N)
+    call(L, "cblas", PFX[dt] + "her2k", dt, ROW, UP, CTR, N, K, np.conj(alpha), A, N, B, N, beta, out, N)
     _check(out, ref, 2 * abs(alpha) * _fro(A) * _fro(B) + abs(beta) * _fro(C0), K, dt,
            "her2k row-major", mask=m)
 

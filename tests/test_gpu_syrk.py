@@ -18,8 +18,9 @@ from syrk_common import COL, CTR, LO, NOTR, ROW, TR, UP, cblas_syrk, f77_syrk, m
 pytestmark = pytest.mark.gpu
 
 EPS = {np.float64: np.finfo(np.float64).eps, np.float32: np.finfo(np.float32).eps,
-       np.complex128: np.finfo(np.float64).eps}
-DTS = [(np.float64, 0), (np.float32, 0), (np.complex128, 0), (np.complex128, 1)]
+       np.complex128: np.finfo(np.float64).eps, np.complex64: np.finfo(np.float32).eps}
+DTS = [(np.float64, 0), (np.float32, 0), (np.complex128, 0), (np.complex128, 1),
+       (np.complex64, 0), (np.complex64, 1)]
 
 
 def say(*a):
@@ -29,8 +30,9 @@ def say(*a):
 
 def _data(orc, dt, herk, trans, N, K, seed):
     shape = (N, K) if trans == NOTR else (K, N)
-    A = orc.fill_uniform(shape[0], shape[1], seed, 0, dtype=dt)
-    C0 = orc.fill_uniform(N, N, seed, 2, dtype=dt)
+    gdt = np.complex128 if dt == np.complex64 else dt
+    A = np.asfortranarray(orc.fill_uniform(shape[0], shape[1], seed, 0, dtype=gdt).astype(dt))
+    C0 = np.asfortranarray(orc.fill_uniform(N, N, seed, 2, dtype=gdt).astype(dt))
     return A, C0, shape[0]
 
 

@@ -16,8 +16,8 @@ from level3_common import (COL, CTR, LEFT, LO, NONUNIT, NOTR, RIGHT, ROW, TR, UN
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HERE = os.path.dirname(os.path.abspath(__file__))
-DTS = [np.float32, np.float64, np.complex128]
-PFX = {np.float32: "s", np.float64: "d", np.complex128: "z"}
+DTS = [np.float32, np.float64, np.complex128, np.complex64]
+PFX = {np.float32: "s", np.float64: "d", np.complex128: "z", np.complex64: "c"}
 
 
 def _rnd(rng, dt, *shape):
@@ -28,7 +28,7 @@ def _rnd(rng, dt, *shape):
 
 
 def _tol(dt):
-    return 2e-4 if dt == np.float32 else 1e-12
+    return 2e-4 if dt in (np.float32, np.complex64) else 1e-12
 
 
 def _run(body):
@@ -112,9 +112,9 @@ def test_symm_hemm_cpu_sized(bx, orc, dt, side, uplo):
     A, B, C0 = _rnd(rng, dt, na, na), _rnd(rng, dt, M, N), _rnd(rng, dt, M, N)
     alpha, beta = _scal(dt, 0.75, -0.25), _scal(dt, -0.5, 1.0)
     L = bx.lib()
-    routines = [PFX[dt] + "symm"] + (["zhemm"] if dt == np.complex128 else [])
+    routines = [PFX[dt] + "symm"] + ([PFX[dt] + "hemm"] if np.dtype(dt).kind == "c" else [])
     for r in routines:
-        ref = ref_symm(orc, side, uplo, r == "zhemm", M, N, alpha, A, B, beta, C0)
+        ref = ref_symm(orc, side, uplo, r.endswith("hemm"), M, N, alpha, A, B, beta, C0)
         for entry in ("cblas", "f77", "row"):
             out = C0.copy(order="F")
             if entry == "cblas":
@@ -210,11 +210,11 @@ def test_trsm_cpu_sized(bx, orc, dt, side, uplo):
                 assert err <= _tol(dt), (trans, diag, entry, err)
 
 
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
 @pytest.mark.parametrize("uplo", [UP, LO])
 @pytest.mark.parametrize("trans", [NOTR, CTR])
-def test_her2k_cpu_sized(bx, orc, uplo, trans):
+def test_her2k_cpu_sized(bx, orc, dt, uplo, trans):
     rng = np.random.default_rng(7)
-    dt = np.complex128
     N, K = 21, 15
     shape = (N, K) if trans == NOTR else (K, N)
     A, B, C0 = _rnd(rng, dt, *shape), _rnd(rng, dt, *shape), _rnd(rng, dt, N, N)
@@ -224,16 +224,16 @@ def test_her2k_cpu_sized(bx, orc, uplo, trans):
     lda = shape[0]
     for entry in ("cblas", "f77", "row"):
         out = C0.copy(order="F")
+        r = PFX[dt] + "her2k"
         if entry == "cblas":
-            call(L, "cblas", "zher2k", dt, COL, uplo, trans, N, K, alpha, A, lda, B, lda, beta,
-                 out, N)
+            call(L, "cblas", r, dt, COL, uplo, trans, N, K, alpha, A, lda, B, lda, beta, out, N)
         elif entry == "f77":
-            call(L, "f77", "zher2k", dt, uplo, trans, N, K, alpha, A, lda, B, lda, beta, out, N)
+            call(L, "f77", r, dt, uplo, trans, N, K, alpha, A, lda, B, lda, beta, out, N)
         else:
             # the same bytes read row-major: the other triangle and transposition,
             # conj(alpha) (C^T = conj(alpha) B'^H A' + alpha A'^H B')
-            call(L, "cblas", "zher2k", dt, ROW, LO if uplo == UP else UP,
+            call(L, "cblas", r, dt, ROW, LO if uplo == UP else UP,
                  CTR if trans == NOTR else NOTR, N, K, np.conj(alpha), A, lda, B, lda, beta,
                  out, N)
-        assert np.max(np.abs(out[m] - ref[m])) <= 1e-12 * 8, entry
+        assert np.max(np.abs(out[m] - ref[m])) <= _tol(dt) * 8, entry
         assert np.array_equal(out[~m].view(np.uint8), C0[~m].view(np.uint8)), entry

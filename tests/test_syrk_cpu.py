@@ -67,7 +67,8 @@ def test_lowest_info_wins(bx):
     assert "parameter number  3" in out
 
 
-DTS = [(np.float32, 0), (np.float64, 0), (np.complex128, 0), (np.complex128, 1)]
+DTS = [(np.float32, 0), (np.float64, 0), (np.complex128, 0), (np.complex128, 1),
+       (np.complex64, 0), (np.complex64, 1)]
 
 
 @pytest.mark.parametrize("dt,herk", DTS)
@@ -88,7 +89,7 @@ def test_cpu_sized_calls(bx, orc, dt, herk, uplo, transname):
     alpha, beta = (0.75, -0.5) if herk or np.dtype(dt).kind != "c" else (0.75 - 0.25j, -0.5 + 1j)
     lda = shape[0]
     ref = reference(orc, dt, herk, uplo, trans, N, K, alpha, A, lda, beta, C0)
-    tol = 1e-4 if dt == np.float32 else 1e-12
+    tol = 1e-4 if dt in (np.float32, np.complex64) else 1e-12
     m = mask(N, uplo)
     L = bx.lib()
     for entry in ("cblas", "f77", "row"):
