@@ -1,6 +1,7 @@
-"""The CPU-forwarded BLAS of libblasx (every CBLAS / f77 level-1/2/3 routine
-besides the GEMMs), the reference's blas/*.c wrappers (e.g. blas/dgemv.c:9-36,
-blas/dot.c:74-90, blas/amax.c:24-40).  Each routine is checked against an
+"""The CPU-forwarded BLAS of libblasx (every CBLAS / f77 level-1/2 routine) and
+the CPU-sized calls of the level-3 routines, which go to the CPU BLAS as well;
+the reference's blas/*.c wrappers (e.g. blas/dgemv.c:9-36, blas/dot.c:74-90,
+blas/amax.c:24-40).  Each routine is checked against an
 independent numpy formula of its BLAS definition, through both the CBLAS and
 the Fortran-77 entry point, including negative increments (which the ILP64
 forwarding must sign-extend), packed and banded storage and RowMajor."""
