@@ -53,16 +53,27 @@ template <typename E> __device__ __forceinline__ E sub_mul(E acc, E a, E b)
         return E{acc.re - (a.re * b.re - a.im * b.im), acc.im - (a.re * b.im + a.im * b.re)};
     else return acc - a * b;
 }
-// complex division by Smith's method (Smith 1962), which avoids overflow in |b|^2
+// c ? a : b by value, component by component (a conditional on two struct
+// lvalues may select their addresses and pin a register array in memory)
+template <typename E> __device__ __forceinline__ E pick(bool c, E a, E b)
+{
+    if constexpr (is_cx<E>::value) return E{c ? a.re : b.re, c ? a.im : b.im};
+    else return c ? a : b;
+}
+// complex division by Smith's method (Smith 1962), which avoids overflow in
+// |b|^2: with p the larger-magnitude part of b and o the other, r = o / p and
+// d = p + o r; |b.re| >= |b.im| gives ((a.re + a.im r) + i (a.im - a.re r)) / d,
+// else ((a.im + a.re r) + i (a.im r - a.re)) / d.  Written with selects, not
+// branches, so unrolled callers stay one basic block.
 template <typename E> __device__ __forceinline__ E divide(E a, E b)
 {
     if constexpr (is_cx<E>::value) {
-        if (fabs(b.re) >= fabs(b.im)) {
-            const auto r = b.im / b.re, d = b.re + b.im * r;
-            return E{(a.re + a.im * r) / d, (a.im - a.re * r) / d};
-        }
-        const auto r = b.re / b.im, d = b.im + b.re * r;
-        return E{(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+        const bool big = fabs(b.re) >= fabs(b.im);
+        const auto p = big ? b.re : b.im, o = big ? b.im : b.re;
+        const auto u = big ? a.re : a.im, v = big ? a.im : a.re;
+        const auto r = o / p, d = p + o * r;
+        const auto t = v - u * r;
+        return E{(u + v * r) / d, (big ? t : -t) / d};
     } else {
         return a / b;
     }
@@ -133,52 +144,120 @@ __global__ __launch_bounds__(256) void k_transpose(int rows, int cols, const E *
     }
 }
 
+// v from lane Q of each group of four lanes, to all four (DPP quad_perm
+// [Q,Q,Q,Q], one move per 32-bit word)
+template <int Q> __device__ __forceinline__ int quad_bcast_word(int w)
+{
+    return __builtin_amdgcn_update_dpp(0, w, Q * 0x55, 0xF, 0xF, false);
+}
+template <int Q> __device__ __forceinline__ float quad_bcast_real(float v)
+{
+    return __builtin_bit_cast(float, quad_bcast_word<Q>(__builtin_bit_cast(int, v)));
+}
+template <int Q> __device__ __forceinline__ double quad_bcast_real(double v)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)quad_bcast_word<Q>((int)(uint32_t)u);
+    const uint32_t hi = (uint32_t)quad_bcast_word<Q>((int)(uint32_t)(u >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+template <int Q, typename E> __device__ __forceinline__ E quad_bcast(E v)
+{
+    if constexpr (is_cx<E>::value) return E{quad_bcast_real<Q>(v.re), quad_bcast_real<Q>(v.im)};
+    else return quad_bcast_real<Q>(v);
+}
+// f(step<0>{}), ..., f(step<N-1>{}): the loop index as a compile-time
+// constant, so register arrays indexed by it stay in registers
+template <int V> struct step {
+    static constexpr int value = V;
+};
+template <int K, int N, typename F> __device__ __forceinline__ void static_for(F &f)
+{
+    if constexpr (K < N) {
+        f(step<K>{});
+        static_for<K + 1, N>(f);
+    }
+}
+
 // Triangular solve of one diagonal block: T (m x m, m <= NB, lower or upper,
 // its own diagonal) times X = B, in place on the m x n block X.  A block of
-// NB threads owns NB columns: T and the columns are staged in LDS with
-// coalesced loads, and each thread substitutes down (lower) or up (upper) its
-// column held in registers.  Rows m..NB-1 are padded as identity rows.
+// LEAF_THREADS threads owns LEAF_COLS columns; T and the NB x LEAF_COLS block
+// of X are staged in LDS, each thread issuing all of its global loads before
+// the first is waited on.  Four lanes share a column, lane q holding rows
+// 4s + q (s < NB/4) in registers.  The solve runs in the loop order of
+// netlib's DTRSM (Left, NoTrans: lower k = 0..m-1, upper k = m-1..0;
+// x[k] /= T(k,k), then x[i] -= T(i,k) x[k] for the rows below / above): at
+// step k the lane owning row k hands x[k] to its three neighbours by a quad
+// broadcast, all four divide it alike, and each updates its own rows, so one
+// lane does a quarter of the column's FMAs.  T is kept by step (tq[k] holds
+// column k, lane q's rows contiguous) for wide LDS reads.  Rows m..NB-1 are
+// padded as identity rows and never stored.
+constexpr int LEAF_THREADS = 256;
+constexpr int LEAF_COLS = LEAF_THREADS / 4;
+
 template <typename E, int NB>
-__global__ __launch_bounds__(NB) void k_trsm_leaf(int lower, int m, int n, const E *__restrict__ T,
-                                                  int64_t ldt, E *__restrict__ X, int64_t ldx)
+__global__ __launch_bounds__(LEAF_THREADS) void k_trsm_leaf(int lower, int m, int n,
+                                                            const E *__restrict__ T, int64_t ldt,
+                                                            E *__restrict__ X, int64_t ldx)
 {
-    __shared__ E t[NB][NB + 1];
-    __shared__ E xs[NB][NB + 1];
-    const int tid = threadIdx.x, j0 = blockIdx.x * NB;
-    for (int c = 0; c < NB; ++c) {
-        // t[r][c] = T(r, c); identity beyond m
-        t[tid][c] = (tid < m && c < m) ? T[tid + (int64_t)c * ldt]
-                                       : (tid == c ? one_of<E>() : zero_of<E>());
-        const int j = j0 + c;
-        xs[tid][c] = (tid < m && j < n) ? X[tid + (int64_t)j * ldx] : zero_of<E>();
+    constexpr int S = NB / 4;                      // rows per lane
+    constexpr int G = LEAF_THREADS / NB;           // staging: column groups
+    constexpr int PT = NB / G, PX = LEAF_COLS / G; // staged columns per thread
+    static_assert(S * 4 == NB && G * NB == LEAF_THREADS && PT * G == NB &&
+                      PX * G == LEAF_COLS, "leaf tiling");
+    __shared__ E tq[NB][NB + 1];        // tq[k][q*S + s] = T(4s + q, k)
+    __shared__ E xs[NB][LEAF_COLS + 1]; // xs[i][c] = X(i, j0 + c)
+    const int tid = threadIdx.x, j0 = blockIdx.x * LEAF_COLS;
+    {
+        const int r = tid % NB, g = tid / NB;
+        E tv[PT], xv[PX];
+#pragma unroll
+        for (int p = 0; p < PT; ++p) {
+            const int c = g + p * G;
+            tv[p] = (r < m && c < m) ? T[r + (int64_t)c * ldt]
+                                     : (r == c ? one_of<E>() : zero_of<E>());
+        }
+#pragma unroll
+        for (int p = 0; p < PX; ++p) {
+            const int j = j0 + g + p * G;
+            xv[p] = (r < m && j < n) ? X[r + (int64_t)j * ldx] : zero_of<E>();
+        }
+#pragma unroll
+        for (int p = 0; p < PT; ++p) tq[g + p * G][(r & 3) * S + (r >> 2)] = tv[p];
+#pragma unroll
+        for (int p = 0; p < PX; ++p) xs[r][g + p * G] = xv[p];
     }
     __syncthreads();
-    E x[NB];
+    const int col = tid >> 2, q = tid & 3;
+    E x[S];
 #pragma unroll
-    for (int r = 0; r < NB; ++r) x[r] = xs[r][tid];
-    if (lower) {
+    for (int s = 0; s < S; ++s) x[s] = xs[4 * s + q][col];
+    // step k of the solve; lower runs k = 0..NB-1, upper k = NB-1..0
+    auto solve_step = [&](auto kc) {
+        constexpr int k = decltype(kc)::value, sk = k >> 2, qk = k & 3;
+        const E xk = divide(quad_bcast<qk>(x[sk]), tq[k][qk * S + sk]);
+        // slot sk: row k itself on lane qk, the rows after it (below for
+        // lower, above for upper) on the lanes past qk in that direction
+        const E upd = sub_mul(x[sk], tq[k][q * S + sk], xk);
+        x[sk] = pick(q == qk, xk, pick(lower ? q > qk : q < qk, upd, x[sk]));
 #pragma unroll
-        for (int r = 0; r < NB; ++r) {
-            E acc = x[r];
+        for (int s = 0; s < S; ++s)
+            if (lower ? s > sk : s < sk) x[s] = sub_mul(x[s], tq[k][q * S + s], xk);
+    };
+    auto lower_step = [&](auto kc) { solve_step(kc); };
+    auto upper_step = [&](auto kc) { solve_step(step<NB - 1 - decltype(kc)::value>{}); };
+    if (lower) static_for<0, NB>(lower_step);
+    else static_for<0, NB>(upper_step);
 #pragma unroll
-            for (int c = 0; c < r; ++c) acc = sub_mul(acc, t[r][c], x[c]);
-            x[r] = divide(acc, t[r][r]);
-        }
-    } else {
-#pragma unroll
-        for (int r = NB - 1; r >= 0; --r) {
-            E acc = x[r];
-#pragma unroll
-            for (int c = r + 1; c < NB; ++c) acc = sub_mul(acc, t[r][c], x[c]);
-            x[r] = divide(acc, t[r][r]);
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < NB; ++r) xs[r][tid] = x[r];
+    for (int s = 0; s < S; ++s) xs[4 * s + q][col] = x[s];
     __syncthreads();
-    for (int c = 0; c < NB; ++c) {
-        const int j = j0 + c;
-        if (tid < m && j < n) X[tid + (int64_t)j * ldx] = xs[tid][c];
+    {
+        const int r = tid % NB, g = tid / NB;
+#pragma unroll
+        for (int p = 0; p < PX; ++p) {
+            const int c = g + p * G, j = j0 + c;
+            if (r < m && j < n) X[r + (int64_t)j * ldx] = xs[r][c];
+        }
     }
 }
 
@@ -263,22 +342,22 @@ int bxh_trsm_leaf(int dtype, int lower, int m, int n, const void *T, int64_t ldt
     if (m <= 0 || n <= 0) return 0;
     if (m > nb || ldt < m || ldx < m) return BLASX_EINVAL;
     hipStream_t st = (hipStream_t)stream;
-    const dim3 grid((n + nb - 1) / nb);
+    const dim3 grid((n + LEAF_COLS - 1) / LEAF_COLS);
     switch (dtype) {
     case BLASX_S:
-        hipLaunchKernelGGL((k_trsm_leaf<float, 64>), grid, dim3(64), 0, st, lower, m, n,
+        hipLaunchKernelGGL((k_trsm_leaf<float, 64>), grid, dim3(LEAF_THREADS), 0, st, lower, m, n,
                            (const float *)T, ldt, (float *)X, ldx);
         break;
     case BLASX_D:
-        hipLaunchKernelGGL((k_trsm_leaf<double, 64>), grid, dim3(64), 0, st, lower, m, n,
+        hipLaunchKernelGGL((k_trsm_leaf<double, 64>), grid, dim3(LEAF_THREADS), 0, st, lower, m, n,
                            (const double *)T, ldt, (double *)X, ldx);
         break;
     case BLASX_Z:
-        hipLaunchKernelGGL((k_trsm_leaf<z16, 32>), grid, dim3(32), 0, st, lower, m, n,
+        hipLaunchKernelGGL((k_trsm_leaf<z16, 32>), grid, dim3(LEAF_THREADS), 0, st, lower, m, n,
                            (const z16 *)T, ldt, (z16 *)X, ldx);
         break;
     case BLASX_C:
-        hipLaunchKernelGGL((k_trsm_leaf<c8, 64>), grid, dim3(64), 0, st, lower, m, n,
+        hipLaunchKernelGGL((k_trsm_leaf<c8, 64>), grid, dim3(LEAF_THREADS), 0, st, lower, m, n,
                            (const c8 *)T, ldt, (c8 *)X, ldx);
         break;
     default: return BLASX_EINVAL;
