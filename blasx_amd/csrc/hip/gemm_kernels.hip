@@ -1110,6 +1110,20 @@ __device__ __forceinline__ void grouped_body(const bxh_problem_t *__restrict__ p
                 const int ta = P.opa != 0, tb = P.opb != 0;
                 if constexpr (sizeof(E) == 8) {
                     double *sd = reinterpret_cast<double *>(smem);
+                    if (P.nseg == 1) {
+                        // one K segment (rank-k, TRMM panels): passed by value
+                        // as the plain kernels pass theirs, which keeps the
+                        // segment walk out of the K loop (half its scalar and
+                        // vector instructions)
+                        const bxh_seg_t s1 = seg_at(ps, unused, 0);
+                        switch (ta * 2 + tb) {
+                        case 0: dma64_block<MAJ_MN, MAJ_K, true>(P, nullptr, s1, tm, tn, sd); break;
+                        case 1: dma64_block<MAJ_MN, MAJ_MN, true>(P, nullptr, s1, tm, tn, sd); break;
+                        case 2: dma64_block<MAJ_K, MAJ_K, true>(P, nullptr, s1, tm, tn, sd); break;
+                        default: dma64_block<MAJ_K, MAJ_MN, true>(P, nullptr, s1, tm, tn, sd); break;
+                        }
+                        continue;
+                    }
                     switch (ta * 2 + tb) {
                     case 0: dma64_block<MAJ_MN, MAJ_K, true>(P, ps, unused, tm, tn, sd); break;
                     case 1: dma64_block<MAJ_MN, MAJ_MN, true>(P, ps, unused, tm, tn, sd); break;
