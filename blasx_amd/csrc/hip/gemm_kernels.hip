@@ -573,45 +573,18 @@ __device__ __forceinline__ void dma64_block(const bxh_problem_t &P, const bxh_se
 
     auto stageA = [&](int buf) { return smem + buf * SA; };
     auto stageB = [&](int buf) { return smem + 2 * SA + buf * SB; };
-    // The K tile being DMA'd is tile k0 / 16 of segment si, whose fields are
-    // kept in `cur` (the table is read once per segment).  advance() moves to
-    // the next 16 of K, else to the first non-empty later segment; si reaches
-    // P.nseg when there is none.
-    int si = 0;
-    int64_t k0 = 0;
-    bxh_seg_t cur = seg_at(segs, seg0, 0);
-    auto next_nonempty = [&]() {
-        while (si < P.nseg) {
-            cur = seg_at(segs, seg0, si);
-            if (cur.k > 0) return;
-            ++si;
-        }
+    // The segments run as one pipeline of K tiles: stage `buf` holds the
+    // tile being multiplied while the next one is DMA'd into the other.  An
+    // inner loop takes every tile of one segment but its last, with that
+    // segment's fields fixed (so addresses advance by a constant step); the
+    // last tile prefetches the first of the next non-empty segment.  A
+    // segment's table entry is read once.
+    auto issue = [&](const bxh_seg_t &sg, int64_t k, int b) {
+        dma_tile<MAJA>((const double *)sg.A, sg.lda, m0, k, stageA(b), wave, lane);
+        dma_tile<MAJB>((const double *)sg.B, sg.ldb, n0, k, stageB(b), wave, lane);
     };
-    auto advance = [&]() {
-        k0 += BK;
-        if (k0 < cur.k) return;
-        k0 = 0;
-        ++si;
-        next_nonempty();
-    };
-    auto issue = [&](int buf) {
-        dma_tile<MAJA>((const double *)cur.A, cur.lda, m0, k0, stageA(buf), wave, lane);
-        dma_tile<MAJB>((const double *)cur.B, cur.ldb, n0, k0, stageB(buf), wave, lane);
-    };
-    next_nonempty();
-    const bool any = si < P.nseg;
-    if (any) {
-        issue(0);
-        wait_dma();
-        __builtin_amdgcn_s_barrier();
-    }
-    int buf = 0;
-    bool more = any; // the tile in stage `buf` is valid
-    while (more) {
-        advance();
-        more = si < P.nseg;
-        if (more) issue(buf ^ 1);
-        const double *As = stageA(buf), *Bs = stageB(buf);
+    auto multiply = [&](int b) {
+        const double *As = stageA(b), *Bs = stageB(b);
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             double a[4], bb[4];
@@ -625,9 +598,42 @@ __device__ __forceinline__ void dma64_block(const bxh_problem_t &P, const bxh_se
 #pragma unroll
                 for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], bb[j], acc[i][j]);
         }
+    };
+    // the first non-empty segment at index i or later (P.nseg if none),
+    // its fields in `out`
+    auto find = [&](int i, bxh_seg_t &out) {
+        for (; i < P.nseg; ++i) {
+            out = seg_at(segs, seg0, i);
+            if (out.k > 0) break;
+        }
+        return i;
+    };
+    bxh_seg_t cur = seg0;
+    int si = find(0, cur);
+    if (si < P.nseg) {
+        issue(cur, 0, 0);
+        wait_dma();
+        __builtin_amdgcn_s_barrier();
+    }
+    int buf = 0;
+    while (si < P.nseg) {
+        int64_t k0 = 0;
+        for (; k0 + BK < cur.k; k0 += BK) {
+            issue(cur, k0 + BK, buf ^ 1);
+            multiply(buf);
+            wait_dma();
+            __builtin_amdgcn_s_barrier();
+            buf ^= 1;
+        }
+        bxh_seg_t nxt = cur;
+        const int sn = find(si + 1, nxt);
+        if (sn < P.nseg) issue(nxt, 0, buf ^ 1);
+        multiply(buf);
         wait_dma();
         __builtin_amdgcn_s_barrier();
         buf ^= 1;
+        si = sn;
+        cur = nxt;
     }
 
     double *Cp = (double *)P.C;
