@@ -555,7 +555,7 @@ constexpr int SMEM_DOUBLES = 2 * 2 * (BK * STR_MN);
 // K tiles of consecutive segments run as one pipeline.
 // MASK: P may be a rank-k diagonal block (P.tri), whose epilogue writes only
 // the stored triangle; the plain kernels instantiate it without.
-template <int MAJA, int MAJB, bool MASK = false>
+template <int MAJA, int MAJB, bool MASK = false, bool NESTED = true>
 __device__ __forceinline__ void dma64_block(const bxh_problem_t &P, const bxh_seg_t *segs,
                                             const bxh_seg_t &seg0, int tm, int tn, double *smem)
 {
@@ -573,67 +573,128 @@ __device__ __forceinline__ void dma64_block(const bxh_problem_t &P, const bxh_se
 
     auto stageA = [&](int buf) { return smem + buf * SA; };
     auto stageB = [&](int buf) { return smem + 2 * SA + buf * SB; };
-    // The segments run as one pipeline of K tiles: stage `buf` holds the
-    // tile being multiplied while the next one is DMA'd into the other.  An
-    // inner loop takes every tile of one segment but its last, with that
-    // segment's fields fixed (so addresses advance by a constant step); the
-    // last tile prefetches the first of the next non-empty segment.  A
-    // segment's table entry is read once.
-    auto issue = [&](const bxh_seg_t &sg, int64_t k, int b) {
-        dma_tile<MAJA>((const double *)sg.A, sg.lda, m0, k, stageA(b), wave, lane);
-        dma_tile<MAJB>((const double *)sg.B, sg.ldb, n0, k, stageB(b), wave, lane);
-    };
-    auto multiply = [&](int b) {
-        const double *As = stageA(b), *Bs = stageB(b);
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            double a[4], bb[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                a[i] = frag<MAJA>(As, wm * 64 + i * 16 + l16, s, lg);
-                bb[i] = frag<MAJB>(Bs, wn * 64 + i * 16 + l16, s, lg);
+    // NESTED picks the K loop: nested per segment, or one loop walking the
+    // segments (kept where it measured faster: see the kernels below)
+    if constexpr (NESTED) {
+        // The segments run as one pipeline of K tiles: stage `buf` holds the
+        // tile being multiplied while the next one is DMA'd into the other.  An
+        // inner loop takes every tile of one segment but its last, with that
+        // segment's fields fixed (so addresses advance by a constant step); the
+        // last tile prefetches the first of the next non-empty segment.  A
+        // segment's table entry is read once.
+        auto issue = [&](const bxh_seg_t &sg, int64_t k, int b) {
+            dma_tile<MAJA>((const double *)sg.A, sg.lda, m0, k, stageA(b), wave, lane);
+            dma_tile<MAJB>((const double *)sg.B, sg.ldb, n0, k, stageB(b), wave, lane);
+        };
+        auto multiply = [&](int b) {
+            const double *As = stageA(b), *Bs = stageB(b);
+    #pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                double a[4], bb[4];
+    #pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    a[i] = frag<MAJA>(As, wm * 64 + i * 16 + l16, s, lg);
+                    bb[i] = frag<MAJB>(Bs, wn * 64 + i * 16 + l16, s, lg);
+                }
+    #pragma unroll
+                for (int i = 0; i < 4; ++i)
+    #pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], bb[j], acc[i][j]);
             }
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], bb[j], acc[i][j]);
+        };
+        // the first non-empty segment at index i or later (P.nseg if none),
+        // its fields in `out`
+        auto find = [&](int i, bxh_seg_t &out) {
+            for (; i < P.nseg; ++i) {
+                out = seg_at(segs, seg0, i);
+                if (out.k > 0) break;
+            }
+            return i;
+        };
+        bxh_seg_t cur = seg0;
+        int si = find(0, cur);
+        if (si < P.nseg) {
+            issue(cur, 0, 0);
+            wait_dma();
+            __builtin_amdgcn_s_barrier();
         }
-    };
-    // the first non-empty segment at index i or later (P.nseg if none),
-    // its fields in `out`
-    auto find = [&](int i, bxh_seg_t &out) {
-        for (; i < P.nseg; ++i) {
-            out = seg_at(segs, seg0, i);
-            if (out.k > 0) break;
-        }
-        return i;
-    };
-    bxh_seg_t cur = seg0;
-    int si = find(0, cur);
-    if (si < P.nseg) {
-        issue(cur, 0, 0);
-        wait_dma();
-        __builtin_amdgcn_s_barrier();
-    }
-    int buf = 0;
-    while (si < P.nseg) {
-        int64_t k0 = 0;
-        for (; k0 + BK < cur.k; k0 += BK) {
-            issue(cur, k0 + BK, buf ^ 1);
+        int buf = 0;
+        while (si < P.nseg) {
+            int64_t k0 = 0;
+            for (; k0 + BK < cur.k; k0 += BK) {
+                issue(cur, k0 + BK, buf ^ 1);
+                multiply(buf);
+                wait_dma();
+                __builtin_amdgcn_s_barrier();
+                buf ^= 1;
+            }
+            bxh_seg_t nxt = cur;
+            const int sn = find(si + 1, nxt);
+            if (sn < P.nseg) issue(nxt, 0, buf ^ 1);
             multiply(buf);
             wait_dma();
             __builtin_amdgcn_s_barrier();
             buf ^= 1;
+            si = sn;
+            cur = nxt;
         }
-        bxh_seg_t nxt = cur;
-        const int sn = find(si + 1, nxt);
-        if (sn < P.nseg) issue(nxt, 0, buf ^ 1);
-        multiply(buf);
-        wait_dma();
-        __builtin_amdgcn_s_barrier();
-        buf ^= 1;
-        si = sn;
-        cur = nxt;
+    } else {
+        // The K tile being DMA'd is tile k0 / 16 of segment si, whose fields are
+        // kept in `cur` (the table is read once per segment).  advance() moves to
+        // the next 16 of K, else to the first non-empty later segment; si reaches
+        // P.nseg when there is none.
+        int si = 0;
+        int64_t k0 = 0;
+        bxh_seg_t cur = seg_at(segs, seg0, 0);
+        auto next_nonempty = [&]() {
+            while (si < P.nseg) {
+                cur = seg_at(segs, seg0, si);
+                if (cur.k > 0) return;
+                ++si;
+            }
+        };
+        auto advance = [&]() {
+            k0 += BK;
+            if (k0 < cur.k) return;
+            k0 = 0;
+            ++si;
+            next_nonempty();
+        };
+        auto issue = [&](int buf) {
+            dma_tile<MAJA>((const double *)cur.A, cur.lda, m0, k0, stageA(buf), wave, lane);
+            dma_tile<MAJB>((const double *)cur.B, cur.ldb, n0, k0, stageB(buf), wave, lane);
+        };
+        next_nonempty();
+        const bool any = si < P.nseg;
+        if (any) {
+            issue(0);
+            wait_dma();
+            __builtin_amdgcn_s_barrier();
+        }
+        int buf = 0;
+        bool more = any; // the tile in stage `buf` is valid
+        while (more) {
+            advance();
+            more = si < P.nseg;
+            if (more) issue(buf ^ 1);
+            const double *As = stageA(buf), *Bs = stageB(buf);
+    #pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                double a[4], bb[4];
+    #pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    a[i] = frag<MAJA>(As, wm * 64 + i * 16 + l16, s, lg);
+                    bb[i] = frag<MAJB>(Bs, wn * 64 + i * 16 + l16, s, lg);
+                }
+    #pragma unroll
+                for (int i = 0; i < 4; ++i)
+    #pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], bb[j], acc[i][j]);
+            }
+            wait_dma();
+            __builtin_amdgcn_s_barrier();
+            buf ^= 1;
+        }
     }
 
     double *Cp = (double *)P.C;
@@ -664,7 +725,8 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm_dma(bxh_problem_t P, bxh_seg_t 
     if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
     int tm, tn;
     tile_coords(b, P.tiles_m, P.tiles_n, tm, tn);
-    dma64_block<MAJA, MAJB>(P, nullptr, S, tm, tn, smem);
+    // nested K loop for NN only: NT and TN measured 7-11% slower with it
+    dma64_block<MAJA, MAJB, false, MAJA == MAJ_MN && MAJB == MAJ_K>(P, nullptr, S, tm, tn, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -716,7 +778,7 @@ __device__ __forceinline__ cplx frag(const cplx *stage, int row, int s, int lg)
 }
 } // namespace dmaz
 
-template <int MAJA, int MAJB, bool CJA, bool CJB, bool MASK = false>
+template <int MAJA, int MAJB, bool CJA, bool CJB, bool MASK = false, bool NESTED = true>
 __device__ __forceinline__ void dmaz_block(const bxh_problem_t &P, const bxh_seg_t *segs,
                                            const bxh_seg_t &seg0, int tm, int tn, cplx *smem);
 
@@ -729,7 +791,8 @@ __global__ __launch_bounds__(NT, 2) void k_zgemm_dma(bxh_problem_t P, bxh_seg_t 
     if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
     int tm, tn;
     tile_coords(b, P.tiles_m, P.tiles_n, tm, tn);
-    dmaz_block<MAJA, MAJB, CJA, CJB>(P, nullptr, S, tm, tn, smem);
+    // the segment-walking K loop: the nested one measured 6% slower here
+    dmaz_block<MAJA, MAJB, CJA, CJB, false, false>(P, nullptr, S, tm, tn, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -838,7 +901,7 @@ __device__ __forceinline__ bool dma_eligible(const bxh_problem_t &P, const bxh_s
     return true;
 }
 
-template <int MAJA, int MAJB, bool MASK = false>
+template <int MAJA, int MAJB, bool MASK = false, bool NESTED = true>
 __device__ __forceinline__ void dmas_block(const bxh_problem_t &P, const bxh_seg_t *segs,
                                            const bxh_seg_t &seg0, int tm, int tn, float *smem)
 {
@@ -855,39 +918,99 @@ __device__ __forceinline__ void dmas_block(const bxh_problem_t &P, const bxh_seg
 
     auto stageA = [&](int buf) { return smem + buf * STAGE; };
     auto stageB = [&](int buf) { return smem + 2 * STAGE + buf * STAGE; };
-    SegWalk<BK> w(segs, seg0, P.nseg);
-    auto issue = [&](int buf) {
-        dma_tile<MAJA>((const float *)w.cur.A, w.cur.lda, m0, w.k0, stageA(buf), wave, lane);
-        dma_tile<MAJB>((const float *)w.cur.B, w.cur.ldb, n0, w.k0, stageB(buf), wave, lane);
-    };
-    bool more = w.valid();
-    if (more) {
-        issue(0);
-        dma64::wait_dma();
-        __builtin_amdgcn_s_barrier();
-    }
-    int buf = 0;
-    while (more) {
-        w.advance();
-        more = w.valid();
-        if (more) issue(buf ^ 1);
-        const float *As = stageA(buf), *Bs = stageB(buf);
-#pragma unroll
-        for (int s = 0; s < BK / 4; ++s) {
-            float a[4], bb[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                a[i] = frag<MAJA>(As, wm * 64 + i * 16 + l16, s, lg);
-                bb[i] = frag<MAJB>(Bs, wn * 64 + i * 16 + l16, s, lg);
+    if constexpr (NESTED) {
+        // K tiles of all segments as one two-stage pipeline; the inner loop
+        // takes one segment's tiles but its last with the segment's fields
+        // fixed, the last prefetches the next non-empty segment's first tile
+        auto issue = [&](const bxh_seg_t &sg, int64_t k, int b) {
+            dma_tile<MAJA>((const float *)sg.A, sg.lda, m0, k, stageA(b), wave, lane);
+            dma_tile<MAJB>((const float *)sg.B, sg.ldb, n0, k, stageB(b), wave, lane);
+        };
+        auto multiply = [&](int b) {
+            const float *As = stageA(b), *Bs = stageB(b);
+    #pragma unroll
+            for (int s = 0; s < BK / 4; ++s) {
+                float a[4], bb[4];
+    #pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    a[i] = frag<MAJA>(As, wm * 64 + i * 16 + l16, s, lg);
+                    bb[i] = frag<MAJB>(Bs, wn * 64 + i * 16 + l16, s, lg);
+                }
+    #pragma unroll
+                for (int i = 0; i < 4; ++i)
+    #pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], bb[j], acc[i][j]);
             }
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], bb[j], acc[i][j]);
+        };
+        auto find = [&](int i, bxh_seg_t &out) {
+            for (; i < P.nseg; ++i) {
+                out = seg_at(segs, seg0, i);
+                if (out.k > 0) break;
+            }
+            return i;
+        };
+        bxh_seg_t cur = seg0;
+        int si = find(0, cur);
+        if (si < P.nseg) {
+            issue(cur, 0, 0);
+            dma64::wait_dma();
+            __builtin_amdgcn_s_barrier();
         }
-        dma64::wait_dma();
-        __builtin_amdgcn_s_barrier();
-        buf ^= 1;
+        int buf = 0;
+        while (si < P.nseg) {
+            int64_t k0 = 0;
+            for (; k0 + BK < cur.k; k0 += BK) {
+                issue(cur, k0 + BK, buf ^ 1);
+                multiply(buf);
+                dma64::wait_dma();
+                __builtin_amdgcn_s_barrier();
+                buf ^= 1;
+            }
+            bxh_seg_t nxt = cur;
+            const int sn = find(si + 1, nxt);
+            if (sn < P.nseg) issue(nxt, 0, buf ^ 1);
+            multiply(buf);
+            dma64::wait_dma();
+            __builtin_amdgcn_s_barrier();
+            buf ^= 1;
+            si = sn;
+            cur = nxt;
+        }
+    } else {
+        SegWalk<BK> w(segs, seg0, P.nseg);
+        auto issue = [&](int buf) {
+            dma_tile<MAJA>((const float *)w.cur.A, w.cur.lda, m0, w.k0, stageA(buf), wave, lane);
+            dma_tile<MAJB>((const float *)w.cur.B, w.cur.ldb, n0, w.k0, stageB(buf), wave, lane);
+        };
+        bool more = w.valid();
+        if (more) {
+            issue(0);
+            dma64::wait_dma();
+            __builtin_amdgcn_s_barrier();
+        }
+        int buf = 0;
+        while (more) {
+            w.advance();
+            more = w.valid();
+            if (more) issue(buf ^ 1);
+            const float *As = stageA(buf), *Bs = stageB(buf);
+    #pragma unroll
+            for (int s = 0; s < BK / 4; ++s) {
+                float a[4], bb[4];
+    #pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    a[i] = frag<MAJA>(As, wm * 64 + i * 16 + l16, s, lg);
+                    bb[i] = frag<MAJB>(Bs, wn * 64 + i * 16 + l16, s, lg);
+                }
+    #pragma unroll
+                for (int i = 0; i < 4; ++i)
+    #pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], bb[j], acc[i][j]);
+            }
+            dma64::wait_dma();
+            __builtin_amdgcn_s_barrier();
+            buf ^= 1;
+        }
     }
 
     float *Cp = (float *)P.C;
@@ -921,7 +1044,7 @@ __global__ __launch_bounds__(NT, 2) void k_sgemm_dma(bxh_problem_t P, bxh_seg_t 
     dmas_block<MAJA, MAJB>(P, nullptr, S, tm, tn, smem);
 }
 
-template <int MAJA, int MAJB, bool CJA, bool CJB, bool MASK>
+template <int MAJA, int MAJB, bool CJA, bool CJB, bool MASK, bool NESTED>
 __device__ __forceinline__ void dmaz_block(const bxh_problem_t &P, const bxh_seg_t *segs,
                                            const bxh_seg_t &seg0, int tm, int tn, cplx *smem)
 {
@@ -939,45 +1062,111 @@ __device__ __forceinline__ void dmaz_block(const bxh_problem_t &P, const bxh_seg
 
     auto stageA = [&](int buf) { return smem + buf * SA; };
     auto stageB = [&](int buf) { return smem + 2 * SA + buf * SB; };
-    SegWalk<BK> w(segs, seg0, P.nseg);
-    auto issue = [&](int buf) {
-        dma_tile<MAJA, BM>((const cplx *)w.cur.A, w.cur.lda, m0, w.k0, stageA(buf), wave, lane);
-        dma_tile<MAJB, BN>((const cplx *)w.cur.B, w.cur.ldb, n0, w.k0, stageB(buf), wave, lane);
-    };
-    bool more = w.valid();
-    if (more) {
-        issue(0);
-        dma64::wait_dma();
-        __builtin_amdgcn_s_barrier();
-    }
-    int buf = 0;
-    while (more) {
-        w.advance();
-        more = w.valid();
-        if (more) issue(buf ^ 1);
-        const cplx *As = stageA(buf), *Bs = stageB(buf);
-#pragma unroll
-        for (int s = 0; s < BK / 4; ++s) {
-            cplx a[4], bb[2];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = frag<MAJA, BM, CJA>(As, wm * 64 + i * 16 + l16, s, lg);
-#pragma unroll
-            for (int j = 0; j < 2; ++j) bb[j] = frag<MAJB, BN, CJB>(Bs, wn * 32 + j * 16 + l16, s, lg);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const double nai = -a[i].im;
-#pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    acc[i][j][0] = mfma(a[i].re, bb[j].re, acc[i][j][0]);
-                    acc[i][j][1] = mfma(a[i].re, bb[j].im, acc[i][j][1]);
-                    acc[i][j][0] = mfma(nai, bb[j].im, acc[i][j][0]);
-                    acc[i][j][1] = mfma(a[i].im, bb[j].re, acc[i][j][1]);
+    if constexpr (NESTED) {
+        // K tiles of all segments as one two-stage pipeline; the inner loop
+        // takes one segment's tiles but its last with the segment's fields
+        // fixed, the last prefetches the next non-empty segment's first tile
+        auto issue = [&](const bxh_seg_t &sg, int64_t k, int b) {
+            dma_tile<MAJA, BM>((const cplx *)sg.A, sg.lda, m0, k, stageA(b), wave, lane);
+            dma_tile<MAJB, BN>((const cplx *)sg.B, sg.ldb, n0, k, stageB(b), wave, lane);
+        };
+        auto multiply = [&](int b) {
+            const cplx *As = stageA(b), *Bs = stageB(b);
+    #pragma unroll
+            for (int s = 0; s < BK / 4; ++s) {
+                cplx a[4], bb[2];
+    #pragma unroll
+                for (int i = 0; i < 4; ++i) a[i] = frag<MAJA, BM, CJA>(As, wm * 64 + i * 16 + l16, s, lg);
+    #pragma unroll
+                for (int j = 0; j < 2; ++j) bb[j] = frag<MAJB, BN, CJB>(Bs, wn * 32 + j * 16 + l16, s, lg);
+    #pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const double nai = -a[i].im;
+    #pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        acc[i][j][0] = mfma(a[i].re, bb[j].re, acc[i][j][0]);
+                        acc[i][j][1] = mfma(a[i].re, bb[j].im, acc[i][j][1]);
+                        acc[i][j][0] = mfma(nai, bb[j].im, acc[i][j][0]);
+                        acc[i][j][1] = mfma(a[i].im, bb[j].re, acc[i][j][1]);
+                    }
                 }
             }
+        };
+        auto find = [&](int i, bxh_seg_t &out) {
+            for (; i < P.nseg; ++i) {
+                out = seg_at(segs, seg0, i);
+                if (out.k > 0) break;
+            }
+            return i;
+        };
+        bxh_seg_t cur = seg0;
+        int si = find(0, cur);
+        if (si < P.nseg) {
+            issue(cur, 0, 0);
+            dma64::wait_dma();
+            __builtin_amdgcn_s_barrier();
         }
-        dma64::wait_dma();
-        __builtin_amdgcn_s_barrier();
-        buf ^= 1;
+        int buf = 0;
+        while (si < P.nseg) {
+            int64_t k0 = 0;
+            for (; k0 + BK < cur.k; k0 += BK) {
+                issue(cur, k0 + BK, buf ^ 1);
+                multiply(buf);
+                dma64::wait_dma();
+                __builtin_amdgcn_s_barrier();
+                buf ^= 1;
+            }
+            bxh_seg_t nxt = cur;
+            const int sn = find(si + 1, nxt);
+            if (sn < P.nseg) issue(nxt, 0, buf ^ 1);
+            multiply(buf);
+            dma64::wait_dma();
+            __builtin_amdgcn_s_barrier();
+            buf ^= 1;
+            si = sn;
+            cur = nxt;
+        }
+    } else {
+        SegWalk<BK> w(segs, seg0, P.nseg);
+        auto issue = [&](int buf) {
+            dma_tile<MAJA, BM>((const cplx *)w.cur.A, w.cur.lda, m0, w.k0, stageA(buf), wave, lane);
+            dma_tile<MAJB, BN>((const cplx *)w.cur.B, w.cur.ldb, n0, w.k0, stageB(buf), wave, lane);
+        };
+        bool more = w.valid();
+        if (more) {
+            issue(0);
+            dma64::wait_dma();
+            __builtin_amdgcn_s_barrier();
+        }
+        int buf = 0;
+        while (more) {
+            w.advance();
+            more = w.valid();
+            if (more) issue(buf ^ 1);
+            const cplx *As = stageA(buf), *Bs = stageB(buf);
+    #pragma unroll
+            for (int s = 0; s < BK / 4; ++s) {
+                cplx a[4], bb[2];
+    #pragma unroll
+                for (int i = 0; i < 4; ++i) a[i] = frag<MAJA, BM, CJA>(As, wm * 64 + i * 16 + l16, s, lg);
+    #pragma unroll
+                for (int j = 0; j < 2; ++j) bb[j] = frag<MAJB, BN, CJB>(Bs, wn * 32 + j * 16 + l16, s, lg);
+    #pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const double nai = -a[i].im;
+    #pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        acc[i][j][0] = mfma(a[i].re, bb[j].re, acc[i][j][0]);
+                        acc[i][j][1] = mfma(a[i].re, bb[j].im, acc[i][j][1]);
+                        acc[i][j][0] = mfma(nai, bb[j].im, acc[i][j][0]);
+                        acc[i][j][1] = mfma(a[i].im, bb[j].re, acc[i][j][1]);
+                    }
+                }
+            }
+            dma64::wait_dma();
+            __builtin_amdgcn_s_barrier();
+            buf ^= 1;
+        }
     }
 
     cplx *Cp = (cplx *)P.C;
