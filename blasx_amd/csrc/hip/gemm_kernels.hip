@@ -588,17 +588,17 @@ __device__ __forceinline__ void dma64_block(const bxh_problem_t &P, const bxh_se
         };
         auto multiply = [&](int b) {
             const double *As = stageA(b), *Bs = stageB(b);
-    #pragma unroll
+#pragma unroll
             for (int s = 0; s < 4; ++s) {
                 double a[4], bb[4];
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     a[i] = frag<MAJA>(As, wm * 64 + i * 16 + l16, s, lg);
                     bb[i] = frag<MAJB>(Bs, wn * 64 + i * 16 + l16, s, lg);
                 }
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i < 4; ++i)
-    #pragma unroll
+#pragma unroll
                     for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], bb[j], acc[i][j]);
             }
         };
@@ -678,17 +678,17 @@ __device__ __forceinline__ void dma64_block(const bxh_problem_t &P, const bxh_se
             more = si < P.nseg;
             if (more) issue(buf ^ 1);
             const double *As = stageA(buf), *Bs = stageB(buf);
-    #pragma unroll
+#pragma unroll
             for (int s = 0; s < 4; ++s) {
                 double a[4], bb[4];
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     a[i] = frag<MAJA>(As, wm * 64 + i * 16 + l16, s, lg);
                     bb[i] = frag<MAJB>(Bs, wn * 64 + i * 16 + l16, s, lg);
                 }
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i < 4; ++i)
-    #pragma unroll
+#pragma unroll
                     for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], bb[j], acc[i][j]);
             }
             wait_dma();
@@ -928,17 +928,17 @@ __device__ __forceinline__ void dmas_block(const bxh_problem_t &P, const bxh_seg
         };
         auto multiply = [&](int b) {
             const float *As = stageA(b), *Bs = stageB(b);
-    #pragma unroll
+#pragma unroll
             for (int s = 0; s < BK / 4; ++s) {
                 float a[4], bb[4];
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     a[i] = frag<MAJA>(As, wm * 64 + i * 16 + l16, s, lg);
                     bb[i] = frag<MAJB>(Bs, wn * 64 + i * 16 + l16, s, lg);
                 }
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i < 4; ++i)
-    #pragma unroll
+#pragma unroll
                     for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], bb[j], acc[i][j]);
             }
         };
@@ -994,17 +994,17 @@ __device__ __forceinline__ void dmas_block(const bxh_problem_t &P, const bxh_seg
             more = w.valid();
             if (more) issue(buf ^ 1);
             const float *As = stageA(buf), *Bs = stageB(buf);
-    #pragma unroll
+#pragma unroll
             for (int s = 0; s < BK / 4; ++s) {
                 float a[4], bb[4];
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     a[i] = frag<MAJA>(As, wm * 64 + i * 16 + l16, s, lg);
                     bb[i] = frag<MAJB>(Bs, wn * 64 + i * 16 + l16, s, lg);
                 }
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i < 4; ++i)
-    #pragma unroll
+#pragma unroll
                     for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], bb[j], acc[i][j]);
             }
             dma64::wait_dma();
@@ -1072,17 +1072,17 @@ __device__ __forceinline__ void dmaz_block(const bxh_problem_t &P, const bxh_seg
         };
         auto multiply = [&](int b) {
             const cplx *As = stageA(b), *Bs = stageB(b);
-    #pragma unroll
+#pragma unroll
             for (int s = 0; s < BK / 4; ++s) {
                 cplx a[4], bb[2];
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i < 4; ++i) a[i] = frag<MAJA, BM, CJA>(As, wm * 64 + i * 16 + l16, s, lg);
-    #pragma unroll
+#pragma unroll
                 for (int j = 0; j < 2; ++j) bb[j] = frag<MAJB, BN, CJB>(Bs, wn * 32 + j * 16 + l16, s, lg);
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const double nai = -a[i].im;
-    #pragma unroll
+#pragma unroll
                     for (int j = 0; j < 2; ++j) {
                         acc[i][j][0] = mfma(a[i].re, bb[j].re, acc[i][j][0]);
                         acc[i][j][1] = mfma(a[i].re, bb[j].im, acc[i][j][1]);
@@ -1144,17 +1144,17 @@ __device__ __forceinline__ void dmaz_block(const bxh_problem_t &P, const bxh_seg
             more = w.valid();
             if (more) issue(buf ^ 1);
             const cplx *As = stageA(buf), *Bs = stageB(buf);
-    #pragma unroll
+#pragma unroll
             for (int s = 0; s < BK / 4; ++s) {
                 cplx a[4], bb[2];
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i < 4; ++i) a[i] = frag<MAJA, BM, CJA>(As, wm * 64 + i * 16 + l16, s, lg);
-    #pragma unroll
+#pragma unroll
                 for (int j = 0; j < 2; ++j) bb[j] = frag<MAJB, BN, CJB>(Bs, wn * 32 + j * 16 + l16, s, lg);
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const double nai = -a[i].im;
-    #pragma unroll
+#pragma unroll
                     for (int j = 0; j < 2; ++j) {
                         acc[i][j][0] = mfma(a[i].re, bb[j].re, acc[i][j][0]);
                         acc[i][j][1] = mfma(a[i].re, bb[j].im, acc[i][j][1]);
