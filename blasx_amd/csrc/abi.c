@@ -10,13 +10,20 @@
  * 1000 (S, D) / 500 (Z, C) goes to the GPU, else the CPU BLAS"
  * (blas/dgemm.c:61-64, blas/zgemm.c:59-62), blocking semantics.  CGEMM,
  * which the reference keeps on the CPU (blas/cgemm.c:59-60), runs on the GPU
- * too, as one real GEMM on the FP32 matrix cores (bx_gpu_cgemm, level3.c).
+ * too, as one real GEMM on the FP32 matrix cores (bx_gpu_gemm_incore,
+ * level3.c).  Operands already in HBM are used in place, whatever the size
+ * of the call, and are never handed to the CPU BLAS.
+ *
+ * Failure policy (the reference asserts on device errors): a GPU-sized call
+ * whose GPU path fails is logged and rerun on the CPU BLAS, unless an
+ * operand is in HBM or BLASX_CPU_FALLBACK=0 (bx_gpu_failed, core.c).
  *
  * Deliberate deviations (netlib/OpenBLAS CBLAS semantics, where the
  * reference's GPU path is wrong): RowMajor is computed as
  * C^T = op(B)^T op(A)^T (the reference ignores Order on its GPU path);
  * alpha == 0 or K == 0 gives C = beta*C without reading A and B; beta == 0
- * does not read C; offsets are 64-bit.
+ * does not read C; offsets are 64-bit.  Kept as the reference has it: two bad
+ * f77 trans characters report info 2.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -125,15 +132,24 @@ static void gemm_entry(int dt, int order, int TransA, int TransB, int M, int N, 
     double al[2], be[2];
     to_double2(dt, alpha, al);
     to_double2(dt, beta, be);
-    if (K == 0 || (al[0] == 0.0 && al[1] == 0.0)) {
-        host_scale(dt, m, n, be, C, ldc);
-        return;
-    }
+    const int no_product = K == 0 || (al[0] == 0.0 && al[1] == 0.0);
+    if (no_product && be[0] == 1.0 && be[1] == 0.0) return; /* C unchanged */
+
+    /* Operands already in HBM (device pointers) never reach host code: the
+     * CPU BLAS and host_scale would dereference them.  A and B count only
+     * when the product reads them. */
+    const int c_hbm = bx_in_hbm(C);
+    const int ab_hbm = !no_product && (bx_in_hbm(a) || bx_in_hbm(b));
+    const int hbm = c_hbm || ab_hbm;
 
     bx_config();
     const int thr = (dt == BLASX_Z || dt == BLASX_C) ? g_bx.min_dim_cplx : g_bx.min_dim_real;
     const int gpu_sized = m > thr || n > thr || K > thr;
-    if (!gpu_sized) {
+    if (!hbm && no_product) {
+        host_scale(dt, m, n, be, C, ldc);
+        return;
+    }
+    if (!hbm && !gpu_sized) {
         if (bx_cpu_gemm(dt, order, TransA, TransB, M, N, K, alpha, A, lda, B, ldb, beta, C,
                         ldc) != 0)
             bx_fatal("%s: no CPU BLAS could be loaded for a CPU-sized call (set BLASX_CPUBLAS)",
@@ -144,21 +160,23 @@ static void gemm_entry(int dt, int order, int TransA, int TransB, int M, int N, 
     int rc = bx_ensure_init();
     if (rc == 0) {
         const int opa = op_code(dt, ta), opb = op_code(dt, tb);
-        if (dt == BLASX_C) {
-            rc = bx_gpu_cgemm(ta, tb, m, n, K, al, a, la, b, lb, be, C, ldc);
-            if (rc == BLASX_ENOMEM) {
-                /* operands beyond HBM: the CPU BLAS, as for any routine that runs in core */
+        if (dt != BLASX_C && c_hbm && (no_product || (bx_in_hbm(a) && bx_in_hbm(b)))) {
+            /* every operand the call reads is in HBM: run in place */
+            rc = bx_device_gemm(dt, ta, tb, m, n, no_product ? 0 : K, al, a, la, b, lb, be, C,
+                                ldc, NULL);
+            if (rc == 0) rc = bxh_stream_sync(g_bx.s_comp);
+        } else if (dt == BLASX_C || hbm) {
+            /* complex64, or operands split between host and HBM: in core */
+            rc = bx_gpu_gemm_incore(dt, ta, tb, m, n, no_product ? 0 : K, al, a, la, b, lb, be, C,
+                                    ldc);
+            if (rc == BLASX_ENOMEM && !hbm) {
+                /* host operands beyond HBM: the CPU BLAS, as for any routine run in core */
                 bx_log(1, "%s: operands do not fit in HBM; running on the CPU BLAS", name);
                 if (bx_cpu_gemm(dt, order, TransA, TransB, M, N, K, alpha, A, lda, B, ldb, beta,
                                 C, ldc) != 0)
                     bx_fatal("%s: no CPU BLAS could be loaded (set BLASX_CPUBLAS)", name);
                 return;
             }
-        } else if (bxh_pointer_is_pinned(a) == 2 && bxh_pointer_is_pinned(b) == 2 &&
-                   bxh_pointer_is_pinned(C) == 2) {
-            /* operands already in HBM: run in place */
-            rc = bx_device_gemm(dt, ta, tb, m, n, K, al, a, la, b, lb, be, C, ldc, NULL);
-            if (rc == 0) rc = bxh_stream_sync(g_bx.s_comp);
         } else {
             rc = bx_tile_gemm(dt, opa, opb, m, n, K, al, a, la, b, lb, be, C, ldc);
         }
@@ -167,18 +185,9 @@ static void gemm_entry(int dt, int order, int TransA, int TransB, int M, int N, 
             return;
         }
     }
-    if (g_bx.cpu_fallback) {
-        bx_log(0, "%s: GPU path failed (%d: %s); BLASX_CPU_FALLBACK=1 -> CPU BLAS", name, rc,
-               bxh_last_error());
-        BX_STAT_ADD(cpu_fallbacks, 1);
-        if (bx_cpu_gemm(dt, order, TransA, TransB, M, N, K, alpha, A, lda, B, ldb, beta, C,
-                        ldc) != 0)
-            bx_fatal("%s: CPU fallback requested but no CPU BLAS could be loaded", name);
-        return;
-    }
-    bx_fatal("%s: GPU path unavailable or failed (code %d: %s). Set BLASX_CPU_FALLBACK=1 to "
-             "run GPU-sized calls on the CPU BLAS instead.",
-             name, rc, bxh_last_error());
+    if (bx_gpu_failed(name, rc, !hbm) &&
+        bx_cpu_gemm(dt, order, TransA, TransB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc) != 0)
+        bx_fatal("%s: no CPU BLAS could be loaded for the fallback (set BLASX_CPUBLAS)", name);
 }
 
 /* ------------------------------------------------------------ CBLAS */
@@ -229,13 +238,15 @@ static int f77_trans(const char *c, int *out)
     }
 }
 
-/* info 1 / 2 for a bad trans character (blas/dgemm.c:143-149); otherwise
- * forwards to the ColMajor CBLAS entry */
+/* info 1 / 2 for a bad trans character; both checks run and the later one
+ * wins, so two bad characters report 2 (blas/dgemm.c:141-147,
+ * blas/sgemm.c:140-146, blas/zgemm.c:143-149, blas/cgemm.c:130-135);
+ * otherwise forwards to the ColMajor CBLAS entry */
 static int f77_prologue(int dt, const char *transa, const char *transb, int *ta, int *tb)
 {
     int info = -1;
     if (f77_trans(transa, ta) < 0) info = 1;
-    else if (f77_trans(transb, tb) < 0) info = 2;
+    if (f77_trans(transb, tb) < 0) info = 2;
     if (info >= 0) {
         xerbla_(dt_name(dt), &info);
         return -1;

@@ -64,12 +64,41 @@ void bx_fatal(const char *fmt, ...)
     abort();
 }
 
+/* A GPU-sized call whose GPU path failed with rc.  Returns 1 when the caller
+ * is to run the CPU BLAS instead: the failure is logged (always printed) and
+ * counted in cpu_fallbacks.  That is the default policy.  It cannot apply when
+ * an operand is in HBM (the CPU BLAS cannot read it: host_operands == 0) or
+ * when BLASX_CPU_FALLBACK=0 asks for the strict policy; both end the process
+ * with a message naming the routine and the failing HIP call. */
+int bx_gpu_failed(const char *name, int rc, int host_operands)
+{
+    if (host_operands && g_bx.cpu_fallback) {
+        bx_log(0, "%s: GPU path failed (code %d: %s); running on the CPU BLAS", name, rc,
+               bxh_last_error());
+        BX_STAT_ADD(cpu_fallbacks, 1);
+        return 1;
+    }
+    if (!host_operands)
+        bx_fatal("%s: GPU path failed (code %d: %s) on operands in HBM, which the CPU BLAS "
+                 "cannot read",
+                 name, rc, bxh_last_error());
+    bx_fatal("%s: GPU path failed (code %d: %s) and BLASX_CPU_FALLBACK=0 forbids the CPU BLAS",
+             name, rc, bxh_last_error());
+}
+
+/* 1 if p is device memory (HBM) of the GPU this library drives; only asked
+ * once that GPU is up, so a machine without one sees host memory only */
+int bx_in_hbm(const void *p)
+{
+    return p != NULL && bx_ensure_init() == 0 && bxh_pointer_is_pinned(p) == 2;
+}
+
 static void read_config(void)
 {
     g_bx.log_level = (int)env_long("BLASX_LOG", 0);
     g_bx.min_dim_real = (int)env_long("BLASX_GPU_MIN_DIM", 1000);
     g_bx.min_dim_cplx = (int)env_long("BLASX_GPU_MIN_DIM_Z", 500);
-    g_bx.cpu_fallback = (int)env_long("BLASX_CPU_FALLBACK", 0);
+    g_bx.cpu_fallback = (int)env_long("BLASX_CPU_FALLBACK", 1);
     long t = env_long("BLASX_TILE", 0);
     for (int i = 0; i < 4; ++i) g_bx.tile_override[i] = (int)t;
     g_bx.hbm_budget = (size_t)env_long("BLASX_HBM_BUDGET", 0);

@@ -37,7 +37,7 @@ typedef struct bx_ctx {
     int tile_override[4];    /* per dtype, 0 = autotune */
     int min_dim_real;        /* dispatch threshold, S and D */
     int min_dim_cplx;        /* dispatch threshold, Z */
-    int cpu_fallback;        /* BLASX_CPU_FALLBACK=1 */
+    int cpu_fallback;        /* BLASX_CPU_FALLBACK (default 1: log, then the CPU BLAS) */
     int log_level;           /* BLASX_LOG */
     int stage_threads;       /* host threads for pageable staging copies */
 } bx_ctx_t;
@@ -48,6 +48,10 @@ void bx_config(void);                /* read the environment once */
 int bx_ensure_init(void);            /* returns 0 if the GPU path is usable */
 void bx_log(int level, const char *fmt, ...);
 void bx_fatal(const char *fmt, ...) __attribute__((noreturn));
+/* failure policy of a GPU-sized call: 1 = run the CPU BLAS (logged, counted),
+ * else the process ends with a message (operands in HBM, or strict policy) */
+int bx_gpu_failed(const char *name, int rc, int host_operands);
+int bx_in_hbm(const void *p);        /* device memory of the library's GPU */
 
 /* ---------------- statistics (core.c) ---------------- */
 typedef struct bx_stats_atomic {
@@ -135,8 +139,11 @@ int bx_device_syr2k(int dt, int tri, int opa, int N, int K, const double *alpha,
                     int64_t lda, const void *B, int64_t ldb, const void *B2, int64_t ldb2,
                     const double *beta, void *C, int64_t ldc, void *stream);
 bx_cache_t *bx_global_cache(void);
-int bx_gpu_cgemm(int transa, int transb, int M, int N, int K, const double *al, const void *A,
-                 int lda, const void *B, int ldb, const double *be, void *C, int ldc);
+/* a GEMM run in core on HBM: host operands copied whole for the call,
+ * operands in HBM used in place (level3.c); complex64 always comes here */
+int bx_gpu_gemm_incore(int dt, int transa, int transb, int M, int N, int K, const double *al,
+                       const void *A, int lda, const void *B, int ldb, const double *be, void *C,
+                       int ldc);
 int bx_gpu_crank(int herm, int uplo, int trans, int N, int K, const double *al, const void *A,
                  int lda, const void *B, int ldb, const double *be, void *C, int ldc);
 /* C = alpha op(A) op(B) + beta C on HBM operands (transa, transb: CBLAS enums) */

@@ -39,11 +39,11 @@
  * already in HBM are used in place.
  *
  * CGEMM (complex64, the ABI in abi.c) also runs here: one real GEMM on the
- * FP32 matrix cores over packed operands (bx_gpu_cgemm).
+ * FP32 matrix cores over packed operands (bx_gpu_gemm_incore), and so does
+ * a GEMM of any type with an operand already in HBM.
  *
  * Deviations (netlib semantics): RowMajor is computed; ZSYR2K rejects
- * ConjTrans (info 2); ZHER2K reports itself as "ZHER2K"; f77 argument errors
- * report the lowest parameter number.
+ * ConjTrans (info 2); ZHER2K reports itself as "ZHER2K".
  */
 #include <pthread.h>
 #include <stdlib.h>
@@ -184,11 +184,16 @@ static int gemm_any(int dt, cwork_t *w, int ta, int tb, int M, int N, int K, con
     return bx_device_gemm(dt, ta, tb, M, N, K, al, A, lda, B, ldb, be, C, ldc, g_bx.s_comp);
 }
 
-/* CGEMM through the ABI (abi.c): operands and C as for the routines here */
-int bx_gpu_cgemm(int transa, int transb, int M, int N, int K, const double *al, const void *A,
-                 int lda, const void *B, int ldb, const double *be, void *C, int ldc)
+/* A GEMM through the ABI (abi.c) run in core: complex64 (on the FP32 matrix
+ * cores), or any type when an operand is already in HBM.  Operands and C as
+ * for the routines here: used in place when in HBM, else copied whole for
+ * the call.  alpha == 0 or K == 0 reads neither A nor B, and beta == 0 does
+ * not read C (C in HBM is then scaled or zeroed on the device). */
+int bx_gpu_gemm_incore(int dt, int transa, int transb, int M, int N, int K, const double *al,
+                       const void *A, int lda, const void *B, int ldb, const double *be, void *C,
+                       int ldc)
 {
-    const int es = 8;
+    const int es = bx_esize(dt);
     const int ra = transa == CblasNoTrans ? M : K, ca = transa == CblasNoTrans ? K : M;
     const int rb = transb == CblasNoTrans ? K : N, cb = transb == CblasNoTrans ? N : K;
     const int bz = be[0] == 0.0 && be[1] == 0.0;
@@ -202,10 +207,15 @@ int bx_gpu_cgemm(int transa, int transb, int M, int N, int K, const double *al, 
     if (product) {
         rc = dev_view(&s, A, ra, ca, lda, es, 1, &dA, &lA);
         if (!rc) rc = dev_view(&s, B, rb, cb, ldb, es, 1, &dB, &lB);
-        if (!rc) rc = cwork_alloc(&s, &w, (size_t)M * 2 * K, (size_t)4 * K * N, (size_t)M * 2 * N);
+        if (!rc && dt == BLASX_C)
+            rc = cwork_alloc(&s, &w, (size_t)M * 2 * K, (size_t)4 * K * N, (size_t)M * 2 * N);
     }
     if (!rc) rc = dev_view(&s, C, M, N, ldc, es, !bz, &dC, &lC);
-    if (!rc) rc = cgemm_dev(&w, transa, transb, M, N, K, al, dA, lA, dB, lB, be, dC, lC);
+    if (!rc && !product) /* C = beta C: shapes that pass the device API's checks with K = 0 */
+        rc = gemm_any(dt, &w, CblasNoTrans, CblasNoTrans, M, N, 0, al, NULL, M > 1 ? M : 1, NULL,
+                      1, be, dC, lC);
+    else if (!rc)
+        rc = gemm_any(dt, &w, transa, transb, M, N, K, al, dA, lA, dB, lB, be, dC, lC);
     if (!rc) rc = dev_store(C, ldc, dC, lC, M, N, es);
     return finish(&s, rc);
 }
@@ -525,27 +535,20 @@ static int gpu_sized(int dt, int a, int b)
     return a > thr || b > thr;
 }
 
-/* after a GPU attempt: done, or the CPU BLAS (operands beyond HBM, or the
- * BLASX_CPU_FALLBACK opt-in), or a loud stop */
-static int gpu_outcome(const char *name, int rc)
+/* after a GPU attempt: done, or the CPU BLAS (host operands beyond HBM, or
+ * a failure under the default policy), or a loud stop (bx_gpu_failed) */
+static int gpu_outcome(const char *name, int rc, const void *A, const void *B, const void *C)
 {
     if (rc == 0) {
         BX_STAT_ADD(calls_gpu, 1);
         return 0;
     }
-    if (rc == BLASX_ENOMEM) {
+    const int host = !bx_in_hbm(A) && !bx_in_hbm(B) && !bx_in_hbm(C);
+    if (rc == BLASX_ENOMEM && host) {
         bx_log(1, "%s: operands do not fit in HBM; running on the CPU BLAS", name);
         return 1;
     }
-    if (g_bx.cpu_fallback) {
-        bx_log(0, "%s: GPU path failed (%d: %s); BLASX_CPU_FALLBACK=1 -> CPU BLAS", name, rc,
-               bxh_last_error());
-        BX_STAT_ADD(cpu_fallbacks, 1);
-        return 1;
-    }
-    bx_fatal("%s: GPU path unavailable or failed (code %d: %s). Set BLASX_CPU_FALLBACK=1 to "
-             "run GPU-sized calls on the CPU BLAS instead.",
-             name, rc, bxh_last_error());
+    return bx_gpu_failed(name, rc, host);
 }
 
 static void need_cpublas(const char *name)
@@ -597,7 +600,7 @@ static void symm_entry(int dt, int herm, int order, int Side, int Uplo, int M, i
     if (gpu_sized(dt, m, n)) {
         int rc = bx_ensure_init();
         if (!rc) rc = gpu_symm(dt, herm, side, uplo, m, n, al, A, lda, B, ldb, be, C, ldc);
-        if (!gpu_outcome(name, rc)) return;
+        if (!gpu_outcome(name, rc, A, B, C)) return;
     }
     need_cpublas(name);
     if (herm && dt == BLASX_C)
@@ -655,7 +658,7 @@ static void trmm_entry(int dt, int order, int Side, int Uplo, int TransA, int Di
     if (gpu_sized(dt, m, n)) {
         int rc = bx_ensure_init();
         if (!rc) rc = gpu_trmm(dt, side, uplo, TransA, Diag, m, n, al, A, lda, B, ldb);
-        if (!gpu_outcome(name, rc)) return;
+        if (!gpu_outcome(name, rc, A, B, NULL)) return;
     }
     need_cpublas(name);
     if (dt == BLASX_S)
@@ -708,7 +711,7 @@ static void trsm_entry(int dt, int order, int Side, int Uplo, int TransA, int Di
     if (gpu_sized(dt, m, n)) {
         int rc = bx_ensure_init();
         if (!rc) rc = gpu_trsm(dt, side, uplo, TransA, Diag, m, n, al, A, lda, B, ldb);
-        if (!gpu_outcome(name, rc)) return;
+        if (!gpu_outcome(name, rc, A, B, NULL)) return;
     }
     need_cpublas(name);
     if (dt == BLASX_S)
@@ -770,7 +773,7 @@ static void syr2k_entry(int dt, int order, int Uplo, int Trans, int N, int K, co
             rc = dt == BLASX_C
                      ? bx_gpu_crank(0, uplo, trans, N, K, al, A, lda, B, ldb, be, C, ldc)
                      : gpu_syr2k(dt, uplo, trans, N, K, al, A, lda, B, ldb, be, C, ldc);
-        if (!gpu_outcome(name, rc)) return;
+        if (!gpu_outcome(name, rc, A, B, C)) return;
     }
     need_cpublas(name);
     if (dt == BLASX_S)
@@ -830,7 +833,7 @@ static void her2k_entry(int dt, int order, int Uplo, int Trans, int N, int K, co
             rc = dt == BLASX_C ? bx_gpu_crank(1, uplo, trans, N, K, al, A, lda, B, ldb, be, C, ldc)
                                : gpu_her2k(uplo, trans, N, K, al, A, lda, B, ldb, beta, C, ldc);
         }
-        if (!gpu_outcome(name, rc)) return;
+        if (!gpu_outcome(name, rc, A, B, C)) return;
     }
     need_cpublas(name);
     if (dt == BLASX_C)
@@ -1040,12 +1043,24 @@ static int f77_diag(const char *c, int *out)
     }
 }
 
-/* a bad character argument: xerbla_ with its position (blas/dsymm.c:83-84,
- * blas/dtrmm.c:76-80, blas/dsyr2k.c:97-98; the lowest position wins) */
+/* a bad character argument: xerbla_ with its position.  SYMM, TRMM and TRSM
+ * check their characters last to first, so the lowest position wins
+ * (blas/dsymm.c:83-84, blas/dtrmm.c:76-80, blas/dtrsm.c:86-90) */
 static int f77_bad(const char *name, int info)
 {
     xerbla_(name, &info);
     return 1;
+}
+
+/* SYR2K and HER2K check UPLO then TRANS, and the later check wins: two bad
+ * characters report 2 (blas/dsyr2k.c:96-98, blas/zher2k.c:97-99) */
+static int f77_uplo_trans(const char *name, const char *uplo, const char *trans, int *u, int *t)
+{
+    int info = 0;
+    if (f77_uplo(uplo, u)) info = 1;
+    if (f77_trans(trans, t)) info = 2;
+    if (info) xerbla_(name, &info);
+    return info;
 }
 
 #define F77_SYMM(NAME, DT, HERM, TNAME, T)                                                     \
@@ -1102,8 +1117,7 @@ F77_TRSM(ztrsm_, BLASX_Z, "ZTRSM ", void)
                         const T *beta, T *C, const int *ldc)                                    \
     {                                                                                          \
         int u, t;                                                                              \
-        if (f77_uplo(uplo, &u) && f77_bad(TNAME, 1)) return;                                   \
-        if (f77_trans(trans, &t) && f77_bad(TNAME, 2)) return;                                 \
+        if (f77_uplo_trans(TNAME, uplo, trans, &u, &t)) return;                                \
         syr2k_entry(DT, CblasColMajor, u, t, *n, *k, alpha, A, *lda, B, *ldb, beta, C, *ldc);  \
     }
 BX_EXPORT void zher2k_(const char *uplo, const char *trans, const int *n, const int *k,
@@ -1111,8 +1125,7 @@ BX_EXPORT void zher2k_(const char *uplo, const char *trans, const int *n, const 
                        const int *ldb, const double *beta, void *C, const int *ldc)
 {
     int u, t;
-    if (f77_uplo(uplo, &u) && f77_bad("ZHER2K", 1)) return;
-    if (f77_trans(trans, &t) && f77_bad("ZHER2K", 2)) return;
+    if (f77_uplo_trans("ZHER2K", uplo, trans, &u, &t)) return;
     her2k_entry(BLASX_Z, CblasColMajor, u, t, *n, *k, alpha, A, *lda, B, *ldb, *beta, C, *ldc);
 }
 
@@ -1121,8 +1134,7 @@ BX_EXPORT void cher2k_(const char *uplo, const char *trans, const int *n, const 
                        const int *ldb, const float *beta, void *C, const int *ldc)
 {
     int u, t;
-    if (f77_uplo(uplo, &u) && f77_bad("CHER2K", 1)) return;
-    if (f77_trans(trans, &t) && f77_bad("CHER2K", 2)) return;
+    if (f77_uplo_trans("CHER2K", uplo, trans, &u, &t)) return;
     her2k_entry(BLASX_C, CblasColMajor, u, t, *n, *k, alpha, A, *lda, B, *ldb, *beta, C, *ldc);
 }
 

@@ -301,17 +301,9 @@ static void syrk_entry(int dt, int herk, int order, int Uplo, int Trans, int N, 
             return;
         }
     }
-    if (g_bx.cpu_fallback) {
-        bx_log(0, "%s: GPU path failed (%d: %s); BLASX_CPU_FALLBACK=1 -> CPU BLAS", name, rc,
-               bxh_last_error());
-        BX_STAT_ADD(cpu_fallbacks, 1);
-        if (cpu_syrk(dt, herk, order, Uplo, Trans, N, K, alpha, A, lda, beta, C, ldc) != 0)
-            bx_fatal("%s: CPU fallback requested but no CPU BLAS could be loaded", name);
-        return;
-    }
-    bx_fatal("%s: GPU path unavailable or failed (code %d: %s). Set BLASX_CPU_FALLBACK=1 to "
-             "run GPU-sized calls on the CPU BLAS instead.",
-             name, rc, bxh_last_error());
+    if (bx_gpu_failed(name, rc, !bx_in_hbm(A) && !bx_in_hbm(C)) &&
+        cpu_syrk(dt, herk, order, Uplo, Trans, N, K, alpha, A, lda, beta, C, ldc) != 0)
+        bx_fatal("%s: no CPU BLAS could be loaded for the fallback (set BLASX_CPUBLAS)", name);
 }
 
 BX_EXPORT void cblas_ssyrk(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
@@ -362,8 +354,9 @@ BX_EXPORT void cblas_cherk(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO U
     syrk_entry(BLASX_C, 1, Order, Uplo, Trans, N, K, &alpha, A, lda, &beta, C, ldc);
 }
 
-/* f77: info 1 for a bad UPLO, 2 for a bad TRANS character
- * (blas/dsyrk.c:83-85), then the ColMajor CBLAS path */
+/* f77: info 1 for a bad UPLO, 2 for a bad TRANS character; both checks
+ * run and the later one wins, so two bad characters report 2
+ * (blas/dsyrk.c:83-85, blas/zherk.c:89-91), then the ColMajor CBLAS path */
 static int f77_syrk_args(const char *name, const char *uplo, const char *trans, int *u, int *t)
 {
     int info = 0;
@@ -376,8 +369,7 @@ static int f77_syrk_args(const char *name, const char *uplo, const char *trans, 
     case 'N': case 'n': *t = CblasNoTrans; break;
     case 'T': case 't': *t = CblasTrans; break;
     case 'C': case 'c': *t = CblasConjTrans; break;
-    default:
-        if (!info) info = 2;
+    default: info = 2;
     }
     if (info) xerbla_(name, &info);
     return info;

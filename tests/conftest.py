@@ -4,6 +4,9 @@ import sys
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+# strict failure policy for every test process: a GPU-sized call whose GPU path
+# fails ends the process instead of passing on the CPU BLAS
+os.environ.setdefault("BLASX_CPU_FALLBACK", "0")
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 

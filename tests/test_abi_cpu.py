@@ -43,7 +43,11 @@ def _run_c_snippet(body, env_extra=None, expect_ok=True):
     code = ("import sys, ctypes as C, numpy as np; sys.path.insert(0, %r)\n"
             "import blasx_amd as bx\nL = bx.lib()\n" % ROOT) + body
     env = dict(os.environ)
-    env.update(env_extra or {})
+    for k, v in (env_extra or {}).items():
+        if v is None:
+            env.pop(k, None)
+        else:
+            env[k] = v
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env,
                        timeout=300)
     if expect_ok:
@@ -155,18 +159,43 @@ def test_row_major_cpu_path(bx):
     assert np.allclose(out, 2.0 * A @ B.T - C0, atol=1e-12)
 
 
-def test_gpu_sized_call_without_gpu_is_loud(bx):
+def test_gpu_sized_call_without_gpu_falls_back_loudly(bx):
+    """The failure policy: a GPU-sized call whose GPU path fails is logged and
+    rerun on the CPU BLAS by default; BLASX_CPU_FALLBACK=0 makes it fatal."""
     if bx.gpu_available():
         pytest.skip("a GPU is present; this checks the no-GPU behaviour")
     body = ("n = 1100; A = np.ones((n, n), order='F'); C_ = np.zeros((n, n), order='F')\n"
             "bx.gemm('N', 'N', 1.0, A, A, 0.0, C_)\n")
-    r = _run_c_snippet(body, expect_ok=False)
+    check = ("st = bx.stats(); assert st['cpu_fallbacks'] == 1 and st['calls_gpu'] == 0, st\n"
+             "assert np.all(C_ == 1100.0)\n")
+    r = _run_c_snippet(body + check, env_extra={"BLASX_CPU_FALLBACK": None})
+    assert "DGEMM : GPU path failed" in r.stderr and "running on the CPU BLAS" in r.stderr
+    r = _run_c_snippet(body, env_extra={"BLASX_CPU_FALLBACK": "0"}, expect_ok=False)
     assert r.returncode != 0
-    assert "GPU path unavailable" in r.stderr
-    # with the explicit opt-in the call runs on the CPU BLAS and is counted
-    body2 = body + ("st = bx.stats(); assert st['cpu_fallbacks'] == 1, st\n"
-                    "assert np.all(C_ == 1100.0)\n")
-    _run_c_snippet(body2, env_extra={"BLASX_CPU_FALLBACK": "1"})
+    assert "BLASX_CPU_FALLBACK=0 forbids the CPU BLAS" in r.stderr
+
+
+def test_f77_two_bad_characters_report_the_later_check(bx):
+    """Both checks run and the later wins: blas/dgemm.c:141-147 (GEMM),
+    blas/dsyrk.c:83-85 (SYRK/HERK), blas/dsyr2k.c:96-98 (SYR2K/HER2K)."""
+    body = ("one = C.c_int(1); buf = np.zeros(16)\n"
+            "for p in 'sdcz':\n"
+            "    getattr(L, p + 'gemm_')(b'X', b'Q', C.byref(one), C.byref(one), C.byref(one),"
+            " buf.ctypes.data, buf.ctypes.data, C.byref(one), buf.ctypes.data, C.byref(one),"
+            " buf.ctypes.data, buf.ctypes.data, C.byref(one))\n"
+            "for r in ('dsyrk_', 'zherk_'):\n"
+            "    getattr(L, r)(b'X', b'Q', C.byref(one), C.byref(one), buf.ctypes.data,"
+            " buf.ctypes.data, C.byref(one), buf.ctypes.data, buf.ctypes.data, C.byref(one))\n"
+            "for r in ('dsyr2k_', 'zher2k_'):\n"
+            "    getattr(L, r)(b'X', b'Q', C.byref(one), C.byref(one), buf.ctypes.data,"
+            " buf.ctypes.data, C.byref(one), buf.ctypes.data, C.byref(one), buf.ctypes.data,"
+            " buf.ctypes.data, C.byref(one))\n"
+            "assert np.all(buf == 0)\n")
+    r = _run_c_snippet(body)
+    lines = [x for x in r.stdout.splitlines() if x.strip()]
+    names = ["SGEMM ", "DGEMM ", "CGEMM ", "ZGEMM ", "DSYRK ", " ZHERK", "DSYR2K", "ZHER2K"]
+    assert lines == [" ** On entry to %6s parameter number  2 had an illegal value" % n
+                     for n in names]
 
 
 def test_host_generator_matches_oracle(bx, orc):

@@ -137,6 +137,57 @@ def test_naive_gemm_matches_numpy(orc, ta, tb):
         assert np.max(np.abs(out - ref)) <= 4 * K * eps * (np.abs(ref).max() + 1)
 
 
+@pytest.mark.parametrize("ta", ["N", "T"])
+@pytest.mark.parametrize("tb", ["N", "T"])
+def test_gemm_s_acc64_is_one_rounding_of_the_fp64_product(orc, ta, tb):
+    """orc_gemm_s_acc64 (the FP32 reference of the SGEMM and grouped checks):
+    float products are exact in FP64, so the result must be the FP64 product
+    of the same float32 inputs rounded once to float32, up to K*eps64 of the
+    accumulation.  A float32 accumulator (error ~K*eps32) fails this."""
+    rng = np.random.default_rng(11)
+    M, N, K = 21, 18, 257
+    A = _stored(rng, M, K, np.float32, ta)
+    B = _stored(rng, K, N, np.float32, tb)
+    C0 = np.asfortranarray(_stored(rng, M, N, np.float32, "N"))
+    for alpha, beta in ((1.5, 0.5), (-0.75, 0.0)):
+        ref = (alpha * (_op(A.astype(np.float64), ta) @ _op(B.astype(np.float64), tb))
+               + beta * C0.astype(np.float64))
+        mag = (abs(alpha) * (np.abs(_op(A.astype(np.float64), ta)) @ np.abs(_op(B.astype(np.float64), tb)))
+               + abs(beta) * np.abs(C0.astype(np.float64)))
+        out = C0.copy(order="F")
+        orc.gemm(ta, tb, M, N, K, alpha, A, A.shape[0], B, B.shape[0], beta, out, M, acc64=True)
+        assert out.dtype == np.float32
+        err = np.abs(out.astype(np.float64) - ref)
+        assert np.all(err <= 0.5 * EPS32 * np.abs(ref) + 4 * K * EPS64 * mag)
+        # the plain float32 loop is measurably worse on the same inputs
+        plain = C0.copy(order="F")
+        orc.gemm(ta, tb, M, N, K, alpha, A, A.shape[0], B, B.shape[0], beta, plain, M)
+        assert np.max(np.abs(plain.astype(np.float64) - ref)) > np.max(err)
+
+
+@pytest.mark.parametrize("ta", ["N", "T"])
+@pytest.mark.parametrize("tb", ["N", "T"])
+def test_gemm_s_acc64_integer_exact(orc, ta, tb):
+    """Integers up to 2^11 with K = 31: every partial sum is exact in FP64
+    (not in FP32: sums reach 2^27), so the result is the exact integer
+    rounded once to float32, bit for bit; beta = 0 never reads C (NaN)."""
+    rng = np.random.default_rng(12)
+    M, N, K = 19, 23, 31
+    Ai = rng.integers(-2048, 2049, (M, K) if ta == "N" else (K, M))
+    Bi = rng.integers(-2048, 2049, (K, N) if tb == "N" else (N, K))
+    Ci = rng.integers(-4096, 4097, (M, N))
+    A = np.asfortranarray(Ai, dtype=np.float32)
+    B = np.asfortranarray(Bi, dtype=np.float32)
+    exact = 3 * (_op(Ai, ta) @ _op(Bi, tb)) - 2 * Ci
+    out = np.asfortranarray(Ci, dtype=np.float32)
+    orc.gemm(ta, tb, M, N, K, 3.0, A, A.shape[0], B, B.shape[0], -2.0, out, M, acc64=True)
+    assert np.array_equal(out, exact.astype(np.float32))
+    assert np.abs(exact).max() > 2 ** 24   # beyond float32's exact integers
+    out = np.full((M, N), np.nan, dtype=np.float32, order="F")
+    orc.gemm(ta, tb, M, N, K, 1.0, A, A.shape[0], B, B.shape[0], 0.0, out, M, acc64=True)
+    assert np.array_equal(out, (_op(Ai, ta) @ _op(Bi, tb)).astype(np.float32))
+
+
 def test_naive_gemm_integer_exact(orc):
     rng = np.random.default_rng(2)
     M, N, K = 23, 19, 31
