@@ -192,7 +192,26 @@ def lib():
     L.blasx_fill_dynrange_host.argtypes = [_vp, i64, i64, i64, C.c_uint64, C.c_uint32, d]
     L.blasx_dd_gemm_block_host.argtypes = [i, i, i, i, i, d, _vp, i64, _vp, i64, d, _vp, i64,
                                            _vp, i64]
-    # CPU-side test hooks
+    _lib = L
+    return L
+
+
+TEST_LIB_PATH = os.path.join(_HERE, "lib", "libblasx_test.so")
+_test_lib = None
+
+
+def test_lib():
+    """Load libblasx_test.so: the same library plus the CPU-side hooks
+    (tile cache over host memory, task queue, tile heuristic, CRT helpers)
+    that only the test-suite calls.  It has its own library state."""
+    global _test_lib
+    if _test_lib is not None:
+        return _test_lib
+    if not os.path.exists(TEST_LIB_PATH):
+        raise RuntimeError(f"{TEST_LIB_PATH} is missing: build it with "
+                           "`python -c 'import __graft_entry__ as g; g.build()'`")
+    L = C.CDLL(TEST_LIB_PATH)
+    i, i64 = C.c_int, C.c_int64
     L.bxtest_cache_create.restype = _vp
     L.bxtest_cache_create.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t]
     L.bxtest_cache_destroy.argtypes = [_vp]
@@ -214,7 +233,8 @@ def lib():
     L.bxtest_queue_next.restype = i64
     L.bxtest_queue_destroy.argtypes = [_vp]
     L.bxtest_autotune_tile.argtypes = [i, i, i, i]
-    _lib = L
+    L.blasx_tune_table_add.argtypes = [i, i, i, i, i]
+    _test_lib = L
     return L
 
 
@@ -237,7 +257,13 @@ def _scalar(code, v):
 
 
 def _addr(a):
-    return a.ctypes.data if isinstance(a, np.ndarray) else a
+    """The address a BLAS entry point receives: a numpy buffer (host), a
+    DeviceArray (HBM) or a raw integer address."""
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    if isinstance(a, DeviceArray):
+        return a.ptr
+    return a
 
 
 def cblas_gemm(order, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C_, ldc):

@@ -4,9 +4,17 @@
  * DMA engines can only read or write pinned host memory; HIP stages
  * pageable copies itself, synchronously and on one thread.  The tile
  * runtime instead packs a tile from the caller's pageable matrix into a
- * pinned staging buffer with this pool (columns split over the workers and
- * the calling thread), then DMAs it asynchronously; copy-out runs the other
- * way.  The pool's workers live for the life of the process.
+ * pinned staging buffer with this pool, then DMAs it asynchronously;
+ * copy-out runs the other way.  The pool's workers live for the life of the
+ * process.
+ *
+ * Work distribution is the reference's task queue (blas/blasx_task_queue.c:
+ * 3-17, bx_queue in queue.c): a job is cut into column chunks, and the
+ * workers and the calling thread take chunk ids from one shared
+ * compare-and-swap counter, highest first, until it reports -1 -- the
+ * reference hands C tiles to its GPU threads the same way
+ * (blas/blasx_dgemm.c:174-183).  A worker that is descheduled simply takes
+ * fewer chunks.
  */
 #include <pthread.h>
 #include <stdlib.h>
@@ -15,6 +23,7 @@
 #include "internal.h"
 
 #define MAXW 32
+#define CHUNK_BYTES ((size_t)1 << 20) /* about 1 MiB of columns per task */
 
 static struct {
     pthread_once_t once;
@@ -25,26 +34,33 @@ static struct {
     int pending;     /* workers still busy on the current job */
     char *dst;
     const char *src;
-    size_t dpitch, spitch, width, height;
+    size_t dpitch, spitch, width, height, cols_per_task;
+    bx_queue_t tasks; /* chunk ids of the current job */
 } P = {PTHREAD_ONCE_INIT, PTHREAD_MUTEX_INITIALIZER, PTHREAD_COND_INITIALIZER,
-       PTHREAD_COND_INITIALIZER, 0, 0, 0, NULL, NULL, 0, 0, 0, 0};
+       PTHREAD_COND_INITIALIZER, 0, 0, 0, NULL, NULL, 0, 0, 0, 0, 0, {0}};
 
-static void copy_range(size_t c0, size_t c1)
+/* take chunks until the queue is empty */
+static void drain(void)
 {
-    for (size_t c = c0; c < c1; ++c) memcpy(P.dst + c * P.dpitch, P.src + c * P.spitch, P.width);
+    for (int64_t t; (t = bx_queue_next(&P.tasks)) >= 0;) {
+        const size_t c0 = (size_t)t * P.cols_per_task;
+        size_t c1 = c0 + P.cols_per_task;
+        if (c1 > P.height) c1 = P.height;
+        for (size_t c = c0; c < c1; ++c)
+            memcpy(P.dst + c * P.dpitch, P.src + c * P.spitch, P.width);
+    }
 }
 
 static void *worker(void *arg)
 {
-    const int me = (int)(intptr_t)arg; /* 1..nw; part 0 is the caller's */
+    (void)arg;
     uint64_t seen = 0;
     pthread_mutex_lock(&P.mu);
     for (;;) {
         while (P.gen == seen) pthread_cond_wait(&P.go, &P.mu);
         seen = P.gen;
-        const size_t h = P.height, parts = (size_t)P.nw + 1;
         pthread_mutex_unlock(&P.mu);
-        copy_range(h * me / parts, h * (me + 1) / parts);
+        drain();
         pthread_mutex_lock(&P.mu);
         if (--P.pending == 0) pthread_cond_signal(&P.done);
     }
@@ -60,7 +76,7 @@ static void start(void)
     int started = 0;
     for (int i = 0; i < n; ++i) {
         pthread_t t;
-        if (pthread_create(&t, NULL, worker, (void *)(intptr_t)(i + 1)) != 0) break;
+        if (pthread_create(&t, NULL, worker, NULL) != 0) break;
         pthread_detach(t);
         ++started;
     }
@@ -87,11 +103,13 @@ void bx_host_copy2d(void *dst, size_t dpitch, const void *src, size_t spitch, si
     P.spitch = spitch;
     P.width = width;
     P.height = height;
+    P.cols_per_task = CHUNK_BYTES / width ? CHUNK_BYTES / width : 1;
+    bx_queue_init(&P.tasks, (int64_t)((height + P.cols_per_task - 1) / P.cols_per_task));
     P.pending = P.nw;
     P.gen++;
     pthread_cond_broadcast(&P.go);
     pthread_mutex_unlock(&P.mu);
-    copy_range(0, height / ((size_t)P.nw + 1));
+    drain();
     pthread_mutex_lock(&P.mu);
     while (P.pending > 0) pthread_cond_wait(&P.done, &P.mu);
     pthread_mutex_unlock(&P.mu);

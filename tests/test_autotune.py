@@ -7,7 +7,7 @@ import pytest
 
 
 def test_tune_table_lookup_cpu(bx):
-    L = bx.lib()
+    L = bx.test_lib()
     L.blasx_tune_table_add.argtypes = [C.c_int] * 5
     # a measured entry overrides the heuristic for nearby shapes only
     assert L.bxtest_autotune_tile(bx.D, 10000, 10000, 10000) == 2048  # heuristic

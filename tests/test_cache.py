@@ -17,7 +17,7 @@ SLOT = 1024
 
 class Cache:
     def __init__(self, bx, slots, chunk_slots=32):
-        self.L = bx.lib()
+        self.L = bx.test_lib()
         self.L.bxtest_cache_validate.argtypes = [C.c_void_p]
         self.L.bxtest_cache_set_budget.argtypes = [C.c_void_p, C.c_size_t]
         self.L.bxtest_cache_slot_key.argtypes = [C.c_void_p, C.c_int]
@@ -179,7 +179,7 @@ def test_index_agrees_with_reference_red_black_tree(bx, orc):
 
 
 def test_queue_exactly_once_under_threads(bx):
-    L = bx.lib()
+    L = bx.test_lib()
     n = 200000
     q = L.bxtest_queue_create(n)
     got = [[] for _ in range(8)]
@@ -208,7 +208,7 @@ def test_host_copy_pool_strided(bx):
     between pitched buffers exactly (multi-threaded above 1 MiB)."""
     import numpy as np
 
-    L = bx.lib()
+    L = bx.test_lib()
     L.bxtest_host_copy2d.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_size_t,
                                      C.c_size_t]
     rng = np.random.default_rng(3)
@@ -225,7 +225,7 @@ def test_queue_order_matches_reference(bx, orc):
     ref = orc.ref_lib()
     if ref is None:
         pytest.skip("oracle/_ref not built (no reference tree here)")
-    L = bx.lib()
+    L = bx.test_lib()
     for n in (1, 2, 17, 256):
         rq = (C.c_int * 1)()
         # the reference's blasx_dgemm initialises with task_num = tiles - 1

@@ -79,7 +79,7 @@ def test_moduli_pairwise_coprime_and_beta(orc, bx):
 @pytest.mark.parametrize("L", [2, 5, 8, 13, 16, 20])
 def test_reconstruction_constants_exact(orc, bx, L):
     M, W, Mk = _exact_consts(L)
-    Lb = bx.lib()
+    Lb = bx.test_lib()
     Lb.bxtest_crt_consts.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
     Wl = np.zeros(80)
     Ml = np.zeros(4)
@@ -103,7 +103,7 @@ def test_residue_helpers_match_exact_arithmetic(orc, bx):
     the library's helpers (host build of the kernel code) and the oracle's
     independent int64 version.  Non-finite values and magnitudes beyond the
     digits give residue 0."""
-    Lb = bx.lib()
+    Lb = bx.test_lib()
     Lb.bxtest_crt_resid.argtypes = [C.c_double, C.c_int]
     Lb.bxtest_crt_resid5.argtypes = [C.c_double, C.c_int]
     Lb.bxtest_crt_resid6.argtypes = [C.c_double, C.c_int]
@@ -153,7 +153,7 @@ def test_norm_exponent_matches_definition(orc, bx):
     4^e, from the binary exponent alone): the library's host build of the
     kernel code and the oracle against exact rational arithmetic, on powers
     of 4 and 2 and their neighbours, tiny and huge sums, zero and inf."""
-    Lb = bx.lib()
+    Lb = bx.test_lib()
     Lb.bxtest_crt_norm_exp.argtypes = [C.c_double]
     rng = random.Random(5)
     vals = [0.0, float("inf"), 5e-324, 1e-300, 1e300]

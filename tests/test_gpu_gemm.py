@@ -400,29 +400,67 @@ def test_out_of_core_small_budget(gpu, orc):
 
 
 def test_resident_operand_reused_across_calls(gpu, orc):
+    """Cross-call residency (blasx_resident_register): the registered A is
+    not sent again, while the unregistered B -- mutated in place between the
+    calls -- is re-read; unregistering A drops its tiles, so a changed A is
+    seen.  Integer inputs: every result must equal the exact product, and a
+    sampled block is also checked against the oracle's naive loop."""
     bx = gpu
     L = bx.lib()
     M = N = K = 2048
-    A = orc.fill_uniform(M, K, 8, 0)
-    B = orc.fill_uniform(K, N, 8, 1)
+    rng = np.random.default_rng(15)
+    A = np.asfortranarray(rng.integers(-64, 65, (M, K)), dtype=np.float64)
+    B = np.asfortranarray(rng.integers(-64, 65, (K, N)), dtype=np.float64)
+
+    def exact(A_, B_):   # integer sums below 2^53: any summation order is exact
+        return A_ @ B_
+
+    def sampled_oracle(C_, A_, B_):
+        rows, cols = np.arange(0, M, 97), np.arange(5, N, 89)
+        a = np.asfortranarray(A_[rows])
+        b = np.asfortranarray(B_[:, cols])
+        ref = np.zeros((len(rows), len(cols)), order="F")
+        orc.gemm("N", "N", len(rows), len(cols), K, 1.0, a, len(rows), b, K, 0.0, ref, len(rows))
+        assert np.array_equal(C_[np.ix_(rows, cols)], ref)
+
+    assert L.blasx_set_tile_size(bx.D, 512) == 0
     assert L.blasx_resident_register(A.ctypes.data, A.nbytes) == 0
+    registered = True
     try:
+        L.blasx_cache_flush()
         C1 = np.zeros((M, N), order="F")
-        C2 = np.zeros((M, N), order="F")
         bx.reset_stats()
         say("resident call 1")
         bx.gemm("N", "N", 1.0, A, B, 0.0, C1)
         s1 = bx.stats()
-        say("resident call 2")
+        assert np.array_equal(C1, exact(A, B))
+        sampled_oracle(C1, A, B)
+        assert s1["bytes_h2d"] >= A.nbytes + B.nbytes
+
+        B[:, ::3] += 1.0          # B is not registered: the next call must re-read it
+        C2 = np.zeros((M, N), order="F")
+        say("resident call 2 (B mutated)")
         bx.gemm("N", "N", 1.0, A, B, 0.0, C2)
         s2 = bx.stats()
-        assert np.array_equal(C1, C2)
+        assert np.array_equal(C2, exact(A, B))
+        sampled_oracle(C2, A, B)
         h2d_second = s2["bytes_h2d"] - s1["bytes_h2d"]
-        assert h2d_second <= s1["bytes_h2d"] - A.nbytes + 1, (s1, s2)
-        assert s2["tile_hits"] > s1["tile_hits"]
-    finally:
+        assert B.nbytes <= h2d_second < A.nbytes + B.nbytes, (s1, s2)
+        assert s2["tile_hits"] - s1["tile_hits"] >= (M // 512) * (K // 512)
+
         assert L.blasx_resident_unregister(A.ctypes.data) == 0
+        registered = False
+        A[::2] -= 1.0             # A's tiles were dropped on unregister: the change is seen
+        C3 = np.zeros((M, N), order="F")
+        say("resident call 3 (A unregistered and mutated)")
+        bx.gemm("N", "N", 1.0, A, B, 0.0, C3)
+        assert np.array_equal(C3, exact(A, B))
+        sampled_oracle(C3, A, B)
+    finally:
+        if registered:
+            assert L.blasx_resident_unregister(A.ctypes.data) == 0
         L.blasx_cache_flush()
+        L.blasx_set_tile_size(bx.D, 0)
 
 
 # ------------------------------------------------------------------ CGEMM
