@@ -192,6 +192,7 @@ def lib():
     L.blasx_fill_dynrange_host.argtypes = [_vp, i64, i64, i64, C.c_uint64, C.c_uint32, d]
     L.blasx_dd_gemm_block_host.argtypes = [i, i, i, i, i, d, _vp, i64, _vp, i64, d, _vp, i64,
                                            _vp, i64]
+    L.blasx_sample_uniform_host.argtypes = [i, _vp, i64, i, _vp, i, _vp, C.c_uint64, C.c_uint32]
     _lib = L
     return L
 
@@ -522,3 +523,16 @@ def fill_uniform_host(a, seed, mid):
     if rc:
         raise RuntimeError(f"blasx_fill_uniform_host failed ({rc})")
     return a
+
+
+def sample_uniform_host(rows, ri, ci, seed, mid, dtype=np.float64):
+    """Entries (ri[a], ci[b]) of the seeded matrix with `rows` rows, as a
+    (len(ri), len(ci)) Fortran array: what the fills hold there."""
+    ri = np.ascontiguousarray(ri, dtype=np.int64)
+    ci = np.ascontiguousarray(ci, dtype=np.int64)
+    out = np.zeros((len(ri), len(ci)), dtype=dtype, order="F")
+    rc = lib().blasx_sample_uniform_host(dtype_code(dtype), out.ctypes.data, rows, len(ri),
+                                         ri.ctypes.data, len(ci), ci.ctypes.data, seed, mid)
+    if rc:
+        raise RuntimeError(f"blasx_sample_uniform_host failed ({rc})")
+    return out

@@ -126,3 +126,15 @@ BX_EXPORT int bxtest_crt_resid5(double v, int m) { return bxh_crt_resid5_host(v,
 BX_EXPORT int bxtest_crt_resid6(double v, int m) { return bxh_crt_resid6_host(v, m); }
 BX_EXPORT int bxtest_crt_norm_exp(double ss) { return bxh_crt_norm_exp_host(ss); }
 BX_EXPORT int bxtest_crt_smod_i32(int x, int m) { return bxh_crt_smod_i32_host(x, m); }
+
+/* the tile runtime's round order and operand traffic, replayed on the host */
+int bx_plan_simulate(int dt, int opa, int opb, int M, int N, int K, int T, int tri, int same_ab,
+                     int cap_slots, uint64_t *op_bytes, uint64_t *evictions, int *mode,
+                     int *nbands);
+BX_EXPORT int bxtest_plan_simulate(int dt, int opa, int opb, int M, int N, int K, int T, int tri,
+                                   int same_ab, int cap_slots, uint64_t *op_bytes,
+                                   uint64_t *evictions, int *mode, int *nbands)
+{
+    return bx_plan_simulate(dt, opa, opb, M, N, K, T, tri, same_ab, cap_slots, op_bytes,
+                            evictions, mode, nbands);
+}

@@ -12,9 +12,12 @@
  * is fetched (blas/blasx_dgemm.c:42-53); edge tiles carry the remainder
  * extents; tiles are fetched through the device cache keyed on the host
  * address.  Redesigned for one GPU:
- *   * the work unit is a ROUND: an RM x RN block of C tiles, traversed
- *     block-row by block-row with alternating direction so the B panels of
- *     one block row are still cached when the next starts;
+ *   * the work unit is a ROUND: an RM x RN block of C tiles; in core the
+ *     rounds go block row by block row with alternating direction, out of
+ *     core in bands that keep one operand's panels resident and stream the
+ *     other past them (see plan_order), so operand bytes stay near the
+ *     reuse bound e(MK + KN) instead of refetching per round
+ *     (blas/blasx_dgemm.c:164-239 fetches every tile of every round);
  *   * one grouped kernel launch per round computes every C tile of the
  *     round over all of K (one segment per cached K tile), so C is read
  *     and written once per call, not once per K step with
@@ -242,6 +245,93 @@ static int tile_on_tri(const plan_t *p, int i, int k)
     return 1;
 }
 
+/* cache key of the stored tile holding op(A) tile (i, j) (block row i, K
+ * tile j); for a transposed operand that is stored tile (j, i)
+ * (blas/blasx_dgemm.c:42-53) */
+static bx_tile_key_t a_key(const plan_t *p, int i, int j)
+{
+    const int T = p->T;
+    const int mc = (p->M - i * T < T) ? p->M - i * T : T;
+    const int kext = (p->K - j * T < T) ? p->K - j * T : T;
+    const int n = p->opa == BX_OP_N;
+    const int64_t r0 = (int64_t)(n ? i : j) * T, c0 = (int64_t)(n ? j : i) * T;
+    bx_tile_key_t key = {(uintptr_t)(p->A + (r0 + c0 * p->lda) * p->es), p->lda,
+                         n ? mc : kext, n ? kext : mc, p->es, 0};
+    return key;
+}
+
+/* the same for op(B) tile (j, k) */
+static bx_tile_key_t b_key(const plan_t *p, int k, int j)
+{
+    const int T = p->T;
+    const int nc = (p->N - k * T < T) ? p->N - k * T : T;
+    const int kext = (p->K - j * T < T) ? p->K - j * T : T;
+    const int n = p->opb == BX_OP_N;
+    const int64_t r0 = (int64_t)(n ? j : k) * T, c0 = (int64_t)(n ? k : j) * T;
+    bx_tile_key_t key = {(uintptr_t)(p->B + (r0 + c0 * p->ldb) * p->es), p->ldb,
+                         n ? kext : nc, n ? nc : kext, p->es, 0};
+    return key;
+}
+
+/* ---- round order ----
+ * A round is the block (bi, bj) of RM x RN C tiles; it reads the A row panel
+ * of block row bi (RM x kt tiles) and the B column panel of block column bj
+ * (RN x kt tiles).  When every operand tile fits in the cache the order is
+ * block row by block row, alternating direction.  Out of core the tile-reuse
+ * lower bound needs each operand read once; with the cache holding only part
+ * of them the order keeps one operand's panels resident and streams the
+ * other: C is cut into BANDS of block columns whose B panels fit in the cache
+ * beside two A row panels (the one being read and the next one being
+ * fetched) and the C staging; a band is walked block row by block row, so B
+ * is fetched once and A once per band.  LRU eviction then only ever drops A
+ * panels of earlier block rows and B panels of finished bands: within a band
+ * every B panel was used during the previous block row, after the A panel
+ * two rows back was last read.  The roles swap (bands of block rows, A
+ * resident, B streamed) when that moves fewer tiles.  Bands alternate the
+ * direction of their outer walk, so the last panel of one band is the first
+ * of the next.  With one band this is the in-core order. */
+typedef struct round_xy {
+    int bi, bj;
+} round_xy_t;
+
+enum { BX_BAND_COLS = 0, BX_BAND_ROWS = 1 };
+
+/* fills ord[] (nbm * nbn entries) and returns the number of bands; *mode
+ * gets BX_BAND_COLS or BX_BAND_ROWS */
+static int plan_order(const plan_t *p, int cap_slots, round_xy_t *ord, int *mode)
+{
+    const int nbm = (p->mt + p->RM - 1) / p->RM, nbn = (p->nt + p->RN - 1) / p->RN;
+    const int64_t pa = (int64_t)p->RM * p->kt, pb = (int64_t)p->RN * p->kt;
+    const int64_t room = (int64_t)cap_slots - (int64_t)DEPTH * p->RM * p->RN - 2;
+    int64_t wc = (room - 2 * pa) / pb, wr = (room - 2 * pb) / pa;
+    if (wc < 1) wc = 1;
+    if (wr < 1) wr = 1;
+    const int64_t nbc = (nbn + wc - 1) / wc, nbr = (nbm + wr - 1) / wr;
+    /* operand tiles fetched: the resident operand once, the streamed one per band */
+    const int64_t cost_c = (int64_t)p->nt * p->kt + nbc * (int64_t)p->mt * p->kt;
+    const int64_t cost_r = (int64_t)p->mt * p->kt + nbr * (int64_t)p->nt * p->kt;
+    const int cols = cost_c <= cost_r;
+    const int outer_n = cols ? nbn : nbm, inner_n = cols ? nbm : nbn;
+    const int nb = (int)(cols ? nbc : nbr);
+    const int w = (outer_n + nb - 1) / nb; /* balanced band width */
+    int n = 0, flip = 0;
+    for (int b = 0; b < nb; ++b) {
+        const int o0 = b * w, o1 = (o0 + w < outer_n) ? o0 + w : outer_n;
+        for (int q = 0; q < inner_n; ++q) {
+            const int in = (b & 1) ? inner_n - 1 - q : q;
+            for (int r = 0; r < o1 - o0; ++r) {
+                const int out = flip ? o1 - 1 - r : o0 + r;
+                ord[n].bi = cols ? in : out;
+                ord[n].bj = cols ? out : in;
+                ++n;
+            }
+            flip ^= 1;
+        }
+    }
+    *mode = cols ? BX_BAND_COLS : BX_BAND_ROWS;
+    return nb;
+}
+
 static void choose_round_shape(plan_t *p, int cap_slots)
 {
     const int bm = bxh_tile_m(p->dt), bn = bxh_tile_n(p->dt);
@@ -292,7 +382,9 @@ static int run_plan(plan_t *p, bx_cache_t *c)
      * operand tiles they use */
     int *tii = (int *)malloc(sizeof(int) * R), *tkk = (int *)malloc(sizeof(int) * R);
     char *use_a = (char *)malloc(p->RM), *use_b = (char *)malloc(p->RN);
-    if (!cst || !sa || !sb || !tii || !tkk || !use_a || !use_b) {
+    const int64_t nord = (int64_t)((p->mt + p->RM - 1) / p->RM) * ((p->nt + p->RN - 1) / p->RN);
+    round_xy_t *ord = (round_xy_t *)malloc(sizeof(round_xy_t) * nord);
+    if (!cst || !sa || !sb || !tii || !tkk || !use_a || !use_b || !ord) {
         free(cst);
         free(sa);
         free(sb);
@@ -300,8 +392,13 @@ static int run_plan(plan_t *p, bx_cache_t *c)
         free(tkk);
         free(use_a);
         free(use_b);
+        free(ord);
         return BLASX_ENOMEM;
     }
+    int mode;
+    const int nbands = plan_order(p, bx_cache_capacity_slots(c), ord, &mode);
+    bx_log(2, "tile plan: %lld rounds in %d band(s) of block %s", (long long)nord, nbands,
+           mode == BX_BAND_COLS ? "columns (B resident)" : "rows (A resident)");
     int ntaken = 0;
     for (; ntaken < DEPTH * R; ++ntaken) {
         cst[ntaken] = bx_cache_take_scratch(c, 0);
@@ -312,200 +409,179 @@ static int run_plan(plan_t *p, bx_cache_t *c)
     }
 
     uint64_t h2d = 0, d2h = 0, hits = 0, misses = 0, evictions = 0;
-    const int nbm = (p->mt + p->RM - 1) / p->RM, nbn = (p->nt + p->RN - 1) / p->RN;
     int64_t round = 0;
-    for (int bi = 0; bi < nbm && !rc; ++bi) {
-        for (int q = 0; q < nbn && !rc; ++q) {
-            const int bj = (bi & 1) ? nbn - 1 - q : q; /* alternate direction */
-            const int i0 = bi * p->RM, i1 = (i0 + p->RM < p->mt) ? i0 + p->RM : p->mt;
-            const int k0 = bj * p->RN, k1 = (k0 + p->RN < p->nt) ? k0 + p->RN : p->nt;
-            const int ni = i1 - i0, nk = k1 - k0;
-            /* the round's C tiles: all of the block, or those on the triangle */
-            int ntile = 0;
-            memset(use_a, 0, ni);
-            memset(use_b, 0, nk);
-            for (int ii = 0; ii < ni; ++ii)
-                for (int kk = 0; kk < nk; ++kk)
-                    if (tile_on_tri(p, i0 + ii, k0 + kk)) {
-                        tii[ntile] = ii;
-                        tkk[ntile] = kk;
-                        use_a[ii] = use_b[kk] = 1;
-                        ++ntile;
-                    }
-            if (ntile == 0) continue; /* a block off the triangle: no round */
-            const int d = (int)(round % DEPTH);
-            if (round >= DEPTH) {
-                /* at most DEPTH rounds in flight: round-DEPTH has computed */
-                if ((rc = bxh_event_sync(g_rt.ev_comp[d]))) break;
-                /* its C staging must be copied out before reuse */
-                if ((rc = bxh_stream_wait_event(g_bx.s_h2d, g_rt.ev_d2h[d]))) break;
-                if ((rc = bxh_stream_wait_event(g_bx.s_comp, g_rt.ev_d2h[d]))) break;
-            }
-            char *desc = (char *)g_rt.desc_host[d];
-            bxh_problem_t *prob = (bxh_problem_t *)desc;
-            bxh_seg_t *seg = (bxh_seg_t *)(desc + seg_off_bytes);
-
-            /* C tiles in, problem descriptors */
-            int tiles_total = 0;
-            for (int t = 0; t < ntile; ++t) {
-                const int ii = tii[t], kk = tkk[t];
-                const int i = i0 + ii, k = k0 + kk;
-                const int mc = (p->M - i * T < T) ? p->M - i * T : T;
-                const int nc = (p->N - k * T < T) ? p->N - k * T : T;
-                void *cs = bx_cache_slot_ptr(c, cst[d * R + t]);
-                /* a masked diagonal tile goes back whole: its other
-                 * triangle must hold the caller's values, so it is read
-                 * even when beta == 0 */
-                const int diag = p->tri && i == k;
-                if (!p->beta_zero || diag) {
-                    const char *hc = p->C + ((int64_t)i * T + (int64_t)k * T * p->ldc) * es;
-                    rc = copy_in(p->pg_c, cs, (size_t)T * es, hc, (size_t)p->ldc * es,
-                                 (size_t)mc * es, (size_t)nc);
-                    if (rc) break;
-                    h2d += (uint64_t)mc * nc * es;
+    for (int64_t oi = 0; oi < nord && !rc; ++oi) {
+        const int bi = ord[oi].bi, bj = ord[oi].bj;
+        const int i0 = bi * p->RM, i1 = (i0 + p->RM < p->mt) ? i0 + p->RM : p->mt;
+        const int k0 = bj * p->RN, k1 = (k0 + p->RN < p->nt) ? k0 + p->RN : p->nt;
+        const int ni = i1 - i0, nk = k1 - k0;
+        /* the round's C tiles: all of the block, or those on the triangle */
+        int ntile = 0;
+        memset(use_a, 0, ni);
+        memset(use_b, 0, nk);
+        for (int ii = 0; ii < ni; ++ii)
+            for (int kk = 0; kk < nk; ++kk)
+                if (tile_on_tri(p, i0 + ii, k0 + kk)) {
+                    tii[ntile] = ii;
+                    tkk[ntile] = kk;
+                    use_a[ii] = use_b[kk] = 1;
+                    ++ntile;
                 }
-                bxh_problem_t *P = &prob[t];
-                memset(P, 0, sizeof *P);
-                P->C = cs;
-                P->ldc = T;
-                P->m = mc;
-                P->n = nc;
-                P->nseg = p->kt;
-                P->seg_off = t * p->kt;
-                P->opa = p->opa;
-                P->opb = p->opb;
-                P->tiles_m = (mc + bm - 1) / bm;
-                P->tiles_n = (nc + bn - 1) / bn;
-                P->tile_start = tiles_total;
-                tiles_total += P->tiles_m * P->tiles_n;
-                P->beta_zero = p->beta_zero;
-                P->alpha_re = p->alpha[0];
-                P->alpha_im = p->alpha[1];
-                P->beta_re = p->beta[0];
-                P->beta_im = p->beta[1];
-                P->tri = diag ? p->tri : 0;
-                P->tri_off = 0;
-            }
-            if (rc) break;
-
-            /* operand tiles of every K step, through the cache */
-            for (int j = 0; j < p->kt && !rc; ++j) {
-                const int kext = (p->K - j * T < T) ? p->K - j * T : T;
-                for (int ii = 0; ii < ni && !rc; ++ii) {
-                    if (!use_a[ii]) continue;
-                    const int i = i0 + ii;
-                    const int mc = (p->M - i * T < T) ? p->M - i * T : T;
-                    int64_t r0, c0;
-                    int rows, cols;
-                    if (p->opa == BX_OP_N) {
-                        r0 = (int64_t)i * T, c0 = (int64_t)j * T, rows = mc, cols = kext;
-                    } else {
-                        r0 = (int64_t)j * T, c0 = (int64_t)i * T, rows = kext, cols = mc;
-                    }
-                    bx_tile_key_t key = {(uintptr_t)(p->A + (r0 + c0 * p->lda) * es), p->lda,
-                                         rows, cols, es, 0};
-                    int hit;
-                    int64_t ev;
-                    sa[ii] = bx_cache_acquire(c, &key, round, &hit, &ev);
-                    if (sa[ii] < 0) {
-                        rc = BLASX_EBUSY;
-                        break;
-                    }
-                    if (hit) {
-                        ++hits;
-                        continue;
-                    }
-                    ++misses;
-                    if (ev >= 0) {
-                        ++evictions;
-                        if (round - ev < DEPTH)
-                            rc = bxh_stream_wait_event(g_bx.s_h2d, g_rt.ev_comp[ev % DEPTH]);
-                    }
-                    if (!rc)
-                        rc = copy_in(p->pg_a, bx_cache_slot_ptr(c, sa[ii]), (size_t)T * es,
-                                     (const void *)key.addr, (size_t)p->lda * es,
-                                     (size_t)rows * es, (size_t)cols);
-                    h2d += (uint64_t)rows * cols * es;
-                }
-                for (int kk = 0; kk < nk && !rc; ++kk) {
-                    if (!use_b[kk]) continue;
-                    const int k = k0 + kk;
-                    const int nc = (p->N - k * T < T) ? p->N - k * T : T;
-                    int64_t r0, c0;
-                    int rows, cols;
-                    if (p->opb == BX_OP_N) {
-                        r0 = (int64_t)j * T, c0 = (int64_t)k * T, rows = kext, cols = nc;
-                    } else {
-                        r0 = (int64_t)k * T, c0 = (int64_t)j * T, rows = nc, cols = kext;
-                    }
-                    bx_tile_key_t key = {(uintptr_t)(p->B + (r0 + c0 * p->ldb) * es), p->ldb,
-                                         rows, cols, es, 0};
-                    int hit;
-                    int64_t ev;
-                    sb[kk] = bx_cache_acquire(c, &key, round, &hit, &ev);
-                    if (sb[kk] < 0) {
-                        rc = BLASX_EBUSY;
-                        break;
-                    }
-                    if (hit) {
-                        ++hits;
-                        continue;
-                    }
-                    ++misses;
-                    if (ev >= 0) {
-                        ++evictions;
-                        if (round - ev < DEPTH)
-                            rc = bxh_stream_wait_event(g_bx.s_h2d, g_rt.ev_comp[ev % DEPTH]);
-                    }
-                    if (!rc)
-                        rc = copy_in(p->pg_b, bx_cache_slot_ptr(c, sb[kk]), (size_t)T * es,
-                                     (const void *)key.addr, (size_t)p->ldb * es,
-                                     (size_t)rows * es, (size_t)cols);
-                    h2d += (uint64_t)rows * cols * es;
-                }
-                if (rc) break;
-                for (int t = 0; t < ntile; ++t) {
-                    bxh_seg_t *S = &seg[t * p->kt + j];
-                    S->A = bx_cache_slot_ptr(c, sa[tii[t]]);
-                    S->B = bx_cache_slot_ptr(c, sb[tkk[t]]);
-                    S->lda = T;
-                    S->ldb = T;
-                    S->k = kext;
-                }
-            }
-            if (rc) break;
-
-            /* compute after this round's copies; descriptors ride the compute stream */
-            const int nprob = ntile;
-            if ((rc = bxh_event_record(g_rt.ev_h2d[d], g_bx.s_h2d))) break;
-            if ((rc = bxh_stream_wait_event(g_bx.s_comp, g_rt.ev_h2d[d]))) break;
-            if ((rc = bxh_memcpy_h2d_async(g_rt.desc_dev[d], g_rt.desc_host[d],
-                                           seg_off_bytes + (size_t)nprob * p->kt * sizeof(bxh_seg_t),
-                                           g_bx.s_comp)))
-                break;
-            char *ddev = (char *)g_rt.desc_dev[d];
-            if ((rc = bxh_gemm_grouped(p->dt, (const bxh_problem_t *)ddev, nprob,
-                                       (const bxh_seg_t *)(ddev + seg_off_bytes), tiles_total, 0,
-                                       g_bx.s_comp)))
-                break;
-            BX_STAT_ADD(kernel_launches, 1);
-            if ((rc = bxh_event_record(g_rt.ev_comp[d], g_bx.s_comp))) break;
-
-            /* C tiles out */
-            if ((rc = bxh_stream_wait_event(g_bx.s_d2h, g_rt.ev_comp[d]))) break;
-            for (int t = 0; t < ntile && !rc; ++t) {
-                const int i = i0 + tii[t], k = k0 + tkk[t];
-                const int mc = (p->M - i * T < T) ? p->M - i * T : T;
-                const int nc = (p->N - k * T < T) ? p->N - k * T : T;
-                char *hc = p->C + ((int64_t)i * T + (int64_t)k * T * p->ldc) * es;
-                rc = copy_out(p->pg_c, hc, (size_t)p->ldc * es,
-                              bx_cache_slot_ptr(c, cst[d * R + t]), (size_t)T * es,
-                              (size_t)mc * es, (size_t)nc);
-                d2h += (uint64_t)mc * nc * es;
-            }
-            if (!rc) rc = bxh_event_record(g_rt.ev_d2h[d], g_bx.s_d2h);
-            ++round;
+        if (ntile == 0) continue; /* a block off the triangle: no round */
+        const int d = (int)(round % DEPTH);
+        if (round >= DEPTH) {
+            /* at most DEPTH rounds in flight: round-DEPTH has computed */
+            if ((rc = bxh_event_sync(g_rt.ev_comp[d]))) break;
+            /* its C staging must be copied out before reuse */
+            if ((rc = bxh_stream_wait_event(g_bx.s_h2d, g_rt.ev_d2h[d]))) break;
+            if ((rc = bxh_stream_wait_event(g_bx.s_comp, g_rt.ev_d2h[d]))) break;
         }
+        char *desc = (char *)g_rt.desc_host[d];
+        bxh_problem_t *prob = (bxh_problem_t *)desc;
+        bxh_seg_t *seg = (bxh_seg_t *)(desc + seg_off_bytes);
+
+        /* C tiles in, problem descriptors */
+        int tiles_total = 0;
+        for (int t = 0; t < ntile; ++t) {
+            const int ii = tii[t], kk = tkk[t];
+            const int i = i0 + ii, k = k0 + kk;
+            const int mc = (p->M - i * T < T) ? p->M - i * T : T;
+            const int nc = (p->N - k * T < T) ? p->N - k * T : T;
+            void *cs = bx_cache_slot_ptr(c, cst[d * R + t]);
+            /* a masked diagonal tile goes back whole: its other
+             * triangle must hold the caller's values, so it is read
+             * even when beta == 0 */
+            const int diag = p->tri && i == k;
+            if (!p->beta_zero || diag) {
+                const char *hc = p->C + ((int64_t)i * T + (int64_t)k * T * p->ldc) * es;
+                rc = copy_in(p->pg_c, cs, (size_t)T * es, hc, (size_t)p->ldc * es,
+                             (size_t)mc * es, (size_t)nc);
+                if (rc) break;
+                h2d += (uint64_t)mc * nc * es;
+            }
+            bxh_problem_t *P = &prob[t];
+            memset(P, 0, sizeof *P);
+            P->C = cs;
+            P->ldc = T;
+            P->m = mc;
+            P->n = nc;
+            P->nseg = p->kt;
+            P->seg_off = t * p->kt;
+            P->opa = p->opa;
+            P->opb = p->opb;
+            P->tiles_m = (mc + bm - 1) / bm;
+            P->tiles_n = (nc + bn - 1) / bn;
+            P->tile_start = tiles_total;
+            tiles_total += P->tiles_m * P->tiles_n;
+            P->beta_zero = p->beta_zero;
+            P->alpha_re = p->alpha[0];
+            P->alpha_im = p->alpha[1];
+            P->beta_re = p->beta[0];
+            P->beta_im = p->beta[1];
+            P->tri = diag ? p->tri : 0;
+            P->tri_off = 0;
+        }
+        if (rc) break;
+
+        /* operand tiles of every K step, through the cache */
+        for (int j = 0; j < p->kt && !rc; ++j) {
+            const int kext = (p->K - j * T < T) ? p->K - j * T : T;
+            for (int ii = 0; ii < ni && !rc; ++ii) {
+                if (!use_a[ii]) continue;
+                const bx_tile_key_t key = a_key(p, i0 + ii, j);
+                const int rows = key.rows, cols = key.cols;
+                int hit;
+                int64_t ev;
+                sa[ii] = bx_cache_acquire(c, &key, round, &hit, &ev);
+                if (sa[ii] < 0) {
+                    rc = BLASX_EBUSY;
+                    break;
+                }
+                if (hit) {
+                    ++hits;
+                    continue;
+                }
+                ++misses;
+                if (ev >= 0) {
+                    ++evictions;
+                    if (round - ev < DEPTH)
+                        rc = bxh_stream_wait_event(g_bx.s_h2d, g_rt.ev_comp[ev % DEPTH]);
+                }
+                if (!rc)
+                    rc = copy_in(p->pg_a, bx_cache_slot_ptr(c, sa[ii]), (size_t)T * es,
+                                 (const void *)key.addr, (size_t)p->lda * es,
+                                 (size_t)rows * es, (size_t)cols);
+                h2d += (uint64_t)rows * cols * es;
+            }
+            for (int kk = 0; kk < nk && !rc; ++kk) {
+                if (!use_b[kk]) continue;
+                const bx_tile_key_t key = b_key(p, k0 + kk, j);
+                const int rows = key.rows, cols = key.cols;
+                int hit;
+                int64_t ev;
+                sb[kk] = bx_cache_acquire(c, &key, round, &hit, &ev);
+                if (sb[kk] < 0) {
+                    rc = BLASX_EBUSY;
+                    break;
+                }
+                if (hit) {
+                    ++hits;
+                    continue;
+                }
+                ++misses;
+                if (ev >= 0) {
+                    ++evictions;
+                    if (round - ev < DEPTH)
+                        rc = bxh_stream_wait_event(g_bx.s_h2d, g_rt.ev_comp[ev % DEPTH]);
+                }
+                if (!rc)
+                    rc = copy_in(p->pg_b, bx_cache_slot_ptr(c, sb[kk]), (size_t)T * es,
+                                 (const void *)key.addr, (size_t)p->ldb * es,
+                                 (size_t)rows * es, (size_t)cols);
+                h2d += (uint64_t)rows * cols * es;
+            }
+            if (rc) break;
+            for (int t = 0; t < ntile; ++t) {
+                bxh_seg_t *S = &seg[t * p->kt + j];
+                S->A = bx_cache_slot_ptr(c, sa[tii[t]]);
+                S->B = bx_cache_slot_ptr(c, sb[tkk[t]]);
+                S->lda = T;
+                S->ldb = T;
+                S->k = kext;
+            }
+        }
+        if (rc) break;
+
+        /* compute after this round's copies; descriptors ride the compute stream */
+        const int nprob = ntile;
+        if ((rc = bxh_event_record(g_rt.ev_h2d[d], g_bx.s_h2d))) break;
+        if ((rc = bxh_stream_wait_event(g_bx.s_comp, g_rt.ev_h2d[d]))) break;
+        if ((rc = bxh_memcpy_h2d_async(g_rt.desc_dev[d], g_rt.desc_host[d],
+                                       seg_off_bytes + (size_t)nprob * p->kt * sizeof(bxh_seg_t),
+                                       g_bx.s_comp)))
+            break;
+        char *ddev = (char *)g_rt.desc_dev[d];
+        if ((rc = bxh_gemm_grouped(p->dt, (const bxh_problem_t *)ddev, nprob,
+                                   (const bxh_seg_t *)(ddev + seg_off_bytes), tiles_total, 0,
+                                   g_bx.s_comp)))
+            break;
+        BX_STAT_ADD(kernel_launches, 1);
+        if ((rc = bxh_event_record(g_rt.ev_comp[d], g_bx.s_comp))) break;
+
+        /* C tiles out */
+        if ((rc = bxh_stream_wait_event(g_bx.s_d2h, g_rt.ev_comp[d]))) break;
+        for (int t = 0; t < ntile && !rc; ++t) {
+            const int i = i0 + tii[t], k = k0 + tkk[t];
+            const int mc = (p->M - i * T < T) ? p->M - i * T : T;
+            const int nc = (p->N - k * T < T) ? p->N - k * T : T;
+            char *hc = p->C + ((int64_t)i * T + (int64_t)k * T * p->ldc) * es;
+            rc = copy_out(p->pg_c, hc, (size_t)p->ldc * es,
+                          bx_cache_slot_ptr(c, cst[d * R + t]), (size_t)T * es,
+                          (size_t)mc * es, (size_t)nc);
+            d2h += (uint64_t)mc * nc * es;
+        }
+        if (!rc) rc = bxh_event_record(g_rt.ev_d2h[d], g_bx.s_d2h);
+        ++round;
     }
 
     /* drain every stream before the buffers are reused or the caller reads C */
@@ -525,6 +601,7 @@ static int run_plan(plan_t *p, bx_cache_t *c)
     free(tkk);
     free(use_a);
     free(use_b);
+    free(ord);
     bx_cache_invalidate(c, keep_resident, NULL);
 
     BX_STAT_ADD(bytes_h2d, h2d);
@@ -532,6 +609,98 @@ static int run_plan(plan_t *p, bx_cache_t *c)
     BX_STAT_ADD(tile_hits, hits);
     BX_STAT_ADD(tile_misses, misses);
     BX_STAT_ADD(tile_evictions, evictions);
+    return rc;
+}
+
+/* ---- host-only replay of a plan's cache traffic ----
+ * Runs the round order and the operand acquisitions of run_plan against a
+ * cache of cap_slots slots with no device memory behind it, and counts the
+ * operand bytes a real call would copy host to device.  For tests and for
+ * choosing budgets without a GPU.  Host addresses are synthetic (operands at
+ * disjoint ranges; A == B when same_ab). */
+static int fake_chunk_alloc(void *u, void **p, size_t bytes)
+{
+    uintptr_t *next = (uintptr_t *)u;
+    *p = (void *)*next;
+    *next += bytes;
+    return 0;
+}
+
+int bx_plan_simulate(int dt, int opa, int opb, int M, int N, int K, int T, int tri, int same_ab,
+                     int cap_slots, uint64_t *op_bytes, uint64_t *evictions, int *mode,
+                     int *nbands)
+{
+    plan_t p;
+    memset(&p, 0, sizeof p);
+    p.dt = dt;
+    p.es = bx_esize(dt);
+    p.opa = opa;
+    p.opb = opb;
+    p.M = M;
+    p.N = N;
+    p.K = K;
+    p.T = T;
+    p.tri = tri;
+    p.mt = (M + T - 1) / T;
+    p.nt = (N + T - 1) / T;
+    p.kt = (K + T - 1) / T;
+    p.lda = opa == BX_OP_N ? M : K;
+    p.ldb = opb == BX_OP_N ? K : N;
+    p.A = (const char *)((uintptr_t)1 << 44);
+    p.B = same_ab ? p.A : (const char *)((uintptr_t)1 << 45);
+    choose_round_shape(&p, cap_slots);
+    if ((int64_t)2 * (p.RM + p.RN) * p.kt + (int64_t)DEPTH * p.RM * p.RN + 2 > cap_slots)
+        return BLASX_ERANGE;
+    uintptr_t fake = (uintptr_t)1 << 40;
+    bx_cache_ops_t ops = {fake_chunk_alloc, NULL, &fake};
+    const size_t slot = (size_t)T * T * p.es;
+    bx_cache_t *c = bx_cache_create(slot, slot * 32, (size_t)cap_slots * slot, &ops);
+    const int64_t nord = (int64_t)((p.mt + p.RM - 1) / p.RM) * ((p.nt + p.RN - 1) / p.RN);
+    round_xy_t *ord = (round_xy_t *)malloc(sizeof(round_xy_t) * nord);
+    char *use_a = (char *)malloc(p.RM), *use_b = (char *)malloc(p.RN);
+    int rc = (!c || !ord || !use_a || !use_b) ? BLASX_ENOMEM : 0;
+    for (int t = 0; t < DEPTH * p.RM * p.RN && !rc; ++t)
+        if (bx_cache_take_scratch(c, 0) < 0) rc = BLASX_EBUSY;
+    uint64_t bytes = 0, ev_n = 0;
+    int64_t round = 0;
+    if (!rc) *nbands = plan_order(&p, cap_slots, ord, mode);
+    for (int64_t oi = 0; oi < nord && !rc; ++oi) {
+        const int i0 = ord[oi].bi * p.RM, k0 = ord[oi].bj * p.RN;
+        const int ni = (i0 + p.RM < p.mt ? i0 + p.RM : p.mt) - i0;
+        const int nk = (k0 + p.RN < p.nt ? k0 + p.RN : p.nt) - k0;
+        int ntile = 0;
+        memset(use_a, 0, ni);
+        memset(use_b, 0, nk);
+        for (int ii = 0; ii < ni; ++ii)
+            for (int kk = 0; kk < nk; ++kk)
+                if (tile_on_tri(&p, i0 + ii, k0 + kk)) {
+                    use_a[ii] = use_b[kk] = 1;
+                    ++ntile;
+                }
+        if (ntile == 0) continue;
+        for (int j = 0; j < p.kt && !rc; ++j) {
+            for (int r = 0; r < ni + nk && !rc; ++r) {
+                const int isa = r < ni;
+                if (isa ? !use_a[r] : !use_b[r - ni]) continue;
+                const bx_tile_key_t key = isa ? a_key(&p, i0 + r, j) : b_key(&p, k0 + r - ni, j);
+                int hit;
+                int64_t ev;
+                if (bx_cache_acquire(c, &key, round, &hit, &ev) < 0) {
+                    rc = BLASX_EBUSY;
+                    break;
+                }
+                if (!hit) bytes += (uint64_t)key.rows * key.cols * p.es;
+                if (ev >= 0) ++ev_n;
+            }
+        }
+        ++round;
+    }
+    *op_bytes = bytes;
+    *evictions = ev_n;
+    free(ord);
+    free(use_a);
+    free(use_b);
+    bx_cache_destroy(c);
     return rc;
 }
 

@@ -160,3 +160,33 @@ BX_EXPORT int blasx_dd_gemm_block_host(int transa, int transb, int m, int n, int
         for (int t = 0; t < nt; ++t) pthread_join(th[t], NULL);
     return 0;
 }
+
+/* Entries (ri[a], ci[b]) of the seeded uniform matrix with `rows` rows, the
+ * values blasx_fill_uniform_host / _device put there (element (i,j) from
+ * counter i + j*rows; complex: counters 2e and 2e+1), column-major into out
+ * (nr x nc, ld nr) of the element type.  Lets a benchmark check sampled
+ * rows and columns of a full-size product without a second copy of the
+ * operands. */
+BX_EXPORT int blasx_sample_uniform_host(int dtype, void *out, int64_t rows, int nr,
+                                        const int64_t *ri, int nc, const int64_t *ci,
+                                        uint64_t seed, uint32_t id)
+{
+    if (nr < 0 || nc < 0 || (nr * nc > 0 && (!out || !ri || !ci))) return BLASX_EINVAL;
+    if (dtype != BLASX_S && dtype != BLASX_D && dtype != BLASX_Z) return BLASX_EINVAL;
+    const uint64_t base = seed ^ ((uint64_t)id << 56);
+    for (int b = 0; b < nc; ++b)
+        for (int a = 0; a < nr; ++a) {
+            if (ri[a] < 0 || ri[a] >= rows || ci[b] < 0) return BLASX_EINVAL;
+            const uint64_t e = (uint64_t)(ri[a] + ci[b] * rows);
+            const int64_t o = a + (int64_t)b * nr;
+            if (dtype == BLASX_Z) {
+                ((double *)out)[2 * o] = u11(base ^ (2 * e));
+                ((double *)out)[2 * o + 1] = u11(base ^ (2 * e + 1));
+            } else if (dtype == BLASX_S) {
+                ((float *)out)[o] = (float)u11(base ^ e);
+            } else {
+                ((double *)out)[o] = u11(base ^ e);
+            }
+        }
+    return 0;
+}

@@ -66,13 +66,16 @@ void blasx_get_stats(blasx_stats_t *out);
 void blasx_reset_stats(void);
 
 /* ---- configuration (also read from the environment at init) --------------
- * BLASX_HBM_BUDGET bytes of tile cache (default: 90% of free HBM)
+ * BLASX_HBM_BUDGET bytes of tile cache (default: 80% of free HBM)
  * BLASX_TILE       tile edge for host-resident GEMMs (default: autotuned)
  * BLASX_GPU_MIN_DIM size threshold (default 1000 for S/D, 500 for Z,
  *                  the reference's blas/dgemm.c:62 and blas/zgemm.c:61)
  * BLASX_CPUBLAS    path of the CPU BLAS to dlopen
- * BLASX_CPU_FALLBACK=1  run GPU-sized calls on the CPU when no GPU works
- *                  (default: print an error and abort, never silently) */
+ * BLASX_CPU_FALLBACK  1 (default): a GPU-sized call on host operands whose
+ *                  GPU path fails is logged, counted in cpu_fallbacks and run
+ *                  on the CPU BLAS; 0: the process ends with a message.
+ *                  Operands in HBM cannot fall back (the CPU BLAS cannot
+ *                  read them): such a failure always ends with a message. */
 int blasx_set_hbm_budget(size_t bytes);
 size_t blasx_get_hbm_budget(void);
 int blasx_set_tile_size(int dtype, int tile);   /* 0 = autotune */
@@ -188,6 +191,7 @@ typedef struct blasx_gemm_desc {
 
 int blasx_gemm_grouped_device(const blasx_gemm_desc_t *descs, int n, void *stream);
 
+
 /* ---- GEMM request server (continuous batching) ------------------------------
  * No counterpart in the reference (its ABI is one blocking, non-reentrant
  * call at a time).  Requests carry HOST pointers (pinned memory gives full
@@ -223,6 +227,21 @@ int blasx_fill_uniform_device(int dtype, void *dptr, int64_t rows, int64_t cols,
                               uint64_t seed, uint32_t id, void *stream);
 int blasx_fill_uniform_host(int dtype, void *hptr, int64_t rows, int64_t cols, int64_t ld,
                             uint64_t seed, uint32_t id);
+/* entries (ri[a], ci[b]) of the matrix the fills above produce, column-major
+ * into out (nr x nc): sampled rows/columns of a full-size operand */
+int blasx_sample_uniform_host(int dtype, void *out, int64_t rows, int nr, const int64_t *ri,
+                              int nc, const int64_t *ci, uint64_t seed, uint32_t id);
+/* split-precision workload inputs: a = (u - 0.5) exp(phi g), u ~ U(0,1),
+ * g ~ N(0,1) by Box-Muller on glibc log/cos/exp, FP64 */
+int blasx_fill_dynrange_host(double *h, int64_t rows, int64_t cols, int64_t ld, uint64_t seed,
+                             uint32_t id, double phi);
+/* out = alpha op(A) op(B) + beta C0 evaluated in double-double and rounded
+ * once (TwoSum/TwoProd, Ogita, Rump & Oishi 2005): an accuracy reference
+ * for one block of C (C0 may be NULL when beta == 0) */
+int blasx_dd_gemm_block_host(int transa, int transb, int m, int n, int K, double alpha,
+                             const double *A, int64_t lda, const double *B, int64_t ldb,
+                             double beta, const double *C0, int64_t ldc, double *out,
+                             int64_t ldo);
 
 #ifdef __cplusplus
 }

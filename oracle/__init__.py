@@ -36,6 +36,7 @@ TRANS = {"N": 111, "T": 112, "C": 113}
 
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
+_i64p = C.POINTER(C.c_int64)
 
 
 def lib():
@@ -53,6 +54,8 @@ def lib():
         for nm, p in (("orc_fill_uniform_d", _dp), ("orc_fill_uniform_s", _fp),
                       ("orc_fill_uniform_z", _dp)):
             getattr(L, nm).argtypes = [p, C.c_int64, C.c_int64, C.c_int64, C.c_uint64, C.c_uint32]
+        L.orc_sample_uniform.argtypes = [_dp, C.c_int, C.c_int64, C.c_int, _i64p, C.c_int,
+                                         _i64p, C.c_uint64, C.c_uint32]
         L.orc_fill_dynrange_d.argtypes = [_dp, C.c_int64, C.c_int64, C.c_int64, C.c_uint64,
                                           C.c_uint32, C.c_double]
         L.orc_gemm_info.argtypes = [C.c_int] * 8
@@ -157,6 +160,19 @@ def fill_uniform(rows, cols, seed, mid, dtype=np.float64, ld=None):
     else:
         raise TypeError(dtype)
     return out
+
+
+def sample_uniform(rows, ri, ci, seed, mid, dtype=np.float64):
+    """Entries (ri[a], ci[b]) of the seeded matrix with `rows` rows, as the
+    (len(ri), len(ci)) array fill_uniform would hold there."""
+    ri = np.ascontiguousarray(ri, dtype=np.int64)
+    ci = np.ascontiguousarray(ci, dtype=np.int64)
+    kind = {np.float64: 0, np.float32: 1, np.complex128: 2}[np.dtype(dtype).type]
+    out = np.zeros((len(ri), len(ci)), dtype=np.complex128 if kind == 2 else np.float64,
+                   order="F")
+    lib().orc_sample_uniform(out.ctypes.data_as(_dp), kind, rows, len(ri), _ptr(ri, _i64p),
+                             len(ci), _ptr(ci, _i64p), seed, mid)
+    return out.astype(np.float32, order="F") if kind == 1 else out
 
 
 def fill_dynrange(rows, cols, seed, mid, phi, ld=None):

@@ -92,6 +92,29 @@ void orc_fill_uniform_z(double *out, int64_t rows, int64_t cols, int64_t ld,
         }
 }
 
+/* Entries (ri[a], ci[b]) of the seeded matrix with `rows` rows (element
+ * (i,j) = U(seed, id, i + j*rows), as the fills above), written column-major
+ * to out (nr x nc, ld nr).  kind: 0 FP64, 1 FP32 (rounded as the FP32 fill),
+ * 2 complex128 (out holds nr*nc (re, im) pairs).  Lets a checker regenerate
+ * sampled rows and columns of a matrix too large to hold twice. */
+void orc_sample_uniform(double *out, int kind, int64_t rows, int nr, const int64_t *ri, int nc,
+                        const int64_t *ci, uint64_t seed, uint32_t id)
+{
+    for (int b = 0; b < nc; ++b)
+        for (int a = 0; a < nr; ++a) {
+            const uint64_t e = (uint64_t)(ri[a] + ci[b] * rows);
+            const int64_t o = a + (int64_t)b * nr;
+            if (kind == 2) {
+                out[2 * o] = orc_uniform(seed, id, 2 * e);
+                out[2 * o + 1] = orc_uniform(seed, id, 2 * e + 1);
+            } else if (kind == 1) {
+                out[o] = (double)(float)orc_uniform(seed, id, e);
+            } else {
+                out[o] = orc_uniform(seed, id, e);
+            }
+        }
+}
+
 /* Dynamic-range matrix a = (u - 0.5) * exp(phi * g), u ~ U(0,1), g ~ N(0,1).
  * u uses matrix id `id` (counter 3*idx), the Box-Muller pair uses counters
  * 3*idx+1 and 3*idx+2. */

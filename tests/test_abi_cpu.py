@@ -230,3 +230,18 @@ def test_dd_block_reference_matches_oracle(bx, orc):
     out = bx.dd_gemm_block("N", "T", 1.5, A, B, -0.25, C0)
     hi, _ = orc.gemm_dd("N", "T", 37, 29, 500, 1.5, A, 37, B, 29, -0.25, C0, 37)
     assert np.array_equal(out, hi)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32, np.complex128])
+def test_sampled_generator_matches_oracle(bx, orc, dtype):
+    """The library's sampled generator (the benchmark's full-size checks
+    regenerate sampled operand rows and columns with it) equals the
+    oracle's fill at the same indices, past 2^31 linear indices too."""
+    rows = 131072
+    ri = [0, 1, 2047, 65536, 131071]
+    ci = [0, 16383, 16384, 131071]          # i + j*rows up to 1.7e10
+    S = bx.sample_uniform_host(rows, ri, ci, 3, 1, dtype)
+    assert np.array_equal(S, orc.sample_uniform(rows, ri, ci, 3, 1, dtype))
+    F = orc.fill_uniform(300, 200, 3, 1, dtype=dtype)
+    S = bx.sample_uniform_host(300, [0, 299, 17], [199, 0], 3, 1, dtype)
+    assert np.array_equal(S, F[np.ix_([0, 299, 17], [199, 0])])

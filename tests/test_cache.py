@@ -245,3 +245,64 @@ def test_queue_order_matches_reference(bx, orc):
             mine.append(v)
         L.bxtest_queue_destroy(q)
         assert mine == ref_seq == list(range(n - 1, -1, -1))
+
+
+def _plan_sim(bx, M, N, K, T, cap_slots, dt=1, opa=0, opb=0, tri=0, same_ab=0):
+    L = bx.test_lib()
+    ob, ev = C.c_uint64(), C.c_uint64()
+    mode, nb = C.c_int(), C.c_int()
+    rc = L.bxtest_plan_simulate(dt, opa, opb, M, N, K, T, tri, same_ab, cap_slots, C.byref(ob),
+                                C.byref(ev), C.byref(mode), C.byref(nb))
+    return rc, ob.value, ev.value, mode.value, nb.value
+
+
+@pytest.mark.parametrize("n", [65536, 131072])
+@pytest.mark.parametrize("frac", [0.4, 0.75, 1.2])
+def test_out_of_core_traffic_bound(bx, n, frac):
+    es, T = 8, 2048
+    slots = int(frac * 2 * n * n * es // (T * T * es))
+    rc, ob, ev, mode, nb = _plan_sim(bx, n, n, n, T, slots)
+    assert rc == 0
+    ab = 2 * n * n * es
+    if frac >= 0.75:
+        assert ob == ab and nb == 1  # every operand tile once
+    else:
+        assert nb == 2 and ab * 1.4 <= ob <= ab * 1.5, (ob / ab, nb)
+        total = ob + 2 * n * n * es
+        assert total <= 1.25 * (ab + 2 * n * n * es)
+    if frac > 1:
+        assert ev == 0
+
+
+@pytest.mark.parametrize("opa,opb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_out_of_core_order_rectangular_picks_cheaper_role(bx, opa, opb):
+    """T = 512 (rounds of 4 x 4 tiles), ragged edges, every transpose.  A
+    tall A (21 x 7 tiles) with a cache of 190 slots: A does not fit beside
+    the staging, B (7 x 10 tiles) does with two A row panels, so B stays
+    resident (bands of block columns) and every operand tile is fetched
+    exactly once.  The mirrored shape keeps A resident instead."""
+    T, es = 512, 8
+    M, N, K = 20 * T + 37, 9 * T + 5, 6 * T + 11
+    rc, ob, ev, mode, nb = _plan_sim(bx, M, N, K, T, 190, opa=opa, opb=opb)
+    assert rc == 0 and (mode, nb) == (0, 1) and ev > 0
+    assert ob == es * (M * K + K * N)
+    rc, ob, ev, mode, nb = _plan_sim(bx, N, M, K, T, 190, opa=opa, opb=opb)
+    assert rc == 0 and (mode, nb) == (1, 1) and ev > 0
+    assert ob == es * (M * K + K * N)
+
+
+def test_out_of_core_order_matches_brute_force_reuse(bx):
+    """Against an exhaustive count on a small grid: operand tiles fetched
+    by the replay never exceed what a per-round refetch of every tile
+    (the reference's pattern, blas/blasx_dgemm.c:164-239) would move, and
+    equal |A|+|B| when the cache holds both."""
+    T, es = 256, 8
+    for (M, N, K) in [(9 * T, 7 * T, 5 * T), (9 * T + 3, 7 * T + 100, 5 * T + 1)]:
+        mt, nt, kt = -(-M // T), -(-N // T), -(-K // T)
+        full = es * (M * K + K * N)
+        rc, ob, _, _, _ = _plan_sim(bx, M, N, K, T, 4096)
+        assert rc == 0 and ob == full
+        rc, ob, _, _, nb = _plan_sim(bx, M, N, K, T, 3 * 16 + 2 * 8 * kt + 4 * kt + 10)
+        assert rc == 0
+        refetch = es * T * T * kt * (mt * nt) * 2
+        assert full <= ob < refetch

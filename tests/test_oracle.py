@@ -365,3 +365,17 @@ def test_relerr_metric_catches_transposed_operand(orc):
     bad = A.T @ B
     assert orc.relerr_fro(good, good, 1.0, A, B, 0.0, C0) == 0.0
     assert orc.relerr_fro(bad, good, 1.0, A, B, 0.0, C0) > 1e3 * K * EPS64
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32, np.complex128])
+def test_sample_uniform_equals_fill_at_indices(orc, dtype):
+    """The sampled generator (used to check full-size results on sampled
+    rows and columns) returns exactly what the full fill holds there,
+    including the last row and column and repeated indices."""
+    rows, cols = 53, 41
+    F = orc.fill_uniform(rows, cols, 11, 2, dtype=dtype)
+    ri = [0, 7, 52, 7, 31]
+    ci = [40, 0, 19, 19]
+    S = orc.sample_uniform(rows, ri, ci, 11, 2, dtype)
+    assert S.dtype == F.dtype
+    assert np.array_equal(S, F[np.ix_(ri, ci)])
