@@ -394,7 +394,6 @@ def fn_type(ret, params):
 def emit_cblas(r, w):
     rt = ret_type(r.ret, r.p)
     names = ", ".join(a.name for a in r.args)
-    fail = "" if rt == "void" else " 0"
     internal = r.base in ON_GPU
     w("/* %s: %s */" % (r.cname, cite(r.cname)))
     params = ", ".join(decl(a.cblas_type(), a.name) for a in r.args)
@@ -406,7 +405,6 @@ def emit_cblas(r, w):
     w("{")
     w("    static void *slot;")
     w("    void *f = fwd_sym(&slot, \"%s\");" % r.cname)
-    w("    if (!f) return%s;" % fail)
     lp = [a.cblas_call_type(False) for a in r.args]
     ilp = [a.cblas_call_type(True) for a in r.args]
     ret = "" if rt == "void" else "return "
@@ -463,7 +461,6 @@ def emit_f77_call(fname, args, w):
     ilp = [a.f77_call_type(True) for a in args] + ["size_t"] * nchar
     w("    static void *slot;")
     w("    void *f = fwd_sym(&slot, \"%s\");" % fname)
-    w("    if (!f) return;")
     ints = [a.f77name for a in args if a.tok == "INT"]
     if ints:
         w("    if (bx_cpublas_ilp64()) {")
@@ -499,15 +496,19 @@ def generate():
     cw("#include \"cpu_forward.h\"")
     cw("#include \"internal.h\"")
     cw("")
-    cw("/* a routine the CPU BLAS lacks is looked up (and reported) once */")
-    cw("#define FWD_MISSING ((void *)-1)")
-    cw("")
+    cw("/* The routine's symbol in the CPU BLAS, looked up once.  A CPU BLAS that")
+    cw(" * lacks it ends the process with a message: returning without computing")
+    cw(" * would hand the caller a wrong result (a dot product of 0, an unchanged y). */")
     cw("static void *fwd_sym(void **slot, const char *name)")
     cw("{")
     cw("    void *f = __atomic_load_n(slot, __ATOMIC_ACQUIRE);")
-    cw("    if (f) return f == FWD_MISSING ? NULL : f;")
+    cw("    if (f) return f;")
     cw("    f = bx_cpublas_sym(name);")
-    cw("    __atomic_store_n(slot, f ? f : FWD_MISSING, __ATOMIC_RELEASE);")
+    cw("    if (!f)")
+    cw("        bx_fatal(\"%s: the CPU BLAS (%s) has no such routine; set BLASX_CPUBLAS to a \"")
+    cw("                 \"complete BLAS\",")
+    cw("                 name, blasx_cpublas_path());")
+    cw("    __atomic_store_n(slot, f, __ATOMIC_RELEASE);")
     cw("    return f;")
     cw("}")
     cw("")

@@ -46,6 +46,7 @@
  * ConjTrans (info 2); ZHER2K reports itself as "ZHER2K".
  */
 #include <pthread.h>
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -469,6 +470,17 @@ static int gpu_syr2k(int dt, int uplo, int trans, int N, int K, const double *al
     return finish(&s, rc);
 }
 
+/* The unimodular factor of HER2K's second term: conj(a)/a when conj_ratio,
+ * else a/conj(a), for a finite a != 0.  With u = a/|a| (hypot: no overflow
+ * or underflow at any magnitude of a) they are conj(u)^2 and u^2. */
+void bx_her2k_factor(const double *a, int conj_ratio, double *out)
+{
+    const double r = hypot(a[0], a[1]), ur = a[0] / r, ui = a[1] / r;
+    const double im = 2.0 * ur * ui;
+    out[0] = ur * ur - ui * ui;
+    out[1] = conj_ratio ? -im : im;
+}
+
 /* HER2K: C = alpha op(A) op(B)^H + conj(alpha) op(B) op(A)^H + beta C, beta
  * real, the diagonal real.  The grouped problem has one alpha, so the second
  * term reads a copy of B scaled by s, |s| = 1: conj(alpha)/alpha for NoTrans
@@ -488,10 +500,8 @@ static int gpu_her2k(int uplo, int trans, int N, int K, const double *al, const 
     if (!rc) rc = dev_view(&s, B, rows, cols, ldb, es, 1, &dB, &lB);
     const int product = K > 0 && (al[0] != 0.0 || al[1] != 0.0);
     if (!rc && product) {
-        const double m2 = al[0] * al[0] + al[1] * al[1];
-        /* conj(a)/a = conj(a)^2/|a|^2; a/conj(a) = a^2/|a|^2 */
-        const double re = (al[0] * al[0] - al[1] * al[1]) / m2, im = 2.0 * al[0] * al[1] / m2;
-        const double sc[2] = {re, trans == CblasNoTrans ? -im : im};
+        double sc[2];
+        bx_her2k_factor(al, trans == CblasNoTrans, sc);
         rc = scratch_alloc(&s, &dB2, (size_t)rows * cols * es);
         if (!rc)
             rc = bxh_memcpy2d_d2d_async(dB2, (size_t)rows * es, dB, (size_t)lB * es,

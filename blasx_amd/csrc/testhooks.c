@@ -138,3 +138,11 @@ BX_EXPORT int bxtest_plan_simulate(int dt, int opa, int opb, int M, int N, int K
     return bx_plan_simulate(dt, opa, opb, M, N, K, T, tri, same_ab, cap_slots, op_bytes,
                             evictions, mode, nbands);
 }
+
+/* HER2K's unimodular factor (level3.c) */
+void bx_her2k_factor(const double *a, int conj_ratio, double *out);
+BX_EXPORT void bxtest_her2k_factor(double re, double im, int conj_ratio, double *out)
+{
+    const double a[2] = {re, im};
+    bx_her2k_factor(a, conj_ratio, out);
+}

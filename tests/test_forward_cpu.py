@@ -625,3 +625,26 @@ def test_generated_sources_are_current():
                           ("cpu_forward.h", os.path.join(csrc, "cpu_forward.h")),
                           ("blasx_blas.h", os.path.join(ROOT, "include", "blasx_blas.h"))):
             assert open(os.path.join(d, got)).read() == open(want).read(), want
+
+
+def test_missing_cpu_routine_ends_the_process_with_a_message(tmp_path):
+    """A CPU BLAS without the routine must not turn the call into a silent
+    no-op (ddot returning 0, y left unchanged): the process ends with a
+    message naming the routine and BLASX_CPUBLAS.  Run in a child process
+    whose CPU BLAS is a stub built here that exports cblas_dgemm (so the
+    loader accepts it) and nothing else."""
+    stub = tmp_path / "libstubblas.so"
+    src = tmp_path / "stub.c"
+    src.write_text("void cblas_dgemm(void) {}\n")
+    subprocess.run(["gcc", "-shared", "-fPIC", "-o", str(stub), str(src)], check=True)
+    code = ("import ctypes as C, numpy as np, blasx_amd as bx\n"
+            "x = np.ones(4)\n"
+            "f = bx.lib().cblas_ddot\n"
+            "f.restype = C.c_double\n"
+            "v = f(4, x.ctypes.data_as(C.c_void_p), 1, x.ctypes.data_as(C.c_void_p), 1)\n"
+            "print('returned', v, flush=True)\n")
+    env = dict(os.environ, BLASX_CPUBLAS=str(stub), PYTHONPATH=ROOT)
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode != 0 and "returned" not in r.stdout, (r.returncode, r.stdout)
+    assert "cblas_ddot" in r.stderr and "BLASX_CPUBLAS" in r.stderr, r.stderr[-2000:]

@@ -237,3 +237,25 @@ def test_her2k_cpu_sized(bx, orc, dt, uplo, trans):
                  out, N)
         assert np.max(np.abs(out[m] - ref[m])) <= _tol(dt) * 8, entry
         assert np.array_equal(out[~m].view(np.uint8), C0[~m].view(np.uint8)), entry
+
+
+@pytest.mark.parametrize("mag", [1e-170, 1e-300, 1.0, 1e170, 1e300])
+@pytest.mark.parametrize("conj_ratio", [1, 0])
+def test_her2k_unimodular_factor_any_magnitude(bx, mag, conj_ratio):
+    """ZHER2K's second term reads B scaled by conj(alpha)/alpha (NoTrans) or
+    alpha/conj(alpha) (ConjTrans); the factor must stay finite and of modulus
+    one for every finite alpha != 0, however small or large."""
+    import ctypes as C
+
+    L = bx.test_lib()
+    L.bxtest_her2k_factor.argtypes = [C.c_double, C.c_double, C.c_int,
+                                      C.POINTER(C.c_double)]
+    for ang in (0.0, 0.3, 1.9, -2.7, np.pi / 2):
+        a = mag * complex(np.cos(ang), np.sin(ang))
+        out = (C.c_double * 2)()
+        L.bxtest_her2k_factor(a.real, a.imag, conj_ratio, out)
+        got = complex(out[0], out[1])
+        u = complex(np.cos(ang), np.sin(ang))
+        want = (u.conjugate() / u) if conj_ratio else (u / u.conjugate())
+        assert np.isfinite(got.real) and np.isfinite(got.imag)
+        assert abs(got - want) <= 8 * np.finfo(float).eps, (mag, ang, got, want)
