@@ -60,11 +60,13 @@ def make_desc(dtype, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C_, ldc):
     return d
 
 
-def gemm_grouped_device(descs, stream=None):
+def gemm_grouped_device(descs, stream=None, noalloc=False):
+    """blasx_gemm_grouped_device, or its allocation-free variant"""
     arr = (GemmDesc * len(descs))(*descs)
-    rc = lib().blasx_gemm_grouped_device(arr, len(descs), stream)
+    fn = lib().blasx_gemm_grouped_device_noalloc if noalloc else lib().blasx_gemm_grouped_device
+    rc = fn(arr, len(descs), stream)
     if rc:
-        raise RuntimeError(f"grouped gemm failed ({rc})")
+        raise RuntimeError(f"grouped gemm{' (noalloc)' if noalloc else ''} failed ({rc})")
 
 
 class Server:
@@ -172,6 +174,7 @@ def lib():
     L.blasx_zgemm_crt_device.argtypes = [i, i, i, i, i, _vp, _vp, i64, _vp, i64, _vp, _vp, i64,
                                          i, _vp]
     L.blasx_gemm_grouped_device.argtypes = [C.POINTER(GemmDesc), i, _vp]
+    L.blasx_gemm_grouped_device_noalloc.argtypes = [C.POINTER(GemmDesc), i, _vp]
     L.blasx_server_create.restype = _vp
     L.blasx_server_create.argtypes = [C.c_size_t, i]
     L.blasx_server_register_operand.argtypes = [_vp, _vp, C.c_size_t]

@@ -131,6 +131,8 @@ static void do_init(void)
         bx_log(0, "stream creation failed: %s", bxh_last_error());
         return;
     }
+    /* the allocation-free grouped entry point's table store on this GPU */
+    if (bxh_noalloc_setup() != 0) bx_log(0, "noalloc table store: %s", bxh_noalloc_error());
     if (g_bx.hbm_budget == 0) {
         size_t fr = 0, tot = 0;
         bxh_mem_get_info(&fr, &tot);

@@ -144,3 +144,77 @@ BX_EXPORT int blasx_gemm_grouped_device(const blasx_gemm_desc_t *d, int n, void 
     }
     return rc;
 }
+
+/* ---- blasx_gemm_grouped_device_noalloc ----
+ * The same launches as blasx_gemm_grouped_device with the problem and
+ * segment tables in the static per-GPU table store of hip/noalloc.hip
+ * instead of a device buffer the library allocates: no hipMalloc, hipFree or
+ * other allocating call on this path.  The host side builds the tables in a
+ * static (pageable) host buffer and copies them on `stream`.  Calls are
+ * ordered on the device: a call's table copy waits for the event the
+ * previous call recorded after its kernels, on whatever stream that ran, so
+ * the store is never overwritten under a running launch.  The store keeps the
+ * last call's tables; nothing in it needs to start at zero. */
+static pthread_mutex_t g_nalock = PTHREAD_MUTEX_INITIALIZER;
+static unsigned char g_na_host[BXH_NOALLOC_SCRATCH_BYTES] __attribute__((aligned(256)));
+static void *g_na_copied; /* recorded after the host tables were last copied */
+
+BX_EXPORT int blasx_gemm_grouped_device_noalloc(const blasx_gemm_desc_t *d, int n, void *stream)
+{
+    if (bx_ensure_init() != 0) return BLASX_ENOGPU;
+    if (n < 0 || (n > 0 && !d)) return BLASX_EINVAL;
+    if (n > BXH_NOALLOC_MAX_PROBLEMS) {
+        bx_log(0, "blasx_gemm_grouped_device_noalloc: %d problems exceed the static table of "
+                  "%d (use blasx_gemm_grouped_device)",
+               n, BXH_NOALLOC_MAX_PROBLEMS);
+        return BLASX_ERANGE;
+    }
+    for (int i = 0; i < n; ++i)
+        if (bx_desc_check(&d[i])) return BLASX_EINVAL;
+    void *st = stream ? stream : g_bx.s_comp;
+    pthread_mutex_lock(&g_nalock);
+    void *store = NULL, *done = NULL;
+    int rc = bxh_noalloc_scratch(&store, &done);
+    if (rc) bx_log(0, "noalloc table store: %s", bxh_noalloc_error());
+    if (!rc && !g_na_copied) rc = bxh_event_create(&g_na_copied, 0);
+    /* the host tables are rewritten only after their last copy has read them */
+    if (!rc) rc = bxh_event_sync(g_na_copied);
+    /* and the device tables only after the previous call's kernels */
+    if (!rc) rc = bxh_stream_wait_event(st, done);
+    for (int i = 0; i < n && !rc; ++i) {
+        const int az = d[i].alpha[0] == 0.0 && (d[i].dtype != BLASX_Z || d[i].alpha[1] == 0.0);
+        if (d[i].M > 0 && d[i].N > 0 && (d[i].K == 0 || az))
+            rc = bxh_scale(d[i].dtype, d[i].M, d[i].N, d[i].beta, d[i].C, d[i].ldc, st);
+    }
+    const int dts[3] = {BLASX_S, BLASX_D, BLASX_Z};
+    int launches = 0;
+    double flops = 0;
+    for (int k = 0; k < 3 && !rc && n > 0; ++k) {
+        unsigned char *h = g_na_host + (size_t)k * BXH_NOALLOC_SLOT_BYTES;
+        char *dv = (char *)store + (size_t)k * BXH_NOALLOC_SLOT_BYTES;
+        bxh_problem_t *probs = (bxh_problem_t *)h;
+        bxh_seg_t *segs = (bxh_seg_t *)(h + BXH_NOALLOC_SEG_OFFSET);
+        int np = 0;
+        const int total = bx_grouped_fill(d, n, dts[k], probs, segs, &np);
+        if (np == 0) continue;
+        rc = bxh_memcpy_h2d_async(dv, h, (size_t)np * sizeof(bxh_problem_t), st);
+        if (!rc)
+            rc = bxh_memcpy_h2d_async(dv + BXH_NOALLOC_SEG_OFFSET, h + BXH_NOALLOC_SEG_OFFSET,
+                                      (size_t)np * sizeof(bxh_seg_t), st);
+        if (!rc)
+            rc = bxh_gemm_grouped(dts[k], (const bxh_problem_t *)dv, np,
+                                  (const bxh_seg_t *)(dv + BXH_NOALLOC_SEG_OFFSET), total, 0, st);
+        ++launches;
+        for (int i = 0; i < n; ++i)
+            if (d[i].dtype == dts[k])
+                flops += (dts[k] == BLASX_Z ? 8.0 : 2.0) * d[i].M * (double)d[i].N * d[i].K;
+    }
+    if (!rc) rc = bxh_event_record(g_na_copied, st);
+    if (!rc) rc = bxh_event_record(done, st);
+    pthread_mutex_unlock(&g_nalock);
+    if (!rc) {
+        BX_STAT_ADD(kernel_launches, launches);
+        bx_stat_flops(flops);
+    }
+    return rc;
+}

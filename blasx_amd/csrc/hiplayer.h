@@ -147,6 +147,23 @@ int bxh_ccombine(int M, int N, const float *P, int64_t ldp, const double *alpha,
 /* streaming device copy kernel (16-byte aligned, multiple of 16 bytes) */
 int bxh_copy_kernel(void *dst, const void *src, size_t bytes, void *stream);
 
+/* ---- allocation-free grouped GEMM (hip/noalloc.hip) ----
+ * The problem and segment tables of up to BXH_NOALLOC_MAX_PROBLEMS
+ * problems per element type live in a static device array, one per GPU:
+ * three slots (S, D, Z) of BXH_NOALLOC_SLOT_BYTES, problems first, then
+ * segments. */
+#define BXH_NOALLOC_MAX_PROBLEMS 1024
+#define BXH_NOALLOC_SEG_OFFSET ((size_t)BXH_NOALLOC_MAX_PROBLEMS * sizeof(bxh_problem_t))
+#define BXH_NOALLOC_SLOT_BYTES \
+    ((BXH_NOALLOC_SEG_OFFSET + (size_t)BXH_NOALLOC_MAX_PROBLEMS * sizeof(bxh_seg_t) + 255) & \
+     ~(size_t)255)
+#define BXH_NOALLOC_SCRATCH_BYTES (3 * BXH_NOALLOC_SLOT_BYTES)
+int bxh_noalloc_setup(void);
+/* the current device's copy of the table store and the event recorded after
+ * the last call's kernels (set up on first use) */
+int bxh_noalloc_scratch(void **ptr, void **done_event);
+const char *bxh_noalloc_error(void);
+
 #ifdef __cplusplus
 }
 #endif

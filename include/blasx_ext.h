@@ -191,6 +191,20 @@ typedef struct blasx_gemm_desc {
 
 int blasx_gemm_grouped_device(const blasx_gemm_desc_t *descs, int n, void *stream);
 
+/* Allocation-free variant of blasx_gemm_grouped_device.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and kernels; the problem tables live
+ * in a static per-GPU table store (about 400 KiB, held until the process
+ * ends) instead of a buffer the library allocates, so at most
+ * BLASX_NOALLOC_MAX_PROBLEMS descriptors per call (more: BLASX_ERANGE).  The
+ * call allocates nothing, on the device or the host.  Calls share the
+ * store: each waits on the device for the previous call's kernels before
+ * rewriting it, so calls on different streams are serialised on the GPU;
+ * their outputs must not overlap. */
+#define BLASX_NOALLOC_MAX_PROBLEMS 1024
+int blasx_gemm_grouped_device_noalloc(const blasx_gemm_desc_t *descs, int n, void *stream);
+
 
 /* ---- GEMM request server (continuous batching) ------------------------------
  * No counterpart in the reference (its ABI is one blocking, non-reentrant
