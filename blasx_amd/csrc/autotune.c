@@ -4,8 +4,11 @@
  * The reference fixes the tile edge at compile time (BLOCKSIZE_DGEMM 2048,
  * BLOCKSIZE_ZGEMM 1024, include/blasx.h:5-7).  Here the edge is chosen per
  * call from a table of measured winners (written by blasx_autotune_tiles)
- * and otherwise by the heuristic in tile_rt.c.  Table file: one entry per
- * line, "dtype M N K T", nearest (log-size) entry of the same dtype wins.
+ * and otherwise by the heuristic in tile_rt.c.  The table starts with the
+ * entries measured on an MI355X by blasx_amd/tune.py (tune_table.inc, built
+ * in), then those of $BLASX_TUNE_FILE (one entry per line, "dtype M N K T"),
+ * then those added at run time; the nearest (log-size) entry of the same
+ * dtype wins, the later one on a tie.
  */
 #define _GNU_SOURCE
 #include <math.h>
@@ -23,9 +26,23 @@ static struct {
 static int g_n = -1;
 static pthread_mutex_t g_lock = PTHREAD_MUTEX_INITIALIZER;
 
+/* measured defaults: {dtype, M, N, K, T} */
+static const int g_builtin[][5] = {
+#include "tune_table.inc"
+    {-1, 0, 0, 0, 0}};
+
 static void load_table(void)
 {
     g_n = 0;
+    for (size_t i = 0; i < sizeof g_builtin / sizeof g_builtin[0] && g_n < MAX_ENTRIES; ++i) {
+        if (g_builtin[i][0] < 0) continue;
+        g_tab[g_n].dt = g_builtin[i][0];
+        g_tab[g_n].M = g_builtin[i][1];
+        g_tab[g_n].N = g_builtin[i][2];
+        g_tab[g_n].K = g_builtin[i][3];
+        g_tab[g_n].T = g_builtin[i][4];
+        ++g_n;
+    }
     const char *path = getenv("BLASX_TUNE_FILE");
     if (!path || !*path) return;
     FILE *f = fopen(path, "r");
@@ -46,7 +63,7 @@ int bx_tune_lookup(int dt, int M, int N, int K)
         if (g_tab[i].dt != dt) continue;
         double d = fabs(log((double)M / g_tab[i].M)) + fabs(log((double)N / g_tab[i].N)) +
                    fabs(log((double)K / g_tab[i].K));
-        if (d < bd) {
+        if (d <= bd) { /* later entries win ties: run-time over file over built-in */
             bd = d;
             best = g_tab[i].T;
         }

@@ -7,14 +7,31 @@ import pytest
 
 
 def test_tune_table_lookup_cpu(bx):
+    """A measured entry overrides the heuristic (or the built-in MI355X table,
+    csrc/tune_table.inc, whose entries are cubes) for nearby shapes only.
+    Shapes here are far from any cube."""
     L = bx.test_lib()
     L.blasx_tune_table_add.argtypes = [C.c_int] * 5
-    # a measured entry overrides the heuristic for nearby shapes only
-    assert L.bxtest_autotune_tile(bx.D, 10000, 10000, 10000) == 2048  # heuristic
-    assert L.blasx_tune_table_add(bx.D, 10000, 10000, 10000, 512) == 0
-    assert L.bxtest_autotune_tile(bx.D, 10100, 9900, 10000) == 512
-    assert L.bxtest_autotune_tile(bx.D, 40000, 40000, 40000) == 2048
-    assert L.bxtest_autotune_tile(bx.S, 10000, 10000, 10000) == 2048
+    assert L.bxtest_autotune_tile(bx.D, 200000, 3000, 3000) == 2048  # heuristic
+    assert L.blasx_tune_table_add(bx.D, 200000, 3000, 3000, 512) == 0
+    assert L.bxtest_autotune_tile(bx.D, 202000, 2970, 3000) == 512
+    assert L.bxtest_autotune_tile(bx.D, 200000, 300000, 3000) == 2048
+    assert L.bxtest_autotune_tile(bx.S, 200000, 3000, 3000) == 2048
+
+
+def test_builtin_tune_table_is_consulted_by_default(bx):
+    """Every entry of the built-in table (measured on an MI355X by
+    blasx_amd/tune.py) is what the per-call choice returns at its own shape
+    with no environment table: the tuned edge is on the default path."""
+    import os
+    import re
+
+    L = bx.test_lib()
+    path = os.path.join(os.path.dirname(bx.__file__), "csrc", "tune_table.inc")
+    rows = re.findall(r"\{(\d+), (\d+), (\d+), (\d+), (\d+)\}", open(path).read())
+    for dt, M, N, K, T in (tuple(int(v) for v in r) for r in rows):
+        assert T in (256, 512, 1024, 2048)
+        assert L.bxtest_autotune_tile(dt, M, N, K) == T, (dt, M, N, K, T)
 
 
 @pytest.mark.gpu
