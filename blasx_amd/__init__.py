@@ -196,6 +196,8 @@ def lib():
     L.blasx_dd_gemm_block_host.argtypes = [i, i, i, i, i, d, _vp, i64, _vp, i64, d, _vp, i64,
                                            _vp, i64]
     L.blasx_sample_uniform_host.argtypes = [i, _vp, i64, i, _vp, i, _vp, C.c_uint64, C.c_uint32]
+    L.blasx_dgemm_bf16x3_device.argtypes = [i, i, i, i, i, d, _vp, i64, _vp, i64, d, _vp, i64,
+                                            _vp]
     _lib = L
     return L
 

@@ -28,6 +28,32 @@
 namespace {
 
 typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// The matrix-core instruction of a tile.  INT8: exact int32 sums of
+// V_MFMA_I32_16X16X64_I8 products (the digit and CRT schemes).  BF16: FP32
+// sums of V_MFMA_F32_16X16X32_BF16 products (the BF16x3 split).  Both take 16
+// bytes per lane per operand fragment -- 16 int8 or 8 bf16 consecutive in K,
+// K block lane / 16 -- and give lane (l16, lg) rows 4 lg .. 4 lg + 3 of
+// column l16, so one byte layout, one pipeline and one epilogue serve both.
+struct MacI8 {
+    typedef i32x4 acc_t;
+    __device__ static acc_t zero() { return i32x4{0, 0, 0, 0}; }
+    __device__ static acc_t mac(i32x4 a, i32x4 b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+    }
+};
+struct MacBF16 {
+    typedef f32x4 acc_t;
+    __device__ static acc_t zero() { return f32x4{0.f, 0.f, 0.f, 0.f}; }
+    __device__ static acc_t mac(i32x4 a, i32x4 b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                       __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+    }
+};
 
 constexpr int OZ_BM = 128, OZ_BN = 128, OZ_BK = 64;
 // LDS layout of a staged K tile: row r (of the M or N extent), 16-byte
@@ -165,7 +191,7 @@ __global__ void k_split_strided(const double *__restrict__ X, int64_t ld, int n,
                                 const int *__restrict__ e, int s, int8_t *__restrict__ D,
                                 int np, int kp)
 {
-    __shared__ int8_t tile[8][SP_T][SP_T + 4];
+    __shared__ int8_t tile[9][SP_T][SP_T + 4];
     const int i0 = blockIdx.x * SP_T, k0 = blockIdx.y * SP_T;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6; // 64 x 4
     for (int kk = ty; kk < SP_T; kk += 4) {
@@ -267,8 +293,9 @@ template <int BT> __device__ __forceinline__ void block_origin(int n, int &m0, i
 
 // 128x128 tile: acc[mb][nb] of wave (wm, wn) holds rows
 // m0 + 64 wm + 16 mb + 4 (lane/16) + v, column n0 + 64 wn + 16 nb + lane%16
+template <typename MAC = MacI8>
 __device__ __forceinline__ void tile128(const OzSeg *__restrict__ segs, int nseg, int kp, int m0,
-                                        int n0, i32x4 (&acc)[4][4])
+                                        int n0, typename MAC::acc_t (&acc)[4][4])
 {
     __shared__ __attribute__((aligned(16))) int8_t sA[2][OZ_BM * OZ_STR];
     __shared__ __attribute__((aligned(16))) int8_t sB[2][OZ_BN * OZ_STR];
@@ -278,7 +305,7 @@ __device__ __forceinline__ void tile128(const OzSeg *__restrict__ segs, int nseg
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = i32x4{0, 0, 0, 0};
+        for (int j = 0; j < 4; ++j) acc[i][j] = MAC::zero();
 
     // loader: 128 rows x 64 bytes per operand = 512 x 16 B, two per thread
     const int lrow = tid >> 2, lk = (tid & 3) * 16;
@@ -319,7 +346,7 @@ __device__ __forceinline__ void tile128(const OzSeg *__restrict__ segs, int nseg
         for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
             for (int nb2 = 0; nb2 < 4; ++nb2)
-                acc[mb][nb2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[mb], fb[nb2], acc[mb][nb2], 0, 0, 0);
+                acc[mb][nb2] = MAC::mac(fa[mb], fb[nb2], acc[mb][nb2]);
         if (more) lstore(buf ^ 1);
         __syncthreads();
     }
@@ -396,9 +423,11 @@ __device__ __forceinline__ void dma16(const int8_t *g, const int8_t *lds_wave_ba
                  : "memory");
 }
 
+template <typename MAC = MacI8>
 __device__ __forceinline__ void tile256_seg(const int8_t *__restrict__ gA,
                                             const int8_t *__restrict__ gB, int kp, int m0, int n0,
-                                            i32x4 (&acc)[8][4], int8_t *sA, int8_t *sB)
+                                            typename MAC::acc_t (&acc)[8][4], int8_t *sA,
+                                            int8_t *sB)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -461,8 +490,7 @@ __device__ __forceinline__ void tile256_seg(const int8_t *__restrict__ gA,
         for (int mb = 0; mb < 8; ++mb) {
 #pragma unroll
             for (int nb = 0; nb < 4; ++nb)
-                acc[mb][nb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[mb], fb[P][nb],
-                                                                    acc[mb][nb], 0, 0, 0);
+                acc[mb][nb] = MAC::mac(fa[mb], fb[P][nb], acc[mb][nb]);
             if (mb == 0) {
 #pragma unroll
                 for (int nb = 0; nb < 4; ++nb) fb[P ^ 1][nb] = rdB(nxt, nb);
@@ -499,25 +527,27 @@ __device__ __forceinline__ void tile256_seg(const int8_t *__restrict__ gA,
     __builtin_amdgcn_s_barrier();
 }
 
+template <typename MAC = MacI8>
 __device__ __forceinline__ void tile256(const OzSeg *__restrict__ segs, int nseg, int kp, int m0,
-                                        int n0, i32x4 (&acc)[8][4])
+                                        int n0, typename MAC::acc_t (&acc)[8][4])
 {
     __shared__ __attribute__((aligned(16))) int8_t sA[OZ_NSTAGE * OZ_STAGE];
     __shared__ __attribute__((aligned(16))) int8_t sB[OZ_NSTAGE * OZ_STAGE];
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = i32x4{0, 0, 0, 0};
+        for (int j = 0; j < 4; ++j) acc[i][j] = MAC::zero();
     // each segment restarts the pipeline (its final barrier has retired every
     // read and DMA of the stages the next prologue overwrites)
-    for (int s = 0; s < nseg; ++s) tile256_seg(segs[s].A, segs[s].B, kp, m0, n0, acc, sA, sB);
+    for (int s = 0; s < nseg; ++s)
+        tile256_seg<MAC>(segs[s].A, segs[s].B, kp, m0, n0, acc, sA, sB);
 }
 
 // W = (first ? 0 : W) + scale * (tile sum); rows >= m / columns >= n skipped.
 // Wave (wm, wn) = (wave & 1, wave >> 1) owns rows m0 + WTM wm + 16 mb + ...,
 // columns n0 + WTN wn + 16 nb + lane % 16.
-template <int WTM, int WTN, int NBM, int NBN>
-__device__ __forceinline__ void fold_w(const i32x4 (&acc)[NBM][NBN], int m0, int n0, int m, int n,
+template <int WTM, int WTN, int NBM, int NBN, typename ACC = i32x4>
+__device__ __forceinline__ void fold_w(const ACC (&acc)[NBM][NBN], int m0, int n0, int m, int n,
                                        double *__restrict__ W, int64_t ldw, double scale,
                                        int first)
 {
@@ -563,6 +593,131 @@ __global__ __launch_bounds__(OZ_NT_BIG, 1) void k_ozaki_group_big(const OzSeg *_
     i32x4 acc[8][4];
     tile256(segs, nseg, kp, m0, n0, acc);
     fold_w<128, 64, 8, 4>(acc, m0, n0, m, n, W, ldw, scale, first);
+}
+
+// ---------------------------------------------------------------------------
+// BF16x3 split (the BF16 variant beside the INT8 digit and CRT schemes): the
+// scaled entry x = 2^-e_i a_ik in (-1, 1) is cut into three bf16 pieces
+//   s1 = rn8(x), s2 = rn8(x - s1), s3 = rn8(x - s1 - s2),
+// rn8 rounding to 8 significant bits (nearest, ties to even) on the double's
+// bit pattern; each subtraction is exact in FP64, so x = s1 + s2 + s3 + r
+// with |r| <= 2^-25 |x| (pieces below bf16's normal range are 0).  Products
+// s_p s_q with p + q <= 4 run on the BF16 matrix cores with FP32 sums; the
+// FP64 workspace adds the three groups (p + q = 4, 3, 2).  Planes [p][i][k]
+// of bf16 in the panel layout, K counted in bytes (kp2 = 2 Kp).
+// ---------------------------------------------------------------------------
+__host__ __device__ __forceinline__ double bf16_rn8(double x)
+{
+    uint64_t u;
+    memcpy(&u, &x, 8);
+    u += 0x00000FFFFFFFFFFFull + ((u >> 45) & 1);
+    u &= ~0x00001FFFFFFFFFFFull;
+    double y;
+    memcpy(&y, &u, 8);
+    return fabs(y) < 0x1p-126 ? 0.0 : y;
+}
+
+// bf16 bits of a value rn8 produced (exact: 8 significant bits, normal range)
+__host__ __device__ __forceinline__ uint16_t bf16_bits(double y)
+{
+    const float f = (float)y;
+    uint32_t b;
+    memcpy(&b, &f, 4);
+    return (uint16_t)(b >> 16);
+}
+
+__host__ __device__ __forceinline__ void bf16x3_pieces(double x, uint16_t *out)
+{
+    const double s1 = bf16_rn8(x), r1 = x - s1;
+    const double s2 = bf16_rn8(r1), s3 = bf16_rn8(r1 - s2);
+    out[0] = bf16_bits(s1);
+    out[1] = bf16_bits(s2);
+    out[2] = bf16_bits(s3);
+}
+
+// contiguous rows (x(i,k) = X[k + i*ld]): one thread per 8 consecutive k
+__global__ void k_bf16_split_contig(const double *__restrict__ X, int64_t ld, int n, int len,
+                                    const int *__restrict__ e, int8_t *__restrict__ D, int np,
+                                    int kp2)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int chunks = kp2 / 16;
+    if (t >= (int64_t)np * chunks) return;
+    const int i = (int)(t / chunks), k0 = (int)(t % chunks) * 8;
+    const int ei = i < n ? e[i] : 0;
+    uint16_t pc[3][8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const double x = (i < n && k0 + b < len) ? ldexp(X[k0 + b + (int64_t)i * ld], -ei) : 0.0;
+        uint16_t q[3];
+        bf16x3_pieces(x, q);
+        pc[0][b] = q[0];
+        pc[1][b] = q[1];
+        pc[2][b] = q[2];
+    }
+    for (int p = 0; p < 3; ++p) {
+        i32x4 w;
+        for (int h = 0; h < 4; ++h) w[h] = (int)((uint32_t)pc[p][2 * h] | ((uint32_t)pc[p][2 * h + 1] << 16));
+        *(i32x4 *)(D + (int64_t)p * np * kp2 + panel_off(i, 2 * k0, kp2)) = w;
+    }
+}
+
+// strided rows (x(i,k) = X[i + k*ld]): a 64 x 32 tile through LDS so that
+// the FP64 reads (along i) and the bf16 writes (along k) are both coalesced
+__global__ void k_bf16_split_strided(const double *__restrict__ X, int64_t ld, int n, int len,
+                                     const int *__restrict__ e, int8_t *__restrict__ D, int np,
+                                     int kp2)
+{
+    __shared__ uint16_t tile[3][64][34];
+    const int i0 = blockIdx.x * 64, k0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6; // 64 x 4
+    for (int kk = ty; kk < 32; kk += 4) {
+        const int i = i0 + tx, k = k0 + kk;
+        const double x = (i < n && k < len) ? ldexp(X[i + (int64_t)k * ld], -e[i]) : 0.0;
+        uint16_t q[3];
+        bf16x3_pieces(x, q);
+        tile[0][tx][kk] = q[0];
+        tile[1][tx][kk] = q[1];
+        tile[2][tx][kk] = q[2];
+    }
+    __syncthreads();
+    // write: each thread 8 consecutive k (16 bytes) of one row
+    const int row = threadIdx.x >> 2, kc = (threadIdx.x & 3) * 8;
+    const int i = i0 + row;
+    if (i >= np) return;
+    for (int p = 0; p < 3; ++p) {
+        i32x4 w;
+        for (int h = 0; h < 4; ++h)
+            w[h] = (int)((uint32_t)tile[p][row][kc + 2 * h] |
+                         ((uint32_t)tile[p][row][kc + 2 * h + 1] << 16));
+        *(i32x4 *)(D + (int64_t)p * np * kp2 + panel_off(i, 2 * (k0 + kc), kp2)) = w;
+    }
+}
+
+// W = (first ? 0 : W) + sum over segments of S_A S_B^T on the BF16 matrix
+// cores (FP32 sums per tile)
+__global__ __launch_bounds__(OZ_NT, 2) void k_bf16_group(const OzSeg *__restrict__ segs, int nseg,
+                                                         int kp2, int m, int n,
+                                                         double *__restrict__ W, int64_t ldw,
+                                                         int first)
+{
+    int m0, n0;
+    block_origin<OZ_BM>(n, m0, n0);
+    f32x4 acc[4][4];
+    tile128<MacBF16>(segs, nseg, kp2, m0, n0, acc);
+    fold_w<64, 64, 4, 4, f32x4>(acc, m0, n0, m, n, W, ldw, 1.0, first);
+}
+
+__global__ __launch_bounds__(OZ_NT_BIG, 1) void k_bf16_group_big(const OzSeg *__restrict__ segs,
+                                                                 int nseg, int kp2, int m, int n,
+                                                                 double *__restrict__ W,
+                                                                 int64_t ldw, int first)
+{
+    int m0, n0;
+    block_origin<OZ_BIG>(n, m0, n0);
+    f32x4 acc[8][4];
+    tile256<MacBF16>(segs, nseg, kp2, m0, n0, acc);
+    fold_w<128, 64, 8, 4, f32x4>(acc, m0, n0, m, n, W, ldw, 1.0, first);
 }
 
 // 4. C = alpha * 2^(e_i + f_j) * W + beta * C
@@ -1117,7 +1272,7 @@ int bxh_oz_exp_chunks(void) { return EXP_CHUNKS; }
 int bxh_oz_split(const double *X, int64_t ld, int n, int len, int contiguous, const int *e,
                  int s, int8_t *D, int np, int kp, void *stream)
 {
-    if (s < 1 || s > 8 || np % 64 || kp % 64) return BLASX_EINVAL;
+    if (s < 1 || s > 9 || np % 64 || kp % 64) return BLASX_EINVAL;
     hipStream_t st = (hipStream_t)stream;
     if (contiguous) {
         const int64_t total = (int64_t)np * (kp / 16);
@@ -1153,6 +1308,48 @@ int bxh_oz_group(const void *segs_dev, int nseg, int kp, int m, int n, int mp, i
     hipError_t r = hipGetLastError();
     return r == hipSuccess ? 0 : oz_fail("ozaki group launch", r);
 }
+
+// ---- BF16x3 split ------------------------------------------------------------
+int bxh_bf16_split(const double *X, int64_t ld, int n, int len, int contiguous, const int *e,
+                   int8_t *D, int np, int kp2, void *stream)
+{
+    if (np % 64 || kp2 % 64) return BLASX_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    if (contiguous) {
+        const int64_t total = (int64_t)np * (kp2 / 16);
+        hipLaunchKernelGGL(k_bf16_split_contig, dim3((unsigned)((total + 255) / 256)), dim3(256),
+                           0, st, X, ld, n, len, e, D, np, kp2);
+    } else {
+        hipLaunchKernelGGL(k_bf16_split_strided, dim3(np / 64, kp2 / 64), dim3(256), 0, st, X, ld,
+                           n, len, e, D, np, kp2);
+    }
+    hipError_t r = hipGetLastError();
+    return r == hipSuccess ? 0 : oz_fail("bf16 split launch", r);
+}
+
+int bxh_bf16_group(const void *segs_dev, int nseg, int kp2, int m, int n, int mp, int np,
+                   double *W, int64_t ldw, int first, void *stream)
+{
+    if (nseg <= 0) return 0;
+    if (kp2 <= 0 || mp % OZ_BM || np % OZ_BN || kp2 % OZ_BK) return BLASX_EINVAL;
+    const int big_tiles = (mp % OZ_BIG == 0 && np % OZ_BIG == 0 && (kp2 / OZ_BK) % 2 == 0)
+                              ? (mp / OZ_BIG) * (np / OZ_BIG)
+                              : 0;
+    if (big_tiles >= 256) {
+        hipLaunchKernelGGL(k_bf16_group_big, dim3(big_tiles), dim3(OZ_NT_BIG), 0,
+                           (hipStream_t)stream, (const OzSeg *)segs_dev, nseg, kp2, m, n, W, ldw,
+                           first);
+    } else {
+        const int grid = (mp / OZ_BM) * (np / OZ_BN);
+        hipLaunchKernelGGL(k_bf16_group, dim3(grid), dim3(OZ_NT), 0, (hipStream_t)stream,
+                           (const OzSeg *)segs_dev, nseg, kp2, m, n, W, ldw, first);
+    }
+    hipError_t r = hipGetLastError();
+    return r == hipSuccess ? 0 : oz_fail("bf16 group launch", r);
+}
+
+// the host build of the piece split, for the tests
+void bxh_bf16x3_pieces_host(double x, uint16_t *out) { bf16x3_pieces(x, out); }
 
 // ---- CRT split -------------------------------------------------------------
 // host view of the constant tables (same layout as the kernels' structs)

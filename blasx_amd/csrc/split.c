@@ -218,7 +218,7 @@ BX_EXPORT int blasx_zgemm_split_device(int transa, int transb, int M, int N, int
     if (lda < (nrowa > 1 ? nrowa : 1) || ldb < (nrowb > 1 ? nrowb : 1) || ldc < (M > 1 ? M : 1))
         return BLASX_EINVAL;
     if (slices <= 0) slices = 7;
-    if (slices < 2 || slices > 8) return BLASX_EINVAL;
+    if (slices < 2 || slices > 9) return BLASX_EINVAL;
     return bx_zgemm_split(opa, opb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, slices,
                           stream ? stream : g_bx.s_comp);
 }
@@ -236,7 +236,7 @@ BX_EXPORT int blasx_dgemm_split_device(int transa, int transb, int M, int N, int
     if (lda < (nrowa > 1 ? nrowa : 1) || ldb < (nrowb > 1 ? nrowb : 1) || ldc < (M > 1 ? M : 1))
         return BLASX_EINVAL;
     if (slices <= 0) slices = 7;
-    if (slices < 2 || slices > 8) return BLASX_EINVAL;
+    if (slices < 2 || slices > 9) return BLASX_EINVAL;
     return bx_dgemm_split(opa, opb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, slices,
                           stream ? stream : g_bx.s_comp);
 }

@@ -146,3 +146,7 @@ BX_EXPORT void bxtest_her2k_factor(double re, double im, int conj_ratio, double 
     const double a[2] = {re, im};
     bx_her2k_factor(a, conj_ratio, out);
 }
+
+/* the BF16x3 piece split, host build of the kernels' code (ozaki_kernels.hip) */
+void bxh_bf16x3_pieces_host(double x, uint16_t *out);
+BX_EXPORT void bxtest_bf16x3_pieces(double x, uint16_t *out) { bxh_bf16x3_pieces_host(x, out); }

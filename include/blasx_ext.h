@@ -130,12 +130,15 @@ int blasx_zgemm_device(int transa, int transb, int M, int N, int K, const double
 /* ---- split-precision DGEMM (Ozaki scheme on INT8 MFMA) --------------------
  * No counterpart in the reference.  C = alpha op(A) op(B) + beta C with
  * op(A) rows and op(B) columns scaled by powers of two and cut into
- * `slices` exact 7-bit int8 digits (2..8; 0 = default 7); the products of
+ * `slices` exact 7-bit int8 digits (2..9; 0 = default 7); the products of
  * digit pairs with p + q <= slices + 1 run on the INT8 matrix cores with
  * exact int32 accumulation and are summed in FP64.  K <= 133143.
  * Accuracy: each entry of op(A) keeps 7*slices bits relative to the
  * largest magnitude in its row (op(B): column), so inputs with a wide
- * dynamic range within a row need more slices for FP64-level error. */
+ * dynamic range within a row need more slices for FP64-level error.
+ * Inputs must be finite: a NaN or infinity in a row (column) makes that
+ * row's (column's) scaling meaningless and its results unspecified, as the
+ * digits of a non-finite value do not exist. */
 int blasx_dgemm_split_device(int transa, int transb, int M, int N, int K, double alpha,
                              const double *dA, int64_t lda, const double *dB, int64_t ldb,
                              double beta, double *dC, int64_t ldc, int slices, void *stream);
@@ -169,6 +172,18 @@ int blasx_zgemm_crt_device(int transa, int transb, int M, int N, int K, const do
                            const double *dA, int64_t lda, const double *dB, int64_t ldb,
                            const double *beta, double *dC, int64_t ldc, int moduli,
                            void *stream);
+
+/* ---- BF16x3 split DGEMM (BF16 matrix cores) ---------------------------------
+ * No counterpart in the reference.  op(A) rows / op(B) columns are scaled
+ * by powers of two into (-1, 1) and each entry is cut into three bf16
+ * pieces (about 24 significant bits); the six piece products with
+ * p + q <= 4 run on the BF16 matrix cores with FP32 sums over K, added into
+ * an FP64 workspace; C = alpha 2^(e_i + f_j) W + beta C in FP64.  Accuracy
+ * is FP32-like (about sqrt(K) 2^-24 relative to sum_k |a_ik b_kj|), not
+ * FP64: a fast variant for callers that accept it.  Inputs must be finite. */
+int blasx_dgemm_bf16x3_device(int transa, int transb, int M, int N, int K, double alpha,
+                              const double *dA, int64_t lda, const double *dB, int64_t ldb,
+                              double beta, double *dC, int64_t ldc, void *stream);
 
 /* ---- grouped GEMM -----------------------------------------------------------
  * No counterpart in the reference (which issues one vendor call per tile
@@ -204,7 +219,6 @@ int blasx_gemm_grouped_device(const blasx_gemm_desc_t *descs, int n, void *strea
  * their outputs must not overlap. */
 #define BLASX_NOALLOC_MAX_PROBLEMS 1024
 int blasx_gemm_grouped_device_noalloc(const blasx_gemm_desc_t *descs, int n, void *stream);
-
 
 /* ---- GEMM request server (continuous batching) ------------------------------
  * No counterpart in the reference (its ABI is one blocking, non-reentrant

@@ -54,6 +54,11 @@ def lib():
         for nm, p in (("orc_fill_uniform_d", _dp), ("orc_fill_uniform_s", _fp),
                       ("orc_fill_uniform_z", _dp)):
             getattr(L, nm).argtypes = [p, C.c_int64, C.c_int64, C.c_int64, C.c_uint64, C.c_uint32]
+        L.orc_bf16_rn8.restype = C.c_double
+        L.orc_bf16_rn8.argtypes = [C.c_double]
+        L.orc_bf16x3_pieces.argtypes = [C.c_double, _dp]
+        L.orc_dgemm_bf16x3.argtypes = [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, C.c_double,
+                                       _dp, C.c_int, _dp, C.c_int, C.c_double, _dp, C.c_int]
         L.orc_sample_uniform.argtypes = [_dp, C.c_int, C.c_int64, C.c_int, _i64p, C.c_int,
                                          _i64p, C.c_uint64, C.c_uint32]
         L.orc_fill_dynrange_d.argtypes = [_dp, C.c_int64, C.c_int64, C.c_int64, C.c_uint64,
@@ -243,7 +248,7 @@ def gemm_dd(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C_, ldc):
 
 
 def ozaki_digits(x, e, s):
-    d = (C.c_int8 * 8)()
+    d = (C.c_int8 * 9)()
     lib().orc_ozaki_digits(x, e, s, d)
     return [int(d[t]) for t in range(s)]
 
@@ -272,6 +277,22 @@ def zgemm_ozaki(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C_, ldc, s, segmax
     lib().orc_zgemm_ozaki(ta.encode(), tb.encode(), M, N, K, _ptr(al, _dp),
                           A.ctypes.data_as(_dp), lda, B.ctypes.data_as(_dp), ldb, _ptr(be, _dp),
                           C_.ctypes.data_as(_dp), ldc, s, segmax)
+    return C_
+
+
+def bf16x3_pieces(x):
+    out = (C.c_double * 3)()
+    lib().orc_bf16x3_pieces(float(x), out)
+    return [out[0], out[1], out[2]]
+
+
+def bf16_rn8(x):
+    return lib().orc_bf16_rn8(float(x))
+
+
+def dgemm_bf16x3(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C_, ldc):
+    lib().orc_dgemm_bf16x3(ta.encode(), tb.encode(), M, N, K, alpha, _ptr(A, _dp), lda,
+                           _ptr(B, _dp), ldb, beta, _ptr(C_, _dp), ldc)
     return C_
 
 

@@ -500,7 +500,7 @@ void orc_dgemm_ozaki(char ta, char tb, int M, int N, int K, double alpha, const 
         for (int p = 0; p < K; ++p) m = fmax(m, fabs(A[OPIDX(ta, i, p, lda)]));
         ea[i] = orc_exp_of(m);
         for (int p = 0; p < K; ++p) {
-            int8_t d[8];
+            int8_t d[9];
             orc_ozaki_digits(A[OPIDX(ta, i, p, lda)], ea[i], s, d);
             for (int t = 0; t < s; ++t) DA[((size_t)t * M + i) * K + p] = d[t];
         }
@@ -510,7 +510,7 @@ void orc_dgemm_ozaki(char ta, char tb, int M, int N, int K, double alpha, const 
         for (int p = 0; p < K; ++p) m = fmax(m, fabs(B[OPIDX(tb, p, j, ldb)]));
         eb[j] = orc_exp_of(m);
         for (int p = 0; p < K; ++p) {
-            int8_t d[8];
+            int8_t d[9];
             orc_ozaki_digits(B[OPIDX(tb, p, j, ldb)], eb[j], s, d);
             for (int t = 0; t < s; ++t) DB[((size_t)t * N + j) * K + p] = d[t];
         }
@@ -956,4 +956,88 @@ void orc_zgemm_crt(char ta, char tb, int M, int N, int K, const double *alpha, c
     free(RB);
     free(ea);
     free(eb);
+}
+
+/* ------------------------------------------------------------------ */
+/* BF16x3 split DGEMM restated on the CPU (no reference counterpart):  */
+/*   e_i, f_j as the digit scheme (frexp exponent of the row / column   */
+/*   maximum); x = 2^-e a in (-1,1);                                    */
+/*   s1 = rn8(x), s2 = rn8(x - s1), s3 = rn8(x - s1 - s2), rn8 rounding */
+/*   to 8 significant bits, nearest, ties to even, and giving 0 below   */
+/*   2^-126 (bf16's normal range);                                      */
+/*   W = sum over groups g = 4, 3, 2 of sum_{p+q=g} sum_k s_p s_q,      */
+/*   here in FP64 (the library sums each group in FP32 on the matrix    */
+/*   cores: the two agree within that rounding, and bit for bit when    */
+/*   every partial sum is exact);                                       */
+/*   C = alpha 2^(e_i+f_j) W (+ fma(beta, C, .)).                       */
+/* ------------------------------------------------------------------ */
+double orc_bf16_rn8(double x)
+{
+    if (x == 0.0 || !isfinite(x)) return x;
+    int e;
+    const double m = frexp(x, &e); /* |m| in [0.5, 1) */
+    /* m * 2^8 has 8 integer bits; round half to even */
+    const double t = ldexp(m, 8), r = nearbyint(t); /* default mode: nearest-even */
+    const double y = ldexp(r, e - 8);
+    return fabs(y) < ldexp(1.0, -126) ? 0.0 : y;
+}
+
+void orc_bf16x3_pieces(double x, double *out)
+{
+    const double s1 = orc_bf16_rn8(x), r1 = x - s1;
+    const double s2 = orc_bf16_rn8(r1), s3 = orc_bf16_rn8(r1 - s2);
+    out[0] = s1;
+    out[1] = s2;
+    out[2] = s3;
+}
+
+void orc_dgemm_bf16x3(char ta, char tb, int M, int N, int K, double alpha, const double *A,
+                      int lda, const double *B, int ldb, double beta, double *C, int ldc)
+{
+    int *ea = (int *)calloc(M + 1, sizeof(int)), *eb = (int *)calloc(N + 1, sizeof(int));
+    double *PA = (double *)calloc((size_t)3 * M * (K > 0 ? K : 1), sizeof(double));
+    double *PB = (double *)calloc((size_t)3 * N * (K > 0 ? K : 1), sizeof(double));
+    for (int i = 0; i < M; ++i) {
+        double m = 0;
+        for (int p = 0; p < K; ++p) m = fmax(m, fabs(A[OPIDX(ta, i, p, lda)]));
+        ea[i] = orc_exp_of(m);
+        for (int p = 0; p < K; ++p) {
+            double q[3];
+            orc_bf16x3_pieces(ldexp(A[OPIDX(ta, i, p, lda)], -ea[i]), q);
+            for (int t = 0; t < 3; ++t) PA[((size_t)t * M + i) * K + p] = q[t];
+        }
+    }
+    for (int j = 0; j < N; ++j) {
+        double m = 0;
+        for (int p = 0; p < K; ++p) m = fmax(m, fabs(B[OPIDX(tb, p, j, ldb)]));
+        eb[j] = orc_exp_of(m);
+        for (int p = 0; p < K; ++p) {
+            double q[3];
+            orc_bf16x3_pieces(ldexp(B[OPIDX(tb, p, j, ldb)], -eb[j]), q);
+            for (int t = 0; t < 3; ++t) PB[((size_t)t * N + j) * K + p] = q[t];
+        }
+    }
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < M; ++i) {
+            double w = 0;
+            for (int g = 4; g >= 2; --g) {
+                double sg = 0;
+                for (int p = 1; p <= 3; ++p) {
+                    const int q = g - p;
+                    if (q < 1 || q > 3) continue;
+                    const double *pa = PA + ((size_t)(p - 1) * M + i) * K;
+                    const double *pb = PB + ((size_t)(q - 1) * N + j) * K;
+                    for (int k = 0; k < K; ++k) sg += pa[k] * pb[k];
+                }
+                w += sg;
+            }
+            const double v = alpha * ldexp(w, ea[i] + eb[j]);
+            double *c = &C[i + (int64_t)j * ldc];
+            *c = beta == 0.0 ? v : fma(beta, *c, v);
+        }
+    free(ea);
+    free(eb);
+    free(PA);
+    free(PB);
 }

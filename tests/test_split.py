@@ -78,15 +78,15 @@ def test_oracle_split_error_vs_dd_falls_with_slices(orc):
     hi, lo = orc.gemm_dd("N", "N", M, N, K, 1.0, A, M, B, K, 0.0, C0, M)
     scale = np.abs(A) @ np.abs(B)
     errs = []
-    for s in (3, 4, 5, 6, 7, 8):
+    for s in (3, 4, 5, 6, 7, 8, 9):
         out = np.zeros((M, N), order="F")
         orc.dgemm_ozaki("N", "N", M, N, K, 1.0, A, M, B, K, 0.0, out, M, s)
         errs.append(np.max(np.abs(out - hi) / scale))
     assert all(errs[i + 1] < errs[i] for i in range(4)), errs
-    # with 8 slices the split product is as accurate as FP64 GEMM allows
+    # with 8 and 9 slices the split product is as accurate as FP64 GEMM allows
     plain = np.zeros((M, N), order="F")
     orc.gemm("N", "N", M, N, K, 1.0, A, M, B, K, 0.0, plain, M)
-    assert errs[-1] <= 2 * np.max(np.abs(plain - hi) / scale) + 4 * EPS
+    assert max(errs[-2:]) <= 2 * np.max(np.abs(plain - hi) / scale) + 4 * EPS
 
 
 # ------------------------------------------------------------------ GPU
@@ -96,7 +96,7 @@ def say(*a):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("s", [3, 5, 7])
+@pytest.mark.parametrize("s", [3, 5, 7, 9])
 def test_gpu_split_bit_exact_vs_oracle(gpu, orc, s):
     bx = gpu
     M, N, K = 203, 151, 333
