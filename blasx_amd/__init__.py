@@ -175,6 +175,9 @@ def lib():
                                          i, _vp]
     L.blasx_gemm_grouped_device.argtypes = [C.POINTER(GemmDesc), i, _vp]
     L.blasx_gemm_grouped_device_noalloc.argtypes = [C.POINTER(GemmDesc), i, _vp]
+    L.blasx_set_grouped_shape.argtypes = [i, i]
+    L.blasx_grouped_shape.argtypes = [i, i, i, i, i64]
+    L.blasx_grouped_shape_add.argtypes = [i, i, i, i, i64, i]
     L.blasx_server_create.restype = _vp
     L.blasx_server_create.argtypes = [C.c_size_t, i]
     L.blasx_server_register_operand.argtypes = [_vp, _vp, C.c_size_t]

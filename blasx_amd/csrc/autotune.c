@@ -153,3 +153,146 @@ BX_EXPORT int blasx_tune_table_add(int dtype, int M, int N, int K, int T)
     pthread_mutex_unlock(&g_lock);
     return rc;
 }
+
+/* ---- block tile shapes of the grouped launches ------------------------------
+ * A grouped launch runs each real problem on 128 x 128, 128 x 64 or 64 x 64
+ * block tiles (BXH_SHAPE_*); complex problems keep 128 x 64.  Smaller tiles
+ * give a launch with few tiles more blocks to spread over the CUs, at a
+ * lower rate per tile.  The table holds measured launch times per shape for
+ * workloads of equal problems (tune_grouped.inc, written by
+ * blasx_amd/tune.py --grouped on an MI355X; entries added at run time carry
+ * only a winner).  A query takes the workload nearest in log M, N, K, then
+ * the shape with the least time at the query's load (the launch's tile count
+ * at the large tile, same element type), each shape's log time interpolated
+ * linearly in log load between the two measured loads around it (the
+ * nearest one outside them); with no entry, a heuristic on the load. */
+#define MAX_GENTRIES 512
+typedef struct {
+    int dt;
+    double lm, ln, lk, ll; /* logs of M, N, K, load */
+    double lt[BXH_NSHAPES]; /* logs of the launch times */
+} gentry_t;
+static gentry_t g_gtab[MAX_GENTRIES];
+static int g_gn = -1;
+static int g_gforce[3] = {-1, -1, -1};
+
+/* measured: {dtype, M, N, K, load, t0, t1, t2} (microseconds) */
+static const long long g_gbuiltin[][8] = {
+#include "tune_grouped.inc"
+    {-1, 0, 0, 0, 0, 0, 0, 0}};
+
+static void gadd(int dt, double M, double N, double K, double load, const double *t)
+{
+    if (g_gn >= MAX_GENTRIES) return;
+    gentry_t *e = &g_gtab[g_gn++];
+    e->dt = dt;
+    e->lm = log(M);
+    e->ln = log(N);
+    e->lk = log(K);
+    e->ll = log(load);
+    for (int s = 0; s < BXH_NSHAPES; ++s) e->lt[s] = log(t[s]);
+}
+
+static void load_gtable(void)
+{
+    g_gn = 0;
+    for (size_t i = 0; i < sizeof g_gbuiltin / sizeof g_gbuiltin[0]; ++i) {
+        const long long *r = g_gbuiltin[i];
+        if (r[0] < 0 || r[1] <= 0 || r[2] <= 0 || r[3] <= 0 || r[4] <= 0) continue;
+        if (r[5] <= 0 || r[6] <= 0 || r[7] <= 0) continue;
+        const double t[3] = {(double)r[5], (double)r[6], (double)r[7]};
+        gadd((int)r[0], (double)r[1], (double)r[2], (double)r[3], (double)r[4], t);
+    }
+}
+
+static int argmin3(const double *v)
+{
+    int b = 0;
+    for (int s = 1; s < BXH_NSHAPES; ++s)
+        if (v[s] < v[b]) b = s;
+    return b;
+}
+
+/* the table's choice, or -1 without an entry of this dtype; g_lock held */
+static int gtable_choice(int dt, int M, int N, int K, int64_t load)
+{
+    const double lm = log((double)M), ln = log((double)N), lk = log((double)K),
+                 ll = log((double)load);
+    /* nearest workload (later entries win ties) */
+    int key = -1;
+    double bd = 1e30;
+    for (int i = 0; i < g_gn; ++i) {
+        const gentry_t *e = &g_gtab[i];
+        if (e->dt != dt) continue;
+        const double d = fabs(lm - e->lm) + fabs(ln - e->ln) + fabs(lk - e->lk);
+        if (d <= bd) {
+            bd = d;
+            key = i;
+        }
+    }
+    if (key < 0) return -1;
+    const gentry_t *k = &g_gtab[key];
+    /* that workload's measured loads around the query (later entries win
+     * equal loads) */
+    int lo = -1, hi = -1;
+    for (int i = 0; i < g_gn; ++i) {
+        const gentry_t *e = &g_gtab[i];
+        if (e->dt != dt || e->lm != k->lm || e->ln != k->ln || e->lk != k->lk) continue;
+        if (e->ll <= ll && (lo < 0 || e->ll >= g_gtab[lo].ll)) lo = i;
+        if (e->ll >= ll && (hi < 0 || e->ll <= g_gtab[hi].ll)) hi = i;
+    }
+    if (lo < 0) return argmin3(g_gtab[hi].lt);
+    if (hi < 0 || g_gtab[hi].ll == g_gtab[lo].ll) return argmin3(g_gtab[lo].lt);
+    const double f = (ll - g_gtab[lo].ll) / (g_gtab[hi].ll - g_gtab[lo].ll);
+    double t[BXH_NSHAPES];
+    for (int s = 0; s < BXH_NSHAPES; ++s)
+        t[s] = g_gtab[lo].lt[s] + f * (g_gtab[hi].lt[s] - g_gtab[lo].lt[s]);
+    return argmin3(t);
+}
+
+int bx_grouped_shape(int dt, int M, int N, int K, int64_t load)
+{
+    if (dt != BLASX_S && dt != BLASX_D) return BXH_SHAPE_128x128;
+    if (M <= 0 || N <= 0 || K <= 0 || load <= 0) return BXH_SHAPE_128x128;
+    pthread_mutex_lock(&g_lock);
+    if (g_gn < 0) load_gtable();
+    int shape = g_gforce[dt];
+    if (shape < 0) shape = gtable_choice(dt, M, N, K, load);
+    pthread_mutex_unlock(&g_lock);
+    if (shape >= 0) return shape;
+    /* no measurement: the large tile once the launch fills every CU twice */
+    const int64_t slots = 2 * (int64_t)(g_bx.cus > 0 ? g_bx.cus : 256);
+    if (load >= slots) return BXH_SHAPE_128x128;
+    return 2 * load >= slots ? BXH_SHAPE_128x64 : BXH_SHAPE_64x64;
+}
+
+/* force one shape for a real element type (-1: back to the table) */
+BX_EXPORT int blasx_set_grouped_shape(int dtype, int shape)
+{
+    if (dtype != BLASX_S && dtype != BLASX_D) return BLASX_EINVAL;
+    if (shape < -1 || shape >= BXH_NSHAPES) return BLASX_EINVAL;
+    pthread_mutex_lock(&g_lock);
+    g_gforce[dtype] = shape;
+    pthread_mutex_unlock(&g_lock);
+    return 0;
+}
+
+BX_EXPORT int blasx_grouped_shape(int dtype, int M, int N, int K, int64_t load)
+{
+    return bx_grouped_shape(dtype, M, N, K, load);
+}
+
+BX_EXPORT int blasx_grouped_shape_add(int dtype, int M, int N, int K, int64_t load, int shape)
+{
+    if (dtype != BLASX_S && dtype != BLASX_D) return BLASX_EINVAL;
+    if (M <= 0 || N <= 0 || K <= 0 || load <= 0 || shape < 0 || shape >= BXH_NSHAPES)
+        return BLASX_EINVAL;
+    pthread_mutex_lock(&g_lock);
+    if (g_gn < 0) load_gtable();
+    const int rc = g_gn < MAX_GENTRIES ? 0 : BLASX_ERANGE;
+    double t[BXH_NSHAPES];
+    for (int i = 0; i < BXH_NSHAPES; ++i) t[i] = i == shape ? 1.0 : 2.0; /* a winner only */
+    if (!rc) gadd(dtype, M, N, K, (double)load, t);
+    pthread_mutex_unlock(&g_lock);
+    return rc;
+}

@@ -35,20 +35,34 @@ int bx_desc_check(const blasx_gemm_desc_t *d)
     return 0;
 }
 
+static int desc_runs(const blasx_gemm_desc_t *x, int dtype)
+{
+    if (x->dtype != dtype || x->M == 0 || x->N == 0) return 0;
+    const int az = x->alpha[0] == 0.0 && (dtype != BLASX_Z || x->alpha[1] == 0.0);
+    return x->K != 0 && !az;
+}
+
 /* Fill problem/segment tables for the descriptors of one dtype (device
  * pointers in the descriptors).  Descriptors with alpha == 0 or K == 0 are
- * skipped (the caller scales C separately).  Returns the tile total. */
+ * skipped (the caller scales C separately).  Each problem's block tile
+ * shape comes from bx_grouped_shape, given the launch's load: its tile
+ * count at the large tile.  Returns the tile total. */
 int bx_grouped_fill(const blasx_gemm_desc_t *d, int n, int dtype, bxh_problem_t *probs,
                     bxh_seg_t *segs, int *nprob)
 {
-    const int bm = bxh_tile_m(dtype), bn = bxh_tile_n(dtype);
+    const int bm0 = bxh_tile_m(dtype), bn0 = bxh_tile_n(dtype);
+    int64_t load = 0;
+    for (int i = 0; i < n; ++i)
+        if (desc_runs(&d[i], dtype))
+            load += (int64_t)((d[i].M + bm0 - 1) / bm0) * ((d[i].N + bn0 - 1) / bn0);
     int np = 0, total = 0;
     for (int i = 0; i < n; ++i) {
         const blasx_gemm_desc_t *x = &d[i];
-        if (x->dtype != dtype || x->M == 0 || x->N == 0) continue;
-        const int az = x->alpha[0] == 0.0 && (dtype != BLASX_Z || x->alpha[1] == 0.0);
-        if (x->K == 0 || az) continue;
+        if (!desc_runs(x, dtype)) continue;
         bxh_problem_t *P = &probs[np];
+        int bm, bn;
+        const int shape = bx_grouped_shape(dtype, x->M, x->N, x->K, load);
+        bxh_tile_dims(dtype, shape, &bm, &bn);
         memset(P, 0, sizeof *P);
         P->C = x->C;
         P->ldc = x->ldc;
@@ -58,6 +72,7 @@ int bx_grouped_fill(const blasx_gemm_desc_t *d, int n, int dtype, bxh_problem_t 
         P->seg_off = np;
         P->opa = op_of(dtype, x->transa);
         P->opb = op_of(dtype, x->transb);
+        P->shape = shape;
         P->tiles_m = (x->M + bm - 1) / bm;
         P->tiles_n = (x->N + bn - 1) / bn;
         P->tile_start = total;

@@ -21,6 +21,7 @@
 // pointers: the tile runtime passes one segment per cached K tile, so one
 // launch covers a whole row of tile products without re-reading C.
 #include <hip/hip_runtime.h>
+#include <pthread.h>
 #include <stdint.h>
 
 #include "../hiplayer.h"
@@ -63,12 +64,37 @@ enum { MAJ_MN = 0, MAJ_K = 1 };
 template <typename E, int VEC> struct alignas(16) Vec {
     E e[VEC];
 };
+// N consecutive elements as one access of N * sizeof(E) bytes
+template <typename E, int N> struct alignas(sizeof(E) * N) Lv {
+    E e[N];
+};
 
 template <typename E> __device__ __forceinline__ E zero_of() { return E(0); }
 template <> __device__ __forceinline__ cplx zero_of<cplx>() { return cplx{0.0, 0.0}; }
 
 template <typename E> __device__ __forceinline__ E conj_of(E x) { return x; }
 template <> __device__ __forceinline__ cplx conj_of<cplx>(cplx x) { return cplx{x.re, -x.im}; }
+
+// The epilogue's arithmetic, one rounding sequence for every kernel (explicit
+// FMAs: no result depends on how the compiler contracts a kernel's code).
+// alpha * s, then + beta * c in one FMA; c is not used when beta_zero.
+template <typename E>
+__device__ __forceinline__ E out_real(E alpha, E s, E beta, E c, bool beta_zero)
+{
+    const E x = alpha * s;
+    return beta_zero ? x : fma(beta, c, x);
+}
+// complex: alpha (sr + i sim), then + beta c
+__device__ __forceinline__ cplx out_cplx(double ar, double ai, double sr, double sim, double br,
+                                         double bi, cplx c, bool beta_zero)
+{
+    double orr = fma(ar, sr, -(ai * sim)), oi = fma(ar, sim, ai * sr);
+    if (!beta_zero) {
+        orr = fma(br, c.re, fma(-bi, c.im, orr));
+        oi = fma(br, c.im, fma(bi, c.re, oi));
+    }
+    return cplx{orr, oi};
+}
 
 __device__ __forceinline__ d4_t mfma(double a, double b, d4_t c)
 {
@@ -203,32 +229,67 @@ __device__ __forceinline__ void tile_coords(int t, int tiles_m, int tiles_n, int
 // ---------------------------------------------------------------------------
 // LDS -> MFMA for one staged K tile (real types)
 // ---------------------------------------------------------------------------
-template <typename E>
-__device__ __forceinline__ void mma_real(const E *__restrict__ As, const E *__restrict__ Bs,
-                                         typename Cfg<E>::acc_t (&acc)[4][4], int wm, int wn,
-                                         int lane)
+// Block tile of the register-staged kernels: 2x2 waves, each WTM x WTN
+// (real types; 64 x 64 is the large tile, 32 x 32 and 64 x 32 the small ones
+// the grouped launches may pick per problem), or Cfg<cplx>'s 128 x 64.  A
+// lane reads RV consecutive rows of a stage with one LDS read and so feeds
+// RV MFMA blocks: rows inside a wave sub-tile are permuted that way, and
+// un-permuted in the epilogue.
+template <typename E, int WTM = 64, int WTN = 64> struct Blk {
+    static constexpr bool CPLX = sizeof(E) == 16;
+    static_assert(!CPLX || (WTM == 64 && WTN == 64), "complex: one tile shape");
+    static constexpr int BM = CPLX ? Cfg<E>::BM : 2 * WTM;
+    static constexpr int BN = CPLX ? Cfg<E>::BN : 2 * WTN;
+    static constexpr int FM = WTM / 16, FN = WTN / 16; // MFMA blocks per wave
+    static constexpr int VEC = Cfg<E>::VEC;
+    static constexpr int RVA = VEC < FM ? VEC : FM, RVB = VEC < FN ? VEC : FN;
+    static constexpr int STRA = BM + Cfg<E>::PADA, STRB = BN + Cfg<E>::PADB;
+    static constexpr int A_ELEMS = Cfg<E>::BK * STRA, B_ELEMS = Cfg<E>::BK * STRB;
+    static constexpr int ELEMS = 2 * (A_ELEMS + B_ELEMS);
+};
+
+// The K value lane group lg (lane / 16) feeds at K step s of a K tile: the
+// order the DMA-staged kernels' LDS layouts need (dma64::frag, dmas::frag),
+// used by the register-staged block too, so that every kernel and block
+// shape sums a C element's products in one order and gives the same bits.
+template <typename E> __device__ __forceinline__ int k_of(int s, int lg)
 {
-    constexpr int BK = Cfg<E>::BK, VEC = Cfg<E>::VEC;
-    constexpr int STRA = Cfg<E>::BM + Cfg<E>::PADA, STRB = Cfg<E>::BN + Cfg<E>::PADB;
+    if constexpr (sizeof(E) == 8)
+        return 2 * (4 * (s >> 1) + lg) + (s & 1);
+    else
+        return 4 * (lg + 4 * (s >> 2)) + (s & 3);
+}
+
+template <typename E, int WTM, int WTN>
+__device__ __forceinline__ void mma_real(const E *__restrict__ As, const E *__restrict__ Bs,
+                                         typename Cfg<E>::acc_t (&acc)[WTM / 16][WTN / 16],
+                                         int wm, int wn, int lane)
+{
+    typedef Blk<E, WTM, WTN> T_;
+    constexpr int BK = Cfg<E>::BK, FM = T_::FM, FN = T_::FN, RVA = T_::RVA, RVB = T_::RVB;
     const int l16 = lane & 15, lg = lane >> 4;
 #pragma unroll
     for (int s = 0; s < BK / 4; ++s) {
-        const int k = s * 4 + lg;
-        E a[4], b[4];
+        const int k = k_of<E>(s, lg);
+        E a[FM], b[FN];
 #pragma unroll
-        for (int h = 0; h < 4 / VEC; ++h) {
-            Vec<E, VEC> va = *(const Vec<E, VEC> *)(As + k * STRA + wm * 64 + h * 16 * VEC + VEC * l16);
-            Vec<E, VEC> vb = *(const Vec<E, VEC> *)(Bs + k * STRB + wn * 64 + h * 16 * VEC + VEC * l16);
+        for (int h = 0; h < FM / RVA; ++h) {
+            Lv<E, RVA> va = *(const Lv<E, RVA> *)(As + k * T_::STRA + wm * WTM + h * 16 * RVA +
+                                                   RVA * l16);
 #pragma unroll
-            for (int e = 0; e < VEC; ++e) {
-                a[h * VEC + e] = va.e[e];
-                b[h * VEC + e] = vb.e[e];
-            }
+            for (int e = 0; e < RVA; ++e) a[h * RVA + e] = va.e[e];
         }
 #pragma unroll
-        for (int mb = 0; mb < 4; ++mb)
+        for (int h = 0; h < FN / RVB; ++h) {
+            Lv<E, RVB> vb = *(const Lv<E, RVB> *)(Bs + k * T_::STRB + wn * WTN + h * 16 * RVB +
+                                                   RVB * l16);
 #pragma unroll
-            for (int nb = 0; nb < 4; ++nb) acc[mb][nb] = mfma(a[mb], b[nb], acc[mb][nb]);
+            for (int e = 0; e < RVB; ++e) b[h * RVB + e] = vb.e[e];
+        }
+#pragma unroll
+        for (int mb = 0; mb < FM; ++mb)
+#pragma unroll
+            for (int nb = 0; nb < FN; ++nb) acc[mb][nb] = mfma(a[mb], b[nb], acc[mb][nb]);
     }
 }
 
@@ -262,9 +323,8 @@ __device__ __forceinline__ void mma_cplx(const cplx *__restrict__ As, const cplx
 }
 
 template <typename E> struct Smem {
-    static constexpr int A_ELEMS = Cfg<E>::BK * (Cfg<E>::BM + Cfg<E>::PADA);
-    static constexpr int B_ELEMS = Cfg<E>::BK * (Cfg<E>::BN + Cfg<E>::PADB);
-    static constexpr int ELEMS = 2 * (A_ELEMS + B_ELEMS);
+    static constexpr int A_ELEMS = Blk<E>::A_ELEMS, B_ELEMS = Blk<E>::B_ELEMS;
+    static constexpr int ELEMS = Blk<E>::ELEMS;
 };
 
 // segment i of the table, or the by-value segment when there is no table;
@@ -293,15 +353,17 @@ __device__ __forceinline__ bool tri_keep(int tri, int row, int col)
     return (!(tri & BX_TRI_LOWER) || row >= col) && (!(tri & BX_TRI_UPPER) || row <= col);
 }
 
-template <typename E, int MAJA, int MAJB, bool CJA, bool CJB>
+template <typename E, int MAJA, int MAJB, bool CJA, bool CJB, int WTM = 64, int WTN = 64>
 __device__ __forceinline__ void gemm_block(const bxh_problem_t &P, const bxh_seg_t *segs,
                                            const bxh_seg_t &seg0, int tm, int tn, E *smem)
 {
     typedef Cfg<E> C_;
-    constexpr int BM = C_::BM, BN = C_::BN, BK = C_::BK, VEC = C_::VEC;
+    typedef Blk<E, WTM, WTN> T_;
+    constexpr int BM = T_::BM, BN = T_::BN, BK = C_::BK, VEC = C_::VEC;
+    constexpr int FM = T_::FM, FN = T_::FN, RVA = T_::RVA, RVB = T_::RVB;
     constexpr bool CPLX = sizeof(E) == 16;
     E *As0 = smem;
-    E *Bs0 = smem + 2 * Smem<E>::A_ELEMS;
+    E *Bs0 = smem + 2 * T_::A_ELEMS;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -312,7 +374,7 @@ __device__ __forceinline__ void gemm_block(const bxh_problem_t &P, const bxh_seg
     Loader<E, VEC, BN, BK, C_::PADB, MAJB, CJB> lb;
 
     typedef typename C_::acc_t acc_t;
-    acc_t accr[4][4];
+    acc_t accr[FM][FN];
     d4_t accz[4][2][2];
     if constexpr (CPLX) {
 #pragma unroll
@@ -321,9 +383,9 @@ __device__ __forceinline__ void gemm_block(const bxh_problem_t &P, const bxh_seg
             for (int j = 0; j < 2; ++j) accz[i][j][0] = accz[i][j][1] = d4_t{0, 0, 0, 0};
     } else {
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < FM; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) accr[i][j] = acc_t{0, 0, 0, 0};
+            for (int j = 0; j < FN; ++j) accr[i][j] = acc_t{0, 0, 0, 0};
     }
 
     auto load_both = [&](int si, int64_t k0) {
@@ -355,16 +417,16 @@ __device__ __forceinline__ void gemm_block(const bxh_problem_t &P, const bxh_seg
         }
         const bool nhave = nsi < nseg;
         if (nhave) load_both(nsi, nk0);
-        const E *As = As0 + buf * Smem<E>::A_ELEMS;
-        const E *Bs = Bs0 + buf * Smem<E>::B_ELEMS;
+        const E *As = As0 + buf * T_::A_ELEMS;
+        const E *Bs = Bs0 + buf * T_::B_ELEMS;
         if constexpr (CPLX)
             mma_cplx(reinterpret_cast<const cplx *>(As), reinterpret_cast<const cplx *>(Bs), accz,
                      wm, wn, lane);
         else
-            mma_real<E>(As, Bs, accr, wm, wn, lane);
+            mma_real<E, WTM, WTN>(As, Bs, accr, wm, wn, lane);
         if (nhave) {
-            la.store(As0 + (buf ^ 1) * Smem<E>::A_ELEMS);
-            lb.store(Bs0 + (buf ^ 1) * Smem<E>::B_ELEMS);
+            la.store(As0 + (buf ^ 1) * T_::A_ELEMS);
+            lb.store(Bs0 + (buf ^ 1) * T_::B_ELEMS);
         }
         __syncthreads();
         si = nsi;
@@ -391,49 +453,49 @@ __device__ __forceinline__ void gemm_block(const bxh_problem_t &P, const bxh_seg
                 for (int v = 0; v < 4; ++v) {
                     const int row = m0 + wm * 64 + mb * 16 + Cfg<cplx>::out_row(lg, v);
                     if (row >= P.m || !tri_keep(tri, row + toff, col)) continue;
-                    const double sr = accz[mb][nb][0][v], sim = accz[mb][nb][1][v];
-                    double orr = ar * sr - ai * sim, oi = ar * sim + ai * sr;
                     cplx *c = Cp + row + (int64_t)col * ldc;
-                    if (!P.beta_zero) {
-                        const cplx c0 = gload(c);
-                        orr += br * c0.re - bi * c0.im;
-                        oi += br * c0.im + bi * c0.re;
-                    }
-                    if ((tri & BX_TRI_HERM) && row + toff == col) oi = 0.0;
-                    gstore(c, cplx{orr, oi});
+                    const cplx c0 = P.beta_zero ? cplx{0.0, 0.0} : gload(c);
+                    cplx o = out_cplx(ar, ai, accz[mb][nb][0][v], accz[mb][nb][1][v], br, bi, c0,
+                                      P.beta_zero);
+                    if ((tri & BX_TRI_HERM) && row + toff == col) o.im = 0.0;
+                    gstore(c, o);
                 }
             }
     } else {
         const E alpha = (E)P.alpha_re, beta = (E)P.beta_re;
         const bool cvec = aligned16(Cp, ldc, sizeof(E));
+        (void)VEC;
 #pragma unroll
-        for (int h = 0; h < 4 / VEC; ++h)
+        for (int h = 0; h < FM / RVA; ++h)
 #pragma unroll
-            for (int nb = 0; nb < 4; ++nb) {
-                const int col = n0 + wn * 64 + (nb / VEC) * 16 * VEC + VEC * l16 + nb % VEC;
+            for (int nb = 0; nb < FN; ++nb) {
+                const int col = n0 + wn * WTN + (nb / RVB) * 16 * RVB + RVB * l16 + nb % RVB;
                 if (col >= P.n) continue;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const int rbase = m0 + wm * 64 + h * 16 * VEC + VEC * C_::out_row(lg, v);
+                    const int rbase = m0 + wm * WTM + h * 16 * RVA + RVA * C_::out_row(lg, v);
                     E *c = Cp + rbase + (int64_t)col * ldc;
-                    if (cvec && rbase + VEC <= P.m && tri == 0) {
-                        Vec<E, VEC> out;
+                    if (cvec && rbase + RVA <= P.m && tri == 0) {
+                        Lv<E, RVA> out;
                         if (P.beta_zero) {
 #pragma unroll
-                            for (int e = 0; e < VEC; ++e) out.e[e] = alpha * accr[h * VEC + e][nb][v];
+                            for (int e = 0; e < RVA; ++e)
+                                out.e[e] = out_real(alpha, accr[h * RVA + e][nb][v], beta, E(0), true);
                         } else {
-                            const Vec<E, VEC> old = gload((const Vec<E, VEC> *)c);
+                            const Lv<E, RVA> old = gload((const Lv<E, RVA> *)c);
 #pragma unroll
-                            for (int e = 0; e < VEC; ++e)
-                                out.e[e] = alpha * accr[h * VEC + e][nb][v] + beta * old.e[e];
+                            for (int e = 0; e < RVA; ++e)
+                                out.e[e] = out_real(alpha, accr[h * RVA + e][nb][v], beta, old.e[e],
+                                                    false);
                         }
-                        gstore((Vec<E, VEC> *)c, out);
+                        gstore((Lv<E, RVA> *)c, out);
                     } else {
 #pragma unroll
-                        for (int e = 0; e < VEC; ++e) {
+                        for (int e = 0; e < RVA; ++e) {
                             if (rbase + e >= P.m || !tri_keep(tri, rbase + e + toff, col)) continue;
-                            const E s = alpha * accr[h * VEC + e][nb][v];
-                            gstore(c + e, P.beta_zero ? s : s + beta * gload(c + e));
+                            const E old = P.beta_zero ? E(0) : gload(c + e);
+                            gstore(c + e, out_real(alpha, accr[h * RVA + e][nb][v], beta, old,
+                                                   (bool)P.beta_zero));
                         }
                     }
                 }
@@ -514,21 +576,56 @@ __device__ __forceinline__ void wait_dma()
 }
 
 // this wave's four DMA instructions for one operand's K tile k0 (rows r0..)
-template <int MAJ>
+// EDGE: the operand has rlim rows; a lane whose rows lie beyond reads the
+// last ones instead (rows r0 + i >= rlim of the stage then hold copies that
+// feed only output rows or columns the epilogue does not write).  A MAJ_MN
+// lane reads rows 2j, 2j + 1: clamped to the even j with 2j <= rlim - 1, its
+// 16 bytes stay inside the 16-byte granule of element rlim - 1 (even ld,
+// aligned base), so inside the allocation.
+template <int MAJ, bool EDGE = false>
 __device__ __forceinline__ void dma_tile(const double *X, int64_t ld, int r0, int64_t k0,
-                                         double *stage, int wave, int lane)
+                                         double *stage, int wave, int lane, int rlim = 0)
 {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int q = wave * 4 + i; // 1 KB run of the stage
         if constexpr (MAJ == MAJ_MN) {
             // K row q: doubles 2 lane, 2 lane + 1 of rows r0 .. r0 + 127
-            dma16(X + (r0 + 2 * lane) + (k0 + q) * ld, stage + q * STR_MN);
+            int row = r0 + 2 * lane;
+            if constexpr (EDGE) row = min(row, (rlim - 1) & ~1);
+            dma16(X + row + (k0 + q) * ld, stage + q * STR_MN);
         } else {
             // rows 8q .. 8q + 7, lane -> row 8q + lane / 8, position lane % 8
             const int r = 8 * q + (lane >> 3);
             const int c = (lane & 7) ^ ((r >> 1) & 7);
-            dma16(X + (k0 + 2 * c) + (int64_t)(r0 + r) * ld, stage + q * 128);
+            int row = r0 + r;
+            if constexpr (EDGE) row = min(row, rlim - 1);
+            dma16(X + (k0 + 2 * c) + (int64_t)row * ld, stage + q * 128);
+        }
+    }
+}
+
+// The last kr < 16 columns of K (from k0) of one operand, staged by
+// ordinary bounds-checked loads in the layout dma_tile writes, columns kr..15
+// zero: a K tile the 16-byte DMA cannot take.  256 threads, 8 elements each,
+// consecutive threads on consecutive addresses.
+// EDGE: rows r0 + row >= rlim are staged as zero too.
+template <int MAJ, bool EDGE = false>
+__device__ __forceinline__ void stage_rem(const double *X, int64_t ld, int r0, int64_t k0, int kr,
+                                          double *stage, int tid, int rlim = 0)
+{
+    // two loads in flight per lane: the 128 accumulators stay live across this
+#pragma unroll 2
+    for (int i = 0; i < 8; ++i) {
+        const int e = tid + 256 * i;
+        if constexpr (MAJ == MAJ_MN) {
+            const int row = e & 127, k = e >> 7;
+            const bool in = k < kr && (!EDGE || r0 + row < rlim);
+            stage[k * STR_MN + row] = in ? gload(X + (r0 + row) + (k0 + k) * ld) : 0.0;
+        } else {
+            const int k = e & 15, row = e >> 4;
+            const bool in = k < kr && (!EDGE || r0 + row < rlim);
+            stage[pos_k(row, k)] = in ? gload(X + (k0 + k) + (int64_t)(r0 + row) * ld) : 0.0;
         }
     }
 }
@@ -555,7 +652,13 @@ constexpr int SMEM_DOUBLES = 2 * 2 * (BK * STR_MN);
 // K tiles of consecutive segments run as one pipeline.
 // MASK: P may be a rank-k diagonal block (P.tri), whose epilogue writes only
 // the stored triangle; the plain kernels instantiate it without.
-template <int MAJA, int MAJB, bool MASK = false, bool NESTED = true>
+// KTAIL: segments may have any K; each one's last K % 16 columns take one
+// more step staged by stage_rem (the plain kernels for ragged K).
+// EDGE (with KTAIL): the tile may cross row P.m or column P.n: operand rows
+// beyond are clamped to the last ones (dma_tile) and the epilogue writes
+// only the elements inside C.
+template <int MAJA, int MAJB, bool MASK = false, bool NESTED = true, bool KTAIL = false,
+          bool EDGE = false>
 __device__ __forceinline__ void dma64_block(const bxh_problem_t &P, const bxh_seg_t *segs,
                                             const bxh_seg_t &seg0, int tm, int tn, double *smem)
 {
@@ -573,9 +676,86 @@ __device__ __forceinline__ void dma64_block(const bxh_problem_t &P, const bxh_se
 
     auto stageA = [&](int buf) { return smem + buf * SA; };
     auto stageB = [&](int buf) { return smem + 2 * SA + buf * SB; };
-    // NESTED picks the K loop: nested per segment, or one loop walking the
-    // segments (kept where it measured faster: see the kernels below)
-    if constexpr (NESTED) {
+    if constexpr (KTAIL) {
+        // Per segment: its full K tiles in the DMA pipeline below, then, if
+        // K % 16 != 0, one step on its last columns staged by stage_rem (the
+        // prefetch of the next segment's first tile moves to that step).  No
+        // DMA is in flight while the tail loads are issued or waited for.
+        auto issue = [&](const bxh_seg_t &sg, int64_t k, int b) {
+            dma_tile<MAJA, EDGE>((const double *)sg.A, sg.lda, m0, k, stageA(b), wave, lane, P.m);
+            dma_tile<MAJB, EDGE>((const double *)sg.B, sg.ldb, n0, k, stageB(b), wave, lane, P.n);
+        };
+        auto multiply = [&](int b) {
+            const double *As = stageA(b), *Bs = stageB(b);
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                double a[4], bb[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    a[i] = frag<MAJA>(As, wm * 64 + i * 16 + l16, s, lg);
+                    bb[i] = frag<MAJB>(Bs, wn * 64 + i * 16 + l16, s, lg);
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], bb[j], acc[i][j]);
+            }
+        };
+        auto find = [&](int i, bxh_seg_t &out) {
+            for (; i < P.nseg; ++i) {
+                out = seg_at(segs, seg0, i);
+                if (out.k > 0) break;
+            }
+            return i;
+        };
+        bxh_seg_t cur = seg0;
+        int si = find(0, cur);
+        // stage 0 holds the current segment's first full tile when it has one
+        if (si < P.nseg && cur.k >= BK) {
+            issue(cur, 0, 0);
+            wait_dma();
+            __builtin_amdgcn_s_barrier();
+        }
+        int buf = 0;
+        while (si < P.nseg) {
+            const int64_t kfull = cur.k & ~(int64_t)(BK - 1);
+            const int kr = (int)(cur.k - kfull);
+            bxh_seg_t nxt = cur;
+            const int sn = find(si + 1, nxt);
+            const bool pre = sn < P.nseg && nxt.k >= BK;
+            if (kfull > 0) {
+                for (int64_t k0 = 0; k0 + BK < kfull; k0 += BK) {
+                    issue(cur, k0 + BK, buf ^ 1);
+                    multiply(buf);
+                    wait_dma();
+                    __builtin_amdgcn_s_barrier();
+                    buf ^= 1;
+                }
+                if (!kr && pre) issue(nxt, 0, buf ^ 1);
+                multiply(buf);
+                wait_dma();
+                __builtin_amdgcn_s_barrier();
+                buf ^= 1;
+            }
+            if (kr) {
+                // stage `buf` was last read two steps ago, before the last barrier
+                stage_rem<MAJA, EDGE>((const double *)cur.A, cur.lda, m0, kfull, kr, stageA(buf),
+                                      tid, P.m);
+                stage_rem<MAJB, EDGE>((const double *)cur.B, cur.ldb, n0, kfull, kr, stageB(buf),
+                                      tid, P.n);
+                __syncthreads();
+                if (pre) issue(nxt, 0, buf ^ 1);
+                multiply(buf);
+                wait_dma();
+                __builtin_amdgcn_s_barrier();
+                buf ^= 1;
+            }
+            si = sn;
+            cur = nxt;
+        }
+    } else if constexpr (NESTED) {
+        // NESTED picks the K loop: nested per segment, or one loop walking the
+        // segments (kept where it measured faster: see the kernels below)
         // The segments run as one pipeline of K tiles: stage `buf` holds the
         // tile being multiplied while the next one is DMA'd into the other.  An
         // inner loop takes every tile of one segment but its last, with that
@@ -709,9 +889,11 @@ __device__ __forceinline__ void dma64_block(const bxh_problem_t &P, const bxh_se
                 const int row = m0 + wm * 64 + i * 16 + Cfg<double>::out_row(lg, v);
                 if constexpr (MASK)
                     if (!tri_keep(P.tri, row + P.tri_off, col)) continue;
+                if constexpr (EDGE)
+                    if (row >= P.m || col >= P.n) continue;
                 double *c = Cp + row + (int64_t)col * P.ldc;
-                const double x = alpha * acc[i][j][v];
-                gstore(c, P.beta_zero ? x : x + beta * gload(c));
+                const double old = P.beta_zero ? 0.0 : gload(c);
+                gstore(c, out_real(alpha, acc[i][j][v], beta, old, (bool)P.beta_zero));
             }
         }
 }
@@ -728,6 +910,23 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm_dma(bxh_problem_t P, bxh_seg_t 
     // the nested K loop for NN only (1% faster at 32768^3); NT and TN ran 7-11%
     // slower with it and TT no faster, so they keep the segment-walking loop
     dma64_block<MAJA, MAJB, false, MAJA == MAJ_MN && MAJB == MAJ_K>(P, nullptr, S, tm, tn, smem);
+}
+
+// A ragged DGEMM: any M, N and K, aligned operands (edge tiles clamp their
+// operand rows and mask their stores, K tails take one register-staged step).
+template <int MAJA, int MAJB>
+__global__ __launch_bounds__(NT, 2) void k_dgemm_dma_kt(bxh_problem_t P, bxh_seg_t S)
+{
+    __shared__ __attribute__((aligned(16))) double smem[dma64::SMEM_DOUBLES];
+    // blocks go to the 8 XCDs round-robin (block b on XCD b % 8): give XCD x
+    // the contiguous tile range [x q + min(x, r), ...) of its q or q + 1
+    // blocks (nb = 8 q + r), so its blocks share A and B panels in its L2
+    // for any grid size
+    const int nb = gridDim.x, x = blockIdx.x & 7, q = nb >> 3, r = nb & 7;
+    const int b = x * q + (x < r ? x : r) + (blockIdx.x >> 3);
+    int tm, tn;
+    tile_coords(b, P.tiles_m, P.tiles_n, tm, tn);
+    dma64_block<MAJA, MAJB, false, true, true, true>(P, nullptr, S, tm, tn, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -889,6 +1088,28 @@ template <int BK> struct SegWalk {
 
 // every segment of P has K a multiple of bk (or is empty) and 16-byte
 // aligned operands, and P has no edge tiles (bm x bn blocks)
+// Whether block tile (tm, tn) of P can run on the DMA-staged block: a full
+// bm x bn tile (the DMA blocks write C without bounds checks) and every
+// segment 16-byte aligned with K a multiple of bk, or any K when the block
+// stages K tails itself (ktail: the FP64 block).  A problem with edge tiles
+// or a ragged K then runs its interior on the DMA block and only its edge
+// tiles on the register-staged block.
+__device__ __forceinline__ bool dma_tile_ok(const bxh_problem_t &P, const bxh_seg_t *segs,
+                                            int tm, int tn, int bm, int bn, int bk, int esize,
+                                            bool ktail)
+{
+    if ((tm + 1) * bm > P.m || (tn + 1) * bn > P.n || P.nseg < 1) return false;
+    const bxh_seg_t unused = {nullptr, nullptr, 0, 0, 0};
+    for (int i = 0; i < P.nseg; ++i) {
+        const bxh_seg_t S = seg_at(segs, unused, i);
+        if (S.k <= 0) continue;
+        if ((!ktail && S.k % bk) || !aligned16(S.A, S.lda, esize) ||
+            !aligned16(S.B, S.ldb, esize))
+            return false;
+    }
+    return true;
+}
+
 __device__ __forceinline__ bool dma_eligible(const bxh_problem_t &P, const bxh_seg_t *segs,
                                              const bxh_seg_t &seg0, int bm, int bn, int bk,
                                              int esize)
@@ -1028,8 +1249,8 @@ __device__ __forceinline__ void dmas_block(const bxh_problem_t &P, const bxh_seg
                 if constexpr (MASK)
                     if (!tri_keep(P.tri, row + P.tri_off, col)) continue;
                 float *c = Cp + row + (int64_t)col * P.ldc;
-                const float x = alpha * acc[i][j][v];
-                gstore(c, P.beta_zero ? x : x + beta * gload(c));
+                const float old = P.beta_zero ? 0.f : gload(c);
+                gstore(c, out_real(alpha, acc[i][j][v], beta, old, (bool)P.beta_zero));
             }
         }
 }
@@ -1185,17 +1406,13 @@ __device__ __forceinline__ void dmaz_block(const bxh_problem_t &P, const bxh_seg
                 const int row = m0 + wm * 64 + i * 16 + Cfg<cplx>::out_row(lg, v);
                 if constexpr (MASK)
                     if (!tri_keep(P.tri, row + P.tri_off, col)) continue;
-                const double sr = acc[i][j][0][v], sim = acc[i][j][1][v];
-                double orr = ar * sr - ai * sim, oi = ar * sim + ai * sr;
                 cplx *c = Cp + row + (int64_t)col * P.ldc;
-                if (!P.beta_zero) {
-                    const cplx c0 = gload(c);
-                    orr += br * c0.re - bi * c0.im;
-                    oi += br * c0.im + bi * c0.re;
-                }
+                const cplx c0 = P.beta_zero ? cplx{0.0, 0.0} : gload(c);
+                cplx o = out_cplx(ar, ai, acc[i][j][0][v], acc[i][j][1][v], br, bi, c0,
+                                  P.beta_zero);
                 if constexpr (MASK)
-                    if ((P.tri & BX_TRI_HERM) && row + P.tri_off == col) oi = 0.0;
-                gstore(c, cplx{orr, oi});
+                    if ((P.tri & BX_TRI_HERM) && row + P.tri_off == col) o.im = 0.0;
+                gstore(c, o);
             }
         }
 }
@@ -1209,6 +1426,17 @@ __device__ __forceinline__ void run_tile(const bxh_problem_t &P, const bxh_seg_t
     // segments come from the table; the by-value segment is never read
     // when a table is given (a problem may have zero segments)
     const bxh_seg_t unused = {nullptr, nullptr, 0, 0, 0};
+    if constexpr (sizeof(E) != 16) {
+        // the smaller block tiles a grouped launch may pick per problem
+        if (P.shape == BXH_SHAPE_128x64) {
+            gemm_block<E, MAJA, MAJB, CJA, CJB, 64, 32>(P, segs + P.seg_off, unused, tm, tn, smem);
+            return;
+        }
+        if (P.shape == BXH_SHAPE_64x64) {
+            gemm_block<E, MAJA, MAJB, CJA, CJB, 32, 32>(P, segs + P.seg_off, unused, tm, tn, smem);
+            return;
+        }
+    }
     gemm_block<E, MAJA, MAJB, CJA, CJB>(P, segs + P.seg_off, unused, tm, tn, smem);
 }
 
@@ -1226,7 +1454,7 @@ __device__ __forceinline__ void grouped_body(const bxh_problem_t *__restrict__ p
 // the other modes the register-staged block
 template <typename E, int MODE> constexpr int grouped_smem_elems()
 {
-    if (MODE != 1) return Smem<E>::ELEMS;
+    if (MODE == 2) return Smem<E>::ELEMS;
     if (sizeof(E) == 8) return dma64::SMEM_DOUBLES;
     if (sizeof(E) == 4) return 2 * 2 * dmas::STAGE;
     return 2 * (dmaz::BM + dmaz::BN) * dmaz::BK;
@@ -1261,7 +1489,7 @@ __device__ __forceinline__ void grouped_body(const bxh_problem_t *__restrict__ p
                                              E *smem)
 {
     constexpr bool CPLX = sizeof(E) == 16;
-    static_assert(MODE == 1 || MODE == 2, "grouped mode");
+    static_assert(MODE == 1 || MODE == 2 || (MODE == 3 && sizeof(E) == 8), "grouped mode");
     const bxh_seg_t unused = {nullptr, nullptr, 0, 0, 0};
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
         int lo = 0, hi = nprob - 1;
@@ -1275,20 +1503,52 @@ __device__ __forceinline__ void grouped_body(const bxh_problem_t *__restrict__ p
         const bxh_problem_t P = probs[lo];
         const int local = t - P.tile_start;
         const bxh_seg_t *ps = segs + P.seg_off;
-        const bool dma = sizeof(E) == 8    ? dma_eligible(P, ps, unused, 128, 128, 16, 8)
-                         : sizeof(E) == 4 ? dma_eligible(P, ps, unused, 128, 128, 32, 4)
-                                          : dma_eligible(P, ps, unused, 128, 64, 8, 16);
-        if (dma != (MODE == 1)) continue; // the other launch's tile
+        int ttm, ttn;
+        tile_coords(local, P.tiles_m, P.tiles_n, ttm, ttn);
+        // problems on the smaller block tiles run on the register-staged block
+        const bool big = sizeof(E) == 16 || P.shape == BXH_SHAPE_128x128;
+        const bool dma = !big                ? false
+                         : sizeof(E) == 8    ? dma_tile_ok(P, ps, ttm, ttn, 128, 128, 16, 8, false)
+                         : sizeof(E) == 4 ? dma_tile_ok(P, ps, ttm, ttn, 128, 128, 32, 4, false)
+                                          : dma_tile_ok(P, ps, ttm, ttn, 128, 64, 8, 16, false);
+        // MODE 3: FP64 tiles the DMA block takes once it stages their K tails
+        // (the tile runtime's last K tile, a ragged K; one segment needs K >=
+        // 16, as the plain path's ragged split does); a third launch, without
+        // the triangle mask: inlined beside MODE 1's blocks, or with the
+        // mask, its MFMA loops spilled registers
+        const bool tail = sizeof(E) == 8 && big && !dma && !P.tri &&
+                          (P.nseg > 1 || (P.nseg == 1 && seg_at(ps, unused, 0).k >= 16)) &&
+                          dma_tile_ok(P, ps, ttm, ttn, 128, 128, 16, 8, true);
+        if (MODE == 1 && !dma) continue; // another launch's tile
+        if (MODE == 2 && (dma || tail)) continue;
+        if (MODE == 3 && !tail) continue;
         if (P.tri) {
             // a block tile of a rank-k diagonal block wholly off the stored
             // triangle writes nothing: skip it before any load
-            int tm, tn;
-            tile_coords(local, P.tiles_m, P.tiles_n, tm, tn);
-            const int r0 = tm * Cfg<E>::BM + P.tri_off, c0 = tn * Cfg<E>::BN;
-            if ((P.tri & BX_TRI_LOWER) && r0 + Cfg<E>::BM - 1 < c0) continue;
-            if ((P.tri & BX_TRI_UPPER) && r0 > c0 + Cfg<E>::BN - 1) continue;
+            int bm = Blk<E>::BM, bn = Blk<E>::BN;
+            if constexpr (sizeof(E) != 16) {
+                if (P.shape == BXH_SHAPE_128x64) {
+                    bm = Blk<E, 64, 32>::BM;
+                    bn = Blk<E, 64, 32>::BN;
+                } else if (P.shape == BXH_SHAPE_64x64) {
+                    bm = Blk<E, 32, 32>::BM;
+                    bn = Blk<E, 32, 32>::BN;
+                }
+            }
+            const int r0 = ttm * bm + P.tri_off, c0 = ttn * bn;
+            if ((P.tri & BX_TRI_LOWER) && r0 + bm - 1 < c0) continue;
+            if ((P.tri & BX_TRI_UPPER) && r0 > c0 + bn - 1) continue;
         }
-        if constexpr (MODE == 1) {
+        if constexpr (MODE == 3) {
+            const int ta = P.opa != 0, tb = P.opb != 0;
+            double *sd = reinterpret_cast<double *>(smem);
+            switch (ta * 2 + tb) {
+            case 0: dma64_block<MAJ_MN, MAJ_K, false, true, true>(P, ps, unused, ttm, ttn, sd); break;
+            case 1: dma64_block<MAJ_MN, MAJ_MN, false, true, true>(P, ps, unused, ttm, ttn, sd); break;
+            case 2: dma64_block<MAJ_K, MAJ_K, false, true, true>(P, ps, unused, ttm, ttn, sd); break;
+            default: dma64_block<MAJ_K, MAJ_MN, false, true, true>(P, ps, unused, ttm, ttn, sd); break;
+            }
+        } else if constexpr (MODE == 1) {
             // block tiles of the DMA-staged blocks equal the register-staged ones
             int tm, tn;
             tile_coords(local, P.tiles_m, P.tiles_n, tm, tn);
@@ -1473,6 +1733,116 @@ bool dgemm_dma_ok(const bxh_problem_t &P, const bxh_seg_t &S)
            aligned16(S.A, S.lda, 8) && aligned16(S.B, S.ldb, 8);
 }
 
+// ---- ragged DGEMM on the DMA-staged kernel ----
+// A problem with edge tiles (M or N not a multiple of 128), any K, or
+// operands whose base or leading dimension is not 16-byte aligned runs as one
+// launch of k_dgemm_dma_kt over all its tiles (edge tiles clamp their operand
+// rows and mask their stores).  A misaligned operand is first copied into an
+// aligned workspace (one HBM pass, leading dimension rounded up to 8 doubles)
+// when the product is large enough to pay for it.
+
+__global__ void k_repack(const double *__restrict__ src, int64_t lds, double *__restrict__ dst,
+                         int64_t ldd, int64_t rows, int64_t cols)
+{
+    for (int64_t c = blockIdx.y; c < cols; c += gridDim.y)
+        for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows;
+             r += (int64_t)gridDim.x * blockDim.x)
+            dst[r + c * ldd] = src[r + c * lds];
+}
+
+void launch_repack(const double *src, int64_t lds, double *dst, int64_t ldd, int64_t rows,
+                   int64_t cols, hipStream_t st)
+{
+    const int gx = (int)((rows + 255) / 256 < 16 ? (rows + 255) / 256 : 16);
+    const int gy = (int)(cols < 8192 ? cols : 8192);
+    hipLaunchKernelGGL(k_repack, dim3(gx, gy), dim3(256), 0, st, src, lds, dst, ldd, rows, cols);
+}
+
+// A device workspace reused across calls: a call's copies wait for the
+// previous call's kernels (event on the device), growth waits on the host.
+struct Workspace {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+};
+Workspace g_rws;
+pthread_mutex_t g_rws_lock = PTHREAD_MUTEX_INITIALIZER;
+
+template <int MAJA, int MAJB>
+int launch_dgemm_kt(const bxh_problem_t &P, const bxh_seg_t &S, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_dgemm_dma_kt<MAJA, MAJB>), dim3(P.tiles_m * P.tiles_n), dim3(NT), 0, st,
+                       P, S);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : fail("dgemm dma (ragged) launch", e);
+}
+
+// rows x cols of the stored operand behind op(A) (m x k) / op(B) (k x n)
+inline void stored_dims(int op, int64_t r, int64_t c, int64_t &rows, int64_t &cols)
+{
+    rows = op ? c : r;
+    cols = op ? r : c;
+}
+
+// 1: ran; 0: not taken (the caller runs the plain kernel); <0: error
+int dgemm_ragged(int ta, int tb, const bxh_problem_t &P, const bxh_seg_t &S, hipStream_t st)
+{
+    if (P.m < 128 || P.n < 128 || S.k < 16 || P.tri) return 0;
+    const bool okA = aligned16(S.A, S.lda, 8), okB = aligned16(S.B, S.ldb, 8);
+    // a copy of A costs ~16 MK bytes against 2 MNK flops: worth it from N ~ 4096
+    if ((!okA && P.n < 4096) || (!okB && P.m < 4096)) return 0;
+    int64_t ra, ca, rb, cb;
+    stored_dims(ta, P.m, S.k, ra, ca);
+    stored_dims(tb, S.k, P.n, rb, cb);
+    const int64_t lda2 = (ra + 7) & ~(int64_t)7, ldb2 = (rb + 7) & ~(int64_t)7;
+    const size_t bytesA = okA ? 0 : (size_t)lda2 * ca * 8, bytesB = okB ? 0 : (size_t)ldb2 * cb * 8;
+    bxh_seg_t S2 = S;
+    const bool ws = bytesA + bytesB > 0;
+    if (ws) {
+        pthread_mutex_lock(&g_rws_lock);
+        hipError_t e = hipSuccess;
+        if (!g_rws.done) e = hipEventCreateWithFlags(&g_rws.done, hipEventDisableTiming);
+        if (e == hipSuccess && bytesA + bytesB > g_rws.cap) {
+            e = hipEventSynchronize(g_rws.done);
+            if (e == hipSuccess && g_rws.p) e = hipFree(g_rws.p);
+            g_rws.p = nullptr;
+            g_rws.cap = 0;
+            if (e == hipSuccess) e = hipMalloc(&g_rws.p, bytesA + bytesB);
+            if (e == hipSuccess) g_rws.cap = bytesA + bytesB;
+        }
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, g_rws.done, 0);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            pthread_mutex_unlock(&g_rws_lock);
+            return fail("ragged dgemm workspace", e);
+        }
+        double *wa = (double *)g_rws.p, *wb = (double *)((char *)g_rws.p + bytesA);
+        if (!okA) {
+            launch_repack((const double *)S.A, S.lda, wa, lda2, ra, ca, st);
+            S2.A = wa;
+            S2.lda = lda2;
+        }
+        if (!okB) {
+            launch_repack((const double *)S.B, S.ldb, wb, ldb2, rb, cb, st);
+            S2.B = wb;
+            S2.ldb = ldb2;
+        }
+    }
+    int rc;
+    switch (ta * 2 + tb) {
+    case 0: rc = launch_dgemm_kt<MAJ_MN, MAJ_K>(P, S2, st); break;
+    case 1: rc = launch_dgemm_kt<MAJ_MN, MAJ_MN>(P, S2, st); break;
+    case 2: rc = launch_dgemm_kt<MAJ_K, MAJ_K>(P, S2, st); break;
+    default: rc = launch_dgemm_kt<MAJ_K, MAJ_MN>(P, S2, st); break;
+    }
+    if (ws) {
+        hipError_t e = hipEventRecord(g_rws.done, st);
+        pthread_mutex_unlock(&g_rws_lock);
+        if (!rc && e != hipSuccess) rc = fail("ragged dgemm event", e);
+    }
+    return rc ? rc : 1;
+}
+
 template <int MAJA, int MAJB>
 int launch_sgemm_dma(const bxh_problem_t &P, const bxh_seg_t &S, hipStream_t st)
 {
@@ -1506,6 +1876,8 @@ template <typename E> int dispatch_real(int opa, int opb, const bxh_problem_t &P
             default: return launch_dgemm_dma<MAJ_K, MAJ_MN>(P, S, st);
             }
         }
+        const int r = dgemm_ragged(ta, tb, P, S, st);
+        if (r != 0) return r < 0 ? r : 0;
     }
     switch (ta * 2 + tb) {
     case 0: return launch_plain<E, MAJ_MN, MAJ_K, false, false>(P, S, st);
@@ -1559,6 +1931,17 @@ int dispatch_cplx(int opa, int opb, const bxh_problem_t &P, const bxh_seg_t &S, 
 extern "C" {
 
 const char *bxh_kernel_error(void) { return g_err; }
+
+void bxh_tile_dims(int dtype, int shape, int *bm, int *bn)
+{
+    if (dtype == BLASX_Z || shape == BXH_SHAPE_128x128 || shape < 0 || shape >= BXH_NSHAPES) {
+        *bm = bxh_tile_m(dtype);
+        *bn = bxh_tile_n(dtype);
+        return;
+    }
+    *bm = shape == BXH_SHAPE_64x64 ? Blk<double, 32, 32>::BM : Blk<double, 64, 32>::BM;
+    *bn = Blk<double, 64, 32>::BN; // the same for both types
+}
 
 int bxh_tile_m(int dtype)
 {
@@ -1620,6 +2003,8 @@ int bxh_gemm_grouped(int dtype, const bxh_problem_t *probs_dev, int nprob,
     case BLASX_D:
         hipLaunchKernelGGL((k_gemm_grouped<double, 1>), dim3(grid), dim3(NT), 0, st, probs_dev,
                            nprob, segs_dev, total_tiles);
+        hipLaunchKernelGGL((k_gemm_grouped<double, 3>), dim3(grid), dim3(NT), 0, st, probs_dev,
+                           nprob, segs_dev, total_tiles);
         hipLaunchKernelGGL((k_gemm_grouped<double, 2>), dim3(grid), dim3(NT), 0, st, probs_dev,
                            nprob, segs_dev, total_tiles);
         break;
@@ -1649,6 +2034,8 @@ int bxh_gemm_grouped_dyn(int dtype, const bxh_problem_t *probs_dev, const bxh_se
     switch (dtype) {
     case BLASX_D:
         hipLaunchKernelGGL((k_gemm_grouped_dyn<double, 1>), dim3(grid), dim3(NT), 0, st,
+                           probs_dev, segs_dev, counts_dev);
+        hipLaunchKernelGGL((k_gemm_grouped_dyn<double, 3>), dim3(grid), dim3(NT), 0, st,
                            probs_dev, segs_dev, counts_dev);
         hipLaunchKernelGGL((k_gemm_grouped_dyn<double, 2>), dim3(grid), dim3(NT), 0, st,
                            probs_dev, segs_dev, counts_dev);

@@ -43,6 +43,9 @@ typedef struct bxh_problem {
      * row + tri_off <= col (row, col relative to C); BX_TRI_HERM also zeroes
      * the imaginary part where row + tri_off == col.  0: every element. */
     int32_t tri, tri_off;
+    /* block tile shape (real types): 0 = 128 x 128, 1 = 128 x 64, 2 = 64 x 64
+     * (BXH_SHAPE_*); complex problems use 0 (128 x 64 complex) */
+    int32_t shape, pad0;
     double alpha_re, alpha_im;
     double beta_re, beta_im;
 } bxh_problem_t;
@@ -54,6 +57,12 @@ typedef struct bxh_problem {
 /* block-tile edges per element type (for computing tiles_m/tiles_n) */
 int bxh_tile_m(int dtype);
 int bxh_tile_n(int dtype);
+#define BXH_SHAPE_128x128 0
+#define BXH_SHAPE_128x64 1
+#define BXH_SHAPE_64x64 2
+#define BXH_NSHAPES 3
+/* block-tile edges of a grouped problem of this element type and shape */
+void bxh_tile_dims(int dtype, int shape, int *bm, int *bn);
 
 /* ---- runtime ---- */
 const char *bxh_last_error(void);

@@ -156,6 +156,9 @@ int bx_autotune_tile(int dt, int M, int N, int K);
 int bx_desc_check(const blasx_gemm_desc_t *d);
 int bx_grouped_fill(const blasx_gemm_desc_t *d, int n, int dtype, bxh_problem_t *probs,
                     bxh_seg_t *segs, int *nprob);
+/* block tile shape (BXH_SHAPE_*) of a grouped problem in a launch of `load`
+ * large tiles of its element type (autotune.c) */
+int bx_grouped_shape(int dt, int M, int N, int K, int64_t load);
 
 /* task queue (queue.c): ids task_num-1 .. 0 exactly once across threads */
 typedef struct bx_queue {

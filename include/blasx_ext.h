@@ -206,11 +206,28 @@ typedef struct blasx_gemm_desc {
 
 int blasx_gemm_grouped_device(const blasx_gemm_desc_t *descs, int n, void *stream);
 
+/* Block tile shape of a real problem in a grouped launch (or a server
+ * batch): 0 = 128 x 128, 1 = 128 x 64, 2 = 64 x 64; complex problems keep
+ * 128 x 64.  No counterpart in the reference (its tile sizes are compile-time
+ * constants, include/blasx.h:5-7).  The shape is picked per problem from a
+ * table of launch times measured on an MI355X (built in, written by
+ * python -m blasx_amd.tune --grouped; entries added at run time carry a
+ * winner only, come later and win ties): the workload nearest in M, N, K,
+ * then the fastest shape at the launch's load (its tile count at the large
+ * tile for that element type), interpolated between measured loads; with
+ * no entry, the large tile once the load reaches two tiles per CU.  Every
+ * shape gives bit-identical results.  blasx_set_grouped_shape forces one
+ * shape for S or D (-1: back to the table); blasx_grouped_shape reports the
+ * choice; blasx_grouped_shape_add adds a measured winner. */
+int blasx_set_grouped_shape(int dtype, int shape);
+int blasx_grouped_shape(int dtype, int M, int N, int K, int64_t load);
+int blasx_grouped_shape_add(int dtype, int M, int N, int K, int64_t load, int shape);
+
 /* Allocation-free variant of blasx_gemm_grouped_device.  No counterpart in
  * the reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  Same arguments, results and kernels; the problem tables live
- * in a static per-GPU table store (about 400 KiB, held until the process
+ * in a static per-GPU table store (about 430 KiB, held until the process
  * ends) instead of a buffer the library allocates, so at most
  * BLASX_NOALLOC_MAX_PROBLEMS descriptors per call (more: BLASX_ERANGE).  The
  * call allocates nothing, on the device or the host.  Calls share the

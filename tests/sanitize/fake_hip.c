@@ -105,6 +105,16 @@ int bxh_memcpy2d_d2h_async(void *d, size_t dp, const void *s, size_t sp, size_t 
 /* ---- kernels ---- */
 int bxh_tile_m(int dtype) { (void)dtype; return 128; }
 int bxh_tile_n(int dtype) { return dtype == BLASX_Z ? 64 : 128; }
+void bxh_tile_dims(int dtype, int shape, int *bm, int *bn)
+{
+    *bm = dtype != BLASX_Z && shape == BXH_SHAPE_64x64 ? 64 : 128;
+    *bn = dtype == BLASX_Z || shape != BXH_SHAPE_128x128 ? 64 : 128;
+}
+/* stand-in for the measured shape table (autotune.c): every shape occurs */
+int bx_grouped_shape(int dt, int M, int N, int K, int64_t load)
+{
+    return dt == BLASX_Z ? 0 : (int)(((int64_t)M + N + K + load) % BXH_NSHAPES);
+}
 
 typedef struct { double re, im; } zc;
 

@@ -88,6 +88,84 @@ def test_device_dgemm_shapes_integer_exact(gpu, orc, shape):
 
 
 @pytest.mark.parametrize("ta,tb", TRANS_REAL)
+def test_device_dgemm_ragged_dma_path(gpu, orc, ta, tb):
+    """Ragged DGEMMs on the DMA-staged kernel, one launch over every tile:
+    edge tiles clamp their operand rows to the last ones and mask their
+    stores, and the last K % 16 columns are staged by bounds-checked loads;
+    operands with an odd leading dimension or an 8-byte-aligned base are
+    first copied to an aligned workspace when N (for A) / M (for B) reaches
+    4096.  Bit-exact on
+    integer inputs, including K < 16 (no full K tile), exact multiples of 128
+    with a K tail, and two calls in a row sharing the workspace; beta = 0
+    ignores NaN in C."""
+    bx = gpu
+    rng = np.random.default_rng(37)
+    # (M, N, K, ld padding, base offset in elements)
+    cases = ((300, 270, 1007, "even", 0), (256, 384, 40, "even", 0), (131, 129, 9, "even", 0),
+             (4097, 4100, 61, "none", 0), (4100, 4097, 45, "even", 1), (640, 4096, 33, "none", 0))
+    for (M, N, K, padk, off) in cases:
+        say("device dgemm ragged", ta, tb, M, N, K, padk, off)
+        ra, ca = (M, K) if ta == "N" else (K, M)
+        rb, cb = (K, N) if tb == "N" else (N, K)
+        lda = ra + (ra % 2 if padk == "even" else 0)
+        ldb = rb + (rb % 2 if padk == "even" else 0)
+        Ai = np.asfortranarray(rng.integers(-256, 257, (lda, ca)), dtype=np.float64)
+        Bi = np.asfortranarray(rng.integers(-256, 257, (ldb, cb)), dtype=np.float64)
+        C0 = np.asfortranarray(rng.integers(-1000, 1001, (M, N)), dtype=np.float64)
+        ref = C0.copy(order="F")
+        orc.gemm(ta, tb, M, N, K, 3.0, Ai, lda, Bi, ldb, -2.0, ref, M)
+        # the operands at an element offset inside a larger buffer
+        dA = bx.DeviceArray.from_host(np.concatenate([np.zeros(off), Ai.ravel(order="F")])[:, None])
+        dB = bx.DeviceArray.from_host(np.concatenate([np.zeros(off), Bi.ravel(order="F")])[:, None])
+        dC = bx.DeviceArray.from_host(C0)
+        for rep in range(2):
+            if rep:
+                dC.upload(C0)
+            bx.device_gemm(ta, tb, M, N, K, 3.0, dA.ptr + 8 * off, lda, dB.ptr + 8 * off, ldb,
+                           -2.0, dC.ptr, M, np.float64)
+            bx.synchronize()
+            assert np.array_equal(dC.download(), ref), (ta, tb, M, N, K, rep)
+        for d in (dA, dB, dC):
+            d.free()
+    # uniform inputs, beta = 0 over NaN, a K tail and both strips
+    M, N, K = 4100, 4099, 1003
+    say("device dgemm ragged uniform", ta, tb, M, N, K)
+    ra, ca = (M, K) if ta == "N" else (K, M)
+    rb, cb = (K, N) if tb == "N" else (N, K)
+    A = _mat(orc, ra, ca, np.float64, 5, 0)
+    B = _mat(orc, rb, cb, np.float64, 5, 1)
+    ref = np.zeros((M, N), order="F")
+    orc.openblas_gemm(ta, tb, M, N, K, 0.75, A, ra, B, rb, 0.0, ref, M)
+    dA, dB = bx.DeviceArray.from_host(A), bx.DeviceArray.from_host(B)
+    dC = bx.DeviceArray.from_host(np.full((M, N), np.nan, order="F"))
+    bx.device_gemm(ta, tb, M, N, K, 0.75, dA.ptr, ra, dB.ptr, rb, 0.0, dC.ptr, M, np.float64)
+    bx.synchronize()
+    out = dC.download()
+    assert np.isfinite(out).all()
+    _check(orc, out, ref, 0.75, A, B, 0.0, np.zeros((M, N)), K, np.float64, f"{ta}{tb} ragged")
+    for d in (dA, dB, dC):
+        d.free()
+    # edge tiles store nothing outside C: rows M.. of a larger ldc keep their values
+    M, N, K, pad = 259, 131, 35, 5
+    say("device dgemm ragged ldc padding", ta, tb, M, N, K, pad)
+    ra, ca = (M, K) if ta == "N" else (K, M)
+    rb, cb = (K, N) if tb == "N" else (N, K)
+    Ai = np.asfortranarray(rng.integers(-256, 257, (ra + ra % 2, ca)), dtype=np.float64)
+    Bi = np.asfortranarray(rng.integers(-256, 257, (rb + rb % 2, cb)), dtype=np.float64)
+    Cbig = np.asfortranarray(np.full((M + pad, N), 7.0))
+    Cbig[:M] = rng.integers(-1000, 1001, (M, N))
+    ref = Cbig.copy(order="F")
+    orc.gemm(ta, tb, M, N, K, 3.0, Ai, Ai.shape[0], Bi, Bi.shape[0], -2.0, ref, M + pad)
+    dA, dB, dC = (bx.DeviceArray.from_host(x) for x in (Ai, Bi, Cbig))
+    bx.device_gemm(ta, tb, M, N, K, 3.0, dA.ptr, Ai.shape[0], dB.ptr, Bi.shape[0], -2.0, dC.ptr,
+                   M + pad, np.float64)
+    bx.synchronize()
+    assert np.array_equal(dC.download(), ref), (ta, tb, "ldc padding")
+    for d in (dA, dB, dC):
+        d.free()
+
+
+@pytest.mark.parametrize("ta,tb", TRANS_REAL)
 def test_device_dgemm_dma_path(gpu, orc, ta, tb):
     """Problems without edge tiles (M, N multiples of 128, K of 16, aligned
     operands) take the DMA-staged FP64 kernel: padded leading dimensions,
@@ -536,3 +614,59 @@ def test_abi_cgemm_exact_integers_beta_zero_row_major(gpu, orc):
     bx.cblas_gemm(bx.ROW_MAJOR, "N", "T", M, N, K, 2 - 1j, np.ascontiguousarray(A), K,
                   np.ascontiguousarray(B.conj()), K, 0.0, out2, N)
     assert np.array_equal(out2.T.astype(np.complex128), ref)
+
+
+@pytest.mark.parametrize("dt,T", [(np.float64, 1024), (np.float64, 2048), (np.float32, 2048),
+                                  (np.complex128, 1024)])
+def test_tile_runtime_pinned_operands_large_tiles(gpu, orc, dt, T):
+    """Pinned host operands (hipHostMalloc: DMA'd directly, no staging copy)
+    through the tiled runtime at the tile edges the 32768 host-resident
+    runs use, with ragged edge tiles in every dimension and a K tail: bit
+    exact on integer inputs against OpenBLAS (every partial sum is exact),
+    every transpose.  Complex at T = 1024 (the largest complex tile a
+    32 MiB cache slot holds).  Integer magnitudes keep every FP32 partial
+    sum below 2^24."""
+    bx = gpu
+    code = bx.dtype_code(dt)
+    assert bx.lib().blasx_set_tile_size(code, T) == 0
+    cplx = np.dtype(dt).kind == "c"
+    try:
+        rng = np.random.default_rng(T)
+        M, N, K = (2 * T + 333, T + 517, 2 * T + 91)
+        pins = []
+        for ta, tb in (TRANS_CPLX[:4] + [("C", "C")] if cplx else TRANS_REAL):
+            say("tile-rt pinned", np.dtype(dt).name, T, ta, tb, M, N, K)
+            ra, ca = (M, K) if ta == "N" else (K, M)
+            rb, cb = (K, N) if tb == "N" else (N, K)
+            pa, pb, pc = (bx.PinnedArray(ra, ca, dt), bx.PinnedArray(rb, cb, dt),
+                          bx.PinnedArray(M, N, dt))
+            pins += [pa, pb, pc]
+
+            lim = 8 if dt == np.float32 else 64
+
+            def ints(shape):
+                v = rng.integers(-lim, lim + 1, shape).astype(np.float64)
+                if cplx:
+                    v = v + 1j * rng.integers(-lim, lim + 1, shape)
+                return v
+
+            pa.array[:] = ints((ra, ca))
+            pb.array[:] = ints((rb, cb))
+            C0 = ints((M, N)).astype(dt)
+            pc.array[:] = C0
+            alpha, beta = (2 - 1j, 1 + 1j) if cplx else (3.0, -2.0)
+            ref = np.asfortranarray(C0.copy())
+            orc.openblas_gemm(ta, tb, M, N, K, alpha, np.asfortranarray(pa.array), ra,
+                              np.asfortranarray(pb.array), rb, beta, ref, M)
+            bx.reset_stats()
+            bx.cblas_gemm(bx.COL_MAJOR, ta, tb, M, N, K, alpha, pa.array, ra, pb.array, rb, beta,
+                          pc.array, M)
+            st = bx.stats()
+            assert st["calls_gpu"] == 1 and st["cpu_fallbacks"] == 0, st
+            assert st["bytes_h2d"] > 0
+            assert np.array_equal(pc.array, ref), (T, ta, tb)
+            for p in pins:
+                p.free()
+            pins = []
+    finally:
+        bx.lib().blasx_set_tile_size(code, 0)

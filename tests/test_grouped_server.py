@@ -90,6 +90,58 @@ def test_grouped_uneven_mixed(gpu, orc):
         assert np.array_equal(solo.download(), out)
 
 
+def _fixed(orc, dt, ta, tb, M, N, K, seed, alpha=1.25, beta=-0.5):
+    A = orc.fill_uniform(M if ta == "N" else K, K if ta == "N" else M, seed, 0, dtype=dt)
+    B = orc.fill_uniform(K if tb == "N" else N, N if tb == "N" else K, seed, 1, dtype=dt)
+    C0 = orc.fill_uniform(M, N, seed, 2, dtype=dt)
+    return dict(dt=dt, ta=ta, tb=tb, M=M, N=N, K=K, alpha=alpha, beta=beta, A=A, B=B, C0=C0)
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2])
+def test_grouped_block_shapes_equal_plain(gpu, orc, shape):
+    """Each block tile shape a grouped launch may pick for a real problem
+    (0: 128x128, 1: 128x64, 2: 64x64; forced here for S and D), on a mix of
+    ragged and aligned problems of every element type: results within the
+    oracle's tolerance, and bit-identical to the plain device GEMM of the same
+    problem, since every kernel and block shape sums a C element's products
+    in one K order."""
+    bx = gpu
+    L = bx.lib()
+    rng = np.random.default_rng(60 + shape)
+    probs = [_problem(orc, rng, i) for i in range(24)]
+    probs += [_fixed(orc, np.float64, "N", "N", 256, 384, 64, 90),
+              _fixed(orc, np.float64, "T", "N", 640, 130, 77, 91),
+              _fixed(orc, np.float32, "N", "T", 256, 256, 96, 92),
+              _fixed(orc, np.float32, "T", "T", 65, 200, 300, 93),
+              _fixed(orc, np.float64, "N", "T", 64, 64, 16, 94, beta=0.0)]
+    dev = [tuple(bx.DeviceArray.from_host(x) for x in (p["A"], p["B"], p["C0"])) for p in probs]
+    descs = [bx.make_desc(p["dt"], p["ta"], p["tb"], p["M"], p["N"], p["K"], p["alpha"],
+                          d[0].ptr, ld(p["A"]), d[1].ptr, ld(p["B"]), p["beta"],
+                          d[2].ptr, p["M"]) for p, d in zip(probs, dev)]
+    say("grouped launch, block shape", shape, len(descs), "problems")
+    try:
+        for dt in (bx.S, bx.D):
+            assert L.blasx_set_grouped_shape(dt, shape) == 0
+        bx.gemm_grouped_device(descs)
+        bx.synchronize()
+    finally:
+        for dt in (bx.S, bx.D):
+            L.blasx_set_grouped_shape(dt, -1)
+    for p, d in zip(probs, dev):
+        out = d[2].download()
+        _check(orc, out, p)
+        solo = bx.DeviceArray.from_host(p["C0"])
+        bx.device_gemm(p["ta"], p["tb"], p["M"], p["N"], p["K"], p["alpha"], d[0].ptr,
+                       ld(p["A"]), d[1].ptr, ld(p["B"]), p["beta"], solo.ptr, p["M"], p["dt"])
+        bx.synchronize()
+        assert np.array_equal(solo.download(), out), (shape, p["dt"], p["ta"], p["tb"],
+                                                      p["M"], p["N"], p["K"])
+        solo.free()
+    for d in dev:
+        for x in d:
+            x.free()
+
+
 def _run_server(bx, srv, probs, resident_B=None, one_at_a_time=False):
     outs, ids = [], []
     for p in probs:
