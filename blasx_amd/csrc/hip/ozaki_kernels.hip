@@ -410,16 +410,16 @@ __device__ __forceinline__ void lgkm_wait0()
     __builtin_amdgcn_s_waitcnt(0xF | (0x3 << 14) | (0x7 << 4));
 }
 
-// 16 bytes per lane from global memory to LDS (M0 + 16 lane).  M0 is a
-// reserved register the compiler does not allocate (a clobber of it is
-// ignored); nothing else in the kernels that call this uses M0 (checked in
-// their gfx950 assembly: every M0 write is this statement's).
-__device__ __forceinline__ void dma16(const int8_t *g, const int8_t *lds_wave_base)
+// 16 bytes per lane from global memory to LDS (M0 + 16 lane); `lds` is the
+// wave's destination, a wave-uniform LDS address held in a scalar register.
+// M0 is a reserved register the compiler does not allocate (a clobber of it
+// is ignored); nothing else in the kernels that call this uses M0 (checked
+// in their gfx950 assembly: every M0 write is this statement's).
+__device__ __forceinline__ void dma16_at(const int8_t *g, uint32_t lds)
 {
-    const uint32_t l = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) const int8_t *)lds_wave_base;
     asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off"
                  :
-                 : "s"(__builtin_amdgcn_readfirstlane(l)), "v"(g)
+                 : "s"(lds), "v"(g)
                  : "memory");
 }
 
@@ -438,22 +438,29 @@ __device__ __forceinline__ void tile256_seg(const int8_t *__restrict__ gA,
     // DMA: instruction i of this wave fills tile rows r = 16 (2 wave + i) +
     // lane / 4 at LDS chunk position lane % 4, which holds logical 16-byte
     // chunk c = position ^ ((r >> 1) & 3) (the lds_at swizzle)
-    int64_t goff[2];
+    // per DMA instruction i: this lane's global source at K tile 0 and the
+    // wave's LDS destination in stage 0 (scalar, M0 takes it as is)
+    const int8_t *srcA[2], *srcB[2];
+    uint32_t dstA[2], dstB[2];
+    const uint32_t lA = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) int8_t *)sA;
+    const uint32_t lB = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) int8_t *)sB;
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
         const int r = (wave * 2 + i) * 16 + (lane >> 2);
         const int c = (lane & 3) ^ ((r >> 1) & 3);
-        goff[i] = (int64_t)(r >> 7) * next_rows + (r & 127) * OZ_BK + c * 16;
+        const int64_t goff = (int64_t)(r >> 7) * next_rows + (r & 127) * OZ_BK + c * 16;
+        srcA[i] = gA + panel_off(m0, 0, kp) + goff;
+        srcB[i] = gB + panel_off(n0, 0, kp) + goff;
+        dstA[i] = __builtin_amdgcn_readfirstlane(lA + (wave * 2 + i) * 1024);
+        dstB[i] = __builtin_amdgcn_readfirstlane(lB + (wave * 2 + i) * 1024);
     }
-    const int8_t *pa = gA + panel_off(m0, 0, kp);
-    const int8_t *pb = gB + panel_off(n0, 0, kp);
     auto dma = [&](int f, int stage) {
         const int64_t off = (int64_t)f * OZ_CHUNK;
+        const uint32_t so = (uint32_t)stage * OZ_STAGE;
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
-            const int l = stage * OZ_STAGE + (wave * 2 + i) * 1024;
-            dma16(pa + off + goff[i], sA + l);
-            dma16(pb + off + goff[i], sB + l);
+            dma16_at(srcA[i] + off, dstA[i] + so);
+            dma16_at(srcB[i] + off, dstB[i] + so);
         }
     };
     const int a_row = wm * 128 + l16, b_row = wn * 64 + l16;
