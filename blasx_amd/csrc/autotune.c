@@ -165,7 +165,8 @@ BX_EXPORT int blasx_tune_table_add(int dtype, int M, int N, int K, int T)
  * the shape with the least time at the query's load (the launch's tile count
  * at the large tile, same element type), each shape's log time interpolated
  * linearly in log load between the two measured loads around it (the
- * nearest one outside them); with no entry, a heuristic on the load. */
+ * nearest one outside them).  A workload far from every measured one, or
+ * no table, goes to a heuristic on the load and the problem's shape. */
 #define MAX_GENTRIES 512
 typedef struct {
     int dt;
@@ -230,7 +231,9 @@ static int gtable_choice(int dt, int M, int N, int K, int64_t load)
             key = i;
         }
     }
-    if (key < 0) return -1;
+    /* a workload far from every measured one (any dimension off by more than
+     * about 4x, or several by less) is left to the heuristic */
+    if (key < 0 || bd > 1.5) return -1;
     const gentry_t *k = &g_gtab[key];
     /* that workload's measured loads around the query (later entries win
      * equal loads) */
@@ -260,9 +263,14 @@ int bx_grouped_shape(int dt, int M, int N, int K, int64_t load)
     if (shape < 0) shape = gtable_choice(dt, M, N, K, load);
     pthread_mutex_unlock(&g_lock);
     if (shape >= 0) return shape;
-    /* no measurement: the large tile once the launch fills every CU twice */
+    /* no nearby measurement: once the launch fills every CU twice, the large
+     * tile, or the narrow ones for a problem that would leave half of each
+     * large tile empty; below that, smaller tiles for more blocks */
     const int64_t slots = 2 * (int64_t)(g_bx.cus > 0 ? g_bx.cus : 256);
-    if (load >= slots) return BXH_SHAPE_128x128;
+    if (load >= slots) {
+        if (M <= 64) return BXH_SHAPE_64x64;
+        return N <= 64 ? BXH_SHAPE_128x64 : BXH_SHAPE_128x128;
+    }
     return 2 * load >= slots ? BXH_SHAPE_128x64 : BXH_SHAPE_64x64;
 }
 

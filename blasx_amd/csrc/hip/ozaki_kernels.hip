@@ -371,12 +371,12 @@ __device__ __forceinline__ void tile128(const OzSeg *__restrict__ segs, int nseg
 //    stage t % 4;
 //  * step t: start the DMA of tile t + 3 into stage (t + 3) % 4 (the stage of
 //    tile t - 1, whose reads all completed before the last barrier), read the
-//    last two A fragments of tile t, then run the 32 MFMAs of tile t while
-//    reading tile t + 1's fragments (DMA'd two steps ago, published by the
-//    last barrier): the B fragments into the second B register set after the
-//    first 4 MFMAs, A fragment mb < 6 after the MFMAs of row block mb; wait
-//    until tile t + 2's DMA has landed (tile t + 3's may still run) and end
-//    with one barrier.
+//    last two A fragments of tile t and issue the 4 MFMAs of row block 0;
+//    then wait until those reads have returned and tile t + 1's DMA has
+//    landed (tile t + 2's and t + 3's may still run) and meet at the step's
+//    one barrier while the MFMAs run; then the other 28 MFMAs of tile t while
+//    reading tile t + 1's fragments: the B fragments into the second B
+//    register set first, A fragment mb < 6 after the MFMAs of row block mb.
 //  * a DMA instruction writes 64 lanes x 16 bytes to LDS at M0 + 16 lane;
 //    wave w's instruction i fills bytes 1024 (2w + i) .. of a stage (rows
 //    16 (2w + i) ..), so each lane fetches the global 16 bytes that the LDS
@@ -493,21 +493,31 @@ __device__ __forceinline__ void tile256_seg(const int8_t *__restrict__ gA,
         dma(min(t + 3, last), wr); // tile t + 3
         fa[6] = rdA(cur, 6);
         fa[7] = rdA(cur, 7);
+        // row 0 first: its MFMAs keep the matrix core busy across the barrier
 #pragma unroll
-        for (int mb = 0; mb < 8; ++mb) {
+        for (int nb = 0; nb < 4; ++nb) acc[0][nb] = MAC::mac(fa[0], fb[P][nb], acc[0][nb]);
+        __builtin_amdgcn_sched_group_barrier(0x100, 2, 0); // last A fragments of tile t
+        __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        // this wave's reads of tile t are done and its DMAs of tile t + 1
+        // have landed (tile t + 2's and t + 3's may run): after the barrier
+        // every wave may read tile t + 1, and the next step's DMA may
+        // overwrite tile t's stage
+        lgkm_wait0();
+        vm_wait<8>();
+        __builtin_amdgcn_s_barrier();
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) fb[P ^ 1][nb] = rdB(nxt, nb);
+        fa[0] = rdA(nxt, 0);
+#pragma unroll
+        for (int mb = 1; mb < 8; ++mb) {
 #pragma unroll
             for (int nb = 0; nb < 4; ++nb)
                 acc[mb][nb] = MAC::mac(fa[mb], fb[P][nb], acc[mb][nb]);
-            if (mb == 0) {
-#pragma unroll
-                for (int nb = 0; nb < 4; ++nb) fb[P ^ 1][nb] = rdB(nxt, nb);
-            }
             if (mb < 6) fa[mb] = rdA(nxt, mb);
         }
         // pin that order (the scheduler would otherwise hoist the reads of
         // tile t + 1 above the MFMAs that still need the registers)
-        __builtin_amdgcn_sched_group_barrier(0x100, 2, 0); // last A fragments of tile t
-        __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
         __builtin_amdgcn_sched_group_barrier(0x100, 5, 0);
 #pragma unroll
         for (int mb = 1; mb < 6; ++mb) {
@@ -515,13 +525,7 @@ __device__ __forceinline__ void tile256_seg(const int8_t *__restrict__ gA,
             __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
         }
         __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);
-        __builtin_amdgcn_sched_barrier(0); // every MFMA of the step before the barrier
-        // the step's LDS reads (the last one 8 MFMAs ago) have returned: the
-        // next step then starts from a known LDS count, so its first MFMAs
-        // do not wait for the two reads issued right before them
-        lgkm_wait0();
-        vm_wait<4>(); // tile t + 2 has landed
-        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_sched_barrier(0);
         cur = nxt;
     };
     for (int t = 0; t < ktiles; t += 2) {

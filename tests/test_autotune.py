@@ -53,8 +53,8 @@ def test_grouped_shape_choice_cpu(bx):
     large tile; a forced shape wins; a run-time entry at the query point wins
     (later entries on ties); every built-in entry's fastest shape is returned
     at its own point, and between two measured loads of one workload the
-    shape whose log-interpolated time is least; with no table the load
-    decides (large tiles once the launch has two per CU)."""
+    shape whose log-interpolated time is least; far from every measured
+    workload (or with no table) the load and the problem's width decide."""
     L = bx.lib()
     assert L.blasx_grouped_shape(bx.Z, 100, 100, 100, 1) == 0
     assert L.blasx_set_grouped_shape(bx.Z, 1) != 0
@@ -94,6 +94,13 @@ def test_grouped_shape_choice_cpu(bx):
         assert L.blasx_grouped_shape(bx.S, 300, 300, 300, 2 * cus) == 0
         assert L.blasx_grouped_shape(bx.S, 300, 300, 300, cus) == 1
         assert L.blasx_grouped_shape(bx.S, 300, 300, 300, cus // 2 - 1) == 2
+    # workloads far from every measured one take the heuristic: a full
+    # launch keeps the large tile (TRMM's 256-row panels) unless a problem
+    # is 64 wide, a small one takes the small tiles
+    assert L.blasx_grouped_shape(bx.D, 256, 32768, 16384, 65536) == 0
+    assert L.blasx_grouped_shape(bx.D, 40000, 48, 1000, 65536) == 1
+    assert L.blasx_grouped_shape(bx.D, 48, 40000, 1000, 65536) == 2
+    assert L.blasx_grouped_shape(bx.D, 100, 20000, 300, 100) == 2
     # run-time entries (an unusual point, so no built-in entry is nearer)
     assert L.blasx_grouped_shape_add(bx.S, 77, 7777, 7, 3, 1) == 0
     assert L.blasx_grouped_shape(bx.S, 77, 7777, 7, 3) == 1
