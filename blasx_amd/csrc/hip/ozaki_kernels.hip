@@ -931,22 +931,37 @@ __device__ __forceinline__ i32x4 pack_resid16(const Digits<J> (&v)[16], const Mo
 }
 
 // contiguous (mode 1): x(i,k) = X[k + i*ld]; one thread per 16 consecutive
-// k of one row; all L residue planes (plane stride np*kp)
+// k of one row; all L residue planes (plane stride np*kp).  Threads walk the
+// planes' panel layout: the 4 chunks of a row's 64-byte K tile, then the
+// 128 rows of its row block, then K tiles, then row blocks, so a wave's
+// stores are 1 KB contiguous and 4 threads read a row's 512 bytes
 template <int J>
 __global__ __launch_bounds__(256) void k_crt_split_contig(const double *__restrict__ X, int64_t ld, int n, int len,
                                    const int *__restrict__ e, int beta, CrtMods md,
                                    int8_t *__restrict__ D, int np, int kp)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int chunks = kp / 16;
-    if (t >= (int64_t)np * chunks) return;
-    const int i = (int)(t / chunks), k0 = (int)(t % chunks) * 16;
+    const int ktiles = kp / OZ_BK;
+    const int64_t rest = t >> 9; // (row block, K tile)
+    const int rb = (int)(rest / ktiles), kt = (int)(rest % ktiles);
+    const int i = rb * OZ_PR + (int)((t >> 2) & (OZ_PR - 1)), k0 = kt * OZ_BK + (int)(t & 3) * 16;
+    if (i >= np) return;
     Digits<J> v[16];
     const int sh = i < n ? beta - e[i] : 0;
+    const double *x = X + k0 + (int64_t)i * ld;
+    if (i < n && k0 + 16 <= len && ((((uintptr_t)x) & 15) == 0)) {
+        // 16-byte loads
 #pragma unroll
-    for (int b = 0; b < 16; ++b)
-        v[b] = digits12<J>((i < n && k0 + b < len) ? trunc(ldexp(X[k0 + b + (int64_t)i * ld], sh))
-                                                   : 0.0);
+        for (int b = 0; b < 16; b += 2) {
+            const double2 w = *(const double2 *)(x + b);
+            v[b] = digits12<J>(trunc(ldexp(w.x, sh)));
+            v[b + 1] = digits12<J>(trunc(ldexp(w.y, sh)));
+        }
+    } else {
+#pragma unroll
+        for (int b = 0; b < 16; ++b)
+            v[b] = digits12<J>((i < n && k0 + b < len) ? trunc(ldexp(x[b], sh)) : 0.0);
+    }
     for (int l = 0; l < md.L; ++l)
         *(i32x4 *)(D + (int64_t)l * np * kp + panel_off(i, k0, kp)) =
             pack_resid16<J>(v, mod_const<J>(md, l));
@@ -1004,10 +1019,13 @@ __global__ __launch_bounds__(256) void k_zcrt_split_contig(const double *__restr
                                                            CrtMods md, int8_t *__restrict__ D,
                                                            int np, int kp)
 {
+    // threads in the planes' panel order, as k_crt_split_contig
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int chunks = kp / 16;
-    if (t >= (int64_t)np * chunks) return;
-    const int i = (int)(t / chunks), k0 = (int)(t % chunks) * 16;
+    const int ktiles = kp / OZ_BK;
+    const int64_t rest = t >> 9;
+    const int rb = (int)(rest / ktiles), kt = (int)(rest % ktiles);
+    const int i = rb * OZ_PR + (int)((t >> 2) & (OZ_PR - 1)), k0 = kt * OZ_BK + (int)(t & 3) * 16;
+    if (i >= np) return;
     Digits<J> vr[16], vi[16];
     const int sh = i < n ? beta - e[i] : 0;
 #pragma unroll
@@ -1410,7 +1428,8 @@ int bxh_crt_split(const double *X, int64_t ld, int n, int len, int contiguous, c
     // |A'| < 2^beta: the fewest 12-bit digits that cover beta bits
     const int J = beta <= 60 ? 5 : beta <= 72 ? 6 : 7;
     if (contiguous) {
-        const int64_t total = (int64_t)np * (kp / 16);
+        // whole 128-row blocks (the planes' panels), 4 threads per 64-byte K tile
+        const int64_t total = (int64_t)((np + OZ_PR - 1) / OZ_PR) * OZ_PR * (kp / 16);
         const dim3 grid((unsigned)((total + 255) / 256));
         auto go = [&](auto kern) {
             hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, X, ld, n, len, e, beta, md, D, np, kp);
@@ -1490,7 +1509,8 @@ int bxh_zcrt_split(const double *Z, int64_t ld, int n, int len, int contiguous, 
     // |A'| < 2^beta: the fewest 12-bit digits that cover beta bits
     const int J = beta <= 60 ? 5 : beta <= 72 ? 6 : 7;
     if (contiguous) {
-        const int64_t total = (int64_t)np * (kp / 16);
+        // whole 128-row blocks (the planes' panels), 4 threads per 64-byte K tile
+        const int64_t total = (int64_t)((np + OZ_PR - 1) / OZ_PR) * OZ_PR * (kp / 16);
         const dim3 grid((unsigned)((total + 255) / 256));
         auto go = [&](auto kern) {
             hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, Z, ld, n, len, conj, e, beta, md, D,

@@ -115,6 +115,13 @@ int bx_autotune_tile(int dt, int M, int N, int K)
     if (t > 0) return t;
     const int es = bx_esize(dt);
     int T = es >= 16 ? 1024 : 2048;
+    /* a problem inside one tile: one tile product (the per-tile costs of
+     * a pipeline would outweigh its overlap; measured at 1024^3 and 2048^3) */
+    const int big = M > N ? (M > K ? M : K) : (N > K ? N : K);
+    if (big <= T) {
+        while (T > 512 && big <= T / 2) T /= 2;
+        return T;
+    }
     /* enough C tiles to form pipelined rounds */
     while (T > 512 && (int64_t)((M + T - 1) / T) * ((N + T - 1) / T) < 16) T /= 2;
     return T;

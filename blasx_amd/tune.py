@@ -133,7 +133,7 @@ def main(argv=None):
             if rc:
                 print(f"tune: {name} {n}^3 failed ({rc})", file=sys.stderr, flush=True)
                 return 1
-            ts = ", ".join(f"{256 << i}: {times[i]:.3f}s" for i in range(4) if times[i] > 0)
+            ts = ", ".join(f"{256 << i}: {times[i] * 1e3:.3f} ms" for i in range(4) if times[i] > 0)
             print(f"tune: {name} {n}^3 -> T = {best.value} ({ts}; "
                   f"{time.perf_counter() - t0:.0f}s)", file=sys.stderr, flush=True)
             rows.append((dt, n, best.value, ts))
@@ -141,7 +141,7 @@ def main(argv=None):
         f.write("/* tune_table.inc -- measured tile edges of the host-operand runtime, one\n"
                 " * {dtype, M, N, K, T} entry per line (dtype: 0 S, 1 D, 2 Z), written by\n"
                 " * blasx_amd/tune.py from blasx_autotune_tile_size runs on an MI355X\n"
-                " * (seconds per call of each candidate edge in the comments). */\n")
+                " * (time per call of each candidate edge in the comments). */\n")
         for dt, n, T, ts in rows:
             f.write(f"{{{dt}, {n}, {n}, {n}, {T}}}, /* {ts} */\n")
     print(f"tune: wrote {args.out}", file=sys.stderr)

@@ -209,12 +209,18 @@ def test_host_generator_matches_oracle(bx, orc):
 
 def test_tile_size_heuristic(bx):
     """The heuristic behind the measured table (shapes far from the table's
-    cubes): 2048 (1024 complex), halved while the C grid has < 16 tiles."""
+    cubes): a problem inside one tile gets the smallest edge holding it;
+    otherwise 2048 (1024 complex), halved while the C grid has < 16 tiles."""
     L = bx.test_lib()
     assert L.bxtest_autotune_tile(bx.D, 100000, 50000, 3000) == 2048
     assert L.bxtest_autotune_tile(bx.Z, 100000, 50000, 3000) == 1024
     assert L.bxtest_autotune_tile(bx.D, 4000, 8000, 100000) == 1024  # 2 x 4 tiles, then 4 x 8
     assert L.bxtest_autotune_tile(bx.D, 300, 3000, 50000) == 512     # 1 x 2, 1 x 3, stops at 512
+    # inside one tile: the smallest edge that holds it
+    assert L.bxtest_autotune_tile(bx.D, 1500, 800, 1900) == 2048
+    assert L.bxtest_autotune_tile(bx.D, 600, 700, 900) == 1024
+    assert L.bxtest_autotune_tile(bx.D, 100, 200, 300) == 512
+    assert L.bxtest_autotune_tile(bx.Z, 1500, 800, 1000) == 1024
     assert bx.lib().blasx_set_tile_size(bx.D, 4096) != 0   # larger than a cache slot
     assert bx.lib().blasx_set_tile_size(bx.D, 300) != 0    # not a supported edge
 

@@ -127,7 +127,7 @@ def test_device_dgemm_ragged_dma_path(gpu, orc, ta, tb):
             assert np.array_equal(dC.download(), ref), (ta, tb, M, N, K, rep)
         for d in (dA, dB, dC):
             d.free()
-    # uniform inputs, beta = 0 over NaN, a K tail and both strips
+    # uniform inputs, beta = 0 over NaN, a K tail and edge tiles in both dimensions
     M, N, K = 4100, 4099, 1003
     say("device dgemm ragged uniform", ta, tb, M, N, K)
     ra, ca = (M, K) if ta == "N" else (K, M)
