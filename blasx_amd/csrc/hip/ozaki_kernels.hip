@@ -1147,6 +1147,8 @@ __global__ void k_crt_recon(int m, int n, const int8_t *__restrict__ R, int64_t 
         const int i0 = 4 * (int)(t % m4), j = (int)(t / m4);
         const int8_t *r = R + i0 + (int64_t)j * ldr;
         double q[4][4] = {};
+        // three planes per iteration: their loads are issued together
+#pragma unroll 3
         for (int l = 0; l < c.L; ++l) {
             const uint32_t w = gload((const uint32_t *)(r + l * plane));
 #pragma unroll

@@ -732,7 +732,18 @@ static int tile_call(int dt, int opa, int opb, int M, int N, int K, const double
         p.N = N;
         p.K = K;
         p.tri = tri;
-        int T = g_bx.tile_override[dt] ? g_bx.tile_override[dt] : bx_autotune_tile(dt, M, N, K);
+        /* DMA needs pinned host memory; anything else goes through staging */
+        const int pageable = bxh_pointer_is_pinned(A) == 0 || bxh_pointer_is_pinned(B) == 0 ||
+                             bxh_pointer_is_pinned(C) == 0;
+        int T = g_bx.tile_override[dt];
+        if (!T) {
+            /* the table's edges were measured on pinned operands; through the
+             * staging pool each tile costs a pack and a hand-off, and 1024
+             * or 2048 won at every size measured (pageable DGEMM 1024^3,
+             * 2048^3, 4096^3: T = 512 1.6-1.9x slower than 1024) */
+            T = bx_autotune_tile(dt, M, N, K);
+            if (pageable && T < 1024) T = 1024;
+        }
         while ((size_t)T * T * p.es > bx_cache_slot_bytes(c)) T /= 2;
         p.T = T;
         p.mt = (M + T - 1) / T;
