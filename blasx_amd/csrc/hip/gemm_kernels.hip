@@ -1978,6 +1978,8 @@ int bxh_gemm(int dtype, int opa, int opb, int m, int n, int k, const double *alp
     P.beta_zero = (P.beta_re == 0.0 && P.beta_im == 0.0);
     P.tri = 0;
     P.tri_off = 0;
+    P.shape = BXH_SHAPE_128x128;
+    P.pad0 = 0;
     bxh_seg_t S;
     S.A = A;
     S.B = B;

@@ -142,6 +142,47 @@ def test_grouped_block_shapes_equal_plain(gpu, orc, shape):
             x.free()
 
 
+def test_grouped_mixed_block_shapes_in_one_launch(gpu):
+    """The measured table picks different block tile shapes for different
+    problems of one launch (skinny 4096x64x4096 problems and 2048^3 cubes
+    of one element type share the launch's load): every problem's result is
+    bit-identical to the plain device GEMM of the same problem."""
+    bx = gpu
+    L = bx.lib()
+    specs = [(4096, 64, 4096)] * 12 + [(2048, 2048, 2048)] * 2 + [(256, 256, 4096)] * 4
+    load = sum(-(-M // 128) * -(-N // 128) for M, N, _ in specs)
+    shapes = {L.blasx_grouped_shape(bx.D, M, N, K, load) for M, N, K in specs}
+    say("grouped mixed shapes", len(specs), "problems, load", load, "shapes", sorted(shapes))
+    assert len(shapes) >= 2, shapes
+    arrs = []
+    for i, (M, N, K) in enumerate(specs):
+        dA, dB, dC = (bx.DeviceArray(r, c, np.float64) for r, c in ((M, K), (K, N), (M, N)))
+        for j, d in enumerate((dA, dB, dC)):
+            d.fill_uniform(70 + i, j)
+        arrs.append((dA, dB, dC))
+    bx.synchronize()
+    C0 = [d[2].download() for d in arrs]
+    # odd problems use B transposed: the K x N buffer read as N x K (ld = N)
+    ops = [("N", "T" if i % 2 else "N", N if i % 2 else K) for i, (M, N, K) in enumerate(specs)]
+    descs = [bx.make_desc(np.float64, ta, tb, M, N, K, 1.25, d[0].ptr, M, d[1].ptr, ldb, -0.5,
+                          d[2].ptr, M)
+             for (M, N, K), d, (ta, tb, ldb) in zip(specs, arrs, ops)]
+    bx.gemm_grouped_device(descs)
+    bx.synchronize()
+    for i, ((M, N, K), d, c0, (ta, tb, ldb)) in enumerate(zip(specs, arrs, C0, ops)):
+        out = d[2].download()
+        solo = bx.DeviceArray.from_host(c0)
+        bx.device_gemm(ta, tb, M, N, K, 1.25, d[0].ptr, M, d[1].ptr, ldb, -0.5, solo.ptr, M,
+                       np.float64)
+        bx.synchronize()
+        assert np.array_equal(solo.download(), out), (i, M, N, K)
+        assert np.isfinite(out).all()
+        solo.free()
+    for d in arrs:
+        for x in d:
+            x.free()
+
+
 def _run_server(bx, srv, probs, resident_B=None, one_at_a_time=False):
     outs, ids = [], []
     for p in probs:
