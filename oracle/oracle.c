@@ -21,6 +21,12 @@
  *     testing/gemm.c:10-24)
  *   - a double-double (Ogita-Rump-Oishi TwoSum/TwoProd) GEMM used as the
  *     high-accuracy oracle for the split-precision GEMM.
+ *   - the split-precision schemes themselves (no reference counterpart; from
+ *     their published methods): the 7-bit digit split (Ozaki, Ogita, Oishi,
+ *     Rump 2012), the CRT split and floating reconstruction (Ozaki scheme
+ *     II, 2025) and the BF16x3 piece split, each pinned by exact rational
+ *     or closed-form checks in tests/test_split.py, tests/test_crt.py and
+ *     tests/test_bf16x3.py.
  *
  * Compiled with -ffp-contract=off so that every a*b+c below rounds twice
  * unless fma() is written explicitly.
