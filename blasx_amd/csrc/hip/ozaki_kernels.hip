@@ -804,10 +804,9 @@ __global__ void k_zcombine(int m, int n, const double *__restrict__ pr, const do
 struct CrtMods {
     int L;
     int m[20];
-    double rinv[20];   // 1 / m_l, rounded (smod_i32)
-    float rinvf[20];   // 1 / m_l, rounded
-    float mf[20];      // m_l
-    float c12[7][20];  // 2^(12 j) smod m_l (odd m_l), j < 7
+    float rinvf[20];     // 1 / m_l, rounded
+    int8_t c16[20];      // 2^16 smod m_l
+    uint32_t c7[3][20];  // bytes 2^(7 j) smod m_l, j < 12 (byte j % 4 of word j / 4)
 };
 
 // r in [lo - m, hi + m] -> [lo, hi], lo = -floor(m/2), hi = ceil(m/2) - 1
@@ -826,108 +825,167 @@ __host__ __device__ __forceinline__ int smod_small(int x, int m, float rinvf)
     return smod_fix(x - q * m, m);
 }
 
-// any int32: the double quotient is within 1 of round(x / m), and x - q m
-// is exact in the FMA (no 32-bit integer multiply, a quarter-rate VALU op)
-__host__ __device__ __forceinline__ int smod_i32(int x, int m, double rinv)
+// The symmetric residue of s modulo m (odd m, or 256) in the low byte of the
+// result, for |s| < 2^22 + 2^17.  t = fl(s fl(1/m) + 1.5 2^23), one rounding
+// in the FMA, has 2^22 + q in its low 24 bits with q = round(s / m) exactly:
+// for odd m, s / m is at least 1/(2m) from every half-integer, and the
+// product's error |s| 2^-24 / m is below 0.26/m.  s - (2^22 + q) m = (s - q
+// m) - 2^22 m, and 2^22 m = 0 mod 256, so the low byte is that of the
+// symmetric residue s - q m.  m = 256: the low byte is s's own, whatever q.
+//
+// The product (2^22 + q) m is a full-rate 24-bit multiply of the low 24 bits
+// of t by m (m wave-uniform).  Written out in assembly: only the product's
+// low byte is used, so the compiler drops the operands' range and emits the
+// quarter-rate 32-bit multiply.
+__host__ __device__ __forceinline__ uint32_t mul_u24(uint32_t a, uint32_t m)
 {
-    const double q = rint((double)x * rinv);
-    return smod_fix((int)fma(-q, (double)m, (double)x), m);
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t r;
+    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "s"(m), "v"(a));
+    return r;
+#else
+    return (a & 0xFFFFFFu) * (m & 0xFFFFFFu);
+#endif
+}
+__host__ __device__ __forceinline__ uint32_t resid_byte(int s, int m, float rinvf)
+{
+    const float t = fmaf((float)s, rinvf, 0x1.8p23f);
+    return (uint32_t)s - mul_u24(__builtin_bit_cast(uint32_t, t), (uint32_t)m);
 }
 
-// An integer-valued double v as signed 12-bit digits, |v| = sum_j d_j
-// 2^(12 j), j < J (J = 5, 6, 7 cover |v| < 2^60, 2^72, 2^84; the split's
-// |A'| < 2^gamma), kept as floats f_j = sign(v) d_j, plus low = the
-// low 12 bits of v's two's complement.  Non-finite values and |v| >=
-// 2^(12 J) are taken as zero.  Every step is exact: top = trunc(|v| 2^-60)
-// and hi = trunc(rest 2^-30) are integers, and the FMA remainders are
-// integers below 2^60 and 2^30.
-template <int J> struct Digits {
-    float f[J];
-    int low;
-};
-template <int J> __host__ __device__ __forceinline__ Digits<J> digits12(double v)
+// any int32 x: x = 2^16 h + l (h = x >> 16, l = x & 0xFFFF) and s = h (2^16
+// smod m) + l = x (mod m) with |s| <= 2^15 127 + 2^16 - 1 < 2^22 + 2^17
+__host__ __device__ __forceinline__ uint32_t resid_byte_i32(int x, int m, int c16, float rinvf)
 {
-    static_assert(J >= 5 && J <= 7, "digit count");
-    Digits<J> r;
+    return resid_byte((x >> 16) * c16 + (x & 0xFFFF), m, rinvf);
+}
+
+// four low bytes -> one word (byte v from b_v): three byte permutes
+__host__ __device__ __forceinline__ uint32_t pack4(uint32_t b0, uint32_t b1, uint32_t b2,
+                                                   uint32_t b3)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t lo = __builtin_amdgcn_perm(b1, b0, 0x0c0c0400u); // b0.0 b1.0 0 0
+    const uint32_t hi = __builtin_amdgcn_perm(b3, b2, 0x0c0c0400u);
+    return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
+#else
+    return (b0 & 0xFF) | (b1 & 0xFF) << 8 | (b2 & 0xFF) << 16 | b3 << 24;
+#endif
+}
+
+// sum over the four byte lanes of a * b (signed bytes) plus c: V_DOT4_I32_I8
+__host__ __device__ __forceinline__ int dot4_i8(uint32_t a, uint32_t b, int c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sdot4((int)a, (int)b, c, false);
+#else
+    for (int v = 0; v < 4; ++v) c += (int8_t)(a >> (8 * v)) * (int8_t)(b >> (8 * v));
+    return c;
+#endif
+}
+
+// the same with c = 0, b wave-uniform.  The builtin with a zero accumulator
+// becomes a register move and the accumulating form; the three-operand form
+// takes the zero as an inline constant (register operations only).
+__host__ __device__ __forceinline__ int dot4_i8_first(uint32_t a, uint32_t b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    int r;
+    asm("v_dot4_i32_i8 %0, %1, %2, 0" : "=v"(r) : "v"(a), "s"(b));
+    return r;
+#else
+    return dot4_i8(a, b, 0);
+#endif
+}
+
+// An integer-valued double v, |v| < 2^(28 ND), as 4 ND signed 7-bit digits:
+// |v| = sum_j d_j 2^(7 j) with d_j in [0, 128), byte j % 4 of word j / 4
+// holding sign(v) d_j in two's complement.  Non-finite values and |v| >=
+// 2^(28 ND) are taken as zero.  Every step is exact: |v| is cut into 28-bit
+// integers (trunc of exact power-of-two multiples, FMA remainders below 2^56
+// and 2^28), each spread to four bytes of 7 bits; the bytewise negation
+// (0x80 - d) ^ 0x80 = -d (mod 256) never borrows across bytes (d < 0x80).
+template <int ND> struct Dig7 {
+    uint32_t w[ND];
+};
+__host__ __device__ __forceinline__ uint32_t spread7(uint32_t x) // x < 2^28
+{
+    x = (x & 0x3FFFu) | ((x << 2) & 0x3FFF0000u);        // 14-bit halves at bits 0, 16
+    return (x & 0x007F007Fu) | ((x << 1) & 0x7F007F00u); // 7-bit digits at bytes
+}
+template <int ND> __host__ __device__ __forceinline__ Dig7<ND> digits7(double v)
+{
+    static_assert(ND == 2 || ND == 3, "digit words");
     double a = fabs(v);
-    if (!(a < (J == 5 ? 0x1p60 : J == 6 ? 0x1p72 : 0x1p84))) a = 0.0;
-    const float sg = v < 0 ? -1.0f : 1.0f;
-    double top = 0.0;
-    if (J > 5) {
-        top = trunc(a * 0x1p-60);
-        a = fma(-top, 0x1p60, a);
+    if (!(a < (ND == 2 ? 0x1p56 : 0x1p84))) a = 0.0;
+    uint32_t c[3];
+    if (ND == 3) {
+        const double top = trunc(a * 0x1p-56);
+        a = fma(-top, 0x1p56, a);
+        c[2] = (uint32_t)top;
     }
-    const double hd = trunc(a * 0x1p-30);
-    const uint32_t h = (uint32_t)hd, l = (uint32_t)fma(-hd, 0x1p30, a);
-    uint32_t d[7];
-    d[0] = l & 0xFFF;
-    d[1] = (l >> 12) & 0xFFF;
-    d[2] = (l >> 24) | ((h & 0x3F) << 6);
-    d[3] = (h >> 6) & 0xFFF;
-    d[4] = h >> 18;
-    if (J > 5) {
-        const uint32_t t = (uint32_t)top;
-        d[5] = t & 0xFFF;
-        d[6] = t >> 12;
-    }
+    const double hd = trunc(a * 0x1p-28);
+    c[1] = (uint32_t)hd;
+    c[0] = (uint32_t)fma(-hd, 0x1p28, a);
+    Dig7<ND> r;
 #pragma unroll
-    for (int j = 0; j < J; ++j) r.f[j] = sg * (float)d[j];
-    r.low = v < 0 ? (int)(0u - d[0]) : (int)d[0];
+    for (int j = 0; j < ND; ++j) {
+        const uint32_t w = spread7(c[j]);
+        r.w[j] = v < 0 ? (0x80808080u - w) ^ 0x80808080u : w;
+    }
     return r;
 }
 
-// Symmetric residue of v modulo m from its digits.  Odd m: s = sum_j f_j
-// (2^(12 j) smod m) is computed exactly in float (|s| <= 7 * 4095 * 127 <
-// 2^22), and for |s| < 2^22 the float quotient q = rint(s * fl(1/m)) is
-// exactly round(s / m) (its error is below 1 / (2m), the least distance of
-// s / m from a half-integer), so s - q m, also exact, is the symmetric
-// residue, with the sign of v carried by the digits.  m = 256: the low
-// byte of v's two's complement.  The constants of one modulus are read
-// once (`ModConst`) and reused for every element.
-template <int J> struct ModConst {
-    float c[J], rinv, m;
-    bool byte; // m = 256
+// Symmetric residue of v modulo m (odd, or 256) from its digits, in the low
+// byte: s = sum_j digit_j (2^(7 j) smod m) = v (mod m) on the dot-product
+// unit, exact (|s| <= 12 * 127 * 128 < 2^18), then resid_byte.  The
+// constants of one modulus are read once (`ModConst`) and reused for every
+// element.
+struct ModConst {
+    uint32_t c[3];
+    float rinv;
+    int m;
 };
-template <int J> __host__ __device__ __forceinline__ ModConst<J> mod_const(const CrtMods &md, int l)
+__host__ __device__ __forceinline__ ModConst mod_const(const CrtMods &md, int l)
 {
-    ModConst<J> k;
+    ModConst k;
 #pragma unroll
-    for (int j = 0; j < J; ++j) k.c[j] = md.c12[j][l];
+    for (int j = 0; j < 3; ++j) k.c[j] = md.c7[j][l];
     k.rinv = md.rinvf[l];
-    k.m = md.mf[l];
-    k.byte = md.m[l] == 256;
+    k.m = md.m[l];
     return k;
 }
-template <int J>
-__host__ __device__ __forceinline__ int digit_resid_odd(const Digits<J> &x, const ModConst<J> &k)
+template <int ND>
+__host__ __device__ __forceinline__ int digit_sum(const Dig7<ND> &x, const ModConst &k)
 {
-    float s = 0.0f;
+    int s = dot4_i8_first(x.w[0], k.c[0]);
 #pragma unroll
-    for (int j = 0; j < J; ++j) s = fmaf(x.f[j], k.c[j], s);
-    const float q = rintf(s * k.rinv);
-    return (int)fmaf(-q, k.m, s);
+    for (int j = 1; j < ND; ++j) s = dot4_i8(x.w[j], k.c[j], s);
+    return s;
 }
-template <int J>
-__host__ __device__ __forceinline__ int digit_resid(const Digits<J> &x, const ModConst<J> &k)
+template <int ND>
+__host__ __device__ __forceinline__ int digit_sum_add(const Dig7<ND> &x, const ModConst &k, int s)
 {
-    return k.byte ? (int)(int8_t)(uint8_t)(x.low & 0xFF) : digit_resid_odd<J>(x, k);
+#pragma unroll
+    for (int j = 0; j < ND; ++j) s = dot4_i8(x.w[j], k.c[j], s);
+    return s;
+}
+template <int ND>
+__host__ __device__ __forceinline__ int digit_resid(const Dig7<ND> &x, const ModConst &k)
+{
+    return (int8_t)resid_byte(digit_sum<ND>(x, k), k.m, k.rinv);
 }
 
 // one plane's residues of 16 consecutive-k elements, packed for a 16-byte
-// store; the modulus test is taken once for all 16
-template <int J>
-__device__ __forceinline__ i32x4 pack_resid16(const Digits<J> (&v)[16], const ModConst<J> &k)
+// store
+template <int ND>
+__device__ __forceinline__ i32x4 pack_resid16(const Dig7<ND> (&v)[16], const ModConst &k)
 {
-    uint32_t w[4] = {0, 0, 0, 0};
-    if (k.byte) {
+    uint32_t b[16];
 #pragma unroll
-        for (int b = 0; b < 16; ++b) w[b >> 2] |= ((uint32_t)v[b].low & 0xFF) << (8 * (b & 3));
-    } else {
-#pragma unroll
-        for (int b = 0; b < 16; ++b)
-            w[b >> 2] |= ((uint32_t)(uint8_t)(int8_t)digit_resid_odd<J>(v[b], k)) << (8 * (b & 3));
-    }
-    return i32x4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+    for (int e = 0; e < 16; ++e) b[e] = resid_byte(digit_sum<ND>(v[e], k), k.m, k.rinv);
+    return i32x4{(int)pack4(b[0], b[1], b[2], b[3]), (int)pack4(b[4], b[5], b[6], b[7]),
+                 (int)pack4(b[8], b[9], b[10], b[11]), (int)pack4(b[12], b[13], b[14], b[15])};
 }
 
 // contiguous (mode 1): x(i,k) = X[k + i*ld]; one thread per 16 consecutive
@@ -935,7 +993,7 @@ __device__ __forceinline__ i32x4 pack_resid16(const Digits<J> (&v)[16], const Mo
 // planes' panel layout: the 4 chunks of a row's 64-byte K tile, then the
 // 128 rows of its row block, then K tiles, then row blocks, so a wave's
 // stores are 1 KB contiguous and 4 threads read a row's 512 bytes
-template <int J>
+template <int ND>
 __global__ __launch_bounds__(256) void k_crt_split_contig(const double *__restrict__ X, int64_t ld, int n, int len,
                                    const int *__restrict__ e, int beta, CrtMods md,
                                    int8_t *__restrict__ D, int np, int kp)
@@ -946,7 +1004,7 @@ __global__ __launch_bounds__(256) void k_crt_split_contig(const double *__restri
     const int rb = (int)(rest / ktiles), kt = (int)(rest % ktiles);
     const int i = rb * OZ_PR + (int)((t >> 2) & (OZ_PR - 1)), k0 = kt * OZ_BK + (int)(t & 3) * 16;
     if (i >= np) return;
-    Digits<J> v[16];
+    Dig7<ND> v[16];
     const int sh = i < n ? beta - e[i] : 0;
     const double *x = X + k0 + (int64_t)i * ld;
     if (i < n && k0 + 16 <= len && ((((uintptr_t)x) & 15) == 0)) {
@@ -954,23 +1012,23 @@ __global__ __launch_bounds__(256) void k_crt_split_contig(const double *__restri
 #pragma unroll
         for (int b = 0; b < 16; b += 2) {
             const double2 w = *(const double2 *)(x + b);
-            v[b] = digits12<J>(trunc(ldexp(w.x, sh)));
-            v[b + 1] = digits12<J>(trunc(ldexp(w.y, sh)));
+            v[b] = digits7<ND>(trunc(ldexp(w.x, sh)));
+            v[b + 1] = digits7<ND>(trunc(ldexp(w.y, sh)));
         }
     } else {
 #pragma unroll
         for (int b = 0; b < 16; ++b)
-            v[b] = digits12<J>((i < n && k0 + b < len) ? trunc(ldexp(x[b], sh)) : 0.0);
+            v[b] = digits7<ND>((i < n && k0 + b < len) ? trunc(ldexp(x[b], sh)) : 0.0);
     }
     for (int l = 0; l < md.L; ++l)
         *(i32x4 *)(D + (int64_t)l * np * kp + panel_off(i, k0, kp)) =
-            pack_resid16<J>(v, mod_const<J>(md, l));
+            pack_resid16<ND>(v, mod_const(md, l));
 }
 
 // strided (mode 0): x(i,k) = X[i + k*ld]; block = 64 rows x 64 k, thread =
 // one row x 16 consecutive k: the FP64 reads are coalesced along i across
 // the wave, and each plane gets one 16-byte store per thread
-template <int J>
+template <int ND>
 __global__ __launch_bounds__(256) void k_crt_split_strided(const double *__restrict__ X, int64_t ld,
                                                            int n, int len,
                                                            const int *__restrict__ e, int beta,
@@ -980,39 +1038,48 @@ __global__ __launch_bounds__(256) void k_crt_split_strided(const double *__restr
     const int i = blockIdx.x * 64 + (threadIdx.x & 63);
     const int k0 = blockIdx.y * 64 + (threadIdx.x >> 6) * 16;
     if (i >= np) return;
-    Digits<J> v[16];
+    Dig7<ND> v[16];
     const int sh = i < n ? beta - e[i] : 0;
+    const double *x = X + i + (int64_t)k0 * ld;
+    if (i < n && k0 + 16 <= len) {
+        // all 16 loads in flight together
+        double xs[16];
 #pragma unroll
-    for (int b = 0; b < 16; ++b)
-        v[b] = digits12<J>((i < n && k0 + b < len) ? trunc(ldexp(X[i + (int64_t)(k0 + b) * ld], sh))
-                                                   : 0.0);
+        for (int b = 0; b < 16; ++b) xs[b] = x[b * ld];
+#pragma unroll
+        for (int b = 0; b < 16; ++b) v[b] = digits7<ND>(trunc(ldexp(xs[b], sh)));
+    } else {
+#pragma unroll
+        for (int b = 0; b < 16; ++b)
+            v[b] = digits7<ND>((i < n && k0 + b < len) ? trunc(ldexp(x[b * ld], sh)) : 0.0);
+    }
     for (int l = 0; l < md.L; ++l)
         *(i32x4 *)(D + (int64_t)l * np * kp + panel_off(i, k0, kp)) =
-            pack_resid16<J>(v, mod_const<J>(md, l));
+            pack_resid16<ND>(v, mod_const(md, l));
 }
 
 // complex operands: planes 3l (real part), 3l+1 (imaginary part), 3l+2
 // (their sum) of modulus l, from one shared exponent per row; conj negates
 // the imaginary part.  Plane p: c = p % 3 of modulus p / 3.
-template <int J>
-__device__ __forceinline__ i32x4 zcrt_pack16(const Digits<J> (&vr)[16], const Digits<J> (&vi)[16],
+template <int ND>
+__device__ __forceinline__ i32x4 zcrt_pack16(const Dig7<ND> (&vr)[16], const Dig7<ND> (&vi)[16],
                                              int p, const CrtMods &md)
 {
     const int l = p / 3, c = p - 3 * l;
-    const ModConst<J> k = mod_const<J>(md, l);
-    if (c == 0) return pack_resid16<J>(vr, k);
-    if (c == 1) return pack_resid16<J>(vi, k);
-    uint32_t w[4] = {0, 0, 0, 0};
+    const ModConst k = mod_const(md, l);
+    if (c == 0) return pack_resid16<ND>(vr, k);
+    if (c == 1) return pack_resid16<ND>(vi, k);
+    // the sum's digit sums add exactly (|s| < 2^19)
+    uint32_t b[16];
 #pragma unroll
-    for (int b = 0; b < 16; ++b) {
-        const int r = smod_fix(digit_resid<J>(vr[b], k) + digit_resid<J>(vi[b], k), md.m[l]);
-        w[b >> 2] |= ((uint32_t)(uint8_t)(int8_t)r) << (8 * (b & 3));
-    }
-    return i32x4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+    for (int e = 0; e < 16; ++e)
+        b[e] = resid_byte(digit_sum_add<ND>(vi[e], k, digit_sum<ND>(vr[e], k)), k.m, k.rinv);
+    return i32x4{(int)pack4(b[0], b[1], b[2], b[3]), (int)pack4(b[4], b[5], b[6], b[7]),
+                 (int)pack4(b[8], b[9], b[10], b[11]), (int)pack4(b[12], b[13], b[14], b[15])};
 }
 
 // contiguous (mode 1): z(i,k) = Z[k + i*ld]; one thread per 16 k of one row
-template <int J>
+template <int ND>
 __global__ __launch_bounds__(256) void k_zcrt_split_contig(const double *__restrict__ Z,
                                                            int64_t ld, int n, int len, int conj,
                                                            const int *__restrict__ e, int beta,
@@ -1026,21 +1093,34 @@ __global__ __launch_bounds__(256) void k_zcrt_split_contig(const double *__restr
     const int rb = (int)(rest / ktiles), kt = (int)(rest % ktiles);
     const int i = rb * OZ_PR + (int)((t >> 2) & (OZ_PR - 1)), k0 = kt * OZ_BK + (int)(t & 3) * 16;
     if (i >= np) return;
-    Digits<J> vr[16], vi[16];
+    Dig7<ND> vr[16], vi[16];
     const int sh = i < n ? beta - e[i] : 0;
+    const double *z0 = Z + 2 * (k0 + (int64_t)i * ld);
+    if (i < n && k0 + 16 <= len && ((((uintptr_t)z0) & 15) == 0)) {
+        // 16-byte loads, all in flight together
+        double2 zs[16];
 #pragma unroll
-    for (int b = 0; b < 16; ++b) {
-        const bool in = i < n && k0 + b < len;
-        const double *z = Z + 2 * (k0 + b + (int64_t)i * ld);
-        vr[b] = digits12<J>(in ? trunc(ldexp(z[0], sh)) : 0.0);
-        vi[b] = digits12<J>(in ? trunc(ldexp(conj ? -z[1] : z[1], sh)) : 0.0);
+        for (int b = 0; b < 16; ++b) zs[b] = *(const double2 *)(z0 + 2 * b);
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+            vr[b] = digits7<ND>(trunc(ldexp(zs[b].x, sh)));
+            vi[b] = digits7<ND>(trunc(ldexp(conj ? -zs[b].y : zs[b].y, sh)));
+        }
+    } else {
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+            const bool in = i < n && k0 + b < len;
+            const double *z = z0 + 2 * b;
+            vr[b] = digits7<ND>(in ? trunc(ldexp(z[0], sh)) : 0.0);
+            vi[b] = digits7<ND>(in ? trunc(ldexp(conj ? -z[1] : z[1], sh)) : 0.0);
+        }
     }
     for (int p = 0; p < 3 * md.L; ++p)
-        *(i32x4 *)(D + (int64_t)p * np * kp + panel_off(i, k0, kp)) = zcrt_pack16<J>(vr, vi, p, md);
+        *(i32x4 *)(D + (int64_t)p * np * kp + panel_off(i, k0, kp)) = zcrt_pack16<ND>(vr, vi, p, md);
 }
 
 // strided (mode 0): z(i,k) = Z[i + k*ld]; thread = one row x 16 k, as above
-template <int J>
+template <int ND>
 __global__ __launch_bounds__(256) void k_zcrt_split_strided(const double *__restrict__ Z,
                                                             int64_t ld, int n, int len, int conj,
                                                             const int *__restrict__ e, int beta,
@@ -1050,39 +1130,53 @@ __global__ __launch_bounds__(256) void k_zcrt_split_strided(const double *__rest
     const int i = blockIdx.x * 64 + (threadIdx.x & 63);
     const int k0 = blockIdx.y * 64 + (threadIdx.x >> 6) * 16;
     if (i >= np) return;
-    Digits<J> vr[16], vi[16];
+    Dig7<ND> vr[16], vi[16];
     const int sh = i < n ? beta - e[i] : 0;
+    const double *z0 = Z + 2 * (i + (int64_t)k0 * ld);
+    if (i < n && k0 + 16 <= len && ((((uintptr_t)Z) & 15) == 0)) {
+        // 16-byte loads (every element 16-byte aligned), all in flight together
+        double2 zs[16];
 #pragma unroll
-    for (int b = 0; b < 16; ++b) {
-        const bool in = i < n && k0 + b < len;
-        const double *z = Z + 2 * (i + (int64_t)(k0 + b) * ld);
-        vr[b] = digits12<J>(in ? trunc(ldexp(z[0], sh)) : 0.0);
-        vi[b] = digits12<J>(in ? trunc(ldexp(conj ? -z[1] : z[1], sh)) : 0.0);
+        for (int b = 0; b < 16; ++b) zs[b] = *(const double2 *)(z0 + 2 * b * ld);
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+            vr[b] = digits7<ND>(trunc(ldexp(zs[b].x, sh)));
+            vi[b] = digits7<ND>(trunc(ldexp(conj ? -zs[b].y : zs[b].y, sh)));
+        }
+    } else {
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+            const bool in = i < n && k0 + b < len;
+            const double *z = z0 + 2 * b * ld;
+            vr[b] = digits7<ND>(in ? trunc(ldexp(z[0], sh)) : 0.0);
+            vi[b] = digits7<ND>(in ? trunc(ldexp(conj ? -z[1] : z[1], sh)) : 0.0);
+        }
     }
     for (int p = 0; p < 3 * md.L; ++p)
-        *(i32x4 *)(D + (int64_t)p * np * kp + panel_off(i, k0, kp)) = zcrt_pack16<J>(vr, vi, p, md);
+        *(i32x4 *)(D + (int64_t)p * np * kp + panel_off(i, k0, kp)) = zcrt_pack16<ND>(vr, vi, p, md);
 }
 
 // residues of one tile sum -> int8 plane (column-major, ld ldr); every lane
 // owns 4 consecutive rows of one column: one 32-bit store each
 template <int WTM, int WTN, int NBM, int NBN>
 __device__ __forceinline__ void store_resid(const i32x4 (&acc)[NBM][NBN], int m0, int n0,
-                                            int mod, double rinv, int8_t *__restrict__ R,
+                                            const CrtMods &md, int l, int8_t *__restrict__ R,
                                             int64_t ldr)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lg = lane >> 4;
+    const int mod = md.m[l], c16 = md.c16[l];
+    const float rinvf = md.rinvf[l];
 #pragma unroll
     for (int mb = 0; mb < NBM; ++mb)
 #pragma unroll
         for (int nb2 = 0; nb2 < NBN; ++nb2) {
             const int col = n0 + wn * WTN + nb2 * 16 + l16;
             const int row = m0 + wm * WTM + mb * 16 + 4 * lg;
-            uint32_t w = 0;
+            uint32_t b[4];
 #pragma unroll
-            for (int v = 0; v < 4; ++v)
-                w |= ((uint32_t)(uint8_t)(int8_t)smod_i32(acc[mb][nb2][v], mod, rinv)) << (8 * v);
-            *(uint32_t *)(R + row + (int64_t)col * ldr) = w;
+            for (int v = 0; v < 4; ++v) b[v] = resid_byte_i32(acc[mb][nb2][v], mod, c16, rinvf);
+            *(uint32_t *)(R + row + (int64_t)col * ldr) = pack4(b[0], b[1], b[2], b[3]);
         }
 }
 
@@ -1099,7 +1193,7 @@ __global__ __launch_bounds__(OZ_NT, 2) void k_crt_gemm(const OzSeg *__restrict__
     const int p = blockIdx.y, l = p / per;
     i32x4 acc[4][4];
     tile128(segs + p, 1, kp, m0, n0, acc);
-    store_resid<64, 64, 4, 4>(acc, m0, n0, md.m[l], md.rinv[l], R + p * plane, ldr);
+    store_resid<64, 64, 4, 4>(acc, m0, n0, md, l, R + p * plane, ldr);
 }
 
 // product p0 + blockIdx.y.  The host launches one product at a time when
@@ -1115,7 +1209,7 @@ __global__ __launch_bounds__(OZ_NT_BIG, 1) void k_crt_gemm_big(const OzSeg *__re
     const int p = p0 + (int)blockIdx.y, l = p / per;
     i32x4 acc[8][4];
     tile256(segs + p, 1, kp, m0, n0, acc);
-    store_resid<128, 64, 8, 4>(acc, m0, n0, md.m[l], md.rinv[l], R + p * plane, ldr);
+    store_resid<128, 64, 8, 4>(acc, m0, n0, md, l, R + p * plane, ldr);
 }
 
 struct CrtRecon {
@@ -1383,22 +1477,13 @@ int bxh_bf16_group(const void *segs_dev, int nseg, int kp2, int m, int n, int mp
 void bxh_bf16x3_pieces_host(double x, uint16_t *out) { bf16x3_pieces(x, out); }
 
 // ---- CRT split -------------------------------------------------------------
-// host view of the constant tables (same layout as the kernels' structs)
-struct bxh_crt_mods_host {
-    int L;
-    int m[20];
-    double rinv[20];
-    float rinvf[20];
-    float mf[20];
-    float c12[7][20];
-};
+// host view of the reconstruction constants (crt.c fills the same layout)
 struct bxh_crt_recon_host {
     int L;
     double W[4][20];
     double Mk[4];
     double invM;
 };
-static_assert(sizeof(bxh_crt_mods_host) == sizeof(CrtMods), "layout");
 static_assert(sizeof(bxh_crt_recon_host) == sizeof(CrtRecon), "layout");
 
 static CrtMods crt_mods(int L, const int *m)
@@ -1408,13 +1493,13 @@ static CrtMods crt_mods(int L, const int *m)
     md.L = L;
     for (int l = 0; l < L; ++l) {
         md.m[l] = m[l];
-        md.rinv[l] = 1.0 / m[l];
         md.rinvf[l] = 1.0f / (float)m[l];
-        md.mf[l] = (float)m[l];
-        uint64_t p = 1; // 2^(12 j) mod m
-        for (int j = 0; j < 7; ++j) {
-            md.c12[j][l] = (float)smod_fix((int)p, m[l]);
-            p = (p << 12) % (uint64_t)m[l];
+        md.c16[l] = smod_fix((int)((1u << 16) % (unsigned)m[l]), m[l]);
+        uint64_t p = 1; // 2^(7 j) mod m
+        for (int j = 0; j < 12; ++j) {
+            const uint32_t c = (uint8_t)(int8_t)smod_fix((int)p, m[l]);
+            md.c7[j / 4][l] |= c << (8 * (j % 4));
+            p = (p << 7) % (uint64_t)m[l];
         }
     }
     return md;
@@ -1427,8 +1512,8 @@ int bxh_crt_split(const double *X, int64_t ld, int n, int len, int contiguous, c
     if (L < 1 || L > 20 || np % 64 || kp % 64) return BLASX_EINVAL;
     hipStream_t st = (hipStream_t)stream;
     const CrtMods md = crt_mods(L, moduli);
-    // |A'| < 2^beta: the fewest 12-bit digits that cover beta bits
-    const int J = beta <= 60 ? 5 : beta <= 72 ? 6 : 7;
+    // |A'| < 2^beta: the fewest 28-bit digit words that cover beta bits
+    const int ND = beta <= 56 ? 2 : 3;
     if (contiguous) {
         // whole 128-row blocks (the planes' panels), 4 threads per 64-byte K tile
         const int64_t total = (int64_t)((np + OZ_PR - 1) / OZ_PR) * OZ_PR * (kp / 16);
@@ -1436,17 +1521,15 @@ int bxh_crt_split(const double *X, int64_t ld, int n, int len, int contiguous, c
         auto go = [&](auto kern) {
             hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, X, ld, n, len, e, beta, md, D, np, kp);
         };
-        if (J == 5) go(k_crt_split_contig<5>);
-        else if (J == 6) go(k_crt_split_contig<6>);
-        else go(k_crt_split_contig<7>);
+        if (ND == 2) go(k_crt_split_contig<2>);
+        else go(k_crt_split_contig<3>);
     } else {
         const dim3 grid(np / 64, kp / 64);
         auto go = [&](auto kern) {
             hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, X, ld, n, len, e, beta, md, D, np, kp);
         };
-        if (J == 5) go(k_crt_split_strided<5>);
-        else if (J == 6) go(k_crt_split_strided<6>);
-        else go(k_crt_split_strided<7>);
+        if (ND == 2) go(k_crt_split_strided<2>);
+        else go(k_crt_split_strided<3>);
     }
     hipError_t r = hipGetLastError();
     return r == hipSuccess ? 0 : oz_fail("crt split launch", r);
@@ -1508,8 +1591,8 @@ int bxh_zcrt_split(const double *Z, int64_t ld, int n, int len, int contiguous, 
     if (L < 1 || L > 20 || np % 64 || kp % 64) return BLASX_EINVAL;
     hipStream_t st = (hipStream_t)stream;
     const CrtMods md = crt_mods(L, moduli);
-    // |A'| < 2^beta: the fewest 12-bit digits that cover beta bits
-    const int J = beta <= 60 ? 5 : beta <= 72 ? 6 : 7;
+    // |A'| < 2^beta: the fewest 28-bit digit words that cover beta bits
+    const int ND = beta <= 56 ? 2 : 3;
     if (contiguous) {
         // whole 128-row blocks (the planes' panels), 4 threads per 64-byte K tile
         const int64_t total = (int64_t)((np + OZ_PR - 1) / OZ_PR) * OZ_PR * (kp / 16);
@@ -1518,18 +1601,16 @@ int bxh_zcrt_split(const double *Z, int64_t ld, int n, int len, int contiguous, 
             hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, Z, ld, n, len, conj, e, beta, md, D,
                                np, kp);
         };
-        if (J == 5) go(k_zcrt_split_contig<5>);
-        else if (J == 6) go(k_zcrt_split_contig<6>);
-        else go(k_zcrt_split_contig<7>);
+        if (ND == 2) go(k_zcrt_split_contig<2>);
+        else go(k_zcrt_split_contig<3>);
     } else {
         const dim3 grid(np / 64, kp / 64);
         auto go = [&](auto kern) {
             hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, Z, ld, n, len, conj, e, beta, md, D,
                                np, kp);
         };
-        if (J == 5) go(k_zcrt_split_strided<5>);
-        else if (J == 6) go(k_zcrt_split_strided<6>);
-        else go(k_zcrt_split_strided<7>);
+        if (ND == 2) go(k_zcrt_split_strided<2>);
+        else go(k_zcrt_split_strided<3>);
     }
     hipError_t r = hipGetLastError();
     return r == hipSuccess ? 0 : oz_fail("zcrt split launch", r);
@@ -1559,20 +1640,19 @@ int bxh_zcrt_recon(int m, int n, const int8_t *R, int64_t ldr, int64_t plane, co
 int bxh_crt_resid_host(double v, int m)
 {
     const CrtMods md = crt_mods(1, &m);
-    return digit_resid<7>(digits12<7>(v), mod_const<7>(md, 0));
+    return digit_resid<3>(digits7<3>(v), mod_const(md, 0));
 }
-int bxh_crt_resid6_host(double v, int m)
+int bxh_crt_resid2_host(double v, int m)
 {
     const CrtMods md = crt_mods(1, &m);
-    return digit_resid<6>(digits12<6>(v), mod_const<6>(md, 0));
+    return digit_resid<2>(digits7<2>(v), mod_const(md, 0));
 }
 int bxh_crt_norm_exp_host(double ss) { return norm_exp_of(ss); }
-int bxh_crt_resid5_host(double v, int m)
+int bxh_crt_smod_i32_host(int x, int m)
 {
     const CrtMods md = crt_mods(1, &m);
-    return digit_resid<5>(digits12<5>(v), mod_const<5>(md, 0));
+    return (int8_t)resid_byte_i32(x, md.m[0], md.c16[0], md.rinvf[0]);
 }
-int bxh_crt_smod_i32_host(int x, int m) { return smod_i32(x, m, 1.0 / m); }
 
 int bxh_oz_deinterleave(const double *Z, int64_t ld, int rows, int cols, int conj, double *re,
                         double *im, void *stream)

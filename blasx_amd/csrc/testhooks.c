@@ -105,8 +105,7 @@ typedef struct {
 } crt_recon_view_t;
 void bx_crt_consts(int L, void *c);
 int bxh_crt_resid_host(double v, int m);
-int bxh_crt_resid5_host(double v, int m);
-int bxh_crt_resid6_host(double v, int m);
+int bxh_crt_resid2_host(double v, int m);
 int bxh_crt_norm_exp_host(double ss);
 int bxh_crt_smod_i32_host(int x, int m);
 
@@ -122,8 +121,7 @@ BX_EXPORT void bxtest_crt_consts(int L, double *W /* [4][20] */, double *Mk, dou
 }
 
 BX_EXPORT int bxtest_crt_resid(double v, int m) { return bxh_crt_resid_host(v, m); }
-BX_EXPORT int bxtest_crt_resid5(double v, int m) { return bxh_crt_resid5_host(v, m); }
-BX_EXPORT int bxtest_crt_resid6(double v, int m) { return bxh_crt_resid6_host(v, m); }
+BX_EXPORT int bxtest_crt_resid2(double v, int m) { return bxh_crt_resid2_host(v, m); }
 BX_EXPORT int bxtest_crt_norm_exp(double ss) { return bxh_crt_norm_exp_host(ss); }
 BX_EXPORT int bxtest_crt_smod_i32(int x, int m) { return bxh_crt_smod_i32_host(x, m); }
 

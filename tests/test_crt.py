@@ -96,42 +96,45 @@ def test_reconstruction_constants_exact(orc, bx, L):
 
 
 def test_residue_helpers_match_exact_arithmetic(orc, bx):
-    """The kernels' residue of an integer-valued double (12-bit digits, float
-    arithmetic) on every magnitude up to 2^83 with seven digits and up to
-    2^60 with five (the split's |A'| < 2^beta), including the digit
-    boundaries, and smod of int32 products, against Python's exact integers:
-    the library's helpers (host build of the kernel code) and the oracle's
-    independent int64 version.  Non-finite values and magnitudes beyond the
-    digits give residue 0."""
+    """The kernels' residue of an integer-valued double (signed 7-bit digits
+    in three 32-bit words, reduced on the int8 dot-product unit and by one
+    float quotient) on every magnitude below 2^84, and below 2^56 with two
+    words (the split's |A'| < 2^beta), including the digit and word
+    boundaries, and the residue of any int32 product sum, against Python's
+    exact integers: the library's helpers (host build of the kernel code)
+    and the oracle's independent int64 version.  Non-finite values and
+    magnitudes beyond the digits give residue 0."""
     Lb = bx.test_lib()
     Lb.bxtest_crt_resid.argtypes = [C.c_double, C.c_int]
-    Lb.bxtest_crt_resid5.argtypes = [C.c_double, C.c_int]
-    Lb.bxtest_crt_resid6.argtypes = [C.c_double, C.c_int]
+    Lb.bxtest_crt_resid2.argtypes = [C.c_double, C.c_int]
     Lb.bxtest_crt_smod_i32.argtypes = [C.c_int, C.c_int]
     rng = random.Random(11)
-    vals = [0.0, -0.0, 1.0, -1.0, 127.0, -128.0, 2.0 ** 53, -(2.0 ** 53), 2.0 ** 53 - 1, 2.0 ** 26,
-            2.0 ** 26 - 1, -(2.0 ** 26), 2.0 ** 78, -(2.0 ** 78) + 2.0 ** 30, 2.0 ** 83,
-            -(2.0 ** 84 - 2.0 ** 31)]
-    for b in (12, 24, 30, 36, 48, 60, 72):  # digit and half boundaries
-        for v in (2.0 ** b, 2.0 ** b - 1, 2.0 ** b + 1, 2.0 ** b - 2.0 ** (b - 12) if b > 12 else 3.0):
+    vals = [0.0, -0.0, 1.0, -1.0, 127.0, -127.0, 128.0, -128.0, 2.0 ** 53, -(2.0 ** 53),
+            2.0 ** 53 - 1, 2.0 ** 26, 2.0 ** 26 - 1, -(2.0 ** 26), 2.0 ** 78,
+            -(2.0 ** 78) + 2.0 ** 30, 2.0 ** 83, -(2.0 ** 84 - 2.0 ** 31), 2.0 ** 56 - 2.0 ** 3,
+            -(2.0 ** 56 - 2.0 ** 4)]
+    for b in range(7, 84, 7):  # digit boundaries (28, 56: word boundaries)
+        for v in (2.0 ** b, 2.0 ** b - 1, 2.0 ** b + 1, 2.0 ** b - 2.0 ** max(b - 52, 0)):
             if v < 2.0 ** 53 or v == math.trunc(v):
                 vals += [float(v), -float(v)]
     for _ in range(3000):
         e = rng.randrange(0, 84)
         vals.append(float(math.trunc(rng.uniform(-1, 1) * 2.0 ** e)))
+    for _ in range(300):  # all-ones digits (127 in every byte) and their negatives
+        e = rng.randrange(1, 84)
+        v = float(2 ** e - 2 ** max(e - 53, 0))
+        vals += [v, -v]
     for v in vals:
         iv = int(v)
         for m in MODS:
             want = _smod(iv, m)
             assert Lb.bxtest_crt_resid(v, m) == want, (v, m)
-            assert Lb.bxtest_crt_resid5(v, m) == (want if abs(v) < 2.0 ** 60 else 0), (v, m)
-            assert Lb.bxtest_crt_resid6(v, m) == (want if abs(v) < 2.0 ** 72 else 0), (v, m)
+            assert Lb.bxtest_crt_resid2(v, m) == (want if abs(v) < 2.0 ** 56 else 0), (v, m)
             if abs(v) < 2.0 ** 79:
                 assert orc.crt_resid(v, m) == want, (v, m)
     for v in (float("inf"), -float("inf"), float("nan"), 2.0 ** 84, -(2.0 ** 90)):
         for m in MODS:
-            assert Lb.bxtest_crt_resid(v, m) == 0 and Lb.bxtest_crt_resid5(v, m) == 0, (v, m)
-            assert Lb.bxtest_crt_resid6(v, m) == 0, (v, m)
+            assert Lb.bxtest_crt_resid(v, m) == 0 and Lb.bxtest_crt_resid2(v, m) == 0, (v, m)
 
 
 def _norm_exp_exact(ss):
