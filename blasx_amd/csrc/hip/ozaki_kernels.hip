@@ -475,7 +475,7 @@ __device__ __forceinline__ void tile256_seg(const int8_t *__restrict__ gA,
     dma(0, 0);
     dma(1, 1);
     dma(min(2, last), 2);
-    vm_wait<0>();
+    vm_wait<8>(); // tile 0 (the first step waits for tile 1)
     __builtin_amdgcn_s_barrier();
 #pragma unroll
     for (int mb = 0; mb < 6; ++mb) fa[mb] = rdA(0, mb);
@@ -1008,12 +1008,14 @@ __global__ __launch_bounds__(256) void k_crt_split_contig(const double *__restri
     const int sh = i < n ? beta - e[i] : 0;
     const double *x = X + k0 + (int64_t)i * ld;
     if (i < n && k0 + 16 <= len && ((((uintptr_t)x) & 15) == 0)) {
-        // 16-byte loads
+        // 16-byte loads, all in flight together
+        double2 w[8];
 #pragma unroll
-        for (int b = 0; b < 16; b += 2) {
-            const double2 w = *(const double2 *)(x + b);
-            v[b] = digits7<ND>(trunc(ldexp(w.x, sh)));
-            v[b + 1] = digits7<ND>(trunc(ldexp(w.y, sh)));
+        for (int b = 0; b < 8; ++b) w[b] = *(const double2 *)(x + 2 * b);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            v[2 * b] = digits7<ND>(trunc(ldexp(w[b].x, sh)));
+            v[2 * b + 1] = digits7<ND>(trunc(ldexp(w[b].y, sh)));
         }
     } else {
 #pragma unroll
@@ -1026,8 +1028,9 @@ __global__ __launch_bounds__(256) void k_crt_split_contig(const double *__restri
 }
 
 // strided (mode 0): x(i,k) = X[i + k*ld]; block = 64 rows x 64 k, thread =
-// one row x 16 consecutive k: the FP64 reads are coalesced along i across
-// the wave, and each plane gets one 16-byte store per thread
+// one row x 16 consecutive k, lanes in the planes' panel order (4 K chunks
+// of a row, then 16 rows): a wave's FP64 reads are four 128-byte runs along
+// i per k, and its store to each plane is 1 KB contiguous
 template <int ND>
 __global__ __launch_bounds__(256) void k_crt_split_strided(const double *__restrict__ X, int64_t ld,
                                                            int n, int len,
@@ -1035,8 +1038,8 @@ __global__ __launch_bounds__(256) void k_crt_split_strided(const double *__restr
                                                            CrtMods md, int8_t *__restrict__ D,
                                                            int np, int kp)
 {
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int k0 = blockIdx.y * 64 + (threadIdx.x >> 6) * 16;
+    const int i = blockIdx.x * 64 + (threadIdx.x >> 2);
+    const int k0 = blockIdx.y * 64 + (threadIdx.x & 3) * 16;
     if (i >= np) return;
     Dig7<ND> v[16];
     const int sh = i < n ? beta - e[i] : 0;
@@ -1119,7 +1122,7 @@ __global__ __launch_bounds__(256) void k_zcrt_split_contig(const double *__restr
         *(i32x4 *)(D + (int64_t)p * np * kp + panel_off(i, k0, kp)) = zcrt_pack16<ND>(vr, vi, p, md);
 }
 
-// strided (mode 0): z(i,k) = Z[i + k*ld]; thread = one row x 16 k, as above
+// strided (mode 0): z(i,k) = Z[i + k*ld]; threads as k_crt_split_strided
 template <int ND>
 __global__ __launch_bounds__(256) void k_zcrt_split_strided(const double *__restrict__ Z,
                                                             int64_t ld, int n, int len, int conj,
@@ -1127,8 +1130,8 @@ __global__ __launch_bounds__(256) void k_zcrt_split_strided(const double *__rest
                                                             CrtMods md, int8_t *__restrict__ D,
                                                             int np, int kp)
 {
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int k0 = blockIdx.y * 64 + (threadIdx.x >> 6) * 16;
+    const int i = blockIdx.x * 64 + (threadIdx.x >> 2);
+    const int k0 = blockIdx.y * 64 + (threadIdx.x & 3) * 16;
     if (i >= np) return;
     Dig7<ND> vr[16], vi[16];
     const int sh = i < n ? beta - e[i] : 0;
@@ -1241,8 +1244,8 @@ __global__ void k_crt_recon(int m, int n, const int8_t *__restrict__ R, int64_t 
         const int i0 = 4 * (int)(t % m4), j = (int)(t / m4);
         const int8_t *r = R + i0 + (int64_t)j * ldr;
         double q[4][4] = {};
-        // three planes per iteration: their loads are issued together
-#pragma unroll 3
+        // six planes per iteration: their loads are issued together
+#pragma unroll 6
         for (int l = 0; l < c.L; ++l) {
             const uint32_t w = gload((const uint32_t *)(r + l * plane));
 #pragma unroll
