@@ -346,6 +346,30 @@ def test_gpu_crt_big_tile_path_sampled(gpu, orc, ta, tb):
 
 
 @pytest.mark.gpu
+def test_gpu_crt_many_tiles_sampled(gpu, orc):
+    """576 tiles of 256x256 (M, N pad to 6144): more tiles than CUs, a
+    group of 8 tile rows per XCD range, ragged M and N.  Sampled blocks
+    from the first, middle and last tiles equal the oracle restatement bit
+    for bit."""
+    bx = gpu
+    M, N, K = 6100, 5889, 300
+    A = orc.fill_dynrange(M, K, 9, 0, 1.0)
+    B = orc.fill_dynrange(K, N, 9, 1, 1.0)
+    C0 = orc.fill_uniform(M, N, 9, 2)
+    dA, dB, dC = (bx.DeviceArray.from_host(x) for x in (A, B, C0))
+    say("crt many tiles", M, N, K)
+    bx.dgemm_crt_device("N", "N", M, N, K, 1.25, dA.ptr, M, dB.ptr, K, -0.5, dC.ptr, M, 15)
+    bx.synchronize()
+    out = dC.download()
+    for (r0, c0) in [(0, 0), (M - 48, N - 40), (3071, 2999), (5000, 200), (255, 5800)]:
+        As = np.asfortranarray(A[r0:r0 + 48])
+        Bs = np.asfortranarray(B[:, c0:c0 + 40])
+        ref = np.asfortranarray(C0[r0:r0 + 48, c0:c0 + 40])
+        orc.dgemm_crt("N", "N", 48, 40, K, 1.25, As, 48, Bs, K, -0.5, ref, 48, 15)
+        assert np.array_equal(out[r0:r0 + 48, c0:c0 + 40], ref), (r0, c0)
+
+
+@pytest.mark.gpu
 def test_gpu_crt_large_within_bound(gpu, orc):
     """2048^3 with a wide dynamic range and the default moduli: sampled
     blocks equal the oracle bit for bit and stay inside the scheme's error
