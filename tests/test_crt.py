@@ -346,13 +346,14 @@ def test_gpu_crt_big_tile_path_sampled(gpu, orc, ta, tb):
 
 
 @pytest.mark.gpu
-def test_gpu_crt_many_tiles_sampled(gpu, orc):
-    """576 tiles of 256x256 (M, N pad to 6144): more tiles than CUs, a
-    group of 8 tile rows per XCD range, ragged M and N.  Sampled blocks
+@pytest.mark.parametrize("M,N,K", [(6100, 5889, 300), (8100, 8050, 200), (12200, 4000, 64)])
+def test_gpu_crt_many_tiles_sampled(gpu, orc, M, N, K):
+    """More 256x256 tiles than CUs, ragged M and N, each tile traversal:
+    24x24 tiles (M, N pad to 6144: XCD ranges of 8 tile rows), 32x32 (two
+    32x16 super-tiles) and 48x16 (three 16x16 super-tiles).  Sampled blocks
     from the first, middle and last tiles equal the oracle restatement bit
     for bit."""
     bx = gpu
-    M, N, K = 6100, 5889, 300
     A = orc.fill_dynrange(M, K, 9, 0, 1.0)
     B = orc.fill_dynrange(K, N, 9, 1, 1.0)
     C0 = orc.fill_uniform(M, N, 9, 2)
@@ -361,7 +362,8 @@ def test_gpu_crt_many_tiles_sampled(gpu, orc):
     bx.dgemm_crt_device("N", "N", M, N, K, 1.25, dA.ptr, M, dB.ptr, K, -0.5, dC.ptr, M, 15)
     bx.synchronize()
     out = dC.download()
-    for (r0, c0) in [(0, 0), (M - 48, N - 40), (3071, 2999), (5000, 200), (255, 5800)]:
+    for (r0, c0) in [(0, 0), (M - 48, N - 40), (3071, 2999), (5000, 200), (255, N - 300),
+                     (M - 300, N // 2), (M // 2, 1000)]:
         As = np.asfortranarray(A[r0:r0 + 48])
         Bs = np.asfortranarray(B[:, c0:c0 + 40])
         ref = np.asfortranarray(C0[r0:r0 + 48, c0:c0 + 40])
