@@ -1207,8 +1207,10 @@ __global__ __launch_bounds__(OZ_NT, 2) void k_crt_gemm(const OzSeg *__restrict__
 // (32 + 16 panels of 4 MB at K = 16384), and the super-tiles go row by row,
 // so the 32 row panels are read once per row of super-tiles.  Measured at
 // 16384^3, 18 moduli: 3.4% faster than XCD ranges of 8 full tile rows (16x16
-// super-tiles: 2.8%).  Shapes that do not divide fall back to 16x16
-// super-tiles, then to block_origin.
+// super-tiles: 2.8%); the digit-split and BF16x3 kernels, whose segments
+// run K several times over and whose epilogue adds into an FP64 W, ran
+// 2-3% slower with it and keep block_origin.  Shapes that do not divide
+// fall back to 16x16 super-tiles, then to block_origin.
 __device__ __forceinline__ void crt_tile_origin(int np, int &m0, int &n0)
 {
     const int tiles_n = np / OZ_BIG, nb = (int)gridDim.x, tiles_m = nb / tiles_n;
