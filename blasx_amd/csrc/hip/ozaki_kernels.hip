@@ -261,13 +261,13 @@ __global__ void k_split_contig(const double *__restrict__ X, int64_t ld, int n, 
 // (see the XCD renumbering in the kernels), so those resident together form
 // an 8 x (blocks / 8) patch: per K step they read 8 A tiles and blocks/8 B
 // tiles through their shared L2 instead of 1 A tile and one B tile each.
-__device__ __forceinline__ void group_tile(int t, int tiles_m, int tiles_n, int &tm, int &tn)
+__host__ __device__ inline void group_tile(int t, int tiles_m, int tiles_n, int &tm, int &tn)
 {
     constexpr int G = 8;
     const int per_group = G * tiles_n;
     const int g = t / per_group;
     const int first = g * G;
-    const int gsz = min(tiles_m - first, G);
+    const int gsz = tiles_m - first < G ? tiles_m - first : G;
     const int r = t - g * per_group;
     tm = first + r % gsz;
     tn = r / gsz;
@@ -278,17 +278,20 @@ struct OzSeg {
     const int8_t *B;
 };
 
-// block -> (m0, n0) of its BT x BT tile: XCD renumbering, then group_tile
-template <int BT> __device__ __forceinline__ void block_origin(int n, int &m0, int &n0)
+// block b of nb -> (m0, n0) of its BT x BT tile: XCD renumbering, then
+// group_tile
+template <int BT> __host__ __device__ inline void block_origin_of(int b, int nb, int n, int &m0, int &n0)
 {
     const int tiles_n = (n + BT - 1) / BT;
-    int b = blockIdx.x;
-    const int nb = gridDim.x;
     if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3); // contiguous tiles per XCD
     int tm, tn;
     group_tile(b, nb / tiles_n, tiles_n, tm, tn);
     m0 = tm * BT;
     n0 = tn * BT;
+}
+template <int BT> __device__ __forceinline__ void block_origin(int n, int &m0, int &n0)
+{
+    block_origin_of<BT>((int)blockIdx.x, (int)gridDim.x, n, m0, n0);
 }
 
 // 128x128 tile: acc[mb][nb] of wave (wm, wn) holds rows
@@ -1211,10 +1214,10 @@ __global__ __launch_bounds__(OZ_NT, 2) void k_crt_gemm(const OzSeg *__restrict__
 // run K several times over and whose epilogue adds into an FP64 W, ran
 // 2-3% slower with it and keep block_origin.  Shapes that do not divide
 // fall back to 16x16 super-tiles, then to block_origin.
-__device__ __forceinline__ void crt_tile_origin(int np, int &m0, int &n0)
+__host__ __device__ inline void crt_tile_origin_of(int b, int nb, int np, int &m0, int &n0)
 {
-    const int tiles_n = np / OZ_BIG, nb = (int)gridDim.x, tiles_m = nb / tiles_n;
-    const int x = (int)blockIdx.x & 7, q = (int)blockIdx.x >> 3, c = q & 31;
+    const int tiles_n = np / OZ_BIG, tiles_m = nb / tiles_n;
+    const int x = b & 7, q = b >> 3, c = q & 31;
     if ((nb & 7) == 0 && tiles_m % 32 == 0 && tiles_n % 16 == 0) {
         // XCD x: rows 4 x .. 4 x + 3 of the super-tile, half h of its columns
         const int sup = q >> 6, h = (q >> 5) & 1, sn = tiles_n / 16;
@@ -1227,8 +1230,12 @@ __device__ __forceinline__ void crt_tile_origin(int np, int &m0, int &n0)
         m0 = (16 * (sup / sn) + 4 * (x & 3) + (c & 3)) * OZ_BIG;
         n0 = (16 * (sup % sn) + 8 * (x >> 2) + (c >> 2)) * OZ_BIG;
     } else {
-        block_origin<OZ_BIG>(np, m0, n0);
+        block_origin_of<OZ_BIG>(b, nb, np, m0, n0);
     }
+}
+__device__ __forceinline__ void crt_tile_origin(int np, int &m0, int &n0)
+{
+    crt_tile_origin_of((int)blockIdx.x, (int)gridDim.x, np, m0, n0);
 }
 
 // product p0 + blockIdx.y.  The host launches one product at a time when
@@ -1683,6 +1690,12 @@ int bxh_crt_resid2_host(double v, int m)
     return digit_resid<2>(digits7<2>(v), mod_const(md, 0));
 }
 int bxh_crt_norm_exp_host(double ss) { return norm_exp_of(ss); }
+// the INT8 CRT GEMM's block -> tile origin (same code, CPU): nb blocks over
+// np columns of BIG-square tiles
+void bxh_crt_tile_origin_host(int b, int nb, int np, int *m0, int *n0)
+{
+    crt_tile_origin_of(b, nb, np, *m0, *n0);
+}
 int bxh_crt_smod_i32_host(int x, int m)
 {
     const CrtMods md = crt_mods(1, &m);

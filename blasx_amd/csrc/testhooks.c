@@ -108,6 +108,7 @@ int bxh_crt_resid_host(double v, int m);
 int bxh_crt_resid2_host(double v, int m);
 int bxh_crt_norm_exp_host(double ss);
 int bxh_crt_smod_i32_host(int x, int m);
+void bxh_crt_tile_origin_host(int b, int nb, int np, int *m0, int *n0);
 
 BX_EXPORT void bxtest_crt_consts(int L, double *W /* [4][20] */, double *Mk, double *invM)
 {
@@ -124,6 +125,10 @@ BX_EXPORT int bxtest_crt_resid(double v, int m) { return bxh_crt_resid_host(v, m
 BX_EXPORT int bxtest_crt_resid2(double v, int m) { return bxh_crt_resid2_host(v, m); }
 BX_EXPORT int bxtest_crt_norm_exp(double ss) { return bxh_crt_norm_exp_host(ss); }
 BX_EXPORT int bxtest_crt_smod_i32(int x, int m) { return bxh_crt_smod_i32_host(x, m); }
+BX_EXPORT void bxtest_crt_tile_origin(int b, int nb, int np, int *m0, int *n0)
+{
+    bxh_crt_tile_origin_host(b, nb, np, m0, n0);
+}
 
 /* the tile runtime's round order and operand traffic, replayed on the host */
 int bx_plan_simulate(int dt, int opa, int opb, int M, int N, int K, int T, int tri, int same_ab,

@@ -7,7 +7,8 @@ No reference counterpart: the method is pinned from its mathematics.
     Python's exact arithmetic; the oracle's floating CRT recovers integers
     to 2u at every magnitude; Garner's digits rebuild the exact integer; on
     integer inputs the oracle pipeline equals the exact product; its error
-    obeys the scheme's bound and falls as moduli are added.
+    obeys the scheme's bound and falls as moduli are added; the INT8 GEMM's
+    block -> tile traversal (host build) takes every tile once.
   * GPU: the library's result equals the oracle restatement bit for bit
     (residues and integer products are exact, the FP64 reconstruction order
     is fixed) and equals the exact product on integer inputs.
@@ -177,6 +178,48 @@ def test_norm_exponent_matches_definition(orc, bx):
     for m in MODS:  # quotient ties and range ends
         for x in (m // 2, -(m // 2), (m + 1) // 2, 3 * m // 2, -3 * m // 2, 2 ** 31 - 1, -2 ** 31):
             assert Lb.bxtest_crt_smod_i32(x, m) == _smod(x, m), (x, m)
+
+
+@pytest.mark.parametrize("mp,np_", [(8192, 4096), (16384, 16384), (4096, 4096),
+                                     (12288, 4096), (6144, 2048), (2560, 2304), (256, 256),
+                                     (8192, 12288)])
+def test_crt_tile_traversal_covers_every_tile_once(bx, mp, np_):
+    """Block -> tile origin of the INT8 CRT GEMM (the kernel's own code, host
+    build), one block per 256x256 tile: every tile is taken exactly once, and
+    where the grid divides into 32x16 super-tiles (16x16 when 32 rows do not
+    divide) each window of 8 x 64 consecutive blocks covers one whole
+    super-tile, and the 32 blocks of one XCD in each 256 of them (b % 8
+    equal) a 4 x 8 patch whose panels they share in L2."""
+    Lb = bx.test_lib()
+    f = Lb.bxtest_crt_tile_origin
+    f.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    T = 256
+    tm, tn = mp // T, np_ // T
+    nb = tm * tn
+    m0, n0 = C.c_int(), C.c_int()
+    tiles = []
+    for b in range(nb):
+        f(b, nb, np_, C.byref(m0), C.byref(n0))
+        assert m0.value % T == 0 and n0.value % T == 0, b
+        assert 0 <= m0.value < mp and 0 <= n0.value < np_, (b, m0.value, n0.value)
+        tiles.append((m0.value // T, n0.value // T))
+    assert sorted(tiles) == [(i, j) for i in range(tm) for j in range(tn)]
+    if nb % 8 or tn % 16 or tm % 16:
+        return
+    sr = 32 if tm % 32 == 0 else 16
+    win = 8 * 2 * sr  # blocks per super-tile of sr x 16 tiles
+    for w0 in range(0, nb, win):
+        sup = tiles[w0:w0 + win]
+        rows = {t[0] for t in sup}
+        cols = {t[1] for t in sup}
+        assert len(rows) == sr and len(cols) == 16 and len(set(sup)) == win, w0
+        assert max(rows) - min(rows) == sr - 1 and max(cols) - min(cols) == 15, w0
+        for x in range(8):
+            for h in range(win // 256):
+                patch = [tiles[w0 + 256 * h + 8 * c + x] for c in range(32)]
+                pr = {t[0] for t in patch}
+                pc = {t[1] for t in patch}
+                assert len(pr) * len(pc) == 32 and len(pr) == 4, (w0, x, h, pr, pc)
 
 
 @pytest.mark.parametrize("L", [8, 12, 16, 20])
