@@ -56,12 +56,21 @@ int bx_grouped_fill(const blasx_gemm_desc_t *d, int n, int dtype, bxh_problem_t 
         if (desc_runs(&d[i], dtype))
             load += (int64_t)((d[i].M + bm0 - 1) / bm0) * ((d[i].N + bn0 - 1) / bn0);
     int np = 0, total = 0;
+    /* the previous problem's choice, reused while M, N and K repeat: the
+     * table lookup takes a lock and interpolates, which on launches of many
+     * small equal problems cost more host time than the kernel took */
+    int lm = -1, ln = -1, lk = -1, shape = 0;
     for (int i = 0; i < n; ++i) {
         const blasx_gemm_desc_t *x = &d[i];
         if (!desc_runs(x, dtype)) continue;
         bxh_problem_t *P = &probs[np];
         int bm, bn;
-        const int shape = bx_grouped_shape(dtype, x->M, x->N, x->K, load);
+        if (x->M != lm || x->N != ln || x->K != lk) {
+            shape = bx_grouped_shape(dtype, x->M, x->N, x->K, load);
+            lm = x->M;
+            ln = x->N;
+            lk = x->K;
+        }
         bxh_tile_dims(dtype, shape, &bm, &bn);
         memset(P, 0, sizeof *P);
         P->C = x->C;
